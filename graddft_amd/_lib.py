@@ -1,0 +1,76 @@
+"""ctypes binding of ``libgdx.so`` (the C-ABI declared in ``include/gdx.h``).
+
+Loading fails loudly: there is no CPU fallback for any entry point.  Device pointers are
+passed as integers (``tensor.data_ptr()``) and the HIP stream as the raw handle of
+torch's current stream, so torch only provides memory and streams.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgdx.so")
+
+GDX_FP64, GDX_FP32, GDX_BF16X3 = 0, 1, 2
+GDX_FUNC_B3LYP, GDX_FUNC_NEURAL_B3LYP = 0, 1
+
+
+class XCDesc(ctypes.Structure):
+    _fields_ = [
+        ("npts", ctypes.c_int32),
+        ("npts_pad", ctypes.c_int32),
+        ("nao", ctypes.c_int32),
+        ("nao_pad", ctypes.c_int32),
+        ("spin_restricted", ctypes.c_int32),
+        ("kind", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("mlp_nc", ctypes.c_int32),
+        ("mlp_h", ctypes.c_int32),
+        ("mlp_nhidden", ctypes.c_int32),
+        ("mlp_nd", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+# name -> (restype, argtypes); every pointer is c_void_p
+_P, _I, _Z = ctypes.c_void_p, ctypes.c_int32, ctypes.c_size_t
+SIGNATURES = {
+    "gdx_last_error": (ctypes.c_char_p, []),
+    "gdx_version": (ctypes.c_int, []),
+    "gdx_ao_eval": (ctypes.c_int, [_P, _I, _P, _P, _P, _P, _I, _I, _P, _I, _I, _P]),
+    "gdx_xc_workspace_size": (ctypes.c_int, [ctypes.POINTER(XCDesc), ctypes.POINTER(_Z)]),
+    "gdx_xc_forward": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P, _P, _P]),
+    "gdx_xc_densities": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P]),
+    "gdx_xc_backward": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P, _P, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the HIP library (once).  Raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"gdx HIP library not found at {path}; build it with "
+            "`python -m graddft_amd.build_ext` (there is no CPU fallback)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class GdxError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load().gdx_last_error()
+        raise GdxError(f"{what} failed with status {rc}: {msg.decode() if msg else ''}")

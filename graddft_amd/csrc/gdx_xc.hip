@@ -1,0 +1,881 @@
+// The exchange-correlation grid path on gfx950: densities, energy densities, coefficient
+// MLP, FP64 grid integration, and the hand-written adjoints dE/dD (V_xc) and dE/dtheta.
+//
+// Reference semantics followed (grad_dft/...):
+//   densities        molecule.py:388-502   rho = sum D phi phi, grad rho = 2 sum D phi dphi,
+//                                          tau = 1/2 sum D dphi.dphi,
+//                                          lapl = 2 sum D dphi.dphi + 2 sum D phi lapl(phi)
+//   energy densities popular_functionals.py:29-326 (gdx_functionals.h)
+//   integration      functional.py:219-253, 316-342 (abs_clip of densities, of the
+//                    per-point product and of the weights; FP64 sum)
+//   dE/dD            train.py:86-121 (value_and_grad w.r.t. rdm1, unsymmetrised)
+//
+// Kernels (FP64 path; all matrix products on v_mfma_f64_16x16x4f64):
+//   pad_dm            rdm1 (2,n,n) -> zero-padded (2,nP,nP)
+//   density_f64       per 32-point tile: X_c = A_c D for c in {phi, dx, dy, dz} on MFMA,
+//                     contracted row-wise against the AO planes in the epilogue
+//   point_fwd         partial sums -> rho, grad rho, tau, lapl per spin; B3LYP energy
+//                     densities (clipped); MLP inputs log(|x| + 1e-4) or the B3LYP sum
+//   layer0            first dense layer (K = nc) + ELU on VALU
+//   gemm_f64          tiled MFMA GEMM with bias+ELU / ELU-backward epilogues and split-K
+//   head_fwd          last dense layer, 2 sigmoid(z/2), sum_f c_f e_f, clip, w-weighted sum
+//   head_bwd          seeds of the adjoint: dE/de_f, dE/dz and the last-layer delta
+//   point_bwd         dE/d(rho, |grad rho|^2, lapl) by forward-mode duals + MLP input grads
+//                     -> per-point potentials
+//   vxc_f64           V_s = phi^T Z_s + sum_j dphi_j^T (u_s dphi_j) on MFMA, split over
+//                     point chunks, Z built while staging tiles in LDS
+//   reduce_*          deterministic FP64 reductions of block / chunk partials
+#include "gdx_common.h"
+#include "gdx_functionals.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace gdx {
+namespace {
+
+constexpr int NT = 256;     // threads per block in every kernel here
+constexpr double SQUASH = 1e-4;
+
+// ---------------------------------------------------------------- small kernels ----
+
+__global__ void pad_dm_kernel(const double* __restrict__ rdm1, int n, int nP, double* __restrict__ Dp) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long tot = 2L * nP * nP;
+  if (i >= tot) return;
+  const int s = (int)(i / ((long)nP * nP));
+  const int r = (int)((i / nP) % nP);
+  const int c = (int)(i % nP);
+  Dp[i] = (r < n && c < n) ? rdm1[((long)s * n + r) * n + c] : 0.0;
+}
+
+// out[i] = sum_s part[s * stride + i], i < len (deterministic order)
+__global__ void reduce_partials_kernel(const double* __restrict__ part, int nsplit, long stride,
+                                       long len, double* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  double acc = 0.0;
+  for (int s = 0; s < nsplit; ++s) acc += part[(long)s * stride + i];
+  out[i] = acc;
+}
+
+// single-block deterministic sum of n block partials -> *out
+__global__ void reduce_scalar_kernel(const double* __restrict__ part, int n, double* __restrict__ out) {
+  __shared__ double red[NT / 64];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += part[i];
+  acc = warp_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < NT / 64; ++w) t += red[w];
+    *out = t;
+  }
+}
+
+// V partials (nchunk, 2, nP, nP) -> d_rdm1 (2, n, n)
+__global__ void reduce_vxc_kernel(const double* __restrict__ part, int nchunk, int nP, int n,
+                                  double* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long tot = 2L * n * n;
+  if (i >= tot) return;
+  const int s = (int)(i / ((long)n * n));
+  const int a = (int)((i / n) % n);
+  const int b = (int)(i % n);
+  const long off = ((long)s * nP + a) * nP + b;
+  const long stride = 2L * nP * nP;
+  double acc = 0.0;
+  for (int c = 0; c < nchunk; ++c) acc += part[c * stride + off];
+  out[i] = acc;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = warp_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+  return t;
+}
+
+__device__ __forceinline__ double elu(double x) { return x > 0.0 ? x : expm1(x); }
+__device__ __forceinline__ double elu_grad_from_out(double y) { return y > 0.0 ? 1.0 : y + 1.0; }
+__device__ __forceinline__ double clip30(double x) { return fabs(x) > 1e-30 ? x : 0.0; }
+
+// ----------------------------------------------------------- density (MFMA f64) ----
+//
+// Block: 4 waves. Tile: PT = 32 points x (4 * 16 * WCB) AO columns of X.  Wave w owns
+// columns [16 * WCB * w, 16 * WCB * (w + 1)) of the tile, both 16-point row blocks and
+// all four AO components, i.e. 2 * 4 * WCB accumulators of 16x16.
+// LDS: As[k][c][p] (AO component c of point p at orbital k), Bs[k][b] (rows of D).
+// Output: raw sums per point (phi.X0, dx.X0, dy.X0, dz.X0, sum_j dj.Xj, lapl.X0) for this
+// column block; point_fwd adds column blocks and applies the factors 2, 1/2.
+
+constexpr int DPT = 32;  // points per tile
+constexpr int DBK = 16;  // k per LDS stage
+
+template <int WCB>
+__global__ __launch_bounds__(NT) void density_f64_kernel(
+    const double* __restrict__ ao, int npts_pad, int nP, const double* __restrict__ Dp,
+    double* __restrict__ dens_part) {
+  constexpr int BN = 64 * WCB;
+  __shared__ double As[DBK][4][DPT + 1];
+  __shared__ double Bs[DBK][BN + 2];
+  __shared__ double red[4][DPT][DENS_STRIDE];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int p0 = blockIdx.x * DPT;
+  const int n0 = blockIdx.y * BN;
+  const int s = blockIdx.z;  // distinct spin index
+  const double* D = Dp + (size_t)s * nP * nP;
+  const size_t plane = (size_t)npts_pad * nP;
+
+  d4 acc[2][4][WCB];
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int cb = 0; cb < WCB; ++cb) acc[rb][c][cb] = (d4){0.0, 0.0, 0.0, 0.0};
+
+  for (int k0 = 0; k0 < nP; k0 += DBK) {
+    // A: 4 comps x 32 points x 16 k = 2048 doubles, 8 per thread (contiguous in k)
+    {
+      const int c = tid >> 6, p = (tid & 63) >> 1, kh = (tid & 1) * 8;
+      const double* src = ao + c * plane + (size_t)(p0 + p) * nP + k0 + kh;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) As[kh + j][c][p] = src[j];
+    }
+    // B: 16 rows of D x BN columns
+    for (int e = tid; e < DBK * BN; e += NT) {
+      const int k = e / BN, b = e % BN;
+      const int col = n0 + b;
+      Bs[k][b] = col < nP ? D[(size_t)(k0 + k) * nP + col] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < DBK / 4; ++kk) {
+      const int kr = kk * 4 + (lane >> 4);
+      double bf[WCB];
+#pragma unroll
+      for (int cb = 0; cb < WCB; ++cb) bf[cb] = Bs[kr][wave * 16 * WCB + cb * 16 + (lane & 15)];
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const double af = As[kr][c][rb * 16 + (lane & 15)];
+#pragma unroll
+          for (int cb = 0; cb < WCB; ++cb)
+            acc[rb][c][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bf[cb], acc[rb][c][cb], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+
+  // epilogue: contract X with AO planes at the same (point, orbital)
+  double part[2][4][DENS_STRIDE];
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int q = 0; q < DENS_STRIDE; ++q) part[rb][i][q] = 0.0;
+#pragma unroll
+  for (int cb = 0; cb < WCB; ++cb) {
+    const int col = n0 + wave * 16 * WCB + cb * 16 + (lane & 15);
+    if (col < nP) {
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int p = p0 + rb * 16 + mfma64_row(lane, i);
+          const size_t o = (size_t)p * nP + col;
+          const double phi = ao[o], gx = ao[plane + o], gy = ao[2 * plane + o],
+                       gz = ao[3 * plane + o], lp = ao[4 * plane + o];
+          const double x0 = acc[rb][0][cb][i];
+          part[rb][i][0] += x0 * phi;
+          part[rb][i][1] += x0 * gx;
+          part[rb][i][2] += x0 * gy;
+          part[rb][i][3] += x0 * gz;
+          part[rb][i][4] += acc[rb][1][cb][i] * gx + acc[rb][2][cb][i] * gy + acc[rb][3][cb][i] * gz;
+          part[rb][i][5] += x0 * lp;
+        }
+    }
+  }
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int q = 0; q < DENS_STRIDE; ++q) {
+        const double v = sum16(part[rb][i][q]);
+        if ((lane & 15) == 0) red[wave][rb * 16 + mfma64_row(lane, i)][q] = v;
+      }
+  __syncthreads();
+  if (tid < DPT * DENS_STRIDE) {
+    const int p = tid / DENS_STRIDE, q = tid % DENS_STRIDE;
+    const double v = red[0][p][q] + red[1][p][q] + red[2][p][q] + red[3][p][q];
+    const size_t nspin_blocks = gridDim.z;
+    dens_part[(((size_t)blockIdx.y * nspin_blocks + s) * npts_pad + p0 + p) * DENS_STRIDE + q] = v;
+  }
+}
+
+// ---------------------------------------------------------------- per-point fwd ----
+//
+// dens_fin (npts_pad, 2, 6): rho, grad rho (3), tau, lapl per spin (final)
+// E (npts_pad, 4): clipped energy densities; X (npts_pad, 6): MLP inputs (neural only)
+__global__ __launch_bounds__(NT) void point_fwd_kernel(
+    const double* __restrict__ dens_part, int ncb, int nsd, int npts, int npts_pad,
+    const double* __restrict__ weights, int neural, double* __restrict__ dens_fin,
+    double* __restrict__ E, double* __restrict__ X, double* __restrict__ eps_part) {
+  __shared__ double red[NT / 64];
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  double contrib = 0.0;
+  if (r < npts_pad) {
+    double d[2][DENS_STRIDE];
+    for (int s = 0; s < 2; ++s) {
+      const int ds = nsd == 1 ? 0 : s;
+      double raw[DENS_STRIDE] = {0, 0, 0, 0, 0, 0};
+      for (int cb = 0; cb < ncb; ++cb) {
+        const double* src = dens_part + (((size_t)cb * nsd + ds) * npts_pad + r) * DENS_STRIDE;
+#pragma unroll
+        for (int q = 0; q < DENS_STRIDE; ++q) raw[q] += src[q];
+      }
+      d[s][0] = raw[0];
+      d[s][1] = 2.0 * raw[1];
+      d[s][2] = 2.0 * raw[2];
+      d[s][3] = 2.0 * raw[3];
+      d[s][4] = 0.5 * raw[4];
+      d[s][5] = 2.0 * raw[4] + 2.0 * raw[5];
+#pragma unroll
+      for (int q = 0; q < DENS_STRIDE; ++q) dens_fin[((size_t)r * 2 + s) * DENS_STRIDE + q] = d[s][q];
+    }
+    const double gu = d[0][1] * d[0][1] + d[0][2] * d[0][2] + d[0][3] * d[0][3];
+    const double gd = d[1][1] * d[1][1] + d[1][2] * d[1][2] + d[1][3] * d[1][3];
+    double e[4];
+    b3lyp_parts<double>(d[0][0], d[1][0], gu, gd, d[0][5], d[1][5], e);
+#pragma unroll
+    for (int f = 0; f < 4; ++f) E[(size_t)r * 4 + f] = e[f];
+    if (neural) {
+      const double feat[6] = {d[0][0], d[1][0], gu, gd, d[0][4], d[1][4]};
+#pragma unroll
+      for (int j = 0; j < 6; ++j) X[(size_t)r * 6 + j] = log(fabs(feat[j]) + SQUASH);
+    } else if (weights != nullptr && r < npts) {
+      const double c[4] = {1.0 - 0.2, 0.72, 1.0 - 0.81, 0.81};
+      double eps = c[0] * e[0] + c[1] * e[1] + c[2] * e[2] + c[3] * e[3];
+      eps = clip30(clip30(eps));
+      contrib = clip30(weights[r]) * eps;
+    }
+  }
+  const double t = block_sum(contrib, red);
+  if (!neural && threadIdx.x == 0) eps_part[blockIdx.x] = t;
+}
+
+// H (npts_pad, h) = ELU(X (npts_pad, nc) W0 (nc, h) + b0)
+__global__ __launch_bounds__(NT) void layer0_kernel(const double* __restrict__ X, int nc, int h,
+                                                    long total, const double* __restrict__ W,
+                                                    const double* __restrict__ b,
+                                                    double* __restrict__ H) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long r = i / h;
+  const int f = (int)(i % h);
+  double z = b[f];
+  for (int j = 0; j < nc; ++j) z += X[r * nc + j] * W[j * h + f];
+  H[i] = elu(z);
+}
+
+// ------------------------------------------------------------------ GEMM (f64) ----
+enum { EPI_STORE = 0, EPI_BIAS_ELU = 1, EPI_ELU_BWD = 2 };
+constexpr int GBK = 16;
+
+// C (+ z * M * ldc for split-K) = epi(op(A) op(B)); op(A)[m][k] = TA ? A[k*lda+m] : A[m*lda+k],
+// op(B)[k][n] = TB ? B[n*ldb+k] : B[k*ldb+n].  colsum (optional, split-K): column sums of
+// op(B) over this block's k range, written by blockIdx.x == 0 to colsum[z * N + n].
+template <int BM, int BN, bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(NT) void gemm_f64_kernel(
+    int M, int N, int K, const double* __restrict__ A, int lda, const double* __restrict__ B,
+    int ldb, double* __restrict__ C, int ldc, const double* __restrict__ bias,
+    const double* __restrict__ Y, int ldy, int kchunk, double* __restrict__ colsum) {
+  constexpr int WM = BM / 2, WN = BN / 2, MB = WM / 16, NB = WN / 16;
+  __shared__ double As[GBK][BM + 2];
+  __shared__ double Bs[GBK][BN + 2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int kbeg = blockIdx.z * kchunk;
+  const int kend = min(K, kbeg + kchunk);
+  double csum = 0.0;
+
+  d4 acc[MB][NB];
+#pragma unroll
+  for (int i = 0; i < MB; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+
+  for (int k0 = kbeg; k0 < kend; k0 += GBK) {
+    for (int e = tid; e < BM * GBK; e += NT) {
+      int m, k;
+      if (TA) { k = e / BM; m = e % BM; } else { m = e / GBK; k = e % GBK; }
+      const int gm = m0 + m, gk = k0 + k;
+      double v = 0.0;
+      if (gm < M && gk < kend) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
+      As[k][m] = v;
+    }
+    for (int e = tid; e < BN * GBK; e += NT) {
+      int n, k;
+      if (TB) { n = e / GBK; k = e % GBK; } else { k = e / BN; n = e % BN; }
+      const int gn = n0 + n, gk = k0 + k;
+      double v = 0.0;
+      if (gn < N && gk < kend) v = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
+      Bs[k][n] = v;
+    }
+    __syncthreads();
+    if (colsum != nullptr && blockIdx.x == 0 && tid < BN) {
+#pragma unroll
+      for (int k = 0; k < GBK; ++k) csum += Bs[k][tid];
+    }
+#pragma unroll
+    for (int kk = 0; kk < GBK / 4; ++kk) {
+      const int kr = kk * 4 + (lane >> 4);
+      double af[MB], bf[NB];
+#pragma unroll
+      for (int i = 0; i < MB; ++i) af[i] = As[kr][wm + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) bf[j] = Bs[kr][wn + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  if (colsum != nullptr && blockIdx.x == 0 && tid < BN && n0 + tid < N)
+    colsum[(size_t)blockIdx.z * N + n0 + tid] = csum;
+
+  double* Cz = C + (size_t)blockIdx.z * M * ldc;
+#pragma unroll
+  for (int i = 0; i < MB; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int col = n0 + wn + j * 16 + (lane & 15);
+      if (col >= N) continue;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = m0 + wm + i * 16 + mfma64_row(lane, q);
+        if (row >= M) continue;
+        double v = acc[i][j][q];
+        if (EPI == EPI_BIAS_ELU) v = elu(v + bias[col]);
+        if (EPI == EPI_ELU_BWD) v *= elu_grad_from_out(Y[(size_t)row * ldy + col]);
+        Cz[(size_t)row * ldc + col] = v;
+      }
+    }
+}
+
+// ----------------------------------------------------------------- MLP head ----
+// One wave per point (grid-stride over points).  z_o = sum_f H[r][f] W[f][o] + b[o],
+// c = 2 sigmoid(z/2) (functional.py:407-419 with sigmoid_scale_factor 2).
+constexpr int MAXND = 8;
+
+__global__ __launch_bounds__(NT) void head_fwd_kernel(
+    const double* __restrict__ H, int h, int nd, const double* __restrict__ W,
+    const double* __restrict__ b, const double* __restrict__ E, const double* __restrict__ weights,
+    int npts, double* __restrict__ Cout, double* __restrict__ eps_part) {
+  __shared__ double red[NT / 64];
+  const int lane = threadIdx.x & 63;
+  const int wave_global = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+  const int nwaves = gridDim.x * (NT / 64);
+  double contrib = 0.0;
+  for (int r = wave_global; r < npts; r += nwaves) {
+    double z[MAXND];
+    for (int o = 0; o < nd; ++o) z[o] = 0.0;
+    for (int f = lane; f < h; f += 64) {
+      const double hv = H[(size_t)r * h + f];
+      for (int o = 0; o < nd; ++o) z[o] += hv * W[f * nd + o];
+    }
+    double eps = 0.0;
+    for (int o = 0; o < nd; ++o) {
+      const double zz = warp_sum(z[o]) + b[o];
+      const double c = 2.0 / (1.0 + exp(-zz / 2.0));
+      if (lane == 0) Cout[(size_t)r * nd + o] = c;
+      eps += c * E[(size_t)r * nd + o];
+    }
+    eps = clip30(clip30(eps));
+    if (lane == 0) contrib += clip30(weights[r]) * eps;
+  }
+  const double t = block_sum(contrib, red);
+  if (threadIdx.x == 0) eps_part[blockIdx.x] = t;
+}
+
+// Adjoint seeds.  g = g_exc * clip(w) * [|eps| > 1e-30]; GE = dE/de_f = g c_f;
+// neural: DZ = dE/dz = g e_f (c_f/2)(1 - c_f/2), G = (DZ W^T) * elu'(H).
+__global__ __launch_bounds__(NT) void head_bwd_kernel(
+    const double* __restrict__ H, int h, int nd, const double* __restrict__ W,
+    const double* __restrict__ Cc, const double* __restrict__ E,
+    const double* __restrict__ weights, int npts, int npts_pad, const double* __restrict__ g_exc,
+    int neural, double* __restrict__ GE, double* __restrict__ DZ, double* __restrict__ G) {
+  const int lane = threadIdx.x & 63;
+  const int wave_global = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+  const int nwaves = gridDim.x * (NT / 64);
+  const double gE = *g_exc;
+  const double bc[4] = {1.0 - 0.2, 0.72, 1.0 - 0.81, 0.81};
+  for (int r = wave_global; r < npts_pad; r += nwaves) {
+    double c[MAXND], e[MAXND], dz[MAXND];
+    double eps = 0.0;
+    // padded rows (r >= npts) have no coefficients stored: they take c = 0, seed 0
+    for (int o = 0; o < nd; ++o) {
+      c[o] = r >= npts ? 0.0 : (neural ? Cc[(size_t)r * nd + o] : bc[o]);
+      e[o] = E[(size_t)r * nd + o];
+      eps += c[o] * e[o];
+    }
+    const double g = (r < npts && fabs(eps) > 1e-30) ? gE * clip30(weights[r]) : 0.0;
+    for (int o = 0; o < nd; ++o) {
+      dz[o] = g * e[o] * (c[o] * 0.5) * (1.0 - c[o] * 0.5);
+      if (lane == 0) {
+        GE[(size_t)r * nd + o] = g * c[o];
+        if (neural) DZ[(size_t)r * nd + o] = dz[o];
+      }
+    }
+    if (neural) {
+      for (int f = lane; f < h; f += 64) {
+        double v = 0.0;
+        for (int o = 0; o < nd; ++o) v += dz[o] * W[f * nd + o];
+        G[(size_t)r * h + f] = v * elu_grad_from_out(H[(size_t)r * h + f]);
+      }
+    }
+  }
+}
+
+// dE/dx = G1 (npts_pad, h) W0^T -> DX (npts_pad, nc); one wave per point.
+__global__ __launch_bounds__(NT) void layer0_bwd_kernel(const double* __restrict__ G, int h, int nc,
+                                                        const double* __restrict__ W, int npts_pad,
+                                                        double* __restrict__ DX) {
+  const int lane = threadIdx.x & 63;
+  const int wave_global = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+  const int nwaves = gridDim.x * (NT / 64);
+  for (int r = wave_global; r < npts_pad; r += nwaves) {
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int f = lane; f < h; f += 64) {
+      const double gv = G[(size_t)r * h + f];
+      for (int j = 0; j < nc; ++j) acc[j] += gv * W[j * h + f];
+    }
+    for (int j = 0; j < nc; ++j) {
+      const double v = warp_sum(acc[j]);
+      if (lane == 0) DX[(size_t)r * nc + j] = v;
+    }
+  }
+}
+
+// Per-point potentials for the V_xc build from dE/de_f (GE) and dE/dx (DX, neural).
+__global__ __launch_bounds__(NT) void point_bwd_kernel(
+    const double* __restrict__ dens_fin, const double* __restrict__ GE,
+    const double* __restrict__ DX, const double* __restrict__ X, int npts, int npts_pad,
+    double* __restrict__ vpt) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= npts_pad) return;
+  double out[2][VPT_STRIDE];
+  for (int s = 0; s < 2; ++s)
+    for (int q = 0; q < VPT_STRIDE; ++q) out[s][q] = 0.0;
+  if (r < npts) {
+    const double* d0 = dens_fin + ((size_t)r * 2 + 0) * DENS_STRIDE;
+    const double* d1 = dens_fin + ((size_t)r * 2 + 1) * DENS_STRIDE;
+    const double gu = d0[1] * d0[1] + d0[2] * d0[2] + d0[3] * d0[3];
+    const double gd = d1[1] * d1[1] + d1[2] * d1[2] + d1[3] * d1[3];
+    typedef Dual<6> D6;
+    D6 e[4];
+    b3lyp_parts<D6>(dvar<6>(d0[0], 0), dvar<6>(d1[0], 1), dvar<6>(gu, 2), dvar<6>(gd, 3),
+                    dvar<6>(d0[5], 4), dvar<6>(d1[5], 5), e);
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (int f = 0; f < 4; ++f) {
+      const double ge = GE[(size_t)r * 4 + f];
+      for (int i = 0; i < 6; ++i) v[i] += ge * e[f].d[i];
+    }
+    double vtau[2] = {0.0, 0.0};
+    if (DX != nullptr) {
+      const double feat[6] = {d0[0], d1[0], gu, gd, d0[4], d1[4]};
+      double df[6];
+      for (int j = 0; j < 6; ++j) {
+        const double sg = feat[j] > 0.0 ? 1.0 : (feat[j] < 0.0 ? -1.0 : 0.0);
+        df[j] = DX[(size_t)r * 6 + j] * sg / (fabs(feat[j]) + SQUASH);
+      }
+      v[0] += df[0]; v[1] += df[1]; v[2] += df[2]; v[3] += df[3];
+      vtau[0] = df[4]; vtau[1] = df[5];
+    }
+    const double* ds[2] = {d0, d1};
+    for (int s = 0; s < 2; ++s) {
+      const double vrho = v[s], vg = v[2 + s], vl = v[4 + s];
+      out[s][0] = vrho;
+      out[s][1] = 4.0 * vg * ds[s][1];
+      out[s][2] = 4.0 * vg * ds[s][2];
+      out[s][3] = 4.0 * vg * ds[s][3];
+      out[s][4] = 2.0 * vl;
+      out[s][5] = 0.5 * vtau[s] + 2.0 * vl;
+    }
+  }
+  for (int s = 0; s < 2; ++s)
+    for (int q = 0; q < VPT_STRIDE; ++q) vpt[((size_t)s * npts_pad + r) * VPT_STRIDE + q] = out[s][q];
+}
+
+// ------------------------------------------------------------------ V_xc (f64) ----
+// Output tile 64 x 64 of V_s for one chunk of points; K' = 4 components x VKP points per
+// LDS stage: A rows = (phi, dx, dy, dz)[r][a], B rows = (Z, u dx, u dy, u dz)[r][b].
+constexpr int VBT = 64;
+constexpr int VKP = 8;
+
+__global__ __launch_bounds__(NT) void vxc_f64_kernel(
+    const double* __restrict__ ao, int npts_pad, int nP, const double* __restrict__ vpt,
+    int chunk, double* __restrict__ vpart) {
+  __shared__ double As[4 * VKP][VBT + 2];
+  __shared__ double Bs[4 * VKP][VBT + 2];
+  __shared__ double sv[VKP][VPT_STRIDE];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ntile = (nP + VBT - 1) / VBT;
+  const int a0 = (blockIdx.x / ntile) * VBT, b0 = (blockIdx.x % ntile) * VBT;
+  const int s = blockIdx.z;
+  const int rbeg = blockIdx.y * chunk;
+  const int rend = min(npts_pad, rbeg + chunk);
+  const size_t plane = (size_t)npts_pad * nP;
+  const double* vp = vpt + (size_t)s * npts_pad * VPT_STRIDE;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+
+  d4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+
+  for (int r0 = rbeg; r0 < rend; r0 += VKP) {
+    if (tid < VKP * VPT_STRIDE) {
+      const int p = tid / VPT_STRIDE, q = tid % VPT_STRIDE;
+      sv[p][q] = (r0 + p < rend) ? vp[(size_t)(r0 + p) * VPT_STRIDE + q] : 0.0;
+    }
+    __syncthreads();
+    for (int e = tid; e < VKP * VBT; e += NT) {
+      const int p = e / VBT, x = e % VBT;
+      const int r = r0 + p;
+      const int ca = a0 + x, cb = b0 + x;
+      const bool okr = r < rend;
+      // A rows
+      for (int c = 0; c < 4; ++c)
+        As[c * VKP + p][x] = (okr && ca < nP) ? ao[c * plane + (size_t)r * nP + ca] : 0.0;
+      // B rows
+      double phi = 0, gx = 0, gy = 0, gz = 0, lp = 0;
+      if (okr && cb < nP) {
+        const size_t o = (size_t)r * nP + cb;
+        phi = ao[o]; gx = ao[plane + o]; gy = ao[2 * plane + o]; gz = ao[3 * plane + o];
+        lp = ao[4 * plane + o];
+      }
+      Bs[p][x] = sv[p][0] * phi + sv[p][1] * gx + sv[p][2] * gy + sv[p][3] * gz + sv[p][4] * lp;
+      Bs[VKP + p][x] = sv[p][5] * gx;
+      Bs[2 * VKP + p][x] = sv[p][5] * gy;
+      Bs[3 * VKP + p][x] = sv[p][5] * gz;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < VKP; ++kk) {
+      const int kr = kk * 4 + (lane >> 4);
+      double af[2], bf[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) af[i] = As[kr][wm + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bf[j] = Bs[kr][wn + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  double* out = vpart + ((size_t)blockIdx.y * 2 + s) * nP * nP;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = b0 + wn + j * 16 + (lane & 15);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = a0 + wm + i * 16 + mfma64_row(lane, q);
+        if (row < nP && col < nP) out[(size_t)row * nP + col] = acc[i][j][q];
+      }
+    }
+}
+
+// ------------------------------------------------------------ workspace layout ----
+struct Layout {
+  size_t Dp, dens_part, dens_fin, E, X, C, H, eps_part, GE, DZ, G, DX, vpt, vpart, wpart, cpart;
+  size_t total;
+  int ncb, wcb, nsd, nchunk, chunk, nsplit, kchunk, nparam, npt_blocks;
+};
+
+constexpr int VCHUNK = 1024;  // points per V_xc split
+
+Layout make_layout(const gdx_xc_desc& d) {
+  Layout L{};
+  const size_t Np = d.npts_pad, nP = d.nao_pad;
+  const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
+  L.wcb = nP <= 64 ? 1 : 2;
+  L.ncb = (int)((nP + 64 * L.wcb - 1) / (64 * L.wcb));
+  L.nsd = d.spin_restricted ? 1 : 2;
+  L.chunk = VCHUNK;
+  L.nchunk = (int)((Np + VCHUNK - 1) / VCHUNK);
+  L.nsplit = (int)std::max<size_t>(1, std::min<size_t>(64, Np / 512));
+  L.kchunk = (int)round_up((Np + L.nsplit - 1) / L.nsplit, GBK);
+  L.npt_blocks = (int)((Np + NT - 1) / NT);
+  const size_t h = neural ? d.mlp_h : 0, nh = neural ? d.mlp_nhidden : 0;
+  size_t np = 0;
+  if (neural) {
+    int din = d.mlp_nc;
+    for (int l = 0; l <= d.mlp_nhidden; ++l) {
+      const int dout = l == d.mlp_nhidden ? d.mlp_nd : d.mlp_h;
+      np += (size_t)din * dout + dout;
+      din = dout;
+    }
+  }
+  L.nparam = (int)np;
+  size_t off = 0;
+  auto take = [&](size_t ndoubles) { size_t o = off; off += round_up(ndoubles * 8, 256); return o; };
+  L.Dp = take(2 * nP * nP);
+  L.dens_part = take((size_t)L.ncb * L.nsd * Np * DENS_STRIDE);
+  L.dens_fin = take(Np * 2 * DENS_STRIDE);
+  L.E = take(Np * 4);
+  L.X = take(neural ? Np * d.mlp_nc : 0);
+  L.C = take(neural ? Np * d.mlp_nd : 0);
+  L.H = take(Np * h * nh);
+  L.eps_part = take(std::max<size_t>(L.npt_blocks, 4096));
+  L.GE = take(Np * 4);
+  L.DZ = take(neural ? Np * d.mlp_nd : 0);
+  L.G = take(neural ? 2 * Np * h : 0);
+  L.DX = take(neural ? Np * d.mlp_nc : 0);
+  L.vpt = take(2 * Np * VPT_STRIDE);
+  L.vpart = take((size_t)L.nchunk * 2 * nP * nP);
+  const size_t wmax = neural ? std::max<size_t>((size_t)d.mlp_h * d.mlp_h,
+                                                (size_t)d.mlp_h * std::max(d.mlp_nc, d.mlp_nd)) : 0;
+  L.wpart = take((size_t)L.nsplit * wmax);
+  L.cpart = take(neural ? (size_t)L.nsplit * d.mlp_h : 0);
+  L.total = off;
+  return L;
+}
+
+int validate(const gdx_xc_desc* d) {
+  GDX_REQUIRE(d != nullptr, "null gdx_xc_desc");
+  GDX_REQUIRE(d->npts > 0 && d->nao > 0, "empty grid or basis");
+  GDX_REQUIRE(d->npts_pad >= d->npts && d->npts_pad % 64 == 0, "npts_pad must be a multiple of 64 >= npts");
+  GDX_REQUIRE(d->nao_pad >= d->nao && d->nao_pad % 16 == 0, "nao_pad must be a multiple of 16 >= nao");
+  GDX_REQUIRE(d->kind == GDX_FUNC_B3LYP || d->kind == GDX_FUNC_NEURAL_B3LYP, "unknown functional kind");
+  if (d->precision != GDX_FP64) {
+    set_error("this build implements the FP64 grid path only");
+    return GDX_ERR_UNSUPPORTED;
+  }
+  if (d->kind == GDX_FUNC_NEURAL_B3LYP) {
+    GDX_REQUIRE(d->mlp_nc == 6, "coefficient inputs must be 6 (rho, |grad rho|^2, tau per spin)");
+    GDX_REQUIRE(d->mlp_nd == 4, "MLP outputs must equal the 4 B3LYP energy densities");
+    GDX_REQUIRE(d->mlp_h >= 16 && d->mlp_h % 16 == 0, "MLP width must be a positive multiple of 16");
+    GDX_REQUIRE(d->mlp_nhidden >= 1 && d->mlp_nhidden <= 16, "MLP hidden layers must be 1..16");
+  }
+  return GDX_OK;
+}
+
+inline unsigned nblocks(long n) { return (unsigned)((n + NT - 1) / NT); }
+
+template <int BM, int BN, bool TA, bool TB, int EPI>
+void launch_gemm(hipStream_t st, int M, int N, int K, const double* A, int lda, const double* B,
+                 int ldb, double* C, int ldc, const double* bias, const double* Y, int ldy,
+                 int nsplit, int kchunk, double* colsum) {
+  dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, nsplit);
+  hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, TA, TB, EPI>), grid, dim3(NT), 0, st, M, N, K, A, lda,
+                     B, ldb, C, ldc, bias, Y, ldy, kchunk, colsum);
+}
+
+}  // namespace
+}  // namespace gdx
+
+using namespace gdx;
+
+extern "C" {
+
+int gdx_xc_workspace_size(const gdx_xc_desc* desc, size_t* bytes) {
+  int rc = validate(desc);
+  if (rc != GDX_OK) return rc;
+  GDX_REQUIRE(bytes != nullptr, "null output pointer");
+  *bytes = make_layout(*desc).total;
+  return GDX_OK;
+}
+
+int gdx_xc_forward(void* stream, const gdx_xc_desc* desc, const double* ao, const double* weights,
+                   const double* rdm1, const double* params, double* exc, void* workspace) {
+  int rc = validate(desc);
+  if (rc != GDX_OK) return rc;
+  const gdx_xc_desc& d = *desc;
+  const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
+  GDX_REQUIRE(ao && weights && rdm1 && exc && workspace, "gdx_xc_forward: null pointer");
+  GDX_REQUIRE(!neural || params, "gdx_xc_forward: neural functional needs params");
+  hipStream_t st = (hipStream_t)stream;
+  const Layout L = make_layout(d);
+  char* ws = (char*)workspace;
+  auto P = [&](size_t off) { return (double*)(ws + off); };
+  const int Np = d.npts_pad, nP = d.nao_pad;
+
+  hipLaunchKernelGGL(pad_dm_kernel, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, rdm1, d.nao, nP, P(L.Dp));
+  if (L.wcb == 1)
+    hipLaunchKernelGGL(density_f64_kernel<1>, dim3(Np / DPT, L.ncb, L.nsd), dim3(NT), 0, st, ao, Np,
+                       nP, P(L.Dp), P(L.dens_part));
+  else
+    hipLaunchKernelGGL(density_f64_kernel<2>, dim3(Np / DPT, L.ncb, L.nsd), dim3(NT), 0, st, ao, Np,
+                       nP, P(L.Dp), P(L.dens_part));
+  hipLaunchKernelGGL(point_fwd_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.dens_part), L.ncb,
+                     L.nsd, d.npts, Np, weights, neural ? 1 : 0, P(L.dens_fin), P(L.E), P(L.X),
+                     P(L.eps_part));
+  int nred = L.npt_blocks;
+  if (neural) {
+    const int h = d.mlp_h;
+    const double* W = params;
+    const double* b = W + (size_t)d.mlp_nc * h;
+    double* H = P(L.H);
+    hipLaunchKernelGGL(layer0_kernel, dim3(nblocks((long)Np * h)), dim3(NT), 0, st, P(L.X), d.mlp_nc, h,
+                       (long)Np * h, W, b, H);
+    const double* pw = b + h;
+    for (int l = 1; l < d.mlp_nhidden; ++l) {
+      const double* Wl = pw;
+      const double* bl = Wl + (size_t)h * h;
+      launch_gemm<128, 128, false, false, EPI_BIAS_ELU>(st, Np, h, h, H + (size_t)(l - 1) * Np * h, h,
+                                                        Wl, h, H + (size_t)l * Np * h, h, bl, nullptr,
+                                                        0, 1, h, nullptr);
+      pw = bl + h;
+    }
+    const double* Wlast = pw;
+    const double* blast = Wlast + (size_t)h * d.mlp_nd;
+    const int hb = 1024;
+    hipLaunchKernelGGL(head_fwd_kernel, dim3(hb), dim3(NT), 0, st,
+                       H + (size_t)(d.mlp_nhidden - 1) * Np * h, h, d.mlp_nd, Wlast, blast, P(L.E),
+                       weights, d.npts, P(L.C), P(L.eps_part));
+    nred = hb;
+  }
+  hipLaunchKernelGGL(reduce_scalar_kernel, dim3(1), dim3(NT), 0, st, P(L.eps_part), nred, exc);
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+int gdx_xc_densities(void* stream, const gdx_xc_desc* desc, const double* ao, const double* rdm1,
+                     double* dens, void* workspace) {
+  int rc = validate(desc);
+  if (rc != GDX_OK) return rc;
+  const gdx_xc_desc& d = *desc;
+  GDX_REQUIRE(ao && rdm1 && dens && workspace, "gdx_xc_densities: null pointer");
+  hipStream_t st = (hipStream_t)stream;
+  const Layout L = make_layout(d);
+  char* ws = (char*)workspace;
+  auto P = [&](size_t off) { return (double*)(ws + off); };
+  const int Np = d.npts_pad, nP = d.nao_pad;
+  hipLaunchKernelGGL(pad_dm_kernel, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, rdm1, d.nao, nP, P(L.Dp));
+  if (L.wcb == 1)
+    hipLaunchKernelGGL(density_f64_kernel<1>, dim3(Np / DPT, L.ncb, L.nsd), dim3(NT), 0, st, ao, Np,
+                       nP, P(L.Dp), P(L.dens_part));
+  else
+    hipLaunchKernelGGL(density_f64_kernel<2>, dim3(Np / DPT, L.ncb, L.nsd), dim3(NT), 0, st, ao, Np,
+                       nP, P(L.Dp), P(L.dens_part));
+  // weights == nullptr: the per-point kernel forms densities and energy densities only
+  const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
+  hipLaunchKernelGGL(point_fwd_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.dens_part), L.ncb,
+                     L.nsd, d.npts, Np, nullptr, neural ? 1 : 0, dens, P(L.E), P(L.X), P(L.eps_part));
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const double* ao,
+                    const double* weights, const double* rdm1, const double* params,
+                    const double* g_exc, double* d_rdm1, double* d_params, void* workspace) {
+  int rc = validate(desc);
+  if (rc != GDX_OK) return rc;
+  const gdx_xc_desc& d = *desc;
+  const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
+  GDX_REQUIRE(ao && weights && rdm1 && g_exc && workspace, "gdx_xc_backward: null pointer");
+  GDX_REQUIRE(!neural || params, "gdx_xc_backward: neural functional needs params");
+  hipStream_t st = (hipStream_t)stream;
+  const Layout L = make_layout(d);
+  char* ws = (char*)workspace;
+  auto P = [&](size_t off) { return (double*)(ws + off); };
+  const int Np = d.npts_pad, nP = d.nao_pad;
+  const int h = neural ? d.mlp_h : 0;
+  const int nd = 4;
+
+  const double* Wlast = nullptr;
+  if (neural) {
+    // parameter offsets
+    std::vector<size_t> woff, boff;
+    size_t off = 0;
+    int din = d.mlp_nc;
+    for (int l = 0; l <= d.mlp_nhidden; ++l) {
+      const int dout = l == d.mlp_nhidden ? d.mlp_nd : h;
+      woff.push_back(off); off += (size_t)din * dout;
+      boff.push_back(off); off += dout;
+      din = dout;
+    }
+    Wlast = params + woff.back();
+    double* H = P(L.H);
+    double* G0 = P(L.G);
+    double* G1 = P(L.G) + (size_t)Np * h;
+    const double* Hlast = H + (size_t)(d.mlp_nhidden - 1) * Np * h;
+    hipLaunchKernelGGL(head_bwd_kernel, dim3(1024), dim3(NT), 0, st, Hlast, h, nd, Wlast, P(L.C), P(L.E),
+                       weights, d.npts, Np, g_exc, 1, P(L.GE), P(L.DZ), G0);
+    const int nl = d.mlp_nhidden;
+    // last layer weight grads: dW = H_last^T DZ (h x nd), db = colsum(DZ)
+    if (d_params) {
+      launch_gemm<64, 64, true, false, EPI_STORE>(st, h, nd, Np, Hlast, h, P(L.DZ), nd, P(L.wpart), nd,
+                                                  nullptr, nullptr, 0, L.nsplit, L.kchunk, P(L.cpart));
+      hipLaunchKernelGGL(reduce_partials_kernel, dim3(nblocks((long)h * nd)), dim3(NT), 0, st, P(L.wpart),
+                         L.nsplit, (long)h * nd, (long)h * nd, d_params + woff[nl]);
+      hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(NT), 0, st, P(L.cpart), L.nsplit,
+                         (long)nd, (long)nd, d_params + boff[nl]);
+    }
+    // hidden h x h layers, from the top
+    double* Gcur = G0;
+    double* Gnext = G1;
+    for (int l = nl - 1; l >= 1; --l) {
+      const double* Hin = H + (size_t)(l - 1) * Np * h;  // input activations of layer l
+      const double* Wl = params + woff[l];
+      if (d_params) {
+        launch_gemm<64, 64, true, false, EPI_STORE>(st, h, h, Np, Hin, h, Gcur, h, P(L.wpart), h, nullptr,
+                                                    nullptr, 0, L.nsplit, L.kchunk, P(L.cpart));
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3(nblocks((long)h * h)), dim3(NT), 0, st, P(L.wpart),
+                           L.nsplit, (long)h * h, (long)h * h, d_params + woff[l]);
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3(nblocks(h)), dim3(NT), 0, st, P(L.cpart), L.nsplit,
+                           (long)h, (long)h, d_params + boff[l]);
+      }
+      // delta for layer l's input: (Gcur W_l^T) * elu'(Hin)
+      launch_gemm<128, 128, false, true, EPI_ELU_BWD>(st, Np, h, h, Gcur, h, Wl, h, Gnext, h, nullptr, Hin,
+                                                      h, 1, h, nullptr);
+      std::swap(Gcur, Gnext);
+    }
+    // first layer: dW0 = X^T G (nc x h), db0 = colsum(G), DX = G W0^T
+    if (d_params) {
+      launch_gemm<64, 64, true, false, EPI_STORE>(st, d.mlp_nc, h, Np, P(L.X), d.mlp_nc, Gcur, h, P(L.wpart),
+                                                  h, nullptr, nullptr, 0, L.nsplit, L.kchunk, P(L.cpart));
+      hipLaunchKernelGGL(reduce_partials_kernel, dim3(nblocks((long)d.mlp_nc * h)), dim3(NT), 0, st,
+                         P(L.wpart), L.nsplit, (long)d.mlp_nc * h, (long)d.mlp_nc * h, d_params + woff[0]);
+      hipLaunchKernelGGL(reduce_partials_kernel, dim3(nblocks(h)), dim3(NT), 0, st, P(L.cpart), L.nsplit,
+                         (long)h, (long)h, d_params + boff[0]);
+    }
+    hipLaunchKernelGGL(layer0_bwd_kernel, dim3(1024), dim3(NT), 0, st, Gcur, h, d.mlp_nc, params + woff[0], Np,
+                       P(L.DX));
+  } else {
+    hipLaunchKernelGGL(head_bwd_kernel, dim3(1024), dim3(NT), 0, st, nullptr, 0, nd, nullptr, nullptr, P(L.E),
+                       weights, d.npts, Np, g_exc, 0, P(L.GE), nullptr, nullptr);
+  }
+  if (d_rdm1) {
+    hipLaunchKernelGGL(point_bwd_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.dens_fin), P(L.GE),
+                       neural ? P(L.DX) : nullptr, P(L.X), d.npts, Np, P(L.vpt));
+    const int ntile = (nP + VBT - 1) / VBT;
+    hipLaunchKernelGGL(vxc_f64_kernel, dim3(ntile * ntile, L.nchunk, 2), dim3(NT), 0, st, ao, Np, nP, P(L.vpt),
+                       L.chunk, P(L.vpart));
+    hipLaunchKernelGGL(reduce_vxc_kernel, dim3(nblocks(2L * d.nao * d.nao)), dim3(NT), 0, st, P(L.vpart),
+                       L.nchunk, nP, d.nao, d_rdm1);
+  }
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,202 @@
+"""Functional plugin API over the gfx950 XC kernels.
+
+Mirrors ``grad_dft/functional.py``: a :class:`Functional` maps a molecule's density
+matrix to E_xc; :class:`NeuralFunctional` adds a coefficient MLP.  The reference lets a
+functional be any composition of Python callables (``functional.py:128-141``); here a
+functional is *declarative* -- a kind (``b3lyp`` energy densities, optionally weighted by
+an MLP of fixed architecture) -- because every piece of it runs in hand-written kernels.
+
+Entry points kept from the reference:
+
+* ``Functional.xc_energy(params, molecule)``   -- ``functional.py:219-253``
+* ``Functional.energy(params, molecule)``      -- ``functional.py:255-288``
+  (E_xc + ``molecule.nonXC()`` when ``is_xc``)
+* ``NeuralFunctional.init(seed)``             -- parameter initialisation (Glorot kernels,
+  zero biases, as ``flax.linen.Dense`` with a Glorot initialiser)
+
+Both are differentiable through :class:`XCEnergyFn` (a ``torch.autograd.Function``) whose
+backward is the hand-written adjoint ``gdx_xc_backward`` (dE/dD and dE/dtheta).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .molecule import Molecule
+
+PRECISIONS = {"fp64": _lib.GDX_FP64, "fp32": _lib.GDX_FP32, "bf16x3": _lib.GDX_BF16X3}
+
+
+@dataclass(frozen=True)
+class MLPSpec:
+    """Coefficient network: log(|x| + 1e-4) -> [dense + ELU] x n_hidden -> dense ->
+    2 sigmoid(z / 2) (the NeuralFunctional head, ``functional.py:407-419``)."""
+    nc: int = 6
+    h: int = 256
+    n_hidden: int = 3
+    nd: int = 4
+
+    def shapes(self):
+        dims = [self.nc] + [self.h] * self.n_hidden + [self.nd]
+        return [(dims[i], dims[i + 1]) for i in range(len(dims) - 1)]
+
+    def n_params(self) -> int:
+        return sum(i * o + o for i, o in self.shapes())
+
+    def flops_per_point(self) -> int:
+        return 2 * sum(i * o for i, o in self.shapes())
+
+
+class XCPlan:
+    """Descriptor + device workspace for one (molecule, functional, precision).
+
+    The workspace carries forward intermediates to the backward call, so a plan serves one
+    forward/backward pair at a time; ``generation`` guards against interleaving."""
+
+    def __init__(self, molecule: Molecule, kind: int, spec: Optional[MLPSpec], precision: str):
+        lib = _lib.load()
+        d = _lib.XCDesc()
+        d.npts = molecule.grid_size
+        d.npts_pad = molecule.npts_pad
+        d.nao = molecule.nao
+        d.nao_pad = molecule.nao_pad
+        d.spin_restricted = 1 if molecule.restricted else 0
+        d.kind = kind
+        d.precision = PRECISIONS[precision]
+        if spec is not None:
+            d.mlp_nc, d.mlp_h, d.mlp_nhidden, d.mlp_nd = spec.nc, spec.h, spec.n_hidden, spec.nd
+        self.desc = d
+        nbytes = ctypes.c_size_t(0)
+        _lib.check(lib.gdx_xc_workspace_size(ctypes.byref(d), ctypes.byref(nbytes)), "gdx_xc_workspace_size")
+        self.workspace = torch.empty(int(nbytes.value), dtype=torch.uint8, device=molecule.ao_planes.device)
+        self.molecule = molecule
+        self.generation = 0
+        self.fwd_generation = -1
+
+    @staticmethod
+    def _ptr(t: Optional[torch.Tensor]) -> int:
+        return 0 if t is None else t.data_ptr()
+
+    def forward(self, rdm1: torch.Tensor, params: Optional[torch.Tensor], exc: torch.Tensor) -> None:
+        m = self.molecule
+        st = torch.cuda.current_stream(m.ao_planes.device).cuda_stream
+        rc = _lib.load().gdx_xc_forward(st, ctypes.byref(self.desc), m.ao_planes.data_ptr(),
+                                        m.grid.weights_pad.data_ptr(), rdm1.data_ptr(),
+                                        self._ptr(params), exc.data_ptr(), self.workspace.data_ptr())
+        _lib.check(rc, "gdx_xc_forward")
+        self.generation += 1
+        self.fwd_generation = self.generation
+
+    def densities(self, rdm1: torch.Tensor) -> torch.Tensor:
+        """(N, 2, 6): rho, grad rho (x, y, z), tau, laplacian per spin."""
+        m = self.molecule
+        out = torch.empty((m.npts_pad, 2, 6), dtype=torch.float64, device=rdm1.device)
+        st = torch.cuda.current_stream(m.ao_planes.device).cuda_stream
+        rc = _lib.load().gdx_xc_densities(st, ctypes.byref(self.desc), m.ao_planes.data_ptr(),
+                                          rdm1.contiguous().data_ptr(), out.data_ptr(),
+                                          self.workspace.data_ptr())
+        _lib.check(rc, "gdx_xc_densities")
+        self.generation += 1
+        return out[: m.grid_size]
+
+    def backward(self, rdm1, params, g_exc, d_rdm1, d_params, generation: int) -> None:
+        if generation != self.fwd_generation:
+            raise RuntimeError("XC workspace was overwritten by another forward before this backward")
+        m = self.molecule
+        st = torch.cuda.current_stream(m.ao_planes.device).cuda_stream
+        rc = _lib.load().gdx_xc_backward(st, ctypes.byref(self.desc), m.ao_planes.data_ptr(),
+                                         m.grid.weights_pad.data_ptr(), rdm1.data_ptr(),
+                                         self._ptr(params), g_exc.data_ptr(), self._ptr(d_rdm1),
+                                         self._ptr(d_params), self.workspace.data_ptr())
+        _lib.check(rc, "gdx_xc_backward")
+
+
+class XCEnergyFn(torch.autograd.Function):
+    """E_xc(rdm1, params) with the hand-written adjoint as its backward."""
+
+    @staticmethod
+    def forward(ctx, rdm1, params, plan: XCPlan):
+        rdm1 = rdm1.contiguous()
+        p = None if params is None else params.contiguous()
+        exc = torch.empty((), dtype=torch.float64, device=rdm1.device)
+        plan.forward(rdm1, p, exc)
+        ctx.plan = plan
+        ctx.gen = plan.generation
+        ctx.has_params = p is not None
+        ctx.save_for_backward(rdm1, p if p is not None else rdm1)
+        return exc
+
+    @staticmethod
+    def backward(ctx, g):
+        rdm1, p = ctx.saved_tensors
+        p = p if ctx.has_params else None
+        d_rdm1 = torch.empty_like(rdm1) if ctx.needs_input_grad[0] else None
+        d_params = torch.empty_like(p) if (p is not None and ctx.needs_input_grad[1]) else None
+        g = g.detach().to(torch.float64).contiguous().reshape(())
+        ctx.plan.backward(rdm1, p, g, d_rdm1, d_params, ctx.gen)
+        return d_rdm1, d_params, None
+
+
+class Functional:
+    """Declarative mirror of ``grad_dft.Functional`` for the kinds the kernels implement."""
+
+    kind = _lib.GDX_FUNC_B3LYP
+    spec: Optional[MLPSpec] = None
+    spin_symmetric = True
+
+    def __init__(self, name: str = "B3LYP", is_xc: bool = True, exact_exchange: float = 0.0):
+        self.name = name
+        self.is_xc = is_xc
+        # fraction a0 of exact exchange added outside the grid (B3LYP: 0.2)
+        self.exact_exchange = exact_exchange
+
+    def plan(self, molecule: Molecule, precision: str = "fp64") -> XCPlan:
+        key = ("xcplan", self.kind, self.spec, precision, molecule.restricted)
+        p = molecule.cache.get(key)
+        if p is None or p.molecule.ao_planes is not molecule.ao_planes:
+            p = XCPlan(molecule, self.kind, self.spec, precision)
+            molecule.cache[key] = p
+        p.molecule = molecule
+        return p
+
+    def xc_energy(self, params: Optional[torch.Tensor], molecule: Molecule,
+                  precision: str = "fp64", rdm1: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Grid part of E_xc (functional.py:219-253), differentiable w.r.t. rdm1/params."""
+        D = molecule.rdm1 if rdm1 is None else rdm1
+        return XCEnergyFn.apply(D, params, self.plan(molecule, precision))
+
+    def energy(self, params: Optional[torch.Tensor], molecule: Molecule, precision: str = "fp64"):
+        """Total energy (functional.py:255-288): E_xc (+ nonXC when is_xc)."""
+        from .scf import non_xc_energy, exact_exchange_energy
+        e = self.xc_energy(params, molecule, precision)
+        if self.exact_exchange:
+            e = e + exact_exchange_energy(molecule, molecule.rdm1, self.exact_exchange)
+        if self.is_xc:
+            e = e + non_xc_energy(molecule, molecule.rdm1)
+        return e
+
+
+class NeuralFunctional(Functional):
+    """B3LYP energy densities weighted by a coefficient MLP (``functional.py:345-419``)."""
+
+    kind = _lib.GDX_FUNC_NEURAL_B3LYP
+    spin_symmetric = False
+
+    def __init__(self, spec: MLPSpec = MLPSpec(), name: str = "neural_b3lyp", is_xc: bool = True):
+        super().__init__(name=name, is_xc=is_xc)
+        self.spec = spec
+
+    def init(self, seed: int = 0, device: str = "cuda") -> torch.Tensor:
+        """Glorot-normal kernels and zero biases, flat [W0, b0, W1, b1, ...], W as (in, out)."""
+        rng = np.random.default_rng(seed)
+        parts = []
+        for i, o in self.spec.shapes():
+            parts.append(rng.normal(0.0, math.sqrt(2.0 / (i + o)), size=(i, o)).ravel())
+            parts.append(np.zeros(o))
+        return torch.as_tensor(np.concatenate(parts), dtype=torch.float64, device=device)

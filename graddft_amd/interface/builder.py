@@ -1,0 +1,103 @@
+"""Build a device-resident :class:`~graddft_amd.molecule.Molecule` from a geometry.
+
+Plays the role of ``molecule_from_pyscf`` (``grad_dft/interface/pyscf.py:105-201``) and
+its packaging step ``_package_outputs`` (``:808-864``) without PySCF:
+
+* basis shells from :mod:`.basis`, integration grid from :mod:`.grids`;
+* S, T, V (and optionally the packed ERI) from the host integral library;
+* the initial density from the core-Hamiltonian guess (lowest ``nelec/2`` orbitals of
+  ``h1e C = S C e``), split into two equal spin halves exactly as the reference splits an
+  RKS density (``pyscf.py:830-838``: ``rdm1 / 2`` per spin, ``mo_occ / 2``);
+* AO planes evaluated on the device by ``gdx_ao_eval``.
+"""
+from __future__ import annotations
+
+from typing import Optional, Sequence
+
+import numpy as np
+import scipy.linalg
+import torch
+
+from .. import _lib
+from ..molecule import Grid, Molecule
+from .basis import atomic_numbers, build_basis, nuclear_repulsion, pack_shells
+from .grids import molecular_grid
+from .integrals import int1e, int2e_packed
+
+
+def pad_to(x: int, m: int) -> int:
+    return (x + m - 1) // m * m
+
+
+def eval_ao_device(basis, coords: np.ndarray, npts_pad: int, nao_pad: int,
+                   device: str = "cuda") -> torch.Tensor:
+    """AO planes (5, npts_pad, nao_pad) on the device via the HIP kernel."""
+    lib = _lib.load()
+    pk = pack_shells(basis)
+    t = {k: torch.as_tensor(v, device=device) for k, v in pk.items()}
+    c = torch.as_tensor(np.ascontiguousarray(coords, dtype=np.float64), device=device)
+    ao = torch.empty((5, npts_pad, nao_pad), dtype=torch.float64, device=device)
+    stream = torch.cuda.current_stream(device).cuda_stream
+    rc = lib.gdx_ao_eval(stream, len(basis.shells), t["shell_i32"].data_ptr(), t["shell_f64"].data_ptr(),
+                         t["prim_f64"].data_ptr(), t["tmat_f64"].data_ptr(), len(coords), npts_pad,
+                         c.data_ptr(), basis.nao, nao_pad, ao.data_ptr())
+    _lib.check(rc, "gdx_ao_eval")
+    return ao
+
+
+def core_guess(h1e: np.ndarray, s1e: np.ndarray, nocc: int):
+    e, C = scipy.linalg.eigh(h1e, s1e)
+    occ = np.zeros(len(e))
+    occ[:nocc] = 1.0
+    D = (C[:, :nocc] @ C[:, :nocc].T)
+    return D, C, e, occ
+
+
+def molecule_from_geometry(symbols: Sequence[str], coords_bohr: np.ndarray, basis: str,
+                           grid_level: int = 2, charge: int = 0, with_eri: bool = False,
+                           device: str = "cuda", rdm1: Optional[np.ndarray] = None,
+                           name: str = "", npts_align: int = 128) -> Molecule:
+    """Closed-shell molecule with a core-Hamiltonian (or given) density, AOs on device."""
+    coords_bohr = np.asarray(coords_bohr, dtype=np.float64)
+    Z = atomic_numbers(symbols)
+    bs = build_basis(symbols, coords_bohr, basis)
+    gc, gw = molecular_grid(Z, coords_bohr, grid_level)
+    S, T, V = int1e(bs, Z, coords_bohr)
+    h1e = T + V
+    nelec = sum(Z) - charge
+    if nelec % 2:
+        raise ValueError("only closed-shell molecules are supported")
+    nocc = nelec // 2
+    D, C, e, occ = core_guess(h1e, S, nocc)
+    if rdm1 is not None:
+        D = np.asarray(rdm1, dtype=np.float64)
+    n = bs.nao
+    N = len(gw)
+    npts_pad = pad_to(N, npts_align)
+    nao_pad = pad_to(n, 16)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=device)  # noqa
+    wpad = np.zeros(npts_pad)
+    wpad[:N] = gw
+    grid = Grid(coords=dev(gc), weights=dev(gw), weights_pad=dev(wpad))
+    ao = eval_ao_device(bs, gc, npts_pad, nao_pad, device)
+    eri = dev(int2e_packed(bs)) if with_eri else None
+    return Molecule(
+        grid=grid,
+        atom_index=torch.as_tensor(Z, device=device),
+        nuclear_pos=dev(coords_bohr),
+        ao_planes=ao,
+        rdm1=dev(np.stack([D, D])),
+        nuclear_repulsion=nuclear_repulsion(Z, coords_bohr),
+        h1e=dev(h1e),
+        s1e=dev(S),
+        mo_coeff=dev(np.stack([C, C])),
+        mo_occ=dev(np.stack([occ, occ])),
+        mo_energy=dev(np.stack([e, e])),
+        basis=bs,
+        rep_tensor=eri,
+        spin=0,
+        charge=charge,
+        restricted=True,
+        grid_level=grid_level,
+        name=name,
+    )

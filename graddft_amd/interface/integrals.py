@@ -1,0 +1,79 @@
+"""Python binding of the host integral library ``libgdx_ints.so`` (S, T, V, packed ERI).
+
+Stands in for PySCF's ``mf.get_ovlp`` / ``mf.get_hcore`` / ``mol.intor('int2e')``
+(``grad_dft/interface/pyscf.py:797, 828-829``).  The library is built from
+``graddft_amd/csrc/gdx_ints.cpp`` by :mod:`graddft_amd.build_ext`.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Sequence, Tuple
+
+import numpy as np
+
+from .basis import BasisSet, pack_shells
+
+_LIB = None
+_HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libgdx_ints.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; run graddft_amd.build_ext.build()")
+        _LIB = ctypes.CDLL(path)
+        P = ctypes.c_void_p
+        _LIB.gdx_int1e.argtypes = [ctypes.c_int, P, P, P, P, ctypes.c_int, P, P, ctypes.c_int, P, P, P]
+        _LIB.gdx_int2e_packed.argtypes = [ctypes.c_int, P, P, P, P, ctypes.c_int, P]
+    return _LIB
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def int1e(bs: BasisSet, charges: Sequence[float], atom_xyz: np.ndarray) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Overlap S, kinetic T and nuclear attraction V, each (nao, nao) float64."""
+    pk = pack_shells(bs)
+    n = bs.nao
+    S = np.zeros((n, n))
+    T = np.zeros((n, n))
+    V = np.zeros((n, n))
+    ch = np.ascontiguousarray(np.asarray(charges, dtype=np.float64))
+    xyz = np.ascontiguousarray(np.asarray(atom_xyz, dtype=np.float64))
+    rc = _lib().gdx_int1e(len(bs.shells), _ptr(pk["shell_i32"]), _ptr(pk["shell_f64"]),
+                          _ptr(pk["prim_f64"]), _ptr(pk["tmat_f64"]), len(ch), _ptr(ch),
+                          _ptr(xyz), n, _ptr(S), _ptr(T), _ptr(V))
+    if rc != 0:
+        raise RuntimeError(f"gdx_int1e failed ({rc})")
+    return S, T, V
+
+
+def npair(n: int) -> int:
+    return n * (n + 1) // 2
+
+
+def int2e_packed(bs: BasisSet) -> np.ndarray:
+    """8-fold packed (ij|kl), length P(P+1)/2 with P = nao(nao+1)/2."""
+    pk = pack_shells(bs)
+    P = npair(bs.nao)
+    eri = np.zeros(P * (P + 1) // 2)
+    rc = _lib().gdx_int2e_packed(len(bs.shells), _ptr(pk["shell_i32"]), _ptr(pk["shell_f64"]),
+                                 _ptr(pk["prim_f64"]), _ptr(pk["tmat_f64"]), bs.nao, _ptr(eri))
+    if rc != 0:
+        raise RuntimeError(f"gdx_int2e_packed failed ({rc})")
+    return eri
+
+
+def unpack_eri(eri: np.ndarray, n: int) -> np.ndarray:
+    """Dense (n,n,n,n) view of a packed ERI (small n only)."""
+    i, j = np.tril_indices(n)
+    pair = np.zeros((n, n), dtype=np.int64)
+    pair[i, j] = np.arange(len(i))
+    pair[j, i] = pair[i, j]
+    a = pair.reshape(-1)
+    A, B = np.maximum(a[:, None], a[None, :]), np.minimum(a[:, None], a[None, :])
+    return eri[A * (A + 1) // 2 + B].reshape(n, n, n, n)

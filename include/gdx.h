@@ -1,0 +1,101 @@
+/* gdx: C-ABI of the MI355X-native GradDFT exchange-correlation grid path.
+ *
+ * Every pointer argument named d_* or described as "device" is device memory owned by
+ * the caller; the library allocates nothing inside these calls.  Work is enqueued on the
+ * HIP stream passed as `stream` (a hipStream_t, NULL = default stream) and is
+ * asynchronous: results are valid once that stream has reached them.  Every entry point
+ * returns GDX_OK (0) or a negative status; gdx_last_error() returns a description of
+ * the last failure on the calling thread.
+ *
+ * Correspondence with the reference (XanaduAI/GradDFT), which has no FFI of its own --
+ * each entry point replaces the Python/JAX computation cited beside it.
+ */
+#ifndef GDX_H_
+#define GDX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GDX_OK = 0,
+  GDX_ERR_INVALID = -1,     /* bad argument or shape */
+  GDX_ERR_HIP = -2,         /* a HIP runtime call failed */
+  GDX_ERR_UNSUPPORTED = -3  /* valid request this build does not implement */
+};
+
+enum { GDX_FP64 = 0, GDX_FP32 = 1, GDX_BF16X3 = 2 };
+
+/* Functional kinds evaluated on the grid.
+ *  GDX_FUNC_B3LYP        E = sum_r w * (0.8 e_lda + 0.72 e_b88 + 0.19 e_vwn + 0.81 e_lyp)
+ *                        (the non-exact-exchange part of grad_dft/popular_functionals.py:375-393)
+ *  GDX_FUNC_NEURAL_B3LYP E = sum_r w * sum_f c_theta(x(r))_f e_f(r): the four B3LYP energy
+ *                        densities weighted by a coefficient MLP (NeuralFunctional,
+ *                        grad_dft/functional.py:345-419). */
+enum { GDX_FUNC_B3LYP = 0, GDX_FUNC_NEURAL_B3LYP = 1 };
+
+/* Shape and functional description of one XC evaluation. */
+typedef struct gdx_xc_desc {
+  int32_t npts;            /* grid points N */
+  int32_t npts_pad;        /* rows of the AO / weight buffers, multiple of 64, >= npts */
+  int32_t nao;             /* basis functions n */
+  int32_t nao_pad;         /* columns of the AO buffer, multiple of 16, >= nao */
+  int32_t spin_restricted; /* 1 when rdm1[0] == rdm1[1]: densities are formed once */
+  int32_t kind;            /* GDX_FUNC_* */
+  int32_t precision;       /* GDX_FP64 | GDX_FP32 | GDX_BF16X3 */
+  int32_t mlp_nc;          /* MLP inputs (6: rho, |grad rho|^2, tau per spin) */
+  int32_t mlp_h;           /* hidden width */
+  int32_t mlp_nhidden;     /* hidden layers (dense + ELU each) */
+  int32_t mlp_nd;          /* outputs = energy densities (4) */
+  int32_t reserved;
+} gdx_xc_desc;
+
+const char* gdx_last_error(void);
+int gdx_version(void);
+
+/* AO values on the grid: replaces PySCF numint.eval_ao(deriv=1) + ao_grads(order=2) as
+ * packaged by grad_dft/interface/pyscf.py:824-827, 677-705.
+ * Inputs (device): shell_i32 (nshell,6) {l, nprim, prim_offset, ao_offset, nfunc,
+ * tmat_offset}, shell_f64 (nshell,3) centres, prim_f64 (nprim,2) {exponent, coef},
+ * tmat_f64 component matrices, coords (npts,3).
+ * Output (device): ao, 5 planes of (npts_pad, nao_pad) doubles: phi, dphi/dx, dphi/dy,
+ * dphi/dz, laplacian(phi) (= the sum of grad_n_ao[2]'s xx, yy, zz).  Padding is zeroed. */
+int gdx_ao_eval(void* stream, int32_t nshell, const int32_t* shell_i32, const double* shell_f64,
+                const double* prim_f64, const double* tmat_f64, int32_t npts, int32_t npts_pad,
+                const double* coords, int32_t nao, int32_t nao_pad, double* ao);
+
+/* Bytes of device workspace gdx_xc_forward / gdx_xc_backward need for `desc`. */
+int gdx_xc_workspace_size(const gdx_xc_desc* desc, size_t* bytes);
+
+/* E_xc = Functional.xc_energy(params, grid, cinputs, densities) of
+ * grad_dft/functional.py:219-253 with densities and coefficient inputs formed from rdm1
+ * as in compute_densities / compute_coefficient_inputs (:160-217) and
+ * grad_dft/molecule.py:388-502.
+ * ao: gdx_ao_eval output; weights (npts_pad) grid weights, zero past npts;
+ * rdm1 (2, nao, nao); params: flat MLP parameters [W0 (nc,h), b0 (h), W1 (h,h), b1, ...,
+ * W_last (h,nd), b_last] (NULL for GDX_FUNC_B3LYP).  exc: one device double.
+ * The workspace keeps what gdx_xc_backward needs; do not touch it in between. */
+int gdx_xc_forward(void* stream, const gdx_xc_desc* desc, const double* ao, const double* weights,
+                   const double* rdm1, const double* params, double* exc, void* workspace);
+
+/* Grid densities per spin, (npts_pad, 2, 6) doubles: rho, d rho/dx, d rho/dy, d rho/dz,
+ * tau, laplacian -- Molecule.density / grad_density / kinetic_density / lapl_density of
+ * grad_dft/molecule.py:388-502 in one pass.  Uses (and overwrites) the workspace. */
+int gdx_xc_densities(void* stream, const gdx_xc_desc* desc, const double* ao, const double* rdm1,
+                     double* dens, void* workspace);
+
+/* Vector-Jacobian product of gdx_xc_forward for the cotangent g_exc (one device double):
+ * d_rdm1 (2, nao, nao) = g * dE_xc/dD_s -- the unsymmetrised Fock XC term of
+ * grad_dft/train.py:86-121 (value_and_grad w.r.t. rdm1) -- and d_params (same layout as
+ * params) = g * dE_xc/dtheta.  Either output may be NULL to skip it. */
+int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const double* ao,
+                    const double* weights, const double* rdm1, const double* params,
+                    const double* g_exc, double* d_rdm1, double* d_params, void* workspace);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GDX_H_ */
