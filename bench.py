@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""Headline benchmark: XC grid points per second, forward + adjoint, on one MI355X.
+
+Workload: benzene C6H6 at a seeded geometry (C-C 1.39 A, C-H 1.09 A, N(0, 0.02 A)
+jitter), 6-31G* with Cartesian d (nao = 102), level-3 Becke grid, per-point features
+rho, |grad rho|^2, tau per spin fed to a coefficient MLP (3 hidden layers of width 256,
+ELU, Glorot init from the seed) weighting the four B3LYP energy densities.  One step is
+E_xc plus its adjoints dE/dtheta and dE/dD over every grid point, with AO values, D and
+theta already resident in HBM.
+
+Prints ONE JSON line.  ``roofline`` is for the kernel with the largest share of the step,
+timed with HIP events on its own stream inside the timed region; ``cpu_baseline`` is the
+FP64 oracle (torch-CPU restatement of the reference) on a bounded sample of the same
+grid.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "XC grid pts/s (fwd+adjoint) & ms/SCF iter at ≤1e-6 Ha vs FP64 GPU path"
+PEAK = {"fp64": ("TFLOP/s", 78.6), "fp32": ("TFLOP/s", 157.3), "bf16x3": ("TFLOP/s", 2500.0)}
+DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3"}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--grid-level", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=16384, help="grid points in the CPU sample")
+    return ap.parse_args()
+
+
+def kernel_flops(name, npts, nao, spec, nsd):
+    """Algorithmic flops of ONE launch of the named kernel (N real points)."""
+    h = spec.h
+    table = {
+        "density_f64": nsd * 8 * nao * nao * npts,       # 4 products (phi, dphi) x D
+        "vxc_f64": 2 * 8 * nao * nao * npts,             # phi^T Z + 3 dphi^T u dphi, per spin
+        "gemm_f64_fwd": 2 * npts * h * h,
+        "gemm_f64_bwd_data": 2 * npts * h * h,
+        "gemm_f64_wgrad": 2 * npts * h * h,
+    }
+    return table.get(name)
+
+
+def cpu_baseline(mol, theta, spec, sample):
+    """FP64 oracle on the first `sample` grid points: returns (pts/s, threads, description)."""
+    import torch
+    from oracle.ao import eval_ao
+    from oracle.xc import MLPSpec as OSpec, to_inputs, xc_value_and_grads
+    nthreads = os.cpu_count() or 1
+    torch.set_num_threads(nthreads)
+    c = mol.grid.coords[:sample].cpu().numpy()
+    w = mol.grid.weights[:sample].cpu().numpy()
+    inp = to_inputs(*eval_ao(mol.basis, c), w)
+    D = mol.rdm1.cpu().numpy()
+    th = theta.cpu().numpy()
+    ospec = OSpec(spec.nc, spec.h, spec.n_hidden, spec.nd)
+    xc_value_and_grads("neural_b3lyp", inp, D, th, ospec)  # warm-up
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 5 and (time.perf_counter() - t_start) < 20.0:
+        t0 = time.perf_counter()
+        xc_value_and_grads("neural_b3lyp", inp, D, th, ospec)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return len(w) / med, torch.get_num_threads(), f"first {len(w)} points of the same grid, median of {len(times)} runs"
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the path shards grid points across GPUs by design, but only one GPU is used here)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from graddft_amd import MLPSpec, NeuralFunctional, _lib
+    from graddft_amd.interface import geometry
+    from graddft_amd.interface.builder import molecule_from_geometry
+
+    rng = np.random.default_rng(args.seed)
+    sym, xyz = geometry.benzene(rng)
+    mol = molecule_from_geometry(sym, xyz, "6-31g*", args.grid_level, name="benzene")
+    spec = MLPSpec(nc=6, h=256, n_hidden=3, nd=4)
+    f = NeuralFunctional(spec)
+    theta = f.init(seed=args.seed)
+    N, n = mol.grid_size, mol.nao
+    plan = f.plan(mol, args.precision)
+    D = mol.rdm1.contiguous()
+    exc = torch.empty((), dtype=torch.float64, device="cuda")
+    one = torch.ones((), dtype=torch.float64, device="cuda")
+    dD = torch.empty_like(D)
+    dth = torch.empty_like(theta)
+
+    def step():
+        plan.forward(D, theta, exc)
+        plan.backward(D, theta, one, dD, dth, plan.generation)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    with _lib.KernelTimer() as kt:
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+    ms = (t1 - t0) / args.steps * 1e3
+    value = N / (ms / 1e3)
+    E = float(exc)
+
+    # dominant kernel and its roofline
+    name, (tot_ms, cnt) = max(kt.result.items(), key=lambda kv: kv[1][0])
+    nsd = 1 if mol.restricted else 2
+    flops = kernel_flops(name, N, n, spec, nsd)
+    unit, peak = PEAK[args.precision]
+    roof = None
+    if flops is not None:
+        avg_s = tot_ms / cnt / 1e3
+        achieved = flops / avg_s / 1e12
+        roof = {"bound": "mfma", "kernel": name, "achieved": round(achieved, 3), "peak": peak,
+                "unit": unit, "frac": round(achieved / peak, 4), "traffic": None,
+                "avg_launch_us": round(avg_s * 1e6, 2), "launches_per_step": cnt // args.steps,
+                "flops_per_launch": flops}
+    w_pt = 16 * n * n * nsd + 3 * spec.flops_per_point()
+    step_tflops = w_pt * N / (ms / 1e3) / 1e12
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        v, cores, sample = cpu_baseline(mol, theta, spec, min(args.cpu_sample, N))
+        cpu = {"value": round(v, 1), "unit": "grid_pts/s", "cores": cores, "kind": "port",
+               "sample": f"FP64 torch-CPU oracle, {sample}"}
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "grid_pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE[args.precision],
+        "data": "synthetic",
+        "config": {
+            "workload": f"benzene 6-31G* (cartesian d) nao={n}, grid level {args.grid_level} "
+                        f"({N} pts), neural B3LYP MLP 3x256 ELU: E_xc + dE/dtheta + dE/dD",
+            "npts": N, "nao": n, "precision": args.precision, "seed": args.seed,
+            "exc": E,
+        },
+        "roofline": roof,
+        "step_mfma": {"algorithmic_tflops": round(step_tflops, 3), "peak": peak,
+                      "frac": round(step_tflops / peak, 4), "flop_per_point": w_pt},
+        "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in
+                                sorted(kt.result.items(), key=lambda kv: -kv[1][0])},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

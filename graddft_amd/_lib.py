@@ -41,6 +41,8 @@ SIGNATURES = {
     "gdx_ao_eval": (ctypes.c_int, [_P, _I, _P, _P, _P, _P, _I, _I, _P, _I, _I, _P]),
     "gdx_xc_workspace_size": (ctypes.c_int, [ctypes.POINTER(XCDesc), ctypes.POINTER(_Z)]),
     "gdx_xc_forward": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P, _P, _P]),
+    "gdx_profile_begin": (ctypes.c_int, []),
+    "gdx_profile_end": (ctypes.c_int, [_I, ctypes.c_char_p, _P, _P, _P]),
     "gdx_xc_densities": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P]),
     "gdx_xc_backward": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P, _P, _P, _P, _P]),
 }
@@ -74,3 +76,31 @@ def check(rc: int, what: str) -> None:
     if rc != 0:
         msg = load().gdx_last_error()
         raise GdxError(f"{what} failed with status {rc}: {msg.decode() if msg else ''}")
+
+
+class KernelTimer:
+    """Context manager: per-kernel HIP-event timing of everything gdx enqueues inside it.
+    ``result`` maps kernel name -> (total_ms, launches) after exit."""
+
+    def __init__(self, max_entries: int = 64):
+        self.max_entries = max_entries
+        self.result = {}
+
+    def __enter__(self):
+        check(load().gdx_profile_begin(), "gdx_profile_begin")
+        return self
+
+    def __exit__(self, *exc):
+        import numpy as np
+        n = self.max_entries
+        names = ctypes.create_string_buffer(64 * n)
+        tot = np.zeros(n, dtype=np.float64)
+        cnt = np.zeros(n, dtype=np.int32)
+        nout = np.zeros(1, dtype=np.int32)
+        check(load().gdx_profile_end(n, names, tot.ctypes.data, cnt.ctypes.data, nout.ctypes.data),
+              "gdx_profile_end")
+        raw = names.raw
+        for i in range(int(nout[0])):
+            nm = raw[64 * i: 64 * (i + 1)].split(b"\0", 1)[0].decode()
+            self.result[nm] = (float(tot[i]), int(cnt[i]))
+        return False

@@ -126,3 +126,94 @@ int gdx_ao_eval(void* stream, int32_t nshell, const int32_t* shell_i32, const do
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ kernel timing ----
+#include "gdx_profile.h"
+#include <map>
+#include <vector>
+
+namespace gdx {
+namespace {
+struct ProfEntry {
+  std::string name;
+  hipEvent_t a, b;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<ProfEntry> g_prof;
+std::vector<hipEvent_t> g_event_pool;
+
+hipEvent_t take_event() {
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+}  // namespace
+
+bool prof_active() { return g_prof_on; }
+
+int prof_start(const char* name, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (!g_prof_on) return -1;
+  ProfEntry e{name, take_event(), take_event()};
+  if (!e.a || !e.b) return -1;
+  (void)hipEventRecord(e.a, st);
+  g_prof.push_back(e);
+  return (int)g_prof.size() - 1;
+}
+
+void prof_stop(int slot, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (slot < 0 || slot >= (int)g_prof.size()) return;
+  (void)hipEventRecord(g_prof[slot].b, st);
+}
+
+}  // namespace gdx
+
+extern "C" {
+
+int gdx_profile_begin(void) {
+  std::lock_guard<std::mutex> lk(gdx::g_prof_mu);
+  gdx::g_prof.clear();
+  gdx::g_prof_on = true;
+  return GDX_OK;
+}
+
+// Stops recording, waits for the recorded events and aggregates durations by kernel
+// name into names (max_entries x 64 chars), total_ms, counts; *n_out = kernels found.
+int gdx_profile_end(int32_t max_entries, char* names, double* total_ms, int32_t* counts,
+                    int32_t* n_out) {
+  std::lock_guard<std::mutex> lk(gdx::g_prof_mu);
+  gdx::g_prof_on = false;
+  std::map<std::string, std::pair<double, int>> agg;
+  std::vector<std::string> order;
+  for (auto& e : gdx::g_prof) {
+    GDX_HIP_CHECK(hipEventSynchronize(e.b));
+    float ms = 0.f;
+    GDX_HIP_CHECK(hipEventElapsedTime(&ms, e.a, e.b));
+    if (!agg.count(e.name)) order.push_back(e.name);
+    agg[e.name].first += ms;
+    agg[e.name].second += 1;
+    gdx::g_event_pool.push_back(e.a);
+    gdx::g_event_pool.push_back(e.b);
+  }
+  gdx::g_prof.clear();
+  int n = 0;
+  for (auto& k : order) {
+    if (n >= max_entries) break;
+    std::strncpy(names + 64 * n, k.c_str(), 63);
+    names[64 * n + 63] = 0;
+    total_ms[n] = agg[k].first;
+    counts[n] = agg[k].second;
+    ++n;
+  }
+  *n_out = n;
+  return GDX_OK;
+}
+
+}  // extern "C"

@@ -27,6 +27,7 @@
 //   reduce_*          deterministic FP64 reductions of block / chunk partials
 #include "gdx_common.h"
 #include "gdx_functionals.h"
+#include "gdx_profile.h"
 
 #include <algorithm>
 #include <vector>
@@ -686,8 +687,10 @@ void launch_gemm(hipStream_t st, int M, int N, int K, const double* A, int lda, 
                  int ldb, double* C, int ldc, const double* bias, const double* Y, int ldy,
                  int nsplit, int kchunk, double* colsum) {
   dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, nsplit);
-  hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, TA, TB, EPI>), grid, dim3(NT), 0, st, M, N, K, A, lda,
-                     B, ldb, C, ldc, bias, Y, ldy, kchunk, colsum);
+  const char* name = TA ? (M == N ? "gemm_f64_wgrad" : "gemm_f64_wgrad_edge")
+                        : (EPI == EPI_BIAS_ELU ? "gemm_f64_fwd" : "gemm_f64_bwd_data");
+  GDX_LAUNCH(name, (gemm_f64_kernel<BM, BN, TA, TB, EPI>), grid, dim3(NT), 0, st, M, N, K, A, lda,
+             B, ldb, C, ldc, bias, Y, ldy, kchunk, colsum);
 }
 
 }  // namespace
@@ -719,14 +722,14 @@ int gdx_xc_forward(void* stream, const gdx_xc_desc* desc, const double* ao, cons
   auto P = [&](size_t off) { return (double*)(ws + off); };
   const int Np = d.npts_pad, nP = d.nao_pad;
 
-  hipLaunchKernelGGL(pad_dm_kernel, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, rdm1, d.nao, nP, P(L.Dp));
+  GDX_LAUNCH("pad_dm", pad_dm_kernel, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, rdm1, d.nao, nP, P(L.Dp));
   if (L.wcb == 1)
-    hipLaunchKernelGGL(density_f64_kernel<1>, dim3(Np / DPT, L.ncb, L.nsd), dim3(NT), 0, st, ao, Np,
+    GDX_LAUNCH("density_f64", density_f64_kernel<1>, dim3(Np / DPT, L.ncb, L.nsd), dim3(NT), 0, st, ao, Np,
                        nP, P(L.Dp), P(L.dens_part));
   else
-    hipLaunchKernelGGL(density_f64_kernel<2>, dim3(Np / DPT, L.ncb, L.nsd), dim3(NT), 0, st, ao, Np,
+    GDX_LAUNCH("density_f64", density_f64_kernel<2>, dim3(Np / DPT, L.ncb, L.nsd), dim3(NT), 0, st, ao, Np,
                        nP, P(L.Dp), P(L.dens_part));
-  hipLaunchKernelGGL(point_fwd_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.dens_part), L.ncb,
+  GDX_LAUNCH("point_fwd", point_fwd_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.dens_part), L.ncb,
                      L.nsd, d.npts, Np, weights, neural ? 1 : 0, P(L.dens_fin), P(L.E), P(L.X),
                      P(L.eps_part));
   int nred = L.npt_blocks;
@@ -735,7 +738,7 @@ int gdx_xc_forward(void* stream, const gdx_xc_desc* desc, const double* ao, cons
     const double* W = params;
     const double* b = W + (size_t)d.mlp_nc * h;
     double* H = P(L.H);
-    hipLaunchKernelGGL(layer0_kernel, dim3(nblocks((long)Np * h)), dim3(NT), 0, st, P(L.X), d.mlp_nc, h,
+    GDX_LAUNCH("layer0", layer0_kernel, dim3(nblocks((long)Np * h)), dim3(NT), 0, st, P(L.X), d.mlp_nc, h,
                        (long)Np * h, W, b, H);
     const double* pw = b + h;
     for (int l = 1; l < d.mlp_nhidden; ++l) {
@@ -749,12 +752,12 @@ int gdx_xc_forward(void* stream, const gdx_xc_desc* desc, const double* ao, cons
     const double* Wlast = pw;
     const double* blast = Wlast + (size_t)h * d.mlp_nd;
     const int hb = 1024;
-    hipLaunchKernelGGL(head_fwd_kernel, dim3(hb), dim3(NT), 0, st,
+    GDX_LAUNCH("head_fwd", head_fwd_kernel, dim3(hb), dim3(NT), 0, st,
                        H + (size_t)(d.mlp_nhidden - 1) * Np * h, h, d.mlp_nd, Wlast, blast, P(L.E),
                        weights, d.npts, P(L.C), P(L.eps_part));
     nred = hb;
   }
-  hipLaunchKernelGGL(reduce_scalar_kernel, dim3(1), dim3(NT), 0, st, P(L.eps_part), nred, exc);
+  GDX_LAUNCH("reduce_scalar", reduce_scalar_kernel, dim3(1), dim3(NT), 0, st, P(L.eps_part), nred, exc);
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
 }
@@ -770,16 +773,16 @@ int gdx_xc_densities(void* stream, const gdx_xc_desc* desc, const double* ao, co
   char* ws = (char*)workspace;
   auto P = [&](size_t off) { return (double*)(ws + off); };
   const int Np = d.npts_pad, nP = d.nao_pad;
-  hipLaunchKernelGGL(pad_dm_kernel, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, rdm1, d.nao, nP, P(L.Dp));
+  GDX_LAUNCH("pad_dm", pad_dm_kernel, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, rdm1, d.nao, nP, P(L.Dp));
   if (L.wcb == 1)
-    hipLaunchKernelGGL(density_f64_kernel<1>, dim3(Np / DPT, L.ncb, L.nsd), dim3(NT), 0, st, ao, Np,
+    GDX_LAUNCH("density_f64", density_f64_kernel<1>, dim3(Np / DPT, L.ncb, L.nsd), dim3(NT), 0, st, ao, Np,
                        nP, P(L.Dp), P(L.dens_part));
   else
-    hipLaunchKernelGGL(density_f64_kernel<2>, dim3(Np / DPT, L.ncb, L.nsd), dim3(NT), 0, st, ao, Np,
+    GDX_LAUNCH("density_f64", density_f64_kernel<2>, dim3(Np / DPT, L.ncb, L.nsd), dim3(NT), 0, st, ao, Np,
                        nP, P(L.Dp), P(L.dens_part));
   // weights == nullptr: the per-point kernel forms densities and energy densities only
   const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
-  hipLaunchKernelGGL(point_fwd_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.dens_part), L.ncb,
+  GDX_LAUNCH("point_fwd", point_fwd_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.dens_part), L.ncb,
                      L.nsd, d.npts, Np, nullptr, neural ? 1 : 0, dens, P(L.E), P(L.X), P(L.eps_part));
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
@@ -819,16 +822,16 @@ int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const double* ao,
     double* G0 = P(L.G);
     double* G1 = P(L.G) + (size_t)Np * h;
     const double* Hlast = H + (size_t)(d.mlp_nhidden - 1) * Np * h;
-    hipLaunchKernelGGL(head_bwd_kernel, dim3(1024), dim3(NT), 0, st, Hlast, h, nd, Wlast, P(L.C), P(L.E),
+    GDX_LAUNCH("head_bwd", head_bwd_kernel, dim3(1024), dim3(NT), 0, st, Hlast, h, nd, Wlast, P(L.C), P(L.E),
                        weights, d.npts, Np, g_exc, 1, P(L.GE), P(L.DZ), G0);
     const int nl = d.mlp_nhidden;
     // last layer weight grads: dW = H_last^T DZ (h x nd), db = colsum(DZ)
     if (d_params) {
       launch_gemm<64, 64, true, false, EPI_STORE>(st, h, nd, Np, Hlast, h, P(L.DZ), nd, P(L.wpart), nd,
                                                   nullptr, nullptr, 0, L.nsplit, L.kchunk, P(L.cpart));
-      hipLaunchKernelGGL(reduce_partials_kernel, dim3(nblocks((long)h * nd)), dim3(NT), 0, st, P(L.wpart),
+      GDX_LAUNCH("reduce_partials", reduce_partials_kernel, dim3(nblocks((long)h * nd)), dim3(NT), 0, st, P(L.wpart),
                          L.nsplit, (long)h * nd, (long)h * nd, d_params + woff[nl]);
-      hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(NT), 0, st, P(L.cpart), L.nsplit,
+      GDX_LAUNCH("reduce_partials", reduce_partials_kernel, dim3(1), dim3(NT), 0, st, P(L.cpart), L.nsplit,
                          (long)nd, (long)nd, d_params + boff[nl]);
     }
     // hidden h x h layers, from the top
@@ -840,9 +843,9 @@ int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const double* ao,
       if (d_params) {
         launch_gemm<64, 64, true, false, EPI_STORE>(st, h, h, Np, Hin, h, Gcur, h, P(L.wpart), h, nullptr,
                                                     nullptr, 0, L.nsplit, L.kchunk, P(L.cpart));
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3(nblocks((long)h * h)), dim3(NT), 0, st, P(L.wpart),
+        GDX_LAUNCH("reduce_partials", reduce_partials_kernel, dim3(nblocks((long)h * h)), dim3(NT), 0, st, P(L.wpart),
                            L.nsplit, (long)h * h, (long)h * h, d_params + woff[l]);
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3(nblocks(h)), dim3(NT), 0, st, P(L.cpart), L.nsplit,
+        GDX_LAUNCH("reduce_partials", reduce_partials_kernel, dim3(nblocks(h)), dim3(NT), 0, st, P(L.cpart), L.nsplit,
                            (long)h, (long)h, d_params + boff[l]);
       }
       // delta for layer l's input: (Gcur W_l^T) * elu'(Hin)
@@ -854,24 +857,24 @@ int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const double* ao,
     if (d_params) {
       launch_gemm<64, 64, true, false, EPI_STORE>(st, d.mlp_nc, h, Np, P(L.X), d.mlp_nc, Gcur, h, P(L.wpart),
                                                   h, nullptr, nullptr, 0, L.nsplit, L.kchunk, P(L.cpart));
-      hipLaunchKernelGGL(reduce_partials_kernel, dim3(nblocks((long)d.mlp_nc * h)), dim3(NT), 0, st,
+      GDX_LAUNCH("reduce_partials", reduce_partials_kernel, dim3(nblocks((long)d.mlp_nc * h)), dim3(NT), 0, st,
                          P(L.wpart), L.nsplit, (long)d.mlp_nc * h, (long)d.mlp_nc * h, d_params + woff[0]);
-      hipLaunchKernelGGL(reduce_partials_kernel, dim3(nblocks(h)), dim3(NT), 0, st, P(L.cpart), L.nsplit,
+      GDX_LAUNCH("reduce_partials", reduce_partials_kernel, dim3(nblocks(h)), dim3(NT), 0, st, P(L.cpart), L.nsplit,
                          (long)h, (long)h, d_params + boff[0]);
     }
-    hipLaunchKernelGGL(layer0_bwd_kernel, dim3(1024), dim3(NT), 0, st, Gcur, h, d.mlp_nc, params + woff[0], Np,
+    GDX_LAUNCH("layer0_bwd", layer0_bwd_kernel, dim3(1024), dim3(NT), 0, st, Gcur, h, d.mlp_nc, params + woff[0], Np,
                        P(L.DX));
   } else {
-    hipLaunchKernelGGL(head_bwd_kernel, dim3(1024), dim3(NT), 0, st, nullptr, 0, nd, nullptr, nullptr, P(L.E),
+    GDX_LAUNCH("head_bwd", head_bwd_kernel, dim3(1024), dim3(NT), 0, st, nullptr, 0, nd, nullptr, nullptr, P(L.E),
                        weights, d.npts, Np, g_exc, 0, P(L.GE), nullptr, nullptr);
   }
   if (d_rdm1) {
-    hipLaunchKernelGGL(point_bwd_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.dens_fin), P(L.GE),
+    GDX_LAUNCH("point_bwd", point_bwd_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.dens_fin), P(L.GE),
                        neural ? P(L.DX) : nullptr, P(L.X), d.npts, Np, P(L.vpt));
     const int ntile = (nP + VBT - 1) / VBT;
-    hipLaunchKernelGGL(vxc_f64_kernel, dim3(ntile * ntile, L.nchunk, 2), dim3(NT), 0, st, ao, Np, nP, P(L.vpt),
+    GDX_LAUNCH("vxc_f64", vxc_f64_kernel, dim3(ntile * ntile, L.nchunk, 2), dim3(NT), 0, st, ao, Np, nP, P(L.vpt),
                        L.chunk, P(L.vpart));
-    hipLaunchKernelGGL(reduce_vxc_kernel, dim3(nblocks(2L * d.nao * d.nao)), dim3(NT), 0, st, P(L.vpart),
+    GDX_LAUNCH("reduce_vxc", reduce_vxc_kernel, dim3(nblocks(2L * d.nao * d.nao)), dim3(NT), 0, st, P(L.vpart),
                        L.nchunk, nP, d.nao, d_rdm1);
   }
   GDX_HIP_CHECK(hipGetLastError());

@@ -95,6 +95,13 @@ int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const double* ao,
                     const double* weights, const double* rdm1, const double* params,
                     const double* g_exc, double* d_rdm1, double* d_params, void* workspace);
 
+/* Kernel timing (no counterpart in the reference): while on, every kernel this library
+ * enqueues is bracketed by HIP events on its own stream.  gdx_profile_end waits for them and
+ * returns per-kernel-name total milliseconds and launch counts (names: 64 chars each). */
+int gdx_profile_begin(void);
+int gdx_profile_end(int32_t max_entries, char* names, double* total_ms, int32_t* counts,
+                    int32_t* n_out);
+
 #ifdef __cplusplus
 }
 #endif
