@@ -1,0 +1,142 @@
+// MFMA kernels for the MLP's narrow edges (input width nc = 6, output width nd = 4, both
+// zero-padded to one 16-wide MFMA tile) and a parallel column reduction of split partials.
+//
+//   skinny_out16   C16 (M x 16) = A (M x K) . Bt^T, Bt (16 x K): per-row products against
+//                  at most 16 weight rows (the last layer's z = H W_last, and dE/dx = G W0^T).
+//                  Block = 4 waves x 16 rows; A staged through LDS in 64-wide K chunks.
+//                  Optional head epilogue: c = 2 sigmoid((z + b)/2), eps = sum_o c_o e_o,
+//                  clipped, weighted FP64 block sum (grad_dft/functional.py:219-253, 407-419).
+//   skinny_wgrad   P[blk] (16 x N) = sum over the block's rows r of A16[r][m] B[r][n]: weight
+//                  gradients whose fan-in or fan-out is <= 16, split over row chunks; the 4
+//                  waves of a block take interleaved 4-row MFMA steps and are summed in LDS.
+//   reduce_cols    out[i] = sum_s part[s * stride + i], 64 columns x 4 split groups per block.
+#pragma once
+#include "gdx_common.h"
+#include "gdx_gemm.h"
+
+namespace gdx {
+
+constexpr int SK_KC = 64;  // K chunk staged per LDS pass
+
+template <typename T, bool HEAD>
+__global__ __launch_bounds__(256) void skinny_out16_kernel(
+    int M, int K, const T* __restrict__ A, int lda, const T* __restrict__ Bt,
+    T* __restrict__ C16, int ldc, int ncols,
+    // head epilogue
+    const T* __restrict__ bias, int nd, const double* __restrict__ E,
+    const double* __restrict__ weights, int npts, double* __restrict__ eps_part) {
+  typedef typename MfmaT<T>::acc_t acc_t;
+  __shared__ T As[64][SK_KC + 2];
+  __shared__ double zs[64][17];
+  __shared__ double red[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r0 = blockIdx.x * 64;
+  acc_t acc = (acc_t){0, 0, 0, 0};
+  for (int k0 = 0; k0 < K; k0 += SK_KC) {
+    // 64 rows x 64 k, 16 elements per thread
+    for (int e = tid; e < 64 * SK_KC; e += 256) {
+      const int r = e / SK_KC, k = e % SK_KC;
+      As[r][k] = (r0 + r < M && k0 + k < K) ? A[(size_t)(r0 + r) * lda + k0 + k] : (T)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < SK_KC / 4; ++kk) {
+      const int k = kk * 4 + (lane >> 4);
+      const T af = As[wave * 16 + (lane & 15)][k];
+      const T bf = (k0 + k < K) ? Bt[(size_t)(lane & 15) * K + k0 + k] : (T)0;
+      acc = MfmaT<T>::op(af, bf, acc);
+    }
+    __syncthreads();
+  }
+  if (!HEAD) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = r0 + wave * 16 + MfmaT<T>::row(lane, q);
+      const int c = lane & 15;
+      if (r < M && c < ncols) C16[(size_t)r * ldc + c] = acc[q];
+    }
+    return;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) zs[wave * 16 + MfmaT<T>::row(lane, q)][lane & 15] = (double)acc[q];
+  __syncthreads();
+  double contrib = 0.0;
+  if (tid < 64) {
+    const int r = r0 + tid;
+    if (r < npts) {
+      double eps = 0.0;
+      for (int o = 0; o < nd; ++o) {
+        const double z = zs[tid][o] + (double)bias[o];
+        const double c = 2.0 / (1.0 + exp(-z / 2.0));
+        C16[(size_t)r * ldc + o] = (T)c;
+        eps += c * E[(size_t)r * nd + o];
+      }
+      eps = fabs(eps) > 1e-30 ? eps : 0.0;
+      const double w = weights[r];
+      contrib = (fabs(w) > 1e-30 ? w : 0.0) * eps;
+    }
+  }
+  contrib = warp_sum(contrib);
+  if (tid == 0) eps_part[blockIdx.x] = contrib;
+  (void)red;
+}
+
+// P[blockIdx.x] (16 x N) += A16[r][m] * B[r][n] over rows [blockIdx.x*rows, +rows)
+template <typename T>
+__global__ __launch_bounds__(256) void skinny_wgrad_kernel(
+    int M, int rows, const T* __restrict__ A16, const T* __restrict__ B, int N,
+    double* __restrict__ part) {
+  typedef typename MfmaT<T>::acc_t acc_t;
+  constexpr int NJ = 16;  // N / 16 column blocks, N <= 256
+  __shared__ double red[16][256 + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int rbeg = blockIdx.x * rows;
+  const int rend = min(M, rbeg + rows);
+  const int nj = N / 16;
+  acc_t acc[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) acc[j] = (acc_t){0, 0, 0, 0};
+  for (int r = rbeg + wave * 4; r < rend; r += 16) {
+    const int rr = r + (lane >> 4);
+    const bool ok = rr < rend;
+    const T af = ok ? A16[(size_t)rr * 16 + (lane & 15)] : (T)0;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j)
+      if (j < nj) {
+        const T bf = ok ? B[(size_t)rr * N + j * 16 + (lane & 15)] : (T)0;
+        acc[j] = MfmaT<T>::op(af, bf, acc[j]);
+      }
+  }
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w) {
+#pragma unroll
+      for (int j = 0; j < NJ; ++j)
+        if (j < nj)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int m = MfmaT<T>::row(lane, q), n = j * 16 + (lane & 15);
+            const double v = (double)acc[j][q];
+            red[m][n] = (w == 0) ? v : red[m][n] + v;
+          }
+    }
+    __syncthreads();
+  }
+  double* out = part + (size_t)blockIdx.x * 16 * N;
+  for (int e = tid; e < 16 * N; e += 256) out[e] = red[e / N][e % N];
+}
+
+__global__ __launch_bounds__(256) void reduce_cols_kernel(const double* __restrict__ part, int nsplit,
+                                                          long stride, long len,
+                                                          double* __restrict__ out) {
+  __shared__ double red[4][64];
+  const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const long i = (long)blockIdx.x * 64 + c;
+  double acc = 0.0;
+  if (i < len)
+    for (int s = g; s < nsplit; s += 4) acc += part[(long)s * stride + i];
+  red[g][c] = acc;
+  __syncthreads();
+  if (g == 0 && i < len) out[i] = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+}
+
+}  // namespace gdx

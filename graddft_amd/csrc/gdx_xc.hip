@@ -28,6 +28,8 @@
 #include "gdx_common.h"
 #include "gdx_functionals.h"
 #include "gdx_profile.h"
+#include "gdx_gemm.h"
+#include "gdx_skinny.h"
 
 #include <algorithm>
 #include <vector>
@@ -260,9 +262,13 @@ __global__ __launch_bounds__(NT) void point_fwd_kernel(
 #pragma unroll
     for (int f = 0; f < 4; ++f) E[(size_t)r * 4 + f] = e[f];
     if (neural) {
+      // X16 row: six squashed features, a 1 (bias column of the first layer's gradient), zeros
       const double feat[6] = {d[0][0], d[1][0], gu, gd, d[0][4], d[1][4]};
 #pragma unroll
-      for (int j = 0; j < 6; ++j) X[(size_t)r * 6 + j] = log(fabs(feat[j]) + SQUASH);
+      for (int j = 0; j < 6; ++j) X[(size_t)r * 16 + j] = log(fabs(feat[j]) + SQUASH);
+      X[(size_t)r * 16 + 6] = 1.0;
+#pragma unroll
+      for (int j = 7; j < 16; ++j) X[(size_t)r * 16 + j] = 0.0;
     } else if (weights != nullptr && r < npts) {
       const double c[4] = {1.0 - 0.2, 0.72, 1.0 - 0.81, 0.81};
       double eps = c[0] * e[0] + c[1] * e[1] + c[2] * e[2] + c[3] * e[3];
@@ -284,12 +290,11 @@ __global__ __launch_bounds__(NT) void layer0_kernel(const double* __restrict__ X
   const long r = i / h;
   const int f = (int)(i % h);
   double z = b[f];
-  for (int j = 0; j < nc; ++j) z += X[r * nc + j] * W[j * h + f];
+  for (int j = 0; j < nc; ++j) z += X[r * 16 + j] * W[j * h + f];
   H[i] = elu(z);
 }
 
 // ------------------------------------------------------------------ GEMM (f64) ----
-enum { EPI_STORE = 0, EPI_BIAS_ELU = 1, EPI_ELU_BWD = 2 };
 constexpr int GBK = 16;
 
 // C (+ z * M * ldc for split-K) = epi(op(A) op(B)); op(A)[m][k] = TA ? A[k*lda+m] : A[m*lda+k],
@@ -377,54 +382,34 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(
 }
 
 // ----------------------------------------------------------------- MLP head ----
-// One wave per point (grid-stride over points).  z_o = sum_f H[r][f] W[f][o] + b[o],
-// c = 2 sigmoid(z/2) (functional.py:407-419 with sigmoid_scale_factor 2).
 constexpr int MAXND = 8;
 
-__global__ __launch_bounds__(NT) void head_fwd_kernel(
-    const double* __restrict__ H, int h, int nd, const double* __restrict__ W,
-    const double* __restrict__ b, const double* __restrict__ E, const double* __restrict__ weights,
-    int npts, double* __restrict__ Cout, double* __restrict__ eps_part) {
-  __shared__ double red[NT / 64];
-  const int lane = threadIdx.x & 63;
-  const int wave_global = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
-  const int nwaves = gridDim.x * (NT / 64);
-  double contrib = 0.0;
-  for (int r = wave_global; r < npts; r += nwaves) {
-    double z[MAXND];
-    for (int o = 0; o < nd; ++o) z[o] = 0.0;
-    for (int f = lane; f < h; f += 64) {
-      const double hv = H[(size_t)r * h + f];
-      for (int o = 0; o < nd; ++o) z[o] += hv * W[f * nd + o];
-    }
-    double eps = 0.0;
-    for (int o = 0; o < nd; ++o) {
-      const double zz = warp_sum(z[o]) + b[o];
-      const double c = 2.0 / (1.0 + exp(-zz / 2.0));
-      if (lane == 0) Cout[(size_t)r * nd + o] = c;
-      eps += c * E[(size_t)r * nd + o];
-    }
-    eps = clip30(clip30(eps));
-    if (lane == 0) contrib += clip30(weights[r]) * eps;
-  }
-  const double t = block_sum(contrib, red);
-  if (threadIdx.x == 0) eps_part[blockIdx.x] = t;
+// Padded weight tiles for the skinny kernels: WT16 (16 x h) rows o < nd = W_last[:, o];
+// W016 (16 x h) rows j < nc = W0[j, :]; every other row zero.
+__global__ void prep_edge_weights_kernel(const double* __restrict__ W0, int nc,
+                                         const double* __restrict__ Wl, int nd, int h,
+                                         double* __restrict__ W016, double* __restrict__ WT16) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 16 * h) return;
+  const int row = i / h, f = i % h;
+  W016[i] = row < nc ? W0[row * h + f] : 0.0;
+  WT16[i] = row < nd ? Wl[f * nd + row] : 0.0;
 }
 
-// Adjoint seeds.  g = g_exc * clip(w) * [|eps| > 1e-30]; GE = dE/de_f = g c_f;
-// neural: DZ = dE/dz = g e_f (c_f/2)(1 - c_f/2), G = (DZ W^T) * elu'(H).
-__global__ __launch_bounds__(NT) void head_bwd_kernel(
-    const double* __restrict__ H, int h, int nd, const double* __restrict__ W,
-    const double* __restrict__ Cc, const double* __restrict__ E,
-    const double* __restrict__ weights, int npts, int npts_pad, const double* __restrict__ g_exc,
-    int neural, double* __restrict__ GE, double* __restrict__ DZ, double* __restrict__ G) {
-  const int lane = threadIdx.x & 63;
-  const int wave_global = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
-  const int nwaves = gridDim.x * (NT / 64);
-  const double gE = *g_exc;
+// Adjoint seeds, one thread per point.  g = g_exc clip(w) [|eps| > 1e-30];
+// GE = dE/de_f = g c_f; neural: DZ16 = dE/dz = g e_f (c_f/2)(1 - c_f/2) zero-padded to 16,
+// and block partials of the last layer's bias gradient sum_r DZ[r][o] -> dbpart[block][o].
+__global__ __launch_bounds__(NT) void head_seed_kernel(
+    const double* __restrict__ Cc, const double* __restrict__ E, const double* __restrict__ weights,
+    int nd, int npts, int npts_pad, const double* __restrict__ g_exc, int neural,
+    double* __restrict__ GE, double* __restrict__ DZ16, double* __restrict__ dbpart) {
+  __shared__ double red[NT / 64];
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const double bc[4] = {1.0 - 0.2, 0.72, 1.0 - 0.81, 0.81};
-  for (int r = wave_global; r < npts_pad; r += nwaves) {
-    double c[MAXND], e[MAXND], dz[MAXND];
+  double dz[MAXND];
+  for (int o = 0; o < MAXND; ++o) dz[o] = 0.0;
+  if (r < npts_pad) {
+    double c[MAXND], e[MAXND];
     double eps = 0.0;
     // padded rows (r >= npts) have no coefficients stored: they take c = 0, seed 0
     for (int o = 0; o < nd; ++o) {
@@ -432,42 +417,42 @@ __global__ __launch_bounds__(NT) void head_bwd_kernel(
       e[o] = E[(size_t)r * nd + o];
       eps += c[o] * e[o];
     }
-    const double g = (r < npts && fabs(eps) > 1e-30) ? gE * clip30(weights[r]) : 0.0;
+    const double g = (r < npts && fabs(eps) > 1e-30) ? (*g_exc) * clip30(weights[r]) : 0.0;
     for (int o = 0; o < nd; ++o) {
+      GE[(size_t)r * nd + o] = g * c[o];
       dz[o] = g * e[o] * (c[o] * 0.5) * (1.0 - c[o] * 0.5);
-      if (lane == 0) {
-        GE[(size_t)r * nd + o] = g * c[o];
-        if (neural) DZ[(size_t)r * nd + o] = dz[o];
-      }
     }
-    if (neural) {
-      for (int f = lane; f < h; f += 64) {
-        double v = 0.0;
-        for (int o = 0; o < nd; ++o) v += dz[o] * W[f * nd + o];
-        G[(size_t)r * h + f] = v * elu_grad_from_out(H[(size_t)r * h + f]);
-      }
-    }
+    if (neural)
+      for (int o = 0; o < 16; ++o) DZ16[(size_t)r * 16 + o] = o < nd ? dz[o] : 0.0;
   }
+  if (neural)
+    for (int o = 0; o < nd; ++o) {
+      const double t = block_sum(dz[o], red);
+      if (threadIdx.x == 0) dbpart[(size_t)blockIdx.x * nd + o] = t;
+    }
 }
 
-// dE/dx = G1 (npts_pad, h) W0^T -> DX (npts_pad, nc); one wave per point.
-__global__ __launch_bounds__(NT) void layer0_bwd_kernel(const double* __restrict__ G, int h, int nc,
-                                                        const double* __restrict__ W, int npts_pad,
-                                                        double* __restrict__ DX) {
-  const int lane = threadIdx.x & 63;
-  const int wave_global = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
-  const int nwaves = gridDim.x * (NT / 64);
-  for (int r = wave_global; r < npts_pad; r += nwaves) {
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int f = lane; f < h; f += 64) {
-      const double gv = G[(size_t)r * h + f];
-      for (int j = 0; j < nc; ++j) acc[j] += gv * W[j * h + f];
-    }
-    for (int j = 0; j < nc; ++j) {
-      const double v = warp_sum(acc[j]);
-      if (lane == 0) DX[(size_t)r * nc + j] = v;
-    }
-  }
+// G[r][f] = (sum_o DZ16[r][o] W_last[f][o]) * elu'(H[r][f]), one thread per element
+__global__ __launch_bounds__(NT) void head_delta_kernel(const double* __restrict__ DZ16,
+                                                        const double* __restrict__ Wl, int nd, int h,
+                                                        const double* __restrict__ H, long total,
+                                                        double* __restrict__ G) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long r = i / h;
+  const int f = (int)(i % h);
+  double v = 0.0;
+  for (int o = 0; o < nd; ++o) v += DZ16[r * 16 + o] * Wl[f * nd + o];
+  G[i] = v * elu_grad_from_out(H[i]);
+}
+
+// d_params last layer: W (h, nd) from the reduced (16 x h) transpose, b from block partials
+__global__ void unpack_last_kernel(const double* __restrict__ dWt16, int h, int nd,
+                                   double* __restrict__ dW) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= h * nd) return;
+  const int f = i / nd, o = i % nd;
+  dW[i] = dWt16[(size_t)o * h + f];
 }
 
 // Per-point potentials for the V_xc build from dE/de_f (GE) and dE/dx (DX, neural).
@@ -608,11 +593,14 @@ __global__ __launch_bounds__(NT) void vxc_f64_kernel(
 // ------------------------------------------------------------ workspace layout ----
 struct Layout {
   size_t Dp, dens_part, dens_fin, E, X, C, H, eps_part, GE, DZ, G, DX, vpt, vpart, wpart, cpart;
+  size_t W16, dbpart, dWt16;
+  int nwblk, wrows;
   size_t total;
   int ncb, wcb, nsd, nchunk, chunk, nsplit, kchunk, nparam, npt_blocks;
 };
 
 constexpr int VCHUNK = 1024;  // points per V_xc split
+constexpr int SKW_ROWS = 512;  // grid points per block of the skinny weight-gradient kernel
 
 Layout make_layout(const gdx_xc_desc& d) {
   Layout L{};
@@ -623,8 +611,10 @@ Layout make_layout(const gdx_xc_desc& d) {
   L.nsd = d.spin_restricted ? 1 : 2;
   L.chunk = VCHUNK;
   L.nchunk = (int)((Np + VCHUNK - 1) / VCHUNK);
-  L.nsplit = (int)std::max<size_t>(1, std::min<size_t>(64, Np / 512));
-  L.kchunk = (int)round_up((Np + L.nsplit - 1) / L.nsplit, GBK);
+  // split-K for the weight-gradient GEMMs over grid points: whole 16-row stages per split
+  L.nsplit = 1;
+  while (L.nsplit < 128 && (Np / GT_K) % (2 * L.nsplit) == 0) L.nsplit *= 2;
+  L.kchunk = (int)(Np / L.nsplit);
   L.npt_blocks = (int)((Np + NT - 1) / NT);
   const size_t h = neural ? d.mlp_h : 0, nh = neural ? d.mlp_nhidden : 0;
   size_t np = 0;
@@ -643,19 +633,28 @@ Layout make_layout(const gdx_xc_desc& d) {
   L.dens_part = take((size_t)L.ncb * L.nsd * Np * DENS_STRIDE);
   L.dens_fin = take(Np * 2 * DENS_STRIDE);
   L.E = take(Np * 4);
-  L.X = take(neural ? Np * d.mlp_nc : 0);
+  L.X = take(neural ? Np * 16 : 0);
   L.C = take(neural ? Np * d.mlp_nd : 0);
   L.H = take(Np * h * nh);
-  L.eps_part = take(std::max<size_t>(L.npt_blocks, 4096));
+  L.eps_part = take(std::max<size_t>(Np / 64 + 1, 4096));
   L.GE = take(Np * 4);
-  L.DZ = take(neural ? Np * d.mlp_nd : 0);
+  L.DZ = take(neural ? Np * 16 : 0);
+  L.W16 = take(neural ? 2 * 16 * (size_t)d.mlp_h : 0);
+  L.dbpart = take(neural ? (size_t)L.npt_blocks * d.mlp_nd : 0);
+  L.dWt16 = take(neural ? 16 * (size_t)d.mlp_h : 0);
   L.G = take(neural ? 2 * Np * h : 0);
   L.DX = take(neural ? Np * d.mlp_nc : 0);
   L.vpt = take(2 * Np * VPT_STRIDE);
   L.vpart = take((size_t)L.nchunk * 2 * nP * nP);
-  const size_t wmax = neural ? std::max<size_t>((size_t)d.mlp_h * d.mlp_h,
-                                                (size_t)d.mlp_h * std::max(d.mlp_nc, d.mlp_nd)) : 0;
-  L.wpart = take((size_t)L.nsplit * wmax);
+  // skinny weight-gradient splits: row chunks of SKW_ROWS points
+  L.wrows = SKW_ROWS;
+  L.nwblk = (int)((Np + SKW_ROWS - 1) / SKW_ROWS);
+  size_t wmax = 0;
+  if (neural) {
+    wmax = (size_t)L.nsplit * d.mlp_h * d.mlp_h;
+    wmax = std::max(wmax, (size_t)L.nwblk * 16 * d.mlp_h);
+  }
+  L.wpart = take(wmax);
   L.cpart = take(neural ? (size_t)L.nsplit * d.mlp_h : 0);
   L.total = off;
   return L;
@@ -674,13 +673,15 @@ int validate(const gdx_xc_desc* d) {
   if (d->kind == GDX_FUNC_NEURAL_B3LYP) {
     GDX_REQUIRE(d->mlp_nc == 6, "coefficient inputs must be 6 (rho, |grad rho|^2, tau per spin)");
     GDX_REQUIRE(d->mlp_nd == 4, "MLP outputs must equal the 4 B3LYP energy densities");
-    GDX_REQUIRE(d->mlp_h >= 16 && d->mlp_h % 16 == 0, "MLP width must be a positive multiple of 16");
+    GDX_REQUIRE(d->mlp_h >= 16 && d->mlp_h % 16 == 0 && d->mlp_h <= 256,
+                "MLP width must be a multiple of 16 in [16, 256]");
     GDX_REQUIRE(d->mlp_nhidden >= 1 && d->mlp_nhidden <= 16, "MLP hidden layers must be 1..16");
   }
   return GDX_OK;
 }
 
 inline unsigned nblocks(long n) { return (unsigned)((n + NT - 1) / NT); }
+inline unsigned nblocks64(long n) { return (unsigned)((n + 63) / 64); }
 
 template <int BM, int BN, bool TA, bool TB, int EPI>
 void launch_gemm(hipStream_t st, int M, int N, int K, const double* A, int lda, const double* B,
@@ -691,6 +692,15 @@ void launch_gemm(hipStream_t st, int M, int N, int K, const double* A, int lda, 
                         : (EPI == EPI_BIAS_ELU ? "gemm_f64_fwd" : "gemm_f64_bwd_data");
   GDX_LAUNCH(name, (gemm_f64_kernel<BM, BN, TA, TB, EPI>), grid, dim3(NT), 0, st, M, N, K, A, lda,
              B, ldb, C, ldc, bias, Y, ldy, kchunk, colsum);
+}
+
+template <typename T, bool TA, bool TB, int EPI>
+void launch_tile(hipStream_t st, const char* name, int M, int N, int nsplit, int kchunk, const T* A,
+                 int lda, const T* B, int ldb, T* C, int ldc, const T* bias, const T* Y, int ldy,
+                 T* colsum) {
+  dim3 grid((M / GT_M) * (N / GT_N), nsplit);
+  GDX_LAUNCH(name, (gemm_tile_kernel<T, TA, TB, EPI>), grid, dim3(NT), 0, st, M, N, A, lda, B, ldb, C,
+             ldc, bias, Y, ldy, kchunk, colsum);
 }
 
 }  // namespace
@@ -735,6 +745,7 @@ int gdx_xc_forward(void* stream, const gdx_xc_desc* desc, const double* ao, cons
   int nred = L.npt_blocks;
   if (neural) {
     const int h = d.mlp_h;
+    const bool tiles = Np % GT_M == 0 && h % GT_N == 0;
     const double* W = params;
     const double* b = W + (size_t)d.mlp_nc * h;
     double* H = P(L.H);
@@ -744,18 +755,26 @@ int gdx_xc_forward(void* stream, const gdx_xc_desc* desc, const double* ao, cons
     for (int l = 1; l < d.mlp_nhidden; ++l) {
       const double* Wl = pw;
       const double* bl = Wl + (size_t)h * h;
-      launch_gemm<128, 128, false, false, EPI_BIAS_ELU>(st, Np, h, h, H + (size_t)(l - 1) * Np * h, h,
-                                                        Wl, h, H + (size_t)l * Np * h, h, bl, nullptr,
-                                                        0, 1, h, nullptr);
+      if (tiles)
+        launch_tile<double, false, false, EPI_BIAS_ELU>(st, "gemm_f64_fwd", Np, h, 1, h,
+                                                        H + (size_t)(l - 1) * Np * h, h, Wl, h,
+                                                        H + (size_t)l * Np * h, h, bl, nullptr, 0, nullptr);
+      else
+        launch_gemm<128, 128, false, false, EPI_BIAS_ELU>(st, Np, h, h, H + (size_t)(l - 1) * Np * h, h,
+                                                          Wl, h, H + (size_t)l * Np * h, h, bl, nullptr,
+                                                          0, 1, h, nullptr);
       pw = bl + h;
     }
     const double* Wlast = pw;
     const double* blast = Wlast + (size_t)h * d.mlp_nd;
-    const int hb = 1024;
-    GDX_LAUNCH("head_fwd", head_fwd_kernel, dim3(hb), dim3(NT), 0, st,
-                       H + (size_t)(d.mlp_nhidden - 1) * Np * h, h, d.mlp_nd, Wlast, blast, P(L.E),
-                       weights, d.npts, P(L.C), P(L.eps_part));
-    nred = hb;
+    double* W016 = P(L.W16);
+    double* WT16 = W016 + 16 * h;
+    GDX_LAUNCH("prep_edge_weights", prep_edge_weights_kernel, dim3(nblocks(16L * h)), dim3(NT), 0, st, W, d.mlp_nc,
+               Wlast, d.mlp_nd, h, W016, WT16);
+    GDX_LAUNCH("head_fwd", (skinny_out16_kernel<double, true>), dim3(Np / 64), dim3(NT), 0, st, Np, h,
+               H + (size_t)(d.mlp_nhidden - 1) * Np * h, h, WT16, P(L.C), d.mlp_nd, d.mlp_nd, blast, d.mlp_nd,
+               P(L.E), weights, d.npts, P(L.eps_part));
+    nred = Np / 64;
   }
   GDX_LAUNCH("reduce_scalar", reduce_scalar_kernel, dim3(1), dim3(NT), 0, st, P(L.eps_part), nred, exc);
   GDX_HIP_CHECK(hipGetLastError());
@@ -818,21 +837,26 @@ int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const double* ao,
       din = dout;
     }
     Wlast = params + woff.back();
+    const bool tiles = Np % GT_M == 0 && h % GT_N == 0;
     double* H = P(L.H);
     double* G0 = P(L.G);
     double* G1 = P(L.G) + (size_t)Np * h;
     const double* Hlast = H + (size_t)(d.mlp_nhidden - 1) * Np * h;
-    GDX_LAUNCH("head_bwd", head_bwd_kernel, dim3(1024), dim3(NT), 0, st, Hlast, h, nd, Wlast, P(L.C), P(L.E),
-                       weights, d.npts, Np, g_exc, 1, P(L.GE), P(L.DZ), G0);
+    const double* W016 = P(L.W16);
     const int nl = d.mlp_nhidden;
-    // last layer weight grads: dW = H_last^T DZ (h x nd), db = colsum(DZ)
+    GDX_LAUNCH("head_seed", head_seed_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.C), P(L.E), weights, nd,
+               d.npts, Np, g_exc, 1, P(L.GE), P(L.DZ), P(L.dbpart));
+    GDX_LAUNCH("head_delta", head_delta_kernel, dim3(nblocks((long)Np * h)), dim3(NT), 0, st, P(L.DZ), Wlast, nd,
+               h, Hlast, (long)Np * h, G0);
     if (d_params) {
-      launch_gemm<64, 64, true, false, EPI_STORE>(st, h, nd, Np, Hlast, h, P(L.DZ), nd, P(L.wpart), nd,
-                                                  nullptr, nullptr, 0, L.nsplit, L.kchunk, P(L.cpart));
-      GDX_LAUNCH("reduce_partials", reduce_partials_kernel, dim3(nblocks((long)h * nd)), dim3(NT), 0, st, P(L.wpart),
-                         L.nsplit, (long)h * nd, (long)h * nd, d_params + woff[nl]);
-      GDX_LAUNCH("reduce_partials", reduce_partials_kernel, dim3(1), dim3(NT), 0, st, P(L.cpart), L.nsplit,
-                         (long)nd, (long)nd, d_params + boff[nl]);
+      GDX_LAUNCH("skinny_wgrad", skinny_wgrad_kernel<double>, dim3(L.nwblk), dim3(NT), 0, st, Np, L.wrows, P(L.DZ),
+                 Hlast, h, P(L.wpart));
+      GDX_LAUNCH("reduce_cols", reduce_cols_kernel, dim3(nblocks64(16L * h)), dim3(NT), 0, st, P(L.wpart),
+                 L.nwblk, 16L * h, 16L * h, P(L.dWt16));
+      GDX_LAUNCH("unpack_last", unpack_last_kernel, dim3(nblocks((long)h * nd)), dim3(NT), 0, st, P(L.dWt16), h,
+                 nd, d_params + woff[nl]);
+      GDX_LAUNCH("reduce_cols", reduce_cols_kernel, dim3(1), dim3(NT), 0, st, P(L.dbpart), L.npt_blocks,
+                 (long)nd, (long)nd, d_params + boff[nl]);
     }
     // hidden h x h layers, from the top
     double* Gcur = G0;
@@ -841,32 +865,39 @@ int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const double* ao,
       const double* Hin = H + (size_t)(l - 1) * Np * h;  // input activations of layer l
       const double* Wl = params + woff[l];
       if (d_params) {
-        launch_gemm<64, 64, true, false, EPI_STORE>(st, h, h, Np, Hin, h, Gcur, h, P(L.wpart), h, nullptr,
-                                                    nullptr, 0, L.nsplit, L.kchunk, P(L.cpart));
-        GDX_LAUNCH("reduce_partials", reduce_partials_kernel, dim3(nblocks((long)h * h)), dim3(NT), 0, st, P(L.wpart),
-                           L.nsplit, (long)h * h, (long)h * h, d_params + woff[l]);
-        GDX_LAUNCH("reduce_partials", reduce_partials_kernel, dim3(nblocks(h)), dim3(NT), 0, st, P(L.cpart), L.nsplit,
-                           (long)h, (long)h, d_params + boff[l]);
+        if (tiles)
+          launch_tile<double, true, false, EPI_STORE>(st, "gemm_f64_wgrad", h, h, L.nsplit, L.kchunk, Hin, h,
+                                                      Gcur, h, P(L.wpart), h, nullptr, nullptr, 0, P(L.cpart));
+        else
+          launch_gemm<64, 64, true, false, EPI_STORE>(st, h, h, Np, Hin, h, Gcur, h, P(L.wpart), h, nullptr,
+                                                      nullptr, 0, L.nsplit, L.kchunk, P(L.cpart));
+        GDX_LAUNCH("reduce_cols", reduce_cols_kernel, dim3(nblocks64((long)h * h)), dim3(NT), 0, st,
+                   P(L.wpart), L.nsplit, (long)h * h, (long)h * h, d_params + woff[l]);
+        GDX_LAUNCH("reduce_cols", reduce_cols_kernel, dim3(nblocks64(h)), dim3(NT), 0, st, P(L.cpart),
+                   L.nsplit, (long)h, (long)h, d_params + boff[l]);
       }
       // delta for layer l's input: (Gcur W_l^T) * elu'(Hin)
-      launch_gemm<128, 128, false, true, EPI_ELU_BWD>(st, Np, h, h, Gcur, h, Wl, h, Gnext, h, nullptr, Hin,
-                                                      h, 1, h, nullptr);
+      if (tiles)
+        launch_tile<double, false, true, EPI_ELU_BWD>(st, "gemm_f64_bwd_data", Np, h, 1, h, Gcur, h, Wl, h, Gnext,
+                                                      h, nullptr, Hin, h, nullptr);
+      else
+        launch_gemm<128, 128, false, true, EPI_ELU_BWD>(st, Np, h, h, Gcur, h, Wl, h, Gnext, h, nullptr, Hin,
+                                                        h, 1, h, nullptr);
       std::swap(Gcur, Gnext);
     }
-    // first layer: dW0 = X^T G (nc x h), db0 = colsum(G), DX = G W0^T
+    // first layer: DX = G W0^T (skinny), dW0 and db0 = X16^T G (rows 0..nc-1 and the ones row nc)
+    GDX_LAUNCH("layer0_bwd", (skinny_out16_kernel<double, false>), dim3(Np / 64), dim3(NT), 0, st, Np, h, Gcur,
+               h, W016, P(L.DX), d.mlp_nc, d.mlp_nc, nullptr, 0, nullptr, nullptr, 0, nullptr);
     if (d_params) {
-      launch_gemm<64, 64, true, false, EPI_STORE>(st, d.mlp_nc, h, Np, P(L.X), d.mlp_nc, Gcur, h, P(L.wpart),
-                                                  h, nullptr, nullptr, 0, L.nsplit, L.kchunk, P(L.cpart));
-      GDX_LAUNCH("reduce_partials", reduce_partials_kernel, dim3(nblocks((long)d.mlp_nc * h)), dim3(NT), 0, st,
-                         P(L.wpart), L.nsplit, (long)d.mlp_nc * h, (long)d.mlp_nc * h, d_params + woff[0]);
-      GDX_LAUNCH("reduce_partials", reduce_partials_kernel, dim3(nblocks(h)), dim3(NT), 0, st, P(L.cpart), L.nsplit,
-                         (long)h, (long)h, d_params + boff[0]);
+      GDX_LAUNCH("skinny_wgrad", skinny_wgrad_kernel<double>, dim3(L.nwblk), dim3(NT), 0, st, Np, L.wrows, P(L.X),
+                 Gcur, h, P(L.wpart));
+      // rows 0..nc of the 16 x h result are W0 (nc x h) followed by b0: contiguous in params
+      GDX_LAUNCH("reduce_cols", reduce_cols_kernel, dim3(nblocks64((long)(d.mlp_nc + 1) * h)), dim3(NT), 0, st,
+                 P(L.wpart), L.nwblk, 16L * h, (long)(d.mlp_nc + 1) * h, d_params + woff[0]);
     }
-    GDX_LAUNCH("layer0_bwd", layer0_bwd_kernel, dim3(1024), dim3(NT), 0, st, Gcur, h, d.mlp_nc, params + woff[0], Np,
-                       P(L.DX));
   } else {
-    GDX_LAUNCH("head_bwd", head_bwd_kernel, dim3(1024), dim3(NT), 0, st, nullptr, 0, nd, nullptr, nullptr, P(L.E),
-                       weights, d.npts, Np, g_exc, 0, P(L.GE), nullptr, nullptr);
+    GDX_LAUNCH("head_seed", head_seed_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, nullptr, P(L.E), weights, nd,
+               d.npts, Np, g_exc, 0, P(L.GE), nullptr, nullptr);
   }
   if (d_rdm1) {
     GDX_LAUNCH("point_bwd", point_bwd_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.dens_fin), P(L.GE),
