@@ -1,0 +1,254 @@
+// AO-side MFMA kernels of the XC path, templated on the element type of the AO planes and
+// the density matrix (double: FP64 path, float: FP32 path).  Both follow the staging scheme
+// of gdx_gemm.h: 16-byte global loads into registers one stage ahead, double-buffered LDS,
+// one barrier per stage.
+//
+// density_ao_kernel  (grad_dft/molecule.py:388-502)
+//   Block = 64 grid points (16 per wave), all four AO components c in {phi, dx, dy, dz}.
+//   For each 64-column chunk of orbitals: X_c = A_c D on MFMA (4 components x 4 column
+//   blocks = 16 accumulators per wave), then the chunk's contribution to the row
+//   contractions is added to per-lane FP64 sums:
+//     rho += X_phi . phi, grad_j += X_phi . d_j phi, lapl += X_phi . lapl(phi),
+//     T += sum_j X_j . d_j phi.
+//   After the last chunk the 16 lanes sharing a row are summed; raw sums are written as
+//   (rho, gx, gy, gz, T, L) and point_fwd applies grad = 2 g, tau = T / 2, lapl = 2 T + 2 L.
+//
+// vxc_ao_kernel  (adjoint of the above, grad_dft/train.py:86-121)
+//   V_s[a][b] = sum_r phi_a Z_b + sum_j d_j phi_a (u d_j phi_b), Z = v_rho phi + sum_j
+//   2 v_grad_j d_j phi + 2 v_lapl lapl(phi), u = v_tau / 2 + 2 v_lapl.  One 128 x 128 output
+//   tile per block (4 waves x 64 x 64), K' = 4 components x 4 points per stage; B rows are
+//   formed from the AO values and the per-point potentials while being stored to LDS.
+//   Blocks split the points into chunks; per-chunk tiles are summed by reduce_vxc.
+#pragma once
+#include "gdx_common.h"
+#include "gdx_gemm.h"
+
+namespace gdx {
+
+constexpr int DN_PT = 64, DN_CB = 64, DN_K = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void density_ao_kernel(const T* __restrict__ ao, int npts_pad, int nP,
+                                                         const T* __restrict__ Dp,
+                                                         double* __restrict__ dens_part) {
+  typedef typename MfmaT<T>::acc_t acc_t;
+  constexpr int VEC = 16 / sizeof(T);
+  constexpr int KV = DN_K / VEC;                           // vectors per 16-k row segment
+  constexpr int NVA = 4 * DN_PT * KV / 256;                // A vectors per thread
+  constexpr int NVB = DN_K * DN_CB / VEC / 256 > 0 ? DN_K * DN_CB / VEC / 256 : 1;
+  constexpr int BVEC = DN_K * DN_CB / VEC;                 // B vectors per stage
+  typedef T vec_t __attribute__((ext_vector_type(VEC)));
+  __shared__ T As[2][DN_K][4][DN_PT + 2];
+  __shared__ T Bs[2][DN_K][DN_CB + 4];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int p0 = blockIdx.x * DN_PT;
+  const int s = blockIdx.y;
+  const T* D = Dp + (size_t)s * nP * nP;
+  const size_t plane = (size_t)npts_pad * nP;
+  const int nk = nP / DN_K;
+
+  double part[4][6];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int v = 0; v < 6; ++v) part[q][v] = 0.0;
+
+  vec_t ra[NVA], rb[NVB];
+  for (int c0 = 0; c0 < nP; c0 += DN_CB) {
+    auto gload = [&](int k0) {
+#pragma unroll
+      for (int i = 0; i < NVA; ++i) {
+        const int idx = tid + 256 * i;
+        const int kv = idx % KV, rest = idx / KV;
+        const int p = rest % DN_PT, c = rest / DN_PT;
+        ra[i] = *(const vec_t*)(ao + c * plane + (size_t)(p0 + p) * nP + k0 + kv * VEC);
+      }
+#pragma unroll
+      for (int i = 0; i < NVB; ++i) {
+        const int idx = tid + 256 * i;
+        if (idx < BVEC) {
+          const int k = idx / (DN_CB / VEC), cv = idx % (DN_CB / VEC);
+          const int col = c0 + cv * VEC;
+          if (col < nP) rb[i] = *(const vec_t*)(D + (size_t)(k0 + k) * nP + col);
+          else rb[i] = (vec_t)(T)0;
+        }
+      }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+      for (int i = 0; i < NVA; ++i) {
+        const int idx = tid + 256 * i;
+        const int kv = idx % KV, rest = idx / KV;
+        const int p = rest % DN_PT, c = rest / DN_PT;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) As[buf][kv * VEC + e][c][p] = ra[i][e];
+      }
+#pragma unroll
+      for (int i = 0; i < NVB; ++i) {
+        const int idx = tid + 256 * i;
+        if (idx < BVEC) {
+          const int k = idx / (DN_CB / VEC), cv = idx % (DN_CB / VEC);
+#pragma unroll
+          for (int e = 0; e < VEC; ++e) Bs[buf][k][cv * VEC + e] = rb[i][e];
+        }
+      }
+    };
+    acc_t acc[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[c][j] = (acc_t){0, 0, 0, 0};
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+      const int cur = kt & 1;
+      if (kt + 1 < nk) gload((kt + 1) * DN_K);
+#pragma unroll
+      for (int kk = 0; kk < DN_K / 4; ++kk) {
+        const int kr = kk * 4 + (lane >> 4);
+        T af[4], bf[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) af[c] = As[cur][kr][c][wave * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) bf[j] = Bs[cur][kr][j * 16 + (lane & 15)];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[c][j] = MfmaT<T>::op(af[c], bf[j], acc[c][j]);
+      }
+      if (kt + 1 < nk) lstore(cur ^ 1);
+      __syncthreads();
+    }
+    // contraction of this column chunk
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int col = c0 + j * 16 + (lane & 15);
+      if (col < nP) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int p = p0 + wave * 16 + MfmaT<T>::row(lane, q);
+          const size_t o = (size_t)p * nP + col;
+          const double phi = ao[o], gx = ao[plane + o], gy = ao[2 * plane + o], gz = ao[3 * plane + o],
+                       lp = ao[4 * plane + o];
+          const double x0 = acc[0][j][q];
+          part[q][0] += x0 * phi;
+          part[q][1] += x0 * gx;
+          part[q][2] += x0 * gy;
+          part[q][3] += x0 * gz;
+          part[q][4] += (double)acc[1][j][q] * gx + (double)acc[2][j][q] * gy + (double)acc[3][j][q] * gz;
+          part[q][5] += x0 * lp;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int v = 0; v < 6; ++v) {
+      const double t = sum16(part[q][v]);
+      if ((lane & 15) == 0) {
+        const int p = p0 + wave * 16 + MfmaT<T>::row(lane, q);
+        dens_part[((size_t)s * npts_pad + p) * DENS_STRIDE + v] = t;
+      }
+    }
+}
+
+constexpr int VX_T = 128, VX_P = 4, VX_K = 4 * VX_P;
+
+template <typename T, bool SAME>
+__global__ __launch_bounds__(256) void vxc_ao_kernel(const T* __restrict__ ao, int npts_pad, int nP,
+                                                     const double* __restrict__ vpt, int chunk,
+                                                     double* __restrict__ vpart) {
+  typedef typename MfmaT<T>::acc_t acc_t;
+  __shared__ T As[2][VX_K][VX_T + 4];
+  __shared__ T Bs[2][VX_K][VX_T + 4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nt = (nP + VX_T - 1) / VX_T;
+  const int a0 = (blockIdx.x / nt) * VX_T, b0 = (blockIdx.x % nt) * VX_T;
+  const int s = blockIdx.z;
+  const int rbeg = blockIdx.y * chunk;
+  const int rend = min(npts_pad, rbeg + chunk);
+  const size_t plane = (size_t)npts_pad * nP;
+  const double* vp = vpt + (size_t)s * npts_pad * VPT_STRIDE;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  // staging role of this thread: point p of the stage, 2 adjacent columns
+  const int sp = tid >> 6, sc = (tid & 63) * 2;
+  typedef T v2 __attribute__((ext_vector_type(2)));
+
+  v2 ra[4], rb[5];
+  double sv[VPT_STRIDE];
+  auto gload = [&](int r0) {
+    const int r = r0 + sp;
+    const bool ok = r < rend;
+    const int ca = a0 + sc, cb = b0 + sc;
+    if (!SAME) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        ra[c] = (ok && ca < nP) ? *(const v2*)(ao + c * plane + (size_t)r * nP + ca) : (v2)(T)0;
+    }
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+      rb[c] = (ok && cb < nP) ? *(const v2*)(ao + c * plane + (size_t)r * nP + cb) : (v2)(T)0;
+#pragma unroll
+    for (int q = 0; q < VPT_STRIDE; ++q) sv[q] = ok ? vp[(size_t)r * VPT_STRIDE + q] : 0.0;
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) As[buf][c * VX_P + sp][sc + e] = SAME ? rb[c][e] : ra[c][e];
+      const double phi = rb[0][e], gx = rb[1][e], gy = rb[2][e], gz = rb[3][e], lp = rb[4][e];
+      Bs[buf][sp][sc + e] = (T)(sv[0] * phi + sv[1] * gx + sv[2] * gy + sv[3] * gz + sv[4] * lp);
+      Bs[buf][VX_P + sp][sc + e] = (T)(sv[5] * gx);
+      Bs[buf][2 * VX_P + sp][sc + e] = (T)(sv[5] * gy);
+      Bs[buf][3 * VX_P + sp][sc + e] = (T)(sv[5] * gz);
+    }
+  };
+
+  acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (acc_t){0, 0, 0, 0};
+  const int nst = (rend - rbeg + VX_P - 1) / VX_P;
+  if (nst > 0) {
+    gload(rbeg);
+    lstore(0);
+    __syncthreads();
+  }
+  for (int st = 0; st < nst; ++st) {
+    const int cur = st & 1;
+    if (st + 1 < nst) gload(rbeg + (st + 1) * VX_P);
+#pragma unroll
+    for (int kk = 0; kk < VX_K / 4; ++kk) {
+      const int kr = kk * 4 + (lane >> 4);
+      T af[4], bf[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) af[i] = As[cur][kr][wm + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bf[j] = Bs[cur][kr][wn + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = MfmaT<T>::op(af[i], bf[j], acc[i][j]);
+    }
+    if (st + 1 < nst) lstore(cur ^ 1);
+    __syncthreads();
+  }
+  double* out = vpart + ((size_t)blockIdx.y * gridDim.z + s) * nP * nP;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int col = b0 + wn + j * 16 + (lane & 15);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = a0 + wm + i * 16 + MfmaT<T>::row(lane, q);
+        if (row < nP && col < nP) out[(size_t)row * nP + col] = (double)acc[i][j][q];
+      }
+    }
+}
+
+}  // namespace gdx

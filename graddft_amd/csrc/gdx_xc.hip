@@ -30,6 +30,7 @@
 #include "gdx_profile.h"
 #include "gdx_gemm.h"
 #include "gdx_skinny.h"
+#include "gdx_aogemm.h"
 
 #include <algorithm>
 #include <vector>
@@ -78,16 +79,17 @@ __global__ void reduce_scalar_kernel(const double* __restrict__ part, int n, dou
 }
 
 // V partials (nchunk, 2, nP, nP) -> d_rdm1 (2, n, n)
-__global__ void reduce_vxc_kernel(const double* __restrict__ part, int nchunk, int nP, int n,
+// nsv = spins computed (1: both spin blocks equal, spin 0 copied)
+__global__ void reduce_vxc_kernel(const double* __restrict__ part, int nchunk, int nsv, int nP, int n,
                                   double* __restrict__ out) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long tot = 2L * n * n;
   if (i >= tot) return;
-  const int s = (int)(i / ((long)n * n));
+  const int s = nsv == 1 ? 0 : (int)(i / ((long)n * n));
   const int a = (int)((i / n) % n);
   const int b = (int)(i % n);
   const long off = ((long)s * nP + a) * nP + b;
-  const long stride = 2L * nP * nP;
+  const long stride = (long)nsv * nP * nP;
   double acc = 0.0;
   for (int c = 0; c < nchunk; ++c) acc += part[c * stride + off];
   out[i] = acc;
@@ -107,123 +109,6 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 __device__ __forceinline__ double elu(double x) { return x > 0.0 ? x : expm1(x); }
 __device__ __forceinline__ double elu_grad_from_out(double y) { return y > 0.0 ? 1.0 : y + 1.0; }
 __device__ __forceinline__ double clip30(double x) { return fabs(x) > 1e-30 ? x : 0.0; }
-
-// ----------------------------------------------------------- density (MFMA f64) ----
-//
-// Block: 4 waves. Tile: PT = 32 points x (4 * 16 * WCB) AO columns of X.  Wave w owns
-// columns [16 * WCB * w, 16 * WCB * (w + 1)) of the tile, both 16-point row blocks and
-// all four AO components, i.e. 2 * 4 * WCB accumulators of 16x16.
-// LDS: As[k][c][p] (AO component c of point p at orbital k), Bs[k][b] (rows of D).
-// Output: raw sums per point (phi.X0, dx.X0, dy.X0, dz.X0, sum_j dj.Xj, lapl.X0) for this
-// column block; point_fwd adds column blocks and applies the factors 2, 1/2.
-
-constexpr int DPT = 32;  // points per tile
-constexpr int DBK = 16;  // k per LDS stage
-
-template <int WCB>
-__global__ __launch_bounds__(NT) void density_f64_kernel(
-    const double* __restrict__ ao, int npts_pad, int nP, const double* __restrict__ Dp,
-    double* __restrict__ dens_part) {
-  constexpr int BN = 64 * WCB;
-  __shared__ double As[DBK][4][DPT + 1];
-  __shared__ double Bs[DBK][BN + 2];
-  __shared__ double red[4][DPT][DENS_STRIDE];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int p0 = blockIdx.x * DPT;
-  const int n0 = blockIdx.y * BN;
-  const int s = blockIdx.z;  // distinct spin index
-  const double* D = Dp + (size_t)s * nP * nP;
-  const size_t plane = (size_t)npts_pad * nP;
-
-  d4 acc[2][4][WCB];
-#pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-      for (int cb = 0; cb < WCB; ++cb) acc[rb][c][cb] = (d4){0.0, 0.0, 0.0, 0.0};
-
-  for (int k0 = 0; k0 < nP; k0 += DBK) {
-    // A: 4 comps x 32 points x 16 k = 2048 doubles, 8 per thread (contiguous in k)
-    {
-      const int c = tid >> 6, p = (tid & 63) >> 1, kh = (tid & 1) * 8;
-      const double* src = ao + c * plane + (size_t)(p0 + p) * nP + k0 + kh;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) As[kh + j][c][p] = src[j];
-    }
-    // B: 16 rows of D x BN columns
-    for (int e = tid; e < DBK * BN; e += NT) {
-      const int k = e / BN, b = e % BN;
-      const int col = n0 + b;
-      Bs[k][b] = col < nP ? D[(size_t)(k0 + k) * nP + col] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < DBK / 4; ++kk) {
-      const int kr = kk * 4 + (lane >> 4);
-      double bf[WCB];
-#pragma unroll
-      for (int cb = 0; cb < WCB; ++cb) bf[cb] = Bs[kr][wave * 16 * WCB + cb * 16 + (lane & 15)];
-#pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const double af = As[kr][c][rb * 16 + (lane & 15)];
-#pragma unroll
-          for (int cb = 0; cb < WCB; ++cb)
-            acc[rb][c][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(af, bf[cb], acc[rb][c][cb], 0, 0, 0);
-        }
-    }
-    __syncthreads();
-  }
-
-  // epilogue: contract X with AO planes at the same (point, orbital)
-  double part[2][4][DENS_STRIDE];
-#pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int q = 0; q < DENS_STRIDE; ++q) part[rb][i][q] = 0.0;
-#pragma unroll
-  for (int cb = 0; cb < WCB; ++cb) {
-    const int col = n0 + wave * 16 * WCB + cb * 16 + (lane & 15);
-    if (col < nP) {
-#pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int p = p0 + rb * 16 + mfma64_row(lane, i);
-          const size_t o = (size_t)p * nP + col;
-          const double phi = ao[o], gx = ao[plane + o], gy = ao[2 * plane + o],
-                       gz = ao[3 * plane + o], lp = ao[4 * plane + o];
-          const double x0 = acc[rb][0][cb][i];
-          part[rb][i][0] += x0 * phi;
-          part[rb][i][1] += x0 * gx;
-          part[rb][i][2] += x0 * gy;
-          part[rb][i][3] += x0 * gz;
-          part[rb][i][4] += acc[rb][1][cb][i] * gx + acc[rb][2][cb][i] * gy + acc[rb][3][cb][i] * gz;
-          part[rb][i][5] += x0 * lp;
-        }
-    }
-  }
-#pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int q = 0; q < DENS_STRIDE; ++q) {
-        const double v = sum16(part[rb][i][q]);
-        if ((lane & 15) == 0) red[wave][rb * 16 + mfma64_row(lane, i)][q] = v;
-      }
-  __syncthreads();
-  if (tid < DPT * DENS_STRIDE) {
-    const int p = tid / DENS_STRIDE, q = tid % DENS_STRIDE;
-    const double v = red[0][p][q] + red[1][p][q] + red[2][p][q] + red[3][p][q];
-    const size_t nspin_blocks = gridDim.z;
-    dens_part[(((size_t)blockIdx.y * nspin_blocks + s) * npts_pad + p0 + p) * DENS_STRIDE + q] = v;
-  }
-}
 
 // ---------------------------------------------------------------- per-point fwd ----
 //
@@ -505,91 +390,6 @@ __global__ __launch_bounds__(NT) void point_bwd_kernel(
     for (int q = 0; q < VPT_STRIDE; ++q) vpt[((size_t)s * npts_pad + r) * VPT_STRIDE + q] = out[s][q];
 }
 
-// ------------------------------------------------------------------ V_xc (f64) ----
-// Output tile 64 x 64 of V_s for one chunk of points; K' = 4 components x VKP points per
-// LDS stage: A rows = (phi, dx, dy, dz)[r][a], B rows = (Z, u dx, u dy, u dz)[r][b].
-constexpr int VBT = 64;
-constexpr int VKP = 8;
-
-__global__ __launch_bounds__(NT) void vxc_f64_kernel(
-    const double* __restrict__ ao, int npts_pad, int nP, const double* __restrict__ vpt,
-    int chunk, double* __restrict__ vpart) {
-  __shared__ double As[4 * VKP][VBT + 2];
-  __shared__ double Bs[4 * VKP][VBT + 2];
-  __shared__ double sv[VKP][VPT_STRIDE];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int ntile = (nP + VBT - 1) / VBT;
-  const int a0 = (blockIdx.x / ntile) * VBT, b0 = (blockIdx.x % ntile) * VBT;
-  const int s = blockIdx.z;
-  const int rbeg = blockIdx.y * chunk;
-  const int rend = min(npts_pad, rbeg + chunk);
-  const size_t plane = (size_t)npts_pad * nP;
-  const double* vp = vpt + (size_t)s * npts_pad * VPT_STRIDE;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-
-  d4 acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
-
-  for (int r0 = rbeg; r0 < rend; r0 += VKP) {
-    if (tid < VKP * VPT_STRIDE) {
-      const int p = tid / VPT_STRIDE, q = tid % VPT_STRIDE;
-      sv[p][q] = (r0 + p < rend) ? vp[(size_t)(r0 + p) * VPT_STRIDE + q] : 0.0;
-    }
-    __syncthreads();
-    for (int e = tid; e < VKP * VBT; e += NT) {
-      const int p = e / VBT, x = e % VBT;
-      const int r = r0 + p;
-      const int ca = a0 + x, cb = b0 + x;
-      const bool okr = r < rend;
-      // A rows
-      for (int c = 0; c < 4; ++c)
-        As[c * VKP + p][x] = (okr && ca < nP) ? ao[c * plane + (size_t)r * nP + ca] : 0.0;
-      // B rows
-      double phi = 0, gx = 0, gy = 0, gz = 0, lp = 0;
-      if (okr && cb < nP) {
-        const size_t o = (size_t)r * nP + cb;
-        phi = ao[o]; gx = ao[plane + o]; gy = ao[2 * plane + o]; gz = ao[3 * plane + o];
-        lp = ao[4 * plane + o];
-      }
-      Bs[p][x] = sv[p][0] * phi + sv[p][1] * gx + sv[p][2] * gy + sv[p][3] * gz + sv[p][4] * lp;
-      Bs[VKP + p][x] = sv[p][5] * gx;
-      Bs[2 * VKP + p][x] = sv[p][5] * gy;
-      Bs[3 * VKP + p][x] = sv[p][5] * gz;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < VKP; ++kk) {
-      const int kr = kk * 4 + (lane >> 4);
-      double af[2], bf[2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) af[i] = As[kr][wm + i * 16 + (lane & 15)];
-#pragma unroll
-      for (int j = 0; j < 2; ++j) bf[j] = Bs[kr][wn + j * 16 + (lane & 15)];
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
-    }
-    __syncthreads();
-  }
-  double* out = vpart + ((size_t)blockIdx.y * 2 + s) * nP * nP;
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int col = b0 + wn + j * 16 + (lane & 15);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int row = a0 + wm + i * 16 + mfma64_row(lane, q);
-        if (row < nP && col < nP) out[(size_t)row * nP + col] = acc[i][j][q];
-      }
-    }
-}
-
 // ------------------------------------------------------------ workspace layout ----
 struct Layout {
   size_t Dp, dens_part, dens_fin, E, X, C, H, eps_part, GE, DZ, G, DX, vpt, vpart, wpart, cpart;
@@ -599,15 +399,15 @@ struct Layout {
   int ncb, wcb, nsd, nchunk, chunk, nsplit, kchunk, nparam, npt_blocks;
 };
 
-constexpr int VCHUNK = 1024;  // points per V_xc split
+constexpr int VCHUNK = 512;   // points per V_xc split
 constexpr int SKW_ROWS = 512;  // grid points per block of the skinny weight-gradient kernel
 
 Layout make_layout(const gdx_xc_desc& d) {
   Layout L{};
   const size_t Np = d.npts_pad, nP = d.nao_pad;
   const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
-  L.wcb = nP <= 64 ? 1 : 2;
-  L.ncb = (int)((nP + 64 * L.wcb - 1) / (64 * L.wcb));
+  L.wcb = 1;
+  L.ncb = 1;
   L.nsd = d.spin_restricted ? 1 : 2;
   L.chunk = VCHUNK;
   L.nchunk = (int)((Np + VCHUNK - 1) / VCHUNK);
@@ -733,12 +533,8 @@ int gdx_xc_forward(void* stream, const gdx_xc_desc* desc, const double* ao, cons
   const int Np = d.npts_pad, nP = d.nao_pad;
 
   GDX_LAUNCH("pad_dm", pad_dm_kernel, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, rdm1, d.nao, nP, P(L.Dp));
-  if (L.wcb == 1)
-    GDX_LAUNCH("density_f64", density_f64_kernel<1>, dim3(Np / DPT, L.ncb, L.nsd), dim3(NT), 0, st, ao, Np,
-                       nP, P(L.Dp), P(L.dens_part));
-  else
-    GDX_LAUNCH("density_f64", density_f64_kernel<2>, dim3(Np / DPT, L.ncb, L.nsd), dim3(NT), 0, st, ao, Np,
-                       nP, P(L.Dp), P(L.dens_part));
+  GDX_LAUNCH("density_f64", density_ao_kernel<double>, dim3(Np / DN_PT, L.nsd), dim3(NT), 0, st, ao, Np, nP,
+             P(L.Dp), P(L.dens_part));
   GDX_LAUNCH("point_fwd", point_fwd_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.dens_part), L.ncb,
                      L.nsd, d.npts, Np, weights, neural ? 1 : 0, P(L.dens_fin), P(L.E), P(L.X),
                      P(L.eps_part));
@@ -793,12 +589,8 @@ int gdx_xc_densities(void* stream, const gdx_xc_desc* desc, const double* ao, co
   auto P = [&](size_t off) { return (double*)(ws + off); };
   const int Np = d.npts_pad, nP = d.nao_pad;
   GDX_LAUNCH("pad_dm", pad_dm_kernel, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, rdm1, d.nao, nP, P(L.Dp));
-  if (L.wcb == 1)
-    GDX_LAUNCH("density_f64", density_f64_kernel<1>, dim3(Np / DPT, L.ncb, L.nsd), dim3(NT), 0, st, ao, Np,
-                       nP, P(L.Dp), P(L.dens_part));
-  else
-    GDX_LAUNCH("density_f64", density_f64_kernel<2>, dim3(Np / DPT, L.ncb, L.nsd), dim3(NT), 0, st, ao, Np,
-                       nP, P(L.Dp), P(L.dens_part));
+  GDX_LAUNCH("density_f64", density_ao_kernel<double>, dim3(Np / DN_PT, L.nsd), dim3(NT), 0, st, ao, Np, nP,
+             P(L.Dp), P(L.dens_part));
   // weights == nullptr: the per-point kernel forms densities and energy densities only
   const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
   GDX_LAUNCH("point_fwd", point_fwd_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.dens_part), L.ncb,
@@ -902,11 +694,17 @@ int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const double* ao,
   if (d_rdm1) {
     GDX_LAUNCH("point_bwd", point_bwd_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.dens_fin), P(L.GE),
                        neural ? P(L.DX) : nullptr, P(L.X), d.npts, Np, P(L.vpt));
-    const int ntile = (nP + VBT - 1) / VBT;
-    GDX_LAUNCH("vxc_f64", vxc_f64_kernel, dim3(ntile * ntile, L.nchunk, 2), dim3(NT), 0, st, ao, Np, nP, P(L.vpt),
-                       L.chunk, P(L.vpart));
+    const int nt = (nP + VX_T - 1) / VX_T;
+    // B3LYP on equal spin densities has equal spin potentials: form V once
+    const int nsv = (!neural && d.spin_restricted) ? 1 : 2;
+    if (nt == 1)
+      GDX_LAUNCH("vxc_f64", (vxc_ao_kernel<double, true>), dim3(1, L.nchunk, nsv), dim3(NT), 0, st, ao, Np, nP,
+                 P(L.vpt), L.chunk, P(L.vpart));
+    else
+      GDX_LAUNCH("vxc_f64", (vxc_ao_kernel<double, false>), dim3(nt * nt, L.nchunk, nsv), dim3(NT), 0, st, ao, Np,
+                 nP, P(L.vpt), L.chunk, P(L.vpart));
     GDX_LAUNCH("reduce_vxc", reduce_vxc_kernel, dim3(nblocks(2L * d.nao * d.nao)), dim3(NT), 0, st, P(L.vpart),
-                       L.nchunk, nP, d.nao, d_rdm1);
+               L.nchunk, nsv, nP, d.nao, d_rdm1);
   }
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
