@@ -46,12 +46,14 @@ def kernel_flops(name, npts, nao, spec, nsd):
     """Algorithmic flops of ONE launch of the named kernel (N real points)."""
     h = spec.h
     table = {
-        "density_f64": nsd * 8 * nao * nao * npts,       # 4 products (phi, dphi) x D
-        "vxc_f64": 2 * 8 * nao * nao * npts,             # phi^T Z + 3 dphi^T u dphi, per spin
-        "gemm_f64_fwd": 2 * npts * h * h,
-        "gemm_f64_bwd_data": 2 * npts * h * h,
-        "gemm_f64_wgrad": 2 * npts * h * h,
+        "density": nsd * 8 * nao * nao * npts,       # 4 products (phi, dphi) x D
+        "vxc": 2 * 8 * nao * nao * npts,             # phi^T Z + 3 dphi^T u dphi, per spin
+        "gemm_fwd": 2 * npts * h * h,
+        "gemm_bwd_data": 2 * npts * h * h,
+        "gemm_wgrad": 2 * npts * h * h,
     }
+    for suffix in ("_f64", "_f32", "_bf16x3"):
+        name = name.replace(suffix, "")
     return table.get(name)
 
 
@@ -135,7 +137,32 @@ def main():
                 "unit": unit, "frac": round(achieved / peak, 4), "traffic": None,
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches_per_step": cnt // args.steps,
                 "flops_per_launch": flops}
-    w_pt = 16 * n * n * nsd + 3 * spec.flops_per_point()
+    # accuracy and speed against the plain FP64 GPU path on the same inputs
+    vs64 = None
+    if args.precision != "fp64":
+        p64 = f.plan(mol, "fp64")
+        e64 = torch.empty_like(exc)
+        dD64 = torch.empty_like(dD)
+        dth64 = torch.empty_like(dth)
+
+        def step64():
+            p64.forward(D, theta, e64)
+            p64.backward(D, theta, one, dD64, dth64, p64.generation)
+
+        step64()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        k64 = max(3, args.steps // 2)
+        for _ in range(k64):
+            step64()
+        torch.cuda.synchronize()
+        ms64 = (time.perf_counter() - t0) / k64 * 1e3
+        rel = lambda a, b: float(torch.linalg.norm(a - b) / torch.linalg.norm(b))  # noqa: E731
+        vs64 = {"fp64_ms_per_step": round(ms64, 4), "fp64_value": round(N / (ms64 / 1e3), 1),
+                "speedup": round(ms64 / ms, 3), "abs_dE_Ha": abs(E - float(e64)),
+                "rel_err_dE_dD": rel(dD, dD64), "rel_err_dE_dtheta": rel(dth, dth64)}
+    nsv = 2
+    w_pt = 8 * n * n * nsd + 8 * n * n * nsv + 3 * spec.flops_per_point()
     step_tflops = w_pt * N / (ms / 1e3) / 1e12
 
     cpu = None
@@ -168,6 +195,7 @@ def main():
                       "frac": round(step_tflops / peak, 4), "flop_per_point": w_pt},
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in
                                 sorted(kt.result.items(), key=lambda kv: -kv[1][0])},
+        "vs_fp64_gpu": vs64,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

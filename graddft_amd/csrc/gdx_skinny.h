@@ -125,7 +125,8 @@ __global__ __launch_bounds__(256) void skinny_wgrad_kernel(
   for (int e = tid; e < 16 * N; e += 256) out[e] = red[e / N][e % N];
 }
 
-__global__ __launch_bounds__(256) void reduce_cols_kernel(const double* __restrict__ part, int nsplit,
+template <typename TP>
+__global__ __launch_bounds__(256) void reduce_cols_kernel(const TP* __restrict__ part, int nsplit,
                                                           long stride, long len,
                                                           double* __restrict__ out) {
   __shared__ double red[4][64];
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(256) void reduce_cols_kernel(const double* __restri
   const long i = (long)blockIdx.x * 64 + c;
   double acc = 0.0;
   if (i < len)
-    for (int s = g; s < nsplit; s += 4) acc += part[(long)s * stride + i];
+    for (int s = g; s < nsplit; s += 4) acc += (double)part[(long)s * stride + i];
   red[g][c] = acc;
   __syncthreads();
   if (g == 0 && i < len) out[i] = red[0][c] + red[1][c] + red[2][c] + red[3][c];

@@ -6,25 +6,21 @@
 //                                          tau = 1/2 sum D dphi.dphi,
 //                                          lapl = 2 sum D dphi.dphi + 2 sum D phi lapl(phi)
 //   energy densities popular_functionals.py:29-326 (gdx_functionals.h)
+//   coefficients     functional.py:143-158, 396-419 (dense layers, 2 sigmoid(z/2) head)
 //   integration      functional.py:219-253, 316-342 (abs_clip of densities, of the
 //                    per-point product and of the weights; FP64 sum)
 //   dE/dD            train.py:86-121 (value_and_grad w.r.t. rdm1, unsymmetrised)
 //
-// Kernels (FP64 path; all matrix products on v_mfma_f64_16x16x4f64):
-//   pad_dm            rdm1 (2,n,n) -> zero-padded (2,nP,nP)
-//   density_f64       per 32-point tile: X_c = A_c D for c in {phi, dx, dy, dz} on MFMA,
-//                     contracted row-wise against the AO planes in the epilogue
-//   point_fwd         partial sums -> rho, grad rho, tau, lapl per spin; B3LYP energy
-//                     densities (clipped); MLP inputs log(|x| + 1e-4) or the B3LYP sum
-//   layer0            first dense layer (K = nc) + ELU on VALU
-//   gemm_f64          tiled MFMA GEMM with bias+ELU / ELU-backward epilogues and split-K
-//   head_fwd          last dense layer, 2 sigmoid(z/2), sum_f c_f e_f, clip, w-weighted sum
-//   head_bwd          seeds of the adjoint: dE/de_f, dE/dz and the last-layer delta
-//   point_bwd         dE/d(rho, |grad rho|^2, lapl) by forward-mode duals + MLP input grads
-//                     -> per-point potentials
-//   vxc_f64           V_s = phi^T Z_s + sum_j dphi_j^T (u_s dphi_j) on MFMA, split over
-//                     point chunks, Z built while staging tiles in LDS
-//   reduce_*          deterministic FP64 reductions of block / chunk partials
+// Precision: the matrix products (density X = A D, V_xc, MLP layers) run on MFMA in the
+// element type T of the path -- double (GDX_FP64) or float (GDX_FP32); the per-point
+// functional math, the density row contractions, every reduction over grid points or
+// split partials and the E_xc sum are FP64 in both paths.
+//
+// Kernel sequence, forward:  pad_dm -> density_ao -> point_fwd -> [layer0 -> gemm_tile
+//   (bias+ELU) x (L-1) -> head_fwd (skinny MFMA + coefficient/energy epilogue)] -> reduce.
+// Backward:  head_seed -> [head_delta -> skinny_wgrad (last layer) -> per hidden layer
+//   gemm_tile wgrad (split-K) + gemm_tile bwd-data (ELU') -> layer0_bwd (skinny) +
+//   skinny_wgrad (first layer)] -> point_bwd (duals) -> vxc_ao -> reduce_vxc.
 #include "gdx_common.h"
 #include "gdx_functionals.h"
 #include "gdx_profile.h"
@@ -38,29 +34,41 @@
 namespace gdx {
 namespace {
 
-constexpr int NT = 256;     // threads per block in every kernel here
+constexpr int NT = 256;  // threads per block in every kernel here
 constexpr double SQUASH = 1e-4;
+constexpr int MAXND = 8;
+
+__device__ __forceinline__ double clip30(double x) { return fabs(x) > 1e-30 ? x : 0.0; }
+__device__ __forceinline__ double elu_grad_from_out(double y) { return y > 0.0 ? 1.0 : y + 1.0; }
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = warp_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+  return t;
+}
 
 // ---------------------------------------------------------------- small kernels ----
 
-__global__ void pad_dm_kernel(const double* __restrict__ rdm1, int n, int nP, double* __restrict__ Dp) {
+template <typename T>
+__global__ void pad_dm_kernel(const double* __restrict__ rdm1, int n, int nP, T* __restrict__ Dp) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long tot = 2L * nP * nP;
   if (i >= tot) return;
   const int s = (int)(i / ((long)nP * nP));
   const int r = (int)((i / nP) % nP);
   const int c = (int)(i % nP);
-  Dp[i] = (r < n && c < n) ? rdm1[((long)s * n + r) * n + c] : 0.0;
+  Dp[i] = (T)((r < n && c < n) ? rdm1[((long)s * n + r) * n + c] : 0.0);
 }
 
-// out[i] = sum_s part[s * stride + i], i < len (deterministic order)
-__global__ void reduce_partials_kernel(const double* __restrict__ part, int nsplit, long stride,
-                                       long len, double* __restrict__ out) {
+template <typename T>
+__global__ void convert_kernel(const double* __restrict__ src, long n, T* __restrict__ dst) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= len) return;
-  double acc = 0.0;
-  for (int s = 0; s < nsplit; ++s) acc += part[(long)s * stride + i];
-  out[i] = acc;
+  if (i < n) dst[i] = (T)src[i];
 }
 
 // single-block deterministic sum of n block partials -> *out
@@ -78,8 +86,7 @@ __global__ void reduce_scalar_kernel(const double* __restrict__ part, int n, dou
   }
 }
 
-// V partials (nchunk, 2, nP, nP) -> d_rdm1 (2, n, n)
-// nsv = spins computed (1: both spin blocks equal, spin 0 copied)
+// V partials (nchunk, nsv, nP, nP) -> d_rdm1 (2, n, n); nsv = 1: spin 0 copied to spin 1
 __global__ void reduce_vxc_kernel(const double* __restrict__ part, int nchunk, int nsv, int nP, int n,
                                   double* __restrict__ out) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -95,29 +102,17 @@ __global__ void reduce_vxc_kernel(const double* __restrict__ part, int nchunk, i
   out[i] = acc;
 }
 
-__device__ __forceinline__ double block_sum(double v, double* red) {
-  v = warp_sum(v);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double t = 0.0;
-  if (threadIdx.x == 0)
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
-  return t;
-}
-
-__device__ __forceinline__ double elu(double x) { return x > 0.0 ? x : expm1(x); }
-__device__ __forceinline__ double elu_grad_from_out(double y) { return y > 0.0 ? 1.0 : y + 1.0; }
-__device__ __forceinline__ double clip30(double x) { return fabs(x) > 1e-30 ? x : 0.0; }
-
 // ---------------------------------------------------------------- per-point fwd ----
 //
 // dens_fin (npts_pad, 2, 6): rho, grad rho (3), tau, lapl per spin (final)
-// E (npts_pad, 4): clipped energy densities; X (npts_pad, 6): MLP inputs (neural only)
+// E (npts_pad, 4): clipped energy densities
+// X16 (npts_pad, 16) (neural): six squashed MLP inputs log(|x| + 1e-4), a 1 (the bias
+//   column of the first layer's gradient), zeros
+template <typename T>
 __global__ __launch_bounds__(NT) void point_fwd_kernel(
-    const double* __restrict__ dens_part, int ncb, int nsd, int npts, int npts_pad,
+    const double* __restrict__ dens_part, int nsd, int npts, int npts_pad,
     const double* __restrict__ weights, int neural, double* __restrict__ dens_fin,
-    double* __restrict__ E, double* __restrict__ X, double* __restrict__ eps_part) {
+    double* __restrict__ E, T* __restrict__ X, double* __restrict__ eps_part) {
   __shared__ double red[NT / 64];
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   double contrib = 0.0;
@@ -125,12 +120,7 @@ __global__ __launch_bounds__(NT) void point_fwd_kernel(
     double d[2][DENS_STRIDE];
     for (int s = 0; s < 2; ++s) {
       const int ds = nsd == 1 ? 0 : s;
-      double raw[DENS_STRIDE] = {0, 0, 0, 0, 0, 0};
-      for (int cb = 0; cb < ncb; ++cb) {
-        const double* src = dens_part + (((size_t)cb * nsd + ds) * npts_pad + r) * DENS_STRIDE;
-#pragma unroll
-        for (int q = 0; q < DENS_STRIDE; ++q) raw[q] += src[q];
-      }
+      const double* raw = dens_part + ((size_t)ds * npts_pad + r) * DENS_STRIDE;
       d[s][0] = raw[0];
       d[s][1] = 2.0 * raw[1];
       d[s][2] = 2.0 * raw[2];
@@ -147,13 +137,12 @@ __global__ __launch_bounds__(NT) void point_fwd_kernel(
 #pragma unroll
     for (int f = 0; f < 4; ++f) E[(size_t)r * 4 + f] = e[f];
     if (neural) {
-      // X16 row: six squashed features, a 1 (bias column of the first layer's gradient), zeros
       const double feat[6] = {d[0][0], d[1][0], gu, gd, d[0][4], d[1][4]};
 #pragma unroll
-      for (int j = 0; j < 6; ++j) X[(size_t)r * 16 + j] = log(fabs(feat[j]) + SQUASH);
-      X[(size_t)r * 16 + 6] = 1.0;
+      for (int j = 0; j < 6; ++j) X[(size_t)r * 16 + j] = (T)log(fabs(feat[j]) + SQUASH);
+      X[(size_t)r * 16 + 6] = (T)1;
 #pragma unroll
-      for (int j = 7; j < 16; ++j) X[(size_t)r * 16 + j] = 0.0;
+      for (int j = 7; j < 16; ++j) X[(size_t)r * 16 + j] = (T)0;
     } else if (weights != nullptr && r < npts) {
       const double c[4] = {1.0 - 0.2, 0.72, 1.0 - 0.81, 0.81};
       double eps = c[0] * e[0] + c[1] * e[1] + c[2] * e[2] + c[3] * e[3];
@@ -165,53 +154,50 @@ __global__ __launch_bounds__(NT) void point_fwd_kernel(
   if (!neural && threadIdx.x == 0) eps_part[blockIdx.x] = t;
 }
 
-// H (npts_pad, h) = ELU(X (npts_pad, nc) W0 (nc, h) + b0)
-__global__ __launch_bounds__(NT) void layer0_kernel(const double* __restrict__ X, int nc, int h,
-                                                    long total, const double* __restrict__ W,
-                                                    const double* __restrict__ b,
-                                                    double* __restrict__ H) {
+// H (npts_pad, h) = ELU(X16[:, :nc] W0 (nc, h) + b0), one thread per element
+template <typename T>
+__global__ __launch_bounds__(NT) void layer0_kernel(const T* __restrict__ X, int nc, int h, long total,
+                                                    const double* __restrict__ W,
+                                                    const double* __restrict__ b, T* __restrict__ H) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= total) return;
   const long r = i / h;
   const int f = (int)(i % h);
   double z = b[f];
-  for (int j = 0; j < nc; ++j) z += X[r * 16 + j] * W[j * h + f];
-  H[i] = elu(z);
+  for (int j = 0; j < nc; ++j) z += (double)X[r * 16 + j] * W[j * h + f];
+  H[i] = (T)(z > 0.0 ? z : expm1(z));
 }
 
-// ------------------------------------------------------------------ GEMM (f64) ----
+// ------------------------------------------------- guarded GEMM (any shape, small) ----
+// Used only when a shape is not whole 128 x 128 tiles (narrow test MLPs).
 constexpr int GBK = 16;
 
-// C (+ z * M * ldc for split-K) = epi(op(A) op(B)); op(A)[m][k] = TA ? A[k*lda+m] : A[m*lda+k],
-// op(B)[k][n] = TB ? B[n*ldb+k] : B[k*ldb+n].  colsum (optional, split-K): column sums of
-// op(B) over this block's k range, written by blockIdx.x == 0 to colsum[z * N + n].
-template <int BM, int BN, bool TA, bool TB, int EPI>
-__global__ __launch_bounds__(NT) void gemm_f64_kernel(
-    int M, int N, int K, const double* __restrict__ A, int lda, const double* __restrict__ B,
-    int ldb, double* __restrict__ C, int ldc, const double* __restrict__ bias,
-    const double* __restrict__ Y, int ldy, int kchunk, double* __restrict__ colsum) {
+template <typename T, int BM, int BN, bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(NT) void gemm_any_kernel(
+    int M, int N, int K, const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb,
+    T* __restrict__ C, int ldc, const T* __restrict__ bias, const T* __restrict__ Y, int ldy, int kchunk,
+    T* __restrict__ colsum) {
+  typedef typename MfmaT<T>::acc_t acc_t;
   constexpr int WM = BM / 2, WN = BN / 2, MB = WM / 16, NB = WN / 16;
-  __shared__ double As[GBK][BM + 2];
-  __shared__ double Bs[GBK][BN + 2];
+  __shared__ T As[GBK][BM + 2];
+  __shared__ T Bs[GBK][BN + 2];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   const int kbeg = blockIdx.z * kchunk;
   const int kend = min(K, kbeg + kchunk);
-  double csum = 0.0;
-
-  d4 acc[MB][NB];
+  T csum = 0;
+  acc_t acc[MB][NB];
 #pragma unroll
   for (int i = 0; i < MB; ++i)
 #pragma unroll
-    for (int j = 0; j < NB; ++j) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
-
+    for (int j = 0; j < NB; ++j) acc[i][j] = (acc_t){0, 0, 0, 0};
   for (int k0 = kbeg; k0 < kend; k0 += GBK) {
     for (int e = tid; e < BM * GBK; e += NT) {
       int m, k;
       if (TA) { k = e / BM; m = e % BM; } else { m = e / GBK; k = e % GBK; }
       const int gm = m0 + m, gk = k0 + k;
-      double v = 0.0;
+      T v = 0;
       if (gm < M && gk < kend) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
       As[k][m] = v;
     }
@@ -219,19 +205,18 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(
       int n, k;
       if (TB) { n = e / GBK; k = e % GBK; } else { k = e / BN; n = e % BN; }
       const int gn = n0 + n, gk = k0 + k;
-      double v = 0.0;
+      T v = 0;
       if (gn < N && gk < kend) v = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
       Bs[k][n] = v;
     }
     __syncthreads();
-    if (colsum != nullptr && blockIdx.x == 0 && tid < BN) {
+    if (colsum != nullptr && blockIdx.x == 0 && tid < BN)
 #pragma unroll
       for (int k = 0; k < GBK; ++k) csum += Bs[k][tid];
-    }
 #pragma unroll
     for (int kk = 0; kk < GBK / 4; ++kk) {
       const int kr = kk * 4 + (lane >> 4);
-      double af[MB], bf[NB];
+      T af[MB], bf[NB];
 #pragma unroll
       for (int i = 0; i < MB; ++i) af[i] = As[kr][wm + i * 16 + (lane & 15)];
 #pragma unroll
@@ -239,15 +224,13 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(
 #pragma unroll
       for (int i = 0; i < MB; ++i)
 #pragma unroll
-        for (int j = 0; j < NB; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < NB; ++j) acc[i][j] = MfmaT<T>::op(af[i], bf[j], acc[i][j]);
     }
     __syncthreads();
   }
   if (colsum != nullptr && blockIdx.x == 0 && tid < BN && n0 + tid < N)
     colsum[(size_t)blockIdx.z * N + n0 + tid] = csum;
-
-  double* Cz = C + (size_t)blockIdx.z * M * ldc;
+  T* Cz = C + (size_t)blockIdx.z * M * ldc;
 #pragma unroll
   for (int i = 0; i < MB; ++i)
 #pragma unroll
@@ -256,38 +239,42 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(
       if (col >= N) continue;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int row = m0 + wm + i * 16 + mfma64_row(lane, q);
+        const int row = m0 + wm + i * 16 + MfmaT<T>::row(lane, q);
         if (row >= M) continue;
-        double v = acc[i][j][q];
-        if (EPI == EPI_BIAS_ELU) v = elu(v + bias[col]);
-        if (EPI == EPI_ELU_BWD) v *= elu_grad_from_out(Y[(size_t)row * ldy + col]);
+        T v = acc[i][j][q];
+        if (EPI == EPI_BIAS_ELU) v = elu_any(v + bias[col]);
+        if (EPI == EPI_ELU_BWD) {
+          const T y = Y[(size_t)row * ldy + col];
+          v *= (y > (T)0 ? (T)1 : y + (T)1);
+        }
         Cz[(size_t)row * ldc + col] = v;
       }
     }
 }
 
 // ----------------------------------------------------------------- MLP head ----
-constexpr int MAXND = 8;
 
 // Padded weight tiles for the skinny kernels: WT16 (16 x h) rows o < nd = W_last[:, o];
 // W016 (16 x h) rows j < nc = W0[j, :]; every other row zero.
+template <typename T>
 __global__ void prep_edge_weights_kernel(const double* __restrict__ W0, int nc,
                                          const double* __restrict__ Wl, int nd, int h,
-                                         double* __restrict__ W016, double* __restrict__ WT16) {
+                                         T* __restrict__ W016, T* __restrict__ WT16) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 16 * h) return;
   const int row = i / h, f = i % h;
-  W016[i] = row < nc ? W0[row * h + f] : 0.0;
-  WT16[i] = row < nd ? Wl[f * nd + row] : 0.0;
+  W016[i] = (T)(row < nc ? W0[row * h + f] : 0.0);
+  WT16[i] = (T)(row < nd ? Wl[f * nd + row] : 0.0);
 }
 
 // Adjoint seeds, one thread per point.  g = g_exc clip(w) [|eps| > 1e-30];
 // GE = dE/de_f = g c_f; neural: DZ16 = dE/dz = g e_f (c_f/2)(1 - c_f/2) zero-padded to 16,
 // and block partials of the last layer's bias gradient sum_r DZ[r][o] -> dbpart[block][o].
+template <typename T>
 __global__ __launch_bounds__(NT) void head_seed_kernel(
-    const double* __restrict__ Cc, const double* __restrict__ E, const double* __restrict__ weights,
+    const T* __restrict__ Cc, const double* __restrict__ E, const double* __restrict__ weights,
     int nd, int npts, int npts_pad, const double* __restrict__ g_exc, int neural,
-    double* __restrict__ GE, double* __restrict__ DZ16, double* __restrict__ dbpart) {
+    double* __restrict__ GE, T* __restrict__ DZ16, double* __restrict__ dbpart) {
   __shared__ double red[NT / 64];
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const double bc[4] = {1.0 - 0.2, 0.72, 1.0 - 0.81, 0.81};
@@ -298,7 +285,7 @@ __global__ __launch_bounds__(NT) void head_seed_kernel(
     double eps = 0.0;
     // padded rows (r >= npts) have no coefficients stored: they take c = 0, seed 0
     for (int o = 0; o < nd; ++o) {
-      c[o] = r >= npts ? 0.0 : (neural ? Cc[(size_t)r * nd + o] : bc[o]);
+      c[o] = r >= npts ? 0.0 : (neural ? (double)Cc[(size_t)r * nd + o] : bc[o]);
       e[o] = E[(size_t)r * nd + o];
       eps += c[o] * e[o];
     }
@@ -308,7 +295,7 @@ __global__ __launch_bounds__(NT) void head_seed_kernel(
       dz[o] = g * e[o] * (c[o] * 0.5) * (1.0 - c[o] * 0.5);
     }
     if (neural)
-      for (int o = 0; o < 16; ++o) DZ16[(size_t)r * 16 + o] = o < nd ? dz[o] : 0.0;
+      for (int o = 0; o < 16; ++o) DZ16[(size_t)r * 16 + o] = (T)(o < nd ? dz[o] : 0.0);
   }
   if (neural)
     for (int o = 0; o < nd; ++o) {
@@ -318,33 +305,34 @@ __global__ __launch_bounds__(NT) void head_seed_kernel(
 }
 
 // G[r][f] = (sum_o DZ16[r][o] W_last[f][o]) * elu'(H[r][f]), one thread per element
-__global__ __launch_bounds__(NT) void head_delta_kernel(const double* __restrict__ DZ16,
-                                                        const double* __restrict__ Wl, int nd, int h,
-                                                        const double* __restrict__ H, long total,
-                                                        double* __restrict__ G) {
+template <typename T>
+__global__ __launch_bounds__(NT) void head_delta_kernel(const T* __restrict__ DZ16, const double* __restrict__ Wl,
+                                                        int nd, int h, const T* __restrict__ H, long total,
+                                                        T* __restrict__ G) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= total) return;
   const long r = i / h;
   const int f = (int)(i % h);
   double v = 0.0;
-  for (int o = 0; o < nd; ++o) v += DZ16[r * 16 + o] * Wl[f * nd + o];
-  G[i] = v * elu_grad_from_out(H[i]);
+  for (int o = 0; o < nd; ++o) v += (double)DZ16[r * 16 + o] * Wl[f * nd + o];
+  G[i] = (T)(v * elu_grad_from_out((double)H[i]));
 }
 
-// d_params last layer: W (h, nd) from the reduced (16 x h) transpose, b from block partials
-__global__ void unpack_last_kernel(const double* __restrict__ dWt16, int h, int nd,
-                                   double* __restrict__ dW) {
+// d_params of the last layer: W (h, nd) from the reduced transposed (16 x h) tile
+__global__ void unpack_last_kernel(const double* __restrict__ dWt16, int h, int nd, double* __restrict__ dW) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= h * nd) return;
   const int f = i / nd, o = i % nd;
   dW[i] = dWt16[(size_t)o * h + f];
 }
 
-// Per-point potentials for the V_xc build from dE/de_f (GE) and dE/dx (DX, neural).
-__global__ __launch_bounds__(NT) void point_bwd_kernel(
-    const double* __restrict__ dens_fin, const double* __restrict__ GE,
-    const double* __restrict__ DX, const double* __restrict__ X, int npts, int npts_pad,
-    double* __restrict__ vpt) {
+// Per-point potentials for the V_xc build from dE/de_f (GE) and dE/dx (DX, neural):
+// forward-mode duals over (rho_up, rho_dn, |grad rho_up|^2, |grad rho_dn|^2, lapl_up,
+// lapl_dn) give sum_f GE_f de_f/dv; the MLP input gradient adds through the log squash.
+template <typename T>
+__global__ __launch_bounds__(NT) void point_bwd_kernel(const double* __restrict__ dens_fin,
+                                                       const double* __restrict__ GE, const T* __restrict__ DX,
+                                                       int nc, int npts, int npts_pad, double* __restrict__ vpt) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= npts_pad) return;
   double out[2][VPT_STRIDE];
@@ -370,7 +358,7 @@ __global__ __launch_bounds__(NT) void point_bwd_kernel(
       double df[6];
       for (int j = 0; j < 6; ++j) {
         const double sg = feat[j] > 0.0 ? 1.0 : (feat[j] < 0.0 ? -1.0 : 0.0);
-        df[j] = DX[(size_t)r * 6 + j] * sg / (fabs(feat[j]) + SQUASH);
+        df[j] = (double)DX[(size_t)r * nc + j] * sg / (fabs(feat[j]) + SQUASH);
       }
       v[0] += df[0]; v[1] += df[1]; v[2] += df[2]; v[3] += df[3];
       vtau[0] = df[4]; vtau[1] = df[5];
@@ -391,31 +379,34 @@ __global__ __launch_bounds__(NT) void point_bwd_kernel(
 }
 
 // ------------------------------------------------------------ workspace layout ----
+constexpr int VCHUNK = 512;    // points per V_xc split
+constexpr int SKW_ROWS = 512;  // grid points per block of the skinny weight-gradient kernel
+
 struct Layout {
-  size_t Dp, dens_part, dens_fin, E, X, C, H, eps_part, GE, DZ, G, DX, vpt, vpart, wpart, cpart;
-  size_t W16, dbpart, dWt16;
-  int nwblk, wrows;
+  size_t Dp, dens_part, dens_fin, E, X, C, H, eps_part, GE, DZ, W16, Wlp, dbpart, dWt16, G, DX, vpt, vpart,
+      wpart, cpart;
   size_t total;
-  int ncb, wcb, nsd, nchunk, chunk, nsplit, kchunk, nparam, npt_blocks;
+  int nsd, nchunk, chunk, nsplit, kchunk, npt_blocks, nwblk, wrows;
+  size_t nparam;
 };
 
-constexpr int VCHUNK = 512;   // points per V_xc split
-constexpr int SKW_ROWS = 512;  // grid points per block of the skinny weight-gradient kernel
+size_t elem_size(int precision) { return precision == GDX_FP64 ? 8 : 4; }
 
 Layout make_layout(const gdx_xc_desc& d) {
   Layout L{};
   const size_t Np = d.npts_pad, nP = d.nao_pad;
+  const size_t es = elem_size(d.precision);
   const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
-  L.wcb = 1;
-  L.ncb = 1;
   L.nsd = d.spin_restricted ? 1 : 2;
   L.chunk = VCHUNK;
   L.nchunk = (int)((Np + VCHUNK - 1) / VCHUNK);
-  // split-K for the weight-gradient GEMMs over grid points: whole 16-row stages per split
+  // split-K for the hidden weight-gradient GEMMs: whole 16-row stages per split
   L.nsplit = 1;
   while (L.nsplit < 128 && (Np / GT_K) % (2 * L.nsplit) == 0) L.nsplit *= 2;
   L.kchunk = (int)(Np / L.nsplit);
   L.npt_blocks = (int)((Np + NT - 1) / NT);
+  L.wrows = SKW_ROWS;
+  L.nwblk = (int)((Np + SKW_ROWS - 1) / SKW_ROWS);
   const size_t h = neural ? d.mlp_h : 0, nh = neural ? d.mlp_nhidden : 0;
   size_t np = 0;
   if (neural) {
@@ -426,36 +417,34 @@ Layout make_layout(const gdx_xc_desc& d) {
       din = dout;
     }
   }
-  L.nparam = (int)np;
+  L.nparam = np;
   size_t off = 0;
-  auto take = [&](size_t ndoubles) { size_t o = off; off += round_up(ndoubles * 8, 256); return o; };
-  L.Dp = take(2 * nP * nP);
-  L.dens_part = take((size_t)L.ncb * L.nsd * Np * DENS_STRIDE);
-  L.dens_fin = take(Np * 2 * DENS_STRIDE);
-  L.E = take(Np * 4);
-  L.X = take(neural ? Np * 16 : 0);
-  L.C = take(neural ? Np * d.mlp_nd : 0);
-  L.H = take(Np * h * nh);
-  L.eps_part = take(std::max<size_t>(Np / 64 + 1, 4096));
-  L.GE = take(Np * 4);
-  L.DZ = take(neural ? Np * 16 : 0);
-  L.W16 = take(neural ? 2 * 16 * (size_t)d.mlp_h : 0);
-  L.dbpart = take(neural ? (size_t)L.npt_blocks * d.mlp_nd : 0);
-  L.dWt16 = take(neural ? 16 * (size_t)d.mlp_h : 0);
-  L.G = take(neural ? 2 * Np * h : 0);
-  L.DX = take(neural ? Np * d.mlp_nc : 0);
-  L.vpt = take(2 * Np * VPT_STRIDE);
-  L.vpart = take((size_t)L.nchunk * 2 * nP * nP);
-  // skinny weight-gradient splits: row chunks of SKW_ROWS points
-  L.wrows = SKW_ROWS;
-  L.nwblk = (int)((Np + SKW_ROWS - 1) / SKW_ROWS);
-  size_t wmax = 0;
+  auto take = [&](size_t bytes) { size_t o = off; off += round_up(bytes, 256); return o; };
+  L.Dp = take(2 * nP * nP * es);
+  L.dens_part = take((size_t)L.nsd * Np * DENS_STRIDE * 8);
+  L.dens_fin = take(Np * 2 * DENS_STRIDE * 8);
+  L.E = take(Np * 4 * 8);
+  L.X = take(neural ? Np * 16 * es : 0);
+  L.C = take(neural ? Np * d.mlp_nd * es : 0);
+  L.H = take(Np * h * nh * es);
+  L.eps_part = take(std::max<size_t>(Np / 64 + 1, 4096) * 8);
+  L.GE = take(Np * 4 * 8);
+  L.DZ = take(neural ? Np * 16 * es : 0);
+  L.W16 = take(neural ? 2 * 16 * h * es : 0);
+  L.Wlp = take(neural && es != 8 ? np * es : 0);
+  L.dbpart = take(neural ? (size_t)L.npt_blocks * d.mlp_nd * 8 : 0);
+  L.dWt16 = take(neural ? 16 * h * 8 : 0);
+  L.G = take(neural ? 2 * Np * h * es : 0);
+  L.DX = take(neural ? Np * d.mlp_nc * es : 0);
+  L.vpt = take(2 * Np * VPT_STRIDE * 8);
+  L.vpart = take((size_t)L.nchunk * 2 * nP * nP * 8);
+  size_t wbytes = 0;
   if (neural) {
-    wmax = (size_t)L.nsplit * d.mlp_h * d.mlp_h;
-    wmax = std::max(wmax, (size_t)L.nwblk * 16 * d.mlp_h);
+    wbytes = (size_t)L.nsplit * h * h * es;
+    wbytes = std::max(wbytes, (size_t)L.nwblk * 16 * h * 8);
   }
-  L.wpart = take(wmax);
-  L.cpart = take(neural ? (size_t)L.nsplit * d.mlp_h : 0);
+  L.wpart = take(wbytes);
+  L.cpart = take(neural ? (size_t)L.nsplit * h * es : 0);
   L.total = off;
   return L;
 }
@@ -466,8 +455,8 @@ int validate(const gdx_xc_desc* d) {
   GDX_REQUIRE(d->npts_pad >= d->npts && d->npts_pad % 64 == 0, "npts_pad must be a multiple of 64 >= npts");
   GDX_REQUIRE(d->nao_pad >= d->nao && d->nao_pad % 16 == 0, "nao_pad must be a multiple of 16 >= nao");
   GDX_REQUIRE(d->kind == GDX_FUNC_B3LYP || d->kind == GDX_FUNC_NEURAL_B3LYP, "unknown functional kind");
-  if (d->precision != GDX_FP64) {
-    set_error("this build implements the FP64 grid path only");
+  if (d->precision != GDX_FP64 && d->precision != GDX_FP32) {
+    set_error("precision not implemented by this build (GDX_FP64, GDX_FP32 are)");
     return GDX_ERR_UNSUPPORTED;
   }
   if (d->kind == GDX_FUNC_NEURAL_B3LYP) {
@@ -483,24 +472,216 @@ int validate(const gdx_xc_desc* d) {
 inline unsigned nblocks(long n) { return (unsigned)((n + NT - 1) / NT); }
 inline unsigned nblocks64(long n) { return (unsigned)((n + 63) / 64); }
 
-template <int BM, int BN, bool TA, bool TB, int EPI>
-void launch_gemm(hipStream_t st, int M, int N, int K, const double* A, int lda, const double* B,
-                 int ldb, double* C, int ldc, const double* bias, const double* Y, int ldy,
-                 int nsplit, int kchunk, double* colsum) {
+template <typename T> const char* tname(const char* f64, const char* f32) { return sizeof(T) == 8 ? f64 : f32; }
+
+template <typename T, int BM, int BN, bool TA, bool TB, int EPI>
+void launch_any(hipStream_t st, const char* name, int M, int N, int K, const T* A, int lda, const T* B, int ldb,
+                T* C, int ldc, const T* bias, const T* Y, int ldy, int nsplit, int kchunk, T* colsum) {
   dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, nsplit);
-  const char* name = TA ? (M == N ? "gemm_f64_wgrad" : "gemm_f64_wgrad_edge")
-                        : (EPI == EPI_BIAS_ELU ? "gemm_f64_fwd" : "gemm_f64_bwd_data");
-  GDX_LAUNCH(name, (gemm_f64_kernel<BM, BN, TA, TB, EPI>), grid, dim3(NT), 0, st, M, N, K, A, lda,
-             B, ldb, C, ldc, bias, Y, ldy, kchunk, colsum);
+  GDX_LAUNCH(name, (gemm_any_kernel<T, BM, BN, TA, TB, EPI>), grid, dim3(NT), 0, st, M, N, K, A, lda, B, ldb, C,
+             ldc, bias, Y, ldy, kchunk, colsum);
 }
 
 template <typename T, bool TA, bool TB, int EPI>
-void launch_tile(hipStream_t st, const char* name, int M, int N, int nsplit, int kchunk, const T* A,
-                 int lda, const T* B, int ldb, T* C, int ldc, const T* bias, const T* Y, int ldy,
-                 T* colsum) {
+void launch_tile(hipStream_t st, const char* name, int M, int N, int nsplit, int kchunk, const T* A, int lda,
+                 const T* B, int ldb, T* C, int ldc, const T* bias, const T* Y, int ldy, T* colsum) {
   dim3 grid((M / GT_M) * (N / GT_N), nsplit);
-  GDX_LAUNCH(name, (gemm_tile_kernel<T, TA, TB, EPI>), grid, dim3(NT), 0, st, M, N, A, lda, B, ldb, C,
-             ldc, bias, Y, ldy, kchunk, colsum);
+  GDX_LAUNCH(name, (gemm_tile_kernel<T, TA, TB, EPI>), grid, dim3(NT), 0, st, M, N, A, lda, B, ldb, C, ldc,
+             bias, Y, ldy, kchunk, colsum);
+}
+
+// parameter offsets (elements) of W_l and b_l in the flat parameter vector
+struct ParamOffsets {
+  std::vector<size_t> w, b;
+};
+ParamOffsets param_offsets(const gdx_xc_desc& d) {
+  ParamOffsets o;
+  size_t off = 0;
+  int din = d.mlp_nc;
+  for (int l = 0; l <= d.mlp_nhidden; ++l) {
+    const int dout = l == d.mlp_nhidden ? d.mlp_nd : d.mlp_h;
+    o.w.push_back(off); off += (size_t)din * dout;
+    o.b.push_back(off); off += dout;
+    din = dout;
+  }
+  return o;
+}
+
+struct Ctx {
+  const gdx_xc_desc& d;
+  Layout L;
+  char* ws;
+  hipStream_t st;
+  template <typename U> U* P(size_t off) const { return (U*)(ws + off); }
+};
+
+// densities: pad D -> density_ao -> point_fwd
+template <typename T>
+void run_densities(const Ctx& c, const T* ao, const double* rdm1, const double* weights, double* dens_fin) {
+  const gdx_xc_desc& d = c.d;
+  const int Np = d.npts_pad, nP = d.nao_pad;
+  const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
+  hipStream_t st = c.st;
+  GDX_LAUNCH("pad_dm", pad_dm_kernel<T>, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, rdm1, d.nao, nP,
+             c.P<T>(c.L.Dp));
+  GDX_LAUNCH(tname<T>("density_f64", "density_f32"), density_ao_kernel<T>, dim3(Np / DN_PT, c.L.nsd), dim3(NT), 0,
+             st, ao, Np, nP, c.P<T>(c.L.Dp), c.P<double>(c.L.dens_part));
+  GDX_LAUNCH("point_fwd", point_fwd_kernel<T>, dim3(c.L.npt_blocks), dim3(NT), 0, st, c.P<double>(c.L.dens_part),
+             c.L.nsd, d.npts, Np, weights, neural ? 1 : 0, dens_fin, c.P<double>(c.L.E), c.P<T>(c.L.X),
+             c.P<double>(c.L.eps_part));
+}
+
+// hidden-layer weights in the path's element type (params are FP64 masters)
+template <typename T>
+const T* lowp_params(const Ctx& c, const double* params) {
+  if (sizeof(T) == 8) return (const T*)params;
+  T* dst = c.P<T>(c.L.Wlp);
+  GDX_LAUNCH("convert_params", convert_kernel<T>, dim3(nblocks((long)c.L.nparam)), dim3(NT), 0, c.st, params,
+             (long)c.L.nparam, dst);
+  return dst;
+}
+
+template <typename T>
+int forward_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm1, const double* params,
+                 double* exc) {
+  const gdx_xc_desc& d = c.d;
+  const Layout& L = c.L;
+  hipStream_t st = c.st;
+  const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
+  const int Np = d.npts_pad;
+  run_densities<T>(c, ao, rdm1, weights, c.P<double>(L.dens_fin));
+  int nred = L.npt_blocks;
+  if (neural) {
+    const int h = d.mlp_h;
+    const bool tiles = Np % GT_M == 0 && h % GT_N == 0;
+    const ParamOffsets po = param_offsets(d);
+    const T* plp = lowp_params<T>(c, params);
+    T* H = c.P<T>(L.H);
+    GDX_LAUNCH("layer0", layer0_kernel<T>, dim3(nblocks((long)Np * h)), dim3(NT), 0, st, c.P<T>(L.X), d.mlp_nc, h,
+               (long)Np * h, params + po.w[0], params + po.b[0], H);
+    for (int l = 1; l < d.mlp_nhidden; ++l) {
+      const T* Wl = plp + po.w[l];
+      const T* bl = plp + po.b[l];
+      const T* Hin = H + (size_t)(l - 1) * Np * h;
+      T* Hout = H + (size_t)l * Np * h;
+      if (tiles)
+        launch_tile<T, false, false, EPI_BIAS_ELU>(st, tname<T>("gemm_f64_fwd", "gemm_f32_fwd"), Np, h, 1, h, Hin,
+                                                   h, Wl, h, Hout, h, bl, nullptr, 0, nullptr);
+      else
+        launch_any<T, 128, 128, false, false, EPI_BIAS_ELU>(st, "gemm_any", Np, h, h, Hin, h, Wl, h, Hout, h, bl,
+                                                            nullptr, 0, 1, h, nullptr);
+    }
+    const int nl = d.mlp_nhidden;
+    T* W016 = c.P<T>(L.W16);
+    T* WT16 = W016 + 16 * h;
+    GDX_LAUNCH("prep_edge_weights", prep_edge_weights_kernel<T>, dim3(nblocks(16L * h)), dim3(NT), 0, st,
+               params + po.w[0], d.mlp_nc, params + po.w[nl], d.mlp_nd, h, W016, WT16);
+    GDX_LAUNCH("head_fwd", (skinny_out16_kernel<T, true>), dim3(Np / 64), dim3(NT), 0, st, Np, h,
+               H + (size_t)(nl - 1) * Np * h, h, WT16, c.P<T>(L.C), d.mlp_nd, d.mlp_nd, plp + po.b[nl], d.mlp_nd,
+               c.P<double>(L.E), weights, d.npts, c.P<double>(L.eps_part));
+    nred = Np / 64;
+  }
+  GDX_LAUNCH("reduce_scalar", reduce_scalar_kernel, dim3(1), dim3(NT), 0, st, c.P<double>(L.eps_part), nred, exc);
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+template <typename T>
+int backward_impl(const Ctx& c, const T* ao, const double* weights, const double* params, const double* g_exc,
+                  double* d_rdm1, double* d_params) {
+  const gdx_xc_desc& d = c.d;
+  const Layout& L = c.L;
+  hipStream_t st = c.st;
+  const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
+  const int Np = d.npts_pad, nP = d.nao_pad;
+  const int nd = 4;
+  if (neural) {
+    const int h = d.mlp_h;
+    const bool tiles = Np % GT_M == 0 && h % GT_N == 0;
+    const ParamOffsets po = param_offsets(d);
+    const int nl = d.mlp_nhidden;
+    // the low-precision parameter copy and edge tiles were written by the forward call
+    const T* plp = sizeof(T) == 8 ? (const T*)params : c.P<T>(L.Wlp);
+    T* H = c.P<T>(L.H);
+    T* G0 = c.P<T>(L.G);
+    T* G1 = G0 + (size_t)Np * h;
+    const T* Hlast = H + (size_t)(nl - 1) * Np * h;
+    const T* W016 = c.P<T>(L.W16);
+    GDX_LAUNCH("head_seed", head_seed_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, c.P<T>(L.C),
+               c.P<double>(L.E), weights, nd, d.npts, Np, g_exc, 1, c.P<double>(L.GE), c.P<T>(L.DZ),
+               c.P<double>(L.dbpart));
+    GDX_LAUNCH("head_delta", head_delta_kernel<T>, dim3(nblocks((long)Np * h)), dim3(NT), 0, st, c.P<T>(L.DZ),
+               params + po.w[nl], nd, h, Hlast, (long)Np * h, G0);
+    if (d_params) {
+      GDX_LAUNCH("skinny_wgrad", skinny_wgrad_kernel<T>, dim3(L.nwblk), dim3(NT), 0, st, Np, L.wrows, c.P<T>(L.DZ),
+                 Hlast, h, c.P<double>(L.wpart));
+      GDX_LAUNCH("reduce_cols", reduce_cols_kernel<double>, dim3(nblocks64(16L * h)), dim3(NT), 0, st,
+                 c.P<double>(L.wpart), L.nwblk, 16L * h, 16L * h, c.P<double>(L.dWt16));
+      GDX_LAUNCH("unpack_last", unpack_last_kernel, dim3(nblocks((long)h * nd)), dim3(NT), 0, st,
+                 c.P<double>(L.dWt16), h, nd, d_params + po.w[nl]);
+      GDX_LAUNCH("reduce_cols", reduce_cols_kernel<double>, dim3(1), dim3(NT), 0, st, c.P<double>(L.dbpart),
+                 L.npt_blocks, (long)nd, (long)nd, d_params + po.b[nl]);
+    }
+    T* Gcur = G0;
+    T* Gnext = G1;
+    for (int l = nl - 1; l >= 1; --l) {
+      const T* Hin = H + (size_t)(l - 1) * Np * h;  // input activations of layer l
+      const T* Wl = plp + po.w[l];
+      if (d_params) {
+        T* wp = c.P<T>(L.wpart);
+        T* cp = c.P<T>(L.cpart);
+        if (tiles)
+          launch_tile<T, true, false, EPI_STORE>(st, tname<T>("gemm_f64_wgrad", "gemm_f32_wgrad"), h, h, L.nsplit,
+                                                 L.kchunk, Hin, h, Gcur, h, wp, h, nullptr, nullptr, 0, cp);
+        else
+          launch_any<T, 64, 64, true, false, EPI_STORE>(st, "gemm_any", h, h, Np, Hin, h, Gcur, h, wp, h, nullptr,
+                                                        nullptr, 0, L.nsplit, L.kchunk, cp);
+        GDX_LAUNCH("reduce_cols", reduce_cols_kernel<T>, dim3(nblocks64((long)h * h)), dim3(NT), 0, st, wp, L.nsplit,
+                   (long)h * h, (long)h * h, d_params + po.w[l]);
+        GDX_LAUNCH("reduce_cols", reduce_cols_kernel<T>, dim3(nblocks64(h)), dim3(NT), 0, st, cp, L.nsplit, (long)h,
+                   (long)h, d_params + po.b[l]);
+      }
+      // delta for layer l's input: (Gcur W_l^T) * elu'(Hin)
+      if (tiles)
+        launch_tile<T, false, true, EPI_ELU_BWD>(st, tname<T>("gemm_f64_bwd_data", "gemm_f32_bwd_data"), Np, h, 1,
+                                                 h, Gcur, h, Wl, h, Gnext, h, nullptr, Hin, h, nullptr);
+      else
+        launch_any<T, 128, 128, false, true, EPI_ELU_BWD>(st, "gemm_any", Np, h, h, Gcur, h, Wl, h, Gnext, h,
+                                                          nullptr, Hin, h, 1, h, nullptr);
+      std::swap(Gcur, Gnext);
+    }
+    // first layer: DX = G W0^T (skinny), dW0 and db0 = X16^T G (rows 0..nc-1 and the ones row nc)
+    GDX_LAUNCH("layer0_bwd", (skinny_out16_kernel<T, false>), dim3(Np / 64), dim3(NT), 0, st, Np, h, Gcur, h,
+               W016, c.P<T>(L.DX), d.mlp_nc, d.mlp_nc, nullptr, 0, nullptr, nullptr, 0, nullptr);
+    if (d_params) {
+      GDX_LAUNCH("skinny_wgrad", skinny_wgrad_kernel<T>, dim3(L.nwblk), dim3(NT), 0, st, Np, L.wrows, c.P<T>(L.X),
+                 Gcur, h, c.P<double>(L.wpart));
+      // rows 0..nc of the 16 x h result are W0 (nc x h) followed by b0: contiguous in params
+      GDX_LAUNCH("reduce_cols", reduce_cols_kernel<double>, dim3(nblocks64((long)(d.mlp_nc + 1) * h)), dim3(NT), 0,
+                 st, c.P<double>(L.wpart), L.nwblk, 16L * h, (long)(d.mlp_nc + 1) * h, d_params + po.w[0]);
+    }
+  } else {
+    GDX_LAUNCH("head_seed", head_seed_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, nullptr, c.P<double>(L.E),
+               weights, nd, d.npts, Np, g_exc, 0, c.P<double>(L.GE), nullptr, nullptr);
+  }
+  if (d_rdm1) {
+    GDX_LAUNCH("point_bwd", point_bwd_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, c.P<double>(L.dens_fin),
+               c.P<double>(L.GE), neural ? c.P<T>(L.DX) : nullptr, d.mlp_nc, d.npts, Np, c.P<double>(L.vpt));
+    const int nt = (nP + VX_T - 1) / VX_T;
+    // B3LYP on equal spin densities has equal spin potentials: form V once
+    const int nsv = (!neural && d.spin_restricted) ? 1 : 2;
+    const char* nm = tname<T>("vxc_f64", "vxc_f32");
+    if (nt == 1)
+      GDX_LAUNCH(nm, (vxc_ao_kernel<T, true>), dim3(1, L.nchunk, nsv), dim3(NT), 0, st, ao, Np, nP,
+                 c.P<double>(L.vpt), L.chunk, c.P<double>(L.vpart));
+    else
+      GDX_LAUNCH(nm, (vxc_ao_kernel<T, false>), dim3(nt * nt, L.nchunk, nsv), dim3(NT), 0, st, ao, Np, nP,
+                 c.P<double>(L.vpt), L.chunk, c.P<double>(L.vpart));
+    GDX_LAUNCH("reduce_vxc", reduce_vxc_kernel, dim3(nblocks(2L * d.nao * d.nao)), dim3(NT), 0, st,
+               c.P<double>(L.vpart), L.nchunk, nsv, nP, d.nao, d_rdm1);
+  }
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
 }
 
 }  // namespace
@@ -518,196 +699,43 @@ int gdx_xc_workspace_size(const gdx_xc_desc* desc, size_t* bytes) {
   return GDX_OK;
 }
 
-int gdx_xc_forward(void* stream, const gdx_xc_desc* desc, const double* ao, const double* weights,
+int gdx_xc_forward(void* stream, const gdx_xc_desc* desc, const void* ao, const double* weights,
                    const double* rdm1, const double* params, double* exc, void* workspace) {
   int rc = validate(desc);
   if (rc != GDX_OK) return rc;
-  const gdx_xc_desc& d = *desc;
-  const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
+  const bool neural = desc->kind == GDX_FUNC_NEURAL_B3LYP;
   GDX_REQUIRE(ao && weights && rdm1 && exc && workspace, "gdx_xc_forward: null pointer");
   GDX_REQUIRE(!neural || params, "gdx_xc_forward: neural functional needs params");
-  hipStream_t st = (hipStream_t)stream;
-  const Layout L = make_layout(d);
-  char* ws = (char*)workspace;
-  auto P = [&](size_t off) { return (double*)(ws + off); };
-  const int Np = d.npts_pad, nP = d.nao_pad;
-
-  GDX_LAUNCH("pad_dm", pad_dm_kernel, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, rdm1, d.nao, nP, P(L.Dp));
-  GDX_LAUNCH("density_f64", density_ao_kernel<double>, dim3(Np / DN_PT, L.nsd), dim3(NT), 0, st, ao, Np, nP,
-             P(L.Dp), P(L.dens_part));
-  GDX_LAUNCH("point_fwd", point_fwd_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.dens_part), L.ncb,
-                     L.nsd, d.npts, Np, weights, neural ? 1 : 0, P(L.dens_fin), P(L.E), P(L.X),
-                     P(L.eps_part));
-  int nred = L.npt_blocks;
-  if (neural) {
-    const int h = d.mlp_h;
-    const bool tiles = Np % GT_M == 0 && h % GT_N == 0;
-    const double* W = params;
-    const double* b = W + (size_t)d.mlp_nc * h;
-    double* H = P(L.H);
-    GDX_LAUNCH("layer0", layer0_kernel, dim3(nblocks((long)Np * h)), dim3(NT), 0, st, P(L.X), d.mlp_nc, h,
-                       (long)Np * h, W, b, H);
-    const double* pw = b + h;
-    for (int l = 1; l < d.mlp_nhidden; ++l) {
-      const double* Wl = pw;
-      const double* bl = Wl + (size_t)h * h;
-      if (tiles)
-        launch_tile<double, false, false, EPI_BIAS_ELU>(st, "gemm_f64_fwd", Np, h, 1, h,
-                                                        H + (size_t)(l - 1) * Np * h, h, Wl, h,
-                                                        H + (size_t)l * Np * h, h, bl, nullptr, 0, nullptr);
-      else
-        launch_gemm<128, 128, false, false, EPI_BIAS_ELU>(st, Np, h, h, H + (size_t)(l - 1) * Np * h, h,
-                                                          Wl, h, H + (size_t)l * Np * h, h, bl, nullptr,
-                                                          0, 1, h, nullptr);
-      pw = bl + h;
-    }
-    const double* Wlast = pw;
-    const double* blast = Wlast + (size_t)h * d.mlp_nd;
-    double* W016 = P(L.W16);
-    double* WT16 = W016 + 16 * h;
-    GDX_LAUNCH("prep_edge_weights", prep_edge_weights_kernel, dim3(nblocks(16L * h)), dim3(NT), 0, st, W, d.mlp_nc,
-               Wlast, d.mlp_nd, h, W016, WT16);
-    GDX_LAUNCH("head_fwd", (skinny_out16_kernel<double, true>), dim3(Np / 64), dim3(NT), 0, st, Np, h,
-               H + (size_t)(d.mlp_nhidden - 1) * Np * h, h, WT16, P(L.C), d.mlp_nd, d.mlp_nd, blast, d.mlp_nd,
-               P(L.E), weights, d.npts, P(L.eps_part));
-    nred = Np / 64;
-  }
-  GDX_LAUNCH("reduce_scalar", reduce_scalar_kernel, dim3(1), dim3(NT), 0, st, P(L.eps_part), nred, exc);
-  GDX_HIP_CHECK(hipGetLastError());
-  return GDX_OK;
+  Ctx c{*desc, make_layout(*desc), (char*)workspace, (hipStream_t)stream};
+  if (desc->precision == GDX_FP64) return forward_impl<double>(c, (const double*)ao, weights, rdm1, params, exc);
+  return forward_impl<float>(c, (const float*)ao, weights, rdm1, params, exc);
 }
 
-int gdx_xc_densities(void* stream, const gdx_xc_desc* desc, const double* ao, const double* rdm1,
-                     double* dens, void* workspace) {
+int gdx_xc_densities(void* stream, const gdx_xc_desc* desc, const void* ao, const double* rdm1, double* dens,
+                     void* workspace) {
   int rc = validate(desc);
   if (rc != GDX_OK) return rc;
-  const gdx_xc_desc& d = *desc;
   GDX_REQUIRE(ao && rdm1 && dens && workspace, "gdx_xc_densities: null pointer");
-  hipStream_t st = (hipStream_t)stream;
-  const Layout L = make_layout(d);
-  char* ws = (char*)workspace;
-  auto P = [&](size_t off) { return (double*)(ws + off); };
-  const int Np = d.npts_pad, nP = d.nao_pad;
-  GDX_LAUNCH("pad_dm", pad_dm_kernel, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, rdm1, d.nao, nP, P(L.Dp));
-  GDX_LAUNCH("density_f64", density_ao_kernel<double>, dim3(Np / DN_PT, L.nsd), dim3(NT), 0, st, ao, Np, nP,
-             P(L.Dp), P(L.dens_part));
+  Ctx c{*desc, make_layout(*desc), (char*)workspace, (hipStream_t)stream};
   // weights == nullptr: the per-point kernel forms densities and energy densities only
-  const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
-  GDX_LAUNCH("point_fwd", point_fwd_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.dens_part), L.ncb,
-                     L.nsd, d.npts, Np, nullptr, neural ? 1 : 0, dens, P(L.E), P(L.X), P(L.eps_part));
+  if (desc->precision == GDX_FP64) run_densities<double>(c, (const double*)ao, rdm1, nullptr, dens);
+  else run_densities<float>(c, (const float*)ao, rdm1, nullptr, dens);
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
 }
 
-int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const double* ao,
-                    const double* weights, const double* rdm1, const double* params,
-                    const double* g_exc, double* d_rdm1, double* d_params, void* workspace) {
+int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const void* ao, const double* weights,
+                    const double* rdm1, const double* params, const double* g_exc, double* d_rdm1,
+                    double* d_params, void* workspace) {
   int rc = validate(desc);
   if (rc != GDX_OK) return rc;
-  const gdx_xc_desc& d = *desc;
-  const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
+  const bool neural = desc->kind == GDX_FUNC_NEURAL_B3LYP;
   GDX_REQUIRE(ao && weights && rdm1 && g_exc && workspace, "gdx_xc_backward: null pointer");
   GDX_REQUIRE(!neural || params, "gdx_xc_backward: neural functional needs params");
-  hipStream_t st = (hipStream_t)stream;
-  const Layout L = make_layout(d);
-  char* ws = (char*)workspace;
-  auto P = [&](size_t off) { return (double*)(ws + off); };
-  const int Np = d.npts_pad, nP = d.nao_pad;
-  const int h = neural ? d.mlp_h : 0;
-  const int nd = 4;
-
-  const double* Wlast = nullptr;
-  if (neural) {
-    // parameter offsets
-    std::vector<size_t> woff, boff;
-    size_t off = 0;
-    int din = d.mlp_nc;
-    for (int l = 0; l <= d.mlp_nhidden; ++l) {
-      const int dout = l == d.mlp_nhidden ? d.mlp_nd : h;
-      woff.push_back(off); off += (size_t)din * dout;
-      boff.push_back(off); off += dout;
-      din = dout;
-    }
-    Wlast = params + woff.back();
-    const bool tiles = Np % GT_M == 0 && h % GT_N == 0;
-    double* H = P(L.H);
-    double* G0 = P(L.G);
-    double* G1 = P(L.G) + (size_t)Np * h;
-    const double* Hlast = H + (size_t)(d.mlp_nhidden - 1) * Np * h;
-    const double* W016 = P(L.W16);
-    const int nl = d.mlp_nhidden;
-    GDX_LAUNCH("head_seed", head_seed_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.C), P(L.E), weights, nd,
-               d.npts, Np, g_exc, 1, P(L.GE), P(L.DZ), P(L.dbpart));
-    GDX_LAUNCH("head_delta", head_delta_kernel, dim3(nblocks((long)Np * h)), dim3(NT), 0, st, P(L.DZ), Wlast, nd,
-               h, Hlast, (long)Np * h, G0);
-    if (d_params) {
-      GDX_LAUNCH("skinny_wgrad", skinny_wgrad_kernel<double>, dim3(L.nwblk), dim3(NT), 0, st, Np, L.wrows, P(L.DZ),
-                 Hlast, h, P(L.wpart));
-      GDX_LAUNCH("reduce_cols", reduce_cols_kernel, dim3(nblocks64(16L * h)), dim3(NT), 0, st, P(L.wpart),
-                 L.nwblk, 16L * h, 16L * h, P(L.dWt16));
-      GDX_LAUNCH("unpack_last", unpack_last_kernel, dim3(nblocks((long)h * nd)), dim3(NT), 0, st, P(L.dWt16), h,
-                 nd, d_params + woff[nl]);
-      GDX_LAUNCH("reduce_cols", reduce_cols_kernel, dim3(1), dim3(NT), 0, st, P(L.dbpart), L.npt_blocks,
-                 (long)nd, (long)nd, d_params + boff[nl]);
-    }
-    // hidden h x h layers, from the top
-    double* Gcur = G0;
-    double* Gnext = G1;
-    for (int l = nl - 1; l >= 1; --l) {
-      const double* Hin = H + (size_t)(l - 1) * Np * h;  // input activations of layer l
-      const double* Wl = params + woff[l];
-      if (d_params) {
-        if (tiles)
-          launch_tile<double, true, false, EPI_STORE>(st, "gemm_f64_wgrad", h, h, L.nsplit, L.kchunk, Hin, h,
-                                                      Gcur, h, P(L.wpart), h, nullptr, nullptr, 0, P(L.cpart));
-        else
-          launch_gemm<64, 64, true, false, EPI_STORE>(st, h, h, Np, Hin, h, Gcur, h, P(L.wpart), h, nullptr,
-                                                      nullptr, 0, L.nsplit, L.kchunk, P(L.cpart));
-        GDX_LAUNCH("reduce_cols", reduce_cols_kernel, dim3(nblocks64((long)h * h)), dim3(NT), 0, st,
-                   P(L.wpart), L.nsplit, (long)h * h, (long)h * h, d_params + woff[l]);
-        GDX_LAUNCH("reduce_cols", reduce_cols_kernel, dim3(nblocks64(h)), dim3(NT), 0, st, P(L.cpart),
-                   L.nsplit, (long)h, (long)h, d_params + boff[l]);
-      }
-      // delta for layer l's input: (Gcur W_l^T) * elu'(Hin)
-      if (tiles)
-        launch_tile<double, false, true, EPI_ELU_BWD>(st, "gemm_f64_bwd_data", Np, h, 1, h, Gcur, h, Wl, h, Gnext,
-                                                      h, nullptr, Hin, h, nullptr);
-      else
-        launch_gemm<128, 128, false, true, EPI_ELU_BWD>(st, Np, h, h, Gcur, h, Wl, h, Gnext, h, nullptr, Hin,
-                                                        h, 1, h, nullptr);
-      std::swap(Gcur, Gnext);
-    }
-    // first layer: DX = G W0^T (skinny), dW0 and db0 = X16^T G (rows 0..nc-1 and the ones row nc)
-    GDX_LAUNCH("layer0_bwd", (skinny_out16_kernel<double, false>), dim3(Np / 64), dim3(NT), 0, st, Np, h, Gcur,
-               h, W016, P(L.DX), d.mlp_nc, d.mlp_nc, nullptr, 0, nullptr, nullptr, 0, nullptr);
-    if (d_params) {
-      GDX_LAUNCH("skinny_wgrad", skinny_wgrad_kernel<double>, dim3(L.nwblk), dim3(NT), 0, st, Np, L.wrows, P(L.X),
-                 Gcur, h, P(L.wpart));
-      // rows 0..nc of the 16 x h result are W0 (nc x h) followed by b0: contiguous in params
-      GDX_LAUNCH("reduce_cols", reduce_cols_kernel, dim3(nblocks64((long)(d.mlp_nc + 1) * h)), dim3(NT), 0, st,
-                 P(L.wpart), L.nwblk, 16L * h, (long)(d.mlp_nc + 1) * h, d_params + woff[0]);
-    }
-  } else {
-    GDX_LAUNCH("head_seed", head_seed_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, nullptr, P(L.E), weights, nd,
-               d.npts, Np, g_exc, 0, P(L.GE), nullptr, nullptr);
-  }
-  if (d_rdm1) {
-    GDX_LAUNCH("point_bwd", point_bwd_kernel, dim3(L.npt_blocks), dim3(NT), 0, st, P(L.dens_fin), P(L.GE),
-                       neural ? P(L.DX) : nullptr, P(L.X), d.npts, Np, P(L.vpt));
-    const int nt = (nP + VX_T - 1) / VX_T;
-    // B3LYP on equal spin densities has equal spin potentials: form V once
-    const int nsv = (!neural && d.spin_restricted) ? 1 : 2;
-    if (nt == 1)
-      GDX_LAUNCH("vxc_f64", (vxc_ao_kernel<double, true>), dim3(1, L.nchunk, nsv), dim3(NT), 0, st, ao, Np, nP,
-                 P(L.vpt), L.chunk, P(L.vpart));
-    else
-      GDX_LAUNCH("vxc_f64", (vxc_ao_kernel<double, false>), dim3(nt * nt, L.nchunk, nsv), dim3(NT), 0, st, ao, Np,
-                 nP, P(L.vpt), L.chunk, P(L.vpart));
-    GDX_LAUNCH("reduce_vxc", reduce_vxc_kernel, dim3(nblocks(2L * d.nao * d.nao)), dim3(NT), 0, st, P(L.vpart),
-               L.nchunk, nsv, nP, d.nao, d_rdm1);
-  }
-  GDX_HIP_CHECK(hipGetLastError());
-  return GDX_OK;
+  Ctx c{*desc, make_layout(*desc), (char*)workspace, (hipStream_t)stream};
+  if (desc->precision == GDX_FP64)
+    return backward_impl<double>(c, (const double*)ao, weights, params, g_exc, d_rdm1, d_params);
+  return backward_impl<float>(c, (const float*)ao, weights, params, g_exc, d_rdm1, d_params);
 }
 
 }  // extern "C"

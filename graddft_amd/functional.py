@@ -75,6 +75,7 @@ class XCPlan:
         nbytes = ctypes.c_size_t(0)
         _lib.check(lib.gdx_xc_workspace_size(ctypes.byref(d), ctypes.byref(nbytes)), "gdx_xc_workspace_size")
         self.workspace = torch.empty(int(nbytes.value), dtype=torch.uint8, device=molecule.ao_planes.device)
+        self.precision = precision
         self.molecule = molecule
         self.generation = 0
         self.fwd_generation = -1
@@ -83,10 +84,22 @@ class XCPlan:
     def _ptr(t: Optional[torch.Tensor]) -> int:
         return 0 if t is None else t.data_ptr()
 
+    def _ao(self) -> torch.Tensor:
+        """AO planes in the element type of the path (FP32 copy cached on the molecule)."""
+        m = self.molecule
+        if self.precision == "fp64":
+            return m.ao_planes
+        key = ("ao_planes_f32", m.ao_planes.data_ptr())
+        a = m.cache.get(key)
+        if a is None:
+            a = m.ao_planes.to(torch.float32)
+            m.cache[key] = a
+        return a
+
     def forward(self, rdm1: torch.Tensor, params: Optional[torch.Tensor], exc: torch.Tensor) -> None:
         m = self.molecule
         st = torch.cuda.current_stream(m.ao_planes.device).cuda_stream
-        rc = _lib.load().gdx_xc_forward(st, ctypes.byref(self.desc), m.ao_planes.data_ptr(),
+        rc = _lib.load().gdx_xc_forward(st, ctypes.byref(self.desc), self._ao().data_ptr(),
                                         m.grid.weights_pad.data_ptr(), rdm1.data_ptr(),
                                         self._ptr(params), exc.data_ptr(), self.workspace.data_ptr())
         _lib.check(rc, "gdx_xc_forward")
@@ -98,7 +111,7 @@ class XCPlan:
         m = self.molecule
         out = torch.empty((m.npts_pad, 2, 6), dtype=torch.float64, device=rdm1.device)
         st = torch.cuda.current_stream(m.ao_planes.device).cuda_stream
-        rc = _lib.load().gdx_xc_densities(st, ctypes.byref(self.desc), m.ao_planes.data_ptr(),
+        rc = _lib.load().gdx_xc_densities(st, ctypes.byref(self.desc), self._ao().data_ptr(),
                                           rdm1.contiguous().data_ptr(), out.data_ptr(),
                                           self.workspace.data_ptr())
         _lib.check(rc, "gdx_xc_densities")
@@ -110,7 +123,7 @@ class XCPlan:
             raise RuntimeError("XC workspace was overwritten by another forward before this backward")
         m = self.molecule
         st = torch.cuda.current_stream(m.ao_planes.device).cuda_stream
-        rc = _lib.load().gdx_xc_backward(st, ctypes.byref(self.desc), m.ao_planes.data_ptr(),
+        rc = _lib.load().gdx_xc_backward(st, ctypes.byref(self.desc), self._ao().data_ptr(),
                                          m.grid.weights_pad.data_ptr(), rdm1.data_ptr(),
                                          self._ptr(params), g_exc.data_ptr(), self._ptr(d_rdm1),
                                          self._ptr(d_params), self.workspace.data_ptr())

@@ -27,6 +27,8 @@ enum {
   GDX_ERR_UNSUPPORTED = -3  /* valid request this build does not implement */
 };
 
+/* Element type of the matrix products (densities, V_xc, MLP layers).  Per-point functional
+ * math, contractions and every reduction over grid points are FP64 in all modes. */
 enum { GDX_FP64 = 0, GDX_FP32 = 1, GDX_BF16X3 = 2 };
 
 /* Functional kinds evaluated on the grid.
@@ -74,24 +76,25 @@ int gdx_xc_workspace_size(const gdx_xc_desc* desc, size_t* bytes);
  * grad_dft/functional.py:219-253 with densities and coefficient inputs formed from rdm1
  * as in compute_densities / compute_coefficient_inputs (:160-217) and
  * grad_dft/molecule.py:388-502.
- * ao: gdx_ao_eval output; weights (npts_pad) grid weights, zero past npts;
+ * ao: gdx_ao_eval output in the element type of desc->precision (double planes for
+ * GDX_FP64, float planes for GDX_FP32); weights (npts_pad) grid weights, zero past npts;
  * rdm1 (2, nao, nao); params: flat MLP parameters [W0 (nc,h), b0 (h), W1 (h,h), b1, ...,
  * W_last (h,nd), b_last] (NULL for GDX_FUNC_B3LYP).  exc: one device double.
  * The workspace keeps what gdx_xc_backward needs; do not touch it in between. */
-int gdx_xc_forward(void* stream, const gdx_xc_desc* desc, const double* ao, const double* weights,
+int gdx_xc_forward(void* stream, const gdx_xc_desc* desc, const void* ao, const double* weights,
                    const double* rdm1, const double* params, double* exc, void* workspace);
 
 /* Grid densities per spin, (npts_pad, 2, 6) doubles: rho, d rho/dx, d rho/dy, d rho/dz,
  * tau, laplacian -- Molecule.density / grad_density / kinetic_density / lapl_density of
  * grad_dft/molecule.py:388-502 in one pass.  Uses (and overwrites) the workspace. */
-int gdx_xc_densities(void* stream, const gdx_xc_desc* desc, const double* ao, const double* rdm1,
+int gdx_xc_densities(void* stream, const gdx_xc_desc* desc, const void* ao, const double* rdm1,
                      double* dens, void* workspace);
 
 /* Vector-Jacobian product of gdx_xc_forward for the cotangent g_exc (one device double):
  * d_rdm1 (2, nao, nao) = g * dE_xc/dD_s -- the unsymmetrised Fock XC term of
  * grad_dft/train.py:86-121 (value_and_grad w.r.t. rdm1) -- and d_params (same layout as
  * params) = g * dE_xc/dtheta.  Either output may be NULL to skip it. */
-int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const double* ao,
+int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const void* ao,
                     const double* weights, const double* rdm1, const double* params,
                     const double* g_exc, double* d_rdm1, double* d_params, void* workspace);
 
