@@ -8,6 +8,10 @@ from .molecule import Grid, Molecule
 from .functional import Functional, NeuralFunctional, MLPSpec, XCPlan, XCEnergyFn
 from .popular_functionals import B3LYP, B3LYP_GRID
 from .interface.builder import molecule_from_geometry
+from .train import energy_predictor
+from .evaluate import JittableDiis, diff_scf_loop
+from .eigenproblem import safe_eigh_vjp, safe_fock_solver, safe_general_eigh
 
 __all__ = ["Grid", "Molecule", "Functional", "NeuralFunctional", "MLPSpec", "XCPlan",
-           "XCEnergyFn", "B3LYP", "B3LYP_GRID", "molecule_from_geometry"]
+           "XCEnergyFn", "B3LYP", "B3LYP_GRID", "molecule_from_geometry", "energy_predictor",
+           "JittableDiis", "diff_scf_loop", "safe_eigh_vjp", "safe_fock_solver", "safe_general_eigh"]

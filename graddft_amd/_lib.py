@@ -41,6 +41,15 @@ SIGNATURES = {
     "gdx_ao_eval": (ctypes.c_int, [_P, _I, _P, _P, _P, _P, _I, _I, _P, _I, _I, _P]),
     "gdx_xc_workspace_size": (ctypes.c_int, [ctypes.POINTER(XCDesc), ctypes.POINTER(_Z)]),
     "gdx_xc_forward": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P, _P, _P]),
+    "gdx_coulomb_workspace_size": (ctypes.c_int, [_I, ctypes.POINTER(_Z)]),
+    "gdx_coulomb_packed": (ctypes.c_int, [_P, _I, _P, _P, _P, _P]),
+    "gdx_nonxc_energy": (ctypes.c_int, [_P, _I, _P, _P, _P, _P, ctypes.c_double, ctypes.c_double, _P]),
+    "gdx_fock_assemble": (ctypes.c_int, [_P, _I, _P, _P, _P, _P, ctypes.c_double, _P]),
+    "gdx_exchange_packed": (ctypes.c_int, [_P, _I, _I, _P, _P, _P]),
+    "gdx_cholesky_inv": (ctypes.c_int, [_P, _I, _P, _P, _P, _P]),
+    "gdx_eigh_workspace_size": (ctypes.c_int, [_I, _I, ctypes.POINTER(_Z)]),
+    "gdx_eigh_batched": (ctypes.c_int, [_P, _I, _I, _P, _P, _P, _P, _I, ctypes.c_double, _P]),
+    "gdx_diis_solve": (ctypes.c_int, [_P, _I, _I, _P, _P]),
     "gdx_profile_begin": (ctypes.c_int, []),
     "gdx_profile_end": (ctypes.c_int, [_I, ctypes.c_char_p, _P, _P, _P]),
     "gdx_xc_densities": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P]),

@@ -1,0 +1,516 @@
+// SCF-side kernels: Coulomb matrix from the 8-fold packed ERI, exchange (set-up), Cholesky
+// of the overlap, batched symmetric eigensolver, DIIS linear solve, non-XC energy.
+//
+// Reference computations replaced:
+//   coulomb_potential / coulomb_energy / nonXC   grad_dft/molecule.py:694-811
+//   exchange of the initial density (the stale semi-numerical HF term of B3LYP,
+//     molecule.py:507-613 with chi from interface/pyscf.py:152-160) taken from the ERI
+//   safe_general_eigh: cholesky, inv, eigh        grad_dft/utils/eigenproblem.py:110-149
+//   JittableDiis.cdiis_minimize: inv(B) @ e_0     grad_dft/evaluate.py:1162-1205
+//
+// Packed ERI order (shared with the host integral library): pair index I = i(i+1)/2 + j
+// (i >= j), element index I(I+1)/2 + K (I >= K).
+#include "gdx_common.h"
+#include "gdx_profile.h"
+
+namespace gdx {
+namespace {
+
+constexpr int NT = 256;
+
+// ------------------------------------------------------------- Coulomb (J) ----------
+// J_ij = sum_kl (ij|kl) D_kl = sum_{K} M_IK Dt_K with Dt_K = D_kl + D_lk (k > l), D_kk,
+// M the symmetric P x P matrix whose lower triangle is the packed ERI.  Tiles of
+// JT x JT elements of the lower triangle: each block forms row sums (M Dt over its
+// columns) and column sums (M^T Dt over its strictly-lower rows); partial vectors are
+// added by coulomb_finish.  Every packed element is read exactly once.
+constexpr int JT = 256;
+
+__global__ void pack_density_kernel(const double* __restrict__ rdm1, int n, double* __restrict__ Dt) {
+  const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long P = (long)n * (n + 1) / 2;
+  if (I >= P) return;
+  const int i = (int)((sqrt(8.0 * (double)I + 1.0) - 1.0) / 2.0);
+  int ii = i;
+  while ((long)ii * (ii + 1) / 2 > I) --ii;
+  while ((long)(ii + 1) * (ii + 2) / 2 <= I) ++ii;
+  const int j = (int)(I - (long)ii * (ii + 1) / 2);
+  const double* D0 = rdm1;
+  const double* D1 = rdm1 + (size_t)n * n;
+  const double dij = D0[(size_t)ii * n + j] + D1[(size_t)ii * n + j];
+  const double dji = D0[(size_t)j * n + ii] + D1[(size_t)j * n + ii];
+  Dt[I] = ii == j ? dij : dij + dji;
+}
+
+__global__ __launch_bounds__(NT) void coulomb_tile_kernel(const double* __restrict__ eri, long P,
+                                                          const double* __restrict__ Dt,
+                                                          double* __restrict__ prow, double* __restrict__ pcol) {
+  __shared__ double cs[16][JT + 1];
+  // triangular tile index -> (bi, bk), bk <= bi
+  const long x = blockIdx.x;
+  long bi = (long)((sqrt(8.0 * (double)x + 1.0) - 1.0) / 2.0);
+  while (bi * (bi + 1) / 2 > x) --bi;
+  while ((bi + 1) * (bi + 2) / 2 <= x) ++bi;
+  const long bk = x - bi * (bi + 1) / 2;
+  const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+  const long I0 = bi * JT, K0 = bk * JT;
+  double dk[16], csum[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const long K = K0 + j * 16 + tx;
+    dk[j] = K < P ? Dt[K] : 0.0;
+    csum[j] = 0.0;
+  }
+  double rsum[16];
+#pragma unroll
+  for (int a = 0; a < 16; ++a) {
+    const long I = I0 + ty + 16 * a;
+    double r = 0.0;
+    if (I < P) {
+      const double di = Dt[I];
+      const double* row = eri + I * (I + 1) / 2;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const long K = K0 + j * 16 + tx;
+        if (K <= I) {
+          const double m = row[K];
+          r += m * dk[j];
+          if (K < I) csum[j] += m * di;
+        }
+      }
+    }
+    rsum[a] = r;
+  }
+  // row sums over the 16 threads of this ty (consecutive lanes)
+#pragma unroll
+  for (int a = 0; a < 16; ++a) {
+    const double r = sum16(rsum[a]);
+    const long I = I0 + ty + 16 * a;
+    if (tx == 0 && I < P) prow[bk * P + I] = r;
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j) cs[ty][j * 16 + tx] = csum[j];
+  __syncthreads();
+  {
+    const long K = K0 + tid;
+    double c = 0.0;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) c += cs[t][tid];
+    if (K < P) pcol[bi * P + K] = c;
+  }
+}
+
+// J~_I = sum_{bk <= I/JT} prow[bk][I] + sum_{bi >= I/JT} pcol[bi][I], unpacked to J (n x n)
+__global__ void coulomb_finish_kernel(const double* __restrict__ prow, const double* __restrict__ pcol, long P,
+                                      int nb, int n, double* __restrict__ J) {
+  const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= P) return;
+  const int t = (int)(I / JT);
+  double acc = 0.0;
+  for (int b = 0; b <= t; ++b) acc += prow[b * P + I];
+  for (int b = t; b < nb; ++b) acc += pcol[b * P + I];
+  int i = (int)((sqrt(8.0 * (double)I + 1.0) - 1.0) / 2.0);
+  while ((long)i * (i + 1) / 2 > I) --i;
+  while ((long)(i + 1) * (i + 2) / 2 <= I) ++i;
+  const int j = (int)(I - (long)i * (i + 1) / 2);
+  J[(size_t)i * n + j] = acc;
+  J[(size_t)j * n + i] = acc;
+}
+
+// E = e_nuc + sum D h + 1/2 sum D J (D = D_up + D_dn), plus, when K0 is given, the stale
+// exact-exchange energy -(a0/2) sum_s sum D_s K0_s (single block)
+__global__ void nonxc_energy_kernel(const double* __restrict__ rdm1, const double* __restrict__ h1e,
+                                    const double* __restrict__ J, const double* __restrict__ K0, double a0, int n,
+                                    double e_nuc, double* __restrict__ out) {
+  __shared__ double red[NT / 64];
+  double acc = 0.0;
+  const long nn = (long)n * n;
+  for (long i = threadIdx.x; i < nn; i += blockDim.x) {
+    const double D = rdm1[i] + rdm1[nn + i];
+    acc += D * h1e[i] + 0.5 * D * J[i];
+    if (K0 != nullptr) acc += -0.5 * a0 * (rdm1[i] * K0[i] + rdm1[nn + i] * K0[nn + i]);
+  }
+  acc = warp_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = e_nuc + red[0] + red[1] + red[2] + red[3];
+}
+
+// Fock assembly of predict (grad_dft/train.py:148-215), per spin s:
+//   F = clip(h + J + V_s); F = clip((F + F^T)/2); [F = clip(F + X_s + X_s^T), X_s = -(a0/2) K0_s];
+//   F = clip(F), clip(x) = x if |x| > 1e-30 else 0.  One thread per (s, i, j).
+__device__ __forceinline__ double clip30s(double x) { return fabs(x) > 1e-30 ? x : 0.0; }
+
+__global__ void fock_assemble_kernel(const double* __restrict__ h1e, const double* __restrict__ J,
+                                     const double* __restrict__ V, const double* __restrict__ K0, double a0, int n,
+                                     double* __restrict__ F) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nn = (long)n * n;
+  if (e >= 2 * nn) return;
+  const int s = (int)(e / nn);
+  const int i = (int)((e % nn) / n), j = (int)(e % n);
+  const long ij = (long)i * n + j, ji = (long)j * n + i;
+  const double fij = clip30s(h1e[ij] + J[ij] + V[s * nn + ij]);
+  const double fji = clip30s(h1e[ji] + J[ji] + V[s * nn + ji]);
+  double f = clip30s(0.5 * (fij + fji));
+  if (K0 != nullptr) {
+    const double x = -0.5 * a0 * K0[s * nn + ij], xt = -0.5 * a0 * K0[s * nn + ji];
+    f = clip30s(f + x + xt);
+  }
+  F[e] = clip30s(f);
+}
+
+// ------------------------------------------------------------- exchange (K) ---------
+// K_s[a][c] += (ab|cd) D_s[b][d] for every distinct index permutation of each unique
+// packed integral (atomic scatter; used once per molecule for the initial density).
+__device__ __forceinline__ void unpair(long I, int& i, int& j) {
+  i = (int)((sqrt(8.0 * (double)I + 1.0) - 1.0) / 2.0);
+  while ((long)i * (i + 1) / 2 > I) --i;
+  while ((long)(i + 1) * (i + 2) / 2 <= I) ++i;
+  j = (int)(I - (long)i * (i + 1) / 2);
+}
+
+__global__ void exchange_scatter_kernel(const double* __restrict__ eri, long P, int n, int ns,
+                                        const double* __restrict__ D, double* __restrict__ K) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = P * (P + 1) / 2;
+  if (idx >= total) return;
+  long I;
+  {
+    long t = (long)((sqrt(8.0 * (double)idx + 1.0) - 1.0) / 2.0);
+    while (t * (t + 1) / 2 > idx) --t;
+    while ((t + 1) * (t + 2) / 2 <= idx) ++t;
+    I = t;
+  }
+  const long Kp = idx - I * (I + 1) / 2;
+  const double v = eri[idx];
+  if (v == 0.0) return;
+  int i, j, k, l;
+  unpair(I, i, j);
+  unpair(Kp, k, l);
+  int perm[8][4] = {{i, j, k, l}, {j, i, k, l}, {i, j, l, k}, {j, i, l, k},
+                    {k, l, i, j}, {l, k, i, j}, {k, l, j, i}, {l, k, j, i}};
+  for (int p = 0; p < 8; ++p) {
+    bool dup = false;
+    for (int q = 0; q < p; ++q)
+      if (perm[q][0] == perm[p][0] && perm[q][1] == perm[p][1] && perm[q][2] == perm[p][2] &&
+          perm[q][3] == perm[p][3])
+        dup = true;
+    if (dup) continue;
+    const int a = perm[p][0], b = perm[p][1], c = perm[p][2], d = perm[p][3];
+    for (int s = 0; s < ns; ++s)
+      atomicAdd(&K[((size_t)s * n + a) * n + c], v * D[((size_t)s * n + b) * n + d]);
+  }
+}
+
+__global__ void zero_kernel(double* __restrict__ p, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// ------------------------------------------------------------- Cholesky ---------------
+// S = L L^T (lower) and Linv = L^-1, one workgroup; n sequential column steps.
+__global__ __launch_bounds__(1024) void cholesky_inv_kernel(const double* __restrict__ S, int n,
+                                                            double* __restrict__ L, double* __restrict__ Linv,
+                                                            int* __restrict__ status) {
+  const int tid = threadIdx.x;
+  for (long e = tid; e < (long)n * n; e += blockDim.x) L[e] = 0.0;
+  __syncthreads();
+  __shared__ double diag;
+  __shared__ int bad;
+  if (tid == 0) bad = 0;
+  for (int j = 0; j < n; ++j) {
+    if (tid == 0) {
+      double s = S[(size_t)j * n + j];
+      for (int k = 0; k < j; ++k) s -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+      if (!(s > 0.0)) bad = 1;
+      diag = sqrt(s > 0.0 ? s : 1.0);
+      L[(size_t)j * n + j] = diag;
+    }
+    __syncthreads();
+    for (int i = j + 1 + tid; i < n; i += blockDim.x) {
+      double s = S[(size_t)i * n + j];
+      for (int k = 0; k < j; ++k) s -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
+      L[(size_t)i * n + j] = s / diag;
+    }
+    __syncthreads();
+  }
+  // Linv column c by forward substitution (one thread per column)
+  for (int c = tid; c < n; c += blockDim.x) {
+    for (int i = 0; i < n; ++i) {
+      double s = (i == c) ? 1.0 : 0.0;
+      for (int k = c; k < i; ++k) s -= L[(size_t)i * n + k] * Linv[(size_t)k * n + c];
+      Linv[(size_t)i * n + c] = i < c ? 0.0 : s / L[(size_t)i * n + i];
+    }
+  }
+  if (tid == 0) *status = bad;
+}
+
+// ------------------------------------------------------------- eigensolver --------------
+// Parallel cyclic Jacobi (round-robin pair schedule), one workgroup per symmetric matrix.
+// Each round rotates n/2 disjoint (p, q) pairs: rotation angles from the current 2x2
+// blocks, then all row updates, then all column updates (A <- J^T A J, V <- V J).  Sweeps
+// stop when the off-diagonal Frobenius norm is below tol * ||A||_F.  Eigenvalues are
+// returned ascending with eigenvectors as columns of V (the order of jnp.linalg.eigh).
+// A and V live in LDS when 2 n^2 doubles fit (SMEM), otherwise in the workspace.
+constexpr int EIG_THREADS = 1024;
+
+__device__ __forceinline__ int rr_partner(int r, int k, int m, int* p, int* q) {
+  // round r of the round-robin tournament on m (even) players: pair k
+  int a, b;
+  if (k == 0) { a = m - 1; b = r; }
+  else { a = (r + k) % (m - 1); b = (r - k + (m - 1)) % (m - 1); }
+  *p = a < b ? a : b;
+  *q = a < b ? b : a;
+  return 0;
+}
+
+template <bool SMEM>
+__global__ __launch_bounds__(EIG_THREADS) void jacobi_eigh_kernel(const double* __restrict__ Ain, int n,
+                                                                  double* __restrict__ w_out,
+                                                                  double* __restrict__ V_out,
+                                                                  double* __restrict__ gws, int max_sweeps,
+                                                                  double tol, int* __restrict__ sweeps_out) {
+  extern __shared__ double smem[];
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const size_t nn = (size_t)n * n;
+  double* A = SMEM ? smem : gws + (size_t)b * 2 * nn;
+  double* V = A + nn;
+  __shared__ double cs_c[512], cs_s[512];
+  __shared__ int pp[512], qq[512];
+  __shared__ double red[EIG_THREADS / 64];
+  __shared__ int done;
+  __shared__ double anorm;
+  const double* Asrc = Ain + (size_t)b * nn;
+  for (size_t e = tid; e < nn; e += blockDim.x) {
+    A[e] = Asrc[e];
+    V[e] = (e / n == e % n) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const int m = n + (n & 1);  // players incl. a dummy when n is odd
+  const int npair = m / 2;
+  int sweep = 0;
+  for (; sweep <= max_sweeps; ++sweep) {
+    // convergence test
+    double off = 0.0, tot = 0.0;
+    for (size_t e = tid; e < nn; e += blockDim.x) {
+      const double v = A[e] * A[e];
+      tot += v;
+      if (e / n != e % n) off += v;
+    }
+    off = warp_sum(off);
+    tot = warp_sum(tot);
+    if ((tid & 63) == 0) red[tid >> 6] = off;
+    __syncthreads();
+    if (tid == 0) {
+      double o = 0.0;
+      for (int i = 0; i < EIG_THREADS / 64; ++i) o += red[i];
+      red[0] = o;
+    }
+    __syncthreads();
+    const double offs = red[0];
+    __syncthreads();
+    if ((tid & 63) == 0) red[tid >> 6] = tot;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int i = 0; i < EIG_THREADS / 64; ++i) t += red[i];
+      anorm = t;
+      done = (offs <= tol * tol * t) || sweep == max_sweeps;
+    }
+    __syncthreads();
+    if (done) break;
+    for (int r = 0; r < m - 1; ++r) {
+      // rotation parameters
+      for (int k = tid; k < npair; k += blockDim.x) {
+        int p, q;
+        rr_partner(r, k, m, &p, &q);
+        double c = 1.0, s = 0.0;
+        if (q < n) {
+          const double apq = A[(size_t)p * n + q];
+          if (apq != 0.0) {
+            const double app = A[(size_t)p * n + p], aqq = A[(size_t)q * n + q];
+            const double theta = (aqq - app) / (2.0 * apq);
+            const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            c = 1.0 / sqrt(t * t + 1.0);
+            s = t * c;
+          }
+        }
+        cs_c[k] = c; cs_s[k] = s; pp[k] = p; qq[k] = q;
+      }
+      __syncthreads();
+      // rows: A <- J^T A
+      for (int e = tid; e < npair * n; e += blockDim.x) {
+        const int k = e / n, j = e % n;
+        const int p = pp[k], q = qq[k];
+        if (q >= n) continue;
+        const double c = cs_c[k], s = cs_s[k];
+        const double apj = A[(size_t)p * n + j], aqj = A[(size_t)q * n + j];
+        A[(size_t)p * n + j] = c * apj - s * aqj;
+        A[(size_t)q * n + j] = s * apj + c * aqj;
+      }
+      __syncthreads();
+      // columns: A <- A J, V <- V J
+      for (int e = tid; e < npair * n; e += blockDim.x) {
+        const int k = e / n, i = e % n;
+        const int p = pp[k], q = qq[k];
+        if (q >= n) continue;
+        const double c = cs_c[k], s = cs_s[k];
+        const double aip = A[(size_t)i * n + p], aiq = A[(size_t)i * n + q];
+        A[(size_t)i * n + p] = c * aip - s * aiq;
+        A[(size_t)i * n + q] = s * aip + c * aiq;
+        const double vip = V[(size_t)i * n + p], viq = V[(size_t)i * n + q];
+        V[(size_t)i * n + p] = c * vip - s * viq;
+        V[(size_t)i * n + q] = s * vip + c * viq;
+      }
+      __syncthreads();
+    }
+  }
+  // ascending order: rank of each diagonal element (ties by index)
+  double* w = w_out + (size_t)b * n;
+  double* Vo = V_out + (size_t)b * nn;
+  for (int i = tid; i < n; i += blockDim.x) {
+    const double di = A[(size_t)i * n + i];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) {
+      const double dj = A[(size_t)j * n + j];
+      rank += (dj < di) || (dj == di && j < i);
+    }
+    w[rank] = di;
+    for (int row = 0; row < n; ++row) Vo[(size_t)row * n + rank] = V[(size_t)row * n + i];
+  }
+  if (tid == 0 && sweeps_out) sweeps_out[b] = sweep;
+  (void)anorm;
+}
+
+// ------------------------------------------------------------- DIIS solve ----------
+// x_s = B_s^-1 e_0 for each spin by Gaussian elimination with partial pivoting
+// (B_s is (m+1) x (m+1), m <= 31); one thread per spin.
+__global__ void diis_solve_kernel(const double* __restrict__ B, int m1, int nspin, double* __restrict__ x) {
+  const int s = threadIdx.x;
+  if (s >= nspin) return;
+  double a[32][33];
+  for (int i = 0; i < m1; ++i) {
+    for (int j = 0; j < m1; ++j) a[i][j] = B[((size_t)s * m1 + i) * m1 + j];
+    a[i][m1] = i == 0 ? 1.0 : 0.0;
+  }
+  for (int c = 0; c < m1; ++c) {
+    int piv = c;
+    for (int i = c + 1; i < m1; ++i)
+      if (fabs(a[i][c]) > fabs(a[piv][c])) piv = i;
+    if (piv != c)
+      for (int j = 0; j <= m1; ++j) { const double t = a[c][j]; a[c][j] = a[piv][j]; a[piv][j] = t; }
+    const double d = a[c][c];
+    for (int i = c + 1; i < m1; ++i) {
+      const double f = a[i][c] / d;
+      for (int j = c; j <= m1; ++j) a[i][j] -= f * a[c][j];
+    }
+  }
+  for (int i = m1 - 1; i >= 0; --i) {
+    double t = a[i][m1];
+    for (int j = i + 1; j < m1; ++j) t -= a[i][j] * x[(size_t)s * m1 + j];
+    x[(size_t)s * m1 + i] = t / a[i][i];
+  }
+}
+
+inline unsigned nblk(long n) { return (unsigned)((n + NT - 1) / NT); }
+
+}  // namespace
+}  // namespace gdx
+
+using namespace gdx;
+
+extern "C" {
+
+int gdx_coulomb_workspace_size(int32_t n, size_t* bytes) {
+  GDX_REQUIRE(n > 0 && bytes, "gdx_coulomb_workspace_size: bad arguments");
+  const long P = (long)n * (n + 1) / 2;
+  const long nb = (P + JT - 1) / JT;
+  *bytes = (size_t)(P + 2 * nb * P) * sizeof(double);
+  return GDX_OK;
+}
+
+int gdx_coulomb_packed(void* stream, int32_t n, const double* eri, const double* rdm1, double* J, void* workspace) {
+  GDX_REQUIRE(n > 0 && eri && rdm1 && J && workspace, "gdx_coulomb_packed: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const long P = (long)n * (n + 1) / 2;
+  const long nb = (P + JT - 1) / JT;
+  double* Dt = (double*)workspace;
+  double* prow = Dt + P;
+  double* pcol = prow + nb * P;
+  GDX_LAUNCH("pack_density", pack_density_kernel, dim3(nblk(P)), dim3(NT), 0, st, rdm1, n, Dt);
+  GDX_LAUNCH("coulomb_tile", coulomb_tile_kernel, dim3((unsigned)(nb * (nb + 1) / 2)), dim3(NT), 0, st, eri, P, Dt,
+             prow, pcol);
+  GDX_LAUNCH("coulomb_finish", coulomb_finish_kernel, dim3(nblk(P)), dim3(NT), 0, st, prow, pcol, P, (int)nb, n, J);
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+int gdx_nonxc_energy(void* stream, int32_t n, const double* rdm1, const double* h1e, const double* J,
+                     const double* K0, double a0, double e_nuc, double* out) {
+  GDX_REQUIRE(n > 0 && rdm1 && h1e && J && out, "gdx_nonxc_energy: bad arguments");
+  GDX_LAUNCH("nonxc_energy", nonxc_energy_kernel, dim3(1), dim3(NT), 0, (hipStream_t)stream, rdm1, h1e, J, K0, a0, n,
+             e_nuc, out);
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+int gdx_fock_assemble(void* stream, int32_t n, const double* h1e, const double* J, const double* vxc,
+                      const double* K0, double a0, double* fock) {
+  GDX_REQUIRE(n > 0 && h1e && J && vxc && fock, "gdx_fock_assemble: bad arguments");
+  GDX_LAUNCH("fock_assemble", fock_assemble_kernel, dim3(nblk(2L * n * n)), dim3(NT), 0, (hipStream_t)stream, h1e, J,
+             vxc, K0, a0, n, fock);
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+int gdx_exchange_packed(void* stream, int32_t n, int32_t nspin, const double* eri, const double* D, double* K) {
+  GDX_REQUIRE(n > 0 && nspin > 0 && eri && D && K, "gdx_exchange_packed: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const long P = (long)n * (n + 1) / 2;
+  const long total = P * (P + 1) / 2;
+  GDX_LAUNCH("zero", zero_kernel, dim3(nblk((long)nspin * n * n)), dim3(NT), 0, st, K, (long)nspin * n * n);
+  GDX_LAUNCH("exchange_scatter", exchange_scatter_kernel, dim3(nblk(total)), dim3(NT), 0, st, eri, P, n, nspin, D, K);
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+int gdx_cholesky_inv(void* stream, int32_t n, const double* S, double* L, double* Linv, int32_t* status) {
+  GDX_REQUIRE(n > 0 && S && L && Linv && status, "gdx_cholesky_inv: bad arguments");
+  GDX_LAUNCH("cholesky_inv", cholesky_inv_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, S, n, L, Linv, status);
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+int gdx_eigh_workspace_size(int32_t nbatch, int32_t n, size_t* bytes) {
+  GDX_REQUIRE(nbatch > 0 && n > 0 && bytes, "gdx_eigh_workspace_size: bad arguments");
+  *bytes = (size_t)nbatch * 2 * n * n * sizeof(double);
+  return GDX_OK;
+}
+
+int gdx_eigh_batched(void* stream, int32_t nbatch, int32_t n, const double* A, double* w, double* V,
+                     void* workspace, int32_t max_sweeps, double tol, int32_t* sweeps) {
+  GDX_REQUIRE(nbatch > 0 && n > 0 && n <= 1022 && A && w && V, "gdx_eigh_batched: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t smem = (size_t)2 * n * n * sizeof(double);
+  if (smem <= 64 * 1024) {
+    GDX_LAUNCH("jacobi_eigh", jacobi_eigh_kernel<true>, dim3(nbatch), dim3(EIG_THREADS), smem, st, A, n, w, V,
+               (double*)workspace, max_sweeps, tol, sweeps);
+  } else {
+    GDX_REQUIRE(workspace, "gdx_eigh_batched: workspace required for n > 64");
+    GDX_LAUNCH("jacobi_eigh", jacobi_eigh_kernel<false>, dim3(nbatch), dim3(EIG_THREADS), 0, st, A, n, w, V,
+               (double*)workspace, max_sweeps, tol, sweeps);
+  }
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+int gdx_diis_solve(void* stream, int32_t nspin, int32_t m1, const double* B, double* x) {
+  GDX_REQUIRE(nspin > 0 && nspin <= 64 && m1 > 0 && m1 <= 32 && B && x, "gdx_diis_solve: bad arguments");
+  GDX_LAUNCH("diis_solve", diis_solve_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, B, m1, nspin, x);
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+}  // extern "C"

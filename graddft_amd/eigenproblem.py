@@ -1,0 +1,81 @@
+"""Generalised symmetric eigenproblem on the device (``grad_dft/utils/eigenproblem.py``).
+
+``safe_general_eigh(A, B)``: L = chol(B), Linv = L^-1, C = Linv A Linv^T, eigh(C), Linv^T V
+(``eigenproblem.py:110-129``); ``safe_fock_solver`` applies it per spin (``:132-149``).
+Cholesky, the triangular inverse and the eigensolver are hand-written kernels
+(``gdx_cholesky_inv``, ``gdx_eigh_batched``); the two congruence products are plain GEMMs.
+The Lorentzian-broadened reverse mode of ``safe_eigh`` (``:54-103``) is :func:`safe_eigh_vjp`.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Tuple
+
+import torch
+
+from . import _lib
+
+DEGEN_TOL = 1e-6   # eigenproblem.py:23
+BROADENING = 1e-10  # eigenproblem.py:24
+JACOBI_TOL = 1e-14
+JACOBI_MAX_SWEEPS = 40
+
+
+def _stream(t):
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def cholesky_inv(S: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
+    S = S.contiguous()
+    n = S.shape[-1]
+    L = torch.empty_like(S)
+    Linv = torch.empty_like(S)
+    status = torch.zeros(1, dtype=torch.int32, device=S.device)
+    _lib.check(_lib.load().gdx_cholesky_inv(_stream(S), n, S.data_ptr(), L.data_ptr(), Linv.data_ptr(),
+                                            status.data_ptr()), "gdx_cholesky_inv")
+    return L, Linv
+
+
+def eigh(A: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Batched eigh of symmetric A (..., n, n): ascending eigenvalues, eigenvectors as columns."""
+    shape = A.shape
+    n = shape[-1]
+    Ab = A.reshape(-1, n, n).contiguous()
+    nb = Ab.shape[0]
+    w = torch.empty((nb, n), dtype=torch.float64, device=A.device)
+    V = torch.empty_like(Ab)
+    nbytes = ctypes.c_size_t(0)
+    lib = _lib.load()
+    _lib.check(lib.gdx_eigh_workspace_size(nb, n, ctypes.byref(nbytes)), "gdx_eigh_workspace_size")
+    ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=A.device)
+    _lib.check(lib.gdx_eigh_batched(_stream(A), nb, n, Ab.data_ptr(), w.data_ptr(), V.data_ptr(), ws.data_ptr(),
+                                    JACOBI_MAX_SWEEPS, JACOBI_TOL, 0), "gdx_eigh_batched")
+    return w.reshape(shape[:-1]), V.reshape(shape)
+
+
+def safe_general_eigh(A: torch.Tensor, B: torch.Tensor, Linv: torch.Tensor = None):
+    if Linv is None:
+        _, Linv = cholesky_inv(B)
+    C = Linv @ A @ Linv.transpose(-1, -2)
+    C = 0.5 * (C + C.transpose(-1, -2))
+    w, V = eigh(C)
+    return w, Linv.transpose(-1, -2) @ V
+
+
+def safe_fock_solver(fock: torch.Tensor, overlap: torch.Tensor, Linv: torch.Tensor = None):
+    """(mo_energy (2, n), mo_coeff (2, n, n)) of both spin Fock matrices."""
+    return safe_general_eigh(fock, overlap, Linv)
+
+
+def safe_eigh_vjp(evals: torch.Tensor, evecs: torch.Tensor, g_evals: torch.Tensor, g_evecs: torch.Tensor):
+    """Reverse mode of safe_eigh (eigenproblem.py:54-103): Lorentzian-broadened gaps for
+    near-degenerate pairs, symmetrised result."""
+    diff = evals[None, :] - evals[:, None]
+    degen = diff.abs() < DEGEN_TOL
+    regular = torch.nan_to_num(1.0 / diff)
+    broadened = diff / (diff * diff + BROADENING)
+    F = 0.5 * torch.where(degen, broadened, regular)
+    F.fill_diagonal_(0.0)
+    Vt = evecs.T
+    grad = torch.linalg.inv(Vt) @ (0.5 * torch.diag(g_evals) + F * (Vt @ g_evecs)) @ Vt
+    return grad + grad.T

@@ -1,0 +1,106 @@
+"""Self-consistent field driver (``grad_dft/evaluate.py:917-1205``).
+
+``diff_scf_loop(functional, cycles)(params, molecule) -> molecule`` runs a fixed number of
+CDIIS iterations exactly as the reference's jitted loop: predict on the input density, then
+per cycle DIIS extrapolation of the Fock matrix, generalised diagonalisation, aufbau
+occupations, new density, predict.  ``JittableDiis`` keeps the reference's buffer semantics
+(``evaluate.py:1111-1205``): slot ``cycle`` is written while ``cycle < max_diis``, the write at
+``cycle == max_diis`` is dropped (JAX's out-of-bounds scatter), later cycles shift the
+buffers; the bordered matrix has +1 borders for active slots and 1 on the diagonal of
+inactive ones, and the coefficients are B^-1 e_0 (hand-written solve, ``gdx_diis_solve``).
+The reference's extra diagonalisation after the loop is discarded by it and not computed.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Callable, List, Optional
+
+import torch
+
+from . import _lib
+from .eigenproblem import cholesky_inv, safe_fock_solver
+from .functional import Functional
+from .molecule import Molecule
+from .train import energy_predictor
+
+
+@dataclass
+class JittableDiis:
+    overlap_matrix: torch.Tensor
+    max_diis: int = 10
+
+    def zeros(self, n: int, device):
+        m = self.max_diis
+        z = lambda *s: torch.zeros(s, dtype=torch.float64, device=device)  # noqa: E731
+        return [z(m, 2, n, n), z(m, 2, n, n), z(m), z(m, 2, n, n)]
+
+    def update(self, rdm1, fock, energy, data, cycle: int):
+        dens, focks, energies, errs = data
+        fds = fock @ rdm1 @ self.overlap_matrix
+        err = fds - fds.transpose(1, 2)
+        if cycle > self.max_diis:
+            errs = torch.cat([errs[1:], err[None]])
+            dens = torch.cat([dens[1:], rdm1[None]])
+            focks = torch.cat([focks[1:], fock[None]])
+            energies = torch.cat([energies[1:], energy.reshape(1)])
+        elif cycle < self.max_diis:
+            errs = errs.clone(); errs[cycle] = err
+            dens = dens.clone(); dens[cycle] = rdm1
+            focks = focks.clone(); focks[cycle] = fock
+            energies = energies.clone(); energies[cycle] = energy
+        return [dens, focks, energies, errs]
+
+    def coefficients(self, errs: torch.Tensor, cycle: int) -> torch.Tensor:
+        m = errs.shape[0]
+        E = errs.reshape(m, 2, -1).transpose(0, 1)              # (2, m, n*n)
+        gram = E @ E.transpose(1, 2)                             # (2, m, m)
+        B = torch.zeros((2, m + 1, m + 1), dtype=torch.float64, device=errs.device)
+        B[:, 1:, 1:] = gram
+        active = torch.tensor([1.0 if i <= cycle else 0.0 for i in range(m)], dtype=torch.float64,
+                              device=errs.device)
+        B[:, 0, 1:] = active
+        B[:, 1:, 0] = active
+        idx = torch.arange(m, device=errs.device)
+        B[:, idx + 1, idx + 1] = torch.where(active > 0, gram[:, idx, idx], torch.ones_like(gram[:, idx, idx]))
+        x = torch.empty((2, m + 1), dtype=torch.float64, device=errs.device)
+        _lib.check(_lib.load().gdx_diis_solve(torch.cuda.current_stream(errs.device).cuda_stream, 2, m + 1,
+                                              B.contiguous().data_ptr(), x.data_ptr()), "gdx_diis_solve")
+        return x[:, 1:]
+
+    def run(self, rdm1, fock, energy, data, cycle: int):
+        data = self.update(rdm1, fock, energy, data, cycle)
+        x = self.coefficients(data[3], cycle)
+        F = torch.einsum("si,isjk->sjk", x, data[1])
+        return F, data
+
+
+def diff_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp64", **kwargs) -> Callable:
+    compute_energy = energy_predictor(functional, precision=precision, **kwargs)
+
+    def scf_iterator(params: Optional[torch.Tensor], molecule: Molecule) -> Molecule:
+        S = molecule.s1e
+        Linv = molecule.cache.get("Linv")
+        if Linv is None:
+            _, Linv = cholesky_inv(S)
+            molecule.cache["Linv"] = Linv
+        predicted_e, fock = compute_energy(params, molecule)
+        molecule = molecule.replace(fock=fock)
+        diis = JittableDiis(S, max_diis=10)
+        data = diis.zeros(molecule.nao, S.device)
+        nelecs = [int(round(float(v))) for v in molecule.mo_occ.sum(dim=1).tolist()]
+        n = molecule.nao
+        occ = torch.zeros((2, n), dtype=torch.float64, device=S.device)
+        for s in range(2):
+            occ[s, : nelecs[s]] = 1.0
+        for cycle in range(cycles):
+            F, data = diis.run(molecule.rdm1, molecule.fock, predicted_e, data, cycle)
+            mo_energy, mo_coeff = safe_fock_solver(F, S, Linv)
+            # eigenvalues ascend, so the aufbau occupation of get_occ is the first nelec_s
+            rdm1 = torch.einsum("sij,sj,skj->sik", mo_coeff, occ, mo_coeff)
+            molecule = molecule.replace(fock=F, mo_coeff=mo_coeff, mo_energy=mo_energy, mo_occ=occ,
+                                        rdm1=rdm1, restricted=molecule.restricted and functional.spin_symmetric)
+            predicted_e, fock = compute_energy(params, molecule)
+            molecule = molecule.replace(fock=fock)
+        return molecule.replace(energy=predicted_e)
+
+    return scf_iterator

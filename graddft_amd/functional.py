@@ -185,8 +185,9 @@ class Functional:
         return XCEnergyFn.apply(D, params, self.plan(molecule, precision))
 
     def energy(self, params: Optional[torch.Tensor], molecule: Molecule, precision: str = "fp64"):
-        """Total energy (functional.py:255-288): E_xc (+ nonXC when is_xc)."""
-        from .scf import non_xc_energy, exact_exchange_energy
+        """Total energy (functional.py:255-288): E_xc (+ nonXC when is_xc).  The exact-exchange
+        part of B3LYP (from the molecule's initial density) is added to E_xc."""
+        from .scf import exact_exchange_energy, non_xc_energy
         e = self.xc_energy(params, molecule, precision)
         if self.exact_exchange:
             e = e + exact_exchange_energy(molecule, molecule.rdm1, self.exact_exchange)

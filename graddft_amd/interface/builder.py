@@ -81,7 +81,7 @@ def molecule_from_geometry(symbols: Sequence[str], coords_bohr: np.ndarray, basi
     grid = Grid(coords=dev(gc), weights=dev(gw), weights_pad=dev(wpad))
     ao = eval_ao_device(bs, gc, npts_pad, nao_pad, device)
     eri = dev(int2e_packed(bs)) if with_eri else None
-    return Molecule(
+    mol = Molecule(
         grid=grid,
         atom_index=torch.as_tensor(Z, device=device),
         nuclear_pos=dev(coords_bohr),
@@ -101,3 +101,5 @@ def molecule_from_geometry(symbols: Sequence[str], coords_bohr: np.ndarray, basi
         grid_level=grid_level,
         name=name,
     )
+    mol.cache["rdm1_init"] = mol.rdm1
+    return mol

@@ -98,6 +98,51 @@ int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const void* ao,
                     const double* weights, const double* rdm1, const double* params,
                     const double* g_exc, double* d_rdm1, double* d_params, void* workspace);
 
+/* ---- SCF side ---------------------------------------------------------------------
+ * eri: 8-fold packed (ij|kl), pair index I = i(i+1)/2 + j (i >= j), element I(I+1)/2 + K
+ * (I >= K); every matrix row-major, rdm1 (2, n, n). */
+
+/* Coulomb matrix J (n, n) of D_up + D_dn: coulomb_potential(rdm1.sum(0), rep_tensor) of
+ * grad_dft/molecule.py:108-115, 788-811, from the packed ERI (each element read once).
+ * workspace: gdx_coulomb_workspace_size bytes. */
+int gdx_coulomb_workspace_size(int32_t n, size_t* bytes);
+int gdx_coulomb_packed(void* stream, int32_t n, const double* eri, const double* rdm1, double* J,
+                       void* workspace);
+
+/* Non-XC energy e_nuc + sum D h1e + 1/2 sum D J (D = D_up + D_dn) into one device double:
+ * nonXC of grad_dft/molecule.py:694-783.  With K0 (2, n, n) non-NULL the exact-exchange energy
+ * of B3LYP from the initial density, -(a0/2) sum_s sum D_s K0_s, is added (its grid form is
+ * HF_energy_density, molecule.py:507-541, integrated in xc_energy). */
+int gdx_nonxc_energy(void* stream, int32_t n, const double* rdm1, const double* h1e, const double* J,
+                     const double* K0, double a0, double e_nuc, double* out);
+
+/* Fock matrix of predict (grad_dft/train.py:148-215): per spin F = clip(h1e + J + vxc_s),
+ * symmetrise, clip, add the explicit exchange term X_s + X_s^T with X_s = -(a0/2) K0_s when
+ * K0 is non-NULL, clip; clip(x) = x if |x| > 1e-30 else 0.  vxc: gdx_xc_backward's d_rdm1. */
+int gdx_fock_assemble(void* stream, int32_t n, const double* h1e, const double* J, const double* vxc,
+                      const double* K0, double a0, double* fock);
+
+/* Exchange K_s[a][c] = sum_bd (ab|cd) D_s[b][d] for nspin densities (nspin, n, n): the
+ * exact-exchange term of B3LYP from the initial density (stands in for the semi-numerical
+ * chi term, grad_dft/molecule.py:507-613, interface/pyscf.py:1067-1126). */
+int gdx_exchange_packed(void* stream, int32_t n, int32_t nspin, const double* eri, const double* D, double* K);
+
+/* S = L L^T and Linv = L^-1 (the cholesky / inv of safe_general_eigh,
+ * grad_dft/utils/eigenproblem.py:125-126).  status: device int, 1 if S is not positive definite. */
+int gdx_cholesky_inv(void* stream, int32_t n, const double* S, double* L, double* Linv, int32_t* status);
+
+/* Batched symmetric eigensolver (jnp.linalg.eigh in safe_eigh, eigenproblem.py:26-38):
+ * A (nbatch, n, n) -> w (nbatch, n) ascending, V (nbatch, n, n) eigenvectors as columns.
+ * workspace: gdx_eigh_workspace_size bytes (used when n > 64).  sweeps: optional device
+ * int per matrix (Jacobi sweeps used). */
+int gdx_eigh_workspace_size(int32_t nbatch, int32_t n, size_t* bytes);
+int gdx_eigh_batched(void* stream, int32_t nbatch, int32_t n, const double* A, double* w, double* V,
+                     void* workspace, int32_t max_sweeps, double tol, int32_t* sweeps);
+
+/* x_s = B_s^-1 e_0 for nspin bordered DIIS matrices B (nspin, m1, m1), m1 <= 32:
+ * JittableDiis.cdiis_minimize, grad_dft/evaluate.py:1198-1205. */
+int gdx_diis_solve(void* stream, int32_t nspin, int32_t m1, const double* B, double* x);
+
 /* Kernel timing (no counterpart in the reference): while on, every kernel this library
  * enqueues is bracketed by HIP events on its own stream.  gdx_profile_end waits for them and
  * returns per-kernel-name total milliseconds and launch counts (names: 64 chars each). */
