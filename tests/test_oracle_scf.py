@@ -1,0 +1,120 @@
+"""CPU pins of the SCF oracle (oracle/scf.py): J/K against brute-force loops, the
+JittableDiis buffer semantics of grad_dft/evaluate.py:1111-1205 on hand-built inputs,
+the generalised eigenproblem, aufbau occupations, and an oracle diff_scf_loop that reaches
+a self-consistent, idempotent density."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import scf as O
+
+
+def test_coulomb_exchange_brute_force():
+    rng = np.random.default_rng(0)
+    n = 4
+    eri = rng.normal(size=(n, n, n, n))
+    eri = eri + eri.transpose(1, 0, 2, 3)
+    eri = eri + eri.transpose(0, 1, 3, 2)
+    eri = eri + eri.transpose(2, 3, 0, 1)
+    D = rng.normal(size=(n, n))
+    J = O.coulomb_potential(D, eri)
+    K = O.exchange_potential(D, eri)
+    for p in range(n):
+        for q in range(n):
+            assert J[p, q] == pytest.approx(sum(eri[p, q, r, t] * D[r, t] for r in range(n) for t in range(n)))
+            assert K[p, q] == pytest.approx(sum(eri[p, b, q, d] * D[b, d] for b in range(n) for d in range(n)))
+
+
+def _fake_data(n, cycles, rng):
+    S = np.eye(n) + 0.1 * np.diag(np.ones(n - 1), 1) + 0.1 * np.diag(np.ones(n - 1), -1)
+    diis = O.JittableDiis(S, 10)
+    data = diis.zeros(n)
+    mats = []
+    for c in range(cycles):
+        F = rng.normal(size=(2, n, n))
+        F = F + F.transpose(0, 2, 1)
+        D = rng.normal(size=(2, n, n))
+        D = D + D.transpose(0, 2, 1)
+        mats.append((D, F))
+        data = diis.update(D, F, float(c), data, c)
+    return diis, data, mats
+
+
+def test_diis_drops_cycle_max_and_shifts_after():
+    rng = np.random.default_rng(1)
+    diis, data, mats = _fake_data(4, 11, rng)          # cycles 0..10
+    focks = data[1]
+    for c in range(10):
+        assert np.array_equal(focks[c], mats[c][1])      # slots 0..9 hold cycles 0..9
+    assert not any(np.array_equal(focks[c], mats[10][1]) for c in range(10))  # cycle 10 dropped
+    F11 = np.full((2, 4, 4), 7.0)
+    data = diis.update(np.zeros((2, 4, 4)), F11, 11.0, data, 11)
+    assert np.array_equal(data[1][0], mats[1][1])        # slot 0 now cycle 1
+    assert np.array_equal(data[1][9], F11)               # appended at the end
+    assert data[2][9] == 11.0
+
+
+def test_diis_bordered_matrix_and_first_cycle():
+    rng = np.random.default_rng(2)
+    diis, data, mats = _fake_data(4, 1, rng)
+    x = diis.coefficients(data[3], 0)
+    assert np.allclose(x[:, 0], 1.0) and np.allclose(x[:, 1:], 0.0)   # F = F_0 at cycle 0
+    diis, data, mats = _fake_data(4, 4, rng)
+    x = diis.coefficients(data[3], 3)
+    assert np.allclose(x.sum(axis=1), 1.0)                           # sum x = 1 (border row)
+    assert np.allclose(x[:, 4:], 0.0)                                # inactive slots get 0
+
+
+def test_safe_fock_solver_generalised_eigenproblem():
+    rng = np.random.default_rng(3)
+    n = 6
+    A = rng.normal(size=(n, n))
+    S = A @ A.T + n * np.eye(n)
+    F = rng.normal(size=(2, n, n))
+    F = F + F.transpose(0, 2, 1)
+    w, C = O.safe_fock_solver(F, S)
+    for s in range(2):
+        assert np.allclose(F[s] @ C[s], S @ C[s] * w[s], atol=1e-10)
+        assert np.allclose(C[s].T @ S @ C[s], np.eye(n), atol=1e-10)
+        assert np.all(np.diff(w[s]) >= 0)
+
+
+def test_get_occ_and_make_rdm1():
+    e = np.array([[0.3, -1.0, 0.1, -0.2], [1.0, 2.0, -3.0, 0.0]])
+    occ = O.get_occ(e, np.array([2, 1]))
+    assert np.array_equal(occ, [[0, 1, 0, 1], [0, 0, 1, 0]])
+    C = np.stack([np.eye(4), np.eye(4)])
+    D = O.make_rdm1(C, occ)
+    assert np.array_equal(np.diag(D[0]), occ[0])
+
+
+def test_oracle_scf_reaches_self_consistency_h2o():
+    """Oracle diff_scf_loop with B3LYP on H2O / def2-SVP: the energy settles, the density
+    stays idempotent with N_e / 2 electrons per spin, and [F, D S] vanishes."""
+    from graddft_amd.interface.basis import build_basis, nuclear_repulsion
+    from graddft_amd.interface.builder import core_guess
+    from graddft_amd.interface.grids import molecular_grid
+    from graddft_amd.interface.integrals import int1e, int2e_packed, unpack_eri
+    from oracle.ao import eval_ao
+    from oracle.xc import to_inputs, xc_value_and_grads
+    xyz = np.array([[0, 0, 0], [1.81, 0, 0], [1.81 * math.cos(1.824), 1.81 * math.sin(1.824), 0]])
+    bs = build_basis(["O", "H", "H"], xyz, "def2-svp")
+    S, T, V = int1e(bs, [8, 1, 1], xyz)
+    eri = unpack_eri(int2e_packed(bs), bs.nao)
+    D, C, e, occ = core_guess(T + V, S, 5)
+    rdm1 = np.stack([D, D])
+    K0 = np.stack([O.exchange_potential(rdm1[s], eri) for s in range(2)])
+    inp = O.SCFInputs(T + V, S, eri, nuclear_repulsion([8, 1, 1], xyz), K0, 0.2)
+    c, w = molecular_grid([8, 1, 1], xyz, 1)
+    xin = to_inputs(*eval_ao(bs, c), w)
+    xc = lambda d: xc_value_and_grads("b3lyp", xin, d)[:2]  # noqa: E731
+    st = O.diff_scf_loop(xc, inp, rdm1, np.stack([occ, occ]), 15)
+    assert abs(st.energies[-1] - st.energies[-2]) < 1e-6
+    for s in range(2):
+        Ds = st.rdm1[s]
+        assert np.allclose(Ds @ S @ Ds, Ds, atol=1e-10)
+        assert np.trace(Ds @ S) == pytest.approx(5.0, abs=1e-10)
+        comm = st.fock[s] @ Ds @ S - S @ Ds @ st.fock[s]
+        assert np.abs(comm).max() < 1e-3
+    assert -77.0 < st.energy < -75.0
