@@ -34,7 +34,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--no-scf", action="store_true", help="skip the ms/SCF-iteration measurement")
+    ap.add_argument("--scf-cycles", type=int, default=15)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--grid-level", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -79,6 +81,37 @@ def cpu_baseline(mol, theta, spec, sample):
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
     return len(w) / med, torch.get_num_threads(), f"first {len(w)} points of the same grid, median of {len(times)} runs"
+
+
+def scf_bench(args):
+    """ms per SCF iteration: H2O (seeded geometry) def2-TZVP (nao 43), level-3 grid, B3LYP
+    with exact exchange from the packed ERI, diff_scf_loop for args.scf_cycles cycles."""
+    import torch
+    from graddft_amd import B3LYP, _lib, diff_scf_loop
+    from graddft_amd.interface import geometry
+    from graddft_amd.interface.builder import molecule_from_geometry
+    sym, xyz = geometry.water(np.random.default_rng(args.seed))
+    mol = molecule_from_geometry(sym, xyz, "def2-tzvp", 3, with_eri=True, name="water")
+    out = {"workload": f"H2O def2-TZVP nao={mol.nao}, grid level 3 ({mol.grid_size} pts), B3LYP "
+                       f"(exact exchange from packed ERI), diff_scf_loop {args.scf_cycles} cycles",
+           "npts": mol.grid_size, "nao": mol.nao}
+    for prec in (["fp64", args.precision] if args.precision != "fp64" else ["fp64"]):
+        loop = diff_scf_loop(B3LYP, cycles=args.scf_cycles, precision=prec)
+        res = loop(None, mol)  # warm-up (plans, workspaces, Cholesky of S, exchange of D0)
+        torch.cuda.synchronize()
+        with _lib.KernelTimer() as kt:
+            t0 = time.perf_counter()
+            res = loop(None, mol)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+        # cycles + 1 energy/Fock evaluations (the initial one plus one per cycle)
+        out[prec] = {"ms_per_iter": round((t1 - t0) / args.scf_cycles * 1e3, 4),
+                     "energy": float(res.energy),
+                     "kernels_ms_per_iter": {k: round(v[0] / args.scf_cycles, 4) for k, v in
+                                             sorted(kt.result.items(), key=lambda kv: -kv[1][0])[:8]}}
+    if args.precision != "fp64":
+        out["abs_dE_vs_fp64_Ha"] = abs(out[args.precision]["energy"] - out["fp64"]["energy"])
+    return out
 
 
 def main():
@@ -165,6 +198,8 @@ def main():
     w_pt = 8 * n * n * nsd + 8 * n * n * nsv + 3 * spec.flops_per_point()
     step_tflops = w_pt * N / (ms / 1e3) / 1e12
 
+    scf = None if args.no_scf else scf_bench(args)
+
     cpu = None
     if not args.no_cpu_baseline:
         v, cores, sample = cpu_baseline(mol, theta, spec, min(args.cpu_sample, N))
@@ -196,6 +231,8 @@ def main():
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in
                                 sorted(kt.result.items(), key=lambda kv: -kv[1][0])},
         "vs_fp64_gpu": vs64,
+        "gate_1e-6_Ha": None if vs64 is None else vs64["abs_dE_Ha"] <= 1e-6,
+        "scf": scf,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
