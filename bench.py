@@ -25,8 +25,8 @@ import time
 import numpy as np
 
 METRIC = "XC grid pts/s (fwd+adjoint) & ms/SCF iter at ≤1e-6 Ha vs FP64 GPU path"
-PEAK = {"fp64": ("TFLOP/s", 78.6), "fp32": ("TFLOP/s", 157.3), "bf16x3": ("TFLOP/s", 2500.0)}
-DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x3": "bf16x3"}
+PEAK = {"fp64": ("TFLOP/s", 78.6), "fp32": ("TFLOP/s", 157.3), "bf16x6": ("TFLOP/s", 2500.0)}
+DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x6": "bf16x6"}
 
 
 def parse():
@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16x6"])
     ap.add_argument("--no-scf", action="store_true", help="skip the ms/SCF-iteration measurement")
     ap.add_argument("--scf-cycles", type=int, default=15)
     ap.add_argument("--seed", type=int, default=2)
@@ -54,7 +54,7 @@ def kernel_flops(name, npts, nao, spec, nsd):
         "gemm_bwd_data": 2 * npts * h * h,
         "gemm_wgrad": 2 * npts * h * h,
     }
-    for suffix in ("_f64", "_f32", "_bf16x3"):
+    for suffix in ("_f64", "_f32", "_bf16x6"):
         name = name.replace(suffix, "")
     return table.get(name)
 

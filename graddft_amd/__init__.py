@@ -2,16 +2,20 @@
 
 Public API (names follow XanaduAI/GradDFT's ``grad_dft`` package):
 ``Grid``, ``Molecule``, ``Functional``, ``NeuralFunctional``, ``MLPSpec``, ``B3LYP``,
-``energy_predictor``, ``diff_scf_loop``, ``molecule_from_geometry``.
+``energy_predictor``, ``non_scf_predictor``, ``diff_scf_loop``, ``simple_energy_loss``,
+``mse_energy_loss``, ``train_kernel``, ``adam``, ``molecule_from_geometry``.
 """
 from .molecule import Grid, Molecule
-from .functional import Functional, NeuralFunctional, MLPSpec, XCPlan, XCEnergyFn
+from .functional import Functional, NeuralFunctional, MLPSpec, XCPlan, XCEnergyFn, XCEnergyVxcFn
 from .popular_functionals import B3LYP, B3LYP_GRID
 from .interface.builder import molecule_from_geometry
-from .train import energy_predictor
-from .evaluate import JittableDiis, diff_scf_loop
+from .train import (energy_predictor, simple_energy_loss, mse_energy_loss, train_kernel, adam,
+                    apply_updates)
+from .evaluate import JittableDiis, diff_scf_loop, non_scf_predictor
 from .eigenproblem import safe_eigh_vjp, safe_fock_solver, safe_general_eigh
 
 __all__ = ["Grid", "Molecule", "Functional", "NeuralFunctional", "MLPSpec", "XCPlan",
-           "XCEnergyFn", "B3LYP", "B3LYP_GRID", "molecule_from_geometry", "energy_predictor",
-           "JittableDiis", "diff_scf_loop", "safe_eigh_vjp", "safe_fock_solver", "safe_general_eigh"]
+           "XCEnergyFn", "XCEnergyVxcFn", "B3LYP", "B3LYP_GRID", "molecule_from_geometry",
+           "energy_predictor", "non_scf_predictor", "simple_energy_loss", "mse_energy_loss",
+           "train_kernel", "adam", "apply_updates", "JittableDiis", "diff_scf_loop", "safe_eigh_vjp",
+           "safe_fock_solver", "safe_general_eigh"]

@@ -12,7 +12,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgdx.so")
 
-GDX_FP64, GDX_FP32, GDX_BF16X3 = 0, 1, 2
+GDX_FP64, GDX_FP32, GDX_BF16X6 = 0, 1, 2
 GDX_FUNC_B3LYP, GDX_FUNC_NEURAL_B3LYP = 0, 1
 
 
@@ -54,6 +54,8 @@ SIGNATURES = {
     "gdx_profile_end": (ctypes.c_int, [_I, ctypes.c_char_p, _P, _P, _P]),
     "gdx_xc_densities": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P]),
     "gdx_xc_backward": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P, _P, _P, _P, _P]),
+    "gdx_xc_hvp_workspace_size": (ctypes.c_int, [ctypes.POINTER(XCDesc), ctypes.POINTER(_Z)]),
+    "gdx_xc_hvp": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

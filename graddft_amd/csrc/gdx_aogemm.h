@@ -155,27 +155,30 @@ __global__ __launch_bounds__(256) void density_ao_kernel(const T* __restrict__ a
     }
 }
 
-constexpr int VX_T = 128, VX_P = 4, VX_K = 4 * VX_P;
+constexpr int VX_P = 4, VX_K = 4 * VX_P;
 
-template <typename T, bool SAME>
+// VT: output tile edge (128, or 64 for nao_pad <= 64); each wave owns a (VT/2)^2 quarter.
+template <typename T, bool SAME, int VT>
 __global__ __launch_bounds__(256) void vxc_ao_kernel(const T* __restrict__ ao, int npts_pad, int nP,
                                                      const double* __restrict__ vpt, int chunk,
                                                      double* __restrict__ vpart) {
   typedef typename MfmaT<T>::acc_t acc_t;
-  __shared__ T As[2][VX_K][VX_T + 4];
-  __shared__ T Bs[2][VX_K][VX_T + 4];
+  constexpr int WB = VT / 32;          // 16-wide MFMA blocks per wave edge
+  constexpr int CPT = VT / 64;         // columns staged per thread
+  __shared__ T As[2][VX_K][VT + 4];
+  __shared__ T Bs[2][VX_K][VT + 4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nt = (nP + VX_T - 1) / VX_T;
-  const int a0 = (blockIdx.x / nt) * VX_T, b0 = (blockIdx.x % nt) * VX_T;
+  const int nt = (nP + VT - 1) / VT;
+  const int a0 = (blockIdx.x / nt) * VT, b0 = (blockIdx.x % nt) * VT;
   const int s = blockIdx.z;
   const int rbeg = blockIdx.y * chunk;
   const int rend = min(npts_pad, rbeg + chunk);
   const size_t plane = (size_t)npts_pad * nP;
   const double* vp = vpt + (size_t)s * npts_pad * VPT_STRIDE;
-  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-  // staging role of this thread: point p of the stage, 2 adjacent columns
-  const int sp = tid >> 6, sc = (tid & 63) * 2;
-  typedef T v2 __attribute__((ext_vector_type(2)));
+  const int wm = (wave >> 1) * (VT / 2), wn = (wave & 1) * (VT / 2);
+  // staging role of this thread: point p of the stage, CPT adjacent columns
+  const int sp = tid >> 6, sc = (tid & 63) * CPT;
+  typedef T v2 __attribute__((ext_vector_type(CPT)));
 
   v2 ra[4], rb[5];
   double sv[VPT_STRIDE];
@@ -196,7 +199,7 @@ __global__ __launch_bounds__(256) void vxc_ao_kernel(const T* __restrict__ ao, i
   };
   auto lstore = [&](int buf) {
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
+    for (int e = 0; e < CPT; ++e) {
 #pragma unroll
       for (int c = 0; c < 4; ++c) As[buf][c * VX_P + sp][sc + e] = SAME ? rb[c][e] : ra[c][e];
       const double phi = rb[0][e], gx = rb[1][e], gy = rb[2][e], gz = rb[3][e], lp = rb[4][e];
@@ -207,11 +210,11 @@ __global__ __launch_bounds__(256) void vxc_ao_kernel(const T* __restrict__ ao, i
     }
   };
 
-  acc_t acc[4][4];
+  acc_t acc[WB][WB];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < WB; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = (acc_t){0, 0, 0, 0};
+    for (int j = 0; j < WB; ++j) acc[i][j] = (acc_t){0, 0, 0, 0};
   const int nst = (rend - rbeg + VX_P - 1) / VX_P;
   if (nst > 0) {
     gload(rbeg);
@@ -224,24 +227,24 @@ __global__ __launch_bounds__(256) void vxc_ao_kernel(const T* __restrict__ ao, i
 #pragma unroll
     for (int kk = 0; kk < VX_K / 4; ++kk) {
       const int kr = kk * 4 + (lane >> 4);
-      T af[4], bf[4];
+      T af[WB], bf[WB];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = As[cur][kr][wm + i * 16 + (lane & 15)];
+      for (int i = 0; i < WB; ++i) af[i] = As[cur][kr][wm + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bf[j] = Bs[cur][kr][wn + j * 16 + (lane & 15)];
+      for (int j = 0; j < WB; ++j) bf[j] = Bs[cur][kr][wn + j * 16 + (lane & 15)];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < WB; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = MfmaT<T>::op(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < WB; ++j) acc[i][j] = MfmaT<T>::op(af[i], bf[j], acc[i][j]);
     }
     if (st + 1 < nst) lstore(cur ^ 1);
     __syncthreads();
   }
   double* out = vpart + ((size_t)blockIdx.y * gridDim.z + s) * nP * nP;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < WB; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < WB; ++j) {
       const int col = b0 + wn + j * 16 + (lane & 15);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {

@@ -1,8 +1,8 @@
 // Per-grid-point B3LYP energy densities, written once as templates over a scalar type so
-// the same expression tree yields values (T = double) and exact first derivatives
-// (T = Dual<K>, forward mode).  Every expression follows the reference term by term,
-// including its clips and where-selects, so the derivative matches what jax.grad of the
-// reference produces:
+// the same expression tree yields values (T = double), exact first derivatives
+// (T = Dual<K>, forward mode) and directional second derivatives (T = Dual<K, Dual<1>>).
+// Every expression follows the reference term by term, including its clips and
+// where-selects, so the derivatives match what jax.grad of the reference produces:
 //   lsda_x_e        grad_dft/popular_functionals.py:29-50
 //   b88_x_e         grad_dft/popular_functionals.py:52-103
 //   vwn_c_e         grad_dft/popular_functionals.py:141-195
@@ -18,76 +18,89 @@ namespace gdx {
 
 #define GDX_HD __host__ __device__ __forceinline__
 
-template <int K>
+// Dual<K, S>: value and K partial derivatives over the scalar type S.  S = double gives
+// first derivatives; S = Dual<1> (a value with a tangent along a fixed direction) gives,
+// in d[i].d[0], the directional derivative of the i-th partial (a Hessian-vector row).
+template <int K, typename S = double>
 struct Dual {
-  double v;
-  double d[K];
+  S v;
+  S d[K];
+  GDX_HD Dual() {}
+  GDX_HD explicit Dual(double c) : v(c) {
+#pragma unroll
+    for (int i = 0; i < K; ++i) d[i] = S(0.0);
+  }
 };
 
-template <int K>
-GDX_HD Dual<K> dconst(double c) {
-  Dual<K> r; r.v = c;
-#pragma unroll
-  for (int i = 0; i < K; ++i) r.d[i] = 0.0;
+template <int K, typename S = double>
+GDX_HD Dual<K, S> dconst(double c) { return Dual<K, S>(c); }
+// independent variable idx with value x
+template <int K, typename S = double>
+GDX_HD Dual<K, S> dvar(const S& x, int idx) {
+  Dual<K, S> r(0.0);
+  r.v = x;
+  r.d[idx] = S(1.0);
   return r;
 }
-template <int K>
-GDX_HD Dual<K> dvar(double x, int idx) {
-  Dual<K> r = dconst<K>(x); r.d[idx] = 1.0; return r;
-}
 // chain rule helper: r = f(a), r' = fp * a'
-template <int K>
-GDX_HD Dual<K> dchain(const Dual<K>& a, double f, double fp) {
-  Dual<K> r; r.v = f;
+template <int K, typename S, typename FP>
+GDX_HD Dual<K, S> dchain(const Dual<K, S>& a, const S& f, const FP& fp) {
+  Dual<K, S> r; r.v = f;
 #pragma unroll
   for (int i = 0; i < K; ++i) r.d[i] = fp * a.d[i];
   return r;
 }
-template <int K> GDX_HD Dual<K> operator+(const Dual<K>& a, const Dual<K>& b) {
-  Dual<K> r; r.v = a.v + b.v;
+template <int K, typename S> GDX_HD Dual<K, S> operator+(const Dual<K, S>& a, const Dual<K, S>& b) {
+  Dual<K, S> r; r.v = a.v + b.v;
 #pragma unroll
   for (int i = 0; i < K; ++i) r.d[i] = a.d[i] + b.d[i];
   return r;
 }
-template <int K> GDX_HD Dual<K> operator-(const Dual<K>& a, const Dual<K>& b) {
-  Dual<K> r; r.v = a.v - b.v;
+template <int K, typename S> GDX_HD Dual<K, S> operator-(const Dual<K, S>& a, const Dual<K, S>& b) {
+  Dual<K, S> r; r.v = a.v - b.v;
 #pragma unroll
   for (int i = 0; i < K; ++i) r.d[i] = a.d[i] - b.d[i];
   return r;
 }
-template <int K> GDX_HD Dual<K> operator-(const Dual<K>& a) {
-  Dual<K> r; r.v = -a.v;
+template <int K, typename S> GDX_HD Dual<K, S> operator-(const Dual<K, S>& a) {
+  Dual<K, S> r; r.v = -a.v;
 #pragma unroll
   for (int i = 0; i < K; ++i) r.d[i] = -a.d[i];
   return r;
 }
-template <int K> GDX_HD Dual<K> operator*(const Dual<K>& a, const Dual<K>& b) {
-  Dual<K> r; r.v = a.v * b.v;
+template <int K, typename S> GDX_HD Dual<K, S> operator*(const Dual<K, S>& a, const Dual<K, S>& b) {
+  Dual<K, S> r; r.v = a.v * b.v;
 #pragma unroll
   for (int i = 0; i < K; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
   return r;
 }
-template <int K> GDX_HD Dual<K> operator/(const Dual<K>& a, const Dual<K>& b) {
-  Dual<K> r; r.v = a.v / b.v;
-  const double ib = 1.0 / b.v;
+template <int K, typename S> GDX_HD Dual<K, S> operator/(const Dual<K, S>& a, const Dual<K, S>& b) {
+  Dual<K, S> r; r.v = a.v / b.v;
+  const S ib = 1.0 / b.v;
 #pragma unroll
   for (int i = 0; i < K; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) * ib;
   return r;
 }
-template <int K> GDX_HD Dual<K> operator+(const Dual<K>& a, double c) { Dual<K> r = a; r.v += c; return r; }
-template <int K> GDX_HD Dual<K> operator+(double c, const Dual<K>& a) { return a + c; }
-template <int K> GDX_HD Dual<K> operator-(const Dual<K>& a, double c) { Dual<K> r = a; r.v -= c; return r; }
-template <int K> GDX_HD Dual<K> operator-(double c, const Dual<K>& a) { return (-a) + c; }
-template <int K> GDX_HD Dual<K> operator*(const Dual<K>& a, double c) { return dchain(a, a.v * c, c); }
-template <int K> GDX_HD Dual<K> operator*(double c, const Dual<K>& a) { return a * c; }
-template <int K> GDX_HD Dual<K> operator/(const Dual<K>& a, double c) { return a * (1.0 / c); }
-template <int K> GDX_HD Dual<K> operator/(double c, const Dual<K>& a) {
-  const double r = c / a.v; return dchain(a, r, -r / a.v);
+template <int K, typename S> GDX_HD Dual<K, S> operator+(const Dual<K, S>& a, double c) {
+  Dual<K, S> r = a; r.v = r.v + c; return r;
+}
+template <int K, typename S> GDX_HD Dual<K, S> operator+(double c, const Dual<K, S>& a) { return a + c; }
+template <int K, typename S> GDX_HD Dual<K, S> operator-(const Dual<K, S>& a, double c) {
+  Dual<K, S> r = a; r.v = r.v - c; return r;
+}
+template <int K, typename S> GDX_HD Dual<K, S> operator-(double c, const Dual<K, S>& a) { return (-a) + c; }
+template <int K, typename S> GDX_HD Dual<K, S> operator*(const Dual<K, S>& a, double c) {
+  return dchain(a, a.v * c, c);
+}
+template <int K, typename S> GDX_HD Dual<K, S> operator*(double c, const Dual<K, S>& a) { return a * c; }
+template <int K, typename S> GDX_HD Dual<K, S> operator/(const Dual<K, S>& a, double c) { return a * (1.0 / c); }
+template <int K, typename S> GDX_HD Dual<K, S> operator/(double c, const Dual<K, S>& a) {
+  const S r = c / a.v; return dchain(a, r, -r / a.v);
 }
 
 // ---- elementary functions (value overloads for double, chain rule for Dual) ----
 GDX_HD double val(double x) { return x; }
-template <int K> GDX_HD double val(const Dual<K>& x) { return x.v; }
+template <int K, typename S> GDX_HD double val(const Dual<K, S>& x) { return val(x.v); }
 
 GDX_HD double flog(double x) { return log(x); }
 GDX_HD double flog2(double x) { return log2(x); }
@@ -98,33 +111,41 @@ GDX_HD double fasinh(double x) { return asinh(x); }
 GDX_HD double fatan(double x) { return atan(x); }
 GDX_HD double fpow(double x, double p) { return pow(x, p); }
 
-template <int K> GDX_HD Dual<K> flog(const Dual<K>& a) { return dchain(a, log(a.v), 1.0 / a.v); }
-template <int K> GDX_HD Dual<K> flog2(const Dual<K>& a) {
-  return dchain(a, log2(a.v), 1.4426950408889634 / a.v);
+template <int K, typename S> GDX_HD Dual<K, S> flog(const Dual<K, S>& a) {
+  return dchain(a, flog(a.v), 1.0 / a.v);
 }
-template <int K> GDX_HD Dual<K> fexp(const Dual<K>& a) { const double e = exp(a.v); return dchain(a, e, e); }
-template <int K> GDX_HD Dual<K> fexp2(const Dual<K>& a) {
-  const double e = exp2(a.v); return dchain(a, e, e * 0.6931471805599453);
+template <int K, typename S> GDX_HD Dual<K, S> flog2(const Dual<K, S>& a) {
+  return dchain(a, flog2(a.v), 1.4426950408889634 / a.v);
 }
-template <int K> GDX_HD Dual<K> fsqrt(const Dual<K>& a) { const double s = sqrt(a.v); return dchain(a, s, 0.5 / s); }
-template <int K> GDX_HD Dual<K> fasinh(const Dual<K>& a) {
-  return dchain(a, asinh(a.v), 1.0 / sqrt(a.v * a.v + 1.0));
+template <int K, typename S> GDX_HD Dual<K, S> fexp(const Dual<K, S>& a) {
+  const S e = fexp(a.v); return dchain(a, e, e);
 }
-template <int K> GDX_HD Dual<K> fatan(const Dual<K>& a) { return dchain(a, atan(a.v), 1.0 / (1.0 + a.v * a.v)); }
-template <int K> GDX_HD Dual<K> fpow(const Dual<K>& a, double p) {
-  const double r = pow(a.v, p);
-  return dchain(a, r, p * pow(a.v, p - 1.0));
+template <int K, typename S> GDX_HD Dual<K, S> fexp2(const Dual<K, S>& a) {
+  const S e = fexp2(a.v); return dchain(a, e, e * 0.6931471805599453);
+}
+template <int K, typename S> GDX_HD Dual<K, S> fsqrt(const Dual<K, S>& a) {
+  const S s = fsqrt(a.v); return dchain(a, s, 0.5 / s);
+}
+template <int K, typename S> GDX_HD Dual<K, S> fasinh(const Dual<K, S>& a) {
+  return dchain(a, fasinh(a.v), 1.0 / fsqrt(a.v * a.v + 1.0));
+}
+template <int K, typename S> GDX_HD Dual<K, S> fatan(const Dual<K, S>& a) {
+  return dchain(a, fatan(a.v), 1.0 / (1.0 + a.v * a.v));
+}
+template <int K, typename S> GDX_HD Dual<K, S> fpow(const Dual<K, S>& a, double p) {
+  const S r = fpow(a.v, p);
+  return dchain(a, r, p * fpow(a.v, p - 1.0));
 }
 
 // jnp.clip(x, a_min=c): max(x, c) with gradient 1 where x > c
 GDX_HD double clipmin(double x, double c) { return x > c ? x : c; }
-template <int K> GDX_HD Dual<K> clipmin(const Dual<K>& x, double c) { return x.v > c ? x : dconst<K>(c); }
+template <int K, typename S> GDX_HD Dual<K, S> clipmin(const Dual<K, S>& x, double c) {
+  return val(x) > c ? x : Dual<K, S>(c);
+}
 // jnp.where(cond, a, b): selects whole value (and tangent)
 template <typename T> GDX_HD T sel(bool c, const T& a, const T& b) { return c ? a : b; }
 
-template <typename T> GDX_HD T cst(double c);
-template <> GDX_HD double cst<double>(double c) { return c; }
-template <> GDX_HD Dual<6> cst<Dual<6>>(double c) { return dconst<6>(c); }
+template <typename T> GDX_HD T cst(double c) { return T(c); }
 
 constexpr double CLIP = 1e-30;
 constexpr double PI = 3.14159265358979323846;

@@ -24,7 +24,8 @@ constexpr int NT = 256;
 // JT x JT elements of the lower triangle: each block forms row sums (M Dt over its
 // columns) and column sums (M^T Dt over its strictly-lower rows); partial vectors are
 // added by coulomb_finish.  Every packed element is read exactly once.
-constexpr int JT = 256;
+constexpr int JT = 256;   // large tile (P >= 4096)
+constexpr int JTS = 64;   // small tile
 
 __global__ void pack_density_kernel(const double* __restrict__ rdm1, int n, double* __restrict__ Dt) {
   const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -42,10 +43,14 @@ __global__ void pack_density_kernel(const double* __restrict__ rdm1, int n, doub
   Dt[I] = ii == j ? dij : dij + dji;
 }
 
+template <int TT>
 __global__ __launch_bounds__(NT) void coulomb_tile_kernel(const double* __restrict__ eri, long P,
                                                           const double* __restrict__ Dt,
                                                           double* __restrict__ prow, double* __restrict__ pcol) {
-  __shared__ double cs[16][JT + 1];
+  // TT x TT tile of the lower triangle; 256 threads as 16 (ty) x 16 (tx); rows I = I0 + ty + 16 a,
+  // columns K = K0 + 16 j + tx
+  constexpr int NA = TT / 16;
+  __shared__ double cs[16][TT + 1];
   // triangular tile index -> (bi, bk), bk <= bi
   const long x = blockIdx.x;
   long bi = (long)((sqrt(8.0 * (double)x + 1.0) - 1.0) / 2.0);
@@ -53,24 +58,24 @@ __global__ __launch_bounds__(NT) void coulomb_tile_kernel(const double* __restri
   while ((bi + 1) * (bi + 2) / 2 <= x) ++bi;
   const long bk = x - bi * (bi + 1) / 2;
   const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
-  const long I0 = bi * JT, K0 = bk * JT;
-  double dk[16], csum[16];
+  const long I0 = bi * TT, K0 = bk * TT;
+  double dk[NA], csum[NA];
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
+  for (int j = 0; j < NA; ++j) {
     const long K = K0 + j * 16 + tx;
     dk[j] = K < P ? Dt[K] : 0.0;
     csum[j] = 0.0;
   }
-  double rsum[16];
+  double rsum[NA];
 #pragma unroll
-  for (int a = 0; a < 16; ++a) {
+  for (int a = 0; a < NA; ++a) {
     const long I = I0 + ty + 16 * a;
     double r = 0.0;
     if (I < P) {
       const double di = Dt[I];
       const double* row = eri + I * (I + 1) / 2;
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
+      for (int j = 0; j < NA; ++j) {
         const long K = K0 + j * 16 + tx;
         if (K <= I) {
           const double m = row[K];
@@ -83,29 +88,29 @@ __global__ __launch_bounds__(NT) void coulomb_tile_kernel(const double* __restri
   }
   // row sums over the 16 threads of this ty (consecutive lanes)
 #pragma unroll
-  for (int a = 0; a < 16; ++a) {
+  for (int a = 0; a < NA; ++a) {
     const double r = sum16(rsum[a]);
     const long I = I0 + ty + 16 * a;
     if (tx == 0 && I < P) prow[bk * P + I] = r;
   }
 #pragma unroll
-  for (int j = 0; j < 16; ++j) cs[ty][j * 16 + tx] = csum[j];
+  for (int j = 0; j < NA; ++j) cs[ty][j * 16 + tx] = csum[j];
   __syncthreads();
-  {
-    const long K = K0 + tid;
+  for (int k = tid; k < TT; k += NT) {
+    const long K = K0 + k;
     double c = 0.0;
 #pragma unroll
-    for (int t = 0; t < 16; ++t) c += cs[t][tid];
+    for (int t = 0; t < 16; ++t) c += cs[t][k];
     if (K < P) pcol[bi * P + K] = c;
   }
 }
 
 // J~_I = sum_{bk <= I/JT} prow[bk][I] + sum_{bi >= I/JT} pcol[bi][I], unpacked to J (n x n)
 __global__ void coulomb_finish_kernel(const double* __restrict__ prow, const double* __restrict__ pcol, long P,
-                                      int nb, int n, double* __restrict__ J) {
+                                      int nb, int tt, int n, double* __restrict__ J) {
   const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= P) return;
-  const int t = (int)(I / JT);
+  const int t = (int)(I / tt);
   double acc = 0.0;
   for (int b = 0; b <= t; ++b) acc += prow[b * P + I];
   for (int b = t; b < nb; ++b) acc += pcol[b * P + I];
@@ -386,31 +391,42 @@ __global__ __launch_bounds__(EIG_THREADS) void jacobi_eigh_kernel(const double* 
 
 // ------------------------------------------------------------- DIIS solve ----------
 // x_s = B_s^-1 e_0 for each spin by Gaussian elimination with partial pivoting
-// (B_s is (m+1) x (m+1), m <= 31); one thread per spin.
-__global__ void diis_solve_kernel(const double* __restrict__ B, int m1, int nspin, double* __restrict__ x) {
-  const int s = threadIdx.x;
-  if (s >= nspin) return;
-  double a[32][33];
-  for (int i = 0; i < m1; ++i) {
-    for (int j = 0; j < m1; ++j) a[i][j] = B[((size_t)s * m1 + i) * m1 + j];
-    a[i][m1] = i == 0 ? 1.0 : 0.0;
-  }
+// (B_s is (m+1) x (m+1), m1 <= 32).  One 64-thread block per spin, augmented matrix in LDS,
+// one thread per row during elimination.
+__global__ __launch_bounds__(64) void diis_solve_kernel(const double* __restrict__ B, int m1,
+                                                        double* __restrict__ x) {
+  __shared__ double a[32][34];
+  __shared__ int piv;
+  const int s = blockIdx.x, t = threadIdx.x;
+  for (int e = t; e < m1 * m1; e += blockDim.x) a[e / m1][e % m1] = B[(size_t)s * m1 * m1 + e];
+  if (t < m1) a[t][m1] = t == 0 ? 1.0 : 0.0;
+  __syncthreads();
   for (int c = 0; c < m1; ++c) {
-    int piv = c;
-    for (int i = c + 1; i < m1; ++i)
-      if (fabs(a[i][c]) > fabs(a[piv][c])) piv = i;
-    if (piv != c)
-      for (int j = 0; j <= m1; ++j) { const double t = a[c][j]; a[c][j] = a[piv][j]; a[piv][j] = t; }
-    const double d = a[c][c];
-    for (int i = c + 1; i < m1; ++i) {
-      const double f = a[i][c] / d;
-      for (int j = c; j <= m1; ++j) a[i][j] -= f * a[c][j];
+    if (t == 0) {
+      int p = c;
+      for (int i = c + 1; i < m1; ++i)
+        if (fabs(a[i][c]) > fabs(a[p][c])) p = i;
+      piv = p;
     }
+    __syncthreads();
+    if (piv != c && t <= m1) {
+      const double tmp = a[c][t];
+      a[c][t] = a[piv][t];
+      a[piv][t] = tmp;
+    }
+    __syncthreads();
+    if (t > c && t < m1) {
+      const double f = a[t][c] / a[c][c];
+      for (int j = c; j <= m1; ++j) a[t][j] -= f * a[c][j];
+    }
+    __syncthreads();
   }
-  for (int i = m1 - 1; i >= 0; --i) {
-    double t = a[i][m1];
-    for (int j = i + 1; j < m1; ++j) t -= a[i][j] * x[(size_t)s * m1 + j];
-    x[(size_t)s * m1 + i] = t / a[i][i];
+  if (t == 0) {
+    for (int i = m1 - 1; i >= 0; --i) {
+      double v = a[i][m1];
+      for (int j = i + 1; j < m1; ++j) v -= a[i][j] * x[(size_t)s * m1 + j];
+      x[(size_t)s * m1 + i] = v / a[i][i];
+    }
   }
 }
 
@@ -423,10 +439,12 @@ using namespace gdx;
 
 extern "C" {
 
+inline int coulomb_tile(long P) { return P >= 4096 ? JT : JTS; }
+
 int gdx_coulomb_workspace_size(int32_t n, size_t* bytes) {
   GDX_REQUIRE(n > 0 && bytes, "gdx_coulomb_workspace_size: bad arguments");
   const long P = (long)n * (n + 1) / 2;
-  const long nb = (P + JT - 1) / JT;
+  const long nb = (P + coulomb_tile(P) - 1) / coulomb_tile(P);
   *bytes = (size_t)(P + 2 * nb * P) * sizeof(double);
   return GDX_OK;
 }
@@ -435,14 +453,19 @@ int gdx_coulomb_packed(void* stream, int32_t n, const double* eri, const double*
   GDX_REQUIRE(n > 0 && eri && rdm1 && J && workspace, "gdx_coulomb_packed: bad arguments");
   hipStream_t st = (hipStream_t)stream;
   const long P = (long)n * (n + 1) / 2;
-  const long nb = (P + JT - 1) / JT;
+  const int tt = coulomb_tile(P);
+  const long nb = (P + tt - 1) / tt;
   double* Dt = (double*)workspace;
   double* prow = Dt + P;
   double* pcol = prow + nb * P;
   GDX_LAUNCH("pack_density", pack_density_kernel, dim3(nblk(P)), dim3(NT), 0, st, rdm1, n, Dt);
-  GDX_LAUNCH("coulomb_tile", coulomb_tile_kernel, dim3((unsigned)(nb * (nb + 1) / 2)), dim3(NT), 0, st, eri, P, Dt,
-             prow, pcol);
-  GDX_LAUNCH("coulomb_finish", coulomb_finish_kernel, dim3(nblk(P)), dim3(NT), 0, st, prow, pcol, P, (int)nb, n, J);
+  const dim3 grid((unsigned)(nb * (nb + 1) / 2));
+  if (tt == JT)
+    GDX_LAUNCH("coulomb_tile", coulomb_tile_kernel<JT>, grid, dim3(NT), 0, st, eri, P, Dt, prow, pcol);
+  else
+    GDX_LAUNCH("coulomb_tile", coulomb_tile_kernel<JTS>, grid, dim3(NT), 0, st, eri, P, Dt, prow, pcol);
+  GDX_LAUNCH("coulomb_finish", coulomb_finish_kernel, dim3(nblk(P)), dim3(NT), 0, st, prow, pcol, P, (int)nb, tt, n,
+             J);
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
 }
@@ -508,7 +531,7 @@ int gdx_eigh_batched(void* stream, int32_t nbatch, int32_t n, const double* A, d
 
 int gdx_diis_solve(void* stream, int32_t nspin, int32_t m1, const double* B, double* x) {
   GDX_REQUIRE(nspin > 0 && nspin <= 64 && m1 > 0 && m1 <= 32 && B && x, "gdx_diis_solve: bad arguments");
-  GDX_LAUNCH("diis_solve", diis_solve_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, B, m1, nspin, x);
+  GDX_LAUNCH("diis_solve", diis_solve_kernel, dim3(nspin), dim3(64), 0, (hipStream_t)stream, B, m1, x);
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
 }

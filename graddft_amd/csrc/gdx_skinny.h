@@ -32,11 +32,25 @@ __global__ __launch_bounds__(256) void skinny_out16_kernel(
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r0 = blockIdx.x * 64;
   acc_t acc = (acc_t){0, 0, 0, 0};
+  constexpr int VEC = 16 / sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(VEC)));
+  const bool whole = (K % SK_KC == 0) && (r0 + 64 <= M) && (lda % VEC == 0);
   for (int k0 = 0; k0 < K; k0 += SK_KC) {
-    // 64 rows x 64 k, 16 elements per thread
-    for (int e = tid; e < 64 * SK_KC; e += 256) {
-      const int r = e / SK_KC, k = e % SK_KC;
-      As[r][k] = (r0 + r < M && k0 + k < K) ? A[(size_t)(r0 + r) * lda + k0 + k] : (T)0;
+    // 64 rows x 64 k staged in LDS
+    if (whole) {
+#pragma unroll
+      for (int i = 0; i < 64 * SK_KC / VEC / 256; ++i) {
+        const int idx = tid + 256 * i;
+        const int r = idx / (SK_KC / VEC), cv = idx % (SK_KC / VEC);
+        const vec_t v = *(const vec_t*)(A + (size_t)(r0 + r) * lda + k0 + cv * VEC);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) As[r][cv * VEC + e] = v[e];
+      }
+    } else {
+      for (int e = tid; e < 64 * SK_KC; e += 256) {
+        const int r = e / SK_KC, k = e % SK_KC;
+        As[r][k] = (r0 + r < M && k0 + k < K) ? A[(size_t)(r0 + r) * lda + k0 + k] : (T)0;
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -81,48 +95,96 @@ __global__ __launch_bounds__(256) void skinny_out16_kernel(
   (void)red;
 }
 
-// P[blockIdx.x] (16 x N) += A16[r][m] * B[r][n] over rows [blockIdx.x*rows, +rows)
+// P[blockIdx.x] (16 x N) = sum over rows r in [blockIdx.x*rows, +rows) of A16[r][m] B[r][n].
+// N <= 256 and a multiple of 16; wave w owns columns [64 w, 64 w + 64), so the four waves
+// never share an output.  32-row stages of A16 and B go through double-buffered LDS with
+// 16-byte global loads prefetched one stage ahead.
+constexpr int SW_KS = 32;
+
 template <typename T>
 __global__ __launch_bounds__(256) void skinny_wgrad_kernel(
     int M, int rows, const T* __restrict__ A16, const T* __restrict__ B, int N,
     double* __restrict__ part) {
   typedef typename MfmaT<T>::acc_t acc_t;
-  constexpr int NJ = 16;  // N / 16 column blocks, N <= 256
-  __shared__ double red[16][256 + 1];
+  constexpr int VEC = 16 / sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(VEC)));
+  constexpr int SLOTS = 256 / VEC;               // 16-byte vectors per 256-column row
+  constexpr int NVB = SW_KS * SLOTS / 256;       // B vectors per thread per stage
+  constexpr int NVA = (SW_KS * 16 / VEC + 255) / 256;
+  __shared__ T As[2][SW_KS][16 + 1];
+  __shared__ T Bs[2][SW_KS][256 + 4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int rbeg = blockIdx.x * rows;
   const int rend = min(M, rbeg + rows);
-  const int nj = N / 16;
-  acc_t acc[NJ];
+  const int nvrow = N / VEC;  // vectors per B row
+  acc_t acc[4];
 #pragma unroll
-  for (int j = 0; j < NJ; ++j) acc[j] = (acc_t){0, 0, 0, 0};
-  for (int r = rbeg + wave * 4; r < rend; r += 16) {
-    const int rr = r + (lane >> 4);
-    const bool ok = rr < rend;
-    const T af = ok ? A16[(size_t)rr * 16 + (lane & 15)] : (T)0;
+  for (int j = 0; j < 4; ++j) acc[j] = (acc_t){0, 0, 0, 0};
+  vec_t rb[NVB], ra[NVA];
+  auto gload = [&](int r0) {
 #pragma unroll
-    for (int j = 0; j < NJ; ++j)
-      if (j < nj) {
-        const T bf = ok ? B[(size_t)rr * N + j * 16 + (lane & 15)] : (T)0;
+    for (int i = 0; i < NVB; ++i) {
+      const int idx = tid + 256 * i;
+      const int r = idx / SLOTS, cv = idx % SLOTS;   // vector slots of a 256-column row
+      const int rr = r0 + r;
+      rb[i] = (rr < rend && cv < nvrow) ? *(const vec_t*)(B + (size_t)rr * N + cv * VEC) : (vec_t)(T)0;
+    }
+#pragma unroll
+    for (int i = 0; i < NVA; ++i) {
+      const int idx = tid + 256 * i;
+      const int r = idx / (16 / VEC), cv = idx % (16 / VEC);
+      const int rr = r0 + r;
+      if (idx < SW_KS * 16 / VEC)
+        ra[i] = rr < rend ? *(const vec_t*)(A16 + (size_t)rr * 16 + cv * VEC) : (vec_t)(T)0;
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < NVB; ++i) {
+      const int idx = tid + 256 * i;
+      const int r = idx / SLOTS, cv = idx % SLOTS;
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) Bs[buf][r][cv * VEC + e] = rb[i][e];
+    }
+#pragma unroll
+    for (int i = 0; i < NVA; ++i) {
+      const int idx = tid + 256 * i;
+      const int r = idx / (16 / VEC), cv = idx % (16 / VEC);
+      if (idx < SW_KS * 16 / VEC)
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) As[buf][r][cv * VEC + e] = ra[i][e];
+    }
+  };
+  const int nst = (rend - rbeg + SW_KS - 1) / SW_KS;
+  if (nst > 0) {
+    gload(rbeg);
+    lstore(0);
+    __syncthreads();
+  }
+  for (int st = 0; st < nst; ++st) {
+    const int cur = st & 1;
+    if (st + 1 < nst) gload(rbeg + (st + 1) * SW_KS);
+#pragma unroll
+    for (int kk = 0; kk < SW_KS / 4; ++kk) {
+      const int kr = kk * 4 + (lane >> 4);
+      const T af = As[cur][kr][lane & 15];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const T bf = Bs[cur][kr][wave * 64 + j * 16 + (lane & 15)];
         acc[j] = MfmaT<T>::op(af, bf, acc[j]);
       }
-  }
-  for (int w = 0; w < 4; ++w) {
-    if (wave == w) {
-#pragma unroll
-      for (int j = 0; j < NJ; ++j)
-        if (j < nj)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int m = MfmaT<T>::row(lane, q), n = j * 16 + (lane & 15);
-            const double v = (double)acc[j][q];
-            red[m][n] = (w == 0) ? v : red[m][n] + v;
-          }
     }
+    if (st + 1 < nst) lstore(cur ^ 1);
     __syncthreads();
   }
   double* out = part + (size_t)blockIdx.x * 16 * N;
-  for (int e = tid; e < 16 * N; e += 256) out[e] = red[e / N][e % N];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int n = wave * 64 + j * 16 + (lane & 15);
+    if (n < N)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) out[(size_t)MfmaT<T>::row(lane, q) * N + n] = (double)acc[j][q];
+  }
 }
 
 template <typename TP>

@@ -25,6 +25,7 @@
 #include "gdx_functionals.h"
 #include "gdx_profile.h"
 #include "gdx_gemm.h"
+#include "gdx_gemm_bf16.h"
 #include "gdx_skinny.h"
 #include "gdx_aogemm.h"
 
@@ -163,9 +164,9 @@ __global__ __launch_bounds__(NT) void layer0_kernel(const T* __restrict__ X, int
   if (i >= total) return;
   const long r = i / h;
   const int f = (int)(i % h);
-  double z = b[f];
-  for (int j = 0; j < nc; ++j) z += (double)X[r * 16 + j] * W[j * h + f];
-  H[i] = (T)(z > 0.0 ? z : expm1(z));
+  T z = (T)b[f];
+  for (int j = 0; j < nc; ++j) z += X[r * 16 + j] * (T)W[j * h + f];
+  H[i] = elu_any(z);
 }
 
 // ------------------------------------------------- guarded GEMM (any shape, small) ----
@@ -305,7 +306,8 @@ __global__ __launch_bounds__(NT) void head_seed_kernel(
 }
 
 // G[r][f] = (sum_o DZ16[r][o] W_last[f][o]) * elu'(H[r][f]), one thread per element
-template <typename T>
+// (RAW: without the elu' factor)
+template <typename T, bool RAW = false>
 __global__ __launch_bounds__(NT) void head_delta_kernel(const T* __restrict__ DZ16, const double* __restrict__ Wl,
                                                         int nd, int h, const T* __restrict__ H, long total,
                                                         T* __restrict__ G) {
@@ -313,9 +315,14 @@ __global__ __launch_bounds__(NT) void head_delta_kernel(const T* __restrict__ DZ
   if (i >= total) return;
   const long r = i / h;
   const int f = (int)(i % h);
-  double v = 0.0;
-  for (int o = 0; o < nd; ++o) v += (double)DZ16[r * 16 + o] * Wl[f * nd + o];
-  G[i] = (T)(v * elu_grad_from_out((double)H[i]));
+  T v = 0;
+  for (int o = 0; o < nd; ++o) v += DZ16[r * 16 + o] * (T)Wl[f * nd + o];
+  if (RAW) {
+    G[i] = v;
+    return;
+  }
+  const T y = H[i];
+  G[i] = v * (y > (T)0 ? (T)1 : y + (T)1);
 }
 
 // d_params of the last layer: W (h, nd) from the reduced transposed (16 x h) tile
@@ -378,36 +385,265 @@ __global__ __launch_bounds__(NT) void point_bwd_kernel(const double* __restrict_
     for (int q = 0; q < VPT_STRIDE; ++q) vpt[((size_t)s * npts_pad + r) * VPT_STRIDE + q] = out[s][q];
 }
 
+// ------------------------------------------------- second order (HVP) kernels ----
+//
+// For a direction dD (2, n, n) the HVP pass differentiates
+//   Phi(theta, D) = g E(theta, D) + <dD, dE/dD>
+// where <dD, dE/dD> = Edot is the forward-mode derivative of E along dD.  Tangents (dot)
+// are carried through densities, energy densities, features and the MLP; the reverse sweep
+// then runs on stacked [value; tangent] rows (2 N_pad rows per activation buffer).
+
+__device__ __forceinline__ Dual<1> tan1(double v, double t) {
+  Dual<1> x(0.0);
+  x.v = v;
+  x.d[0] = t;
+  return x;
+}
+
+// Final tangent densities from the raw sums of dD; energy-density tangents (forward mode
+// along the tangent of (rho, |grad rho|^2, lapl) per spin); MLP input tangents (Xdot16,
+// zero bias column).
+template <typename T>
+__global__ __launch_bounds__(NT) void point_tangent_kernel(const double* __restrict__ dpart_t,
+                                                           const double* __restrict__ dens_fin, int npts_pad,
+                                                           int neural, double* __restrict__ dfin_t,
+                                                           double* __restrict__ Edot, T* __restrict__ Xdot) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= npts_pad) return;
+  double u[2][DENS_STRIDE], t[2][DENS_STRIDE];
+  for (int s = 0; s < 2; ++s) {
+    const double* raw = dpart_t + ((size_t)s * npts_pad + r) * DENS_STRIDE;
+    t[s][0] = raw[0];
+    t[s][1] = 2.0 * raw[1];
+    t[s][2] = 2.0 * raw[2];
+    t[s][3] = 2.0 * raw[3];
+    t[s][4] = 0.5 * raw[4];
+    t[s][5] = 2.0 * raw[4] + 2.0 * raw[5];
+#pragma unroll
+    for (int q = 0; q < DENS_STRIDE; ++q) {
+      dfin_t[((size_t)r * 2 + s) * DENS_STRIDE + q] = t[s][q];
+      u[s][q] = dens_fin[((size_t)r * 2 + s) * DENS_STRIDE + q];
+    }
+  }
+  double gam[2], gamt[2];
+  for (int s = 0; s < 2; ++s) {
+    gam[s] = u[s][1] * u[s][1] + u[s][2] * u[s][2] + u[s][3] * u[s][3];
+    gamt[s] = 2.0 * (u[s][1] * t[s][1] + u[s][2] * t[s][2] + u[s][3] * t[s][3]);
+  }
+  Dual<1> e[4];
+  b3lyp_parts<Dual<1>>(tan1(u[0][0], t[0][0]), tan1(u[1][0], t[1][0]), tan1(gam[0], gamt[0]),
+                       tan1(gam[1], gamt[1]), tan1(u[0][5], t[0][5]), tan1(u[1][5], t[1][5]), e);
+#pragma unroll
+  for (int f = 0; f < 4; ++f) Edot[(size_t)r * 4 + f] = e[f].d[0];
+  if (neural) {
+    const double feat[6] = {u[0][0], u[1][0], gam[0], gam[1], u[0][4], u[1][4]};
+    const double featt[6] = {t[0][0], t[1][0], gamt[0], gamt[1], t[0][4], t[1][4]};
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const double sg = feat[j] > 0.0 ? 1.0 : (feat[j] < 0.0 ? -1.0 : 0.0);
+      Xdot[(size_t)r * 16 + j] = (T)(sg * featt[j] / (fabs(feat[j]) + SQUASH));
+    }
+#pragma unroll
+    for (int j = 6; j < 16; ++j) Xdot[(size_t)r * 16 + j] = (T)0;
+  }
+}
+
+// Hdot0 = (Xdot16[:, :nc] W0) * elu'(z0), elu'(z0) from H0 = elu(z0)
+template <typename T>
+__global__ __launch_bounds__(NT) void layer0_tangent_kernel(const T* __restrict__ Xd, int nc, int h, long total,
+                                                            const double* __restrict__ W, const T* __restrict__ H0,
+                                                            T* __restrict__ Hd0) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long r = i / h;
+  const int f = (int)(i % h);
+  T z = 0;
+  for (int j = 0; j < nc; ++j) z += Xd[r * 16 + j] * (T)W[j * h + f];
+  const T y = H0[i];
+  Hd0[i] = z * (y > (T)0 ? (T)1 : y + (T)1);
+}
+
+// Seeds of the HVP reverse sweep, one thread per point.  With m = clip(w) [|eps| > 1e-30],
+// c = 2 sigmoid(z/2), c' = (c/2)(1 - c/2), c'' = c'(1 - c)/2, cdot = c' zdot:
+//   Edot    += m sum_o (cdot_o e_o + c_o edot_o)                     (block partials)
+//   GE_o     = dPhi/de_o    = m (g c_o + cdot_o),   GEd_o = dPhi/dedot_o = m c_o
+//   zbar_o   = m (g e_o + edot_o) c'_o + m e_o c''_o zdot_o,  zdotbar_o = m e_o c'_o
+// DZ2 rows [0, N_pad) hold zbar, rows [N_pad, 2 N_pad) zdotbar (zero-padded to 16).
+template <typename T>
+__global__ __launch_bounds__(NT) void hvp_seed_kernel(
+    const T* __restrict__ Cc, const T* __restrict__ ZT, const double* __restrict__ E,
+    const double* __restrict__ Edot, const double* __restrict__ weights, int nd, int npts, int npts_pad,
+    const double* __restrict__ g_exc, int neural, double* __restrict__ GE, double* __restrict__ GEd,
+    T* __restrict__ DZ2, double* __restrict__ dbpart, double* __restrict__ edot_part) {
+  __shared__ double red[NT / 64];
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const double bc[4] = {1.0 - 0.2, 0.72, 1.0 - 0.81, 0.81};
+  double zb[MAXND];
+  for (int o = 0; o < MAXND; ++o) zb[o] = 0.0;
+  double ed_contrib = 0.0;
+  if (r < npts_pad) {
+    double c[MAXND], e[MAXND], et[MAXND], cp[MAXND], zt[MAXND];
+    double eps = 0.0;
+    for (int o = 0; o < nd; ++o) {
+      c[o] = r >= npts ? 0.0 : (neural ? (double)Cc[(size_t)r * nd + o] : bc[o]);
+      e[o] = E[(size_t)r * nd + o];
+      et[o] = Edot[(size_t)r * nd + o];
+      cp[o] = neural ? 0.5 * c[o] * (1.0 - 0.5 * c[o]) : 0.0;
+      zt[o] = (neural && r < npts) ? (double)ZT[(size_t)r * nd + o] : 0.0;
+      eps += c[o] * e[o];
+    }
+    const double m = (r < npts && fabs(eps) > 1e-30) ? clip30(weights[r]) : 0.0;
+    const double g = *g_exc;
+    double zdb[MAXND];
+    for (int o = 0; o < nd; ++o) {
+      const double cd = cp[o] * zt[o];
+      ed_contrib += m * (cd * e[o] + c[o] * et[o]);
+      GE[(size_t)r * nd + o] = m * (g * c[o] + cd);
+      GEd[(size_t)r * nd + o] = m * c[o];
+      const double cpp = 0.5 * cp[o] * (1.0 - c[o]);
+      zb[o] = m * (g * e[o] + et[o]) * cp[o] + m * e[o] * cpp * zt[o];
+      zdb[o] = m * e[o] * cp[o];
+    }
+    if (neural)
+      for (int o = 0; o < 16; ++o) {
+        DZ2[(size_t)r * 16 + o] = (T)(o < nd ? zb[o] : 0.0);
+        DZ2[((size_t)npts_pad + r) * 16 + o] = (T)(o < nd ? zdb[o] : 0.0);
+      }
+  }
+  if (neural)
+    for (int o = 0; o < nd; ++o) {
+      const double t = block_sum(zb[o], red);
+      if (threadIdx.x == 0) dbpart[(size_t)blockIdx.x * nd + o] = t;
+    }
+  const double t = block_sum(ed_contrib, red);
+  if (threadIdx.x == 0) edot_part[blockIdx.x] = t;
+}
+
+// Stacked activation adjoints -> pre-activation adjoints, in place.  With a = elu(z),
+// adot = elu'(z) zdot, G2 = [abar; adotbar] (raw products with W^T), H2 = [a; adot]:
+//   zbar    = abar elu'(z) + adotbar elu''(z) zdot = abar elu'(z) + [a <= 0] adotbar adot
+//   zdotbar = adotbar elu'(z)
+template <typename T>
+__global__ __launch_bounds__(NT) void hvp_combine_kernel(T* __restrict__ G2, const T* __restrict__ H2, long half) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= half) return;
+  const T a = H2[i], ad = H2[half + i];
+  const T gb = G2[i], gd = G2[half + i];
+  const bool pos = a > (T)0;
+  const T ep = pos ? (T)1 : a + (T)1;
+  G2[i] = gb * ep + (pos ? (T)0 : gd * ad);
+  G2[half + i] = gd * ep;
+}
+
+// Per-point potentials of the HVP: the gradient of
+//   Phi_r(u) = sum_o [GE_o e_o + GEd_o edot_o] + sum_j [xbar_j x_j + xdotbar_j xdot_j]
+// with respect to the final densities u of the point (tangents udot fixed), mapped to the
+// raw-sum layout consumed by vxc_ao_kernel (as point_bwd_kernel).  Second derivatives of the
+// energy densities come from Dual<6, Dual<1>>: d[i].v = de/dq_i, d[i].d[0] = (H qdot)_i.
+template <typename T>
+__global__ __launch_bounds__(NT) void point_hvp_bwd_kernel(const double* __restrict__ dens_fin,
+                                                           const double* __restrict__ dfin_t,
+                                                           const double* __restrict__ GE,
+                                                           const double* __restrict__ GEd, const T* __restrict__ DX2,
+                                                           int nc, int npts, int npts_pad, double* __restrict__ vpt) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= npts_pad) return;
+  double out[2][VPT_STRIDE];
+  for (int s = 0; s < 2; ++s)
+    for (int q = 0; q < VPT_STRIDE; ++q) out[s][q] = 0.0;
+  if (r < npts) {
+    double u[2][DENS_STRIDE], t[2][DENS_STRIDE];
+    for (int s = 0; s < 2; ++s)
+      for (int q = 0; q < DENS_STRIDE; ++q) {
+        u[s][q] = dens_fin[((size_t)r * 2 + s) * DENS_STRIDE + q];
+        t[s][q] = dfin_t[((size_t)r * 2 + s) * DENS_STRIDE + q];
+      }
+    double gam[2], gamt[2];
+    for (int s = 0; s < 2; ++s) {
+      gam[s] = u[s][1] * u[s][1] + u[s][2] * u[s][2] + u[s][3] * u[s][3];
+      gamt[s] = 2.0 * (u[s][1] * t[s][1] + u[s][2] * t[s][2] + u[s][3] * t[s][3]);
+    }
+    typedef Dual<6, Dual<1>> H6;
+    H6 e[4];
+    b3lyp_parts<H6>(dvar<6, Dual<1>>(tan1(u[0][0], t[0][0]), 0), dvar<6, Dual<1>>(tan1(u[1][0], t[1][0]), 1),
+                    dvar<6, Dual<1>>(tan1(gam[0], gamt[0]), 2), dvar<6, Dual<1>>(tan1(gam[1], gamt[1]), 3),
+                    dvar<6, Dual<1>>(tan1(u[0][5], t[0][5]), 4), dvar<6, Dual<1>>(tan1(u[1][5], t[1][5]), 5), e);
+    // P_i = dPhi/dq_i, Pd_i = dPhi/dqdot_i over q = (rho_u, rho_d, gam_u, gam_d, lapl_u, lapl_d)
+    double P[6] = {0, 0, 0, 0, 0, 0}, Pd[6] = {0, 0, 0, 0, 0, 0};
+    for (int f = 0; f < 4; ++f) {
+      const double ge = GE[(size_t)r * 4 + f], ged = GEd[(size_t)r * 4 + f];
+      for (int i = 0; i < 6; ++i) {
+        P[i] += ge * e[f].d[i].v + ged * e[f].d[i].d[0];
+        Pd[i] += ged * e[f].d[i].v;
+      }
+    }
+    // Q_j = dPhi/df_j, Qd_j = dPhi/dfdot_j over f = (rho_u, rho_d, gam_u, gam_d, tau_u, tau_d)
+    double Q[6] = {0, 0, 0, 0, 0, 0}, Qd[6] = {0, 0, 0, 0, 0, 0};
+    if (DX2 != nullptr) {
+      const double feat[6] = {u[0][0], u[1][0], gam[0], gam[1], u[0][4], u[1][4]};
+      const double featt[6] = {t[0][0], t[1][0], gamt[0], gamt[1], t[0][4], t[1][4]};
+      for (int j = 0; j < 6; ++j) {
+        const double xb = (double)DX2[(size_t)r * nc + j];
+        const double xdb = (double)DX2[((size_t)npts_pad + r) * nc + j];
+        const double sg = feat[j] > 0.0 ? 1.0 : (feat[j] < 0.0 ? -1.0 : 0.0);
+        const double inv = 1.0 / (fabs(feat[j]) + SQUASH);
+        Q[j] = xb * sg * inv - xdb * sg * sg * featt[j] * inv * inv;
+        Qd[j] = xdb * sg * inv;
+      }
+    }
+    for (int s = 0; s < 2; ++s) {
+      const double vr = P[s] + Q[s];
+      const double vg = P[2 + s] + Q[2 + s], vgd = Pd[2 + s] + Qd[2 + s];
+      const double vl = P[4 + s], vt = Q[4 + s];
+      out[s][0] = vr;
+      for (int j = 0; j < 3; ++j) out[s][1 + j] = 4.0 * (vg * u[s][1 + j] + vgd * t[s][1 + j]);
+      out[s][4] = 2.0 * vl;
+      out[s][5] = 0.5 * vt + 2.0 * vl;
+    }
+  }
+  for (int s = 0; s < 2; ++s)
+    for (int q = 0; q < VPT_STRIDE; ++q) vpt[((size_t)s * npts_pad + r) * VPT_STRIDE + q] = out[s][q];
+}
+
 // ------------------------------------------------------------ workspace layout ----
 constexpr int VCHUNK = 512;    // points per V_xc split
 constexpr int SKW_ROWS = 512;  // grid points per block of the skinny weight-gradient kernel
 
+// hvp = true: activation-side buffers hold stacked [value; tangent] rows (R = 2 N_pad), plus
+// the tangent buffers of the HVP pass.
 struct Layout {
   size_t Dp, dens_part, dens_fin, E, X, C, H, eps_part, GE, DZ, W16, Wlp, dbpart, dWt16, G, DX, vpt, vpart,
       wpart, cpart;
+  size_t Dp2, dpart_t, dfin_t, Edot, ZT, GEd, edot_part, scal;
   size_t total;
-  int nsd, nchunk, chunk, nsplit, kchunk, npt_blocks, nwblk, wrows;
+  size_t hstride;  // elements between the activation buffers of consecutive layers
+  int nsd, nchunk, chunk, nsplit, kchunk, npt_blocks, nwblk, wrows, rows;
   size_t nparam;
 };
 
 size_t elem_size(int precision) { return precision == GDX_FP64 ? 8 : 4; }
+bool mlp_bf16(const gdx_xc_desc& d) { return d.precision == GDX_BF16X6; }
 
-Layout make_layout(const gdx_xc_desc& d) {
+Layout make_layout(const gdx_xc_desc& d, bool hvp = false) {
   Layout L{};
   const size_t Np = d.npts_pad, nP = d.nao_pad;
+  const size_t R = hvp ? 2 * Np : Np;  // rows of the activation-side buffers
   const size_t es = elem_size(d.precision);
   const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
   L.nsd = d.spin_restricted ? 1 : 2;
+  L.rows = (int)R;
   L.chunk = VCHUNK;
   L.nchunk = (int)((Np + VCHUNK - 1) / VCHUNK);
-  // split-K for the hidden weight-gradient GEMMs: whole 16-row stages per split
+  // split-K for the hidden weight-gradient GEMMs: whole 32-row stages per split (for
+  // hvp, an even count so the first half of the splits covers the value rows exactly)
   L.nsplit = 1;
-  while (L.nsplit < 128 && (Np / GT_K) % (2 * L.nsplit) == 0) L.nsplit *= 2;
-  L.kchunk = (int)(Np / L.nsplit);
+  while (L.nsplit < 128 && (R / GB_K) % (2 * L.nsplit) == 0) L.nsplit *= 2;
+  L.kchunk = (int)(R / L.nsplit);
   L.npt_blocks = (int)((Np + NT - 1) / NT);
   L.wrows = SKW_ROWS;
-  L.nwblk = (int)((Np + SKW_ROWS - 1) / SKW_ROWS);
+  L.nwblk = (int)((R + SKW_ROWS - 1) / SKW_ROWS);
   const size_t h = neural ? d.mlp_h : 0, nh = neural ? d.mlp_nhidden : 0;
+  L.hstride = R * h;
   size_t np = 0;
   if (neural) {
     int din = d.mlp_nc;
@@ -424,18 +660,18 @@ Layout make_layout(const gdx_xc_desc& d) {
   L.dens_part = take((size_t)L.nsd * Np * DENS_STRIDE * 8);
   L.dens_fin = take(Np * 2 * DENS_STRIDE * 8);
   L.E = take(Np * 4 * 8);
-  L.X = take(neural ? Np * 16 * es : 0);
+  L.X = take(neural ? R * 16 * es : 0);
   L.C = take(neural ? Np * d.mlp_nd * es : 0);
-  L.H = take(Np * h * nh * es);
+  L.H = take(R * h * nh * es);
   L.eps_part = take(std::max<size_t>(Np / 64 + 1, 4096) * 8);
   L.GE = take(Np * 4 * 8);
-  L.DZ = take(neural ? Np * 16 * es : 0);
+  L.DZ = take(neural ? R * 16 * es : 0);
   L.W16 = take(neural ? 2 * 16 * h * es : 0);
   L.Wlp = take(neural && es != 8 ? np * es : 0);
   L.dbpart = take(neural ? (size_t)L.npt_blocks * d.mlp_nd * 8 : 0);
   L.dWt16 = take(neural ? 16 * h * 8 : 0);
-  L.G = take(neural ? 2 * Np * h * es : 0);
-  L.DX = take(neural ? Np * d.mlp_nc * es : 0);
+  L.G = take(neural ? 2 * R * h * es : 0);
+  L.DX = take(neural ? R * d.mlp_nc * es : 0);
   L.vpt = take(2 * Np * VPT_STRIDE * 8);
   L.vpart = take((size_t)L.nchunk * 2 * nP * nP * 8);
   size_t wbytes = 0;
@@ -445,6 +681,16 @@ Layout make_layout(const gdx_xc_desc& d) {
   }
   L.wpart = take(wbytes);
   L.cpart = take(neural ? (size_t)L.nsplit * h * es : 0);
+  if (hvp) {
+    L.Dp2 = take(2 * nP * nP * es);
+    L.dpart_t = take(2 * Np * DENS_STRIDE * 8);
+    L.dfin_t = take(Np * 2 * DENS_STRIDE * 8);
+    L.Edot = take(Np * 4 * 8);
+    L.ZT = take(neural ? Np * d.mlp_nd * es : 0);
+    L.GEd = take(Np * 4 * 8);
+    L.edot_part = take((size_t)L.npt_blocks * 8);
+    L.scal = take(2 * 8);
+  }
   L.total = off;
   return L;
 }
@@ -455,8 +701,8 @@ int validate(const gdx_xc_desc* d) {
   GDX_REQUIRE(d->npts_pad >= d->npts && d->npts_pad % 64 == 0, "npts_pad must be a multiple of 64 >= npts");
   GDX_REQUIRE(d->nao_pad >= d->nao && d->nao_pad % 16 == 0, "nao_pad must be a multiple of 16 >= nao");
   GDX_REQUIRE(d->kind == GDX_FUNC_B3LYP || d->kind == GDX_FUNC_NEURAL_B3LYP, "unknown functional kind");
-  if (d->precision != GDX_FP64 && d->precision != GDX_FP32) {
-    set_error("precision not implemented by this build (GDX_FP64, GDX_FP32 are)");
+  if (d->precision != GDX_FP64 && d->precision != GDX_FP32 && d->precision != GDX_BF16X6) {
+    set_error("unknown precision (GDX_FP64, GDX_FP32, GDX_BF16X6)");
     return GDX_ERR_UNSUPPORTED;
   }
   if (d->kind == GDX_FUNC_NEURAL_B3LYP) {
@@ -484,8 +730,21 @@ void launch_any(hipStream_t st, const char* name, int M, int N, int K, const T* 
 
 template <typename T, bool TA, bool TB, int EPI>
 void launch_tile(hipStream_t st, const char* name, int M, int N, int nsplit, int kchunk, const T* A, int lda,
-                 const T* B, int ldb, T* C, int ldc, const T* bias, const T* Y, int ldy, T* colsum) {
+                 const T* B, int ldb, T* C, int ldc, const T* bias, const T* Y, int ldy, T* colsum,
+                 bool bf16 = false) {
   dim3 grid((M / GT_M) * (N / GT_N), nsplit);
+  if constexpr (sizeof(T) == 4) {
+    if (bf16) {
+      char nm[64];
+      snprintf(nm, sizeof(nm), "%s", name);
+      // "gemm_f32_*" -> "gemm_bf16x6_*" in the timing table
+      const char* tag = TA ? "gemm_bf16x6_wgrad" : (EPI == EPI_BIAS_ELU ? "gemm_bf16x6_fwd" : "gemm_bf16x6_bwd_data");
+      GDX_LAUNCH(tag, (gemm_bf16x6_kernel<TA, TB, EPI>), grid, dim3(NT), 0, st, M, N, A, lda, B, ldb, C, ldc, bias,
+                 Y, ldy, kchunk, colsum);
+      (void)nm;
+      return;
+    }
+  }
   GDX_LAUNCH(name, (gemm_tile_kernel<T, TA, TB, EPI>), grid, dim3(NT), 0, st, M, N, A, lda, B, ldb, C, ldc,
              bias, Y, ldy, kchunk, colsum);
 }
@@ -562,11 +821,11 @@ int forward_impl(const Ctx& c, const T* ao, const double* weights, const double*
     for (int l = 1; l < d.mlp_nhidden; ++l) {
       const T* Wl = plp + po.w[l];
       const T* bl = plp + po.b[l];
-      const T* Hin = H + (size_t)(l - 1) * Np * h;
-      T* Hout = H + (size_t)l * Np * h;
+      const T* Hin = H + (size_t)(l - 1) * L.hstride;
+      T* Hout = H + (size_t)l * L.hstride;
       if (tiles)
         launch_tile<T, false, false, EPI_BIAS_ELU>(st, tname<T>("gemm_f64_fwd", "gemm_f32_fwd"), Np, h, 1, h, Hin,
-                                                   h, Wl, h, Hout, h, bl, nullptr, 0, nullptr);
+                                                   h, Wl, h, Hout, h, bl, nullptr, 0, nullptr, mlp_bf16(d));
       else
         launch_any<T, 128, 128, false, false, EPI_BIAS_ELU>(st, "gemm_any", Np, h, h, Hin, h, Wl, h, Hout, h, bl,
                                                             nullptr, 0, 1, h, nullptr);
@@ -577,7 +836,7 @@ int forward_impl(const Ctx& c, const T* ao, const double* weights, const double*
     GDX_LAUNCH("prep_edge_weights", prep_edge_weights_kernel<T>, dim3(nblocks(16L * h)), dim3(NT), 0, st,
                params + po.w[0], d.mlp_nc, params + po.w[nl], d.mlp_nd, h, W016, WT16);
     GDX_LAUNCH("head_fwd", (skinny_out16_kernel<T, true>), dim3(Np / 64), dim3(NT), 0, st, Np, h,
-               H + (size_t)(nl - 1) * Np * h, h, WT16, c.P<T>(L.C), d.mlp_nd, d.mlp_nd, plp + po.b[nl], d.mlp_nd,
+               H + (size_t)(nl - 1) * L.hstride, h, WT16, c.P<T>(L.C), d.mlp_nd, d.mlp_nd, plp + po.b[nl], d.mlp_nd,
                c.P<double>(L.E), weights, d.npts, c.P<double>(L.eps_part));
     nred = Np / 64;
   }
@@ -605,7 +864,7 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
     T* H = c.P<T>(L.H);
     T* G0 = c.P<T>(L.G);
     T* G1 = G0 + (size_t)Np * h;
-    const T* Hlast = H + (size_t)(nl - 1) * Np * h;
+    const T* Hlast = H + (size_t)(nl - 1) * L.hstride;
     const T* W016 = c.P<T>(L.W16);
     GDX_LAUNCH("head_seed", head_seed_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, c.P<T>(L.C),
                c.P<double>(L.E), weights, nd, d.npts, Np, g_exc, 1, c.P<double>(L.GE), c.P<T>(L.DZ),
@@ -625,14 +884,15 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
     T* Gcur = G0;
     T* Gnext = G1;
     for (int l = nl - 1; l >= 1; --l) {
-      const T* Hin = H + (size_t)(l - 1) * Np * h;  // input activations of layer l
+      const T* Hin = H + (size_t)(l - 1) * L.hstride;  // input activations of layer l
       const T* Wl = plp + po.w[l];
       if (d_params) {
         T* wp = c.P<T>(L.wpart);
         T* cp = c.P<T>(L.cpart);
         if (tiles)
           launch_tile<T, true, false, EPI_STORE>(st, tname<T>("gemm_f64_wgrad", "gemm_f32_wgrad"), h, h, L.nsplit,
-                                                 L.kchunk, Hin, h, Gcur, h, wp, h, nullptr, nullptr, 0, cp);
+                                                 L.kchunk, Hin, h, Gcur, h, wp, h, nullptr, nullptr, 0, cp,
+                                                 mlp_bf16(d));
         else
           launch_any<T, 64, 64, true, false, EPI_STORE>(st, "gemm_any", h, h, Np, Hin, h, Gcur, h, wp, h, nullptr,
                                                         nullptr, 0, L.nsplit, L.kchunk, cp);
@@ -644,7 +904,7 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
       // delta for layer l's input: (Gcur W_l^T) * elu'(Hin)
       if (tiles)
         launch_tile<T, false, true, EPI_ELU_BWD>(st, tname<T>("gemm_f64_bwd_data", "gemm_f32_bwd_data"), Np, h, 1,
-                                                 h, Gcur, h, Wl, h, Gnext, h, nullptr, Hin, h, nullptr);
+                                                 h, Gcur, h, Wl, h, Gnext, h, nullptr, Hin, h, nullptr, mlp_bf16(d));
       else
         launch_any<T, 128, 128, false, true, EPI_ELU_BWD>(st, "gemm_any", Np, h, h, Gcur, h, Wl, h, Gnext, h,
                                                           nullptr, Hin, h, 1, h, nullptr);
@@ -667,18 +927,165 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
   if (d_rdm1) {
     GDX_LAUNCH("point_bwd", point_bwd_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, c.P<double>(L.dens_fin),
                c.P<double>(L.GE), neural ? c.P<T>(L.DX) : nullptr, d.mlp_nc, d.npts, Np, c.P<double>(L.vpt));
-    const int nt = (nP + VX_T - 1) / VX_T;
     // B3LYP on equal spin densities has equal spin potentials: form V once
     const int nsv = (!neural && d.spin_restricted) ? 1 : 2;
     const char* nm = tname<T>("vxc_f64", "vxc_f32");
-    if (nt == 1)
-      GDX_LAUNCH(nm, (vxc_ao_kernel<T, true>), dim3(1, L.nchunk, nsv), dim3(NT), 0, st, ao, Np, nP,
+    if (nP <= 64) {
+      GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 64>), dim3(1, L.nchunk, nsv), dim3(NT), 0, st, ao, Np, nP,
                  c.P<double>(L.vpt), L.chunk, c.P<double>(L.vpart));
-    else
-      GDX_LAUNCH(nm, (vxc_ao_kernel<T, false>), dim3(nt * nt, L.nchunk, nsv), dim3(NT), 0, st, ao, Np, nP,
-                 c.P<double>(L.vpt), L.chunk, c.P<double>(L.vpart));
+    } else {
+      const int nt = (nP + 127) / 128;
+      if (nt == 1)
+        GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 128>), dim3(1, L.nchunk, nsv), dim3(NT), 0, st, ao, Np, nP,
+                   c.P<double>(L.vpt), L.chunk, c.P<double>(L.vpart));
+      else
+        GDX_LAUNCH(nm, (vxc_ao_kernel<T, false, 128>), dim3(nt * nt, L.nchunk, nsv), dim3(NT), 0, st, ao, Np, nP,
+                   c.P<double>(L.vpt), L.chunk, c.P<double>(L.vpart));
+    }
     GDX_LAUNCH("reduce_vxc", reduce_vxc_kernel, dim3(nblocks(2L * d.nao * d.nao)), dim3(NT), 0, st,
                c.P<double>(L.vpart), L.nchunk, nsv, nP, d.nao, d_rdm1);
+  }
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+// Gradient of Phi = g E + <dD, dE/dD> w.r.t. D and theta (see the HVP kernels above).
+template <typename T>
+int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm1, const double* params,
+             const double* dir, const double* g_exc, double* exc, double* edot, double* d_rdm1, double* d_params) {
+  const gdx_xc_desc& d = c.d;
+  const Layout& L = c.L;
+  hipStream_t st = c.st;
+  const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
+  const int Np = d.npts_pad, nP = d.nao_pad;
+  const int nd = 4;
+  double* scal = c.P<double>(L.scal);
+  int rc = forward_impl<T>(c, ao, weights, rdm1, params, exc ? exc : scal);
+  if (rc != GDX_OK) return rc;
+  // tangent densities along dir (both spins, general direction)
+  GDX_LAUNCH("pad_dm", pad_dm_kernel<T>, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, dir, d.nao, nP,
+             c.P<T>(L.Dp2));
+  GDX_LAUNCH(tname<T>("density_f64", "density_f32"), density_ao_kernel<T>, dim3(Np / DN_PT, 2), dim3(NT), 0, st,
+             ao, Np, nP, c.P<T>(L.Dp2), c.P<double>(L.dpart_t));
+  T* X2 = c.P<T>(L.X);
+  GDX_LAUNCH("point_tangent", point_tangent_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st,
+             c.P<double>(L.dpart_t), c.P<double>(L.dens_fin), Np, neural ? 1 : 0, c.P<double>(L.dfin_t),
+             c.P<double>(L.Edot), X2 + (size_t)Np * 16);
+  const ParamOffsets po = neural ? param_offsets(d) : ParamOffsets{};
+  const int h = neural ? d.mlp_h : 0;
+  const int nl = neural ? d.mlp_nhidden : 0;
+  const bool tiles = neural && Np % GT_M == 0 && h % GT_N == 0;
+  const T* plp = neural ? (sizeof(T) == 8 ? (const T*)params : c.P<T>(L.Wlp)) : nullptr;
+  T* H = c.P<T>(L.H);
+  const T* W016 = c.P<T>(L.W16);
+  const T* WT16 = W016 + 16 * h;
+  if (neural) {
+    // tangent MLP: Hdot_l = (Hdot_{l-1} W_l) * elu'(z_l); zdot of the head
+    GDX_LAUNCH("layer0_tangent", layer0_tangent_kernel<T>, dim3(nblocks((long)Np * h)), dim3(NT), 0, st,
+               X2 + (size_t)Np * 16, d.mlp_nc, h, (long)Np * h, params + po.w[0], H, H + (size_t)Np * h);
+    for (int l = 1; l < nl; ++l) {
+      const T* Wl = plp + po.w[l];
+      const T* Hd_in = H + (size_t)(l - 1) * L.hstride + (size_t)Np * h;
+      const T* Hl = H + (size_t)l * L.hstride;
+      T* Hd_out = H + (size_t)l * L.hstride + (size_t)Np * h;
+      if (tiles)
+        launch_tile<T, false, false, EPI_ELU_BWD>(st, tname<T>("gemm_f64_tangent", "gemm_f32_tangent"), Np, h, 1,
+                                                  h, Hd_in, h, Wl, h, Hd_out, h, nullptr, Hl, h, nullptr,
+                                                  mlp_bf16(d));
+      else
+        launch_any<T, 128, 128, false, false, EPI_ELU_BWD>(st, "gemm_any", Np, h, h, Hd_in, h, Wl, h, Hd_out, h,
+                                                           nullptr, Hl, h, 1, h, nullptr);
+    }
+    GDX_LAUNCH("head_tangent", (skinny_out16_kernel<T, false>), dim3(Np / 64), dim3(NT), 0, st, Np, h,
+               H + (size_t)(nl - 1) * L.hstride + (size_t)Np * h, h, WT16, c.P<T>(L.ZT), nd, nd, nullptr, 0,
+               nullptr, nullptr, 0, nullptr);
+  }
+  GDX_LAUNCH("hvp_seed", hvp_seed_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, neural ? c.P<T>(L.C) : nullptr,
+             neural ? c.P<T>(L.ZT) : nullptr, c.P<double>(L.E), c.P<double>(L.Edot), weights, nd, d.npts, Np, g_exc,
+             neural ? 1 : 0, c.P<double>(L.GE), c.P<double>(L.GEd), neural ? c.P<T>(L.DZ) : nullptr,
+             c.P<double>(L.dbpart), c.P<double>(L.edot_part));
+  if (edot)
+    GDX_LAUNCH("reduce_scalar", reduce_scalar_kernel, dim3(1), dim3(NT), 0, st, c.P<double>(L.edot_part),
+               L.npt_blocks, edot);
+  if (neural) {
+    const int R = 2 * Np;
+    const long half = (long)Np * h;
+    T* G0 = c.P<T>(L.G);
+    T* G1 = G0 + (size_t)R * h;
+    const T* Hlast2 = H + (size_t)(nl - 1) * L.hstride;
+    T* DZ2 = c.P<T>(L.DZ);
+    GDX_LAUNCH("head_delta", (head_delta_kernel<T, true>), dim3(nblocks((long)R * h)), dim3(NT), 0, st, DZ2,
+               params + po.w[nl], nd, h, nullptr, (long)R * h, G0);
+    GDX_LAUNCH("hvp_combine", hvp_combine_kernel<T>, dim3(nblocks(half)), dim3(NT), 0, st, G0, Hlast2, half);
+    if (d_params) {
+      GDX_LAUNCH("skinny_wgrad", skinny_wgrad_kernel<T>, dim3(L.nwblk), dim3(NT), 0, st, R, L.wrows, DZ2, Hlast2, h,
+                 c.P<double>(L.wpart));
+      GDX_LAUNCH("reduce_cols", reduce_cols_kernel<double>, dim3(nblocks64(16L * h)), dim3(NT), 0, st,
+                 c.P<double>(L.wpart), L.nwblk, 16L * h, 16L * h, c.P<double>(L.dWt16));
+      GDX_LAUNCH("unpack_last", unpack_last_kernel, dim3(nblocks((long)h * nd)), dim3(NT), 0, st,
+                 c.P<double>(L.dWt16), h, nd, d_params + po.w[nl]);
+      GDX_LAUNCH("reduce_cols", reduce_cols_kernel<double>, dim3(1), dim3(NT), 0, st, c.P<double>(L.dbpart),
+                 L.npt_blocks, (long)nd, (long)nd, d_params + po.b[nl]);
+    }
+    T* Gcur = G0;
+    T* Gnext = G1;
+    for (int l = nl - 1; l >= 1; --l) {
+      const T* Hin2 = H + (size_t)(l - 1) * L.hstride;  // [a_{l-1}; adot_{l-1}]
+      const T* Wl = plp + po.w[l];
+      if (d_params) {
+        T* wp = c.P<T>(L.wpart);
+        T* cp = c.P<T>(L.cpart);
+        if (tiles)
+          launch_tile<T, true, false, EPI_STORE>(st, tname<T>("gemm_f64_wgrad", "gemm_f32_wgrad"), h, h, L.nsplit,
+                                                 L.kchunk, Hin2, h, Gcur, h, wp, h, nullptr, nullptr, 0, cp,
+                                                 mlp_bf16(d));
+        else
+          launch_any<T, 64, 64, true, false, EPI_STORE>(st, "gemm_any", h, h, R, Hin2, h, Gcur, h, wp, h, nullptr,
+                                                        nullptr, 0, L.nsplit, L.kchunk, cp);
+        GDX_LAUNCH("reduce_cols", reduce_cols_kernel<T>, dim3(nblocks64((long)h * h)), dim3(NT), 0, st, wp, L.nsplit,
+                   (long)h * h, (long)h * h, d_params + po.w[l]);
+        // bias: column sums over the value rows only (the first half of the splits)
+        GDX_LAUNCH("reduce_cols", reduce_cols_kernel<T>, dim3(nblocks64(h)), dim3(NT), 0, st, cp, L.nsplit / 2,
+                   (long)h, (long)h, d_params + po.b[l]);
+      }
+      if (tiles)
+        launch_tile<T, false, true, EPI_STORE>(st, tname<T>("gemm_f64_bwd_data", "gemm_f32_bwd_data"), R, h, 1, h,
+                                               Gcur, h, Wl, h, Gnext, h, nullptr, nullptr, 0, nullptr, mlp_bf16(d));
+      else
+        launch_any<T, 128, 128, false, true, EPI_STORE>(st, "gemm_any", R, h, h, Gcur, h, Wl, h, Gnext, h, nullptr,
+                                                        nullptr, 0, 1, h, nullptr);
+      GDX_LAUNCH("hvp_combine", hvp_combine_kernel<T>, dim3(nblocks(half)), dim3(NT), 0, st, Gnext, Hin2, half);
+      std::swap(Gcur, Gnext);
+    }
+    // first layer: [xbar; xdotbar] = G W0^T; dW0, db0 = [X16; Xdot16]^T G
+    GDX_LAUNCH("layer0_bwd", (skinny_out16_kernel<T, false>), dim3(R / 64), dim3(NT), 0, st, R, h, Gcur, h, W016,
+               c.P<T>(L.DX), d.mlp_nc, d.mlp_nc, nullptr, 0, nullptr, nullptr, 0, nullptr);
+    if (d_params) {
+      GDX_LAUNCH("skinny_wgrad", skinny_wgrad_kernel<T>, dim3(L.nwblk), dim3(NT), 0, st, R, L.wrows, X2, Gcur, h,
+                 c.P<double>(L.wpart));
+      GDX_LAUNCH("reduce_cols", reduce_cols_kernel<double>, dim3(nblocks64((long)(d.mlp_nc + 1) * h)), dim3(NT), 0,
+                 st, c.P<double>(L.wpart), L.nwblk, 16L * h, (long)(d.mlp_nc + 1) * h, d_params + po.w[0]);
+    }
+  }
+  if (d_rdm1) {
+    GDX_LAUNCH("point_hvp_bwd", point_hvp_bwd_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st,
+               c.P<double>(L.dens_fin), c.P<double>(L.dfin_t), c.P<double>(L.GE), c.P<double>(L.GEd),
+               neural ? c.P<T>(L.DX) : nullptr, d.mlp_nc, d.npts, Np, c.P<double>(L.vpt));
+    const char* nm = tname<T>("vxc_f64", "vxc_f32");
+    if (nP <= 64) {
+      GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 64>), dim3(1, L.nchunk, 2), dim3(NT), 0, st, ao, Np, nP,
+                 c.P<double>(L.vpt), L.chunk, c.P<double>(L.vpart));
+    } else {
+      const int nt = (nP + 127) / 128;
+      if (nt == 1)
+        GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 128>), dim3(1, L.nchunk, 2), dim3(NT), 0, st, ao, Np, nP,
+                   c.P<double>(L.vpt), L.chunk, c.P<double>(L.vpart));
+      else
+        GDX_LAUNCH(nm, (vxc_ao_kernel<T, false, 128>), dim3(nt * nt, L.nchunk, 2), dim3(NT), 0, st, ao, Np, nP,
+                   c.P<double>(L.vpt), L.chunk, c.P<double>(L.vpart));
+    }
+    GDX_LAUNCH("reduce_vxc", reduce_vxc_kernel, dim3(nblocks(2L * d.nao * d.nao)), dim3(NT), 0, st,
+               c.P<double>(L.vpart), L.nchunk, 2, nP, d.nao, d_rdm1);
   }
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
@@ -722,6 +1129,28 @@ int gdx_xc_densities(void* stream, const gdx_xc_desc* desc, const void* ao, cons
   else run_densities<float>(c, (const float*)ao, rdm1, nullptr, dens);
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
+}
+
+int gdx_xc_hvp_workspace_size(const gdx_xc_desc* desc, size_t* bytes) {
+  int rc = validate(desc);
+  if (rc != GDX_OK) return rc;
+  GDX_REQUIRE(bytes != nullptr, "null output pointer");
+  *bytes = make_layout(*desc, true).total;
+  return GDX_OK;
+}
+
+int gdx_xc_hvp(void* stream, const gdx_xc_desc* desc, const void* ao, const double* weights, const double* rdm1,
+               const double* params, const double* dir, const double* g_exc, double* exc, double* edot,
+               double* d_rdm1, double* d_params, void* workspace) {
+  int rc = validate(desc);
+  if (rc != GDX_OK) return rc;
+  const bool neural = desc->kind == GDX_FUNC_NEURAL_B3LYP;
+  GDX_REQUIRE(ao && weights && rdm1 && dir && g_exc && workspace, "gdx_xc_hvp: null pointer");
+  GDX_REQUIRE(!neural || params, "gdx_xc_hvp: neural functional needs params");
+  Ctx c{*desc, make_layout(*desc, true), (char*)workspace, (hipStream_t)stream};
+  if (desc->precision == GDX_FP64)
+    return hvp_impl<double>(c, (const double*)ao, weights, rdm1, params, dir, g_exc, exc, edot, d_rdm1, d_params);
+  return hvp_impl<float>(c, (const float*)ao, weights, rdm1, params, dir, g_exc, exc, edot, d_rdm1, d_params);
 }
 
 int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const void* ao, const double* weights,

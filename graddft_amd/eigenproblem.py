@@ -4,7 +4,8 @@
 (``eigenproblem.py:110-129``); ``safe_fock_solver`` applies it per spin (``:132-149``).
 Cholesky, the triangular inverse and the eigensolver are hand-written kernels
 (``gdx_cholesky_inv``, ``gdx_eigh_batched``); the two congruence products are plain GEMMs.
-The Lorentzian-broadened reverse mode of ``safe_eigh`` (``:54-103``) is :func:`safe_eigh_vjp`.
+The Lorentzian-broadened reverse mode of ``safe_eigh`` (``:54-103``) is :func:`safe_eigh_vjp`,
+attached to the device eigensolver by :class:`EighFn` when autograd is recording.
 """
 from __future__ import annotations
 
@@ -53,12 +54,36 @@ def eigh(A: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
     return w.reshape(shape[:-1]), V.reshape(shape)
 
 
+class EighFn(torch.autograd.Function):
+    """Batched eigh on the device with the reverse mode of safe_eigh (safe_eigh_vjp)."""
+
+    @staticmethod
+    def forward(ctx, A):
+        w, V = eigh(A)
+        ctx.save_for_backward(w, V)
+        return w, V
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, gw, gV):
+        w, V = ctx.saved_tensors
+        n = w.shape[-1]
+        wb, Vb = w.reshape(-1, n), V.reshape(-1, n, n)
+        gw = torch.zeros_like(wb) if gw is None else gw.reshape(-1, n)
+        gV = torch.zeros_like(Vb) if gV is None else gV.reshape(-1, n, n)
+        out = torch.stack([safe_eigh_vjp(wb[i], Vb[i], gw[i], gV[i]) for i in range(wb.shape[0])])
+        return out.reshape(V.shape)
+
+
 def safe_general_eigh(A: torch.Tensor, B: torch.Tensor, Linv: torch.Tensor = None):
     if Linv is None:
         _, Linv = cholesky_inv(B)
     C = Linv @ A @ Linv.transpose(-1, -2)
     C = 0.5 * (C + C.transpose(-1, -2))
-    w, V = eigh(C)
+    if torch.is_grad_enabled() and C.requires_grad:
+        w, V = EighFn.apply(C)
+    else:
+        w, V = eigh(C)
     return w, Linv.transpose(-1, -2) @ V
 
 
@@ -77,5 +102,6 @@ def safe_eigh_vjp(evals: torch.Tensor, evecs: torch.Tensor, g_evals: torch.Tenso
     F = 0.5 * torch.where(degen, broadened, regular)
     F.fill_diagonal_(0.0)
     Vt = evecs.T
-    grad = torch.linalg.inv(Vt) @ (0.5 * torch.diag(g_evals) + F * (Vt @ g_evecs)) @ Vt
+    # inv(V^T) of the reference is V: the eigenvectors are orthonormal to rounding
+    grad = evecs @ (0.5 * torch.diag(g_evals) + F * (Vt @ g_evecs)) @ Vt
     return grad + grad.T

@@ -9,6 +9,10 @@ occupations, new density, predict.  ``JittableDiis`` keeps the reference's buffe
 buffers; the bordered matrix has +1 borders for active slots and 1 on the diagonal of
 inactive ones, and the coefficients are B^-1 e_0 (hand-written solve, ``gdx_diis_solve``).
 The reference's extra diagonalisation after the loop is discarded by it and not computed.
+
+The loop is differentiable w.r.t. params (reverse mode through every cycle, as jax.grad
+through the reference's fori_loop): the predictor, the DIIS solve and the eigensolver are
+autograd Functions over the device kernels when autograd is recording.
 """
 from __future__ import annotations
 
@@ -22,6 +26,31 @@ from .eigenproblem import cholesky_inv, safe_fock_solver
 from .functional import Functional
 from .molecule import Molecule
 from .train import energy_predictor
+
+
+def _diis_solve(B: torch.Tensor) -> torch.Tensor:
+    B = B.detach().contiguous()
+    x = torch.empty(B.shape[:2], dtype=torch.float64, device=B.device)
+    _lib.check(_lib.load().gdx_diis_solve(torch.cuda.current_stream(B.device).cuda_stream, B.shape[0],
+                                          B.shape[1], B.data_ptr(), x.data_ptr()), "gdx_diis_solve")
+    return x
+
+
+class DiisSolveFn(torch.autograd.Function):
+    """x_s = B_s^-1 e_0; reverse: Bbar_s = -B_s^-T xbar_s x_s^T (B_s is symmetric)."""
+
+    @staticmethod
+    def forward(ctx, B):
+        x = _diis_solve(B)
+        ctx.save_for_backward(B, x)
+        return x
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, gx):
+        B, x = ctx.saved_tensors
+        y = torch.linalg.solve(B.transpose(1, 2), gx.unsqueeze(-1)).squeeze(-1)
+        return -y.unsqueeze(-1) * x.unsqueeze(-2)
 
 
 @dataclass
@@ -62,9 +91,7 @@ class JittableDiis:
         B[:, 1:, 0] = active
         idx = torch.arange(m, device=errs.device)
         B[:, idx + 1, idx + 1] = torch.where(active > 0, gram[:, idx, idx], torch.ones_like(gram[:, idx, idx]))
-        x = torch.empty((2, m + 1), dtype=torch.float64, device=errs.device)
-        _lib.check(_lib.load().gdx_diis_solve(torch.cuda.current_stream(errs.device).cuda_stream, 2, m + 1,
-                                              B.contiguous().data_ptr(), x.data_ptr()), "gdx_diis_solve")
+        x = DiisSolveFn.apply(B)
         return x[:, 1:]
 
     def run(self, rdm1, fock, energy, data, cycle: int):
@@ -72,6 +99,17 @@ class JittableDiis:
         x = self.coefficients(data[3], cycle)
         F = torch.einsum("si,isjk->sjk", x, data[1])
         return F, data
+
+
+def non_scf_predictor(functional: Functional, precision: str = "fp64", **kwargs) -> Callable:
+    """``non_scf_predictor`` (evaluate.py:88-128): energy and Fock at the molecule's density."""
+    compute_energy = energy_predictor(functional, precision=precision, **kwargs)
+
+    def predictor(params: Optional[torch.Tensor], molecule: Molecule) -> Molecule:
+        e, fock = compute_energy(params, molecule)
+        return molecule.replace(fock=fock, energy=e)
+
+    return predictor
 
 
 def diff_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp64", **kwargs) -> Callable:

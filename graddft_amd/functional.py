@@ -16,6 +16,9 @@ Entry points kept from the reference:
 
 Both are differentiable through :class:`XCEnergyFn` (a ``torch.autograd.Function``) whose
 backward is the hand-written adjoint ``gdx_xc_backward`` (dE/dD and dE/dtheta).
+:class:`XCEnergyVxcFn` returns (E_xc, dE/dD) -- the ``value_and_grad`` of
+``train.py:86-121`` -- and is differentiable in turn: its backward is the hand-written
+second-order pass ``gdx_xc_hvp``, which is what reverse mode through an SCF loop needs.
 """
 from __future__ import annotations
 
@@ -30,7 +33,7 @@ import torch
 from . import _lib
 from .molecule import Molecule
 
-PRECISIONS = {"fp64": _lib.GDX_FP64, "fp32": _lib.GDX_FP32, "bf16x3": _lib.GDX_BF16X3}
+PRECISIONS = {"fp64": _lib.GDX_FP64, "fp32": _lib.GDX_FP32, "bf16x6": _lib.GDX_BF16X6}
 
 
 @dataclass(frozen=True)
@@ -79,6 +82,7 @@ class XCPlan:
         self.molecule = molecule
         self.generation = 0
         self.fwd_generation = -1
+        self.hvp_workspace = None
 
     @staticmethod
     def _ptr(t: Optional[torch.Tensor]) -> int:
@@ -130,6 +134,23 @@ class XCPlan:
         _lib.check(rc, "gdx_xc_backward")
 
 
+    def hvp(self, rdm1, params, direction, g_exc, exc=None, edot=None, d_rdm1=None, d_params=None) -> None:
+        """gdx_xc_hvp: gradients of g E + <direction, dE/dD> w.r.t. rdm1 and params."""
+        m = self.molecule
+        lib = _lib.load()
+        if self.hvp_workspace is None:
+            nbytes = ctypes.c_size_t(0)
+            _lib.check(lib.gdx_xc_hvp_workspace_size(ctypes.byref(self.desc), ctypes.byref(nbytes)),
+                       "gdx_xc_hvp_workspace_size")
+            self.hvp_workspace = torch.empty(int(nbytes.value), dtype=torch.uint8, device=m.ao_planes.device)
+        st = torch.cuda.current_stream(m.ao_planes.device).cuda_stream
+        rc = lib.gdx_xc_hvp(st, ctypes.byref(self.desc), self._ao().data_ptr(), m.grid.weights_pad.data_ptr(),
+                            rdm1.data_ptr(), self._ptr(params), direction.data_ptr(), g_exc.data_ptr(),
+                            self._ptr(exc), self._ptr(edot), self._ptr(d_rdm1), self._ptr(d_params),
+                            self.hvp_workspace.data_ptr())
+        _lib.check(rc, "gdx_xc_hvp")
+
+
 class XCEnergyFn(torch.autograd.Function):
     """E_xc(rdm1, params) with the hand-written adjoint as its backward."""
 
@@ -153,6 +174,38 @@ class XCEnergyFn(torch.autograd.Function):
         d_params = torch.empty_like(p) if (p is not None and ctx.needs_input_grad[1]) else None
         g = g.detach().to(torch.float64).contiguous().reshape(())
         ctx.plan.backward(rdm1, p, g, d_rdm1, d_params, ctx.gen)
+        return d_rdm1, d_params, None
+
+
+class XCEnergyVxcFn(torch.autograd.Function):
+    """(E_xc, dE_xc/dD) of (rdm1, params); backward = the second-order pass gdx_xc_hvp with
+    the cotangent of dE/dD as the direction and the cotangent of E_xc as g."""
+
+    @staticmethod
+    def forward(ctx, rdm1, params, plan: XCPlan):
+        rdm1 = rdm1.contiguous()
+        p = None if params is None else params.contiguous()
+        exc = torch.empty((), dtype=torch.float64, device=rdm1.device)
+        vxc = torch.empty_like(rdm1)
+        one = torch.ones((), dtype=torch.float64, device=rdm1.device)
+        plan.forward(rdm1, p, exc)
+        plan.backward(rdm1, p, one, vxc, None, plan.generation)
+        ctx.plan = plan
+        ctx.has_params = p is not None
+        ctx.save_for_backward(rdm1, p if p is not None else rdm1)
+        return exc, vxc
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, g_exc, g_vxc):
+        rdm1, p = ctx.saved_tensors
+        p = p if ctx.has_params else None
+        g = torch.zeros((), dtype=torch.float64, device=rdm1.device) if g_exc is None else \
+            g_exc.detach().to(torch.float64).contiguous().reshape(())
+        direction = torch.zeros_like(rdm1) if g_vxc is None else g_vxc.detach().contiguous()
+        d_rdm1 = torch.empty_like(rdm1) if ctx.needs_input_grad[0] else None
+        d_params = torch.empty_like(p) if (p is not None and ctx.needs_input_grad[1]) else None
+        ctx.plan.hvp(rdm1, p, direction, g, d_rdm1=d_rdm1, d_params=d_params)
         return d_rdm1, d_params, None
 
 

@@ -4,6 +4,11 @@ density (the stale exact-exchange term of B3LYP), Fock assembly.
 Reference: ``grad_dft/molecule.py:694-811`` (nonXC, coulomb_potential),
 ``grad_dft/train.py:148-215`` (Fock assembly), ``molecule.py:507-613`` (exact exchange from
 the initial density; here from the packed ERI instead of the semi-numerical chi tensor).
+
+The ``*Fn`` classes are the same device computations as ``torch.autograd.Function``s, for
+reverse mode through an SCF loop: J is linear and self-adjoint (the ERI is symmetric under
+pair exchange), the non-XC energy's adjoint is h + J/2 per spin plus the J cotangent, and
+the Fock assembly's adjoint applies the clip masks and the symmetrisation in reverse.
 """
 from __future__ import annotations
 
@@ -98,3 +103,79 @@ def fock_assemble(molecule: Molecule, J: torch.Tensor, vxc: torch.Tensor, exact_
                                              vxc.contiguous().data_ptr(), 0 if K0 is None else K0.data_ptr(),
                                              float(exact_exchange), F.data_ptr()), "gdx_fock_assemble")
     return F
+
+
+CLIP = 1e-30
+
+
+class CoulombFn(torch.autograd.Function):
+    """J(D_up + D_dn); cotangent Jbar -> dD_s = J[Jbar] for both spins."""
+
+    @staticmethod
+    def forward(ctx, rdm1, molecule: Molecule):
+        ctx.molecule = molecule
+        return coulomb(molecule, rdm1)
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, g):
+        g = g.detach().contiguous()
+        src = torch.stack([g, torch.zeros_like(g)])
+        dD = coulomb(ctx.molecule, src)
+        return torch.stack([dD, dD]), None
+
+
+class NonXCFn(torch.autograd.Function):
+    """non_xc_energy(D, J) with J an input: dE/dD_s = h + J/2 (- a0/2 K0_s), dE/dJ = D_t/2."""
+
+    @staticmethod
+    def forward(ctx, rdm1, J, molecule: Molecule, exact_exchange: float):
+        ctx.molecule = molecule
+        ctx.a0 = exact_exchange
+        ctx.save_for_backward(rdm1, J)
+        return non_xc_energy(molecule, rdm1, J, exact_exchange=exact_exchange)
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, g):
+        rdm1, J = ctx.saved_tensors
+        m = ctx.molecule
+        dD = (m.h1e + 0.5 * J).expand(2, -1, -1).clone()
+        if ctx.a0:
+            dD = dD - 0.5 * ctx.a0 * initial_exchange(m)
+        dJ = 0.5 * rdm1.sum(0)
+        return g * dD, g * dJ, None, None
+
+
+class FockAssembleFn(torch.autograd.Function):
+    """fock_assemble(J, vxc) with the reverse of its clips and symmetrisation."""
+
+    @staticmethod
+    def forward(ctx, J, vxc, molecule: Molecule, exact_exchange: float):
+        ctx.molecule = molecule
+        ctx.a0 = exact_exchange
+        ctx.save_for_backward(J, vxc)
+        return fock_assemble(molecule, J, vxc, exact_exchange=exact_exchange)
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, g):
+        J, vxc = ctx.saved_tensors
+        m = ctx.molecule
+        x1 = m.h1e + J + vxc
+        m1 = x1.abs() > CLIP
+        y1 = torch.where(m1, x1, torch.zeros_like(x1))
+        x2 = 0.5 * (y1 + y1.transpose(1, 2))
+        m2 = x2.abs() > CLIP
+        y = torch.where(m2, x2, torch.zeros_like(x2))
+        if ctx.a0:
+            X = -0.5 * ctx.a0 * initial_exchange(m)
+            x3 = y + X + X.transpose(1, 2)
+            m3 = x3.abs() > CLIP
+            g = torch.where(m3, g, torch.zeros_like(g))
+            y = torch.where(m3, x3, torch.zeros_like(x3))
+        g = torch.where(y.abs() > CLIP, g, torch.zeros_like(g))
+        g = torch.where(m2, g, torch.zeros_like(g))
+        g = 0.5 * (g + g.transpose(1, 2))
+        g = torch.where(m1, g, torch.zeros_like(g))
+        return g.sum(0), g, None, None

@@ -28,8 +28,13 @@ enum {
 };
 
 /* Element type of the matrix products (densities, V_xc, MLP layers).  Per-point functional
- * math, contractions and every reduction over grid points are FP64 in all modes. */
-enum { GDX_FP64 = 0, GDX_FP32 = 1, GDX_BF16X3 = 2 };
+ * math, contractions and every reduction over grid points are FP64 in all modes.
+ *  GDX_FP64   all products FP64 (v_mfma_f64_16x16x4f64)
+ *  GDX_FP32   all products FP32 (v_mfma_f32_16x16x4f32), FP32 AO planes
+ *  GDX_BF16X6 as GDX_FP32, but the MLP's hidden h x h layers use split-BF16: every FP32
+ *             operand split into three bf16 terms, six products on v_mfma_f32_16x16x32_bf16,
+ *             FP32 accumulation (FP32-level accuracy; AO planes FP32). */
+enum { GDX_FP64 = 0, GDX_FP32 = 1, GDX_BF16X6 = 2 };
 
 /* Functional kinds evaluated on the grid.
  *  GDX_FUNC_B3LYP        E = sum_r w * (0.8 e_lda + 0.72 e_b88 + 0.19 e_vwn + 0.81 e_lyp)
@@ -77,7 +82,7 @@ int gdx_xc_workspace_size(const gdx_xc_desc* desc, size_t* bytes);
  * as in compute_densities / compute_coefficient_inputs (:160-217) and
  * grad_dft/molecule.py:388-502.
  * ao: gdx_ao_eval output in the element type of desc->precision (double planes for
- * GDX_FP64, float planes for GDX_FP32); weights (npts_pad) grid weights, zero past npts;
+ * GDX_FP64, float planes otherwise); weights (npts_pad) grid weights, zero past npts;
  * rdm1 (2, nao, nao); params: flat MLP parameters [W0 (nc,h), b0 (h), W1 (h,h), b1, ...,
  * W_last (h,nd), b_last] (NULL for GDX_FUNC_B3LYP).  exc: one device double.
  * The workspace keeps what gdx_xc_backward needs; do not touch it in between. */
@@ -97,6 +102,19 @@ int gdx_xc_densities(void* stream, const gdx_xc_desc* desc, const void* ao, cons
 int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const void* ao,
                     const double* weights, const double* rdm1, const double* params,
                     const double* g_exc, double* d_rdm1, double* d_params, void* workspace);
+
+/* Second-order pass for reverse mode through an SCF loop.  With E(theta, D) = E_xc and a
+ * direction dir (2, nao, nao), forms Phi = g E + <dir, dE/dD> (g = *g_exc) and returns
+ * exc = E, edot = <dir, dE/dD> (forward-mode derivative of E along dir), d_rdm1 = dPhi/dD
+ * (2, nao, nao) and d_params = dPhi/dtheta.  In the reference this is what jax.grad
+ * computes when it differentiates the Fock matrix's XC term (train.py:86-121) inside
+ * diff_scf_loop (grad_dft/evaluate.py:917-1038): the cotangent of that term is `dir`, the
+ * cotangent of E_xc is g.  exc, edot, d_rdm1, d_params may each be NULL.  workspace:
+ * gdx_xc_hvp_workspace_size bytes (its own buffer; independent of gdx_xc_forward's). */
+int gdx_xc_hvp_workspace_size(const gdx_xc_desc* desc, size_t* bytes);
+int gdx_xc_hvp(void* stream, const gdx_xc_desc* desc, const void* ao, const double* weights,
+               const double* rdm1, const double* params, const double* dir, const double* g_exc,
+               double* exc, double* edot, double* d_rdm1, double* d_params, void* workspace);
 
 /* ---- SCF side ---------------------------------------------------------------------
  * eri: 8-fold packed (ij|kl), pair index I = i(i+1)/2 + j (i >= j), element I(I+1)/2 + K

@@ -37,7 +37,7 @@ def test_validation_errors_are_reported_without_a_gpu():
     assert rc == -1
     assert b"multiple of 64" in lib.gdx_last_error()
     d.npts_pad = 128
-    d.precision = _lib.GDX_BF16X3 + 5
+    d.precision = _lib.GDX_BF16X6 + 5
     assert lib.gdx_xc_workspace_size(ctypes.byref(d), ctypes.byref(nbytes)) == -3
     d.precision = _lib.GDX_FP64
     assert lib.gdx_xc_workspace_size(ctypes.byref(d), ctypes.byref(nbytes)) == 0

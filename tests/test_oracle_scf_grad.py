@@ -1,0 +1,119 @@
+"""CPU pins of the reverse-mode oracle (oracle/scf_grad.py).
+
+* edot = <dD, dE/dD> against the central difference of E_xc along dD;
+* the HVP (dPhi/dD, dPhi/dtheta for Phi = g E + edot) against central differences of the
+  oracle's first derivatives along dD, plus the g E part;
+* the torch SCF loop's energy equals the numpy oracle's (oracle/scf.py, pinned separately);
+* the SCF-loop parameter gradient against the central difference of the final energy along a
+  random parameter direction.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import scf as O
+from oracle import scf_grad as G
+from oracle import xc as X
+
+DT = torch.float64
+
+
+def _water(basis="def2-svp", level=1):
+    from graddft_amd.interface.basis import build_basis, nuclear_repulsion
+    from graddft_amd.interface.builder import core_guess
+    from graddft_amd.interface.grids import molecular_grid
+    from graddft_amd.interface.integrals import int1e, int2e_packed, unpack_eri
+    from oracle.ao import eval_ao
+    xyz = np.array([[0, 0, 0], [1.81, 0, 0], [1.81 * math.cos(1.824), 1.81 * math.sin(1.824), 0.05]])
+    bs = build_basis(["O", "H", "H"], xyz, basis)
+    S, T, V = int1e(bs, [8, 1, 1], xyz)
+    eri = unpack_eri(int2e_packed(bs), bs.nao)
+    D, C, e, occ = core_guess(T + V, S, 5)
+    rdm1 = np.stack([D, D])
+    K0 = np.stack([O.exchange_potential(rdm1[s], eri) for s in range(2)])
+    inp = O.SCFInputs(T + V, S, eri, nuclear_repulsion([8, 1, 1], xyz), K0, 0.2)
+    c, w = molecular_grid([8, 1, 1], xyz, level)
+    xin = X.to_inputs(*eval_ao(bs, c), w)
+    return inp, xin, rdm1, np.stack([occ, occ])
+
+
+def _direction(n, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.normal(size=(2, n, n)) * 1e-2
+    return A + 0.3 * A.transpose(0, 2, 1)   # deliberately not symmetric
+
+
+@pytest.mark.parametrize("kind", ["b3lyp", "neural_b3lyp"])
+def test_xc_hvp_matches_finite_differences(kind):
+    inp, xin, rdm1, _ = _water()
+    spec = X.MLPSpec(h=16, n_hidden=2)
+    theta = X.init_mlp_params(spec, 5) if kind == "neural_b3lyp" else None
+    n = rdm1.shape[-1]
+    dD = _direction(n, 1)
+    # a slightly spin-polarised density, so both spin channels carry distinct terms
+    D = rdm1.copy()
+    D[1] *= 0.97
+    E, edot, hD, hth = G.xc_hvp(kind, xin, D, theta, spec, dD, 0.0)
+
+    def central(h):
+        Ep, gDp, gtp = X.xc_value_and_grads(kind, xin, D + h * dD, theta, spec)
+        Em, gDm, gtm = X.xc_value_and_grads(kind, xin, D - h * dD, theta, spec)
+        ft = None if theta is None else (gtp - gtm) / (2 * h)
+        return (Ep - Em) / (2 * h), (gDp - gDm) / (2 * h), ft
+
+    # Richardson extrapolation of two central differences (the low-density tails make the
+    # third derivatives large, so a plain central difference needs very small steps)
+    h = 2e-5
+    a, b = central(h), central(h / 2)
+    rich = lambda x, y: (4.0 * y - x) / 3.0  # noqa: E731
+    assert edot == pytest.approx(rich(a[0], b[0]), rel=1e-7)
+    fd_D = rich(a[1], b[1])
+    assert np.abs(hD - fd_D).max() <= 1e-6 * np.abs(fd_D).max()
+    if theta is not None:
+        fd_t = rich(a[2], b[2])
+        assert np.abs(hth - fd_t).max() <= 1e-6 * np.abs(fd_t).max()
+    # the g E part adds g dE/dD and g dE/dtheta
+    g = 1.7
+    _, _, hD2, hth2 = G.xc_hvp(kind, xin, D, theta, spec, dD, g)
+    _, gD0, gt0 = X.xc_value_and_grads(kind, xin, D, theta, spec)
+    assert np.abs(hD2 - (hD + g * gD0)).max() <= 1e-10 * np.abs(hD2).max()
+    if theta is not None:
+        assert np.abs(hth2 - (hth + g * gt0)).max() <= 1e-10 * np.abs(hth2).max()
+
+
+def test_torch_scf_loop_matches_numpy_oracle():
+    inp, xin, rdm1, occ = _water()
+    xc_np = lambda d: X.xc_value_and_grads("b3lyp", xin, d)[:2]  # noqa: E731
+    st = O.diff_scf_loop(xc_np, inp, rdm1, occ, 6)
+    ti = G.TorchSCFInputs.from_numpy(inp)
+    E = G.diff_scf_energy(lambda d: X.xc_energy_from_rdm1("b3lyp", xin, d), ti, torch.tensor(rdm1, dtype=DT),
+                          torch.tensor(occ, dtype=DT), 6)
+    assert E.item() == pytest.approx(st.energy, abs=1e-10)
+
+
+def test_scf_parameter_gradient_matches_finite_differences():
+    inp, xin, rdm1, occ = _water()
+    spec = X.MLPSpec(h=16, n_hidden=2)
+    theta0 = X.init_mlp_params(spec, 3)
+    ti = G.TorchSCFInputs.from_numpy(inp)
+    ti.K0 = None                      # neural functional: no exact exchange
+    D0 = torch.tensor(rdm1, dtype=DT)
+    occ_t = torch.tensor(occ, dtype=DT)
+
+    def energy(th):
+        return G.diff_scf_energy(lambda d: X.xc_energy_from_rdm1("neural_b3lyp", xin, d, th, spec), ti, D0,
+                                 occ_t, 4)
+
+    th = torch.tensor(theta0, dtype=DT, requires_grad=True)
+    E = energy(th)
+    grad, = torch.autograd.grad(E, [th])
+    v = np.random.default_rng(9).normal(size=theta0.shape)
+    v /= np.linalg.norm(v)
+    h = 1e-4
+    Ep = float(energy(torch.tensor(theta0 + h * v, dtype=DT)))
+    Em = float(energy(torch.tensor(theta0 - h * v, dtype=DT)))
+    fd = (Ep - Em) / (2 * h)
+    an = float(grad.numpy() @ v)
+    assert an == pytest.approx(fd, rel=1e-5, abs=1e-9)

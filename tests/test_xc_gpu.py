@@ -139,3 +139,26 @@ def test_neural_water_unrestricted():
 def test_neural_benzene():
     _say("case: neural benzene/6-31g* level 1 h=64")
     _neural_case("benzene", "6-31g*", 1, 64, seed=5)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x6"])
+def test_mixed_precision_within_gates(precision):
+    """Mixed-precision paths against the device FP64 path on benzene (level 1, MLP 3 x 256,
+    whole 128-tiles so the tile GEMMs run): |dE| <= 1e-6 Ha, gradient rel. error <= 1e-5."""
+    _say(f"case: {precision} vs fp64 benzene/6-31g* level 1 h=256")
+    from graddft_amd import MLPSpec, NeuralFunctional
+    mol = _build("benzene", "6-31g*", 1, seed=5)
+    f = NeuralFunctional(MLPSpec(nc=6, h=256, n_hidden=3, nd=4))
+    theta = f.init(seed=12)
+    out = {}
+    for prec in ("fp64", precision):
+        D = mol.rdm1.detach().clone().requires_grad_(True)
+        th = theta.detach().clone().requires_grad_(True)
+        E = f.xc_energy(th, mol, precision=prec, rdm1=D)
+        gD, gth = torch.autograd.grad(E, [D, th])
+        out[prec] = (float(E.detach()), gD.cpu().numpy(), gth.cpu().numpy())
+    E0, gD0, gt0 = out["fp64"]
+    E1, gD1, gt1 = out[precision]
+    assert abs(E1 - E0) <= 1e-6, (E1, E0)
+    assert _rel(gD1, gD0) <= 1e-5
+    assert _rel(gt1, gt0) <= 1e-5
