@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16x6"])
     ap.add_argument("--no-scf", action="store_true", help="skip the ms/SCF-iteration measurement")
     ap.add_argument("--scf-cycles", type=int, default=15)
+    ap.add_argument("--no-train", action="store_true", help="skip the training-step measurement")
+    ap.add_argument("--train-cycles", type=int, default=10)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--grid-level", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -111,6 +113,42 @@ def scf_bench(args):
                                              sorted(kt.result.items(), key=lambda kv: -kv[1][0])[:8]}}
     if args.precision != "fp64":
         out["abs_dE_vs_fp64_Ha"] = abs(out[args.precision]["energy"] - out["fp64"]["energy"])
+    return out
+
+
+def train_bench(args):
+    """One training step through the differentiable SCF loop: H2O (seeded geometry)
+    def2-TZVP, level-3 grid, neural B3LYP with an MLP of 3 x 128, args.train_cycles-cycle
+    diff_scf_loop forward + reverse mode to d loss / d theta (simple_energy_loss)."""
+    import torch
+    from graddft_amd import MLPSpec, NeuralFunctional, _lib, diff_scf_loop, simple_energy_loss
+    from graddft_amd.interface import geometry
+    from graddft_amd.interface.builder import molecule_from_geometry
+    sym, xyz = geometry.water(np.random.default_rng(args.seed))
+    mol = molecule_from_geometry(sym, xyz, "def2-tzvp", 3, with_eri=True, name="water")
+    f = NeuralFunctional(MLPSpec(nc=6, h=128, n_hidden=3, nd=4))
+    theta = f.init(seed=args.seed)
+    out = {"workload": f"H2O def2-TZVP nao={mol.nao}, grid level 3 ({mol.grid_size} pts), neural B3LYP "
+                       f"MLP 3x128, diff_scf_loop {args.train_cycles} cycles forward + reverse "
+                       "(simple_energy_loss)"}
+    grads = {}
+    for prec in (["fp64", args.precision] if args.precision != "fp64" else ["fp64"]):
+        loop = diff_scf_loop(f, cycles=args.train_cycles, precision=prec)
+        simple_energy_loss(theta, loop, mol, -76.4)   # warm-up
+        torch.cuda.synchronize()
+        with _lib.KernelTimer() as kt:
+            t0 = time.perf_counter()
+            (cost, e), g = simple_energy_loss(theta, loop, mol, -76.4)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+        grads[prec] = g.cpu().numpy()
+        out[prec] = {"ms_per_step": round((t1 - t0) * 1e3, 3), "energy": float(e),
+                     "kernels_ms_per_step": {k: round(v[0], 4) for k, v in
+                                             sorted(kt.result.items(), key=lambda kv: -kv[1][0])[:8]}}
+    if args.precision != "fp64":
+        out["abs_dE_vs_fp64_Ha"] = abs(out[args.precision]["energy"] - out["fp64"]["energy"])
+        g0, g1 = grads["fp64"], grads[args.precision]
+        out["grad_rel_err_vs_fp64"] = float(np.linalg.norm(g1 - g0) / np.linalg.norm(g0))
     return out
 
 
@@ -199,6 +237,7 @@ def main():
     step_tflops = w_pt * N / (ms / 1e3) / 1e12
 
     scf = None if args.no_scf else scf_bench(args)
+    train = None if args.no_train else train_bench(args)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -233,6 +272,7 @@ def main():
         "vs_fp64_gpu": vs64,
         "gate_1e-6_Ha": None if vs64 is None else vs64["abs_dE_Ha"] <= 1e-6,
         "scf": scf,
+        "train_step": train,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

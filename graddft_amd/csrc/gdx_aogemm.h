@@ -22,6 +22,7 @@
 #pragma once
 #include "gdx_common.h"
 #include "gdx_gemm.h"
+#include "gdx_gemm_bf16.h"
 
 namespace gdx {
 
@@ -249,6 +250,142 @@ __global__ __launch_bounds__(256) void vxc_ao_kernel(const T* __restrict__ ao, i
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int row = a0 + wm + i * 16 + MfmaT<T>::row(lane, q);
+        if (row < nP && col < nP) out[(size_t)row * nP + col] = (double)acc[i][j][q];
+      }
+    }
+}
+
+// vxc_bf16x6_kernel: vxc_ao_kernel's product on bf16 MFMA (v_mfma_f32_16x16x32_bf16) with
+// the three-term bf16 split of every FP32 operand and the six leading products (see
+// gdx_gemm_bf16.h), for T = float AO planes.  Stage = 8 points x 4 components = 32 k;
+// LDS tiles are k-contiguous per output row/column ([split][row][k], k = 8 c + p) so a
+// lane's 8-k fragment is one 16-byte read.  Each thread stages 2 points x 2 columns, so the
+// split terms of consecutive points pack into 4-byte LDS stores.
+constexpr int VB_P = 8, VB_K = 4 * VB_P, VB_KP = VB_K + 8;
+
+template <bool SAME>
+__global__ __launch_bounds__(256) void vxc_bf16x6_kernel(const float* __restrict__ ao, int npts_pad, int nP,
+                                                         const double* __restrict__ vpt, int chunk,
+                                                         double* __restrict__ vpart) {
+  constexpr int VT = 128;
+  __shared__ __align__(16) __bf16 As[2][3][VT][VB_KP];
+  __shared__ __align__(16) __bf16 Bs[2][3][VT][VB_KP];
+  typedef float v2f __attribute__((ext_vector_type(2)));
+  typedef __bf16 v2b __attribute__((ext_vector_type(2)));
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nt = (nP + VT - 1) / VT;
+  const int a0 = (blockIdx.x / nt) * VT, b0 = (blockIdx.x % nt) * VT;
+  const int s = blockIdx.z;
+  const int rbeg = blockIdx.y * chunk;
+  const int rend = min(npts_pad, rbeg + chunk);
+  const size_t plane = (size_t)npts_pad * nP;
+  const double* vp = vpt + (size_t)s * npts_pad * VPT_STRIDE;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  // staging role: points 2 pp, 2 pp + 1 of the stage; columns 2 cc, 2 cc + 1 of the tile
+  const int pp = tid >> 6, cc = tid & 63;
+  v2f ra[2][4], rb[2][5];
+  double sv[2][VPT_STRIDE];
+  auto gload = [&](int r0) {
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int r = r0 + 2 * pp + e;
+      const bool ok = r < rend;
+      const int ca = a0 + 2 * cc, cb = b0 + 2 * cc;
+      if (!SAME) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          ra[e][c] = (ok && ca < nP) ? *(const v2f*)(ao + c * plane + (size_t)r * nP + ca) : (v2f)0.f;
+      }
+#pragma unroll
+      for (int c = 0; c < 5; ++c)
+        rb[e][c] = (ok && cb < nP) ? *(const v2f*)(ao + c * plane + (size_t)r * nP + cb) : (v2f)0.f;
+#pragma unroll
+      for (int q = 0; q < VPT_STRIDE; ++q) sv[e][q] = ok ? vp[(size_t)r * VPT_STRIDE + q] : 0.0;
+    }
+  };
+  auto put = [&](__bf16 (*T)[VT][VB_KP], int col, int k, float x0, float x1) {
+    __bf16 h0, m0, l0, h1, m1, l1;
+    split3_bf16(x0, h0, m0, l0);
+    split3_bf16(x1, h1, m1, l1);
+    *(v2b*)&T[0][col][k] = (v2b){h0, h1};
+    *(v2b*)&T[1][col][k] = (v2b){m0, m1};
+    *(v2b*)&T[2][col][k] = (v2b){l0, l1};
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = 2 * cc + j;
+      const int k0 = 2 * pp;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const float x0 = SAME ? rb[0][c][j] : ra[0][c][j];
+        const float x1 = SAME ? rb[1][c][j] : ra[1][c][j];
+        put(As[buf], col, c * VB_P + k0, x0, x1);
+      }
+      float z[2], u[2][3];
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const double phi = rb[e][0][j], gx = rb[e][1][j], gy = rb[e][2][j], gz = rb[e][3][j], lp = rb[e][4][j];
+        z[e] = (float)(sv[e][0] * phi + sv[e][1] * gx + sv[e][2] * gy + sv[e][3] * gz + sv[e][4] * lp);
+        u[e][0] = (float)(sv[e][5] * gx);
+        u[e][1] = (float)(sv[e][5] * gy);
+        u[e][2] = (float)(sv[e][5] * gz);
+      }
+      put(Bs[buf], col, k0, z[0], z[1]);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) put(Bs[buf], col, (q + 1) * VB_P + k0, u[0][q], u[1][q]);
+    }
+  };
+
+  f4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+  const int nst = (rend - rbeg + VB_P - 1) / VB_P;
+  if (nst > 0) {
+    gload(rbeg);
+    lstore(0);
+    __syncthreads();
+  }
+  const int g = lane >> 4, r = lane & 15;
+  for (int st = 0; st < nst; ++st) {
+    const int cur = st & 1;
+    if (st + 1 < nst) gload(rbeg + (st + 1) * VB_P);
+    bf8 af[4][3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int p = 0; p < 3; ++p) af[i][p] = *(const bf8*)&As[cur][p][wm + i * 16 + r][8 * g];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      bf8 bq[3];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) bq[p] = *(const bf8*)&Bs[cur][p][wn + j * 16 + r][8 * g];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        f4 cacc = acc[i][j];
+        cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][2], bq[0], cacc, 0, 0, 0);
+        cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][1], bq[1], cacc, 0, 0, 0);
+        cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][0], bq[2], cacc, 0, 0, 0);
+        cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][1], bq[0], cacc, 0, 0, 0);
+        cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][0], bq[1], cacc, 0, 0, 0);
+        cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][0], bq[0], cacc, 0, 0, 0);
+        acc[i][j] = cacc;
+      }
+    }
+    if (st + 1 < nst) lstore(cur ^ 1);
+    __syncthreads();
+  }
+  double* out = vpart + ((size_t)blockIdx.y * gridDim.z + s) * nP * nP;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int col = b0 + wn + j * 16 + r;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = a0 + wm + i * 16 + mfma32_row(lane, q);
         if (row < nP && col < nP) out[(size_t)row * nP + col] = (double)acc[i][j][q];
       }
     }

@@ -155,18 +155,62 @@ __global__ __launch_bounds__(NT) void point_fwd_kernel(
   if (!neural && threadIdx.x == 0) eps_part[blockIdx.x] = t;
 }
 
-// H (npts_pad, h) = ELU(X16[:, :nc] W0 (nc, h) + b0), one thread per element
-template <typename T>
-__global__ __launch_bounds__(NT) void layer0_kernel(const T* __restrict__ X, int nc, int h, long total,
-                                                    const double* __restrict__ W,
-                                                    const double* __restrict__ b, T* __restrict__ H) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  const long r = i / h;
-  const int f = (int)(i % h);
-  T z = (T)b[f];
-  for (int j = 0; j < nc; ++j) z += X[r * 16 + j] * (T)W[j * h + f];
-  H[i] = elu_any(z);
+// Row-wise products against at most 16 weight rows, for the MLP's narrow edges:
+//   v[r][f] = sum_{j < kj} A[r][j] Wt[j][f]   (A with row stride 16, Wt (16 x h) in LDS)
+//   ER_LAYER0:    out = ELU(v + b[f])             (first layer, forward)
+//   ER_TANGENT0:  out = v * elu'(Y[r][f])         (first layer, tangent)
+//   ER_DELTA:     out = v * elu'(Y[r][f])         (last layer's input adjoint)
+//   ER_RAW:       out = v
+// elu'(z) is taken from Y = ELU(z).  Each thread forms 8 consecutive features of one row
+// (16- / 32-byte stores); a block covers ER_ROWS rows.
+enum { ER_LAYER0 = 0, ER_TANGENT0 = 1, ER_DELTA = 2, ER_RAW = 3 };
+constexpr int ER_ROWS = 64;
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(NT) void edge_rows_kernel(const T* __restrict__ A, int kj, const T* __restrict__ Wt,
+                                                       const double* __restrict__ bias, int h, long rows,
+                                                       const T* __restrict__ Y, T* __restrict__ out) {
+  typedef T v8 __attribute__((ext_vector_type(8)));
+  __shared__ __align__(16) T Ws[16][256];
+  __shared__ __align__(16) T bs[256];
+  for (int e = threadIdx.x; e < kj * h; e += NT) Ws[e / h][e % h] = Wt[e];
+  if (MODE == ER_LAYER0)
+    for (int e = threadIdx.x; e < h; e += NT) bs[e] = (T)bias[e];
+  __syncthreads();
+  const int tpr = h / 8;            // threads per row
+  const int rpp = NT / tpr;         // rows per pass
+  const int t = threadIdx.x;
+  if (t >= rpp * tpr) return;
+  const int f0 = (t % tpr) * 8;
+  const long rbase = (long)blockIdx.x * ER_ROWS;
+  for (int rr = t / tpr; rr < ER_ROWS; rr += rpp) {
+    const long r = rbase + rr;
+    if (r >= rows) break;
+    T a[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) a[j] = j < kj ? A[r * 16 + j] : (T)0;
+    v8 v;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = (T)0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      if (j < kj) {
+        const v8 w = *(const v8*)&Ws[j][f0];
+        v += a[j] * w;
+      }
+    }
+    const size_t o = (size_t)r * h + f0;
+    if (MODE == ER_LAYER0) {
+      const v8 b = *(const v8*)&bs[f0];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = elu_any(v[e] + b[e]);
+    } else if (MODE == ER_TANGENT0 || MODE == ER_DELTA) {
+      const v8 y = *(const v8*)(Y + o);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] *= (y[e] > (T)0 ? (T)1 : y[e] + (T)1);
+    }
+    *(v8*)(out + o) = v;
+  }
 }
 
 // ------------------------------------------------- guarded GEMM (any shape, small) ----
@@ -305,26 +349,6 @@ __global__ __launch_bounds__(NT) void head_seed_kernel(
     }
 }
 
-// G[r][f] = (sum_o DZ16[r][o] W_last[f][o]) * elu'(H[r][f]), one thread per element
-// (RAW: without the elu' factor)
-template <typename T, bool RAW = false>
-__global__ __launch_bounds__(NT) void head_delta_kernel(const T* __restrict__ DZ16, const double* __restrict__ Wl,
-                                                        int nd, int h, const T* __restrict__ H, long total,
-                                                        T* __restrict__ G) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  const long r = i / h;
-  const int f = (int)(i % h);
-  T v = 0;
-  for (int o = 0; o < nd; ++o) v += DZ16[r * 16 + o] * (T)Wl[f * nd + o];
-  if (RAW) {
-    G[i] = v;
-    return;
-  }
-  const T y = H[i];
-  G[i] = v * (y > (T)0 ? (T)1 : y + (T)1);
-}
-
 // d_params of the last layer: W (h, nd) from the reduced transposed (16 x h) tile
 __global__ void unpack_last_kernel(const double* __restrict__ dWt16, int h, int nd, double* __restrict__ dW) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -446,21 +470,6 @@ __global__ __launch_bounds__(NT) void point_tangent_kernel(const double* __restr
 #pragma unroll
     for (int j = 6; j < 16; ++j) Xdot[(size_t)r * 16 + j] = (T)0;
   }
-}
-
-// Hdot0 = (Xdot16[:, :nc] W0) * elu'(z0), elu'(z0) from H0 = elu(z0)
-template <typename T>
-__global__ __launch_bounds__(NT) void layer0_tangent_kernel(const T* __restrict__ Xd, int nc, int h, long total,
-                                                            const double* __restrict__ W, const T* __restrict__ H0,
-                                                            T* __restrict__ Hd0) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  const long r = i / h;
-  const int f = (int)(i % h);
-  T z = 0;
-  for (int j = 0; j < nc; ++j) z += Xd[r * 16 + j] * (T)W[j * h + f];
-  const T y = H0[i];
-  Hd0[i] = z * (y > (T)0 ? (T)1 : y + (T)1);
 }
 
 // Seeds of the HVP reverse sweep, one thread per point.  With m = clip(w) [|eps| > 1e-30],
@@ -606,7 +615,31 @@ __global__ __launch_bounds__(NT) void point_hvp_bwd_kernel(const double* __restr
 }
 
 // ------------------------------------------------------------ workspace layout ----
-constexpr int VCHUNK = 512;    // points per V_xc split
+constexpr int NUM_CU = 256;    // MI355X compute units
+
+// Point chunking of the V_xc build: the grid (tiles x chunks x spins) is sized to whole
+// rounds of co-resident blocks (resident blocks per CU from the kernels' register and LDS
+// use), so no CU is left with an extra block at the end.
+struct VxcSplit {
+  int vt, ntile, nchunk, chunk;
+};
+// prec: GDX_FP64 / GDX_FP32 / GDX_BF16X6 (bf16 MFMA for tiles of 128)
+VxcSplit vxc_split(int Np, int nP, int nsv, int prec) {
+  VxcSplit v{};
+  const bool fp64 = prec == GDX_FP64;
+  v.vt = nP <= 64 ? 64 : 128;
+  const int nt = (nP + v.vt - 1) / v.vt;
+  v.ntile = nt * nt;
+  int resident = v.vt == 64 ? (fp64 ? 4 : 7) : (fp64 ? 2 : 3);
+  if (prec == GDX_BF16X6 && v.vt == 128) resident = 1;
+  const int per = v.ntile * nsv;
+  int nchunk = std::max(1, (NUM_CU * resident + per - 1) / per);
+  nchunk = std::min(nchunk, std::max(1, Np / 128));   // at least 128 points per chunk
+  v.chunk = (Np + nchunk - 1) / nchunk;
+  v.chunk = (v.chunk + VX_P - 1) / VX_P * VX_P;
+  v.nchunk = (Np + v.chunk - 1) / v.chunk;
+  return v;
+}
 constexpr int SKW_ROWS = 512;  // grid points per block of the skinny weight-gradient kernel
 
 // hvp = true: activation-side buffers hold stacked [value; tangent] rows (R = 2 N_pad), plus
@@ -632,8 +665,10 @@ Layout make_layout(const gdx_xc_desc& d, bool hvp = false) {
   const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
   L.nsd = d.spin_restricted ? 1 : 2;
   L.rows = (int)R;
-  L.chunk = VCHUNK;
-  L.nchunk = (int)((Np + VCHUNK - 1) / VCHUNK);
+  {
+    const VxcSplit v1 = vxc_split((int)Np, (int)nP, 1, d.precision), v2 = vxc_split((int)Np, (int)nP, 2, d.precision);
+    L.nchunk = std::max(v1.nchunk, 2 * v2.nchunk);   // partial slots: nchunk x nsv
+  }
   // split-K for the hidden weight-gradient GEMMs: whole 32-row stages per split (for
   // hvp, an even count so the first half of the splits covers the value rows exactly)
   L.nsplit = 1;
@@ -673,7 +708,7 @@ Layout make_layout(const gdx_xc_desc& d, bool hvp = false) {
   L.G = take(neural ? 2 * R * h * es : 0);
   L.DX = take(neural ? R * d.mlp_nc * es : 0);
   L.vpt = take(2 * Np * VPT_STRIDE * 8);
-  L.vpart = take((size_t)L.nchunk * 2 * nP * nP * 8);
+  L.vpart = take((size_t)L.nchunk * nP * nP * 8);
   size_t wbytes = 0;
   if (neural) {
     wbytes = (size_t)L.nsplit * h * h * es;
@@ -800,6 +835,42 @@ const T* lowp_params(const Ctx& c, const double* params) {
   return dst;
 }
 
+// V_s (nsv spins) from the per-point potentials in L.vpt, reduced into d_rdm1 (2, n, n)
+template <typename T>
+void launch_vxc(const Ctx& c, const T* ao, int nsv, double* d_rdm1) {
+  const gdx_xc_desc& d = c.d;
+  const Layout& L = c.L;
+  hipStream_t st = c.st;
+  const int Np = d.npts_pad, nP = d.nao_pad;
+  const VxcSplit v = vxc_split(Np, nP, nsv, d.precision);
+  const char* nm = tname<T>("vxc_f64", "vxc_f32");
+  const dim3 grid(v.ntile, v.nchunk, nsv);
+  bool done = false;
+  if constexpr (sizeof(T) == 4) {
+    if (d.precision == GDX_BF16X6 && v.vt == 128) {
+      if (v.ntile == 1)
+        GDX_LAUNCH("vxc_bf16x6", vxc_bf16x6_kernel<true>, grid, dim3(NT), 0, st, ao, Np, nP, c.P<double>(L.vpt),
+                   v.chunk, c.P<double>(L.vpart));
+      else
+        GDX_LAUNCH("vxc_bf16x6", vxc_bf16x6_kernel<false>, grid, dim3(NT), 0, st, ao, Np, nP, c.P<double>(L.vpt),
+                   v.chunk, c.P<double>(L.vpart));
+      done = true;
+    }
+  }
+  if (done) {
+  } else if (v.vt == 64)
+    GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 64>), grid, dim3(NT), 0, st, ao, Np, nP, c.P<double>(L.vpt), v.chunk,
+               c.P<double>(L.vpart));
+  else if (v.ntile == 1)
+    GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 128>), grid, dim3(NT), 0, st, ao, Np, nP, c.P<double>(L.vpt), v.chunk,
+               c.P<double>(L.vpart));
+  else
+    GDX_LAUNCH(nm, (vxc_ao_kernel<T, false, 128>), grid, dim3(NT), 0, st, ao, Np, nP, c.P<double>(L.vpt), v.chunk,
+               c.P<double>(L.vpart));
+  GDX_LAUNCH("reduce_vxc", reduce_vxc_kernel, dim3(nblocks(2L * d.nao * d.nao)), dim3(NT), 0, st,
+             c.P<double>(L.vpart), v.nchunk, nsv, nP, d.nao, d_rdm1);
+}
+
 template <typename T>
 int forward_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm1, const double* params,
                  double* exc) {
@@ -816,8 +887,13 @@ int forward_impl(const Ctx& c, const T* ao, const double* weights, const double*
     const ParamOffsets po = param_offsets(d);
     const T* plp = lowp_params<T>(c, params);
     T* H = c.P<T>(L.H);
-    GDX_LAUNCH("layer0", layer0_kernel<T>, dim3(nblocks((long)Np * h)), dim3(NT), 0, st, c.P<T>(L.X), d.mlp_nc, h,
-               (long)Np * h, params + po.w[0], params + po.b[0], H);
+    const int nl = d.mlp_nhidden;
+    T* W016 = c.P<T>(L.W16);
+    T* WT16 = W016 + 16 * h;
+    GDX_LAUNCH("prep_edge_weights", prep_edge_weights_kernel<T>, dim3(nblocks(16L * h)), dim3(NT), 0, st,
+               params + po.w[0], d.mlp_nc, params + po.w[nl], d.mlp_nd, h, W016, WT16);
+    GDX_LAUNCH("layer0", (edge_rows_kernel<T, ER_LAYER0>), dim3((Np + ER_ROWS - 1) / ER_ROWS), dim3(NT), 0, st,
+               c.P<T>(L.X), d.mlp_nc, W016, params + po.b[0], h, (long)Np, nullptr, H);
     for (int l = 1; l < d.mlp_nhidden; ++l) {
       const T* Wl = plp + po.w[l];
       const T* bl = plp + po.b[l];
@@ -830,11 +906,6 @@ int forward_impl(const Ctx& c, const T* ao, const double* weights, const double*
         launch_any<T, 128, 128, false, false, EPI_BIAS_ELU>(st, "gemm_any", Np, h, h, Hin, h, Wl, h, Hout, h, bl,
                                                             nullptr, 0, 1, h, nullptr);
     }
-    const int nl = d.mlp_nhidden;
-    T* W016 = c.P<T>(L.W16);
-    T* WT16 = W016 + 16 * h;
-    GDX_LAUNCH("prep_edge_weights", prep_edge_weights_kernel<T>, dim3(nblocks(16L * h)), dim3(NT), 0, st,
-               params + po.w[0], d.mlp_nc, params + po.w[nl], d.mlp_nd, h, W016, WT16);
     GDX_LAUNCH("head_fwd", (skinny_out16_kernel<T, true>), dim3(Np / 64), dim3(NT), 0, st, Np, h,
                H + (size_t)(nl - 1) * L.hstride, h, WT16, c.P<T>(L.C), d.mlp_nd, d.mlp_nd, plp + po.b[nl], d.mlp_nd,
                c.P<double>(L.E), weights, d.npts, c.P<double>(L.eps_part));
@@ -869,8 +940,8 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
     GDX_LAUNCH("head_seed", head_seed_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, c.P<T>(L.C),
                c.P<double>(L.E), weights, nd, d.npts, Np, g_exc, 1, c.P<double>(L.GE), c.P<T>(L.DZ),
                c.P<double>(L.dbpart));
-    GDX_LAUNCH("head_delta", head_delta_kernel<T>, dim3(nblocks((long)Np * h)), dim3(NT), 0, st, c.P<T>(L.DZ),
-               params + po.w[nl], nd, h, Hlast, (long)Np * h, G0);
+    GDX_LAUNCH("head_delta", (edge_rows_kernel<T, ER_DELTA>), dim3((Np + ER_ROWS - 1) / ER_ROWS), dim3(NT), 0, st,
+               c.P<T>(L.DZ), nd, W016 + 16 * h, nullptr, h, (long)Np, Hlast, G0);
     if (d_params) {
       GDX_LAUNCH("skinny_wgrad", skinny_wgrad_kernel<T>, dim3(L.nwblk), dim3(NT), 0, st, Np, L.wrows, c.P<T>(L.DZ),
                  Hlast, h, c.P<double>(L.wpart));
@@ -928,22 +999,7 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
     GDX_LAUNCH("point_bwd", point_bwd_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, c.P<double>(L.dens_fin),
                c.P<double>(L.GE), neural ? c.P<T>(L.DX) : nullptr, d.mlp_nc, d.npts, Np, c.P<double>(L.vpt));
     // B3LYP on equal spin densities has equal spin potentials: form V once
-    const int nsv = (!neural && d.spin_restricted) ? 1 : 2;
-    const char* nm = tname<T>("vxc_f64", "vxc_f32");
-    if (nP <= 64) {
-      GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 64>), dim3(1, L.nchunk, nsv), dim3(NT), 0, st, ao, Np, nP,
-                 c.P<double>(L.vpt), L.chunk, c.P<double>(L.vpart));
-    } else {
-      const int nt = (nP + 127) / 128;
-      if (nt == 1)
-        GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 128>), dim3(1, L.nchunk, nsv), dim3(NT), 0, st, ao, Np, nP,
-                   c.P<double>(L.vpt), L.chunk, c.P<double>(L.vpart));
-      else
-        GDX_LAUNCH(nm, (vxc_ao_kernel<T, false, 128>), dim3(nt * nt, L.nchunk, nsv), dim3(NT), 0, st, ao, Np, nP,
-                   c.P<double>(L.vpt), L.chunk, c.P<double>(L.vpart));
-    }
-    GDX_LAUNCH("reduce_vxc", reduce_vxc_kernel, dim3(nblocks(2L * d.nao * d.nao)), dim3(NT), 0, st,
-               c.P<double>(L.vpart), L.nchunk, nsv, nP, d.nao, d_rdm1);
+    launch_vxc<T>(c, ao, (!neural && d.spin_restricted) ? 1 : 2, d_rdm1);
   }
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
@@ -981,8 +1037,8 @@ int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm
   const T* WT16 = W016 + 16 * h;
   if (neural) {
     // tangent MLP: Hdot_l = (Hdot_{l-1} W_l) * elu'(z_l); zdot of the head
-    GDX_LAUNCH("layer0_tangent", layer0_tangent_kernel<T>, dim3(nblocks((long)Np * h)), dim3(NT), 0, st,
-               X2 + (size_t)Np * 16, d.mlp_nc, h, (long)Np * h, params + po.w[0], H, H + (size_t)Np * h);
+    GDX_LAUNCH("layer0_tangent", (edge_rows_kernel<T, ER_TANGENT0>), dim3((Np + ER_ROWS - 1) / ER_ROWS), dim3(NT),
+               0, st, X2 + (size_t)Np * 16, d.mlp_nc, W016, nullptr, h, (long)Np, H, H + (size_t)Np * h);
     for (int l = 1; l < nl; ++l) {
       const T* Wl = plp + po.w[l];
       const T* Hd_in = H + (size_t)(l - 1) * L.hstride + (size_t)Np * h;
@@ -1014,8 +1070,8 @@ int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm
     T* G1 = G0 + (size_t)R * h;
     const T* Hlast2 = H + (size_t)(nl - 1) * L.hstride;
     T* DZ2 = c.P<T>(L.DZ);
-    GDX_LAUNCH("head_delta", (head_delta_kernel<T, true>), dim3(nblocks((long)R * h)), dim3(NT), 0, st, DZ2,
-               params + po.w[nl], nd, h, nullptr, (long)R * h, G0);
+    GDX_LAUNCH("head_delta", (edge_rows_kernel<T, ER_RAW>), dim3((R + ER_ROWS - 1) / ER_ROWS), dim3(NT), 0, st,
+               DZ2, nd, WT16, nullptr, h, (long)R, nullptr, G0);
     GDX_LAUNCH("hvp_combine", hvp_combine_kernel<T>, dim3(nblocks(half)), dim3(NT), 0, st, G0, Hlast2, half);
     if (d_params) {
       GDX_LAUNCH("skinny_wgrad", skinny_wgrad_kernel<T>, dim3(L.nwblk), dim3(NT), 0, st, R, L.wrows, DZ2, Hlast2, h,
@@ -1071,21 +1127,7 @@ int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm
     GDX_LAUNCH("point_hvp_bwd", point_hvp_bwd_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st,
                c.P<double>(L.dens_fin), c.P<double>(L.dfin_t), c.P<double>(L.GE), c.P<double>(L.GEd),
                neural ? c.P<T>(L.DX) : nullptr, d.mlp_nc, d.npts, Np, c.P<double>(L.vpt));
-    const char* nm = tname<T>("vxc_f64", "vxc_f32");
-    if (nP <= 64) {
-      GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 64>), dim3(1, L.nchunk, 2), dim3(NT), 0, st, ao, Np, nP,
-                 c.P<double>(L.vpt), L.chunk, c.P<double>(L.vpart));
-    } else {
-      const int nt = (nP + 127) / 128;
-      if (nt == 1)
-        GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 128>), dim3(1, L.nchunk, 2), dim3(NT), 0, st, ao, Np, nP,
-                   c.P<double>(L.vpt), L.chunk, c.P<double>(L.vpart));
-      else
-        GDX_LAUNCH(nm, (vxc_ao_kernel<T, false, 128>), dim3(nt * nt, L.nchunk, 2), dim3(NT), 0, st, ao, Np, nP,
-                   c.P<double>(L.vpt), L.chunk, c.P<double>(L.vpart));
-    }
-    GDX_LAUNCH("reduce_vxc", reduce_vxc_kernel, dim3(nblocks(2L * d.nao * d.nao)), dim3(NT), 0, st,
-               c.P<double>(L.vpart), L.nchunk, 2, nP, d.nao, d_rdm1);
+    launch_vxc<T>(c, ao, 2, d_rdm1);
   }
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
