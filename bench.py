@@ -87,7 +87,9 @@ def cpu_baseline(mol, theta, spec, sample):
 
 def scf_bench(args):
     """ms per SCF iteration: H2O (seeded geometry) def2-TZVP (nao 43), level-3 grid, B3LYP
-    with exact exchange from the packed ERI, diff_scf_loop for args.scf_cycles cycles."""
+    with exact exchange from the packed ERI, diff_scf_loop for args.scf_cycles cycles;
+    headline = the graph-captured loop (mean of 5 replays), with the eager loop's time and
+    per-kernel breakdown beside it."""
     import torch
     from graddft_amd import B3LYP, _lib, diff_scf_loop
     from graddft_amd.interface import geometry
@@ -97,7 +99,9 @@ def scf_bench(args):
     out = {"workload": f"H2O def2-TZVP nao={mol.nao}, grid level 3 ({mol.grid_size} pts), B3LYP "
                        f"(exact exchange from packed ERI), diff_scf_loop {args.scf_cycles} cycles",
            "npts": mol.grid_size, "nao": mol.nao}
+    reps = 5
     for prec in (["fp64", args.precision] if args.precision != "fp64" else ["fp64"]):
+        # eager loop: per-kernel breakdown (HIP events around every launch)
         loop = diff_scf_loop(B3LYP, cycles=args.scf_cycles, precision=prec)
         res = loop(None, mol)  # warm-up (plans, workspaces, Cholesky of S, exchange of D0)
         torch.cuda.synchronize()
@@ -106,9 +110,21 @@ def scf_bench(args):
             res = loop(None, mol)
             torch.cuda.synchronize()
             t1 = time.perf_counter()
+        # graph-captured loop: the whole diff_scf_loop replayed as one HIP graph
+        gloop = diff_scf_loop(B3LYP, cycles=args.scf_cycles, precision=prec, graph=True)
+        gloop(None, mol)       # eager run + capture
+        gloop(None, mol)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for _ in range(reps):
+            gres = gloop(None, mol)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
         # cycles + 1 energy/Fock evaluations (the initial one plus one per cycle)
-        out[prec] = {"ms_per_iter": round((t1 - t0) / args.scf_cycles * 1e3, 4),
-                     "energy": float(res.energy),
+        out[prec] = {"ms_per_iter": round((t3 - t2) / reps / args.scf_cycles * 1e3, 4),
+                     "ms_per_iter_eager": round((t1 - t0) / args.scf_cycles * 1e3, 4),
+                     "energy": float(gres.energy),
+                     "graph_vs_eager_abs_dE": abs(float(gres.energy) - float(res.energy)),
                      "kernels_ms_per_iter": {k: round(v[0] / args.scf_cycles, 4) for k, v in
                                              sorted(kt.result.items(), key=lambda kv: -kv[1][0])[:8]}}
     if args.precision != "fp64":

@@ -1,0 +1,361 @@
+// Streaming kernels for the MLP's narrow edges (fan-in nc = 6 or fan-out nd = 4 against a
+// hidden width h): HBM-bound passes over an (N x h) activation, one read per pass.
+//
+// A row of h values is spread over LPR = h / 4 lanes, 4 consecutive features per lane
+// (one 16- or 32-byte load); a wave holds 64 / LPR rows at a time.  Dot products over the
+// row are finished with xor shuffles inside each LPR-lane group.  Weight-gradient sums over
+// a block's rows accumulate per lane and go out as per-block partials (reduced afterwards in
+// FP64, fixed order).
+//
+//   delta_wgrad_rows_kernel G = (DZ W_last^T) * elu'(H) and dW_last partials sum_r DZ^T H
+//   l0bwd_rows_kernel       DX = G W0^T and [dW0; db0] partials sum_r [X; 1]^T G
+//   wgrad0_rows_kernel      the [dW0; db0] partials alone
+// and, with each lane loading 16 bytes of one row and the row dot products done on MFMA
+// (no cross-lane reductions):
+//   rowdot_mfma_kernel      HEAD: z = H W_last + b, c = 2 sigmoid(z / 2), eps = sum_o c_o e_o,
+//                           clipped weighted E_xc block partial (grad_dft/functional.py:219-253,
+//                           407-419); otherwise out = A Bt^T for at most 16 output columns
+#pragma once
+#include "gdx_common.h"
+#include "gdx_gemm.h"
+
+namespace gdx {
+
+constexpr int RW_ROWS = 256;  // rows per block
+constexpr int RW_UNROLL = 8;  // rows per lane loaded before any is consumed (FP32; FP64 half)
+
+template <typename T>
+struct Vec4T;
+template <> struct Vec4T<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct Vec4T<double> { typedef double type __attribute__((ext_vector_type(4))); };
+
+// sum over the aligned group of `width` lanes (width a power of two <= 64)
+template <typename V>
+__device__ __forceinline__ V group_sum(V v, int width) {
+  for (int off = width >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void head_rows_kernel(const T* __restrict__ H, int h, int npts_pad,
+                                                        const T* __restrict__ WT16, const T* __restrict__ bias,
+                                                        int nd, const double* __restrict__ E,
+                                                        const double* __restrict__ weights, int npts,
+                                                        T* __restrict__ C, double* __restrict__ eps_part) {
+  typedef typename Vec4T<T>::type v4;
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lpr = h >> 2, rpw = 64 / lpr;
+  const int sub = lane / lpr, li = lane % lpr;
+  const int f0 = li * 4;
+  T w[4][4];
+#pragma unroll
+  for (int o = 0; o < 4; ++o)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) w[o][e] = o < nd ? WT16[(size_t)o * h + f0 + e] : (T)0;
+  double contrib = 0.0;
+  const int r0 = blockIdx.x * RW_ROWS + wave * (RW_ROWS / 4);
+  constexpr int U = sizeof(T) == 8 ? RW_UNROLL / 2 : RW_UNROLL;
+  for (int rb = sub; rb < RW_ROWS / 4; rb += U * rpw) {
+    // U rows in flight per lane before any of them is reduced
+    v4 x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = r0 + rb + u * rpw;
+      x[u] = (rb + u * rpw < RW_ROWS / 4 && r < npts_pad) ? *(const v4*)(H + (size_t)r * h + f0) : (v4)(T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = r0 + rb + u * rpw;
+      T z[4];
+#pragma unroll
+      for (int o = 0; o < 4; ++o)
+        z[o] = group_sum<T>(x[u][0] * w[o][0] + x[u][1] * w[o][1] + x[u][2] * w[o][2] + x[u][3] * w[o][3], lpr);
+      if (li == 0 && rb + u * rpw < RW_ROWS / 4 && r < npts) {
+        double eps = 0.0;
+        for (int o = 0; o < nd; ++o) {
+          const double zz = (double)z[o] + (double)bias[o];
+          const double c = 2.0 / (1.0 + exp(-zz / 2.0));
+          C[(size_t)r * nd + o] = (T)c;
+          eps += c * E[(size_t)r * nd + o];
+        }
+        eps = fabs(eps) > 1e-30 ? eps : 0.0;
+        const double wt = weights[r];
+        contrib += (fabs(wt) > 1e-30 ? wt : 0.0) * eps;
+      }
+    }
+  }
+  contrib = warp_sum(contrib);
+  if (lane == 0) red[wave] = contrib;
+  __syncthreads();
+  if (threadIdx.x == 0) eps_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// G[r][f] = (sum_o DZ16[r][o] WT16[o][f]) * elu'(H[r][f]);  part[blk][o][f] = sum_r DZ16[r][o] H[r][f]
+template <typename T>
+__global__ __launch_bounds__(256) void delta_wgrad_rows_kernel(const T* __restrict__ H, int h, int rows,
+                                                               const T* __restrict__ DZ16,
+                                                               const T* __restrict__ WT16, int nd,
+                                                               T* __restrict__ G, double* __restrict__ part) {
+  typedef typename Vec4T<T>::type v4;
+  __shared__ double red[4][4][256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lpr = h >> 2, rpw = 64 / lpr;
+  const int sub = lane / lpr, li = lane % lpr;
+  const int f0 = li * 4;
+  T w[4][4];
+#pragma unroll
+  for (int o = 0; o < 4; ++o)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) w[o][e] = o < nd ? WT16[(size_t)o * h + f0 + e] : (T)0;
+  T acc[4][4];
+#pragma unroll
+  for (int o = 0; o < 4; ++o)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[o][e] = (T)0;
+  const int r0 = blockIdx.x * RW_ROWS + wave * (RW_ROWS / 4);
+  constexpr int U = sizeof(T) == 8 ? RW_UNROLL / 2 : RW_UNROLL;
+  for (int rb = sub; rb < RW_ROWS / 4; rb += U * rpw) {
+    v4 y[U], dzv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = r0 + rb + u * rpw;
+      const bool ok = rb + u * rpw < RW_ROWS / 4 && r < rows;
+      y[u] = ok ? *(const v4*)(H + (size_t)r * h + f0) : (v4)(T)0;
+      dzv[u] = ok ? *(const v4*)(DZ16 + (size_t)r * 16) : (v4)(T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = r0 + rb + u * rpw;
+      v4 g;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const T v = dzv[u][0] * w[0][e] + dzv[u][1] * w[1][e] + dzv[u][2] * w[2][e] + dzv[u][3] * w[3][e];
+        g[e] = v * (y[u][e] > (T)0 ? (T)1 : y[u][e] + (T)1);
+#pragma unroll
+        for (int o = 0; o < 4; ++o) acc[o][e] += dzv[u][o] * y[u][e];
+      }
+      if (rb + u * rpw < RW_ROWS / 4 && r < rows) *(v4*)(G + (size_t)r * h + f0) = g;
+    }
+  }
+  // sum the rpw row groups of each wave, then the 4 waves (fixed order)
+#pragma unroll
+  for (int o = 0; o < 4; ++o)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      double v = (double)acc[o][e];
+      for (int off = lpr; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+      if (sub == 0) red[wave][o][f0 + e] = v;
+    }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nd * h; i += 256) {
+    const int o = i / h, f = i % h;
+    part[(size_t)blockIdx.x * nd * h + i] = red[0][o][f] + red[1][o][f] + red[2][o][f] + red[3][o][f];
+  }
+}
+
+// DX[r][j] = sum_f G[r][f] W016[j][f] (j < nc);  part[blk][j][f] = sum_r X16[r][j] G[r][f] for
+// j <= nc (X16's column nc is the ones column: the bias gradient)
+template <typename T>
+__global__ __launch_bounds__(256) void l0bwd_rows_kernel(const T* __restrict__ G, int h, int rows,
+                                                         const T* __restrict__ X16, const T* __restrict__ W016,
+                                                         int nc, T* __restrict__ DX, double* __restrict__ part) {
+  typedef typename Vec4T<T>::type v4;
+  constexpr int NJ = 7;  // nc = 6 inputs + the ones column
+  __shared__ double red[4][NJ][256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lpr = h >> 2, rpw = 64 / lpr;
+  const int sub = lane / lpr, li = lane % lpr;
+  const int f0 = li * 4;
+  T w[6][4];
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) w[j][e] = j < nc ? W016[(size_t)j * h + f0 + e] : (T)0;
+  T acc[NJ][4];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[j][e] = (T)0;
+  const int r0 = blockIdx.x * RW_ROWS + wave * (RW_ROWS / 4);
+  constexpr int U = sizeof(T) == 8 ? RW_UNROLL / 2 : RW_UNROLL;
+  for (int rb = sub; rb < RW_ROWS / 4; rb += U * rpw) {
+    v4 g[U], xa[U], xb[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = r0 + rb + u * rpw;
+      const bool ok = rb + u * rpw < RW_ROWS / 4 && r < rows;
+      g[u] = ok ? *(const v4*)(G + (size_t)r * h + f0) : (v4)(T)0;
+      xa[u] = ok ? *(const v4*)(X16 + (size_t)r * 16) : (v4)(T)0;
+      xb[u] = ok ? *(const v4*)(X16 + (size_t)r * 16 + 4) : (v4)(T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = r0 + rb + u * rpw;
+      const T x[NJ] = {xa[u][0], xa[u][1], xa[u][2], xa[u][3], xb[u][0], xb[u][1], xb[u][2]};
+      T dx[6];
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+        dx[j] = group_sum<T>(g[u][0] * w[j][0] + g[u][1] * w[j][1] + g[u][2] * w[j][2] + g[u][3] * w[j][3], lpr);
+      if (li == 0 && rb + u * rpw < RW_ROWS / 4 && r < rows)
+        for (int j = 0; j < nc; ++j) DX[(size_t)r * nc + j] = dx[j];
+#pragma unroll
+      for (int j = 0; j < NJ; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[j][e] += x[j] * g[u][e];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NJ; ++j)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      double v = (double)acc[j][e];
+      for (int off = lpr; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+      if (sub == 0) red[wave][j][f0 + e] = v;
+    }
+  __syncthreads();
+  for (int i = threadIdx.x; i < (nc + 1) * h; i += 256) {
+    const int j = i / h, f = i % h;
+    const int jj = j == nc ? NJ - 1 : j;   // row nc of the output is the bias (ones column)
+    part[(size_t)blockIdx.x * (nc + 1) * h + i] = red[0][jj][f] + red[1][jj][f] + red[2][jj][f] + red[3][jj][f];
+  }
+}
+
+// out (rows x nout) = A (rows x K) Bt^T, Bt (16 x K) in LDS; 16-row groups per wave.  Lane l
+// loads VEC = 16 / sizeof(T) consecutive elements of row l % 16 at column
+// t (4 VEC) + (l / 16) VEC for every t; element q of chunk t is the A operand of MFMA
+// (t, q), whose k-slice l / 16 then stands for that column -- B is read with the same
+// column map, so the products sum over every k exactly once.
+template <typename T, bool HEAD>
+__global__ __launch_bounds__(256) void rowdot_mfma_kernel(int rows, int K, const T* __restrict__ A,
+                                                          const T* __restrict__ Bt, T* __restrict__ out, int nout,
+                                                          const T* __restrict__ bias, const double* __restrict__ E,
+                                                          const double* __restrict__ weights, int npts,
+                                                          double* __restrict__ eps_part) {
+  typedef typename MfmaT<T>::acc_t acc_t;
+  constexpr int VEC = 16 / sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(VEC)));
+  constexpr int MAXC = 256 / (4 * VEC);  // chunks for K <= 256
+  __shared__ __align__(16) T Bs[16 * 256];
+  __shared__ double zs[4][16][17];
+  __shared__ double red[4];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int e = tid; e < 16 * K; e += 256) Bs[e] = Bt[e];
+  __syncthreads();
+  const int nch = K / (4 * VEC);
+  const int rl = lane & 15, ks = lane >> 4;
+  double contrib = 0.0;
+  const int ngroups = RW_ROWS / 16;
+  for (int gi = wave; gi < ngroups; gi += 4) {
+    const int r0 = blockIdx.x * RW_ROWS + gi * 16;
+    if (r0 >= rows) break;
+    const int r = r0 + rl;
+    vec_t x[MAXC];
+#pragma unroll
+    for (int t = 0; t < MAXC; ++t)
+      if (t < nch) x[t] = r < rows ? *(const vec_t*)(A + (size_t)r * K + t * 4 * VEC + ks * VEC) : (vec_t)(T)0;
+    acc_t acc = (acc_t){0, 0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < MAXC; ++t) {
+      if (t < nch) {
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) {
+          const T b = Bs[rl * K + t * 4 * VEC + ks * VEC + q];
+          acc = MfmaT<T>::op(x[t][q], b, acc);
+        }
+      }
+    }
+    if (!HEAD) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int rr = r0 + MfmaT<T>::row(lane, q);
+        if (rr < rows && rl < nout) out[(size_t)rr * nout + rl] = acc[q];
+      }
+      continue;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) zs[wave][MfmaT<T>::row(lane, q)][rl] = (double)acc[q];
+    // zs is private to this wave: lanes 0..15 finish one row each (wave-scope ordering of the
+    // LDS writes above before the reads below)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane < 16) {
+      const int rr = r0 + lane;
+      if (rr < npts) {
+        double eps = 0.0;
+        for (int o = 0; o < nout; ++o) {
+          const double zz = zs[wave][lane][o] + (double)bias[o];
+          const double c = 2.0 / (1.0 + exp(-zz / 2.0));
+          out[(size_t)rr * nout + o] = (T)c;
+          eps += c * E[(size_t)rr * nout + o];
+        }
+        eps = fabs(eps) > 1e-30 ? eps : 0.0;
+        const double wt = weights[rr];
+        contrib += (fabs(wt) > 1e-30 ? wt : 0.0) * eps;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (HEAD) {
+    contrib = warp_sum(contrib);
+    if (lane == 0) red[wave] = contrib;
+    __syncthreads();
+    if (tid == 0) eps_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  }
+}
+
+// part[blk][j][f] = sum_r X16[r][j] G[r][f] for j <= nc (X16's column nc is the ones column)
+template <typename T>
+__global__ __launch_bounds__(256) void wgrad0_rows_kernel(const T* __restrict__ G, int h, int rows,
+                                                          const T* __restrict__ X16, int nc,
+                                                          double* __restrict__ part) {
+  typedef typename Vec4T<T>::type v4;
+  constexpr int NJ = 7;
+  __shared__ double red[4][NJ][256];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lpr = h >> 2, rpw = 64 / lpr;
+  const int sub = lane / lpr, li = lane % lpr;
+  const int f0 = li * 4;
+  T acc[NJ][4];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[j][e] = (T)0;
+  const int r0 = blockIdx.x * RW_ROWS + wave * (RW_ROWS / 4);
+  constexpr int U = sizeof(T) == 8 ? RW_UNROLL / 2 : RW_UNROLL;
+  for (int rb = sub; rb < RW_ROWS / 4; rb += U * rpw) {
+    v4 g[U], xa[U], xb[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = r0 + rb + u * rpw;
+      const bool ok = rb + u * rpw < RW_ROWS / 4 && r < rows;
+      g[u] = ok ? *(const v4*)(G + (size_t)r * h + f0) : (v4)(T)0;
+      xa[u] = ok ? *(const v4*)(X16 + (size_t)r * 16) : (v4)(T)0;
+      xb[u] = ok ? *(const v4*)(X16 + (size_t)r * 16 + 4) : (v4)(T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const T x[NJ] = {xa[u][0], xa[u][1], xa[u][2], xa[u][3], xb[u][0], xb[u][1], xb[u][2]};
+#pragma unroll
+      for (int j = 0; j < NJ; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[j][e] += x[j] * g[u][e];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NJ; ++j)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      double v = (double)acc[j][e];
+      for (int off = lpr; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+      if (sub == 0) red[wave][j][f0 + e] = v;
+    }
+  __syncthreads();
+  for (int i = threadIdx.x; i < (nc + 1) * h; i += 256) {
+    const int j = i / h, f = i % h;
+    part[(size_t)blockIdx.x * (nc + 1) * h + i] = red[0][j][f] + red[1][j][f] + red[2][j][f] + red[3][j][f];
+  }
+}
+
+}  // namespace gdx

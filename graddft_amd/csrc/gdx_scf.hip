@@ -270,8 +270,12 @@ __device__ __forceinline__ int rr_partner(int r, int k, int m, int* p, int* q) {
   return 0;
 }
 
+// Warm start (V0 != NULL, orthonormal columns): the sweeps run on V0^T A V0 starting from
+// V = V0, so a matrix close to one already diagonalised by V0 needs few sweeps; the result
+// is the same eigendecomposition (eigenvalues, and eigenvectors up to sign).
 template <bool SMEM>
 __global__ __launch_bounds__(EIG_THREADS) void jacobi_eigh_kernel(const double* __restrict__ Ain, int n,
+                                                                  const double* __restrict__ V0in,
                                                                   double* __restrict__ w_out,
                                                                   double* __restrict__ V_out,
                                                                   double* __restrict__ gws, int max_sweeps,
@@ -288,9 +292,38 @@ __global__ __launch_bounds__(EIG_THREADS) void jacobi_eigh_kernel(const double* 
   __shared__ int done;
   __shared__ double anorm;
   const double* Asrc = Ain + (size_t)b * nn;
-  for (size_t e = tid; e < nn; e += blockDim.x) {
-    A[e] = Asrc[e];
-    V[e] = (e / n == e % n) ? 1.0 : 0.0;
+  if (V0in == nullptr) {
+    for (size_t e = tid; e < nn; e += blockDim.x) {
+      A[e] = Asrc[e];
+      V[e] = (e / n == e % n) ? 1.0 : 0.0;
+    }
+  } else {
+    const double* V0 = V0in + (size_t)b * nn;
+    // V <- A V0, then A <- V0^T V, then V <- V0
+    for (size_t e = tid; e < nn; e += blockDim.x) {
+      const int i = (int)(e / n), j = (int)(e % n);
+      double t = 0.0;
+      for (int k = 0; k < n; ++k) t += Asrc[(size_t)i * n + k] * V0[(size_t)k * n + j];
+      V[e] = t;
+    }
+    __syncthreads();
+    for (size_t e = tid; e < nn; e += blockDim.x) {
+      const int i = (int)(e / n), j = (int)(e % n);
+      double t = 0.0;
+      for (int k = 0; k < n; ++k) t += V0[(size_t)k * n + i] * V[(size_t)k * n + j];
+      A[e] = t;
+    }
+    __syncthreads();
+    // symmetrise the rotated matrix (rounding) and load V0
+    for (size_t e = tid; e < nn; e += blockDim.x) {
+      const int i = (int)(e / n), j = (int)(e % n);
+      if (i < j) {
+        const double a = 0.5 * (A[(size_t)i * n + j] + A[(size_t)j * n + i]);
+        A[(size_t)i * n + j] = a;
+        A[(size_t)j * n + i] = a;
+      }
+      V[e] = V0[e];
+    }
   }
   __syncthreads();
   const int m = n + (n & 1);  // players incl. a dummy when n is odd
@@ -512,17 +545,18 @@ int gdx_eigh_workspace_size(int32_t nbatch, int32_t n, size_t* bytes) {
   return GDX_OK;
 }
 
-int gdx_eigh_batched(void* stream, int32_t nbatch, int32_t n, const double* A, double* w, double* V,
-                     void* workspace, int32_t max_sweeps, double tol, int32_t* sweeps) {
+int gdx_eigh_batched(void* stream, int32_t nbatch, int32_t n, const double* A, const double* V0, double* w,
+                     double* V, void* workspace, int32_t max_sweeps, double tol, int32_t* sweeps) {
   GDX_REQUIRE(nbatch > 0 && n > 0 && n <= 1022 && A && w && V, "gdx_eigh_batched: bad arguments");
+  GDX_REQUIRE(V0 == nullptr || (V0 != V && V0 != A), "gdx_eigh_batched: V0 must not alias A or V");
   hipStream_t st = (hipStream_t)stream;
   const size_t smem = (size_t)2 * n * n * sizeof(double);
   if (smem <= 64 * 1024) {
-    GDX_LAUNCH("jacobi_eigh", jacobi_eigh_kernel<true>, dim3(nbatch), dim3(EIG_THREADS), smem, st, A, n, w, V,
+    GDX_LAUNCH("jacobi_eigh", jacobi_eigh_kernel<true>, dim3(nbatch), dim3(EIG_THREADS), smem, st, A, n, V0, w, V,
                (double*)workspace, max_sweeps, tol, sweeps);
   } else {
     GDX_REQUIRE(workspace, "gdx_eigh_batched: workspace required for n > 64");
-    GDX_LAUNCH("jacobi_eigh", jacobi_eigh_kernel<false>, dim3(nbatch), dim3(EIG_THREADS), 0, st, A, n, w, V,
+    GDX_LAUNCH("jacobi_eigh", jacobi_eigh_kernel<false>, dim3(nbatch), dim3(EIG_THREADS), 0, st, A, n, V0, w, V,
                (double*)workspace, max_sweeps, tol, sweeps);
   }
   GDX_HIP_CHECK(hipGetLastError());

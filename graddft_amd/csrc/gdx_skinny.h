@@ -195,8 +195,18 @@ __global__ __launch_bounds__(256) void reduce_cols_kernel(const TP* __restrict__
   const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
   const long i = (long)blockIdx.x * 64 + c;
   double acc = 0.0;
-  if (i < len)
-    for (int s = g; s < nsplit; s += 4) acc += (double)part[(long)s * stride + i];
+  if (i < len) {
+    // 8 independent loads in flight per thread; the sum order stays fixed
+    int s = g;
+    for (; s + 28 < nsplit; s += 32) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = (double)part[(long)(s + 4 * u) * stride + i];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += v[u];
+    }
+    for (; s < nsplit; s += 4) acc += (double)part[(long)s * stride + i];
+  }
   red[g][c] = acc;
   __syncthreads();
   if (g == 0 && i < len) out[i] = red[0][c] + red[1][c] + red[2][c] + red[3][c];

@@ -28,6 +28,7 @@
 #include "gdx_gemm_bf16.h"
 #include "gdx_skinny.h"
 #include "gdx_aogemm.h"
+#include "gdx_rows.h"
 
 #include <algorithm>
 #include <vector>
@@ -101,6 +102,33 @@ __global__ void reduce_vxc_kernel(const double* __restrict__ part, int nchunk, i
   double acc = 0.0;
   for (int c = 0; c < nchunk; ++c) acc += part[c * stride + off];
   out[i] = acc;
+}
+
+// First stage of a two-stage sum over many slices: out[y][i] = sum over slices s in
+// [y per, (y+1) per) of part[s * stride + i]; a block covers 64 consecutive i with 4 slice
+// subgroups combined in LDS (fixed order: deterministic).
+template <typename TP>
+__global__ __launch_bounds__(256) void reduce_stage1_kernel(const TP* __restrict__ part, int nslices, long stride,
+                                                            long len, int per, double* __restrict__ out) {
+  __shared__ double red[4][64];
+  const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const long i = (long)blockIdx.x * 64 + c;
+  const int s0 = blockIdx.y * per, s1 = min(nslices, s0 + per);
+  double acc = 0.0;
+  if (i < len) {
+    int s = s0 + g;
+    for (; s + 12 < s1; s += 16) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = (double)part[(long)(s + 4 * u) * stride + i];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc += v[u];
+    }
+    for (; s < s1; s += 4) acc += (double)part[(long)s * stride + i];
+  }
+  red[g][c] = acc;
+  __syncthreads();
+  if (g == 0 && i < len) out[(long)blockIdx.y * len + i] = red[0][c] + red[1][c] + red[2][c] + red[3][c];
 }
 
 // ---------------------------------------------------------------- per-point fwd ----
@@ -616,6 +644,7 @@ __global__ __launch_bounds__(NT) void point_hvp_bwd_kernel(const double* __restr
 
 // ------------------------------------------------------------ workspace layout ----
 constexpr int NUM_CU = 256;    // MI355X compute units
+constexpr int RED_GROUPS = 32; // slice groups of the first reduction stage
 
 // Point chunking of the V_xc build: the grid (tiles x chunks x spins) is sized to whole
 // rounds of co-resident blocks (resident blocks per CU from the kernels' register and LDS
@@ -647,7 +676,7 @@ constexpr int SKW_ROWS = 512;  // grid points per block of the skinny weight-gra
 struct Layout {
   size_t Dp, dens_part, dens_fin, E, X, C, H, eps_part, GE, DZ, W16, Wlp, dbpart, dWt16, G, DX, vpt, vpart,
       wpart, cpart;
-  size_t Dp2, dpart_t, dfin_t, Edot, ZT, GEd, edot_part, scal;
+  size_t Dp2, dpart_t, dfin_t, Edot, ZT, GEd, edot_part, scal, rtmp;
   size_t total;
   size_t hstride;  // elements between the activation buffers of consecutive layers
   int nsd, nchunk, chunk, nsplit, kchunk, npt_blocks, nwblk, wrows, rows;
@@ -713,9 +742,11 @@ Layout make_layout(const gdx_xc_desc& d, bool hvp = false) {
   if (neural) {
     wbytes = (size_t)L.nsplit * h * h * es;
     wbytes = std::max(wbytes, (size_t)L.nwblk * 16 * h * 8);
+    wbytes = std::max(wbytes, (size_t)((R + RW_ROWS - 1) / RW_ROWS) * 7 * h * 8);
   }
   L.wpart = take(wbytes);
   L.cpart = take(neural ? (size_t)L.nsplit * h * es : 0);
+  L.rtmp = take((size_t)RED_GROUPS * std::max(std::max(h * h, 16 * h), 2 * nP * nP) * 8);
   if (hvp) {
     L.Dp2 = take(2 * nP * nP * es);
     L.dpart_t = take(2 * Np * DENS_STRIDE * 8);
@@ -751,6 +782,8 @@ int validate(const gdx_xc_desc* d) {
 }
 
 inline unsigned nblocks(long n) { return (unsigned)((n + NT - 1) / NT); }
+// the streaming edge kernels split a row over h / 4 lanes in xor groups: h a power of two
+inline bool rows_path(int h) { return h >= 16 && (h & (h - 1)) == 0; }
 inline unsigned nblocks64(long n) { return (unsigned)((n + 63) / 64); }
 
 template <typename T> const char* tname(const char* f64, const char* f32) { return sizeof(T) == 8 ? f64 : f32; }
@@ -835,6 +868,24 @@ const T* lowp_params(const Ctx& c, const double* params) {
   return dst;
 }
 
+// out[i] = sum_s part[s * stride + i] for i < len: one pass for few slices, else two
+// stages through L.rtmp (RED_GROUPS groups), so long slice lists run at full width.
+template <typename TP>
+void reduce_slices(const Ctx& c, const TP* part, int nslices, long stride, long len, double* out) {
+  hipStream_t st = c.st;
+  if (nslices <= 16 || (nslices <= 128 && len >= 16384)) {
+    GDX_LAUNCH("reduce_cols", reduce_cols_kernel<TP>, dim3(nblocks64(len)), dim3(NT), 0, st, part, nslices, stride,
+               len, out);
+    return;
+  }
+  const int per = (nslices + RED_GROUPS - 1) / RED_GROUPS;
+  const int G = (nslices + per - 1) / per;
+  double* tmp = c.P<double>(c.L.rtmp);
+  GDX_LAUNCH("reduce_cols", reduce_stage1_kernel<TP>, dim3(nblocks64(len), G), dim3(NT), 0, st, part, nslices, stride,
+             len, per, tmp);
+  GDX_LAUNCH("reduce_cols", reduce_cols_kernel<double>, dim3(nblocks64(len)), dim3(NT), 0, st, tmp, G, len, len, out);
+}
+
 // V_s (nsv spins) from the per-point potentials in L.vpt, reduced into d_rdm1 (2, n, n)
 template <typename T>
 void launch_vxc(const Ctx& c, const T* ao, int nsv, double* d_rdm1) {
@@ -867,8 +918,14 @@ void launch_vxc(const Ctx& c, const T* ao, int nsv, double* d_rdm1) {
   else
     GDX_LAUNCH(nm, (vxc_ao_kernel<T, false, 128>), grid, dim3(NT), 0, st, ao, Np, nP, c.P<double>(L.vpt), v.chunk,
                c.P<double>(L.vpart));
+  // chunk partials -> RED_GROUPS partials (first stage), then the (2, n, n) output
+  const long len = (long)nsv * nP * nP;
+  const int per = (v.nchunk + RED_GROUPS - 1) / RED_GROUPS;
+  const int G = (v.nchunk + per - 1) / per;
+  GDX_LAUNCH("reduce_vxc", reduce_stage1_kernel<double>, dim3(nblocks64(len), G), dim3(NT), 0, st,
+             c.P<double>(L.vpart), v.nchunk, len, len, per, c.P<double>(L.rtmp));
   GDX_LAUNCH("reduce_vxc", reduce_vxc_kernel, dim3(nblocks(2L * d.nao * d.nao)), dim3(NT), 0, st,
-             c.P<double>(L.vpart), v.nchunk, nsv, nP, d.nao, d_rdm1);
+             c.P<double>(L.rtmp), G, nsv, nP, d.nao, d_rdm1);
 }
 
 template <typename T>
@@ -906,10 +963,17 @@ int forward_impl(const Ctx& c, const T* ao, const double* weights, const double*
         launch_any<T, 128, 128, false, false, EPI_BIAS_ELU>(st, "gemm_any", Np, h, h, Hin, h, Wl, h, Hout, h, bl,
                                                             nullptr, 0, 1, h, nullptr);
     }
-    GDX_LAUNCH("head_fwd", (skinny_out16_kernel<T, true>), dim3(Np / 64), dim3(NT), 0, st, Np, h,
-               H + (size_t)(nl - 1) * L.hstride, h, WT16, c.P<T>(L.C), d.mlp_nd, d.mlp_nd, plp + po.b[nl], d.mlp_nd,
-               c.P<double>(L.E), weights, d.npts, c.P<double>(L.eps_part));
-    nred = Np / 64;
+    if (rows_path(h)) {
+      nred = (Np + RW_ROWS - 1) / RW_ROWS;
+      GDX_LAUNCH("head_fwd", (rowdot_mfma_kernel<T, true>), dim3(nred), dim3(NT), 0, st, Np, h,
+                 H + (size_t)(nl - 1) * L.hstride, WT16, c.P<T>(L.C), d.mlp_nd, plp + po.b[nl], c.P<double>(L.E),
+                 weights, d.npts, c.P<double>(L.eps_part));
+    } else {
+      GDX_LAUNCH("head_fwd", (skinny_out16_kernel<T, true>), dim3(Np / 64), dim3(NT), 0, st, Np, h,
+                 H + (size_t)(nl - 1) * L.hstride, h, WT16, c.P<T>(L.C), d.mlp_nd, d.mlp_nd, plp + po.b[nl],
+                 d.mlp_nd, c.P<double>(L.E), weights, d.npts, c.P<double>(L.eps_part));
+      nred = Np / 64;
+    }
   }
   GDX_LAUNCH("reduce_scalar", reduce_scalar_kernel, dim3(1), dim3(NT), 0, st, c.P<double>(L.eps_part), nred, exc);
   GDX_HIP_CHECK(hipGetLastError());
@@ -940,17 +1004,25 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
     GDX_LAUNCH("head_seed", head_seed_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, c.P<T>(L.C),
                c.P<double>(L.E), weights, nd, d.npts, Np, g_exc, 1, c.P<double>(L.GE), c.P<T>(L.DZ),
                c.P<double>(L.dbpart));
-    GDX_LAUNCH("head_delta", (edge_rows_kernel<T, ER_DELTA>), dim3((Np + ER_ROWS - 1) / ER_ROWS), dim3(NT), 0, st,
-               c.P<T>(L.DZ), nd, W016 + 16 * h, nullptr, h, (long)Np, Hlast, G0);
+    const int rwblk = (Np + RW_ROWS - 1) / RW_ROWS;
+    if (rows_path(h)) {
+      GDX_LAUNCH("head_delta", delta_wgrad_rows_kernel<T>, dim3(rwblk), dim3(NT), 0, st, Hlast, h, Np, c.P<T>(L.DZ),
+                 W016 + 16 * h, nd, G0, c.P<double>(L.wpart));
+    } else {
+      GDX_LAUNCH("head_delta", (edge_rows_kernel<T, ER_DELTA>), dim3((Np + ER_ROWS - 1) / ER_ROWS), dim3(NT), 0, st,
+                 c.P<T>(L.DZ), nd, W016 + 16 * h, nullptr, h, (long)Np, Hlast, G0);
+      if (d_params)
+        GDX_LAUNCH("skinny_wgrad", skinny_wgrad_kernel<T>, dim3(L.nwblk), dim3(NT), 0, st, Np, L.wrows, c.P<T>(L.DZ),
+                   Hlast, h, c.P<double>(L.wpart));
+    }
     if (d_params) {
-      GDX_LAUNCH("skinny_wgrad", skinny_wgrad_kernel<T>, dim3(L.nwblk), dim3(NT), 0, st, Np, L.wrows, c.P<T>(L.DZ),
-                 Hlast, h, c.P<double>(L.wpart));
-      GDX_LAUNCH("reduce_cols", reduce_cols_kernel<double>, dim3(nblocks64(16L * h)), dim3(NT), 0, st,
-                 c.P<double>(L.wpart), L.nwblk, 16L * h, 16L * h, c.P<double>(L.dWt16));
+      if (rows_path(h))
+        reduce_slices<double>(c, c.P<double>(L.wpart), rwblk, (long)nd * h, (long)nd * h, c.P<double>(L.dWt16));
+      else
+        reduce_slices<double>(c, c.P<double>(L.wpart), L.nwblk, 16L * h, 16L * h, c.P<double>(L.dWt16));
       GDX_LAUNCH("unpack_last", unpack_last_kernel, dim3(nblocks((long)h * nd)), dim3(NT), 0, st,
                  c.P<double>(L.dWt16), h, nd, d_params + po.w[nl]);
-      GDX_LAUNCH("reduce_cols", reduce_cols_kernel<double>, dim3(1), dim3(NT), 0, st, c.P<double>(L.dbpart),
-                 L.npt_blocks, (long)nd, (long)nd, d_params + po.b[nl]);
+      reduce_slices<double>(c, c.P<double>(L.dbpart), L.npt_blocks, (long)nd, (long)nd, d_params + po.b[nl]);
     }
     T* Gcur = G0;
     T* Gnext = G1;
@@ -967,10 +1039,8 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
         else
           launch_any<T, 64, 64, true, false, EPI_STORE>(st, "gemm_any", h, h, Np, Hin, h, Gcur, h, wp, h, nullptr,
                                                         nullptr, 0, L.nsplit, L.kchunk, cp);
-        GDX_LAUNCH("reduce_cols", reduce_cols_kernel<T>, dim3(nblocks64((long)h * h)), dim3(NT), 0, st, wp, L.nsplit,
-                   (long)h * h, (long)h * h, d_params + po.w[l]);
-        GDX_LAUNCH("reduce_cols", reduce_cols_kernel<T>, dim3(nblocks64(h)), dim3(NT), 0, st, cp, L.nsplit, (long)h,
-                   (long)h, d_params + po.b[l]);
+        reduce_slices<T>(c, wp, L.nsplit, (long)h * h, (long)h * h, d_params + po.w[l]);
+        reduce_slices<T>(c, cp, L.nsplit, (long)h, (long)h, d_params + po.b[l]);
       }
       // delta for layer l's input: (Gcur W_l^T) * elu'(Hin)
       if (tiles)
@@ -981,15 +1051,25 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
                                                           nullptr, Hin, h, 1, h, nullptr);
       std::swap(Gcur, Gnext);
     }
-    // first layer: DX = G W0^T (skinny), dW0 and db0 = X16^T G (rows 0..nc-1 and the ones row nc)
-    GDX_LAUNCH("layer0_bwd", (skinny_out16_kernel<T, false>), dim3(Np / 64), dim3(NT), 0, st, Np, h, Gcur, h,
-               W016, c.P<T>(L.DX), d.mlp_nc, d.mlp_nc, nullptr, 0, nullptr, nullptr, 0, nullptr);
-    if (d_params) {
-      GDX_LAUNCH("skinny_wgrad", skinny_wgrad_kernel<T>, dim3(L.nwblk), dim3(NT), 0, st, Np, L.wrows, c.P<T>(L.X),
-                 Gcur, h, c.P<double>(L.wpart));
-      // rows 0..nc of the 16 x h result are W0 (nc x h) followed by b0: contiguous in params
-      GDX_LAUNCH("reduce_cols", reduce_cols_kernel<double>, dim3(nblocks64((long)(d.mlp_nc + 1) * h)), dim3(NT), 0,
-                 st, c.P<double>(L.wpart), L.nwblk, 16L * h, (long)(d.mlp_nc + 1) * h, d_params + po.w[0]);
+    // first layer: DX = G W0^T, dW0 and db0 = X16^T G (rows 0..nc-1 and the ones row nc);
+    // W0 (nc x h) followed by b0 is contiguous in params
+    const long l0len = (long)(d.mlp_nc + 1) * h;
+    if (rows_path(h)) {
+      GDX_LAUNCH("layer0_bwd", (rowdot_mfma_kernel<T, false>), dim3(rwblk), dim3(NT), 0, st, Np, h, Gcur, W016,
+                 c.P<T>(L.DX), d.mlp_nc, nullptr, nullptr, nullptr, 0, nullptr);
+      if (d_params) {
+        GDX_LAUNCH("wgrad0", wgrad0_rows_kernel<T>, dim3(rwblk), dim3(NT), 0, st, Gcur, h, Np, c.P<T>(L.X), d.mlp_nc,
+                   c.P<double>(L.wpart));
+        reduce_slices<double>(c, c.P<double>(L.wpart), rwblk, l0len, l0len, d_params + po.w[0]);
+      }
+    } else {
+      GDX_LAUNCH("layer0_bwd", (skinny_out16_kernel<T, false>), dim3(Np / 64), dim3(NT), 0, st, Np, h, Gcur, h,
+                 W016, c.P<T>(L.DX), d.mlp_nc, d.mlp_nc, nullptr, 0, nullptr, nullptr, 0, nullptr);
+      if (d_params) {
+        GDX_LAUNCH("skinny_wgrad", skinny_wgrad_kernel<T>, dim3(L.nwblk), dim3(NT), 0, st, Np, L.wrows, c.P<T>(L.X),
+                   Gcur, h, c.P<double>(L.wpart));
+        reduce_slices<double>(c, c.P<double>(L.wpart), L.nwblk, 16L * h, l0len, d_params + po.w[0]);
+      }
     }
   } else {
     GDX_LAUNCH("head_seed", head_seed_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, nullptr, c.P<double>(L.E),
@@ -1076,12 +1156,10 @@ int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm
     if (d_params) {
       GDX_LAUNCH("skinny_wgrad", skinny_wgrad_kernel<T>, dim3(L.nwblk), dim3(NT), 0, st, R, L.wrows, DZ2, Hlast2, h,
                  c.P<double>(L.wpart));
-      GDX_LAUNCH("reduce_cols", reduce_cols_kernel<double>, dim3(nblocks64(16L * h)), dim3(NT), 0, st,
-                 c.P<double>(L.wpart), L.nwblk, 16L * h, 16L * h, c.P<double>(L.dWt16));
+      reduce_slices<double>(c, c.P<double>(L.wpart), L.nwblk, 16L * h, 16L * h, c.P<double>(L.dWt16));
       GDX_LAUNCH("unpack_last", unpack_last_kernel, dim3(nblocks((long)h * nd)), dim3(NT), 0, st,
                  c.P<double>(L.dWt16), h, nd, d_params + po.w[nl]);
-      GDX_LAUNCH("reduce_cols", reduce_cols_kernel<double>, dim3(1), dim3(NT), 0, st, c.P<double>(L.dbpart),
-                 L.npt_blocks, (long)nd, (long)nd, d_params + po.b[nl]);
+      reduce_slices<double>(c, c.P<double>(L.dbpart), L.npt_blocks, (long)nd, (long)nd, d_params + po.b[nl]);
     }
     T* Gcur = G0;
     T* Gnext = G1;
@@ -1098,11 +1176,9 @@ int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm
         else
           launch_any<T, 64, 64, true, false, EPI_STORE>(st, "gemm_any", h, h, R, Hin2, h, Gcur, h, wp, h, nullptr,
                                                         nullptr, 0, L.nsplit, L.kchunk, cp);
-        GDX_LAUNCH("reduce_cols", reduce_cols_kernel<T>, dim3(nblocks64((long)h * h)), dim3(NT), 0, st, wp, L.nsplit,
-                   (long)h * h, (long)h * h, d_params + po.w[l]);
+        reduce_slices<T>(c, wp, L.nsplit, (long)h * h, (long)h * h, d_params + po.w[l]);
         // bias: column sums over the value rows only (the first half of the splits)
-        GDX_LAUNCH("reduce_cols", reduce_cols_kernel<T>, dim3(nblocks64(h)), dim3(NT), 0, st, cp, L.nsplit / 2,
-                   (long)h, (long)h, d_params + po.b[l]);
+        reduce_slices<T>(c, cp, L.nsplit / 2, (long)h, (long)h, d_params + po.b[l]);
       }
       if (tiles)
         launch_tile<T, false, true, EPI_STORE>(st, tname<T>("gemm_f64_bwd_data", "gemm_f32_bwd_data"), R, h, 1, h,
@@ -1119,8 +1195,7 @@ int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm
     if (d_params) {
       GDX_LAUNCH("skinny_wgrad", skinny_wgrad_kernel<T>, dim3(L.nwblk), dim3(NT), 0, st, R, L.wrows, X2, Gcur, h,
                  c.P<double>(L.wpart));
-      GDX_LAUNCH("reduce_cols", reduce_cols_kernel<double>, dim3(nblocks64((long)(d.mlp_nc + 1) * h)), dim3(NT), 0,
-                 st, c.P<double>(L.wpart), L.nwblk, 16L * h, (long)(d.mlp_nc + 1) * h, d_params + po.w[0]);
+      reduce_slices<double>(c, c.P<double>(L.wpart), L.nwblk, 16L * h, (long)(d.mlp_nc + 1) * h, d_params + po.w[0]);
     }
   }
   if (d_rdm1) {

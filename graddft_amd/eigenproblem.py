@@ -10,7 +10,7 @@ attached to the device eigensolver by :class:`EighFn` when autograd is recording
 from __future__ import annotations
 
 import ctypes
-from typing import Tuple
+from typing import Optional, Tuple
 
 import torch
 
@@ -37,20 +37,23 @@ def cholesky_inv(S: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
     return L, Linv
 
 
-def eigh(A: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
-    """Batched eigh of symmetric A (..., n, n): ascending eigenvalues, eigenvectors as columns."""
+def eigh(A: torch.Tensor, V0: Optional[torch.Tensor] = None) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Batched eigh of symmetric A (..., n, n): ascending eigenvalues, eigenvectors as columns.
+    V0 (same shape, orthonormal columns): starting eigenvector estimates (warm start)."""
     shape = A.shape
     n = shape[-1]
     Ab = A.reshape(-1, n, n).contiguous()
     nb = Ab.shape[0]
     w = torch.empty((nb, n), dtype=torch.float64, device=A.device)
     V = torch.empty_like(Ab)
+    V0b = None if V0 is None else V0.detach().reshape(-1, n, n).contiguous()
     nbytes = ctypes.c_size_t(0)
     lib = _lib.load()
     _lib.check(lib.gdx_eigh_workspace_size(nb, n, ctypes.byref(nbytes)), "gdx_eigh_workspace_size")
     ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=A.device)
-    _lib.check(lib.gdx_eigh_batched(_stream(A), nb, n, Ab.data_ptr(), w.data_ptr(), V.data_ptr(), ws.data_ptr(),
-                                    JACOBI_MAX_SWEEPS, JACOBI_TOL, 0), "gdx_eigh_batched")
+    _lib.check(lib.gdx_eigh_batched(_stream(A), nb, n, Ab.data_ptr(), 0 if V0b is None else V0b.data_ptr(),
+                                    w.data_ptr(), V.data_ptr(), ws.data_ptr(), JACOBI_MAX_SWEEPS, JACOBI_TOL, 0),
+               "gdx_eigh_batched")
     return w.reshape(shape[:-1]), V.reshape(shape)
 
 
@@ -58,8 +61,8 @@ class EighFn(torch.autograd.Function):
     """Batched eigh on the device with the reverse mode of safe_eigh (safe_eigh_vjp)."""
 
     @staticmethod
-    def forward(ctx, A):
-        w, V = eigh(A)
+    def forward(ctx, A, V0=None):
+        w, V = eigh(A, V0)
         ctx.save_for_backward(w, V)
         return w, V
 
@@ -72,19 +75,25 @@ class EighFn(torch.autograd.Function):
         gw = torch.zeros_like(wb) if gw is None else gw.reshape(-1, n)
         gV = torch.zeros_like(Vb) if gV is None else gV.reshape(-1, n, n)
         out = torch.stack([safe_eigh_vjp(wb[i], Vb[i], gw[i], gV[i]) for i in range(wb.shape[0])])
-        return out.reshape(V.shape)
+        return out.reshape(V.shape), None
+
+
+def orthogonal_eigh(A: torch.Tensor, Linv: torch.Tensor, V0: Optional[torch.Tensor] = None):
+    """(w, C, V): eigh of Linv A Linv^T = V diag(w) V^T and C = Linv^T V; V0 warm-starts."""
+    C = Linv @ A @ Linv.transpose(-1, -2)
+    C = 0.5 * (C + C.transpose(-1, -2))
+    if torch.is_grad_enabled() and C.requires_grad:
+        w, V = EighFn.apply(C, V0)
+    else:
+        w, V = eigh(C, V0)
+    return w, Linv.transpose(-1, -2) @ V, V
 
 
 def safe_general_eigh(A: torch.Tensor, B: torch.Tensor, Linv: torch.Tensor = None):
     if Linv is None:
         _, Linv = cholesky_inv(B)
-    C = Linv @ A @ Linv.transpose(-1, -2)
-    C = 0.5 * (C + C.transpose(-1, -2))
-    if torch.is_grad_enabled() and C.requires_grad:
-        w, V = EighFn.apply(C)
-    else:
-        w, V = eigh(C)
-    return w, Linv.transpose(-1, -2) @ V
+    w, C, _ = orthogonal_eigh(A, Linv)
+    return w, C
 
 
 def safe_fock_solver(fock: torch.Tensor, overlap: torch.Tensor, Linv: torch.Tensor = None):

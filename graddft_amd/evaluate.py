@@ -22,7 +22,7 @@ from typing import Callable, List, Optional
 import torch
 
 from . import _lib
-from .eigenproblem import cholesky_inv, safe_fock_solver
+from .eigenproblem import cholesky_inv, orthogonal_eigh
 from .functional import Functional
 from .molecule import Molecule
 from .train import energy_predictor
@@ -85,8 +85,7 @@ class JittableDiis:
         gram = E @ E.transpose(1, 2)                             # (2, m, m)
         B = torch.zeros((2, m + 1, m + 1), dtype=torch.float64, device=errs.device)
         B[:, 1:, 1:] = gram
-        active = torch.tensor([1.0 if i <= cycle else 0.0 for i in range(m)], dtype=torch.float64,
-                              device=errs.device)
+        active = (torch.arange(m, device=errs.device) <= cycle).to(torch.float64)
         B[:, 0, 1:] = active
         B[:, 1:, 0] = active
         idx = torch.arange(m, device=errs.device)
@@ -112,10 +111,27 @@ def non_scf_predictor(functional: Functional, precision: str = "fp64", **kwargs)
     return predictor
 
 
-def diff_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp64", **kwargs) -> Callable:
+def _aufbau_occupations(molecule: Molecule) -> torch.Tensor:
+    """get_occ for ascending eigenvalues: the first N_e,s orbitals of each spin (cached)."""
+    occ = molecule.cache.get("aufbau_occ")
+    if occ is None:
+        nelecs = [int(round(float(v))) for v in molecule.mo_occ.sum(dim=1).tolist()]
+        occ = torch.zeros((2, molecule.nao), dtype=torch.float64, device=molecule.s1e.device)
+        for s in range(2):
+            occ[s, : nelecs[s]] = 1.0
+        molecule.cache["aufbau_occ"] = occ
+    return occ
+
+
+def diff_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp64", graph: bool = False,
+                  **kwargs) -> Callable:
+    """graph=True (forward only, autograd not recording): the whole loop is captured once per
+    molecule as one HIP graph (after an eager run that sets up plans and workspaces) and
+    replayed on later calls -- no host work between the kernels, as the reference's jitted
+    fori_loop.  The rdm1 and params of each call are copied into the captured inputs."""
     compute_energy = energy_predictor(functional, precision=precision, **kwargs)
 
-    def scf_iterator(params: Optional[torch.Tensor], molecule: Molecule) -> Molecule:
+    def run_loop(params: Optional[torch.Tensor], molecule: Molecule) -> Molecule:
         S = molecule.s1e
         Linv = molecule.cache.get("Linv")
         if Linv is None:
@@ -125,14 +141,12 @@ def diff_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp
         molecule = molecule.replace(fock=fock)
         diis = JittableDiis(S, max_diis=10)
         data = diis.zeros(molecule.nao, S.device)
-        nelecs = [int(round(float(v))) for v in molecule.mo_occ.sum(dim=1).tolist()]
-        n = molecule.nao
-        occ = torch.zeros((2, n), dtype=torch.float64, device=S.device)
-        for s in range(2):
-            occ[s, : nelecs[s]] = 1.0
+        occ = _aufbau_occupations(molecule)
+        V = None
         for cycle in range(cycles):
             F, data = diis.run(molecule.rdm1, molecule.fock, predicted_e, data, cycle)
-            mo_energy, mo_coeff = safe_fock_solver(F, S, Linv)
+            # safe_fock_solver, warm-started from the previous cycle's orthonormal eigenvectors
+            mo_energy, mo_coeff, V = orthogonal_eigh(F, Linv, V)
             # eigenvalues ascend, so the aufbau occupation of get_occ is the first nelec_s
             rdm1 = torch.einsum("sij,sj,skj->sik", mo_coeff, occ, mo_coeff)
             molecule = molecule.replace(fock=F, mo_coeff=mo_coeff, mo_energy=mo_energy, mo_occ=occ,
@@ -140,5 +154,36 @@ def diff_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp
             predicted_e, fock = compute_energy(params, molecule)
             molecule = molecule.replace(fock=fock)
         return molecule.replace(energy=predicted_e)
+
+    def scf_iterator(params: Optional[torch.Tensor], molecule: Molecule) -> Molecule:
+        recording = torch.is_grad_enabled() and (
+            (params is not None and params.requires_grad) or molecule.rdm1.requires_grad)
+        if not graph or recording:
+            return run_loop(params, molecule)
+        key = ("scf_graph", id(functional), cycles, precision, params is None)
+        entry = molecule.cache.get(key)
+        if entry is None:
+            out = run_loop(params, molecule)          # plans, workspaces, Cholesky, K0
+            static_p = None if params is None else params.detach().clone()
+            static_D = molecule.rdm1.detach().clone()
+            m_in = molecule.replace(rdm1=static_D)
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                run_loop(static_p, m_in)
+            torch.cuda.current_stream().wait_stream(side)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                m_out = run_loop(static_p, m_in)
+            molecule.cache[key] = (g, static_p, static_D, m_out)
+            return out
+        g, static_p, static_D, m_out = entry
+        if static_p is not None:
+            static_p.copy_(params.detach())
+        static_D.copy_(molecule.rdm1.detach())
+        g.replay()
+        return molecule.replace(energy=m_out.energy.clone(), rdm1=m_out.rdm1.clone(), fock=m_out.fock.clone(),
+                                mo_coeff=m_out.mo_coeff.clone(), mo_energy=m_out.mo_energy.clone(),
+                                mo_occ=m_out.mo_occ.clone(), restricted=m_out.restricted)
 
     return scf_iterator

@@ -52,7 +52,7 @@ typedef struct gdx_xc_desc {
   int32_t nao_pad;         /* columns of the AO buffer, multiple of 16, >= nao */
   int32_t spin_restricted; /* 1 when rdm1[0] == rdm1[1]: densities are formed once */
   int32_t kind;            /* GDX_FUNC_* */
-  int32_t precision;       /* GDX_FP64 | GDX_FP32 | GDX_BF16X3 */
+  int32_t precision;       /* GDX_FP64 | GDX_FP32 | GDX_BF16X6 */
   int32_t mlp_nc;          /* MLP inputs (6: rho, |grad rho|^2, tau per spin) */
   int32_t mlp_h;           /* hidden width */
   int32_t mlp_nhidden;     /* hidden layers (dense + ELU each) */
@@ -151,10 +151,13 @@ int gdx_cholesky_inv(void* stream, int32_t n, const double* S, double* L, double
 
 /* Batched symmetric eigensolver (jnp.linalg.eigh in safe_eigh, eigenproblem.py:26-38):
  * A (nbatch, n, n) -> w (nbatch, n) ascending, V (nbatch, n, n) eigenvectors as columns.
- * workspace: gdx_eigh_workspace_size bytes (used when n > 64).  sweeps: optional device
- * int per matrix (Jacobi sweeps used). */
+ * V0 (nbatch, n, n), optional (NULL = none): orthonormal starting eigenvector estimates (the
+ * previous SCF cycle's), which cut the Jacobi sweeps for a nearby matrix; it must not alias
+ * A or V.  workspace: gdx_eigh_workspace_size bytes (used when n > 64).  sweeps: optional
+ * device int per matrix (Jacobi sweeps used). */
 int gdx_eigh_workspace_size(int32_t nbatch, int32_t n, size_t* bytes);
-int gdx_eigh_batched(void* stream, int32_t nbatch, int32_t n, const double* A, double* w, double* V,
+int gdx_eigh_batched(void* stream, int32_t nbatch, int32_t n, const double* A, const double* V0,
+                     double* w, double* V,
                      void* workspace, int32_t max_sweeps, double tol, int32_t* sweeps);
 
 /* x_s = B_s^-1 e_0 for nspin bordered DIIS matrices B (nspin, m1, m1), m1 <= 32:

@@ -59,6 +59,43 @@ def test_eigh_matches_lapack(n):
         assert np.abs(V[s].T @ V[s] - np.eye(n)).max() <= 1e-12
 
 
+def test_eigh_warm_start():
+    """A warm start from the eigenvectors of a nearby matrix gives the same decomposition
+    in fewer sweeps (the SCF loop passes the previous cycle's eigenvectors)."""
+    _say("case: eigh warm start n=43")
+    import ctypes
+    from graddft_amd import _lib
+    n = 43
+    rng = np.random.default_rng(7)
+    A = rng.normal(size=(2, n, n))
+    A = A + A.transpose(0, 2, 1)
+    dA = rng.normal(size=(2, n, n)) * 1e-3
+    A1 = A + dA + dA.transpose(0, 2, 1)
+    V0 = np.stack([np.linalg.eigh(A[s])[1] for s in range(2)])
+    lib = _lib.load()
+    nbytes = ctypes.c_size_t(0)
+    _lib.check(lib.gdx_eigh_workspace_size(2, n, ctypes.byref(nbytes)), "ws")
+    ws = torch.empty(max(int(nbytes.value), 8), dtype=torch.uint8, device="cuda")
+    At = torch.as_tensor(A1, device="cuda")
+    V0t = torch.as_tensor(V0, device="cuda")
+    out = {}
+    for name, v0 in (("cold", None), ("warm", V0t)):
+        w = torch.empty((2, n), dtype=torch.float64, device="cuda")
+        V = torch.empty_like(At)
+        sw = torch.zeros(2, dtype=torch.int32, device="cuda")
+        _lib.check(lib.gdx_eigh_batched(torch.cuda.current_stream().cuda_stream, 2, n, At.data_ptr(),
+                                        0 if v0 is None else v0.data_ptr(), w.data_ptr(), V.data_ptr(),
+                                        ws.data_ptr(), 40, 1e-14, sw.data_ptr()), "eigh")
+        out[name] = (w.cpu().numpy(), V.cpu().numpy(), sw.cpu().numpy())
+    for s in range(2):
+        w0 = np.linalg.eigvalsh(A1[s])
+        w, V, _ = out["warm"]
+        assert np.abs(w[s] - w0).max() <= 1e-12 * np.abs(w0).max()
+        assert np.abs(A1[s] @ V[s] - V[s] * w[s]).max() <= 1e-11 * np.abs(w0).max()
+        assert np.abs(V[s].T @ V[s] - np.eye(n)).max() <= 1e-12
+    assert out["warm"][2].max() < out["cold"][2].min(), (out["warm"][2], out["cold"][2])
+
+
 def test_cholesky_inverse():
     _say("case: cholesky water/def2-tzvp overlap")
     from graddft_amd.eigenproblem import cholesky_inv
@@ -129,3 +166,27 @@ def test_diff_scf_loop_matches_oracle():
     for s in range(2):
         assert np.abs(D[s] @ S @ D[s] - D[s]).max() <= 1e-9          # idempotent
         assert abs(np.trace(D[s] @ S) - 5.0) <= 1e-9                 # N_e / 2 per spin
+
+
+@pytest.mark.parametrize("kind", ["b3lyp", "neural"])
+def test_graph_captured_scf_loop_replays_match_eager(kind):
+    """diff_scf_loop(graph=True): every replay, with eager device work in between and with
+    new inputs copied in, equals the eager loop on the same inputs."""
+    _say(f"case: graph-captured diff_scf_loop {kind} water/def2-svp level 1, 6 cycles")
+    from graddft_amd import B3LYP, MLPSpec, NeuralFunctional, diff_scf_loop, energy_predictor
+    mol = _water()
+    if kind == "b3lyp":
+        f, thetas = B3LYP, [None, None, None]
+    else:
+        f = NeuralFunctional(MLPSpec(nc=6, h=32, n_hidden=2, nd=4))
+        thetas = [f.init(seed=1), f.init(seed=2), f.init(seed=1)]
+    eager = diff_scf_loop(f, cycles=6)
+    graphed = diff_scf_loop(f, cycles=6, graph=True)
+    graphed(thetas[0], mol)                      # eager run + capture
+    for i, th in enumerate(thetas):
+        ref = eager(th, mol)
+        energy_predictor(f)(th, mol)             # unrelated eager work between replays
+        out = graphed(th, mol)
+        torch.cuda.synchronize()
+        assert abs(out.energy.item() - ref.energy.item()) <= 1e-12, (i, out.energy.item(), ref.energy.item())
+        assert torch.allclose(out.rdm1, ref.rdm1, rtol=0, atol=1e-11), i
