@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--no-scf", action="store_true", help="skip the ms/SCF-iteration measurement")
     ap.add_argument("--scf-cycles", type=int, default=15)
     ap.add_argument("--no-train", action="store_true", help="skip the training-step measurement")
+    ap.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes (calibrates HBM byte counters)")
     ap.add_argument("--train-cycles", type=int, default=10)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--grid-level", type=int, default=3)
@@ -168,6 +170,33 @@ def train_bench(args):
     return out
 
 
+def measured_traffic():
+    """{timer name: HBM bytes per launch} from the committed PMC summaries (profiles/traffic_*.txt,
+    lines 'traffic <name> <bytes>'), measured on this workload with rocprofv3."""
+    import glob
+    out = {}
+    for path in sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                                              "traffic_*.txt"))):
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic":
+                out[parts[1]] = int(parts[2])
+    return out
+
+
+def copy_only(nbytes: int):
+    """A plain device-to-device copy of a known size, five times: the reference launch for
+    scaling the FETCH_SIZE / WRITE_SIZE counters of a profiled kernel to HBM bytes."""
+    import torch
+    n = nbytes // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "launches": 5}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -175,6 +204,9 @@ def main():
               "(the path shards grid points across GPUs by design, but only one GPU is used here)",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_only:
+        copy_only(args.copy_only)
+        return
     import torch
     from graddft_amd import MLPSpec, NeuralFunctional, _lib
     from graddft_amd.interface import geometry
@@ -220,8 +252,11 @@ def main():
     if flops is not None:
         avg_s = tot_ms / cnt / 1e3
         achieved = flops / avg_s / 1e12
+        tr = measured_traffic().get(name)
         roof = {"bound": "mfma", "kernel": name, "achieved": round(achieved, 3), "peak": peak,
-                "unit": unit, "frac": round(achieved / peak, 4), "traffic": None,
+                "unit": unit, "frac": round(achieved / peak, 4), "traffic": tr,
+                "traffic_unit": "bytes per launch (HBM, rocprofv3 FETCH_SIZE/WRITE_SIZE scaled by a "
+                                "copy-kernel calibration; profiles/traffic_*.txt)" if tr else None,
                 "avg_launch_us": round(avg_s * 1e6, 2), "launches_per_step": cnt // args.steps,
                 "flops_per_launch": flops}
     # accuracy and speed against the plain FP64 GPU path on the same inputs

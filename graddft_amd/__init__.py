@@ -6,7 +6,8 @@ Public API (names follow XanaduAI/GradDFT's ``grad_dft`` package):
 ``mse_energy_loss``, ``train_kernel``, ``adam``, ``molecule_from_geometry``.
 """
 from .molecule import Grid, Molecule
-from .functional import Functional, NeuralFunctional, MLPSpec, XCPlan, XCEnergyFn, XCEnergyVxcFn
+from .functional import (Functional, NeuralFunctional, B3LYPNeuralCorrection, MLPSpec, XCPlan, XCEnergyFn,
+                         XCEnergyVxcFn)
 from .popular_functionals import B3LYP, B3LYP_GRID
 from .interface.builder import molecule_from_geometry
 from .train import (energy_predictor, simple_energy_loss, mse_energy_loss, train_kernel, adam,
@@ -14,7 +15,7 @@ from .train import (energy_predictor, simple_energy_loss, mse_energy_loss, train
 from .evaluate import JittableDiis, diff_scf_loop, non_scf_predictor
 from .eigenproblem import safe_eigh_vjp, safe_fock_solver, safe_general_eigh
 
-__all__ = ["Grid", "Molecule", "Functional", "NeuralFunctional", "MLPSpec", "XCPlan",
+__all__ = ["Grid", "Molecule", "Functional", "NeuralFunctional", "B3LYPNeuralCorrection", "MLPSpec", "XCPlan",
            "XCEnergyFn", "XCEnergyVxcFn", "B3LYP", "B3LYP_GRID", "molecule_from_geometry",
            "energy_predictor", "non_scf_predictor", "simple_energy_loss", "mse_energy_loss",
            "train_kernel", "adam", "apply_updates", "JittableDiis", "diff_scf_loop", "safe_eigh_vjp",

@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgdx.so")
 
 GDX_FP64, GDX_FP32, GDX_BF16X6 = 0, 1, 2
-GDX_FUNC_B3LYP, GDX_FUNC_NEURAL_B3LYP = 0, 1
+GDX_FUNC_B3LYP, GDX_FUNC_NEURAL_B3LYP, GDX_FUNC_B3LYP_NEURAL_CORR = 0, 1, 2
 
 
 class XCDesc(ctypes.Structure):

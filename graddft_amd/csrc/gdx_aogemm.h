@@ -5,9 +5,9 @@
 //
 // density_ao_kernel  (grad_dft/molecule.py:388-502)
 //   Block = 64 grid points (16 per wave), all four AO components c in {phi, dx, dy, dz}.
-//   For each 64-column chunk of orbitals: X_c = A_c D on MFMA (4 components x 4 column
-//   blocks = 16 accumulators per wave), then the chunk's contribution to the row
-//   contractions is added to per-lane FP64 sums:
+//   For each 128-column (FP64: 64) chunk of orbitals: X_c = A_c D on MFMA (4 components x
+//   8 column blocks per wave), then the AO values at the chunk's columns are staged through
+//   LDS 16 columns at a time and the row contractions added to per-lane FP64 sums:
 //     rho += X_phi . phi, grad_j += X_phi . d_j phi, lapl += X_phi . lapl(phi),
 //     T += sum_j X_j . d_j phi.
 //   After the last chunk the 16 lanes sharing a row are summed; raw sums are written as
@@ -26,21 +26,36 @@
 
 namespace gdx {
 
-constexpr int DN_PT = 64, DN_CB = 64, DN_K = 16;
+constexpr int DN_PT = 64, DN_K = 16;
+
+// Column chunk of the density kernel: 128 output columns per pass for FP32, 64 for FP64
+// (accumulator registers: 4 components x CB/16 blocks of 16 x 16 per wave).
+template <typename T> struct DensCB { static constexpr int value = sizeof(T) == 4 ? 128 : 64; };
 
 template <typename T>
 __global__ __launch_bounds__(256) void density_ao_kernel(const T* __restrict__ ao, int npts_pad, int nP,
                                                          const T* __restrict__ Dp,
                                                          double* __restrict__ dens_part) {
   typedef typename MfmaT<T>::acc_t acc_t;
+  constexpr int CB = DensCB<T>::value;
+  constexpr int NJ = CB / 16;
   constexpr int VEC = 16 / sizeof(T);
   constexpr int KV = DN_K / VEC;                           // vectors per 16-k row segment
   constexpr int NVA = 4 * DN_PT * KV / 256;                // A vectors per thread
-  constexpr int NVB = DN_K * DN_CB / VEC / 256 > 0 ? DN_K * DN_CB / VEC / 256 : 1;
-  constexpr int BVEC = DN_K * DN_CB / VEC;                 // B vectors per stage
+  constexpr int BVEC = DN_K * CB / VEC;                    // B vectors per stage
+  constexpr int NVB = (BVEC + 255) / 256;
+  constexpr int CVEC = 5 * DN_PT * 16 / VEC;               // contraction vectors per stage
+  constexpr int NVC = CVEC / 256;
   typedef T vec_t __attribute__((ext_vector_type(VEC)));
-  __shared__ T As[2][DN_K][4][DN_PT + 2];
-  __shared__ T Bs[2][DN_K][DN_CB + 4];
+  constexpr int AK = DN_K + 4;   // A tile row: 16 k of one (component, point), 16-byte aligned
+  constexpr int AS_ELEMS = 2 * 4 * DN_PT * AK;
+  constexpr int CS_ELEMS = 5 * DN_PT * 17;
+  static_assert(CS_ELEMS <= AS_ELEMS, "contraction tile must fit in the A tile's LDS");
+  __shared__ __align__(16) T Araw[AS_ELEMS];
+  __shared__ __align__(16) T Bs[2][DN_K][CB + 4];
+  T (*As)[4][DN_PT][AK] = (T (*)[4][DN_PT][AK])Araw;   // phase 1: [buf][c][p][k]
+  T (*Cs)[DN_PT][17] = (T (*)[DN_PT][17])Araw;         // phase 2 (aliases As)
+  typedef T v4_t __attribute__((ext_vector_type(4)));
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int p0 = blockIdx.x * DN_PT;
@@ -55,8 +70,9 @@ __global__ __launch_bounds__(256) void density_ao_kernel(const T* __restrict__ a
 #pragma unroll
     for (int v = 0; v < 6; ++v) part[q][v] = 0.0;
 
-  vec_t ra[NVA], rb[NVB];
-  for (int c0 = 0; c0 < nP; c0 += DN_CB) {
+  vec_t ra[NVA], rb[NVB], rc[NVC];
+  for (int c0 = 0; c0 < nP; c0 += CB) {
+    // ---- phase 1: X_c = A_c D[:, c0:c0+CB] for the 4 AO components ----
     auto gload = [&](int k0) {
 #pragma unroll
       for (int i = 0; i < NVA; ++i) {
@@ -69,10 +85,9 @@ __global__ __launch_bounds__(256) void density_ao_kernel(const T* __restrict__ a
       for (int i = 0; i < NVB; ++i) {
         const int idx = tid + 256 * i;
         if (idx < BVEC) {
-          const int k = idx / (DN_CB / VEC), cv = idx % (DN_CB / VEC);
+          const int k = idx / (CB / VEC), cv = idx % (CB / VEC);
           const int col = c0 + cv * VEC;
-          if (col < nP) rb[i] = *(const vec_t*)(D + (size_t)(k0 + k) * nP + col);
-          else rb[i] = (vec_t)(T)0;
+          rb[i] = col < nP ? *(const vec_t*)(D + (size_t)(k0 + k) * nP + col) : (vec_t)(T)0;
         }
       }
     };
@@ -82,66 +97,91 @@ __global__ __launch_bounds__(256) void density_ao_kernel(const T* __restrict__ a
         const int idx = tid + 256 * i;
         const int kv = idx % KV, rest = idx / KV;
         const int p = rest % DN_PT, c = rest / DN_PT;
-#pragma unroll
-        for (int e = 0; e < VEC; ++e) As[buf][kv * VEC + e][c][p] = ra[i][e];
+        *(vec_t*)&As[buf][c][p][kv * VEC] = ra[i];
       }
 #pragma unroll
       for (int i = 0; i < NVB; ++i) {
         const int idx = tid + 256 * i;
         if (idx < BVEC) {
-          const int k = idx / (DN_CB / VEC), cv = idx % (DN_CB / VEC);
+          const int k = idx / (CB / VEC), cv = idx % (CB / VEC);
 #pragma unroll
           for (int e = 0; e < VEC; ++e) Bs[buf][k][cv * VEC + e] = rb[i][e];
         }
       }
     };
-    acc_t acc[4][4];
+    acc_t acc[4][NJ];
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[c][j] = (acc_t){0, 0, 0, 0};
+      for (int j = 0; j < NJ; ++j) acc[c][j] = (acc_t){0, 0, 0, 0};
     gload(0);
     lstore(0);
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
       const int cur = kt & 1;
       if (kt + 1 < nk) gload((kt + 1) * DN_K);
+      {
+        // k-group g = lane / 16 supplies k = 4 g + kk to MFMA step kk (one permutation of the
+        // stage's 16 k for both operands): one 4-element LDS read per A fragment
+        const int g = lane >> 4, r = lane & 15;
+        v4_t af[4];
 #pragma unroll
-      for (int kk = 0; kk < DN_K / 4; ++kk) {
-        const int kr = kk * 4 + (lane >> 4);
-        T af[4], bf[4];
+        for (int c = 0; c < 4; ++c) af[c] = *(const v4_t*)&As[cur][c][wave * 16 + r][4 * g];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) af[c] = As[cur][kr][c][wave * 16 + (lane & 15)];
+        for (int kk = 0; kk < 4; ++kk) {
+          T bf[NJ];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) bf[j] = Bs[cur][kr][j * 16 + (lane & 15)];
+          for (int j = 0; j < NJ; ++j) bf[j] = Bs[cur][4 * g + kk][j * 16 + r];
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
+          for (int c = 0; c < 4; ++c)
 #pragma unroll
-          for (int j = 0; j < 4; ++j) acc[c][j] = MfmaT<T>::op(af[c], bf[j], acc[c][j]);
+            for (int j = 0; j < NJ; ++j) acc[c][j] = MfmaT<T>::op(af[c][kk], bf[j], acc[c][j]);
+        }
       }
       if (kt + 1 < nk) lstore(cur ^ 1);
       __syncthreads();
     }
-    // contraction of this column chunk
+    // ---- phase 2: contraction with the AO values at the chunk's columns, 16 columns per
+    // stage staged through LDS (coalesced loads of the next stage overlap this one) ----
+    const int nst = min(NJ, (nP - c0) / 16);
+    auto cload = [&](int st) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int col = c0 + j * 16 + (lane & 15);
-      if (col < nP) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int p = p0 + wave * 16 + MfmaT<T>::row(lane, q);
-          const size_t o = (size_t)p * nP + col;
-          const double phi = ao[o], gx = ao[plane + o], gy = ao[2 * plane + o], gz = ao[3 * plane + o],
-                       lp = ao[4 * plane + o];
-          const double x0 = acc[0][j][q];
-          part[q][0] += x0 * phi;
-          part[q][1] += x0 * gx;
-          part[q][2] += x0 * gy;
-          part[q][3] += x0 * gz;
-          part[q][4] += (double)acc[1][j][q] * gx + (double)acc[2][j][q] * gy + (double)acc[3][j][q] * gz;
-          part[q][5] += x0 * lp;
-        }
+      for (int i = 0; i < NVC; ++i) {
+        const int idx = tid + 256 * i;
+        const int cv = idx % (16 / VEC), rest = idx / (16 / VEC);
+        const int p = rest % DN_PT, pl = rest / DN_PT;
+        rc[i] = *(const vec_t*)(ao + pl * plane + (size_t)(p0 + p) * nP + c0 + st * 16 + cv * VEC);
       }
+    };
+    if (nst > 0) cload(0);
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      if (j >= nst) break;
+#pragma unroll
+      for (int i = 0; i < NVC; ++i) {
+        const int idx = tid + 256 * i;
+        const int cv = idx % (16 / VEC), rest = idx / (16 / VEC);
+        const int p = rest % DN_PT, pl = rest / DN_PT;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) Cs[pl][p][cv * VEC + e] = rc[i][e];
+      }
+      __syncthreads();
+      if (j + 1 < nst) cload(j + 1);
+      const int col = lane & 15;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int p = wave * 16 + MfmaT<T>::row(lane, q);
+        const double phi = Cs[0][p][col], gx = Cs[1][p][col], gy = Cs[2][p][col], gz = Cs[3][p][col],
+                     lp = Cs[4][p][col];
+        const double x0 = acc[0][j][q];
+        part[q][0] += x0 * phi;
+        part[q][1] += x0 * gx;
+        part[q][2] += x0 * gy;
+        part[q][3] += x0 * gz;
+        part[q][4] += (double)acc[1][j][q] * gx + (double)acc[2][j][q] * gy + (double)acc[3][j][q] * gz;
+        part[q][5] += x0 * lp;
+      }
+      __syncthreads();
     }
   }
 #pragma unroll
@@ -157,9 +197,14 @@ __global__ __launch_bounds__(256) void density_ao_kernel(const T* __restrict__ a
 }
 
 constexpr int VX_P = 4, VX_K = 4 * VX_P;
+constexpr int VX_FLUSH = 8;   // FP32 path: stages (of VX_P points) per FP32 partial sum
 
 // VT: output tile edge (128, or 64 for nao_pad <= 64); each wave owns a (VT/2)^2 quarter.
-template <typename T, bool SAME, int VT>
+// FLUSH (FP32 only): the MFMA accumulators are added into FP64 registers every VX_FLUSH
+// stages, so no FP32 sum runs over more than VX_FLUSH * VX_K terms -- needed when a block's
+// point chunk is long, because the V_xc integrand cancels strongly near the nuclei (large
+// Laplacian terms); short chunks go without it (fewer registers, more blocks per CU).
+template <typename T, bool SAME, int VT, bool FLUSH = false>
 __global__ __launch_bounds__(256) void vxc_ao_kernel(const T* __restrict__ ao, int npts_pad, int nP,
                                                      const double* __restrict__ vpt, int chunk,
                                                      double* __restrict__ vpart) {
@@ -216,6 +261,16 @@ __global__ __launch_bounds__(256) void vxc_ao_kernel(const T* __restrict__ ao, i
   for (int i = 0; i < WB; ++i)
 #pragma unroll
     for (int j = 0; j < WB; ++j) acc[i][j] = (acc_t){0, 0, 0, 0};
+  static_assert(!FLUSH || sizeof(T) == 4, "FP64 accumulates in FP64 already");
+  double dacc[FLUSH ? WB : 1][FLUSH ? WB : 1][4];
+  if (FLUSH) {
+#pragma unroll
+    for (int i = 0; i < WB; ++i)
+#pragma unroll
+      for (int j = 0; j < WB; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dacc[i][j][q] = 0.0;
+  }
   const int nst = (rend - rbeg + VX_P - 1) / VX_P;
   if (nst > 0) {
     gload(rbeg);
@@ -224,6 +279,16 @@ __global__ __launch_bounds__(256) void vxc_ao_kernel(const T* __restrict__ ao, i
   }
   for (int st = 0; st < nst; ++st) {
     const int cur = st & 1;
+    if (FLUSH && st > 0 && st % VX_FLUSH == 0) {
+#pragma unroll
+      for (int i = 0; i < WB; ++i)
+#pragma unroll
+        for (int j = 0; j < WB; ++j) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) dacc[i][j][q] += (double)acc[i][j][q];
+          acc[i][j] = (acc_t){0, 0, 0, 0};
+        }
+    }
     if (st + 1 < nst) gload(rbeg + (st + 1) * VX_P);
 #pragma unroll
     for (int kk = 0; kk < VX_K / 4; ++kk) {
@@ -250,7 +315,9 @@ __global__ __launch_bounds__(256) void vxc_ao_kernel(const T* __restrict__ ao, i
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int row = a0 + wm + i * 16 + MfmaT<T>::row(lane, q);
-        if (row < nP && col < nP) out[(size_t)row * nP + col] = (double)acc[i][j][q];
+        double v = (double)acc[i][j][q];
+        if (FLUSH) v += dacc[FLUSH ? i : 0][FLUSH ? j : 0][q];
+        if (row < nP && col < nP) out[(size_t)row * nP + col] = v;
       }
     }
 }
@@ -338,10 +405,15 @@ __global__ __launch_bounds__(256) void vxc_bf16x6_kernel(const float* __restrict
   };
 
   f4 acc[4][4];
+  double dacc[4][4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < 4; ++j) {
+      acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) dacc[i][j][q] = 0.0;
+    }
   const int nst = (rend - rbeg + VB_P - 1) / VB_P;
   if (nst > 0) {
     gload(rbeg);
@@ -351,6 +423,16 @@ __global__ __launch_bounds__(256) void vxc_bf16x6_kernel(const float* __restrict
   const int g = lane >> 4, r = lane & 15;
   for (int st = 0; st < nst; ++st) {
     const int cur = st & 1;
+    if (st > 0 && st % (VX_FLUSH / 2) == 0) {   // same point count per FP32 sum as vxc_ao_kernel
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) dacc[i][j][q] += (double)acc[i][j][q];
+          acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+        }
+    }
     if (st + 1 < nst) gload(rbeg + (st + 1) * VB_P);
     bf8 af[4][3];
 #pragma unroll
@@ -386,7 +468,7 @@ __global__ __launch_bounds__(256) void vxc_bf16x6_kernel(const float* __restrict
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int row = a0 + wm + i * 16 + mfma32_row(lane, q);
-        if (row < nP && col < nP) out[(size_t)row * nP + col] = (double)acc[i][j][q];
+        if (row < nP && col < nP) out[(size_t)row * nP + col] = dacc[i][j][q] + (double)acc[i][j][q];
       }
     }
 }

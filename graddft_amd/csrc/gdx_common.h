@@ -60,4 +60,19 @@ __device__ __forceinline__ double sum16(double v) {
   return v;
 }
 
+// MLP head: c_o(z) for the pre-activation z = (H W_last + b)_o.
+//  mode 0 (NeuralFunctional head, grad_dft/functional.py:407-419): c = 2 sigmoid(z / 2)
+//  mode 1 (B3LYP plus a neural correction): c = c_B3LYP,o + z
+__device__ __forceinline__ double b3lyp_coef(int o) {
+  return o == 0 ? 1.0 - 0.2 : (o == 1 ? 0.72 : (o == 2 ? 1.0 - 0.81 : 0.81));
+}
+__device__ __forceinline__ double head_coef(double z, int mode, int o) {
+  return mode == 1 ? b3lyp_coef(o) + z : 2.0 / (1.0 + exp(-z / 2.0));
+}
+// dc/dz and d2c/dz2 from c
+__device__ __forceinline__ double head_d1(double c, int mode) { return mode == 1 ? 1.0 : 0.5 * c * (1.0 - 0.5 * c); }
+__device__ __forceinline__ double head_d2(double c, int mode) {
+  return mode == 1 ? 0.0 : 0.5 * head_d1(c, 0) * (1.0 - c);
+}
+
 }  // namespace gdx

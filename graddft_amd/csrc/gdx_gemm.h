@@ -3,12 +3,13 @@
 // op(A)[m][k] = TA ? A[k*lda + m] : A[m*lda + k];  op(B)[k][n] = TB ? B[n*ldb + k] : B[k*ldb + n]
 // Block 256 threads = 4 waves (2 x 2), tile 128 x 128 x 16, each wave 64 x 64 = 4 x 4 MFMA
 // 16x16 accumulators.  Global tiles are fetched with 16-byte vector loads into registers
-// one stage ahead and stored into a double-buffered LDS tile, so HBM latency overlaps the
-// MFMA work of the current stage (one barrier per stage).  Shapes must be whole tiles:
+// one stage ahead and stored, in their global orientation, into a double-buffered LDS tile
+// (16-byte stores), so HBM latency overlaps the MFMA work of the current stage (one barrier
+// per stage).  Shapes must be whole tiles:
 // M % 128 == N % 128 == 0 and the K range of every split a multiple of 16 (callers pad).
 // Blocks are renumbered so consecutive tiles (which share the same A rows) are dispatched to
 // the same XCD and its L2.
-// Split-K: blockIdx.y selects K range [y*kchunk, (y+1)*kchunk); C is offset by y*M*ldc.
+// Split-K: split y (see xcd_tile_split) covers K range [y*kchunk, (y+1)*kchunk); C is offset by y*M*ldc.
 // colsum (optional): per split, the column sums of op(B) over its K range (bias gradients).
 #pragma once
 #include "gdx_common.h"
@@ -38,7 +39,28 @@ __device__ __forceinline__ float elu_f(float x) { return x > 0.0f ? x : expm1f(x
 __device__ __forceinline__ double elu_any(double x) { return elu_d(x); }
 __device__ __forceinline__ float elu_any(float x) { return elu_f(x); }
 
-constexpr int GT_M = 128, GT_N = 128, GT_K = 16;
+constexpr int GT_M = 128, GT_N = 128;
+// k per pipeline stage (16: keeps three FP32 blocks resident per CU; 32 measured slower)
+template <typename T> struct GemmK { static constexpr int value = 16; };
+
+// Tile b and K-split of this block, with blocks sharing operands placed on one XCD (the
+// hardware dispatches linear block L to XCD L % 8, and each XCD has its own L2):
+//  * split-K grids (gridDim.y splits, a multiple of 8): the tiles of one split -- which read
+//    the same K-range rows of both operands -- go to the same XCD, consecutively;
+//  * otherwise, when the tile count is a multiple of 8, consecutive tiles (same A rows) go
+//    to the same XCD.
+__device__ __forceinline__ void xcd_tile_split(int nblk, int& b, int& ysplit) {
+  b = blockIdx.x;
+  ysplit = blockIdx.y;
+  if (gridDim.y > 1 && gridDim.y % 8 == 0) {
+    const int L = blockIdx.x + blockIdx.y * gridDim.x;
+    const int xcd = L % 8, slot = L / 8;
+    b = slot % gridDim.x;
+    ysplit = xcd + 8 * (slot / gridDim.x);
+  } else if (gridDim.y == 1 && nblk % 8 == 0) {
+    b = (b % 8) * (nblk / 8) + b / 8;
+  }
+}
 
 template <typename T, bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void gemm_tile_kernel(
@@ -46,22 +68,27 @@ __global__ __launch_bounds__(256) void gemm_tile_kernel(
     T* __restrict__ C, int ldc, const T* __restrict__ bias, const T* __restrict__ Y, int ldy,
     int kchunk, T* __restrict__ colsum) {
   typedef typename MfmaT<T>::acc_t acc_t;
+  constexpr int GT_K = GemmK<T>::value;
   constexpr int VEC = 16 / sizeof(T);                // elements per 16-byte load
   constexpr int NVA = GT_M * GT_K / VEC / 256;       // vector loads per thread per tile
   constexpr int NVB = GT_N * GT_K / VEC / 256;
-  constexpr int PAD = 4;
   typedef T vec_t __attribute__((ext_vector_type(VEC)));
-  __shared__ T As[2][GT_K][GT_M + PAD];
-  __shared__ T Bs[2][GT_K][GT_N + PAD];
+  typedef T v4_t __attribute__((ext_vector_type(4)));
+  // LDS tiles keep the global orientation (16-byte stores, no transposition):
+  //   A: TA ? [k][m] (row GT_M + 4) : [m][k] (row GT_K + 4);  B: TB ? [n][k] : [k][n]
+  constexpr int AKM = GT_M + 4, AMK = GT_K + 4, BKN = GT_N + 4, BNK = GT_K + 4;
+  constexpr int ABUF = TA ? GT_K * AKM : GT_M * AMK;
+  constexpr int BBUF = TB ? GT_N * BNK : GT_K * BKN;
+  __shared__ __align__(16) T As[2 * ABUF];
+  __shared__ __align__(16) T Bs[2 * BBUF];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-  // XCD-aware renumbering: hardware dispatches block b to XCD b % 8
   const int nbn = N / GT_N, nblk = (M / GT_M) * nbn;
-  int b = blockIdx.x;
-  if (nblk % 8 == 0) b = (b % 8) * (nblk / 8) + b / 8;
+  int b, ysplit;
+  xcd_tile_split(nblk, b, ysplit);
   const int m0 = (b / nbn) * GT_M, n0 = (b % nbn) * GT_N;
-  const int kbeg = blockIdx.y * kchunk;
+  const int kbeg = ysplit * kchunk;
   const int nk = kchunk / GT_K;
 
   vec_t ra[NVA], rb[NVB];
@@ -95,12 +122,10 @@ __global__ __launch_bounds__(256) void gemm_tile_kernel(
       const int idx = tid + 256 * i;
       if (TA) {
         const int k = idx / (GT_M / VEC), c = idx % (GT_M / VEC);
-#pragma unroll
-        for (int e = 0; e < VEC; ++e) As[buf][k][c * VEC + e] = ra[i][e];
+        *(vec_t*)&As[buf * ABUF + k * AKM + c * VEC] = ra[i];
       } else {
         const int r = idx / (GT_K / VEC), c = idx % (GT_K / VEC);
-#pragma unroll
-        for (int e = 0; e < VEC; ++e) As[buf][c * VEC + e][r] = ra[i][e];
+        *(vec_t*)&As[buf * ABUF + r * AMK + c * VEC] = ra[i];
       }
     }
 #pragma unroll
@@ -108,13 +133,34 @@ __global__ __launch_bounds__(256) void gemm_tile_kernel(
       const int idx = tid + 256 * i;
       if (!TB) {
         const int k = idx / (GT_N / VEC), c = idx % (GT_N / VEC);
-#pragma unroll
-        for (int e = 0; e < VEC; ++e) Bs[buf][k][c * VEC + e] = rb[i][e];
+        *(vec_t*)&Bs[buf * BBUF + k * BKN + c * VEC] = rb[i];
       } else {
         const int r = idx / (GT_K / VEC), c = idx % (GT_K / VEC);
-#pragma unroll
-        for (int e = 0; e < VEC; ++e) Bs[buf][c * VEC + e][r] = rb[i][e];
+        *(vec_t*)&Bs[buf * BBUF + r * BNK + c * VEC] = rb[i];
       }
+    }
+  };
+  // in each 16-k half of the stage, k-group lane / 16 supplies k = 4 g + kk to MFMA step kk
+  // (a permutation of those 16 k, the same for both operands), so a [m][k] or [n][k] tile is
+  // read with one 4-element vector per fragment
+  auto afrag = [&](int buf, int m, int g, T* out) {
+    if (TA) {
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) out[kk] = As[buf * ABUF + (4 * g + kk) * AKM + m];
+    } else {
+      const v4_t v = *(const v4_t*)&As[buf * ABUF + m * AMK + 4 * g];
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) out[kk] = v[kk];
+    }
+  };
+  auto bfrag = [&](int buf, int n, int g, T* out) {
+    if (TB) {
+      const v4_t v = *(const v4_t*)&Bs[buf * BBUF + n * BNK + 4 * g];
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) out[kk] = v[kk];
+    } else {
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) out[kk] = Bs[buf * BBUF + (4 * g + kk) * BKN + n];
     }
   };
 
@@ -133,29 +179,31 @@ __global__ __launch_bounds__(256) void gemm_tile_kernel(
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) gload(kbeg + (kt + 1) * GT_K);
-    if (colsum != nullptr && m0 == 0 && tid < GT_N) {
+    if (colsum != nullptr && m0 == 0 && tid < GT_N && !TB) {
 #pragma unroll
-      for (int k = 0; k < GT_K; ++k) csum += Bs[cur][k][tid];
+      for (int k = 0; k < GT_K; ++k) csum += Bs[cur * BBUF + k * BKN + tid];
     }
 #pragma unroll
-    for (int kk = 0; kk < GT_K / 4; ++kk) {
-      const int kr = kk * 4 + (lane >> 4);
-      T af[4], bf[4];
+    for (int kb = 0; kb < GT_K / 16; ++kb) {
+      const int g = kb * 4 + (lane >> 4), r = lane & 15;
+      T af[4][4], bf[4][4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = As[cur][kr][wm + i * 16 + (lane & 15)];
+      for (int i = 0; i < 4; ++i) afrag(cur, wm + i * 16 + r, g, af[i]);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bf[j] = Bs[cur][kr][wn + j * 16 + (lane & 15)];
+      for (int j = 0; j < 4; ++j) bfrag(cur, wn + j * 16 + r, g, bf[j]);
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int kk = 0; kk < 4; ++kk)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = MfmaT<T>::op(af[i], bf[j], acc[i][j]);
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = MfmaT<T>::op(af[i][kk], bf[j][kk], acc[i][j]);
     }
     if (kt + 1 < nk) lstore(cur ^ 1);
     __syncthreads();
   }
-  if (colsum != nullptr && m0 == 0 && tid < GT_N) colsum[(size_t)blockIdx.y * N + n0 + tid] = csum;
+  if (colsum != nullptr && m0 == 0 && tid < GT_N) colsum[(size_t)ysplit * N + n0 + tid] = csum;
 
-  T* Cz = C + (size_t)blockIdx.y * M * ldc;
+  T* Cz = C + (size_t)ysplit * M * ldc;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll

@@ -41,10 +41,10 @@ __global__ __launch_bounds__(256) void gemm_bf16x6_kernel(
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
   const int nbn = N / GB_N, nblk = (M / GB_M) * nbn;
-  int b = blockIdx.x;
-  if (nblk % 8 == 0) b = (b % 8) * (nblk / 8) + b / 8;
+  int b, ysplit;
+  xcd_tile_split(nblk, b, ysplit);
   const int m0 = (b / nbn) * GB_M, n0 = (b % nbn) * GB_N;
-  const int kbeg = blockIdx.y * kchunk;
+  const int kbeg = ysplit * kchunk;
   const int nk = kchunk / GB_K;
   const bool do_cs = colsum != nullptr && m0 == 0;
 
@@ -142,10 +142,10 @@ __global__ __launch_bounds__(256) void gemm_bf16x6_kernel(
       float t = 0.f;
 #pragma unroll
       for (int q = 0; q < 8; ++q) t += cs_red[q][tid];
-      colsum[(size_t)blockIdx.y * N + n0 + tid] = t;
+      colsum[(size_t)ysplit * N + n0 + tid] = t;
     }
   }
-  float* Cz = C + (size_t)blockIdx.y * M * ldc;
+  float* Cz = C + (size_t)ysplit * M * ldc;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll

@@ -8,11 +8,10 @@
 // FP64, fixed order).
 //
 //   delta_wgrad_rows_kernel G = (DZ W_last^T) * elu'(H) and dW_last partials sum_r DZ^T H
-//   l0bwd_rows_kernel       DX = G W0^T and [dW0; db0] partials sum_r [X; 1]^T G
-//   wgrad0_rows_kernel      the [dW0; db0] partials alone
+//   wgrad0_rows_kernel      [dW0; db0] partials sum_r [X; 1]^T G
 // and, with each lane loading 16 bytes of one row and the row dot products done on MFMA
 // (no cross-lane reductions):
-//   rowdot_mfma_kernel      HEAD: z = H W_last + b, c = 2 sigmoid(z / 2), eps = sum_o c_o e_o,
+//   rowdot_mfma_kernel      HEAD: z = H W_last + b, c = head_coef(z), eps = sum_o c_o e_o,
 //                           clipped weighted E_xc block partial (grad_dft/functional.py:219-253,
 //                           407-419); otherwise out = A Bt^T for at most 16 output columns
 #pragma once
@@ -34,61 +33,6 @@ template <typename V>
 __device__ __forceinline__ V group_sum(V v, int width) {
   for (int off = width >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void head_rows_kernel(const T* __restrict__ H, int h, int npts_pad,
-                                                        const T* __restrict__ WT16, const T* __restrict__ bias,
-                                                        int nd, const double* __restrict__ E,
-                                                        const double* __restrict__ weights, int npts,
-                                                        T* __restrict__ C, double* __restrict__ eps_part) {
-  typedef typename Vec4T<T>::type v4;
-  __shared__ double red[4];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int lpr = h >> 2, rpw = 64 / lpr;
-  const int sub = lane / lpr, li = lane % lpr;
-  const int f0 = li * 4;
-  T w[4][4];
-#pragma unroll
-  for (int o = 0; o < 4; ++o)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) w[o][e] = o < nd ? WT16[(size_t)o * h + f0 + e] : (T)0;
-  double contrib = 0.0;
-  const int r0 = blockIdx.x * RW_ROWS + wave * (RW_ROWS / 4);
-  constexpr int U = sizeof(T) == 8 ? RW_UNROLL / 2 : RW_UNROLL;
-  for (int rb = sub; rb < RW_ROWS / 4; rb += U * rpw) {
-    // U rows in flight per lane before any of them is reduced
-    v4 x[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int r = r0 + rb + u * rpw;
-      x[u] = (rb + u * rpw < RW_ROWS / 4 && r < npts_pad) ? *(const v4*)(H + (size_t)r * h + f0) : (v4)(T)0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int r = r0 + rb + u * rpw;
-      T z[4];
-#pragma unroll
-      for (int o = 0; o < 4; ++o)
-        z[o] = group_sum<T>(x[u][0] * w[o][0] + x[u][1] * w[o][1] + x[u][2] * w[o][2] + x[u][3] * w[o][3], lpr);
-      if (li == 0 && rb + u * rpw < RW_ROWS / 4 && r < npts) {
-        double eps = 0.0;
-        for (int o = 0; o < nd; ++o) {
-          const double zz = (double)z[o] + (double)bias[o];
-          const double c = 2.0 / (1.0 + exp(-zz / 2.0));
-          C[(size_t)r * nd + o] = (T)c;
-          eps += c * E[(size_t)r * nd + o];
-        }
-        eps = fabs(eps) > 1e-30 ? eps : 0.0;
-        const double wt = weights[r];
-        contrib += (fabs(wt) > 1e-30 ? wt : 0.0) * eps;
-      }
-    }
-  }
-  contrib = warp_sum(contrib);
-  if (lane == 0) red[wave] = contrib;
-  __syncthreads();
-  if (threadIdx.x == 0) eps_part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
 // G[r][f] = (sum_o DZ16[r][o] WT16[o][f]) * elu'(H[r][f]);  part[blk][o][f] = sum_r DZ16[r][o] H[r][f]
@@ -154,73 +98,6 @@ __global__ __launch_bounds__(256) void delta_wgrad_rows_kernel(const T* __restri
   }
 }
 
-// DX[r][j] = sum_f G[r][f] W016[j][f] (j < nc);  part[blk][j][f] = sum_r X16[r][j] G[r][f] for
-// j <= nc (X16's column nc is the ones column: the bias gradient)
-template <typename T>
-__global__ __launch_bounds__(256) void l0bwd_rows_kernel(const T* __restrict__ G, int h, int rows,
-                                                         const T* __restrict__ X16, const T* __restrict__ W016,
-                                                         int nc, T* __restrict__ DX, double* __restrict__ part) {
-  typedef typename Vec4T<T>::type v4;
-  constexpr int NJ = 7;  // nc = 6 inputs + the ones column
-  __shared__ double red[4][NJ][256];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int lpr = h >> 2, rpw = 64 / lpr;
-  const int sub = lane / lpr, li = lane % lpr;
-  const int f0 = li * 4;
-  T w[6][4];
-#pragma unroll
-  for (int j = 0; j < 6; ++j)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) w[j][e] = j < nc ? W016[(size_t)j * h + f0 + e] : (T)0;
-  T acc[NJ][4];
-#pragma unroll
-  for (int j = 0; j < NJ; ++j)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) acc[j][e] = (T)0;
-  const int r0 = blockIdx.x * RW_ROWS + wave * (RW_ROWS / 4);
-  constexpr int U = sizeof(T) == 8 ? RW_UNROLL / 2 : RW_UNROLL;
-  for (int rb = sub; rb < RW_ROWS / 4; rb += U * rpw) {
-    v4 g[U], xa[U], xb[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int r = r0 + rb + u * rpw;
-      const bool ok = rb + u * rpw < RW_ROWS / 4 && r < rows;
-      g[u] = ok ? *(const v4*)(G + (size_t)r * h + f0) : (v4)(T)0;
-      xa[u] = ok ? *(const v4*)(X16 + (size_t)r * 16) : (v4)(T)0;
-      xb[u] = ok ? *(const v4*)(X16 + (size_t)r * 16 + 4) : (v4)(T)0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int r = r0 + rb + u * rpw;
-      const T x[NJ] = {xa[u][0], xa[u][1], xa[u][2], xa[u][3], xb[u][0], xb[u][1], xb[u][2]};
-      T dx[6];
-#pragma unroll
-      for (int j = 0; j < 6; ++j)
-        dx[j] = group_sum<T>(g[u][0] * w[j][0] + g[u][1] * w[j][1] + g[u][2] * w[j][2] + g[u][3] * w[j][3], lpr);
-      if (li == 0 && rb + u * rpw < RW_ROWS / 4 && r < rows)
-        for (int j = 0; j < nc; ++j) DX[(size_t)r * nc + j] = dx[j];
-#pragma unroll
-      for (int j = 0; j < NJ; ++j)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) acc[j][e] += x[j] * g[u][e];
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < NJ; ++j)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      double v = (double)acc[j][e];
-      for (int off = lpr; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
-      if (sub == 0) red[wave][j][f0 + e] = v;
-    }
-  __syncthreads();
-  for (int i = threadIdx.x; i < (nc + 1) * h; i += 256) {
-    const int j = i / h, f = i % h;
-    const int jj = j == nc ? NJ - 1 : j;   // row nc of the output is the bias (ones column)
-    part[(size_t)blockIdx.x * (nc + 1) * h + i] = red[0][jj][f] + red[1][jj][f] + red[2][jj][f] + red[3][jj][f];
-  }
-}
-
 // out (rows x nout) = A (rows x K) Bt^T, Bt (16 x K) in LDS; 16-row groups per wave.  Lane l
 // loads VEC = 16 / sizeof(T) consecutive elements of row l % 16 at column
 // t (4 VEC) + (l / 16) VEC for every t; element q of chunk t is the A operand of MFMA
@@ -231,7 +108,7 @@ __global__ __launch_bounds__(256) void rowdot_mfma_kernel(int rows, int K, const
                                                           const T* __restrict__ Bt, T* __restrict__ out, int nout,
                                                           const T* __restrict__ bias, const double* __restrict__ E,
                                                           const double* __restrict__ weights, int npts,
-                                                          double* __restrict__ eps_part) {
+                                                          double* __restrict__ eps_part, int head_mode) {
   typedef typename MfmaT<T>::acc_t acc_t;
   constexpr int VEC = 16 / sizeof(T);
   typedef T vec_t __attribute__((ext_vector_type(VEC)));
@@ -285,7 +162,7 @@ __global__ __launch_bounds__(256) void rowdot_mfma_kernel(int rows, int K, const
         double eps = 0.0;
         for (int o = 0; o < nout; ++o) {
           const double zz = zs[wave][lane][o] + (double)bias[o];
-          const double c = 2.0 / (1.0 + exp(-zz / 2.0));
+          const double c = head_coef(zz, head_mode, o);
           out[(size_t)rr * nout + o] = (T)c;
           eps += c * E[(size_t)rr * nout + o];
         }

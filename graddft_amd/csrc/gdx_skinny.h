@@ -4,7 +4,7 @@
 //   skinny_out16   C16 (M x 16) = A (M x K) . Bt^T, Bt (16 x K): per-row products against
 //                  at most 16 weight rows (the last layer's z = H W_last, and dE/dx = G W0^T).
 //                  Block = 4 waves x 16 rows; A staged through LDS in 64-wide K chunks.
-//                  Optional head epilogue: c = 2 sigmoid((z + b)/2), eps = sum_o c_o e_o,
+//                  Optional head epilogue: c = head_coef(z + b), eps = sum_o c_o e_o,
 //                  clipped, weighted FP64 block sum (grad_dft/functional.py:219-253, 407-419).
 //   skinny_wgrad   P[blk] (16 x N) = sum over the block's rows r of A16[r][m] B[r][n]: weight
 //                  gradients whose fan-in or fan-out is <= 16, split over row chunks; the 4
@@ -24,7 +24,7 @@ __global__ __launch_bounds__(256) void skinny_out16_kernel(
     T* __restrict__ C16, int ldc, int ncols,
     // head epilogue
     const T* __restrict__ bias, int nd, const double* __restrict__ E,
-    const double* __restrict__ weights, int npts, double* __restrict__ eps_part) {
+    const double* __restrict__ weights, int npts, double* __restrict__ eps_part, int head_mode = 0) {
   typedef typename MfmaT<T>::acc_t acc_t;
   __shared__ T As[64][SK_KC + 2];
   __shared__ double zs[64][17];
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void skinny_out16_kernel(
       double eps = 0.0;
       for (int o = 0; o < nd; ++o) {
         const double z = zs[tid][o] + (double)bias[o];
-        const double c = 2.0 / (1.0 + exp(-z / 2.0));
+        const double c = head_coef(z, head_mode, o);
         C16[(size_t)r * ldc + o] = (T)c;
         eps += c * E[(size_t)r * nd + o];
       }

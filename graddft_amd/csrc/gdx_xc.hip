@@ -341,13 +341,13 @@ __global__ void prep_edge_weights_kernel(const double* __restrict__ W0, int nc,
 }
 
 // Adjoint seeds, one thread per point.  g = g_exc clip(w) [|eps| > 1e-30];
-// GE = dE/de_f = g c_f; neural: DZ16 = dE/dz = g e_f (c_f/2)(1 - c_f/2) zero-padded to 16,
+// GE = dE/de_f = g c_f; neural: DZ16 = dE/dz = g e_f dc_f/dz zero-padded to 16,
 // and block partials of the last layer's bias gradient sum_r DZ[r][o] -> dbpart[block][o].
 template <typename T>
 __global__ __launch_bounds__(NT) void head_seed_kernel(
     const T* __restrict__ Cc, const double* __restrict__ E, const double* __restrict__ weights,
     int nd, int npts, int npts_pad, const double* __restrict__ g_exc, int neural,
-    double* __restrict__ GE, T* __restrict__ DZ16, double* __restrict__ dbpart) {
+    double* __restrict__ GE, T* __restrict__ DZ16, double* __restrict__ dbpart, int head_mode) {
   __shared__ double red[NT / 64];
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const double bc[4] = {1.0 - 0.2, 0.72, 1.0 - 0.81, 0.81};
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(NT) void head_seed_kernel(
     const double g = (r < npts && fabs(eps) > 1e-30) ? (*g_exc) * clip30(weights[r]) : 0.0;
     for (int o = 0; o < nd; ++o) {
       GE[(size_t)r * nd + o] = g * c[o];
-      dz[o] = g * e[o] * (c[o] * 0.5) * (1.0 - c[o] * 0.5);
+      dz[o] = g * e[o] * head_d1(c[o], head_mode);
     }
     if (neural)
       for (int o = 0; o < 16; ++o) DZ16[(size_t)r * 16 + o] = (T)(o < nd ? dz[o] : 0.0);
@@ -501,7 +501,8 @@ __global__ __launch_bounds__(NT) void point_tangent_kernel(const double* __restr
 }
 
 // Seeds of the HVP reverse sweep, one thread per point.  With m = clip(w) [|eps| > 1e-30],
-// c = 2 sigmoid(z/2), c' = (c/2)(1 - c/2), c'' = c'(1 - c)/2, cdot = c' zdot:
+// c = head_coef(z), c' and c'' its derivatives (sigmoid head: c' = (c/2)(1 - c/2),
+// c'' = c'(1 - c)/2; linear correction: 1, 0), cdot = c' zdot:
 //   Edot    += m sum_o (cdot_o e_o + c_o edot_o)                     (block partials)
 //   GE_o     = dPhi/de_o    = m (g c_o + cdot_o),   GEd_o = dPhi/dedot_o = m c_o
 //   zbar_o   = m (g e_o + edot_o) c'_o + m e_o c''_o zdot_o,  zdotbar_o = m e_o c'_o
@@ -511,7 +512,7 @@ __global__ __launch_bounds__(NT) void hvp_seed_kernel(
     const T* __restrict__ Cc, const T* __restrict__ ZT, const double* __restrict__ E,
     const double* __restrict__ Edot, const double* __restrict__ weights, int nd, int npts, int npts_pad,
     const double* __restrict__ g_exc, int neural, double* __restrict__ GE, double* __restrict__ GEd,
-    T* __restrict__ DZ2, double* __restrict__ dbpart, double* __restrict__ edot_part) {
+    T* __restrict__ DZ2, double* __restrict__ dbpart, double* __restrict__ edot_part, int head_mode) {
   __shared__ double red[NT / 64];
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const double bc[4] = {1.0 - 0.2, 0.72, 1.0 - 0.81, 0.81};
@@ -525,7 +526,7 @@ __global__ __launch_bounds__(NT) void hvp_seed_kernel(
       c[o] = r >= npts ? 0.0 : (neural ? (double)Cc[(size_t)r * nd + o] : bc[o]);
       e[o] = E[(size_t)r * nd + o];
       et[o] = Edot[(size_t)r * nd + o];
-      cp[o] = neural ? 0.5 * c[o] * (1.0 - 0.5 * c[o]) : 0.0;
+      cp[o] = neural ? head_d1(c[o], head_mode) : 0.0;
       zt[o] = (neural && r < npts) ? (double)ZT[(size_t)r * nd + o] : 0.0;
       eps += c[o] * e[o];
     }
@@ -537,7 +538,7 @@ __global__ __launch_bounds__(NT) void hvp_seed_kernel(
       ed_contrib += m * (cd * e[o] + c[o] * et[o]);
       GE[(size_t)r * nd + o] = m * (g * c[o] + cd);
       GEd[(size_t)r * nd + o] = m * c[o];
-      const double cpp = 0.5 * cp[o] * (1.0 - c[o]);
+      const double cpp = neural ? head_d2(c[o], head_mode) : 0.0;
       zb[o] = m * (g * e[o] + et[o]) * cp[o] + m * e[o] * cpp * zt[o];
       zdb[o] = m * e[o] * cp[o];
     }
@@ -651,9 +652,16 @@ constexpr int RED_GROUPS = 32; // slice groups of the first reduction stage
 // use), so no CU is left with an extra block at the end.
 struct VxcSplit {
   int vt, ntile, nchunk, chunk;
+  bool flush;   // FP32 accumulators flushed into FP64 (long chunks)
 };
-// prec: GDX_FP64 / GDX_FP32 / GDX_BF16X6 (bf16 MFMA for tiles of 128)
-VxcSplit vxc_split(int Np, int nP, int nsv, int prec) {
+// FP32 V_xc: every build flushes its FP32 partial sums into FP64 (VX_FLUSH_MIN = 0): the
+// SCF adjoint through many cycles is sensitive to the forward Fock matrices, and 500-point
+// FP32 sums measurably degrade it (relative gradient error 7e-5 against 5e-6 flushed)
+constexpr int VX_FLUSH_MIN = 0;
+// prec: GDX_FP64 / GDX_FP32 / GDX_BF16X6 (bf16 MFMA for tiles of 128).  force_flush: the
+// FP32 build always flushes its accumulators into FP64 (used by the second-order pass, whose
+// potentials cancel more strongly than the first-order ones).
+VxcSplit vxc_split(int Np, int nP, int nsv, int prec, bool force_flush = false) {
   VxcSplit v{};
   const bool fp64 = prec == GDX_FP64;
   v.vt = nP <= 64 ? 64 : 128;
@@ -662,11 +670,19 @@ VxcSplit vxc_split(int Np, int nP, int nsv, int prec) {
   int resident = v.vt == 64 ? (fp64 ? 4 : 7) : (fp64 ? 2 : 3);
   if (prec == GDX_BF16X6 && v.vt == 128) resident = 1;
   const int per = v.ntile * nsv;
-  int nchunk = std::max(1, (NUM_CU * resident + per - 1) / per);
-  nchunk = std::min(nchunk, std::max(1, Np / 128));   // at least 128 points per chunk
-  v.chunk = (Np + nchunk - 1) / nchunk;
-  v.chunk = (v.chunk + VX_P - 1) / VX_P * VX_P;
-  v.nchunk = (Np + v.chunk - 1) / v.chunk;
+  auto split = [&](int res) {
+    int nchunk = std::max(1, (NUM_CU * res + per - 1) / per);
+    nchunk = std::min(nchunk, std::max(1, Np / 128));   // at least 128 points per chunk
+    v.chunk = (Np + nchunk - 1) / nchunk;
+    v.chunk = (v.chunk + VX_P - 1) / VX_P * VX_P;
+    v.nchunk = (Np + v.chunk - 1) / v.chunk;
+  };
+  split(resident);
+  v.flush = false;
+  if (!fp64 && prec != GDX_BF16X6 && v.vt == 128 && (force_flush || v.chunk > VX_FLUSH_MIN)) {
+    v.flush = true;                 // the flushing kernel keeps one block per CU
+    split(1);
+  }
   return v;
 }
 constexpr int SKW_ROWS = 512;  // grid points per block of the skinny weight-gradient kernel
@@ -691,11 +707,13 @@ Layout make_layout(const gdx_xc_desc& d, bool hvp = false) {
   const size_t Np = d.npts_pad, nP = d.nao_pad;
   const size_t R = hvp ? 2 * Np : Np;  // rows of the activation-side buffers
   const size_t es = elem_size(d.precision);
-  const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
+  const bool neural = d.kind != GDX_FUNC_B3LYP;
+  const int hmode = d.kind == GDX_FUNC_B3LYP_NEURAL_CORR ? 1 : 0;
   L.nsd = d.spin_restricted ? 1 : 2;
   L.rows = (int)R;
   {
-    const VxcSplit v1 = vxc_split((int)Np, (int)nP, 1, d.precision), v2 = vxc_split((int)Np, (int)nP, 2, d.precision);
+    const VxcSplit v1 = vxc_split((int)Np, (int)nP, 1, d.precision, hvp),
+                   v2 = vxc_split((int)Np, (int)nP, 2, d.precision, hvp);
     L.nchunk = std::max(v1.nchunk, 2 * v2.nchunk);   // partial slots: nchunk x nsv
   }
   // split-K for the hidden weight-gradient GEMMs: whole 32-row stages per split (for
@@ -766,12 +784,13 @@ int validate(const gdx_xc_desc* d) {
   GDX_REQUIRE(d->npts > 0 && d->nao > 0, "empty grid or basis");
   GDX_REQUIRE(d->npts_pad >= d->npts && d->npts_pad % 64 == 0, "npts_pad must be a multiple of 64 >= npts");
   GDX_REQUIRE(d->nao_pad >= d->nao && d->nao_pad % 16 == 0, "nao_pad must be a multiple of 16 >= nao");
-  GDX_REQUIRE(d->kind == GDX_FUNC_B3LYP || d->kind == GDX_FUNC_NEURAL_B3LYP, "unknown functional kind");
+  GDX_REQUIRE(d->kind == GDX_FUNC_B3LYP || d->kind == GDX_FUNC_NEURAL_B3LYP || d->kind == GDX_FUNC_B3LYP_NEURAL_CORR,
+              "unknown functional kind");
   if (d->precision != GDX_FP64 && d->precision != GDX_FP32 && d->precision != GDX_BF16X6) {
     set_error("unknown precision (GDX_FP64, GDX_FP32, GDX_BF16X6)");
     return GDX_ERR_UNSUPPORTED;
   }
-  if (d->kind == GDX_FUNC_NEURAL_B3LYP) {
+  if (d->kind != GDX_FUNC_B3LYP) {
     GDX_REQUIRE(d->mlp_nc == 6, "coefficient inputs must be 6 (rho, |grad rho|^2, tau per spin)");
     GDX_REQUIRE(d->mlp_nd == 4, "MLP outputs must equal the 4 B3LYP energy densities");
     GDX_REQUIRE(d->mlp_h >= 16 && d->mlp_h % 16 == 0 && d->mlp_h <= 256,
@@ -847,7 +866,8 @@ template <typename T>
 void run_densities(const Ctx& c, const T* ao, const double* rdm1, const double* weights, double* dens_fin) {
   const gdx_xc_desc& d = c.d;
   const int Np = d.npts_pad, nP = d.nao_pad;
-  const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
+  const bool neural = d.kind != GDX_FUNC_B3LYP;
+  const int hmode = d.kind == GDX_FUNC_B3LYP_NEURAL_CORR ? 1 : 0;
   hipStream_t st = c.st;
   GDX_LAUNCH("pad_dm", pad_dm_kernel<T>, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, rdm1, d.nao, nP,
              c.P<T>(c.L.Dp));
@@ -888,12 +908,12 @@ void reduce_slices(const Ctx& c, const TP* part, int nslices, long stride, long 
 
 // V_s (nsv spins) from the per-point potentials in L.vpt, reduced into d_rdm1 (2, n, n)
 template <typename T>
-void launch_vxc(const Ctx& c, const T* ao, int nsv, double* d_rdm1) {
+void launch_vxc(const Ctx& c, const T* ao, int nsv, double* d_rdm1, bool force_flush = false) {
   const gdx_xc_desc& d = c.d;
   const Layout& L = c.L;
   hipStream_t st = c.st;
   const int Np = d.npts_pad, nP = d.nao_pad;
-  const VxcSplit v = vxc_split(Np, nP, nsv, d.precision);
+  const VxcSplit v = vxc_split(Np, nP, nsv, d.precision, force_flush);
   const char* nm = tname<T>("vxc_f64", "vxc_f32");
   const dim3 grid(v.ntile, v.nchunk, nsv);
   bool done = false;
@@ -904,6 +924,17 @@ void launch_vxc(const Ctx& c, const T* ao, int nsv, double* d_rdm1) {
                    v.chunk, c.P<double>(L.vpart));
       else
         GDX_LAUNCH("vxc_bf16x6", vxc_bf16x6_kernel<false>, grid, dim3(NT), 0, st, ao, Np, nP, c.P<double>(L.vpt),
+                   v.chunk, c.P<double>(L.vpart));
+      done = true;
+    }
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (!done && v.flush) {
+      if (v.ntile == 1)
+        GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 128, true>), grid, dim3(NT), 0, st, ao, Np, nP, c.P<double>(L.vpt),
+                   v.chunk, c.P<double>(L.vpart));
+      else
+        GDX_LAUNCH(nm, (vxc_ao_kernel<T, false, 128, true>), grid, dim3(NT), 0, st, ao, Np, nP, c.P<double>(L.vpt),
                    v.chunk, c.P<double>(L.vpart));
       done = true;
     }
@@ -934,7 +965,8 @@ int forward_impl(const Ctx& c, const T* ao, const double* weights, const double*
   const gdx_xc_desc& d = c.d;
   const Layout& L = c.L;
   hipStream_t st = c.st;
-  const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
+  const bool neural = d.kind != GDX_FUNC_B3LYP;
+  const int hmode = d.kind == GDX_FUNC_B3LYP_NEURAL_CORR ? 1 : 0;
   const int Np = d.npts_pad;
   run_densities<T>(c, ao, rdm1, weights, c.P<double>(L.dens_fin));
   int nred = L.npt_blocks;
@@ -967,11 +999,11 @@ int forward_impl(const Ctx& c, const T* ao, const double* weights, const double*
       nred = (Np + RW_ROWS - 1) / RW_ROWS;
       GDX_LAUNCH("head_fwd", (rowdot_mfma_kernel<T, true>), dim3(nred), dim3(NT), 0, st, Np, h,
                  H + (size_t)(nl - 1) * L.hstride, WT16, c.P<T>(L.C), d.mlp_nd, plp + po.b[nl], c.P<double>(L.E),
-                 weights, d.npts, c.P<double>(L.eps_part));
+                 weights, d.npts, c.P<double>(L.eps_part), hmode);
     } else {
       GDX_LAUNCH("head_fwd", (skinny_out16_kernel<T, true>), dim3(Np / 64), dim3(NT), 0, st, Np, h,
                  H + (size_t)(nl - 1) * L.hstride, h, WT16, c.P<T>(L.C), d.mlp_nd, d.mlp_nd, plp + po.b[nl],
-                 d.mlp_nd, c.P<double>(L.E), weights, d.npts, c.P<double>(L.eps_part));
+                 d.mlp_nd, c.P<double>(L.E), weights, d.npts, c.P<double>(L.eps_part), hmode);
       nred = Np / 64;
     }
   }
@@ -986,7 +1018,8 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
   const gdx_xc_desc& d = c.d;
   const Layout& L = c.L;
   hipStream_t st = c.st;
-  const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
+  const bool neural = d.kind != GDX_FUNC_B3LYP;
+  const int hmode = d.kind == GDX_FUNC_B3LYP_NEURAL_CORR ? 1 : 0;
   const int Np = d.npts_pad, nP = d.nao_pad;
   const int nd = 4;
   if (neural) {
@@ -1003,7 +1036,7 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
     const T* W016 = c.P<T>(L.W16);
     GDX_LAUNCH("head_seed", head_seed_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, c.P<T>(L.C),
                c.P<double>(L.E), weights, nd, d.npts, Np, g_exc, 1, c.P<double>(L.GE), c.P<T>(L.DZ),
-               c.P<double>(L.dbpart));
+               c.P<double>(L.dbpart), hmode);
     const int rwblk = (Np + RW_ROWS - 1) / RW_ROWS;
     if (rows_path(h)) {
       GDX_LAUNCH("head_delta", delta_wgrad_rows_kernel<T>, dim3(rwblk), dim3(NT), 0, st, Hlast, h, Np, c.P<T>(L.DZ),
@@ -1056,7 +1089,7 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
     const long l0len = (long)(d.mlp_nc + 1) * h;
     if (rows_path(h)) {
       GDX_LAUNCH("layer0_bwd", (rowdot_mfma_kernel<T, false>), dim3(rwblk), dim3(NT), 0, st, Np, h, Gcur, W016,
-                 c.P<T>(L.DX), d.mlp_nc, nullptr, nullptr, nullptr, 0, nullptr);
+                 c.P<T>(L.DX), d.mlp_nc, nullptr, nullptr, nullptr, 0, nullptr, 0);
       if (d_params) {
         GDX_LAUNCH("wgrad0", wgrad0_rows_kernel<T>, dim3(rwblk), dim3(NT), 0, st, Gcur, h, Np, c.P<T>(L.X), d.mlp_nc,
                    c.P<double>(L.wpart));
@@ -1073,7 +1106,7 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
     }
   } else {
     GDX_LAUNCH("head_seed", head_seed_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, nullptr, c.P<double>(L.E),
-               weights, nd, d.npts, Np, g_exc, 0, c.P<double>(L.GE), nullptr, nullptr);
+               weights, nd, d.npts, Np, g_exc, 0, c.P<double>(L.GE), nullptr, nullptr, 0);
   }
   if (d_rdm1) {
     GDX_LAUNCH("point_bwd", point_bwd_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, c.P<double>(L.dens_fin),
@@ -1092,7 +1125,8 @@ int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm
   const gdx_xc_desc& d = c.d;
   const Layout& L = c.L;
   hipStream_t st = c.st;
-  const bool neural = d.kind == GDX_FUNC_NEURAL_B3LYP;
+  const bool neural = d.kind != GDX_FUNC_B3LYP;
+  const int hmode = d.kind == GDX_FUNC_B3LYP_NEURAL_CORR ? 1 : 0;
   const int Np = d.npts_pad, nP = d.nao_pad;
   const int nd = 4;
   double* scal = c.P<double>(L.scal);
@@ -1139,7 +1173,7 @@ int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm
   GDX_LAUNCH("hvp_seed", hvp_seed_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, neural ? c.P<T>(L.C) : nullptr,
              neural ? c.P<T>(L.ZT) : nullptr, c.P<double>(L.E), c.P<double>(L.Edot), weights, nd, d.npts, Np, g_exc,
              neural ? 1 : 0, c.P<double>(L.GE), c.P<double>(L.GEd), neural ? c.P<T>(L.DZ) : nullptr,
-             c.P<double>(L.dbpart), c.P<double>(L.edot_part));
+             c.P<double>(L.dbpart), c.P<double>(L.edot_part), hmode);
   if (edot)
     GDX_LAUNCH("reduce_scalar", reduce_scalar_kernel, dim3(1), dim3(NT), 0, st, c.P<double>(L.edot_part),
                L.npt_blocks, edot);
@@ -1202,7 +1236,7 @@ int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm
     GDX_LAUNCH("point_hvp_bwd", point_hvp_bwd_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st,
                c.P<double>(L.dens_fin), c.P<double>(L.dfin_t), c.P<double>(L.GE), c.P<double>(L.GEd),
                neural ? c.P<T>(L.DX) : nullptr, d.mlp_nc, d.npts, Np, c.P<double>(L.vpt));
-    launch_vxc<T>(c, ao, 2, d_rdm1);
+    launch_vxc<T>(c, ao, 2, d_rdm1, true);
   }
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
@@ -1227,7 +1261,7 @@ int gdx_xc_forward(void* stream, const gdx_xc_desc* desc, const void* ao, const 
                    const double* rdm1, const double* params, double* exc, void* workspace) {
   int rc = validate(desc);
   if (rc != GDX_OK) return rc;
-  const bool neural = desc->kind == GDX_FUNC_NEURAL_B3LYP;
+  const bool neural = desc->kind != GDX_FUNC_B3LYP;
   GDX_REQUIRE(ao && weights && rdm1 && exc && workspace, "gdx_xc_forward: null pointer");
   GDX_REQUIRE(!neural || params, "gdx_xc_forward: neural functional needs params");
   Ctx c{*desc, make_layout(*desc), (char*)workspace, (hipStream_t)stream};
@@ -1261,7 +1295,7 @@ int gdx_xc_hvp(void* stream, const gdx_xc_desc* desc, const void* ao, const doub
                double* d_rdm1, double* d_params, void* workspace) {
   int rc = validate(desc);
   if (rc != GDX_OK) return rc;
-  const bool neural = desc->kind == GDX_FUNC_NEURAL_B3LYP;
+  const bool neural = desc->kind != GDX_FUNC_B3LYP;
   GDX_REQUIRE(ao && weights && rdm1 && dir && g_exc && workspace, "gdx_xc_hvp: null pointer");
   GDX_REQUIRE(!neural || params, "gdx_xc_hvp: neural functional needs params");
   Ctx c{*desc, make_layout(*desc, true), (char*)workspace, (hipStream_t)stream};
@@ -1275,7 +1309,7 @@ int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const void* ao, const
                     double* d_params, void* workspace) {
   int rc = validate(desc);
   if (rc != GDX_OK) return rc;
-  const bool neural = desc->kind == GDX_FUNC_NEURAL_B3LYP;
+  const bool neural = desc->kind != GDX_FUNC_B3LYP;
   GDX_REQUIRE(ao && weights && rdm1 && g_exc && workspace, "gdx_xc_backward: null pointer");
   GDX_REQUIRE(!neural || params, "gdx_xc_backward: neural functional needs params");
   Ctx c{*desc, make_layout(*desc), (char*)workspace, (hipStream_t)stream};

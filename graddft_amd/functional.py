@@ -255,8 +255,9 @@ class NeuralFunctional(Functional):
     kind = _lib.GDX_FUNC_NEURAL_B3LYP
     spin_symmetric = False
 
-    def __init__(self, spec: MLPSpec = MLPSpec(), name: str = "neural_b3lyp", is_xc: bool = True):
-        super().__init__(name=name, is_xc=is_xc)
+    def __init__(self, spec: MLPSpec = MLPSpec(), name: str = "neural_b3lyp", is_xc: bool = True,
+                 exact_exchange: float = 0.0):
+        super().__init__(name=name, is_xc=is_xc, exact_exchange=exact_exchange)
         self.spec = spec
 
     def init(self, seed: int = 0, device: str = "cuda") -> torch.Tensor:
@@ -267,3 +268,14 @@ class NeuralFunctional(Functional):
             parts.append(rng.normal(0.0, math.sqrt(2.0 / (i + o)), size=(i, o)).ravel())
             parts.append(np.zeros(o))
         return torch.as_tensor(np.concatenate(parts), dtype=torch.float64, device=device)
+
+
+class B3LYPNeuralCorrection(NeuralFunctional):
+    """B3LYP plus a neural correction: per-point coefficients c_f = c_B3LYP,f + MLP_theta(x)_f
+    (linear head) on the four grid energy densities, with B3LYP's exact exchange (a0 = 0.2)
+    from the ERI as for ``B3LYP``.  Zero last-layer weights reproduce B3LYP exactly."""
+
+    kind = _lib.GDX_FUNC_B3LYP_NEURAL_CORR
+
+    def __init__(self, spec: MLPSpec = MLPSpec(), name: str = "b3lyp_neural_corr", is_xc: bool = True):
+        super().__init__(spec=spec, name=name, is_xc=is_xc, exact_exchange=0.2)

@@ -40,9 +40,11 @@ enum { GDX_FP64 = 0, GDX_FP32 = 1, GDX_BF16X6 = 2 };
  *  GDX_FUNC_B3LYP        E = sum_r w * (0.8 e_lda + 0.72 e_b88 + 0.19 e_vwn + 0.81 e_lyp)
  *                        (the non-exact-exchange part of grad_dft/popular_functionals.py:375-393)
  *  GDX_FUNC_NEURAL_B3LYP E = sum_r w * sum_f c_theta(x(r))_f e_f(r): the four B3LYP energy
- *                        densities weighted by a coefficient MLP (NeuralFunctional,
- *                        grad_dft/functional.py:345-419). */
-enum { GDX_FUNC_B3LYP = 0, GDX_FUNC_NEURAL_B3LYP = 1 };
+ *                        densities weighted by a coefficient MLP with the head
+ *                        c = 2 sigmoid(z / 2) (NeuralFunctional, grad_dft/functional.py:345-419).
+ *  GDX_FUNC_B3LYP_NEURAL_CORR  as GDX_FUNC_NEURAL_B3LYP with a linear head around the B3LYP
+ *                        coefficients, c_f = c_B3LYP,f + z_f: B3LYP plus a neural correction. */
+enum { GDX_FUNC_B3LYP = 0, GDX_FUNC_NEURAL_B3LYP = 1, GDX_FUNC_B3LYP_NEURAL_CORR = 2 };
 
 /* Shape and functional description of one XC evaluation. */
 typedef struct gdx_xc_desc {

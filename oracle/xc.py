@@ -252,14 +252,18 @@ def unflatten(spec: MLPSpec, flat):
     return out
 
 
-def mlp_coefficients(spec: MLPSpec, flat_params, x):
-    """log squash -> dense/ELU stack -> head (functional.py:407-419, scale factor 2)."""
+def mlp_coefficients(spec: MLPSpec, flat_params, x, head: str = "sigmoid"):
+    """log squash -> dense/ELU stack -> head: 'sigmoid' is 2 sigmoid(z/2) (functional.py:407-419,
+    scale factor 2); 'b3lyp_correction' is the linear head c = c_B3LYP + z (B3LYP plus a neural
+    correction)."""
     y = torch.log(torch.abs(x) + SQUASH_OFFSET)
     layers = unflatten(spec, flat_params)
     for W, b in layers[:-1]:
         y = torch.nn.functional.elu(y @ W + b)
     W, b = layers[-1]
     z = y @ W + b
+    if head == "b3lyp_correction":
+        return torch.tensor([B3LYP_COEFFS[:4]], dtype=DT) + z
     return 2.0 * torch.sigmoid(z / 2.0)
 
 
@@ -289,7 +293,7 @@ class XCInputs:
 
 def xc_energy_from_rdm1(kind: str, inp: XCInputs, rdm1, params=None,
                         spec: Optional[MLPSpec] = None):
-    """E_xc(D) for ``kind`` in {'b3lyp', 'neural_b3lyp'} following compute_densities ->
+    """E_xc(D) for ``kind`` in {'b3lyp', 'neural_b3lyp', 'b3lyp_neural_corr'} following compute_densities ->
     compute_coefficient_inputs -> xc_energy (functional.py:255-288 without nonXC)."""
     rho = density(rdm1, inp.ao)
     grho = grad_density(rdm1, inp.ao, inp.grad_ao)
@@ -304,11 +308,12 @@ def xc_energy_from_rdm1(kind: str, inp: XCInputs, rdm1, params=None,
             coefs = torch.tensor([B3LYP_COEFFS[:4]], dtype=DT)
         dens = abs_clip(dens)
         return xc_energy(inp.weights, coefs, dens)
-    if kind == "neural_b3lyp":
+    if kind in ("neural_b3lyp", "b3lyp_neural_corr"):
         tau = kinetic_density(rdm1, inp.grad_ao)
         x = neural_features(rho, grho, tau)
         dens = abs_clip(dens)
-        coefs = mlp_coefficients(spec, params, x)
+        coefs = mlp_coefficients(spec, params, x,
+                                 head="b3lyp_correction" if kind == "b3lyp_neural_corr" else "sigmoid")
         return xc_energy(inp.weights, coefs, dens)
     raise ValueError(kind)
 

@@ -45,11 +45,11 @@ def _direction(n, seed):
     return A + 0.3 * A.transpose(0, 2, 1)   # deliberately not symmetric
 
 
-@pytest.mark.parametrize("kind", ["b3lyp", "neural_b3lyp"])
+@pytest.mark.parametrize("kind", ["b3lyp", "neural_b3lyp", "b3lyp_neural_corr"])
 def test_xc_hvp_matches_finite_differences(kind):
     inp, xin, rdm1, _ = _water()
     spec = X.MLPSpec(h=16, n_hidden=2)
-    theta = X.init_mlp_params(spec, 5) if kind == "neural_b3lyp" else None
+    theta = X.init_mlp_params(spec, 5) if kind != "b3lyp" else None
     n = rdm1.shape[-1]
     dD = _direction(n, 1)
     # a slightly spin-polarised density, so both spin channels carry distinct terms

@@ -167,7 +167,7 @@ def test_xc_energy_clips():
     assert float(X.xc_energy(w, c, e)) == pytest.approx(3.0, rel=1e-15)
 
 
-@pytest.mark.parametrize("kind", ["b3lyp", "neural_b3lyp"])
+@pytest.mark.parametrize("kind", ["b3lyp", "neural_b3lyp", "b3lyp_neural_corr"])
 def test_gradients_match_finite_differences(kind):
     inp, D, C, _ = _water_inputs(level=1)
     D1 = C[:, :4] @ C[:, :4].T
@@ -189,3 +189,25 @@ def test_gradients_match_finite_differences(kind):
         Ep, _, _ = X.xc_value_and_grads(kind, inp, rdm1, th + h * v, spec)
         Em, _, _ = X.xc_value_and_grads(kind, inp, rdm1, th - h * v, spec)
         assert (Ep - Em) / (2 * h) == pytest.approx(gth @ v, rel=1e-6)
+
+
+def test_b3lyp_correction_head_reduces_to_b3lyp():
+    """The linear correction head c = c_B3LYP + z gives B3LYP exactly when the last layer is
+    zero, and moves E_xc linearly in the last-layer bias: dE/db_f = sum_r w e_f."""
+    inp, D, _, _ = _water_inputs(level=1)
+    spec = X.MLPSpec(h=16, n_hidden=2)
+    theta = X.init_mlp_params(spec, 4)
+    n_last = 16 * 4 + 4
+    theta[-n_last:] = 0.0
+    rdm1 = np.stack([D, D])
+    E_b3 = float(X.xc_energy_from_rdm1("b3lyp", inp, torch.tensor(rdm1)))
+    E_nc = float(X.xc_energy_from_rdm1("b3lyp_neural_corr", inp, torch.tensor(rdm1), torch.tensor(theta), spec))
+    assert E_nc == pytest.approx(E_b3, abs=1e-13)
+    th = theta.copy()
+    th[-4:] = [0.01, 0.0, 0.0, 0.0]          # shift c_lda by 0.01
+    E1 = float(X.xc_energy_from_rdm1("b3lyp_neural_corr", inp, torch.tensor(rdm1), torch.tensor(th), spec))
+    rho = X.density(torch.tensor(rdm1), inp.ao)
+    grho = X.grad_density(torch.tensor(rdm1), inp.ao, inp.grad_ao)
+    lap = X.lapl_density(torch.tensor(rdm1), inp.ao, inp.grad_ao, inp.d2_ao)
+    e = X.abs_clip(X.b3lyp_exhf_densities(rho, grho, lap))
+    assert E1 - E_nc == pytest.approx(0.01 * float((inp.weights * e[:, 0]).sum()), rel=1e-9)

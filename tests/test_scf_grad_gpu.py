@@ -42,10 +42,11 @@ def _direction(n, seed):
     return A + 0.3 * A.transpose(0, 2, 1)
 
 
-@pytest.mark.parametrize("kind,h", [("b3lyp", 0), ("neural_b3lyp", 32), ("neural_b3lyp", 128)])
+@pytest.mark.parametrize("kind,h", [("b3lyp", 0), ("neural_b3lyp", 32), ("neural_b3lyp", 128),
+                                    ("b3lyp_neural_corr", 32)])
 def test_xc_hvp_matches_oracle(kind, h):
     _say(f"case: hvp {kind} h={h} water/def2-svp level 1, D_up != D_dn")
-    from graddft_amd import B3LYP_GRID, MLPSpec, NeuralFunctional
+    from graddft_amd import B3LYP_GRID, B3LYPNeuralCorrection, MLPSpec, NeuralFunctional
     from oracle import scf_grad as G
     from oracle.xc import MLPSpec as OSpec
     mol = _water()
@@ -55,7 +56,8 @@ def test_xc_hvp_matches_oracle(kind, h):
     if kind == "b3lyp":
         f, theta, ospec = B3LYP_GRID, None, None
     else:
-        f = NeuralFunctional(MLPSpec(nc=6, h=h, n_hidden=3, nd=4))
+        cls = NeuralFunctional if kind == "neural_b3lyp" else B3LYPNeuralCorrection
+        f = cls(MLPSpec(nc=6, h=h, n_hidden=3, nd=4))
         theta = f.init(seed=8)
         ospec = OSpec(6, h, 3, 4)
     n = mol.nao
@@ -104,20 +106,23 @@ def test_edot_is_vxc_contraction_and_autograd_function():
     assert torch.equal(gD, dR) and torch.equal(gth, dT)
 
 
-def _scf_oracle_energy_and_grad(mol, spec_o, theta, cycles):
+def _scf_oracle_energy_and_grad(mol, spec_o, theta, cycles, kind="neural_b3lyp", a0=0.0):
     from graddft_amd.interface.integrals import unpack_eri
     from oracle import scf as O
     from oracle import scf_grad as G
     from oracle import xc as X
     xin = _oracle_inputs(mol)
     eri = unpack_eri(mol.rep_tensor.cpu().numpy(), mol.nao)
-    inp = O.SCFInputs(mol.h1e.cpu().numpy(), mol.s1e.cpu().numpy(), eri, mol.nuclear_repulsion, None, 0.0)
+    K0 = None
+    if a0:
+        D0 = mol.cache["rdm1_init"].cpu().numpy()
+        K0 = np.stack([O.exchange_potential(D0[s], eri) for s in range(2)])
+    inp = O.SCFInputs(mol.h1e.cpu().numpy(), mol.s1e.cpu().numpy(), eri, mol.nuclear_repulsion, K0, a0)
     ti = G.TorchSCFInputs.from_numpy(inp)
     D0 = torch.tensor(mol.rdm1.cpu().numpy(), dtype=torch.float64)
     occ = torch.tensor(mol.mo_occ.cpu().numpy(), dtype=torch.float64)
     th = torch.tensor(theta.cpu().numpy(), dtype=torch.float64, requires_grad=True)
-    E = G.diff_scf_energy(lambda d: X.xc_energy_from_rdm1("neural_b3lyp", xin, d, th, spec_o), ti, D0, occ,
-                          cycles)
+    E = G.diff_scf_energy(lambda d: X.xc_energy_from_rdm1(kind, xin, d, th, spec_o), ti, D0, occ, cycles)
     g, = torch.autograd.grad(E, [th])
     return E.item(), g.numpy()
 
@@ -134,6 +139,22 @@ def test_diff_scf_loop_parameter_gradient_matches_oracle():
     out = diff_scf_loop(f, cycles=5)(th, mol)
     g, = torch.autograd.grad(out.energy, [th])
     E0, g0 = _scf_oracle_energy_and_grad(mol, OSpec(6, 32, 2, 4), theta, 5)
+    assert abs(out.energy.item() - E0) <= 1e-9, (out.energy.item(), E0)
+    assert _rel(g.cpu().numpy(), g0) <= 1e-6, _rel(g.cpu().numpy(), g0)
+
+
+def test_diff_scf_loop_gradient_b3lyp_neural_correction():
+    """B3LYP + neural correction with the exact-exchange term: SCF energy and d E / d theta."""
+    _say("case: d E_scf / d theta, B3LYP + neural correction h=32, water/def2-svp level 1, 4 cycles")
+    from graddft_amd import B3LYPNeuralCorrection, MLPSpec, diff_scf_loop
+    from oracle.xc import MLPSpec as OSpec
+    mol = _water(seed=8, with_eri=True)
+    f = B3LYPNeuralCorrection(MLPSpec(nc=6, h=32, n_hidden=2, nd=4))
+    theta = f.init(seed=4)
+    th = theta.detach().clone().requires_grad_(True)
+    out = diff_scf_loop(f, cycles=4)(th, mol)
+    g, = torch.autograd.grad(out.energy, [th])
+    E0, g0 = _scf_oracle_energy_and_grad(mol, OSpec(6, 32, 2, 4), theta, 4, kind="b3lyp_neural_corr", a0=0.2)
     assert abs(out.energy.item() - E0) <= 1e-9, (out.energy.item(), E0)
     assert _rel(g.cpu().numpy(), g0) <= 1e-6, _rel(g.cpu().numpy(), g0)
 

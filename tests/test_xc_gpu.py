@@ -96,8 +96,8 @@ def test_b3lyp_grid_energy_and_vxc(basis, level):
     assert _rel(gD.cpu().numpy(), gD0) <= G_REL
 
 
-def _neural_case(name, basis, level, h, seed, restricted=True):
-    from graddft_amd import MLPSpec, NeuralFunctional
+def _neural_case(name, basis, level, h, seed, restricted=True, kind="neural_b3lyp"):
+    from graddft_amd import B3LYPNeuralCorrection, MLPSpec, NeuralFunctional
     from oracle.xc import MLPSpec as OSpec, xc_value_and_grads
     rdm1 = None
     mol0 = _build(name, basis, level, seed=seed)
@@ -112,14 +112,14 @@ def _neural_case(name, basis, level, h, seed, restricted=True):
     else:
         mol = mol0
     spec = MLPSpec(nc=6, h=h, n_hidden=3, nd=4)
-    f = NeuralFunctional(spec)
+    f = NeuralFunctional(spec) if kind == "neural_b3lyp" else B3LYPNeuralCorrection(spec)
     theta = f.init(seed=seed + 7)
     D = mol.rdm1.detach().clone().requires_grad_(True)
     th = theta.detach().clone().requires_grad_(True)
     E = f.xc_energy(th, mol, rdm1=D)
     gD, gth = torch.autograd.grad(E, [D, th])
     inp = _oracle_inputs(mol)
-    E0, gD0, gth0 = xc_value_and_grads("neural_b3lyp", inp, mol.rdm1.cpu().numpy(),
+    E0, gD0, gth0 = xc_value_and_grads(kind, inp, mol.rdm1.cpu().numpy(),
                                        theta.cpu().numpy(), OSpec(6, h, 3, 4))
     assert abs(float(E) - E0) <= E_TOL, (float(E), E0)
     assert _rel(gD.cpu().numpy(), gD0) <= G_REL, _rel(gD.cpu().numpy(), gD0)
@@ -134,6 +134,11 @@ def test_neural_water_restricted():
 def test_neural_water_unrestricted():
     _say("case: neural water/def2-svp level 1 h=32, D_up != D_dn")
     _neural_case("water", "def2-svp", 1, 32, seed=4, restricted=False)
+
+
+def test_b3lyp_neural_correction_water():
+    _say("case: B3LYP + neural correction water/def2-svp level 1 h=64, D_up != D_dn")
+    _neural_case("water", "def2-svp", 1, 64, seed=6, restricted=False, kind="b3lyp_neural_corr")
 
 
 def test_neural_benzene():
