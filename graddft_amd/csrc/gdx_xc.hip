@@ -680,7 +680,9 @@ VxcSplit vxc_split(int Np, int nP, int nsv, int prec, bool force_flush = false) 
   split(resident);
   v.flush = false;
   if (!fp64 && prec != GDX_BF16X6 && v.vt == 128 && (force_flush || v.chunk > VX_FLUSH_MIN)) {
-    v.flush = true;                 // the flushing kernel keeps one block per CU
+    // the flushing kernel holds 128 FP64 accumulators a lane: one block per CU (64 x 64 tiles
+    // with more blocks per CU measured slower: their staging doubles)
+    v.flush = true;
     split(1);
   }
   return v;
