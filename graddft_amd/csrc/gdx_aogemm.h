@@ -205,7 +205,7 @@ constexpr int VX_FLUSH = 8;   // FP32 path: stages (of VX_P points) per FP32 par
 // point chunk is long, because the V_xc integrand cancels strongly near the nuclei (large
 // Laplacian terms); short chunks go without it (fewer registers, more blocks per CU).
 template <typename T, bool SAME, int VT, bool FLUSH = false>
-__global__ __launch_bounds__(256) void vxc_ao_kernel(const T* __restrict__ ao, int npts_pad, int nP,
+__global__ __launch_bounds__(256, (FLUSH && SAME) ? 2 : 1) void vxc_ao_kernel(const T* __restrict__ ao, int npts_pad, int nP,
                                                      const double* __restrict__ vpt, int chunk,
                                                      double* __restrict__ vpart) {
   typedef typename MfmaT<T>::acc_t acc_t;
