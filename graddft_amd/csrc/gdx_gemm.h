@@ -170,15 +170,35 @@ __global__ __launch_bounds__(256) void gemm_tile_kernel(
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = (acc_t){0, 0, 0, 0};
   T csum = 0;
+  // EPI_ELU_BWD, FP32: this lane's 64 elements of Y are fetched two stages before the
+  // epilogue, so their latency overlaps the last MFMAs instead of following them (FP64 keeps
+  // the loads in the epilogue: 64 more doubles a lane would spill)
+  constexpr bool YPF = EPI == EPI_ELU_BWD && sizeof(T) == 4;
+  T yv[YPF ? 4 : 1][YPF ? 4 : 1][4];
+  auto yload = [&]() {
+    if constexpr (YPF) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            yv[i][j][q] = Y[(size_t)(m0 + wm + i * 16 + MfmaT<T>::row(lane, q)) * ldy + n0 + wn + j * 16 + (lane & 15)];
+    }
+  };
+  const int ykt = nk > 2 ? nk - 2 : 0;
 
   if (nk > 0) {
     gload(kbeg);
     lstore(0);
     __syncthreads();
+  } else {
+    yload();
   }
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < nk) gload(kbeg + (kt + 1) * GT_K);
+    if (kt == ykt) yload();
     if (colsum != nullptr && m0 == 0 && tid < GT_N && !TB) {
 #pragma unroll
       for (int k = 0; k < GT_K; ++k) csum += Bs[cur * BBUF + k * BKN + tid];
@@ -214,8 +234,8 @@ __global__ __launch_bounds__(256) void gemm_tile_kernel(
         const int row = m0 + wm + i * 16 + MfmaT<T>::row(lane, q);
         T v = acc[i][j][q];
         if (EPI == EPI_BIAS_ELU) v = elu_any(v + bias[col]);
-        if (EPI == EPI_ELU_BWD) {
-          const T y = Y[(size_t)row * ldy + col];
+        if constexpr (EPI == EPI_ELU_BWD) {
+          const T y = YPF ? yv[YPF ? i : 0][YPF ? j : 0][q] : Y[(size_t)row * ldy + col];
           v *= (y > (T)0 ? (T)1 : y + (T)1);
         }
         Cz[(size_t)row * ldc + col] = v;

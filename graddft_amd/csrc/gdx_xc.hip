@@ -670,20 +670,30 @@ VxcSplit vxc_split(int Np, int nP, int nsv, int prec, bool force_flush = false) 
   int resident = v.vt == 64 ? (fp64 ? 4 : 7) : (fp64 ? 2 : 3);
   if (prec == GDX_BF16X6 && v.vt == 128) resident = 1;
   const int per = v.ntile * nsv;
+  // point chunks per (tile, spin): at least one full wave of blocks over the CUs, and among
+  // up to 4x that count the one whose last wave of blocks is fullest (fewest idle slots)
   auto split = [&](int res) {
-    int nchunk = std::max(1, (NUM_CU * res + per - 1) / per);
-    nchunk = std::min(nchunk, std::max(1, Np / 128));   // at least 128 points per chunk
-    v.chunk = (Np + nchunk - 1) / nchunk;
+    const int slots = NUM_CU * res;
+    const int cap = std::max(1, Np / 128);   // at least 128 points per chunk
+    const int n0 = std::min(cap, std::max(1, (slots + per - 1) / per));
+    int best = n0;
+    double best_eff = 0.0;
+    for (int n = n0; n <= std::min(cap, 4 * n0); ++n) {
+      const long blocks = (long)per * n;
+      const double eff = (double)blocks / ((double)((blocks + slots - 1) / slots) * slots);
+      if (eff > best_eff + 1e-9) { best_eff = eff; best = n; }
+    }
+    v.chunk = (Np + best - 1) / best;
     v.chunk = (v.chunk + VX_P - 1) / VX_P * VX_P;
     v.nchunk = (Np + v.chunk - 1) / v.chunk;
   };
   split(resident);
   v.flush = false;
   if (!fp64 && prec != GDX_BF16X6 && v.vt == 128 && (force_flush || v.chunk > VX_FLUSH_MIN)) {
-    // the flushing kernel holds 128 FP64 accumulators a lane: one block per CU (64 x 64 tiles
-    // with more blocks per CU measured slower: their staging doubles)
+    // the flushing kernel holds 64 FP64 accumulators a lane: two blocks per CU for the
+    // single-tile case (its launch bounds), one otherwise
     v.flush = true;
-    split(1);
+    split(v.ntile == 1 ? 2 : 1);
   }
   return v;
 }
