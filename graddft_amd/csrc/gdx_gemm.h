@@ -63,7 +63,7 @@ __device__ __forceinline__ void xcd_tile_split(int nblk, int& b, int& ysplit) {
 }
 
 template <typename T, bool TA, bool TB, int EPI>
-__global__ __launch_bounds__(256) void gemm_tile_kernel(
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? (EPI == EPI_ELU_BWD ? 2 : 3) : 1) void gemm_tile_kernel(
     int M, int N, const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb,
     T* __restrict__ C, int ldc, const T* __restrict__ bias, const T* __restrict__ Y, int ldy,
     int kchunk, T* __restrict__ colsum) {

@@ -20,7 +20,8 @@
 
 namespace gdx {
 
-constexpr int RW_ROWS = 256;  // rows per block
+constexpr int RW_ROWS = 256;  // rows per block of the delta / weight-gradient row kernels
+constexpr int RD_ROWS = 64;   // rows per block of rowdot_mfma_kernel (one 16-row group a wave)
 constexpr int RW_UNROLL = 8;  // rows per lane loaded before any is consumed (FP32; FP64 half)
 
 template <typename T>
@@ -122,9 +123,9 @@ __global__ __launch_bounds__(256) void rowdot_mfma_kernel(int rows, int K, const
   const int nch = K / (4 * VEC);
   const int rl = lane & 15, ks = lane >> 4;
   double contrib = 0.0;
-  const int ngroups = RW_ROWS / 16;
+  const int ngroups = RD_ROWS / 16;
   for (int gi = wave; gi < ngroups; gi += 4) {
-    const int r0 = blockIdx.x * RW_ROWS + gi * 16;
+    const int r0 = blockIdx.x * RD_ROWS + gi * 16;
     if (r0 >= rows) break;
     const int r = r0 + rl;
     vec_t x[MAXC];

@@ -757,7 +757,7 @@ Layout make_layout(const gdx_xc_desc& d, bool hvp = false) {
   L.X = take(neural ? R * 16 * es : 0);
   L.C = take(neural ? Np * d.mlp_nd * es : 0);
   L.H = take(R * h * nh * es);
-  L.eps_part = take(std::max<size_t>(Np / 64 + 1, 4096) * 8);
+  L.eps_part = take(std::max<size_t>(Np / std::min(64, RD_ROWS) + 1, 4096) * 8);
   L.GE = take(Np * 4 * 8);
   L.DZ = take(neural ? R * 16 * es : 0);
   L.W16 = take(neural ? 2 * 16 * h * es : 0);
@@ -1008,7 +1008,7 @@ int forward_impl(const Ctx& c, const T* ao, const double* weights, const double*
                                                             nullptr, 0, 1, h, nullptr);
     }
     if (rows_path(h)) {
-      nred = (Np + RW_ROWS - 1) / RW_ROWS;
+      nred = (Np + RD_ROWS - 1) / RD_ROWS;
       GDX_LAUNCH("head_fwd", (rowdot_mfma_kernel<T, true>), dim3(nred), dim3(NT), 0, st, Np, h,
                  H + (size_t)(nl - 1) * L.hstride, WT16, c.P<T>(L.C), d.mlp_nd, plp + po.b[nl], c.P<double>(L.E),
                  weights, d.npts, c.P<double>(L.eps_part), hmode);
@@ -1100,7 +1100,7 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
     // W0 (nc x h) followed by b0 is contiguous in params
     const long l0len = (long)(d.mlp_nc + 1) * h;
     if (rows_path(h)) {
-      GDX_LAUNCH("layer0_bwd", (rowdot_mfma_kernel<T, false>), dim3(rwblk), dim3(NT), 0, st, Np, h, Gcur, W016,
+      GDX_LAUNCH("layer0_bwd", (rowdot_mfma_kernel<T, false>), dim3((Np + RD_ROWS - 1) / RD_ROWS), dim3(NT), 0, st, Np, h, Gcur, W016,
                  c.P<T>(L.DX), d.mlp_nc, nullptr, nullptr, nullptr, 0, nullptr, 0);
       if (d_params) {
         GDX_LAUNCH("wgrad0", wgrad0_rows_kernel<T>, dim3(rwblk), dim3(NT), 0, st, Gcur, h, Np, c.P<T>(L.X), d.mlp_nc,
