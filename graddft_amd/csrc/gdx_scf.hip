@@ -254,11 +254,15 @@ __global__ __launch_bounds__(1024) void cholesky_inv_kernel(const double* __rest
 // ------------------------------------------------------------- eigensolver --------------
 // Parallel cyclic Jacobi (round-robin pair schedule), one workgroup per symmetric matrix.
 // Each round rotates n/2 disjoint (p, q) pairs: rotation angles from the current 2x2
-// blocks, then all row updates, then all column updates (A <- J^T A J, V <- V J).  Sweeps
-// stop when the off-diagonal Frobenius norm is below tol * ||A||_F.  Eigenvalues are
+// blocks, then one pass that applies both sides of the rotation (A <- J^T A J) to every 2x2
+// block (rows of pair k, columns of pair l) and the column rotation to V (V <- V J) -- the
+// per-element arithmetic of a row pass followed by a column pass, with one barrier less.
+// Sweeps stop when the off-diagonal Frobenius norm is below tol * ||A||_F.  Eigenvalues are
 // returned ascending with eigenvectors as columns of V (the order of jnp.linalg.eigh).
-// A and V live in LDS when 2 n^2 doubles fit (SMEM), otherwise in the workspace.
-constexpr int EIG_THREADS = 1024;
+// A, V and the warm-start product live in LDS when 3 n^2 doubles fit (SMEM),
+// otherwise in the workspace; 1024 threads either way.
+constexpr int EIG_THREADS_SMEM = 1024, EIG_THREADS_GLOBAL = 1024;
+constexpr size_t EIG_SMEM_MAX = 64 * 1024;
 
 __device__ __forceinline__ int rr_partner(int r, int k, int m, int* p, int* q) {
   // round r of the round-robin tournament on m (even) players: pair k
@@ -272,67 +276,77 @@ __device__ __forceinline__ int rr_partner(int r, int k, int m, int* p, int* q) {
 
 // Warm start (V0 != NULL, orthonormal columns): the sweeps run on V0^T A V0 starting from
 // V = V0, so a matrix close to one already diagonalised by V0 needs few sweeps; the result
-// is the same eigendecomposition (eigenvalues, and eigenvectors up to sign).
-template <bool SMEM>
-__global__ __launch_bounds__(EIG_THREADS) void jacobi_eigh_kernel(const double* __restrict__ Ain, int n,
-                                                                  const double* __restrict__ V0in,
-                                                                  double* __restrict__ w_out,
-                                                                  double* __restrict__ V_out,
-                                                                  double* __restrict__ gws, int max_sweeps,
-                                                                  double tol, int* __restrict__ sweeps_out) {
+// is the same eigendecomposition (eigenvalues, and eigenvectors up to sign).  A and V0 are
+// first staged next to each other (LDS or workspace), so the two products read no global
+// memory in their inner loops.
+template <bool SMEM, int NTH>
+__global__ __launch_bounds__(NTH) void jacobi_eigh_kernel(const double* __restrict__ Ain, int n,
+                                                          const double* __restrict__ V0in,
+                                                          double* __restrict__ w_out,
+                                                          double* __restrict__ V_out,
+                                                          double* __restrict__ gws, int max_sweeps,
+                                                          double tol, int* __restrict__ sweeps_out) {
   extern __shared__ double smem[];
   const int b = blockIdx.x;
   const int tid = threadIdx.x;
   const size_t nn = (size_t)n * n;
-  double* A = SMEM ? smem : gws + (size_t)b * 2 * nn;
+  double* A = SMEM ? smem : gws + (size_t)b * 3 * nn;
   double* V = A + nn;
+  double* T = V + nn;   // warm start: the staged input, then V0^T (A V0)
   __shared__ double cs_c[512], cs_s[512];
   __shared__ int pp[512], qq[512];
-  __shared__ double red[EIG_THREADS / 64];
+  __shared__ double red[NTH / 64];
   __shared__ int done;
-  __shared__ double anorm;
   const double* Asrc = Ain + (size_t)b * nn;
   if (V0in == nullptr) {
-    for (size_t e = tid; e < nn; e += blockDim.x) {
+    for (size_t e = tid; e < nn; e += NTH) {
       A[e] = Asrc[e];
       V[e] = (e / n == e % n) ? 1.0 : 0.0;
     }
   } else {
     const double* V0 = V0in + (size_t)b * nn;
-    // V <- A V0, then A <- V0^T V, then V <- V0
-    for (size_t e = tid; e < nn; e += blockDim.x) {
-      const int i = (int)(e / n), j = (int)(e % n);
-      double t = 0.0;
-      for (int k = 0; k < n; ++k) t += Asrc[(size_t)i * n + k] * V0[(size_t)k * n + j];
-      V[e] = t;
+    for (size_t e = tid; e < nn; e += NTH) {
+      T[e] = Asrc[e];
+      V[e] = V0[e];
     }
     __syncthreads();
-    for (size_t e = tid; e < nn; e += blockDim.x) {
+    // A <- Ain V0
+    for (size_t e = tid; e < nn; e += NTH) {
       const int i = (int)(e / n), j = (int)(e % n);
       double t = 0.0;
-      for (int k = 0; k < n; ++k) t += V0[(size_t)k * n + i] * V[(size_t)k * n + j];
+      for (int k = 0; k < n; ++k) t += T[(size_t)i * n + k] * V[(size_t)k * n + j];
       A[e] = t;
     }
     __syncthreads();
-    // symmetrise the rotated matrix (rounding) and load V0
-    for (size_t e = tid; e < nn; e += blockDim.x) {
+    // T <- V0^T A
+    for (size_t e = tid; e < nn; e += NTH) {
+      const int i = (int)(e / n), j = (int)(e % n);
+      double t = 0.0;
+      for (int k = 0; k < n; ++k) t += V[(size_t)k * n + i] * A[(size_t)k * n + j];
+      T[e] = t;
+    }
+    __syncthreads();
+    // A <- T symmetrised (rounding); V already holds V0
+    for (size_t e = tid; e < nn; e += NTH) {
       const int i = (int)(e / n), j = (int)(e % n);
       if (i < j) {
-        const double a = 0.5 * (A[(size_t)i * n + j] + A[(size_t)j * n + i]);
+        const double a = 0.5 * (T[(size_t)i * n + j] + T[(size_t)j * n + i]);
         A[(size_t)i * n + j] = a;
         A[(size_t)j * n + i] = a;
+      } else if (i == j) {
+        A[e] = T[e];
       }
-      V[e] = V0[e];
     }
   }
   __syncthreads();
   const int m = n + (n & 1);  // players incl. a dummy when n is odd
   const int npair = m / 2;
+  const int nA = npair * npair, nV = n * npair;
   int sweep = 0;
   for (; sweep <= max_sweeps; ++sweep) {
     // convergence test
     double off = 0.0, tot = 0.0;
-    for (size_t e = tid; e < nn; e += blockDim.x) {
+    for (size_t e = tid; e < nn; e += NTH) {
       const double v = A[e] * A[e];
       tot += v;
       if (e / n != e % n) off += v;
@@ -343,7 +357,7 @@ __global__ __launch_bounds__(EIG_THREADS) void jacobi_eigh_kernel(const double* 
     __syncthreads();
     if (tid == 0) {
       double o = 0.0;
-      for (int i = 0; i < EIG_THREADS / 64; ++i) o += red[i];
+      for (int i = 0; i < NTH / 64; ++i) o += red[i];
       red[0] = o;
     }
     __syncthreads();
@@ -353,15 +367,14 @@ __global__ __launch_bounds__(EIG_THREADS) void jacobi_eigh_kernel(const double* 
     __syncthreads();
     if (tid == 0) {
       double t = 0.0;
-      for (int i = 0; i < EIG_THREADS / 64; ++i) t += red[i];
-      anorm = t;
+      for (int i = 0; i < NTH / 64; ++i) t += red[i];
       done = (offs <= tol * tol * t) || sweep == max_sweeps;
     }
     __syncthreads();
     if (done) break;
     for (int r = 0; r < m - 1; ++r) {
       // rotation parameters
-      for (int k = tid; k < npair; k += blockDim.x) {
+      for (int k = tid; k < npair; k += NTH) {
         int p, q;
         rr_partner(r, k, m, &p, &q);
         double c = 1.0, s = 0.0;
@@ -378,29 +391,47 @@ __global__ __launch_bounds__(EIG_THREADS) void jacobi_eigh_kernel(const double* 
         cs_c[k] = c; cs_s[k] = s; pp[k] = p; qq[k] = q;
       }
       __syncthreads();
-      // rows: A <- J^T A
-      for (int e = tid; e < npair * n; e += blockDim.x) {
-        const int k = e / n, j = e % n;
-        const int p = pp[k], q = qq[k];
-        if (q >= n) continue;
-        const double c = cs_c[k], s = cs_s[k];
-        const double apj = A[(size_t)p * n + j], aqj = A[(size_t)q * n + j];
-        A[(size_t)p * n + j] = c * apj - s * aqj;
-        A[(size_t)q * n + j] = s * apj + c * aqj;
-      }
-      __syncthreads();
-      // columns: A <- A J, V <- V J
-      for (int e = tid; e < npair * n; e += blockDim.x) {
-        const int k = e / n, i = e % n;
-        const int p = pp[k], q = qq[k];
-        if (q >= n) continue;
-        const double c = cs_c[k], s = cs_s[k];
-        const double aip = A[(size_t)i * n + p], aiq = A[(size_t)i * n + q];
-        A[(size_t)i * n + p] = c * aip - s * aiq;
-        A[(size_t)i * n + q] = s * aip + c * aiq;
-        const double vip = V[(size_t)i * n + p], viq = V[(size_t)i * n + q];
-        V[(size_t)i * n + p] = c * vip - s * viq;
-        V[(size_t)i * n + q] = s * vip + c * viq;
+      for (int e = tid; e < nA + nV; e += NTH) {
+        if (e < nA) {
+          // 2x2 block: rows of pair k (row rotation first), columns of pair l
+          const int k = e / npair, l = e % npair;
+          const int pk = pp[k], qk = qq[k], pl = pp[l], ql = qq[l];
+          const bool rk = qk < n, rl = ql < n;   // false: the dummy player, no rotation
+          const double ck = cs_c[k], sk = cs_s[k], cl = cs_c[l], sl = cs_s[l];
+          double r_pp = A[(size_t)pk * n + pl];
+          double r_pq = rl ? A[(size_t)pk * n + ql] : 0.0;
+          double r_qp = rk ? A[(size_t)qk * n + pl] : 0.0;
+          double r_qq = (rk && rl) ? A[(size_t)qk * n + ql] : 0.0;
+          if (rk) {
+            const double apj = r_pp, aqj = r_qp;
+            r_pp = ck * apj - sk * aqj;
+            r_qp = sk * apj + ck * aqj;
+            const double apj2 = r_pq, aqj2 = r_qq;
+            r_pq = ck * apj2 - sk * aqj2;
+            r_qq = sk * apj2 + ck * aqj2;
+          }
+          if (rl) {
+            A[(size_t)pk * n + pl] = cl * r_pp - sl * r_pq;
+            A[(size_t)pk * n + ql] = sl * r_pp + cl * r_pq;
+            if (rk) {
+              A[(size_t)qk * n + pl] = cl * r_qp - sl * r_qq;
+              A[(size_t)qk * n + ql] = sl * r_qp + cl * r_qq;
+            }
+          } else {
+            A[(size_t)pk * n + pl] = r_pp;
+            if (rk) A[(size_t)qk * n + pl] = r_qp;
+          }
+        } else {
+          // V <- V J: row i, pair l
+          const int ev = e - nA;
+          const int l = ev / n, i = ev % n;
+          const int pl = pp[l], ql = qq[l];
+          if (ql >= n) continue;
+          const double c = cs_c[l], s = cs_s[l];
+          const double vip = V[(size_t)i * n + pl], viq = V[(size_t)i * n + ql];
+          V[(size_t)i * n + pl] = c * vip - s * viq;
+          V[(size_t)i * n + ql] = s * vip + c * viq;
+        }
       }
       __syncthreads();
     }
@@ -408,7 +439,7 @@ __global__ __launch_bounds__(EIG_THREADS) void jacobi_eigh_kernel(const double* 
   // ascending order: rank of each diagonal element (ties by index)
   double* w = w_out + (size_t)b * n;
   double* Vo = V_out + (size_t)b * nn;
-  for (int i = tid; i < n; i += blockDim.x) {
+  for (int i = tid; i < n; i += NTH) {
     const double di = A[(size_t)i * n + i];
     int rank = 0;
     for (int j = 0; j < n; ++j) {
@@ -419,48 +450,140 @@ __global__ __launch_bounds__(EIG_THREADS) void jacobi_eigh_kernel(const double* 
     for (int row = 0; row < n; ++row) Vo[(size_t)row * n + rank] = V[(size_t)row * n + i];
   }
   if (tid == 0 && sweeps_out) sweeps_out[b] = sweep;
-  (void)anorm;
 }
 
 // ------------------------------------------------------------- DIIS solve ----------
-// x_s = B_s^-1 e_0 for each spin by Gaussian elimination with partial pivoting
-// (B_s is (m+1) x (m+1), m1 <= 32).  One 64-thread block per spin, augmented matrix in LDS,
-// one thread per row during elimination.
+// x = B^-1 e_0 for one bordered DIIS matrix B (m1 x m1, m1 <= DIIS_MAXM1) by Gaussian
+// elimination with partial pivoting (the first row of largest |pivot| wins), in one wave:
+// lane t < m1 holds row t and the right-hand side in registers; the pivot search, the row
+// swap and the pivot row are read across lanes with v_readlane (uniform lane indices), so
+// the solve needs no LDS round trips.
+constexpr int DIIS_MAXM1 = 16;
+__device__ __forceinline__ double lane_bcast(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ void diis_solve_wave(const double* __restrict__ Bs, int m1, double* __restrict__ xs) {
+  const int t = threadIdx.x;
+  double r[DIIS_MAXM1 + 1];
+#pragma unroll
+  for (int j = 0; j <= DIIS_MAXM1; ++j) r[j] = 0.0;
+#pragma unroll
+  for (int j = 0; j < DIIS_MAXM1; ++j)
+    if (j < m1 && t < m1) r[j] = Bs[(size_t)t * m1 + j];
+  r[DIIS_MAXM1] = t == 0 ? 1.0 : 0.0;   // right-hand side e_0
+#pragma unroll
+  for (int c = 0; c < DIIS_MAXM1; ++c) {
+    if (c < m1) {
+      // pivot: first row p >= c with the largest |r[c]|
+      const double ac = fabs(r[c]);
+      double best = lane_bcast(ac, c);
+      int p = c;
+#pragma unroll
+      for (int i = c + 1; i < DIIS_MAXM1; ++i) {
+        if (i < m1) {
+          const double v = lane_bcast(ac, i);
+          if (v > best) { best = v; p = i; }
+        }
+      }
+      p = __builtin_amdgcn_readfirstlane(p);
+      if (p != c) {
+#pragma unroll
+        for (int j = 0; j <= DIIS_MAXM1; ++j) {
+          const double vc = lane_bcast(r[j], c), vp = lane_bcast(r[j], p);
+          if (t == c) r[j] = vp;
+          else if (t == p) r[j] = vc;
+        }
+      }
+      // eliminate below the pivot with the pivot row (columns j >= c and the right-hand side)
+      const double pc = lane_bcast(r[c], c);
+      const double f = (t > c && t < m1) ? r[c] / pc : 0.0;
+#pragma unroll
+      for (int j = c; j <= DIIS_MAXM1; ++j) {
+        if (j < m1 || j == DIIS_MAXM1) {
+          const double pj = lane_bcast(r[j], c);
+          if (t > c && t < m1) r[j] -= f * pj;
+        }
+      }
+    }
+  }
+  // back substitution, last row first
+  double x[DIIS_MAXM1];
+#pragma unroll
+  for (int j = 0; j < DIIS_MAXM1; ++j) x[j] = 0.0;
+#pragma unroll
+  for (int i = DIIS_MAXM1 - 1; i >= 0; --i) {
+    if (i < m1) {
+      double v = r[DIIS_MAXM1];
+#pragma unroll
+      for (int j = i + 1; j < DIIS_MAXM1; ++j)
+        if (j < m1) v -= r[j] * x[j];
+      x[i] = lane_bcast(v / r[i], i);
+    }
+  }
+  if (t < m1) {
+    double xv = 0.0;
+#pragma unroll
+    for (int j = 0; j < DIIS_MAXM1; ++j)
+      if (j == t) xv = x[j];
+    xs[t] = xv;
+  }
+}
+
+// x_s = B_s^-1 e_0 for each spin: one wave per spin.
 __global__ __launch_bounds__(64) void diis_solve_kernel(const double* __restrict__ B, int m1,
                                                         double* __restrict__ x) {
-  __shared__ double a[32][34];
-  __shared__ int piv;
-  const int s = blockIdx.x, t = threadIdx.x;
-  for (int e = t; e < m1 * m1; e += blockDim.x) a[e / m1][e % m1] = B[(size_t)s * m1 * m1 + e];
-  if (t < m1) a[t][m1] = t == 0 ? 1.0 : 0.0;
+  const int s = blockIdx.x;
+  diis_solve_wave(B + (size_t)s * m1 * m1, m1, x + (size_t)s * m1);
+}
+
+// JittableDiis coefficients (evaluate.py:1185-1205), two launches:
+//  diis_gram_kernel: block (pair (i, j), spin s) = <err_i, err_j> over the n2 elements of
+//    errs (m, nspin, n2), summed per wave then over the waves in a fixed order, written to
+//    B_s[1 + i][1 + j] and B_s[1 + j][1 + i];
+//  diis_border_solve_kernel: the borders (+1 for the slots i <= cycle, B[0][0] = 0) and 1 on
+//    the diagonal of the inactive slots, B written back, then x = B^-1 e_0.
+constexpr int DIIS_THREADS = 256;
+__global__ __launch_bounds__(DIIS_THREADS) void diis_gram_kernel(const double* __restrict__ errs, int m, int nspin,
+                                                                 long n2, double* __restrict__ B) {
+  __shared__ double wsum[DIIS_THREADS / 64];
+  const int pi = blockIdx.x, s = blockIdx.y, t = threadIdx.x;
+  int i = 0, rem = pi;
+  while (rem >= m - i) { rem -= m - i; ++i; }
+  const int j = i + rem;
+  const size_t stride = (size_t)nspin * n2;
+  const double* Ei = errs + (size_t)i * stride + (size_t)s * n2;
+  const double* Ej = errs + (size_t)j * stride + (size_t)s * n2;
+  double v = 0.0;
+  for (long e = t; e < n2; e += DIIS_THREADS) v += Ei[e] * Ej[e];
+  v = warp_sum(v);
+  if ((t & 63) == 0) wsum[t >> 6] = v;
   __syncthreads();
-  for (int c = 0; c < m1; ++c) {
-    if (t == 0) {
-      int p = c;
-      for (int i = c + 1; i < m1; ++i)
-        if (fabs(a[i][c]) > fabs(a[p][c])) p = i;
-      piv = p;
-    }
-    __syncthreads();
-    if (piv != c && t <= m1) {
-      const double tmp = a[c][t];
-      a[c][t] = a[piv][t];
-      a[piv][t] = tmp;
-    }
-    __syncthreads();
-    if (t > c && t < m1) {
-      const double f = a[t][c] / a[c][c];
-      for (int j = c; j <= m1; ++j) a[t][j] -= f * a[c][j];
-    }
-    __syncthreads();
-  }
   if (t == 0) {
-    for (int i = m1 - 1; i >= 0; --i) {
-      double v = a[i][m1];
-      for (int j = i + 1; j < m1; ++j) v -= a[i][j] * x[(size_t)s * m1 + j];
-      x[(size_t)s * m1 + i] = v / a[i][i];
-    }
+    double g = 0.0;
+    for (int w = 0; w < DIIS_THREADS / 64; ++w) g += wsum[w];
+    const int m1 = m + 1;
+    double* Bs = B + (size_t)s * m1 * m1;
+    Bs[(size_t)(1 + i) * m1 + 1 + j] = g;
+    Bs[(size_t)(1 + j) * m1 + 1 + i] = g;
   }
+}
+
+__global__ __launch_bounds__(64) void diis_border_solve_kernel(int m, int cycle, double* __restrict__ B,
+                                                               double* __restrict__ x) {
+  const int s = blockIdx.x, t = threadIdx.x, m1 = m + 1;
+  double* Bs = B + (size_t)s * m1 * m1;
+  if (t == 0) Bs[0] = 0.0;
+  if (t < m) {
+    const double act = t <= cycle ? 1.0 : 0.0;
+    Bs[1 + t] = act;
+    Bs[(size_t)(1 + t) * m1] = act;
+    if (t > cycle) Bs[(size_t)(1 + t) * m1 + 1 + t] = 1.0;
+  }
+  __syncthreads();
+  diis_solve_wave(Bs, m1, x + (size_t)s * m1);
 }
 
 inline unsigned nblk(long n) { return (unsigned)((n + NT - 1) / NT); }
@@ -541,7 +664,7 @@ int gdx_cholesky_inv(void* stream, int32_t n, const double* S, double* L, double
 
 int gdx_eigh_workspace_size(int32_t nbatch, int32_t n, size_t* bytes) {
   GDX_REQUIRE(nbatch > 0 && n > 0 && bytes, "gdx_eigh_workspace_size: bad arguments");
-  *bytes = (size_t)nbatch * 2 * n * n * sizeof(double);
+  *bytes = (size_t)nbatch * 3 * n * n * sizeof(double);
   return GDX_OK;
 }
 
@@ -550,21 +673,33 @@ int gdx_eigh_batched(void* stream, int32_t nbatch, int32_t n, const double* A, c
   GDX_REQUIRE(nbatch > 0 && n > 0 && n <= 1022 && A && w && V, "gdx_eigh_batched: bad arguments");
   GDX_REQUIRE(V0 == nullptr || (V0 != V && V0 != A), "gdx_eigh_batched: V0 must not alias A or V");
   hipStream_t st = (hipStream_t)stream;
-  const size_t smem = (size_t)2 * n * n * sizeof(double);
-  if (smem <= 64 * 1024) {
-    GDX_LAUNCH("jacobi_eigh", jacobi_eigh_kernel<true>, dim3(nbatch), dim3(EIG_THREADS), smem, st, A, n, V0, w, V,
-               (double*)workspace, max_sweeps, tol, sweeps);
+  const size_t smem = (size_t)3 * n * n * sizeof(double);
+  if (smem <= EIG_SMEM_MAX) {
+    GDX_LAUNCH("jacobi_eigh", (jacobi_eigh_kernel<true, EIG_THREADS_SMEM>), dim3(nbatch), dim3(EIG_THREADS_SMEM),
+               smem, st, A, n, V0, w, V, (double*)workspace, max_sweeps, tol, sweeps);
   } else {
-    GDX_REQUIRE(workspace, "gdx_eigh_batched: workspace required for n > 64");
-    GDX_LAUNCH("jacobi_eigh", jacobi_eigh_kernel<false>, dim3(nbatch), dim3(EIG_THREADS), 0, st, A, n, V0, w, V,
-               (double*)workspace, max_sweeps, tol, sweeps);
+    GDX_REQUIRE(workspace, "gdx_eigh_batched: workspace required for n > 52");
+    GDX_LAUNCH("jacobi_eigh", (jacobi_eigh_kernel<false, EIG_THREADS_GLOBAL>), dim3(nbatch),
+               dim3(EIG_THREADS_GLOBAL), 0, st, A, n, V0, w, V, (double*)workspace, max_sweeps, tol, sweeps);
   }
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
 }
 
+int gdx_diis_coefficients(void* stream, int32_t nspin, int32_t m, int64_t n2, const double* errs, int32_t cycle,
+                          double* B, double* x) {
+  GDX_REQUIRE(nspin > 0 && nspin <= 64 && m > 0 && m + 1 <= DIIS_MAXM1 && n2 > 0 && errs && B && x,
+              "gdx_diis_coefficients: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  GDX_LAUNCH("diis_gram", diis_gram_kernel, dim3(m * (m + 1) / 2, nspin), dim3(DIIS_THREADS), 0, st, errs, m, nspin,
+             (long)n2, B);
+  GDX_LAUNCH("diis_solve", diis_border_solve_kernel, dim3(nspin), dim3(64), 0, st, m, cycle, B, x);
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
 int gdx_diis_solve(void* stream, int32_t nspin, int32_t m1, const double* B, double* x) {
-  GDX_REQUIRE(nspin > 0 && nspin <= 64 && m1 > 0 && m1 <= 32 && B && x, "gdx_diis_solve: bad arguments");
+  GDX_REQUIRE(nspin > 0 && nspin <= 64 && m1 > 0 && m1 <= DIIS_MAXM1 && B && x, "gdx_diis_solve: bad arguments");
   GDX_LAUNCH("diis_solve", diis_solve_kernel, dim3(nspin), dim3(64), 0, (hipStream_t)stream, B, m1, x);
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;

@@ -36,6 +36,47 @@ def _diis_solve(B: torch.Tensor) -> torch.Tensor:
     return x
 
 
+def _diis_coefficients(errs: torch.Tensor, cycle: int):
+    errs = errs.detach().contiguous()
+    m, ns = errs.shape[0], errs.shape[1]
+    B = torch.empty((ns, m + 1, m + 1), dtype=torch.float64, device=errs.device)
+    x = torch.empty((ns, m + 1), dtype=torch.float64, device=errs.device)
+    _lib.check(_lib.load().gdx_diis_coefficients(torch.cuda.current_stream(errs.device).cuda_stream, ns, m,
+                                                 errs[0, 0].numel(), errs.data_ptr(), int(cycle), B.data_ptr(),
+                                                 x.data_ptr()), "gdx_diis_coefficients")
+    return B, x
+
+
+class DiisCoefficientsFn(torch.autograd.Function):
+    """JittableDiis coefficients from the stored errors (m, nspin, n, n) in one device call
+    (Gram matrix, bordered matrix, solve).  Reverse: Bbar = -B^-T xbar x^T as for the solve;
+    the Gram entries get Bbar's inner block (diagonal only for active slots, the inactive ones
+    are the constant 1), and errs get (G + G^T) E per spin."""
+
+    @staticmethod
+    def forward(ctx, errs, cycle):
+        B, x = _diis_coefficients(errs, cycle)
+        ctx.save_for_backward(errs, B, x)
+        ctx.cycle = cycle
+        return x[:, 1:]
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, gc):
+        errs, B, x = ctx.saved_tensors
+        m, ns = errs.shape[0], errs.shape[1]
+        gx = torch.zeros_like(x)
+        gx[:, 1:] = gc
+        y = torch.linalg.solve(B.transpose(1, 2), gx.unsqueeze(-1)).squeeze(-1)
+        G = -(y.unsqueeze(-1) * x.unsqueeze(-2))[:, 1:, 1:]
+        inactive = torch.arange(m, device=errs.device) > ctx.cycle
+        G = G.clone()
+        G[:, inactive, inactive] = 0.0
+        E = errs.reshape(m, ns, -1).transpose(0, 1)                 # (ns, m, n2)
+        Ebar = torch.matmul(G + G.transpose(1, 2), E)               # (ns, m, n2)
+        return Ebar.transpose(0, 1).reshape(errs.shape), None
+
+
 class DiisSolveFn(torch.autograd.Function):
     """x_s = B_s^-1 e_0; reverse: Bbar_s = -B_s^-T xbar_s x_s^T (B_s is symmetric)."""
 
@@ -73,25 +114,19 @@ class JittableDiis:
             focks = torch.cat([focks[1:], fock[None]])
             energies = torch.cat([energies[1:], energy.reshape(1)])
         elif cycle < self.max_diis:
-            errs = errs.clone(); errs[cycle] = err
-            dens = dens.clone(); dens[cycle] = rdm1
-            focks = focks.clone(); focks[cycle] = fock
-            energies = energies.clone(); energies[cycle] = energy
+            if torch.is_grad_enabled() and any(t.requires_grad for t in (err, rdm1, fock, energy)):
+                # functional update: tensors saved for the reverse pass must not change
+                errs, dens, focks, energies = errs.clone(), dens.clone(), focks.clone(), energies.clone()
+            errs[cycle] = err
+            dens[cycle] = rdm1
+            focks[cycle] = fock
+            energies[cycle] = energy
         return [dens, focks, energies, errs]
 
     def coefficients(self, errs: torch.Tensor, cycle: int) -> torch.Tensor:
-        m = errs.shape[0]
-        E = errs.reshape(m, 2, -1).transpose(0, 1)              # (2, m, n*n)
-        gram = E @ E.transpose(1, 2)                             # (2, m, m)
-        B = torch.zeros((2, m + 1, m + 1), dtype=torch.float64, device=errs.device)
-        B[:, 1:, 1:] = gram
-        active = (torch.arange(m, device=errs.device) <= cycle).to(torch.float64)
-        B[:, 0, 1:] = active
-        B[:, 1:, 0] = active
-        idx = torch.arange(m, device=errs.device)
-        B[:, idx + 1, idx + 1] = torch.where(active > 0, gram[:, idx, idx], torch.ones_like(gram[:, idx, idx]))
-        x = DiisSolveFn.apply(B)
-        return x[:, 1:]
+        # Gram matrix, bordered matrix (+1 borders for slots <= cycle, 1 on the diagonal of the
+        # others) and x = B^-1 e_0 in one device call (gdx_diis_coefficients)
+        return DiisCoefficientsFn.apply(errs, cycle)
 
     def run(self, rdm1, fock, energy, data, cycle: int):
         data = self.update(rdm1, fock, energy, data, cycle)

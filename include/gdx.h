@@ -155,16 +155,24 @@ int gdx_cholesky_inv(void* stream, int32_t n, const double* S, double* L, double
  * A (nbatch, n, n) -> w (nbatch, n) ascending, V (nbatch, n, n) eigenvectors as columns.
  * V0 (nbatch, n, n), optional (NULL = none): orthonormal starting eigenvector estimates (the
  * previous SCF cycle's), which cut the Jacobi sweeps for a nearby matrix; it must not alias
- * A or V.  workspace: gdx_eigh_workspace_size bytes (used when n > 64).  sweeps: optional
+ * A or V.  workspace: gdx_eigh_workspace_size bytes (used when n > 52).  sweeps: optional
  * device int per matrix (Jacobi sweeps used). */
 int gdx_eigh_workspace_size(int32_t nbatch, int32_t n, size_t* bytes);
 int gdx_eigh_batched(void* stream, int32_t nbatch, int32_t n, const double* A, const double* V0,
                      double* w, double* V,
                      void* workspace, int32_t max_sweeps, double tol, int32_t* sweeps);
 
-/* x_s = B_s^-1 e_0 for nspin bordered DIIS matrices B (nspin, m1, m1), m1 <= 32:
+/* x_s = B_s^-1 e_0 for nspin bordered DIIS matrices B (nspin, m1, m1), m1 <= 16:
  * JittableDiis.cdiis_minimize, grad_dft/evaluate.py:1198-1205. */
 int gdx_diis_solve(void* stream, int32_t nspin, int32_t m1, const double* B, double* x);
+
+/* JittableDiis coefficients (grad_dft/evaluate.py:1185-1205) in one call: errs (m, nspin,
+ * n2) holds the m stored error vectors FDS - SDF (n2 = n^2 elements each); the Gram matrix per
+ * spin, the bordered matrix B (nspin, m+1, m+1) -- +1 borders for the slots i <= cycle, the
+ * Gram diagonal there and 1 on the diagonal of the others, B[0][0] = 0 -- and x = B^-1 e_0
+ * (nspin, m+1) are written.  m + 1 <= 16. */
+int gdx_diis_coefficients(void* stream, int32_t nspin, int32_t m, int64_t n2, const double* errs, int32_t cycle,
+                          double* B, double* x);
 
 /* Kernel timing (no counterpart in the reference): while on, every kernel this library
  * enqueues is bracketed by HIP events on its own stream.  gdx_profile_end waits for them and

@@ -122,6 +122,54 @@ def test_diis_solve_matches_inverse():
     assert np.abs(x.cpu().numpy() - x0).max() <= 1e-10 * np.abs(x0).max()
 
 
+def test_diis_run_matches_oracle_with_gradients():
+    """JittableDiis.run (update + gdx_diis_coefficients) over 14 cycles -- slots filling, the
+    dropped write at cycle == max_diis, then shifting -- against the oracle's torch restatement:
+    extrapolated Fock matrices, and the gradient of a weighted sum of them w.r.t. every input
+    Fock and density matrix (reverse pass through the coefficient Function)."""
+    _say("case: diis run vs oracle, 14 cycles, n=6, forward and reverse")
+    from graddft_amd.evaluate import JittableDiis
+    from oracle import scf_grad as G
+    rng = np.random.default_rng(12)
+    n, cycles = 6, 14
+    X = rng.normal(size=(n, n))
+    S = X @ X.T + n * np.eye(n)
+    focks = [rng.normal(size=(2, n, n)) for _ in range(cycles)]
+    focks = [f + f.transpose(0, 2, 1) for f in focks]
+    dens = [rng.normal(size=(2, n, n)) * 0.1 for _ in range(cycles)]
+    dens = [d + d.transpose(0, 2, 1) for d in dens]
+    W = [rng.normal(size=(2, n, n)) for _ in range(cycles)]
+
+    def run(dev, ours):
+        Ft = [torch.tensor(f, device=dev, requires_grad=True) for f in focks]
+        Dt = [torch.tensor(d, device=dev, requires_grad=True) for d in dens]
+        St = torch.tensor(S, device=dev)
+        if ours:
+            diis = JittableDiis(St, max_diis=10)
+            data = diis.zeros(n, dev)
+        else:
+            diis = G.JittableDiis(St, 10)
+            data = (torch.zeros((10, 2, n, n), dtype=torch.float64), torch.zeros((10, 2, n, n), dtype=torch.float64))
+        outs, loss = [], 0.0
+        for c in range(cycles):
+            if ours:
+                F, data = diis.run(Dt[c], Ft[c], torch.tensor(0.1 * c, device=dev, dtype=torch.float64), data, c)
+            else:
+                F, data = diis.run(Dt[c], Ft[c], data, c)
+            outs.append(F.detach().cpu().numpy())
+            loss = loss + (F * torch.tensor(W[c], device=dev)).sum()
+        # the inputs of cycle == max_diis are dropped by the reference's buffer update: no gradient
+        gF = torch.autograd.grad(loss, Ft + Dt, allow_unused=True)
+        return outs, [np.zeros((2, n, n)) if g is None else g.cpu().numpy() for g in gF]
+
+    o1, g1 = run("cuda", True)
+    o0, g0 = run("cpu", False)
+    for c in range(cycles):
+        assert np.abs(o1[c] - o0[c]).max() <= 1e-10 * max(1.0, np.abs(o0[c]).max()), c
+    for a, b in zip(g1, g0):
+        assert np.abs(a - b).max() <= 1e-8 * max(1.0, np.abs(b).max())
+
+
 def _oracle_scf_inputs(mol):
     from oracle.ao import eval_ao
     from oracle.scf import SCFInputs, exchange_potential
