@@ -9,7 +9,7 @@
 // M % 128 == N % 128 == 0 and the K range of every split a multiple of 16 (callers pad).
 // Blocks are renumbered so consecutive tiles (which share the same A rows) are dispatched to
 // the same XCD and its L2.
-// Split-K: split y (see xcd_tile_split) covers K range [y*kchunk, (y+1)*kchunk); C is offset by y*M*ldc.
+// Split-K: split y (see xcd_tile_split) covers the K range split_range gives; C is offset by y*M*ldc.
 // colsum (optional): per split, the column sums of op(B) over its K range (bias gradients).
 #pragma once
 #include "gdx_common.h"
@@ -62,11 +62,22 @@ __device__ __forceinline__ void xcd_tile_split(int nblk, int& b, int& ysplit) {
   }
 }
 
+// K range of split y: the splits are sps per segment of kseg rows (segments in order), each
+// kchunk rows long except the last of a segment, which stops at the segment's end (splits
+// past it are empty).  One segment of nsplit * kchunk rows is plain split-K; two segments
+// keep the value and tangent rows of the second-order pass in separate splits.
+__device__ __forceinline__ void split_range(int y, int kchunk, int kseg, int sps, int& kbeg, int& kend) {
+  const int seg = y / sps, js = y - seg * sps;
+  kbeg = seg * kseg + js * kchunk;
+  kend = min(kbeg + kchunk, (seg + 1) * kseg);
+  if (kend < kbeg) kend = kbeg;
+}
+
 template <typename T, bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256, sizeof(T) == 4 ? (EPI == EPI_ELU_BWD ? 2 : 3) : 1) void gemm_tile_kernel(
     int M, int N, const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb,
     T* __restrict__ C, int ldc, const T* __restrict__ bias, const T* __restrict__ Y, int ldy,
-    int kchunk, T* __restrict__ colsum) {
+    int kchunk, T* __restrict__ colsum, int kseg, int sps) {
   typedef typename MfmaT<T>::acc_t acc_t;
   constexpr int GT_K = GemmK<T>::value;
   constexpr int VEC = 16 / sizeof(T);                // elements per 16-byte load
@@ -88,8 +99,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? (EPI == EPI_ELU_BWD ? 2 : 3) 
   int b, ysplit;
   xcd_tile_split(nblk, b, ysplit);
   const int m0 = (b / nbn) * GT_M, n0 = (b % nbn) * GT_N;
-  const int kbeg = ysplit * kchunk;
-  const int nk = kchunk / GT_K;
+  int kbeg, kend;
+  split_range(ysplit, kchunk, kseg, sps, kbeg, kend);
+  const int nk = (kend - kbeg) / GT_K;
 
   vec_t ra[NVA], rb[NVB];
   auto gload = [&](int k0) {

@@ -32,7 +32,7 @@ template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void gemm_bf16x6_kernel(
     int M, int N, const float* __restrict__ A, int lda, const float* __restrict__ B, int ldb,
     float* __restrict__ C, int ldc, const float* __restrict__ bias, const float* __restrict__ Y, int ldy,
-    int kchunk, float* __restrict__ colsum) {
+    int kchunk, float* __restrict__ colsum, int kseg, int sps) {
   typedef float f4v __attribute__((ext_vector_type(4)));
   __shared__ __bf16 As[2][3][GB_M][GB_KP];
   __shared__ __bf16 Bs[2][3][GB_N][GB_KP];
@@ -44,8 +44,9 @@ __global__ __launch_bounds__(256) void gemm_bf16x6_kernel(
   int b, ysplit;
   xcd_tile_split(nblk, b, ysplit);
   const int m0 = (b / nbn) * GB_M, n0 = (b % nbn) * GB_N;
-  const int kbeg = ysplit * kchunk;
-  const int nk = kchunk / GB_K;
+  int kbeg, kend;
+  split_range(ysplit, kchunk, kseg, sps, kbeg, kend);
+  const int nk = (kend - kbeg) / GB_K;
   const bool do_cs = colsum != nullptr && m0 == 0;
 
   // 128 x 32 floats per operand tile = 1024 float4, 4 per thread

@@ -249,7 +249,7 @@ template <typename T, int BM, int BN, bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(NT) void gemm_any_kernel(
     int M, int N, int K, const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb,
     T* __restrict__ C, int ldc, const T* __restrict__ bias, const T* __restrict__ Y, int ldy, int kchunk,
-    T* __restrict__ colsum) {
+    T* __restrict__ colsum, int kseg, int sps) {
   typedef typename MfmaT<T>::acc_t acc_t;
   constexpr int WM = BM / 2, WN = BN / 2, MB = WM / 16, NB = WN / 16;
   __shared__ T As[GBK][BM + 2];
@@ -257,8 +257,9 @@ __global__ __launch_bounds__(NT) void gemm_any_kernel(
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
-  const int kbeg = blockIdx.z * kchunk;
-  const int kend = min(K, kbeg + kchunk);
+  int kbeg, kend;
+  split_range(blockIdx.z, kchunk, kseg, sps, kbeg, kend);
+  kend = min(K, kend);
   T csum = 0;
   acc_t acc[MB][NB];
 #pragma unroll
@@ -707,7 +708,7 @@ struct Layout {
   size_t Dp2, dpart_t, dfin_t, Edot, ZT, GEd, edot_part, scal, rtmp;
   size_t total;
   size_t hstride;  // elements between the activation buffers of consecutive layers
-  int nsd, nchunk, chunk, nsplit, kchunk, npt_blocks, nwblk, wrows, rows;
+  int nsd, nchunk, chunk, nsplit, kchunk, kseg, sps, npt_blocks, nwblk, wrows, rows;
   size_t nparam;
 };
 
@@ -728,11 +729,17 @@ Layout make_layout(const gdx_xc_desc& d, bool hvp = false) {
                    v2 = vxc_split((int)Np, (int)nP, 2, d.precision, hvp);
     L.nchunk = std::max(v1.nchunk, 2 * v2.nchunk);   // partial slots: nchunk x nsv
   }
-  // split-K for the hidden weight-gradient GEMMs: whole 32-row stages per split (for
-  // hvp, an even count so the first half of the splits covers the value rows exactly)
-  L.nsplit = 1;
-  while (L.nsplit < 128 && (R / GB_K) % (2 * L.nsplit) == 0) L.nsplit *= 2;
-  L.kchunk = (int)(R / L.nsplit);
+  // split-K for the hidden weight-gradient GEMMs: 128 splits of whole 32-row stages, the
+  // last one of a segment shorter; for hvp the value and tangent rows are two segments of
+  // 64 splits each, so the first half of the splits covers the value rows exactly
+  {
+    const int segs = hvp ? 2 : 1;
+    L.kseg = (int)Np;
+    L.sps = 128 / segs;
+    const size_t kc = (Np + L.sps - 1) / L.sps;
+    L.kchunk = (int)std::max<size_t>(GB_K, (kc + GB_K - 1) / GB_K * GB_K);
+    L.nsplit = segs * L.sps;
+  }
   L.npt_blocks = (int)((Np + NT - 1) / NT);
   L.wrows = SKW_ROWS;
   L.nwblk = (int)((R + SKW_ROWS - 1) / SKW_ROWS);
@@ -821,17 +828,22 @@ template <typename T> const char* tname(const char* f64, const char* f32) { retu
 
 template <typename T, int BM, int BN, bool TA, bool TB, int EPI>
 void launch_any(hipStream_t st, const char* name, int M, int N, int K, const T* A, int lda, const T* B, int ldb,
-                T* C, int ldc, const T* bias, const T* Y, int ldy, int nsplit, int kchunk, T* colsum) {
+                T* C, int ldc, const T* bias, const T* Y, int ldy, int nsplit, int kchunk, T* colsum,
+                int kseg = 0, int sps = 0) {
   dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, nsplit);
+  if (sps <= 0) sps = nsplit;
+  if (kseg <= 0) kseg = nsplit * kchunk;
   GDX_LAUNCH(name, (gemm_any_kernel<T, BM, BN, TA, TB, EPI>), grid, dim3(NT), 0, st, M, N, K, A, lda, B, ldb, C,
-             ldc, bias, Y, ldy, kchunk, colsum);
+             ldc, bias, Y, ldy, kchunk, colsum, kseg, sps);
 }
 
 template <typename T, bool TA, bool TB, int EPI>
 void launch_tile(hipStream_t st, const char* name, int M, int N, int nsplit, int kchunk, const T* A, int lda,
                  const T* B, int ldb, T* C, int ldc, const T* bias, const T* Y, int ldy, T* colsum,
-                 bool bf16 = false) {
+                 bool bf16 = false, int kseg = 0, int sps = 0) {
   dim3 grid((M / GT_M) * (N / GT_N), nsplit);
+  if (sps <= 0) sps = nsplit;
+  if (kseg <= 0) kseg = nsplit * kchunk;
   if constexpr (sizeof(T) == 4) {
     if (bf16) {
       char nm[64];
@@ -839,13 +851,13 @@ void launch_tile(hipStream_t st, const char* name, int M, int N, int nsplit, int
       // "gemm_f32_*" -> "gemm_bf16x6_*" in the timing table
       const char* tag = TA ? "gemm_bf16x6_wgrad" : (EPI == EPI_BIAS_ELU ? "gemm_bf16x6_fwd" : "gemm_bf16x6_bwd_data");
       GDX_LAUNCH(tag, (gemm_bf16x6_kernel<TA, TB, EPI>), grid, dim3(NT), 0, st, M, N, A, lda, B, ldb, C, ldc, bias,
-                 Y, ldy, kchunk, colsum);
+                 Y, ldy, kchunk, colsum, kseg, sps);
       (void)nm;
       return;
     }
   }
   GDX_LAUNCH(name, (gemm_tile_kernel<T, TA, TB, EPI>), grid, dim3(NT), 0, st, M, N, A, lda, B, ldb, C, ldc,
-             bias, Y, ldy, kchunk, colsum);
+             bias, Y, ldy, kchunk, colsum, kseg, sps);
 }
 
 // parameter offsets (elements) of W_l and b_l in the flat parameter vector
@@ -1080,10 +1092,10 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
         if (tiles)
           launch_tile<T, true, false, EPI_STORE>(st, tname<T>("gemm_f64_wgrad", "gemm_f32_wgrad"), h, h, L.nsplit,
                                                  L.kchunk, Hin, h, Gcur, h, wp, h, nullptr, nullptr, 0, cp,
-                                                 mlp_bf16(d));
+                                                 mlp_bf16(d), L.kseg, L.sps);
         else
           launch_any<T, 64, 64, true, false, EPI_STORE>(st, "gemm_any", h, h, Np, Hin, h, Gcur, h, wp, h, nullptr,
-                                                        nullptr, 0, L.nsplit, L.kchunk, cp);
+                                                        nullptr, 0, L.nsplit, L.kchunk, cp, L.kseg, L.sps);
         reduce_slices<T>(c, wp, L.nsplit, (long)h * h, (long)h * h, d_params + po.w[l]);
         reduce_slices<T>(c, cp, L.nsplit, (long)h, (long)h, d_params + po.b[l]);
       }
@@ -1218,10 +1230,10 @@ int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm
         if (tiles)
           launch_tile<T, true, false, EPI_STORE>(st, tname<T>("gemm_f64_wgrad", "gemm_f32_wgrad"), h, h, L.nsplit,
                                                  L.kchunk, Hin2, h, Gcur, h, wp, h, nullptr, nullptr, 0, cp,
-                                                 mlp_bf16(d));
+                                                 mlp_bf16(d), L.kseg, L.sps);
         else
           launch_any<T, 64, 64, true, false, EPI_STORE>(st, "gemm_any", h, h, R, Hin2, h, Gcur, h, wp, h, nullptr,
-                                                        nullptr, 0, L.nsplit, L.kchunk, cp);
+                                                        nullptr, 0, L.nsplit, L.kchunk, cp, L.kseg, L.sps);
         reduce_slices<T>(c, wp, L.nsplit, (long)h * h, (long)h * h, d_params + po.w[l]);
         // bias: column sums over the value rows only (the first half of the splits)
         reduce_slices<T>(c, cp, L.nsplit / 2, (long)h, (long)h, d_params + po.b[l]);
