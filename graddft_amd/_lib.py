@@ -51,6 +51,9 @@ SIGNATURES = {
     "gdx_eigh_batched": (ctypes.c_int, [_P, _I, _I, _P, _P, _P, _P, _P, _I, ctypes.c_double, _P]),
     "gdx_diis_solve": (ctypes.c_int, [_P, _I, _I, _P, _P]),
     "gdx_diis_coefficients": (ctypes.c_int, [_P, _I, _I, ctypes.c_int64, _P, _I, _P, _P]),
+    "gdx_scf_step_workspace_size": (ctypes.c_int, [_I, _I, ctypes.POINTER(_Z)]),
+    "gdx_scf_step": (ctypes.c_int, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, ctypes.c_double,
+                                    _P]),
     "gdx_profile_begin": (ctypes.c_int, []),
     "gdx_profile_end": (ctypes.c_int, [_I, ctypes.c_char_p, _P, _P, _P]),
     "gdx_xc_densities": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P]),

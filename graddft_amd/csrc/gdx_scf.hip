@@ -274,71 +274,68 @@ __device__ __forceinline__ int rr_partner(int r, int k, int m, int* p, int* q) {
   return 0;
 }
 
+// Shared LDS bookkeeping of one Jacobi workgroup.
+struct JacobiShared {
+  double cs_c[512], cs_s[512];
+  int pp[512], qq[512];
+  double red[16];
+  int done;
+};
+
 // Warm start (V0 != NULL, orthonormal columns): the sweeps run on V0^T A V0 starting from
 // V = V0, so a matrix close to one already diagonalised by V0 needs few sweeps; the result
-// is the same eigendecomposition (eigenvalues, and eigenvectors up to sign).  A and V0 are
-// first staged next to each other (LDS or workspace), so the two products read no global
-// memory in their inner loops.
-template <bool SMEM, int NTH>
-__global__ __launch_bounds__(NTH) void jacobi_eigh_kernel(const double* __restrict__ Ain, int n,
-                                                          const double* __restrict__ V0in,
-                                                          double* __restrict__ w_out,
-                                                          double* __restrict__ V_out,
-                                                          double* __restrict__ gws, int max_sweeps,
-                                                          double tol, int* __restrict__ sweeps_out) {
-  extern __shared__ double smem[];
-  const int b = blockIdx.x;
+// is the same eigendecomposition (eigenvalues, and eigenvectors up to sign).  On entry T
+// holds the matrix; A, V, T are n x n buffers (LDS or workspace) and the two products read
+// no global memory in their inner loops.  On exit A holds the (rotated) matrix, V the start.
+template <int NTH>
+__device__ void jacobi_start(double* A, double* V, double* T, int n, const double* __restrict__ V0) {
   const int tid = threadIdx.x;
   const size_t nn = (size_t)n * n;
-  double* A = SMEM ? smem : gws + (size_t)b * 3 * nn;
-  double* V = A + nn;
-  double* T = V + nn;   // warm start: the staged input, then V0^T (A V0)
-  __shared__ double cs_c[512], cs_s[512];
-  __shared__ int pp[512], qq[512];
-  __shared__ double red[NTH / 64];
-  __shared__ int done;
-  const double* Asrc = Ain + (size_t)b * nn;
-  if (V0in == nullptr) {
+  if (V0 == nullptr) {
     for (size_t e = tid; e < nn; e += NTH) {
-      A[e] = Asrc[e];
+      A[e] = T[e];
       V[e] = (e / n == e % n) ? 1.0 : 0.0;
     }
-  } else {
-    const double* V0 = V0in + (size_t)b * nn;
-    for (size_t e = tid; e < nn; e += NTH) {
-      T[e] = Asrc[e];
-      V[e] = V0[e];
-    }
     __syncthreads();
-    // A <- Ain V0
-    for (size_t e = tid; e < nn; e += NTH) {
-      const int i = (int)(e / n), j = (int)(e % n);
-      double t = 0.0;
-      for (int k = 0; k < n; ++k) t += T[(size_t)i * n + k] * V[(size_t)k * n + j];
-      A[e] = t;
-    }
-    __syncthreads();
-    // T <- V0^T A
-    for (size_t e = tid; e < nn; e += NTH) {
-      const int i = (int)(e / n), j = (int)(e % n);
-      double t = 0.0;
-      for (int k = 0; k < n; ++k) t += V[(size_t)k * n + i] * A[(size_t)k * n + j];
-      T[e] = t;
-    }
-    __syncthreads();
-    // A <- T symmetrised (rounding); V already holds V0
-    for (size_t e = tid; e < nn; e += NTH) {
-      const int i = (int)(e / n), j = (int)(e % n);
-      if (i < j) {
-        const double a = 0.5 * (T[(size_t)i * n + j] + T[(size_t)j * n + i]);
-        A[(size_t)i * n + j] = a;
-        A[(size_t)j * n + i] = a;
-      } else if (i == j) {
-        A[e] = T[e];
-      }
+    return;
+  }
+  for (size_t e = tid; e < nn; e += NTH) V[e] = V0[e];
+  __syncthreads();
+  // A <- T V0
+  for (size_t e = tid; e < nn; e += NTH) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    double t = 0.0;
+    for (int k = 0; k < n; ++k) t += T[(size_t)i * n + k] * V[(size_t)k * n + j];
+    A[e] = t;
+  }
+  __syncthreads();
+  // T <- V0^T A
+  for (size_t e = tid; e < nn; e += NTH) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    double t = 0.0;
+    for (int k = 0; k < n; ++k) t += V[(size_t)k * n + i] * A[(size_t)k * n + j];
+    T[e] = t;
+  }
+  __syncthreads();
+  // A <- T symmetrised (rounding); V already holds V0
+  for (size_t e = tid; e < nn; e += NTH) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    if (i < j) {
+      const double a = 0.5 * (T[(size_t)i * n + j] + T[(size_t)j * n + i]);
+      A[(size_t)i * n + j] = a;
+      A[(size_t)j * n + i] = a;
+    } else if (i == j) {
+      A[e] = T[e];
     }
   }
   __syncthreads();
+}
+
+// Cyclic Jacobi sweeps on A (symmetric, n x n) accumulating V <- V J; returns the sweeps run.
+template <int NTH>
+__device__ int jacobi_sweeps(double* A, double* V, int n, int max_sweeps, double tol, JacobiShared& sh) {
+  const int tid = threadIdx.x;
+  const size_t nn = (size_t)n * n;
   const int m = n + (n & 1);  // players incl. a dummy when n is odd
   const int npair = m / 2;
   const int nA = npair * npair, nV = n * npair;
@@ -353,25 +350,25 @@ __global__ __launch_bounds__(NTH) void jacobi_eigh_kernel(const double* __restri
     }
     off = warp_sum(off);
     tot = warp_sum(tot);
-    if ((tid & 63) == 0) red[tid >> 6] = off;
+    if ((tid & 63) == 0) sh.red[tid >> 6] = off;
     __syncthreads();
     if (tid == 0) {
       double o = 0.0;
-      for (int i = 0; i < NTH / 64; ++i) o += red[i];
-      red[0] = o;
+      for (int i = 0; i < NTH / 64; ++i) o += sh.red[i];
+      sh.red[0] = o;
     }
     __syncthreads();
-    const double offs = red[0];
+    const double offs = sh.red[0];
     __syncthreads();
-    if ((tid & 63) == 0) red[tid >> 6] = tot;
+    if ((tid & 63) == 0) sh.red[tid >> 6] = tot;
     __syncthreads();
     if (tid == 0) {
       double t = 0.0;
-      for (int i = 0; i < NTH / 64; ++i) t += red[i];
-      done = (offs <= tol * tol * t) || sweep == max_sweeps;
+      for (int i = 0; i < NTH / 64; ++i) t += sh.red[i];
+      sh.done = (offs <= tol * tol * t) || sweep == max_sweeps;
     }
     __syncthreads();
-    if (done) break;
+    if (sh.done) break;
     for (int r = 0; r < m - 1; ++r) {
       // rotation parameters
       for (int k = tid; k < npair; k += NTH) {
@@ -388,16 +385,16 @@ __global__ __launch_bounds__(NTH) void jacobi_eigh_kernel(const double* __restri
             s = t * c;
           }
         }
-        cs_c[k] = c; cs_s[k] = s; pp[k] = p; qq[k] = q;
+        sh.cs_c[k] = c; sh.cs_s[k] = s; sh.pp[k] = p; sh.qq[k] = q;
       }
       __syncthreads();
       for (int e = tid; e < nA + nV; e += NTH) {
         if (e < nA) {
           // 2x2 block: rows of pair k (row rotation first), columns of pair l
           const int k = e / npair, l = e % npair;
-          const int pk = pp[k], qk = qq[k], pl = pp[l], ql = qq[l];
+          const int pk = sh.pp[k], qk = sh.qq[k], pl = sh.pp[l], ql = sh.qq[l];
           const bool rk = qk < n, rl = ql < n;   // false: the dummy player, no rotation
-          const double ck = cs_c[k], sk = cs_s[k], cl = cs_c[l], sl = cs_s[l];
+          const double ck = sh.cs_c[k], sk = sh.cs_s[k], cl = sh.cs_c[l], sl = sh.cs_s[l];
           double r_pp = A[(size_t)pk * n + pl];
           double r_pq = rl ? A[(size_t)pk * n + ql] : 0.0;
           double r_qp = rk ? A[(size_t)qk * n + pl] : 0.0;
@@ -425,9 +422,9 @@ __global__ __launch_bounds__(NTH) void jacobi_eigh_kernel(const double* __restri
           // V <- V J: row i, pair l
           const int ev = e - nA;
           const int l = ev / n, i = ev % n;
-          const int pl = pp[l], ql = qq[l];
+          const int pl = sh.pp[l], ql = sh.qq[l];
           if (ql >= n) continue;
-          const double c = cs_c[l], s = cs_s[l];
+          const double c = sh.cs_c[l], s = sh.cs_s[l];
           const double vip = V[(size_t)i * n + pl], viq = V[(size_t)i * n + ql];
           V[(size_t)i * n + pl] = c * vip - s * viq;
           V[(size_t)i * n + ql] = s * vip + c * viq;
@@ -436,10 +433,15 @@ __global__ __launch_bounds__(NTH) void jacobi_eigh_kernel(const double* __restri
       __syncthreads();
     }
   }
-  // ascending order: rank of each diagonal element (ties by index)
-  double* w = w_out + (size_t)b * n;
-  double* Vo = V_out + (size_t)b * nn;
-  for (int i = tid; i < n; i += NTH) {
+  return sweep;
+}
+
+// Ascending eigenvalues (rank of each diagonal element of A, ties by index) to w and the
+// matching columns of V to Vo (either may live anywhere).
+template <int NTH>
+__device__ void jacobi_sort(const double* A, const double* V, int n, double* __restrict__ w,
+                            double* __restrict__ Vo) {
+  for (int i = threadIdx.x; i < n; i += NTH) {
     const double di = A[(size_t)i * n + i];
     int rank = 0;
     for (int j = 0; j < n; ++j) {
@@ -449,7 +451,123 @@ __global__ __launch_bounds__(NTH) void jacobi_eigh_kernel(const double* __restri
     w[rank] = di;
     for (int row = 0; row < n; ++row) Vo[(size_t)row * n + rank] = V[(size_t)row * n + i];
   }
-  if (tid == 0 && sweeps_out) sweeps_out[b] = sweep;
+}
+
+template <bool SMEM, int NTH>
+__global__ __launch_bounds__(NTH) void jacobi_eigh_kernel(const double* __restrict__ Ain, int n,
+                                                          const double* __restrict__ V0in,
+                                                          double* __restrict__ w_out,
+                                                          double* __restrict__ V_out,
+                                                          double* __restrict__ gws, int max_sweeps,
+                                                          double tol, int* __restrict__ sweeps_out) {
+  extern __shared__ double smem[];
+  __shared__ JacobiShared sh;
+  const int b = blockIdx.x;
+  const size_t nn = (size_t)n * n;
+  double* A = SMEM ? smem : gws + (size_t)b * 3 * nn;
+  double* V = A + nn;
+  double* T = V + nn;
+  const double* Asrc = Ain + (size_t)b * nn;
+  for (size_t e = threadIdx.x; e < nn; e += NTH) T[e] = Asrc[e];
+  __syncthreads();
+  jacobi_start<NTH>(A, V, T, n, V0in == nullptr ? nullptr : V0in + (size_t)b * nn);
+  const int sweep = jacobi_sweeps<NTH>(A, V, n, max_sweeps, tol, sh);
+  jacobi_sort<NTH>(A, V, n, w_out + (size_t)b * n, V_out + (size_t)b * nn);
+  if (threadIdx.x == 0 && sweeps_out) sweeps_out[b] = sweep;
+}
+
+// One generalised-eigenproblem step of the SCF loop per spin (forward only):
+//   F = sum_i x_{1+i} F_i                           (DIIS extrapolation, evaluate.py:1198-1205)
+//   C' = (Linv F Linv^T + (Linv F Linv^T)^T) / 2    (safe_general_eigh, eigenproblem.py:110-129)
+//   C' = V diag(w) V^T (Jacobi, warm-started from V0), C = Linv^T V
+//   D = C diag(occ) C^T                             (make_rdm1 with aufbau occupations)
+// focks (m, nspin, n, n), x (nspin, m+1).  Buffers A, V, T, L (n x n each) in LDS when 4 n^2
+// doubles fit (SMEM), otherwise in the workspace.
+template <bool SMEM, int NTH>
+__global__ __launch_bounds__(NTH) void scf_step_kernel(int n, int m, int nspin, const double* __restrict__ focks,
+                                                       const double* __restrict__ x,
+                                                       const double* __restrict__ Linv,
+                                                       const double* __restrict__ V0in,
+                                                       const double* __restrict__ occ, double* __restrict__ F_out,
+                                                       double* __restrict__ w_out, double* __restrict__ V_out,
+                                                       double* __restrict__ C_out, double* __restrict__ D_out,
+                                                       double* __restrict__ gws, int max_sweeps, double tol,
+                                                       int* __restrict__ sweeps_out) {
+  extern __shared__ double smem[];
+  __shared__ JacobiShared sh;
+  const int s = blockIdx.x, tid = threadIdx.x;
+  const size_t nn = (size_t)n * n;
+  double* A = SMEM ? smem : gws + (size_t)s * 4 * nn;
+  double* V = A + nn;
+  double* T = V + nn;
+  double* L = T + nn;
+  const double* xs = x + (size_t)s * (m + 1) + 1;
+  for (size_t e = tid; e < nn; e += NTH) {
+    // the m history entries of this element are loaded before any is summed (m <= 16 at a
+    // time), so their latencies overlap
+    double f = 0.0;
+    for (int i0 = 0; i0 < m; i0 += 16) {
+      double fv[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        fv[u] = i0 + u < m ? focks[((size_t)(i0 + u) * nspin + s) * nn + e] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (i0 + u < m) f += xs[i0 + u] * fv[u];
+    }
+    T[e] = f;
+    F_out[(size_t)s * nn + e] = f;
+    L[e] = Linv[e];
+  }
+  __syncthreads();
+  // A <- L F
+  for (size_t e = tid; e < nn; e += NTH) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    double t = 0.0;
+    for (int k = 0; k < n; ++k) t += L[(size_t)i * n + k] * T[(size_t)k * n + j];
+    A[e] = t;
+  }
+  __syncthreads();
+  // V <- A L^T
+  for (size_t e = tid; e < nn; e += NTH) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    double t = 0.0;
+    for (int k = 0; k < n; ++k) t += A[(size_t)i * n + k] * L[(size_t)j * n + k];
+    V[e] = t;
+  }
+  __syncthreads();
+  // T <- (V + V^T) / 2
+  for (size_t e = tid; e < nn; e += NTH) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    T[e] = 0.5 * (V[e] + V[(size_t)j * n + i]);
+  }
+  __syncthreads();
+  jacobi_start<NTH>(A, V, T, n, V0in == nullptr ? nullptr : V0in + (size_t)s * nn);
+  const int sweep = jacobi_sweeps<NTH>(A, V, n, max_sweeps, tol, sh);
+  // sorted eigenpairs: w, V to the outputs and the sorted V to T
+  jacobi_sort<NTH>(A, V, n, w_out + (size_t)s * n, T);
+  __syncthreads();
+  for (size_t e = tid; e < nn; e += NTH) V_out[(size_t)s * nn + e] = T[e];
+  // A <- L^T T  (C = Linv^T V)
+  for (size_t e = tid; e < nn; e += NTH) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    double t = 0.0;
+    for (int k = 0; k < n; ++k) t += L[(size_t)k * n + i] * T[(size_t)k * n + j];
+    A[e] = t;
+    C_out[(size_t)s * nn + e] = t;
+  }
+  __syncthreads();
+  // T <- C diag(occ), then D[i][k] = sum_j T[i][j] C[k][j]
+  const double* os = occ + (size_t)s * n;
+  for (size_t e = tid; e < nn; e += NTH) T[e] = A[e] * os[e % n];
+  __syncthreads();
+  for (size_t e = tid; e < nn; e += NTH) {
+    const int i = (int)(e / n), k = (int)(e % n);
+    double t = 0.0;
+    for (int j = 0; j < n; ++j) t += T[(size_t)i * n + j] * A[(size_t)k * n + j];
+    D_out[(size_t)s * nn + e] = t;
+  }
+  if (tid == 0 && sweeps_out) sweeps_out[s] = sweep;
 }
 
 // ------------------------------------------------------------- DIIS solve ----------
@@ -681,6 +799,34 @@ int gdx_eigh_batched(void* stream, int32_t nbatch, int32_t n, const double* A, c
     GDX_REQUIRE(workspace, "gdx_eigh_batched: workspace required for n > 52");
     GDX_LAUNCH("jacobi_eigh", (jacobi_eigh_kernel<false, EIG_THREADS_GLOBAL>), dim3(nbatch),
                dim3(EIG_THREADS_GLOBAL), 0, st, A, n, V0, w, V, (double*)workspace, max_sweeps, tol, sweeps);
+  }
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+int gdx_scf_step_workspace_size(int32_t nspin, int32_t n, size_t* bytes) {
+  GDX_REQUIRE(nspin > 0 && n > 0 && bytes, "gdx_scf_step_workspace_size: bad arguments");
+  *bytes = (size_t)nspin * 4 * n * n * sizeof(double);
+  return GDX_OK;
+}
+
+int gdx_scf_step(void* stream, int32_t nspin, int32_t n, int32_t m, const double* focks, const double* x,
+                 const double* Linv, const double* V0, const double* occ, double* F, double* w, double* V,
+                 double* C, double* rdm1, void* workspace, int32_t max_sweeps, double tol, int32_t* sweeps) {
+  GDX_REQUIRE(nspin > 0 && nspin <= 64 && n > 0 && n <= 1022 && m > 0 && focks && x && Linv && occ && F && w && V &&
+                  C && rdm1,
+              "gdx_scf_step: bad arguments");
+  GDX_REQUIRE(V0 == nullptr || (V0 != V && V0 != C && V0 != rdm1), "gdx_scf_step: V0 must not alias an output");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t smem = (size_t)4 * n * n * sizeof(double);
+  if (smem <= EIG_SMEM_MAX) {
+    GDX_LAUNCH("scf_step", (scf_step_kernel<true, EIG_THREADS_SMEM>), dim3(nspin), dim3(EIG_THREADS_SMEM), smem, st,
+               n, m, nspin, focks, x, Linv, V0, occ, F, w, V, C, rdm1, (double*)workspace, max_sweeps, tol, sweeps);
+  } else {
+    GDX_REQUIRE(workspace, "gdx_scf_step: workspace required for n > 45");
+    GDX_LAUNCH("scf_step", (scf_step_kernel<false, EIG_THREADS_GLOBAL>), dim3(nspin), dim3(EIG_THREADS_GLOBAL), 0,
+               st, n, m, nspin, focks, x, Linv, V0, occ, F, w, V, C, rdm1, (double*)workspace, max_sweeps, tol,
+               sweeps);
   }
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;

@@ -78,6 +78,31 @@ class EighFn(torch.autograd.Function):
         return out.reshape(V.shape), None
 
 
+def scf_step(focks: torch.Tensor, x: torch.Tensor, Linv: torch.Tensor, V0: Optional[torch.Tensor],
+             occ: torch.Tensor):
+    """Forward-only diagonalisation step of the SCF loop in one device call (gdx_scf_step):
+    F = sum_i x[:, 1+i] focks[i], the eigenproblem of sym(Linv F Linv^T) warm-started from V0,
+    C = Linv^T V and the density C diag(occ) C^T.  Returns (F, w, C, V, rdm1)."""
+    focks = focks.detach().contiguous()
+    m, ns, n = focks.shape[0], focks.shape[1], focks.shape[-1]
+    dev = focks.device
+    out = lambda *shape: torch.empty(shape, dtype=torch.float64, device=dev)  # noqa: E731
+    F, w, V, C, D = out(ns, n, n), out(ns, n), out(ns, n, n), out(ns, n, n), out(ns, n, n)
+    V0c = None if V0 is None else V0.detach().reshape(ns, n, n).contiguous()
+    nbytes = ctypes.c_size_t(0)
+    lib = _lib.load()
+    _lib.check(lib.gdx_scf_step_workspace_size(ns, n, ctypes.byref(nbytes)), "gdx_scf_step_workspace_size")
+    ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=dev)
+    xc = x.detach().contiguous()
+    Lc = Linv.detach().contiguous()
+    oc = occ.detach().contiguous()
+    _lib.check(lib.gdx_scf_step(_stream(focks), ns, n, m, focks.data_ptr(), xc.data_ptr(), Lc.data_ptr(),
+                                0 if V0c is None else V0c.data_ptr(), oc.data_ptr(), F.data_ptr(), w.data_ptr(),
+                                V.data_ptr(), C.data_ptr(), D.data_ptr(), ws.data_ptr(), JACOBI_MAX_SWEEPS,
+                                JACOBI_TOL, 0), "gdx_scf_step")
+    return F, w, C, V, D
+
+
 def orthogonal_eigh(A: torch.Tensor, Linv: torch.Tensor, V0: Optional[torch.Tensor] = None):
     """(w, C, V): eigh of Linv A Linv^T = V diag(w) V^T and C = Linv^T V; V0 warm-starts."""
     C = Linv @ A @ Linv.transpose(-1, -2)

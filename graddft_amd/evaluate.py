@@ -22,7 +22,7 @@ from typing import Callable, List, Optional
 import torch
 
 from . import _lib
-from .eigenproblem import cholesky_inv, orthogonal_eigh
+from .eigenproblem import cholesky_inv, orthogonal_eigh, scf_step
 from .functional import Functional
 from .molecule import Molecule
 from .train import energy_predictor
@@ -187,13 +187,21 @@ def diff_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp
         data = diis.zeros(molecule.nao, S.device)
         occ = _aufbau_occupations(molecule)
         V = None
+        # nothing recorded for reverse mode: extrapolation, eigenproblem and density in one call
+        fused = not (torch.is_grad_enabled() and (
+            (params is not None and params.requires_grad) or molecule.rdm1.requires_grad))
         for cycle in range(cycles):
             predictor = compute_energy64 if cycle >= cycles - refine_cycles else compute_energy
-            F, data = diis.run(molecule.rdm1, molecule.fock, predicted_e, data, cycle)
-            # safe_fock_solver, warm-started from the previous cycle's orthonormal eigenvectors
-            mo_energy, mo_coeff, V = orthogonal_eigh(F, Linv, V)
-            # eigenvalues ascend, so the aufbau occupation of get_occ is the first nelec_s
-            rdm1 = torch.einsum("sij,sj,skj->sik", mo_coeff, occ, mo_coeff)
+            if fused:
+                data = diis.update(molecule.rdm1, molecule.fock, predicted_e, data, cycle)
+                _, x = _diis_coefficients(data[3], cycle)
+                F, mo_energy, mo_coeff, V, rdm1 = scf_step(data[1], x, Linv, V, occ)
+            else:
+                F, data = diis.run(molecule.rdm1, molecule.fock, predicted_e, data, cycle)
+                # safe_fock_solver, warm-started from the previous cycle's orthonormal eigenvectors
+                mo_energy, mo_coeff, V = orthogonal_eigh(F, Linv, V)
+                # eigenvalues ascend, so the aufbau occupation of get_occ is the first nelec_s
+                rdm1 = torch.einsum("sij,sj,skj->sik", mo_coeff, occ, mo_coeff)
             molecule = molecule.replace(fock=F, mo_coeff=mo_coeff, mo_energy=mo_energy, mo_occ=occ,
                                         rdm1=rdm1, restricted=molecule.restricted and functional.spin_symmetric)
             predicted_e, fock = predictor(params, molecule)

@@ -162,6 +162,19 @@ int gdx_eigh_batched(void* stream, int32_t nbatch, int32_t n, const double* A, c
                      double* w, double* V,
                      void* workspace, int32_t max_sweeps, double tol, int32_t* sweeps);
 
+/* One SCF diagonalisation step per spin, forward only (the body of diff_scf_loop's cycle
+ * after the DIIS coefficients, grad_dft/evaluate.py:989-1005 with cdiis_minimize's
+ * extrapolation :1198-1205, safe_fock_solver / safe_general_eigh utils/eigenproblem.py:110-149
+ * and make_rdm1 molecule.py:816-889): F = sum_i x[s][1+i] focks[i][s] (focks (m, nspin, n, n),
+ * x (nspin, m+1) from gdx_diis_coefficients), C' = sym(Linv F Linv^T) = V diag(w) V^T
+ * (Jacobi as gdx_eigh_batched, warm-started from V0 when given), C = Linv^T V and
+ * rdm1 = C diag(occ) C^T with occ (nspin, n).  Outputs F, w, V (orthonormal eigenvectors, the
+ * next call's V0), C, rdm1.  workspace: gdx_scf_step_workspace_size bytes (used when n > 45). */
+int gdx_scf_step_workspace_size(int32_t nspin, int32_t n, size_t* bytes);
+int gdx_scf_step(void* stream, int32_t nspin, int32_t n, int32_t m, const double* focks, const double* x,
+                 const double* Linv, const double* V0, const double* occ, double* F, double* w, double* V,
+                 double* C, double* rdm1, void* workspace, int32_t max_sweeps, double tol, int32_t* sweeps);
+
 /* x_s = B_s^-1 e_0 for nspin bordered DIIS matrices B (nspin, m1, m1), m1 <= 16:
  * JittableDiis.cdiis_minimize, grad_dft/evaluate.py:1198-1205. */
 int gdx_diis_solve(void* stream, int32_t nspin, int32_t m1, const double* B, double* x);

@@ -170,6 +170,49 @@ def test_diis_run_matches_oracle_with_gradients():
         assert np.abs(a - b).max() <= 1e-8 * max(1.0, np.abs(b).max())
 
 
+@pytest.mark.parametrize("n", [43, 50])
+def test_scf_step_matches_unfused_path(n):
+    """gdx_scf_step (extrapolation, eigenproblem, back-transform, density in one call; n = 43
+    in LDS, n = 50 in the workspace) against the separate path: einsum extrapolation,
+    orthogonal_eigh, einsum density -- cold and warm-started."""
+    _say(f"case: scf_step fused vs unfused, n={n}")
+    from graddft_amd.eigenproblem import orthogonal_eigh, scf_step
+    rng = np.random.default_rng(n)
+    m = 10
+    dev = "cuda"
+    X = rng.normal(size=(n, n))
+    S = X @ X.T / n + np.eye(n)
+    L = np.linalg.cholesky(S)
+    Linv = torch.tensor(np.linalg.inv(L), device=dev)
+    F = rng.normal(size=(m, 2, n, n))
+    focks = torch.tensor(F + F.transpose(0, 1, 3, 2), device=dev)
+    x = torch.tensor(rng.normal(size=(2, m + 1)), device=dev)
+    occ = torch.zeros((2, n), dtype=torch.float64, device=dev)
+    occ[0, :7] = 1.0
+    occ[1, :6] = 1.0
+    Fr = torch.einsum("si,isjk->sjk", x[:, 1:], focks)
+    w0, C0, V0 = orthogonal_eigh(Fr, Linv, None)
+    D0 = torch.einsum("sij,sj,skj->sik", C0, occ, C0)
+    F1, w1, C1, V1, D1 = scf_step(focks, x, Linv, None, occ)
+    tol = 1e-10 * max(1.0, float(w0.abs().max()))
+    assert (F1 - Fr).abs().max().item() <= 1e-12 * max(1.0, float(Fr.abs().max()))
+    assert (w1 - w0).abs().max().item() <= tol
+    assert (D1 - D0).abs().max().item() <= 1e-9
+    # eigenvectors agree up to sign; V orthonormal
+    sgn = torch.sign((C1 * C0).sum(dim=1, keepdim=True))
+    assert (C1 - sgn * C0).abs().max().item() <= 1e-8
+    eye = torch.eye(n, dtype=torch.float64, device=dev)
+    assert (V1.transpose(1, 2) @ V1 - eye).abs().max().item() <= 1e-12
+    # warm start from a perturbed problem's eigenvectors
+    x2 = x + 1e-3 * torch.tensor(rng.normal(size=(2, m + 1)), device=dev)
+    Fr2 = torch.einsum("si,isjk->sjk", x2[:, 1:], focks)
+    w2, C2, _ = orthogonal_eigh(Fr2, Linv, V1)
+    _, w3, C3, _, D3 = scf_step(focks, x2, Linv, V1, occ)
+    assert (w3 - w2).abs().max().item() <= tol
+    D2 = torch.einsum("sij,sj,skj->sik", C2, occ, C2)
+    assert (D3 - D2).abs().max().item() <= 1e-9
+
+
 def _oracle_scf_inputs(mol):
     from oracle.ao import eval_ao
     from oracle.scf import SCFInputs, exchange_potential
