@@ -32,7 +32,11 @@ template <> struct MfmaT<float> {
   static __device__ __forceinline__ int row(int lane, int q) { return mfma32_row(lane, q); }
 };
 
-enum { EPI_STORE = 0, EPI_BIAS_ELU = 1, EPI_ELU_BWD = 2 };
+// EPI_ELU_BWD_W0: EPI_ELU_BWD whose result G also feeds the first-layer weight gradient:
+// per wave, the MFMA sums sum_r G[r][f] X16[r][c] over its 64 rows into
+// w0part[slice][c][f] (c < ncp, slice = 2 (row tile) + row half), in the C layout, so G is
+// contracted over rows without another pass over it.
+enum { EPI_STORE = 0, EPI_BIAS_ELU = 1, EPI_ELU_BWD = 2, EPI_ELU_BWD_W0 = 3 };
 
 __device__ __forceinline__ double elu_d(double x) { return x > 0.0 ? x : expm1(x); }
 __device__ __forceinline__ float elu_f(float x) { return x > 0.0f ? x : expm1f(x); }
@@ -74,10 +78,11 @@ __device__ __forceinline__ void split_range(int y, int kchunk, int kseg, int sps
 }
 
 template <typename T, bool TA, bool TB, int EPI>
-__global__ __launch_bounds__(256, sizeof(T) == 4 ? (EPI == EPI_ELU_BWD ? 2 : 3) : 1) void gemm_tile_kernel(
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? (EPI >= EPI_ELU_BWD ? 2 : 3) : 1) void gemm_tile_kernel(
     int M, int N, const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb,
     T* __restrict__ C, int ldc, const T* __restrict__ bias, const T* __restrict__ Y, int ldy,
-    int kchunk, T* __restrict__ colsum, int kseg, int sps) {
+    int kchunk, T* __restrict__ colsum, int kseg, int sps, const T* __restrict__ X16 = nullptr,
+    double* __restrict__ w0part = nullptr, int ncp = 0) {
   typedef typename MfmaT<T>::acc_t acc_t;
   constexpr int GT_K = GemmK<T>::value;
   constexpr int VEC = 16 / sizeof(T);                // elements per 16-byte load
@@ -182,10 +187,14 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? (EPI == EPI_ELU_BWD ? 2 : 3) 
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = (acc_t){0, 0, 0, 0};
   T csum = 0;
-  // EPI_ELU_BWD, FP32: this lane's 64 elements of Y are fetched two stages before the
+  constexpr bool ELUB = EPI == EPI_ELU_BWD || EPI == EPI_ELU_BWD_W0, W0 = EPI == EPI_ELU_BWD_W0;
+  acc_t wacc[W0 ? 4 : 1];
+#pragma unroll
+  for (int j = 0; j < (W0 ? 4 : 1); ++j) wacc[j] = (acc_t){0, 0, 0, 0};
+  // ELU backward, FP32: this lane's 64 elements of Y are fetched two stages before the
   // epilogue, so their latency overlaps the last MFMAs instead of following them (FP64 keeps
   // the loads in the epilogue: 64 more doubles a lane would spill)
-  constexpr bool YPF = EPI == EPI_ELU_BWD && sizeof(T) == 4;
+  constexpr bool YPF = ELUB && sizeof(T) == 4;
   T yv[YPF ? 4 : 1][YPF ? 4 : 1][4];
   auto yload = [&]() {
     if constexpr (YPF) {
@@ -246,13 +255,31 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? (EPI == EPI_ELU_BWD ? 2 : 3) 
         const int row = m0 + wm + i * 16 + MfmaT<T>::row(lane, q);
         T v = acc[i][j][q];
         if (EPI == EPI_BIAS_ELU) v = elu_any(v + bias[col]);
-        if constexpr (EPI == EPI_ELU_BWD) {
+        if constexpr (ELUB) {
           const T y = YPF ? yv[YPF ? i : 0][YPF ? j : 0][q] : Y[(size_t)row * ldy + col];
           v *= (y > (T)0 ? (T)1 : y + (T)1);
+        }
+        if constexpr (W0) {
+          // A = G^T (f = lane % 16, row slot lane / 16), B = X16 rows of the same slots
+          const T xr = X16[(size_t)row * 16 + (lane & 15)];
+          wacc[W0 ? j : 0] = MfmaT<T>::op(v, xr, wacc[W0 ? j : 0]);
         }
         Cz[(size_t)row * ldc + col] = v;
       }
     }
+  if constexpr (W0) {
+    const int slice = (m0 / GT_M) * 2 + (wm >> 6);
+    const int c = lane & 15;
+    if (c < ncp) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int f = n0 + wn + j * 16 + MfmaT<T>::row(lane, q);
+          w0part[((size_t)slice * ncp + c) * N + f] = (double)wacc[W0 ? j : 0][q];
+        }
+    }
+  }
 }
 
 }  // namespace gdx

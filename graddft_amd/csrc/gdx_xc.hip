@@ -780,6 +780,7 @@ Layout make_layout(const gdx_xc_desc& d, bool hvp = false) {
     wbytes = (size_t)L.nsplit * h * h * es;
     wbytes = std::max(wbytes, (size_t)L.nwblk * 16 * h * 8);
     wbytes = std::max(wbytes, (size_t)((R + RW_ROWS - 1) / RW_ROWS) * 7 * h * 8);
+    wbytes = std::max(wbytes, (size_t)(2 * (R / GT_M)) * (d.mlp_nc + 1) * h * 8);   // EPI_ELU_BWD_W0 slices
   }
   L.wpart = take(wbytes);
   L.cpart = take(neural ? (size_t)L.nsplit * h * es : 0);
@@ -840,7 +841,8 @@ void launch_any(hipStream_t st, const char* name, int M, int N, int K, const T* 
 template <typename T, bool TA, bool TB, int EPI>
 void launch_tile(hipStream_t st, const char* name, int M, int N, int nsplit, int kchunk, const T* A, int lda,
                  const T* B, int ldb, T* C, int ldc, const T* bias, const T* Y, int ldy, T* colsum,
-                 bool bf16 = false, int kseg = 0, int sps = 0) {
+                 bool bf16 = false, int kseg = 0, int sps = 0, const T* X16 = nullptr, double* w0part = nullptr,
+                 int ncp = 0) {
   dim3 grid((M / GT_M) * (N / GT_N), nsplit);
   if (sps <= 0) sps = nsplit;
   if (kseg <= 0) kseg = nsplit * kchunk;
@@ -857,7 +859,7 @@ void launch_tile(hipStream_t st, const char* name, int M, int N, int nsplit, int
     }
   }
   GDX_LAUNCH(name, (gemm_tile_kernel<T, TA, TB, EPI>), grid, dim3(NT), 0, st, M, N, A, lda, B, ldb, C, ldc,
-             bias, Y, ldy, kchunk, colsum, kseg, sps);
+             bias, Y, ldy, kchunk, colsum, kseg, sps, X16, w0part, ncp);
 }
 
 // parameter offsets (elements) of W_l and b_l in the flat parameter vector
@@ -1083,6 +1085,7 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
     }
     T* Gcur = G0;
     T* Gnext = G1;
+    const bool w0_fused = d_params && tiles && rows_path(h) && !mlp_bf16(d) && nl >= 2;
     for (int l = nl - 1; l >= 1; --l) {
       const T* Hin = H + (size_t)(l - 1) * L.hstride;  // input activations of layer l
       const T* Wl = plp + po.w[l];
@@ -1099,8 +1102,13 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
         reduce_slices<T>(c, wp, L.nsplit, (long)h * h, (long)h * h, d_params + po.w[l]);
         reduce_slices<T>(c, cp, L.nsplit, (long)h, (long)h, d_params + po.b[l]);
       }
-      // delta for layer l's input: (Gcur W_l^T) * elu'(Hin)
-      if (tiles)
+      // delta for layer l's input: (Gcur W_l^T) * elu'(Hin); for l = 1 that delta's first-layer
+      // weight-gradient partials come out of the same epilogue (w0_fused)
+      if (tiles && l == 1 && w0_fused)
+        launch_tile<T, false, true, EPI_ELU_BWD_W0>(st, tname<T>("gemm_f64_bwd_data", "gemm_f32_bwd_data"), Np, h,
+                                                    1, h, Gcur, h, Wl, h, Gnext, h, nullptr, Hin, h, nullptr, false,
+                                                    0, 0, c.P<T>(L.X), c.P<double>(L.wpart), d.mlp_nc + 1);
+      else if (tiles)
         launch_tile<T, false, true, EPI_ELU_BWD>(st, tname<T>("gemm_f64_bwd_data", "gemm_f32_bwd_data"), Np, h, 1,
                                                  h, Gcur, h, Wl, h, Gnext, h, nullptr, Hin, h, nullptr, mlp_bf16(d));
       else
@@ -1114,7 +1122,9 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
     if (rows_path(h)) {
       GDX_LAUNCH("layer0_bwd", (rowdot_mfma_kernel<T, false>), dim3((Np + RD_ROWS - 1) / RD_ROWS), dim3(NT), 0, st, Np, h, Gcur, W016,
                  c.P<T>(L.DX), d.mlp_nc, nullptr, nullptr, nullptr, 0, nullptr, 0);
-      if (d_params) {
+      if (w0_fused) {
+        reduce_slices<double>(c, c.P<double>(L.wpart), 2 * (Np / GT_M), l0len, l0len, d_params + po.w[0]);
+      } else if (d_params) {
         GDX_LAUNCH("wgrad0", wgrad0_rows_kernel<T>, dim3(rwblk), dim3(NT), 0, st, Gcur, h, Np, c.P<T>(L.X), d.mlp_nc,
                    c.P<double>(L.wpart));
         reduce_slices<double>(c, c.P<double>(L.wpart), rwblk, l0len, l0len, d_params + po.w[0]);
