@@ -187,13 +187,14 @@ __global__ __launch_bounds__(256) void skinny_wgrad_kernel(
   }
 }
 
+// out[i] = sum over the nsplit slices of part[s * stride + i], 64 consecutive i per block
+// (block bx), 4 slice subgroups combined in LDS in a fixed order.
 template <typename TP>
-__global__ __launch_bounds__(256) void reduce_cols_kernel(const TP* __restrict__ part, int nsplit,
-                                                          long stride, long len,
-                                                          double* __restrict__ out) {
+__device__ __forceinline__ void reduce_cols_block(const TP* __restrict__ part, int nsplit, long stride, long len,
+                                                  double* __restrict__ out, int bx) {
   __shared__ double red[4][64];
   const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const long i = (long)blockIdx.x * 64 + c;
+  const long i = (long)bx * 64 + c;
   double acc = 0.0;
   if (i < len) {
     // 8 independent loads in flight per thread; the sum order stays fixed
@@ -210,6 +211,23 @@ __global__ __launch_bounds__(256) void reduce_cols_kernel(const TP* __restrict__
   red[g][c] = acc;
   __syncthreads();
   if (g == 0 && i < len) out[i] = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+}
+
+template <typename TP>
+__global__ __launch_bounds__(256) void reduce_cols_kernel(const TP* __restrict__ part, int nsplit,
+                                                          long stride, long len,
+                                                          double* __restrict__ out) {
+  reduce_cols_block<TP>(part, nsplit, stride, len, out, blockIdx.x);
+}
+
+// two independent column reductions in one launch: blocks [0, nb0) serve the first
+template <typename TP>
+__global__ __launch_bounds__(256) void reduce_cols2_kernel(const TP* __restrict__ p0, int ns0, long st0, long len0,
+                                                           double* __restrict__ o0, int nb0,
+                                                           const TP* __restrict__ p1, int ns1, long st1, long len1,
+                                                           double* __restrict__ o1) {
+  if ((int)blockIdx.x < nb0) reduce_cols_block<TP>(p0, ns0, st0, len0, o0, blockIdx.x);
+  else reduce_cols_block<TP>(p1, ns1, st1, len1, o1, blockIdx.x - nb0);
 }
 
 }  // namespace gdx

@@ -108,12 +108,12 @@ __global__ void reduce_vxc_kernel(const double* __restrict__ part, int nchunk, i
 // [y per, (y+1) per) of part[s * stride + i]; a block covers 64 consecutive i with 4 slice
 // subgroups combined in LDS (fixed order: deterministic).
 template <typename TP>
-__global__ __launch_bounds__(256) void reduce_stage1_kernel(const TP* __restrict__ part, int nslices, long stride,
-                                                            long len, int per, double* __restrict__ out) {
+__device__ __forceinline__ void reduce_stage1_block(const TP* __restrict__ part, int nslices, long stride, long len,
+                                                    int per, double* __restrict__ out, int bx, int by) {
   __shared__ double red[4][64];
   const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const long i = (long)blockIdx.x * 64 + c;
-  const int s0 = blockIdx.y * per, s1 = min(nslices, s0 + per);
+  const long i = (long)bx * 64 + c;
+  const int s0 = by * per, s1 = min(nslices, s0 + per);
   double acc = 0.0;
   if (i < len) {
     int s = s0 + g;
@@ -128,7 +128,27 @@ __global__ __launch_bounds__(256) void reduce_stage1_kernel(const TP* __restrict
   }
   red[g][c] = acc;
   __syncthreads();
-  if (g == 0 && i < len) out[(long)blockIdx.y * len + i] = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+  if (g == 0 && i < len) out[(long)by * len + i] = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+}
+
+template <typename TP>
+__global__ __launch_bounds__(256) void reduce_stage1_kernel(const TP* __restrict__ part, int nslices, long stride,
+                                                            long len, int per, double* __restrict__ out) {
+  reduce_stage1_block<TP>(part, nslices, stride, len, per, out, blockIdx.x, blockIdx.y);
+}
+
+// the first stage of two independent reductions in one launch (grid.y = the larger group
+// count; blocks past a reduction's own count exit)
+template <typename TP>
+__global__ __launch_bounds__(256) void reduce_stage1_2_kernel(const TP* __restrict__ p0, int ns0, long st0, long len0,
+                                                              int per0, int G0, double* __restrict__ t0, int nb0,
+                                                              const TP* __restrict__ p1, int ns1, long st1, long len1,
+                                                              int per1, int G1, double* __restrict__ t1) {
+  if ((int)blockIdx.x < nb0) {
+    if ((int)blockIdx.y < G0) reduce_stage1_block<TP>(p0, ns0, st0, len0, per0, t0, blockIdx.x, blockIdx.y);
+  } else if ((int)blockIdx.y < G1) {
+    reduce_stage1_block<TP>(p1, ns1, st1, len1, per1, t1, blockIdx.x - nb0, blockIdx.y);
+  }
 }
 
 // ---------------------------------------------------------------- per-point fwd ----
@@ -784,7 +804,8 @@ Layout make_layout(const gdx_xc_desc& d, bool hvp = false) {
   }
   L.wpart = take(wbytes);
   L.cpart = take(neural ? (size_t)L.nsplit * h * es : 0);
-  L.rtmp = take((size_t)RED_GROUPS * std::max(std::max(h * h, 16 * h), 2 * nP * nP) * 8);
+  // two-stage reductions: up to two at once (reduce_slices2: the largest one plus the last layer's bias)
+  L.rtmp = take((size_t)RED_GROUPS * (std::max(std::max(h * h, 16 * h), 2 * nP * nP) + 64) * 8);
   if (hvp) {
     L.Dp2 = take(2 * nP * nP * es);
     L.dpart_t = take(2 * Np * DENS_STRIDE * 8);
@@ -932,6 +953,36 @@ void reduce_slices(const Ctx& c, const TP* part, int nslices, long stride, long 
   GDX_LAUNCH("reduce_cols", reduce_cols_kernel<double>, dim3(nblocks64(len)), dim3(NT), 0, st, tmp, G, len, len, out);
 }
 
+// Two reductions at once (the weight and bias partials of one split-K GEMM, the last layer's
+// weight and bias partials): one launch when both fit one pass -- the second then also runs
+// in one pass -- two when both need two stages, else as two reduce_slices calls.
+template <typename TP>
+void reduce_slices2(const Ctx& c, const TP* p0, int ns0, long st0, long len0, double* o0, const TP* p1, int ns1,
+                    long st1, long len1, double* o1) {
+  hipStream_t st = c.st;
+  auto one_pass = [](int ns, long len) { return ns <= 16 || (ns <= 128 && len >= 16384); };
+  if (one_pass(ns0, len0) && ns1 <= 128) {
+    const int nb0 = (int)nblocks64(len0);
+    GDX_LAUNCH("reduce_cols", reduce_cols2_kernel<TP>, dim3(nb0 + nblocks64(len1)), dim3(NT), 0, st, p0, ns0, st0,
+               len0, o0, nb0, p1, ns1, st1, len1, o1);
+    return;
+  }
+  if (one_pass(ns0, len0) || one_pass(ns1, len1)) {
+    reduce_slices<TP>(c, p0, ns0, st0, len0, o0);
+    reduce_slices<TP>(c, p1, ns1, st1, len1, o1);
+    return;
+  }
+  const int per0 = (ns0 + RED_GROUPS - 1) / RED_GROUPS, G0 = (ns0 + per0 - 1) / per0;
+  const int per1 = (ns1 + RED_GROUPS - 1) / RED_GROUPS, G1 = (ns1 + per1 - 1) / per1;
+  double* t0 = c.P<double>(c.L.rtmp);
+  double* t1 = t0 + (size_t)G0 * len0;
+  const int nb0 = (int)nblocks64(len0);
+  GDX_LAUNCH("reduce_cols", reduce_stage1_2_kernel<TP>, dim3(nb0 + nblocks64(len1), std::max(G0, G1)), dim3(NT), 0,
+             st, p0, ns0, st0, len0, per0, G0, t0, nb0, p1, ns1, st1, len1, per1, G1, t1);
+  GDX_LAUNCH("reduce_cols", reduce_cols2_kernel<double>, dim3(nb0 + nblocks64(len1)), dim3(NT), 0, st, t0, G0, len0,
+             len0, o0, nb0, t1, G1, len1, len1, o1);
+}
+
 // V_s (nsv spins) from the per-point potentials in L.vpt, reduced into d_rdm1 (2, n, n)
 template <typename T>
 void launch_vxc(const Ctx& c, const T* ao, int nsv, double* d_rdm1, bool force_flush = false) {
@@ -1076,12 +1127,13 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
     }
     if (d_params) {
       if (rows_path(h))
-        reduce_slices<double>(c, c.P<double>(L.wpart), rwblk, (long)nd * h, (long)nd * h, c.P<double>(L.dWt16));
+        reduce_slices2<double>(c, c.P<double>(L.wpart), rwblk, (long)nd * h, (long)nd * h, c.P<double>(L.dWt16),
+                               c.P<double>(L.dbpart), L.npt_blocks, (long)nd, (long)nd, d_params + po.b[nl]);
       else
-        reduce_slices<double>(c, c.P<double>(L.wpart), L.nwblk, 16L * h, 16L * h, c.P<double>(L.dWt16));
+        reduce_slices2<double>(c, c.P<double>(L.wpart), L.nwblk, 16L * h, 16L * h, c.P<double>(L.dWt16),
+                               c.P<double>(L.dbpart), L.npt_blocks, (long)nd, (long)nd, d_params + po.b[nl]);
       GDX_LAUNCH("unpack_last", unpack_last_kernel, dim3(nblocks((long)h * nd)), dim3(NT), 0, st,
                  c.P<double>(L.dWt16), h, nd, d_params + po.w[nl]);
-      reduce_slices<double>(c, c.P<double>(L.dbpart), L.npt_blocks, (long)nd, (long)nd, d_params + po.b[nl]);
     }
     T* Gcur = G0;
     T* Gnext = G1;
@@ -1099,8 +1151,8 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
         else
           launch_any<T, 64, 64, true, false, EPI_STORE>(st, "gemm_any", h, h, Np, Hin, h, Gcur, h, wp, h, nullptr,
                                                         nullptr, 0, L.nsplit, L.kchunk, cp, L.kseg, L.sps);
-        reduce_slices<T>(c, wp, L.nsplit, (long)h * h, (long)h * h, d_params + po.w[l]);
-        reduce_slices<T>(c, cp, L.nsplit, (long)h, (long)h, d_params + po.b[l]);
+        reduce_slices2<T>(c, wp, L.nsplit, (long)h * h, (long)h * h, d_params + po.w[l], cp, L.nsplit, (long)h,
+                          (long)h, d_params + po.b[l]);
       }
       // delta for layer l's input: (Gcur W_l^T) * elu'(Hin); for l = 1 that delta's first-layer
       // weight-gradient partials come out of the same epilogue (w0_fused)
