@@ -57,6 +57,8 @@ def kernel_flops(name, npts, nao, spec, nsd):
         "vxc": 2 * 8 * nao * nao * npts,             # phi^T Z + 3 dphi^T u dphi, per spin
         "gemm_fwd": 2 * npts * h * h,
         "gemm_bwd_data": 2 * npts * h * h,
+        # the last one also forms the first-layer weight gradient (7 columns incl. the bias)
+        "gemm_bwd_data_w0": 2 * npts * h * h + 2 * npts * h * (spec.nc + 1),
         "gemm_wgrad": 2 * npts * h * h,
     }
     for suffix in ("_f64", "_f32", "_bf16x6"):

@@ -1157,7 +1157,7 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
       // delta for layer l's input: (Gcur W_l^T) * elu'(Hin); for l = 1 that delta's first-layer
       // weight-gradient partials come out of the same epilogue (w0_fused)
       if (tiles && l == 1 && w0_fused)
-        launch_tile<T, false, true, EPI_ELU_BWD_W0>(st, tname<T>("gemm_f64_bwd_data", "gemm_f32_bwd_data"), Np, h,
+        launch_tile<T, false, true, EPI_ELU_BWD_W0>(st, tname<T>("gemm_f64_bwd_data_w0", "gemm_f32_bwd_data_w0"), Np, h,
                                                     1, h, Gcur, h, Wl, h, Gnext, h, nullptr, Hin, h, nullptr, false,
                                                     0, 0, c.P<T>(L.X), c.P<double>(L.wpart), d.mlp_nc + 1);
       else if (tiles)
