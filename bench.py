@@ -126,7 +126,16 @@ def scf_bench(args):
         torch.cuda.synchronize()
         t3 = time.perf_counter()
         # cycles + 1 energy/Fock evaluations (the initial one plus one per cycle)
+        P = mol.nao * (mol.nao + 1) // 2
+        eri_bytes = 8 * P * (P + 1) // 2
+        ct = kt.result.get("coulomb_tile")
+        coul = None
+        if ct is not None:
+            gbs = eri_bytes * ct[1] / (ct[0] / 1e3) / 1e9
+            coul = {"eri_bytes": eri_bytes, "avg_launch_us": round(ct[0] / ct[1] * 1e3, 2), "GB/s": round(gbs, 1),
+                    "frac_of_8TB/s": round(gbs / 8000.0, 4)}
         out[prec] = {"ms_per_iter": round((t3 - t2) / reps / args.scf_cycles * 1e3, 4),
+                     "coulomb_packed_eri": coul,
                      "ms_per_iter_eager": round((t1 - t0) / args.scf_cycles * 1e3, 4),
                      "energy": float(gres.energy),
                      "graph_vs_eager_abs_dE": abs(float(gres.energy) - float(res.energy)),
@@ -322,6 +331,22 @@ def main():
         vs64 = {"fp64_ms_per_step": round(ms64, 4), "fp64_value": round(N / (ms64 / 1e3), 1),
                 "speedup": round(ms64 / ms, 3), "abs_dE_Ha": abs(E - float(e64)),
                 "rel_err_dE_dD": rel(dD, dD64), "rel_err_dE_dtheta": rel(dth, dth64)}
+    # streaming (HBM-bound) kernels of the step: algorithmic bytes per launch over the
+    # measured launch time, against the 8 TB/s HBM peak
+    Np = int(mol.ao_planes.shape[1])
+    es = 8 if args.precision == "fp64" else 4
+    h, nd, nc = spec.h, spec.nd, spec.nc
+    sbytes = {"layer0": Np * (16 + h) * es,
+              "head_fwd": Np * (h + nd) * es + Np * (4 + 1) * 8,
+              "head_delta": Np * (2 * h + 16) * es,
+              "layer0_bwd": Np * (h + nc) * es}
+    streaming = {}
+    for k, b in sbytes.items():
+        if k in kt.result:
+            tot, cnt = kt.result[k]
+            gbs = b * cnt / (tot / 1e3) / 1e9
+            streaming[k] = {"bytes_per_launch": b, "avg_launch_us": round(tot / cnt * 1e3, 2),
+                            "GB/s": round(gbs, 1), "frac_of_8TB/s": round(gbs / 8000.0, 3)}
     nsv = 2
     w_pt = 8 * n * n * nsd + 8 * n * n * nsv + 3 * spec.flops_per_point()
     step_tflops = w_pt * N / (ms / 1e3) / 1e12
@@ -358,6 +383,7 @@ def main():
             "exc": E,
         },
         "roofline": roof,
+        "streaming": streaming,
         "step_mfma": {"algorithmic_tflops": round(step_tflops, 3), "peak": peak,
                       "frac": round(step_tflops / peak, 4), "flop_per_point": w_pt},
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in

@@ -118,7 +118,21 @@ __global__ __launch_bounds__(256) void rowdot_mfma_kernel(int rows, int K, const
   __shared__ double zs[4][16][17];
   __shared__ double red[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int e = tid; e < 16 * K; e += 256) Bs[e] = Bt[e];
+  {
+    // stage Bt (16 x K, K <= 256) with 16-byte loads, all issued before the LDS stores
+    constexpr int NBV = 16 * 256 / (256 * VEC);
+    vec_t bv[NBV];
+#pragma unroll
+    for (int i = 0; i < NBV; ++i) {
+      const int e = (tid + 256 * i) * VEC;
+      bv[i] = e < 16 * K ? *(const vec_t*)(Bt + e) : (vec_t)(T)0;
+    }
+#pragma unroll
+    for (int i = 0; i < NBV; ++i) {
+      const int e = (tid + 256 * i) * VEC;
+      if (e < 16 * K) *(vec_t*)&Bs[e] = bv[i];
+    }
+  }
   __syncthreads();
   const int nch = K / (4 * VEC);
   const int rl = lane & 15, ks = lane >> 4;

@@ -221,7 +221,23 @@ __global__ __launch_bounds__(NT) void edge_rows_kernel(const T* __restrict__ A, 
   typedef T v8 __attribute__((ext_vector_type(8)));
   __shared__ __align__(16) T Ws[16][256];
   __shared__ __align__(16) T bs[256];
-  for (int e = threadIdx.x; e < kj * h; e += NT) Ws[e / h][e % h] = Wt[e];
+  {
+    // stage the 16 x h weight rows with 16-byte loads, all issued before the LDS stores
+    // (rows >= kj are the buffer's zero padding)
+    constexpr int VEC = 16 / sizeof(T), NBV = 16 * 256 / (NT * VEC);
+    typedef T vec_t __attribute__((ext_vector_type(VEC)));
+    vec_t wv[NBV];
+#pragma unroll
+    for (int i = 0; i < NBV; ++i) {
+      const int e = (threadIdx.x + NT * i) * VEC;
+      wv[i] = e < 16 * h ? *(const vec_t*)(Wt + e) : (vec_t)(T)0;
+    }
+#pragma unroll
+    for (int i = 0; i < NBV; ++i) {
+      const int e = (threadIdx.x + NT * i) * VEC;
+      if (e < 16 * h) *(vec_t*)&Ws[e / h][e % h] = wv[i];
+    }
+  }
   if (MODE == ER_LAYER0)
     for (int e = threadIdx.x; e < h; e += NT) bs[e] = (T)bias[e];
   __syncthreads();
