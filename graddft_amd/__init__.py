@@ -13,11 +13,11 @@ from .interface.builder import molecule_from_geometry
 from .interface.store import load_molecules, save_molecules
 from .train import (energy_predictor, simple_energy_loss, mse_energy_loss, train_kernel, adam,
                     apply_updates)
-from .evaluate import JittableDiis, diff_scf_loop, non_scf_predictor
+from .evaluate import JittableDiis, diff_scf_loop, diff_simple_scf_loop, non_scf_predictor
 from .eigenproblem import safe_eigh_vjp, safe_fock_solver, safe_general_eigh
 
 __all__ = ["Grid", "Molecule", "Functional", "NeuralFunctional", "B3LYPNeuralCorrection", "MLPSpec", "XCPlan",
            "XCEnergyFn", "XCEnergyVxcFn", "B3LYP", "B3LYP_GRID", "molecule_from_geometry",
            "energy_predictor", "non_scf_predictor", "simple_energy_loss", "mse_energy_loss",
-           "train_kernel", "adam", "apply_updates", "JittableDiis", "diff_scf_loop", "safe_eigh_vjp",
+           "train_kernel", "adam", "apply_updates", "JittableDiis", "diff_scf_loop", "diff_simple_scf_loop", "safe_eigh_vjp",
            "safe_fock_solver", "safe_general_eigh", "save_molecules", "load_molecules"]

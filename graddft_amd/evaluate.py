@@ -146,6 +146,37 @@ def non_scf_predictor(functional: Functional, precision: str = "fp64", **kwargs)
     return predictor
 
 
+def diff_simple_scf_loop(functional: Functional, cycles: int = 25, mixing_factor: float = 0.4,
+                         precision: str = "fp64", **kwargs) -> Callable:
+    """``diff_simple_scf_loop`` (evaluate.py:257-352): predict, then per cycle diagonalise the
+    current Fock matrix (safe_fock_solver), aufbau occupations, linear density mixing
+    D = (1 - mixing_factor) D_old + mixing_factor D_new, predict.  Differentiable w.r.t. params
+    like diff_scf_loop (the same predictor and eigensolver Functions); the eigensolver is
+    warm-started from the previous cycle's eigenvectors."""
+    compute_energy = energy_predictor(functional, precision=precision, **kwargs)
+
+    def iterator(params: Optional[torch.Tensor], molecule: Molecule) -> Molecule:
+        Linv = molecule.cache.get("Linv")
+        if Linv is None:
+            _, Linv = cholesky_inv(molecule.s1e)
+            molecule.cache["Linv"] = Linv
+        predicted_e, fock = compute_energy(params, molecule)
+        molecule = molecule.replace(fock=fock, energy=predicted_e)
+        occ = _aufbau_occupations(molecule)
+        V = None
+        for _ in range(cycles):
+            mo_energy, mo_coeff, V = orthogonal_eigh(molecule.fock, Linv, V)
+            new_rdm1 = torch.einsum("sij,sj,skj->sik", mo_coeff, occ, mo_coeff)
+            rdm1 = (1.0 - mixing_factor) * molecule.rdm1 + mixing_factor * new_rdm1
+            molecule = molecule.replace(mo_coeff=mo_coeff, mo_energy=mo_energy, mo_occ=occ, rdm1=rdm1,
+                                        restricted=molecule.restricted and functional.spin_symmetric)
+            predicted_e, fock = compute_energy(params, molecule)
+            molecule = molecule.replace(fock=fock)
+        return molecule.replace(energy=predicted_e)
+
+    return iterator
+
+
 def _aufbau_occupations(molecule: Molecule) -> torch.Tensor:
     """get_occ for ascending eigenvalues: the first N_e,s orbitals of each spin (cached)."""
     occ = molecule.cache.get("aufbau_occ")

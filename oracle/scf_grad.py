@@ -7,6 +7,8 @@ Restates:
   matrix's XC term (``grad_dft/train.py:86-121``) inside ``diff_scf_loop``: with
   E(theta, D) = E_xc, Phi = g E + <dD, dE/dD>, and dPhi/dD, dPhi/dtheta, by double backward
   through ``oracle.xc``;
+* ``diff_simple_scf_energy`` -- the linear-mixing loop of ``evaluate.py:257-352``, same pieces
+  without DIIS;
 * ``diff_scf_energy`` -- the differentiable loop of ``grad_dft/evaluate.py:917-1038`` in
   torch ops: ``predict`` (``train.py:123-216``; V_xc by autograd with create_graph, the
   stale-exchange B3LYP term as in ``oracle.scf.predict``), ``JittableDiis``
@@ -18,7 +20,8 @@ Pinning (``tests/test_oracle_scf_grad.py``): central finite differences of the o
 values -- E_xc along dD for edot; dE/dD and dE/dtheta along dD for the HVP; the final SCF
 energy along a parameter direction for the loop gradient -- and the torch loop's energies
 equal ``oracle.scf.diff_scf_loop``'s (the numpy restatement pinned in
-``tests/test_oracle_scf.py``).
+``tests/test_oracle_scf.py``); the mixing loop reaches the DIIS loop's self-consistent energy
+(the common fixed point) and its parameter gradient matches central differences.
 """
 from __future__ import annotations
 
@@ -185,5 +188,29 @@ def diff_scf_energy(xc_energy: Callable, inp: TorchSCFInputs, rdm1: torch.Tensor
             occ[s, idx[: nelecs[s]]] = 1.0
         C = torch.stack([o[1] for o in outs])
         D = torch.einsum("sij,sj,skj->sik", C, occ, C)
+        e, fock = predict(xc_energy, inp, D)
+    return e
+
+
+def diff_simple_scf_energy(xc_energy: Callable, inp: TorchSCFInputs, rdm1: torch.Tensor, mo_occ: torch.Tensor,
+                           cycles: int, mixing_factor: float = 0.4) -> torch.Tensor:
+    """evaluate.py:257-352 (diff_simple_scf_loop): predict, then per cycle diagonalise the
+    current Fock matrix (safe_fock_solver), aufbau occupations, linear density mixing
+    D = (1 - m) D_old + m D_new, predict; the final predicted energy, on the autograd tape."""
+    n = rdm1.shape[-1]
+    L = torch.linalg.cholesky(inp.s1e)
+    L_inv = torch.linalg.inv(L)
+    e, fock = predict(xc_energy, inp, rdm1)
+    nelecs = [int(round(float(v))) for v in mo_occ.sum(dim=1)]
+    D = rdm1
+    for _ in range(cycles):
+        outs = [safe_general_eigh(fock[s], L_inv) for s in range(2)]
+        occ = torch.zeros((2, n), dtype=DT)
+        for s in range(2):
+            idx = torch.argsort(outs[s][0].detach(), stable=True)
+            occ[s, idx[: nelecs[s]]] = 1.0
+        C = torch.stack([o[1] for o in outs])
+        Dnew = torch.einsum("sij,sj,skj->sik", C, occ, C)
+        D = (1.0 - mixing_factor) * D + mixing_factor * Dnew
         e, fock = predict(xc_energy, inp, D)
     return e

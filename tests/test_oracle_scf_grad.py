@@ -117,3 +117,43 @@ def test_scf_parameter_gradient_matches_finite_differences():
     fd = (Ep - Em) / (2 * h)
     an = float(grad.numpy() @ v)
     assert an == pytest.approx(fd, rel=1e-5, abs=1e-9)
+
+
+def test_simple_mixing_loop_reaches_the_diis_fixed_point():
+    """Linear mixing (evaluate.py:257-352) and DIIS (evaluate.py:917-1038) iterate to the same
+    self-consistent density, so their final energies agree once both have converged; with
+    mixing_factor = 1 the first cycle is the plain diagonalise-and-predict step."""
+    inp, xin, rdm1, occ = _water()
+    ti = G.TorchSCFInputs.from_numpy(inp)
+    xc = lambda d: X.xc_energy_from_rdm1("b3lyp", xin, d)  # noqa: E731
+    D0, occ_t = torch.tensor(rdm1, dtype=DT), torch.tensor(occ, dtype=DT)
+    e_diis = G.diff_scf_energy(xc, ti, D0, occ_t, 25).item()
+    e_mix = G.diff_simple_scf_energy(xc, ti, D0, occ_t, 80, 0.4).item()
+    assert e_mix == pytest.approx(e_diis, abs=1e-8)
+    # one undamped cycle = one DIIS cycle (the first DIIS extrapolation is the plain Fock)
+    e1_mix = G.diff_simple_scf_energy(xc, ti, D0, occ_t, 1, 1.0).item()
+    e1_diis = G.diff_scf_energy(xc, ti, D0, occ_t, 1).item()
+    assert e1_mix == pytest.approx(e1_diis, abs=1e-11)
+
+
+def test_simple_mixing_parameter_gradient_matches_finite_differences():
+    inp, xin, rdm1, occ = _water()
+    spec = X.MLPSpec(h=16, n_hidden=2)
+    theta0 = X.init_mlp_params(spec, 4)
+    ti = G.TorchSCFInputs.from_numpy(inp)
+    ti.K0 = None
+    D0 = torch.tensor(rdm1, dtype=DT)
+    occ_t = torch.tensor(occ, dtype=DT)
+
+    def energy(th):
+        return G.diff_simple_scf_energy(lambda d: X.xc_energy_from_rdm1("neural_b3lyp", xin, d, th, spec), ti, D0,
+                                        occ_t, 4, 0.4)
+
+    th = torch.tensor(theta0, dtype=DT, requires_grad=True)
+    grad, = torch.autograd.grad(energy(th), [th])
+    v = np.random.default_rng(11).normal(size=theta0.shape)
+    v /= np.linalg.norm(v)
+    h = 1e-4
+    fd = (float(energy(torch.tensor(theta0 + h * v, dtype=DT))) -
+          float(energy(torch.tensor(theta0 - h * v, dtype=DT)))) / (2 * h)
+    assert float(grad.numpy() @ v) == pytest.approx(fd, rel=1e-5, abs=1e-9)

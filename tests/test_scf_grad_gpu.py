@@ -159,6 +159,33 @@ def test_diff_scf_loop_gradient_b3lyp_neural_correction():
     assert _rel(g.cpu().numpy(), g0) <= 1e-6, _rel(g.cpu().numpy(), g0)
 
 
+def test_diff_simple_scf_loop_matches_oracle():
+    """Linear-mixing loop (evaluate.py:257-352): energy and d E / d theta against the oracle."""
+    _say("case: diff_simple_scf_loop, neural h=32, water/def2-svp level 1, 4 cycles, mixing 0.4")
+    from graddft_amd import MLPSpec, NeuralFunctional, diff_simple_scf_loop
+    from graddft_amd.interface.integrals import unpack_eri
+    from oracle import scf as O
+    from oracle import scf_grad as G
+    from oracle import xc as X
+    from oracle.xc import MLPSpec as OSpec
+    mol = _water(seed=10, with_eri=True)
+    f = NeuralFunctional(MLPSpec(nc=6, h=32, n_hidden=2, nd=4))
+    theta = f.init(seed=6)
+    th = theta.detach().clone().requires_grad_(True)
+    out = diff_simple_scf_loop(f, cycles=4, mixing_factor=0.4)(th, mol)
+    g, = torch.autograd.grad(out.energy, [th])
+    xin = _oracle_inputs(mol)
+    eri = unpack_eri(mol.rep_tensor.cpu().numpy(), mol.nao)
+    inp = O.SCFInputs(mol.h1e.cpu().numpy(), mol.s1e.cpu().numpy(), eri, mol.nuclear_repulsion, None, 0.0)
+    th0 = torch.tensor(theta.cpu().numpy(), dtype=torch.float64, requires_grad=True)
+    E0 = G.diff_simple_scf_energy(lambda d: X.xc_energy_from_rdm1("neural_b3lyp", xin, d, th0, OSpec(6, 32, 2, 4)),
+                                  G.TorchSCFInputs.from_numpy(inp), torch.tensor(mol.rdm1.cpu().numpy()),
+                                  torch.tensor(mol.mo_occ.cpu().numpy()), 4, 0.4)
+    g0, = torch.autograd.grad(E0, [th0])
+    assert abs(out.energy.item() - E0.item()) <= 1e-9, (out.energy.item(), E0.item())
+    assert _rel(g.cpu().numpy(), g0.numpy()) <= 1e-6, _rel(g.cpu().numpy(), g0.numpy())
+
+
 @pytest.mark.parametrize("predictor", ["non_scf", "diff_scf"])
 def test_training_steps_reduce_the_loss(predictor):
     """train_kernel + adam + simple_energy_loss (train.py:312-359, 532-555): NaN-free gradients
