@@ -2,7 +2,8 @@
 
 Public API (names follow XanaduAI/GradDFT's ``grad_dft`` package):
 ``Grid``, ``Molecule``, ``Functional``, ``NeuralFunctional``, ``MLPSpec``, ``B3LYP``,
-``energy_predictor``, ``non_scf_predictor``, ``diff_scf_loop``, ``simple_energy_loss``,
+``energy_predictor``, ``non_scf_predictor``, ``diff_scf_loop``, ``diff_simple_scf_loop``,
+``scf_loop``, ``simple_scf_loop``, ``Diis``, ``damping``, ``level_shift``, ``simple_energy_loss``,
 ``mse_energy_loss``, ``train_kernel``, ``adam``, ``molecule_from_geometry``.
 """
 from .molecule import Grid, Molecule
@@ -13,11 +14,13 @@ from .interface.builder import molecule_from_geometry
 from .interface.store import load_molecules, save_molecules
 from .train import (energy_predictor, simple_energy_loss, mse_energy_loss, train_kernel, adam,
                     apply_updates)
-from .evaluate import JittableDiis, diff_scf_loop, diff_simple_scf_loop, non_scf_predictor
+from .evaluate import (Diis, JittableDiis, damping, diff_scf_loop, diff_simple_scf_loop, level_shift,
+                       non_scf_predictor, scf_loop, simple_scf_loop)
 from .eigenproblem import safe_eigh_vjp, safe_fock_solver, safe_general_eigh
 
 __all__ = ["Grid", "Molecule", "Functional", "NeuralFunctional", "B3LYPNeuralCorrection", "MLPSpec", "XCPlan",
            "XCEnergyFn", "XCEnergyVxcFn", "B3LYP", "B3LYP_GRID", "molecule_from_geometry",
            "energy_predictor", "non_scf_predictor", "simple_energy_loss", "mse_energy_loss",
-           "train_kernel", "adam", "apply_updates", "JittableDiis", "diff_scf_loop", "diff_simple_scf_loop", "safe_eigh_vjp",
+           "train_kernel", "adam", "apply_updates", "JittableDiis", "Diis", "diff_scf_loop", "diff_simple_scf_loop",
+           "scf_loop", "simple_scf_loop", "damping", "level_shift", "safe_eigh_vjp",
            "safe_fock_solver", "safe_general_eigh", "save_molecules", "load_molecules"]

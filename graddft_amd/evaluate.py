@@ -24,7 +24,7 @@ import torch
 from . import _lib
 from .eigenproblem import cholesky_inv, orthogonal_eigh, scf_step
 from .functional import Functional
-from .molecule import Molecule
+from .molecule import Molecule, orbital_grad
 from .train import energy_predictor
 
 
@@ -271,3 +271,325 @@ def diff_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp
                                 mo_occ=m_out.mo_occ.clone(), restricted=m_out.restricted)
 
     return scf_iterator
+
+
+# ---------------------------------------------------------------------------------------
+# Non-differentiable drivers: scf_loop (CDIIS / EDIIS / ADIIS, damping, level shift,
+# smearing) and simple_scf_loop (linear mixing), on the same device predictor and
+# eigensolver.  Their control flow is host-side, as in the reference (Python while loop,
+# scipy's bisect, the BFGS minimisation of EDIIS / ADIIS over at most max_diis weights).
+# ---------------------------------------------------------------------------------------
+
+HARTREE2KCALMOL = 627.50947   # utils/types.py:35
+
+
+def damping(s: torch.Tensor, d: torch.Tensor, f: torch.Tensor, factor: float) -> torch.Tensor:
+    """Fock damping (``evaluate.py:1477-1503``, pyscf's ``hf.damping``):
+    f - (f0 + f0^T) factor / (factor + 1) with f0 = (I - S D) F D S.  In an S-orthonormal
+    orbital basis that diagonalises D it scales the occupied-virtual blocks by 1/(1+factor)."""
+    eye = torch.eye(s.shape[-1], dtype=s.dtype, device=s.device)
+    f0 = (eye - s @ d) @ f @ d @ s
+    return f - (f0 + f0.transpose(-1, -2)) * (factor / (factor + 1.0))
+
+
+def level_shift(s: torch.Tensor, d: torch.Tensor, f: torch.Tensor, factor: float) -> torch.Tensor:
+    """Level shift of the virtual space (``evaluate.py:1506-1545``): F + factor (S - S D S)."""
+    return f + (s - s @ d @ s) * factor
+
+
+def _simplex_minimize(lin, quad):
+    """Minimise c.lin + c^T quad c over c = x^2 / sum(x^2) by BFGS from x = 1 with tol 1e-9
+    (the reference's ``jax.scipy.optimize.minimize(costf, ones, method="BFGS", tol=1e-9)``,
+    here scipy's BFGS with the analytic gradient).  Returns (c, cost)."""
+    import numpy as np
+    from scipy.optimize import minimize
+
+    sym = quad + quad.T
+
+    def cost(x):
+        s = x @ x
+        c = x * x / s
+        gc = lin + sym @ c
+        return c @ lin + c @ quad @ c, 2.0 * x / s * (gc - gc @ c)
+
+    res = minimize(cost, np.ones(len(lin)), jac=True, method="BFGS", tol=1e-9)
+    return res.x ** 2 / (res.x ** 2).sum(), float(res.fun)
+
+
+@dataclass
+class Diis:
+    """DIIS extrapolation with the reference's variants (``evaluate.py:1211-1474``).
+
+    History tensors grow by one entry per ``run`` to ``max_diis`` and then drop the oldest.
+    CDIIS: per spin, errors e_i = A (F_i D_i S - S D_i F_i) A^T, the bordered Gram matrix
+    (+1 borders, 0 corner) solved for x on the device (``gdx_diis_solve``), and
+    F = A^T (sum_i x_i F_i) A.  EDIIS (JCP 116, 8255) and ADIIS (JCP 132, 054109): one weight
+    vector for both spins from the (k, k) matrix tr(D_i F_j) (reduced on the device) and the
+    reference's cost functions, F = sum_i x_i F_i.  EDIIS2 / ADIIS2 switch to CDIIS when the
+    relative energy change (E_k - E_{k-1}) / E_{k-1} is below the threshold.  ADIIS takes the
+    newest entry at index cycle % max_diis, as the reference does."""
+    overlap_matrix: torch.Tensor
+    A: Optional[torch.Tensor] = None
+    max_diis: int = 8
+    diis_method: str = "EDIIS2"
+    ediis2_threshold: float = 1e-2
+    adiis2_threshold: float = 1e-2
+
+    METHODS = ("CDIIS", "EDIIS", "ADIIS", "EDIIS2", "ADIIS2")
+
+    def _a(self) -> torch.Tensor:
+        S = self.overlap_matrix
+        if self.A is None:
+            return torch.eye(S.shape[0], dtype=S.dtype, device=S.device)
+        return self.A
+
+    def empty(self):
+        n = self.overlap_matrix.shape[0]
+        z = lambda *s: torch.zeros(s, dtype=torch.float64, device=self.overlap_matrix.device)  # noqa: E731
+        return (z(0, 2, n, n), z(0, 2, n, n), z(0), z(0, 2, n, n))
+
+    def update(self, new_data, diis_data):
+        rdm1, fock, energy = new_data
+        dens, focks, energies, errs = diis_data
+        A = self._a()
+        fds = A @ fock @ rdm1 @ self.overlap_matrix @ A.T
+        err = fds - fds.transpose(1, 2)
+        errs = torch.cat([errs, err[None]])
+        dens = torch.cat([dens, rdm1[None]])
+        focks = torch.cat([focks, fock[None]])
+        energy = torch.as_tensor(energy, dtype=torch.float64, device=energies.device).reshape(1)
+        energies = torch.cat([energies, energy])
+        if errs.shape[0] > self.max_diis:
+            errs, dens, focks, energies = errs[1:], dens[1:], focks[1:], energies[1:]
+        return dens, focks, energies, errs
+
+    def run(self, new_data, diis_data, cycle: int = 0):
+        if self.diis_method not in self.METHODS:
+            raise ValueError(f"unknown DIIS method {self.diis_method!r}; one of {self.METHODS}")
+        diis_data = self.update(new_data, diis_data)
+        dens, focks, energies, errs = diis_data
+        if errs.shape[0] == 1:
+            return focks[0], diis_data
+        method = self.diis_method
+        rel = None
+        if method in ("EDIIS2", "ADIIS2"):
+            e_prev, e_last = energies[-2:].tolist()
+            rel = (e_last - e_prev) / e_prev
+        if (method == "CDIIS" or (method == "EDIIS2" and rel < self.ediis2_threshold)
+                or (method == "ADIIS2" and rel < self.adiis2_threshold)):
+            x = self.cdiis_minimize(errs)
+            F = torch.einsum("si,isjk->sjk", x, focks)
+            A = self._a()
+            return A.T @ F @ A, diis_data
+        if method in ("EDIIS", "EDIIS2"):
+            x, _ = self.ediis_minimize(dens, focks, energies)
+        else:
+            x, _ = self.adiis_minimize(dens, focks, cycle % self.max_diis)
+        F = torch.einsum("i,isjk->sjk", x, focks)
+        return F, diis_data
+
+    def cdiis_minimize(self, errs: torch.Tensor) -> torch.Tensor:
+        """(2, k) weights: x = B^-1 e_0 with B = [[0, 1^T], [1, G_s]], G_s the Gram matrix."""
+        k = errs.shape[0]
+        E = errs.transpose(0, 1).reshape(2, k, -1)
+        B = torch.zeros((2, k + 1, k + 1), dtype=torch.float64, device=errs.device)
+        B[:, 1:, 1:] = E @ E.transpose(1, 2)
+        B[:, 0, 1:] = 1.0
+        B[:, 1:, 0] = 1.0
+        if k + 1 <= 16:
+            x = _diis_solve(B)
+        else:
+            e0 = torch.zeros((2, k + 1, 1), dtype=torch.float64, device=errs.device)
+            e0[:, 0] = 1.0
+            x = torch.linalg.solve(B, e0)[..., 0]
+        if not bool(torch.isfinite(x).all()):
+            raise AssertionError("CDIIS coefficients are not finite (singular error Gram matrix)")
+        return x[:, 1:]
+
+    @staticmethod
+    def _trace_matrix(dens: torch.Tensor, focks: torch.Tensor) -> torch.Tensor:
+        # df_ij = sum_s tr(D_i,s F_j,s), one (k, 2n^2) x (2n^2, k) product on the device
+        k = dens.shape[0]
+        return dens.reshape(k, -1) @ focks.transpose(2, 3).reshape(k, -1).T
+
+    def ediis_minimize(self, dens, focks, energies):
+        df = self._trace_matrix(dens, focks)
+        diag = df.diagonal()
+        df = diag[:, None] + diag - df - df.T
+        c, fun = _simplex_minimize(energies.cpu().numpy(), -df.cpu().numpy())
+        return torch.as_tensor(c, dtype=torch.float64, device=dens.device), fun
+
+    def adiis_minimize(self, dens, focks, idnewest: int):
+        df = self._trace_matrix(dens, focks)
+        d_fn = df[:, idnewest]
+        dn_f = df[idnewest]
+        dn_fn = df[idnewest, idnewest]
+        dd_fn = d_fn - dn_fn
+        df = df - d_fn[:, None] - dn_f + dn_fn
+        c, fun = _simplex_minimize(2.0 * dd_fn.cpu().numpy(), df.cpu().numpy())
+        return torch.as_tensor(c, dtype=torch.float64, device=dens.device), fun
+
+
+def smearing_occupations(mo_energy: torch.Tensor, nelectron: float, smearing: str, sigma: float) -> torch.Tensor:
+    """Fractional occupations of ``scf_loop``'s smearing (``evaluate.py:471-497``): one chemical
+    potential mu for all spin orbitals, found by bisection (xtol = rtol = 1e-10) between the
+    lowest and highest orbital energy so that the occupations sum to nelectron; "gaussian":
+    erfc((e - mu) / sigma) / 2, "fermi-dirac": 1 / (exp((e - mu) / sigma) + 1)."""
+    import numpy as np
+    from scipy.optimize import bisect
+    from scipy.special import erfc
+
+    e = mo_energy.detach().double().cpu().numpy().reshape(-1)
+    if smearing == "gaussian":
+        occf = lambda m: 0.5 * erfc((e - m) / sigma)  # noqa: E731
+    elif smearing == "fermi-dirac":
+        def occf(m):
+            with np.errstate(over="ignore"):
+                return 1.0 / (np.exp((e - m) / sigma) + 1.0)
+    else:
+        raise ValueError(f"unknown smearing {smearing!r}; 'gaussian' or 'fermi-dirac'")
+    mu = bisect(lambda m: occf(m).sum() - nelectron, e.min(), e.max(), xtol=1e-10, rtol=1e-10, maxiter=10000)
+    return torch.as_tensor(occf(mu).reshape(tuple(mo_energy.shape)), dtype=torch.float64, device=mo_energy.device)
+
+
+def _check_charge(functional: Functional, molecule: Molecule, precision: str, nelectron: float) -> None:
+    """The loops' charge-conservation assert: sum_r w_r (rho_up + rho_dn)(r) = N_e within
+    atol 1e-3 (and jnp.isclose's rtol 1e-5); rho from the device density kernel."""
+    rho = functional.plan(molecule, precision).densities(molecule.rdm1.detach())[:, :, 0].sum(dim=1)
+    charge = float((molecule.grid.weights * rho).sum())
+    if not abs(nelectron - charge) <= 1e-3 + 1e-5 * abs(charge):
+        raise AssertionError("Total charge is not conserved. given electrons: %.3f, computed electrons: %.3f"
+                             % (nelectron, charge))
+
+
+def _linv(molecule: Molecule) -> torch.Tensor:
+    Linv = molecule.cache.get("Linv")
+    if Linv is None:
+        _, Linv = cholesky_inv(molecule.s1e)
+        molecule.cache["Linv"] = Linv
+    return Linv
+
+
+def scf_loop(functional: Functional, level_shift_factor=(0.0, 0.0), damp_factor=(0.0, 0.0),
+             chunk_size: int = 1024, cycles: int = 50, diis_start_cycle: int = 0, e_conv: float = 1e-5,
+             g_conv: float = 1e-5, diis_method: str = "CDIIS", smearing: Optional[str] = None,
+             smearing_sigma: float = 0.0, verbose: int = 0, precision: str = "fp64", **kwargs) -> Callable:
+    """``scf_loop`` (``evaluate.py:355-592``): predict at the molecule's density, then while
+    (|dE| in kcal/mol > e_conv or |g_orb| > g_conv) and cycle < cycles: damping of the Fock
+    matrix for cycle < diis_start_cycle - 1 (when the factors are nonzero), DIIS from
+    diis_start_cycle on (``Diis``, max_diis 10, A = I), level shift, diagonalisation, aufbau
+    (or smeared) occupations, new density, charge check, predict.  A converged loop ends with
+    one more unshifted diagonalisation and prediction.
+
+    The orbital-gradient norm is taken, as in the reference, with the Fock matrix the molecule
+    carried in (``Molecule.get_mo_grads`` on the loop's molecule, whose ``fock`` field the loop
+    does not update); a molecule without one uses the loop's current Fock matrix.  Smearing
+    uses one chemical potential for the spin orbitals of both spins and keeps the (2, n)
+    occupation layout.  The range-separated ``chi`` update is not part of this path
+    (exact exchange is the packed-ERI term).  Not differentiable (runs without autograd)."""
+    compute_energy = energy_predictor(functional, precision=precision, **kwargs)
+    spin_equal = damp_factor[0] == damp_factor[1] and level_shift_factor[0] == level_shift_factor[1]
+
+    def solve(fock, Linv, V, molecule, nelectron):
+        mo_energy, mo_coeff, V = orthogonal_eigh(fock, Linv, V)
+        molecule = molecule.replace(mo_coeff=mo_coeff, mo_energy=mo_energy)
+        mo_occ = molecule.get_occ()
+        if smearing:
+            mo_occ = smearing_occupations(mo_energy, nelectron, smearing, smearing_sigma)
+        molecule = molecule.replace(mo_occ=mo_occ)
+        molecule = molecule.replace(rdm1=molecule.make_rdm1(),
+                                    restricted=molecule.restricted and functional.spin_symmetric and spin_equal)
+        return molecule, V
+
+    def scf_iterator(params: Optional[torch.Tensor], molecule: Molecule) -> Molecule:
+        with torch.no_grad():
+            S = molecule.s1e
+            Linv = _linv(molecule)
+            carried_fock = molecule.fock
+            nelectron = float(molecule.atom_index.sum()) - molecule.charge
+            predicted_e, fock = compute_energy(params, molecule)
+            e = float(predicted_e)
+            old_e = float("inf")
+            norm_gorb = float("inf")
+            cycle = 0
+            diis = Diis(S, None, max_diis=10, diis_method=diis_method)
+            data = diis.empty()
+            V = None
+            damp = abs(damp_factor[0]) + abs(damp_factor[1]) > 1e-4
+            shift = abs(level_shift_factor[0]) + abs(level_shift_factor[1]) > 1e-4
+
+            def gnorm(mol, f):
+                g = orbital_grad(mol.mo_coeff, mol.mo_occ, carried_fock if carried_fock is not None else f)
+                return float(torch.linalg.norm(g))
+
+            while (abs(e - old_e) * HARTREE2KCALMOL > e_conv or norm_gorb > g_conv) and cycle < cycles:
+                old_e = e
+                if 0 <= cycle < diis_start_cycle - 1 and damp:
+                    fock = torch.stack([damping(S, molecule.rdm1[s], fock[s], damp_factor[s]) for s in range(2)])
+                if cycle >= diis_start_cycle:
+                    fock, data = diis.run((molecule.rdm1, fock, predicted_e), data, cycle)
+                if shift:
+                    fock = torch.stack([level_shift(S, molecule.rdm1[s], fock[s], level_shift_factor[s])
+                                        for s in range(2)])
+                molecule, V = solve(fock, Linv, V, molecule, nelectron)
+                _check_charge(functional, molecule, precision, nelectron)
+                predicted_e, fock = compute_energy(params, molecule)
+                e = float(predicted_e)
+                norm_gorb = gnorm(molecule, fock)
+                if verbose > 1:
+                    print(f"cycle: {cycle}, energy: {e:.7e}, energy difference: {abs(e - old_e):.4e}, "
+                          f"norm_gradient_orbitals: {norm_gorb:.2e}")
+                cycle += 1
+
+            if abs(e - old_e) * HARTREE2KCALMOL < e_conv and norm_gorb < g_conv:
+                # one more diagonalisation of the unshifted Fock matrix
+                molecule, V = solve(fock, Linv, V, molecule, nelectron)
+                predicted_e, fock = compute_energy(params, molecule)
+                e = float(predicted_e)
+                norm_gorb = gnorm(molecule, fock)
+            if verbose > 1:
+                print(f"cycle: {cycle}, predicted energy: {e:.7e}, energy difference: {abs(e - old_e):.4e}, "
+                      f"norm_gradient_orbitals: {norm_gorb:.2e}")
+            out = molecule.replace(fock=fock, energy=predicted_e)
+            out.cache["scf_cycles"] = cycle
+            return out
+
+    return scf_iterator
+
+
+def simple_scf_loop(functional: Functional, mixing_factor: float = 0.4, chunk_size: int = 1024,
+                    cycles: int = 50, start_cycle: int = 0, e_conv: float = 1e-5, g_conv: float = 1e-5,
+                    verbose: int = 0, precision: str = "fp64", **kwargs) -> Callable:
+    """``simple_scf_loop`` (``evaluate.py:131-254``): a fixed number of cycles; cycle 0 keeps
+    the molecule's orbitals (aufbau occupations from its mo_energy) and predicts at its
+    density, which becomes the mixing reference make_rdm1(); every later cycle diagonalises the
+    previous Fock matrix and mixes D = (1 - mixing_factor) D_old + mixing_factor D_new.  Charge
+    check each cycle.  Autograd records through it when the caller enables it (the device
+    predictor and eigensolver are differentiable), unlike the reference's asserting loop."""
+    compute_energy = energy_predictor(functional, precision=precision, **kwargs)
+
+    def simple_scf_iterator(params: Optional[torch.Tensor], atoms: Molecule, clip_cte: float = 1e-30) -> Molecule:
+        Linv = _linv(atoms)
+        nelectron = float(atoms.atom_index.sum()) - atoms.charge
+        fock = atoms.fock
+        old_rdm1 = None
+        V = None
+        predicted_e = None
+        for cycle in range(cycles):
+            if cycle > 0:
+                mo_energy, mo_coeff, V = orthogonal_eigh(fock, Linv, V)
+                atoms = atoms.replace(mo_coeff=mo_coeff, mo_energy=mo_energy)
+            atoms = atoms.replace(mo_occ=atoms.get_occ())
+            if cycle == 0:
+                old_rdm1 = atoms.make_rdm1()
+            else:
+                rdm1 = (1 - mixing_factor) * old_rdm1 + mixing_factor * atoms.make_rdm1()
+                atoms = atoms.replace(rdm1=rdm1, restricted=atoms.restricted and functional.spin_symmetric)
+                old_rdm1 = rdm1
+            _check_charge(functional, atoms, precision, nelectron)
+            predicted_e, fock = compute_energy(params, atoms)
+            if verbose > 0:
+                print(f"cycle: {cycle}, energy: {float(predicted_e):.7e}")
+        return atoms.replace(fock=fock, energy=predicted_e)
+
+    return simple_scf_iterator

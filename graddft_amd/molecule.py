@@ -99,3 +99,39 @@ class Molecule:
 
     def nelec(self) -> torch.Tensor:
         return self.mo_occ.sum(dim=1)
+
+    # -- orbital helpers (molecule.py:302-330) ------------------------------------------
+    def make_rdm1(self) -> torch.Tensor:
+        return make_rdm1(self.mo_coeff, self.mo_occ)
+
+    def get_occ(self) -> torch.Tensor:
+        nelecs = [int(v) for v in self.mo_occ.sum(dim=1).tolist()]
+        return get_occ(self.mo_energy, nelecs)
+
+    def get_mo_grads(self) -> torch.Tensor:
+        return orbital_grad(self.mo_coeff, self.mo_occ, self.fock)
+
+
+def make_rdm1(mo_coeff: torch.Tensor, mo_occ: torch.Tensor) -> torch.Tensor:
+    """D_s = C_s diag(occ_s) C_s^T (``molecule.py:816-846``)."""
+    return torch.einsum("sij,sj,skj->sik", mo_coeff, mo_occ, mo_coeff)
+
+
+def get_occ(mo_energy: torch.Tensor, nelecs) -> torch.Tensor:
+    """Aufbau occupations (``molecule.py:851-889``): per spin, the nelec_s orbitals of lowest
+    energy (ties in index order) get 1, the others 0.  nelecs is truncated to integers, as the
+    reference's int64 cast of the occupation sums."""
+    occ = torch.zeros_like(mo_energy)
+    order = torch.argsort(mo_energy, dim=1, stable=True)
+    for s in range(2):
+        occ[s, order[s, : int(nelecs[s])]] = 1.0
+    return occ
+
+
+def orbital_grad(mo_coeff: torch.Tensor, mo_occ: torch.Tensor, fock: torch.Tensor) -> torch.Tensor:
+    """Orbital gradient (``molecule.py:344-373``): sum over spins of C_vir^T F C_occ, where
+    C_occ keeps the columns with occ > 0 and C_vir those with occ == 0 (others zeroed);
+    an (n, n) matrix."""
+    c_occ = torch.where(mo_occ[:, None, :] > 0, mo_coeff, torch.zeros_like(mo_coeff))
+    c_vir = torch.where(mo_occ[:, None, :] == 0, mo_coeff, torch.zeros_like(mo_coeff))
+    return torch.einsum("sab,sac,scd->bd", c_vir, fock, c_occ)

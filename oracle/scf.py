@@ -194,3 +194,216 @@ def diff_scf_loop(xc_value_and_vxc: Callable, inp: SCFInputs, rdm1: np.ndarray, 
         e, fock = predict(xc_value_and_vxc, inp, D)
         st = SCFState(D, fock, e, w, C, occ, st.energies + [e])
     return st
+
+
+# ---------------------------------------------------------------------------------------
+# scf_loop / simple_scf_loop and their pieces (grad_dft/evaluate.py:131-592, 1211-1545;
+# grad_dft/molecule.py:344-373).
+# ---------------------------------------------------------------------------------------
+
+HARTREE2KCALMOL = 627.50947   # utils/types.py:35
+
+
+def damping(s, d, f, factor):
+    """evaluate.py:1477-1503."""
+    dm_vir = np.eye(s.shape[0]) - s @ d
+    f0 = dm_vir @ f @ d @ s
+    f0 = (f0 + f0.T) * (factor / (factor + 1.0))
+    return f - f0
+
+
+def level_shift(s, d, f, factor):
+    """evaluate.py:1506-1545."""
+    return f + (s - s @ d @ s) * factor
+
+
+def orbital_grad(mo_coeff, mo_occ, F):
+    """molecule.py:344-373: (n, n), summed over spins."""
+    out = np.zeros(mo_coeff.shape[1:])
+    for s in range(2):
+        occ = mo_occ[s] > 0
+        vir = mo_occ[s] == 0
+        Cocc = mo_coeff[s] * occ[None, :]
+        Cvir = mo_coeff[s] * vir[None, :]
+        out += Cvir.T @ F[s] @ Cocc
+    return out
+
+
+def _bfgs_weights(costf_and_grad, nx):
+    """jax.scipy.optimize.minimize(costf, ones(nx), method="BFGS", tol=1e-9) -> (c, fun)."""
+    from scipy.optimize import minimize
+    res = minimize(costf_and_grad, np.ones(nx), jac=True, method="BFGS", tol=1e-9)
+    return res.x ** 2 / np.sum(res.x ** 2), float(res.fun)
+
+
+def _weights_cost(lin, quad):
+    """cost(x) = c.lin + c^T quad c with c = x^2 / |x|^2, and its gradient in x."""
+    def f(x):
+        n2 = float(np.sum(x * x))
+        c = x * x / n2
+        val = float(lin @ c + c @ quad @ c)
+        dc = lin + quad @ c + quad.T @ c                     # d cost / d c
+        jac = (2.0 * x / n2) * dc - (2.0 * x / n2) * float(dc @ c)
+        return val, jac
+    return f
+
+
+@dataclass
+class Diis:
+    """evaluate.py:1211-1474 with Python lists for the history."""
+    s1e: np.ndarray
+    A: np.ndarray
+    max_diis: int = 8
+    diis_method: str = "EDIIS2"
+    ediis2_threshold: float = 1e-2
+    adiis2_threshold: float = 1e-2
+
+    def update(self, rdm1, fock, energy, data):
+        dens, focks, energies, errs = [list(d) for d in data]
+        fds = np.stack([self.A @ fock[s] @ rdm1[s] @ self.s1e @ self.A.T for s in range(2)])
+        errs.append(fds - fds.transpose(0, 2, 1))
+        dens.append(rdm1)
+        focks.append(fock)
+        energies.append(float(energy))
+        if len(errs) > self.max_diis:
+            dens, focks, energies, errs = dens[1:], focks[1:], energies[1:], errs[1:]
+        return dens, focks, energies, errs
+
+    def run(self, rdm1, fock, energy, data, cycle=0):
+        data = self.update(rdm1, fock, energy, data)
+        dens, focks, energies, errs = data
+        if len(errs) == 1:
+            return focks[0], data
+        rel = (energies[-1] - energies[-2]) / energies[-2]
+        m = self.diis_method
+        if m == "CDIIS" or (m == "EDIIS2" and rel < self.ediis2_threshold) or (m == "ADIIS2" and rel < self.adiis2_threshold):
+            x = self.cdiis_minimize(np.stack(errs))
+            F = np.einsum("si,isjk->sjk", x, np.stack(focks))
+            return np.einsum("ji,sjk,kl->sil", self.A, F, self.A), data
+        if m in ("EDIIS", "EDIIS2"):
+            x, _ = self.ediis_minimize(np.stack(dens), np.stack(focks), np.array(energies))
+        elif m in ("ADIIS", "ADIIS2"):
+            x, _ = self.adiis_minimize(np.stack(dens), np.stack(focks), cycle % self.max_diis)
+        else:
+            raise ValueError(m)
+        return np.einsum("i,isjk->sjk", x, np.stack(focks)), data
+
+    @staticmethod
+    def cdiis_minimize(errs):
+        k = errs.shape[0]
+        xs = []
+        for s in range(2):
+            B = np.zeros((k + 1, k + 1))
+            B[1:, 1:] = np.einsum("ikl,jkl->ij", errs[:, s], errs[:, s])
+            B[0, 1:] = 1.0
+            B[1:, 0] = 1.0
+            C = np.zeros(k + 1)
+            C[0] = 1.0
+            w, v = np.linalg.eigh(B)
+            xs.append(v @ np.diag(1.0 / w) @ v.T @ C)
+        x = np.stack(xs)
+        assert not np.any(np.isnan(x))
+        return x[:, 1:]
+
+    @staticmethod
+    def _df(dens, focks):
+        k = dens.shape[0]
+        return np.array([[sum(np.trace(dens[i, s] @ focks[j, s]) for s in range(dens.shape[1]))
+                          for j in range(k)] for i in range(k)])
+
+    def ediis_minimize(self, dens, focks, energies):
+        df = self._df(dens, focks)
+        d = np.diag(df)
+        df = d[:, None] + d[None, :] - df - df.T
+        return _bfgs_weights(_weights_cost(energies, -df), len(energies))
+
+    def adiis_minimize(self, dens, focks, idnewest):
+        df = self._df(dens, focks)
+        d_fn = df[:, idnewest]
+        dn_f = df[idnewest]
+        dn_fn = df[idnewest, idnewest]
+        dd_fn = d_fn - dn_fn
+        df = df - d_fn[:, None] - dn_f[None, :] + dn_fn
+        return _bfgs_weights(_weights_cost(2.0 * dd_fn, df), dens.shape[0])
+
+
+def smearing_occ(mo_energy, nelectron, smearing, sigma):
+    """evaluate.py:471-497 with one chemical potential over both spins' orbitals, (2, n)."""
+    from scipy.optimize import bisect
+    from scipy.special import erfc
+    es = np.hstack(mo_energy)
+    if smearing == "gaussian":
+        occ = lambda m, e: 0.5 * erfc((e - m) / sigma)  # noqa: E731
+    else:
+        def occ(m, e):
+            with np.errstate(over="ignore"):
+                return 1.0 / (np.exp((e - m) / sigma) + 1.0)
+    mu = bisect(lambda m: occ(m, es).sum() - nelectron, es.min(), es.max(), xtol=1e-10, rtol=1e-10, maxiter=10000)
+    return occ(mu, es).reshape(mo_energy.shape)
+
+
+def scf_loop(xc_value_and_vxc: Callable, inp: SCFInputs, rdm1: np.ndarray, mo_occ: np.ndarray, *,
+             level_shift_factor=(0.0, 0.0), damp_factor=(0.0, 0.0), cycles=50, diis_start_cycle=0,
+             e_conv=1e-5, g_conv=1e-5, diis_method="CDIIS", smearing=None, smearing_sigma=0.0,
+             carried_fock=None, nelectron=None):
+    """evaluate.py:387-589 (no chi update).  Returns (SCFState, cycles run).  The orbital
+    gradient uses the Fock matrix the molecule carried in (carried_fock), or the current one
+    when there is none."""
+    S = inp.s1e
+    n = S.shape[0]
+    if nelectron is None:
+        nelectron = float(mo_occ.sum())
+    e, fock = predict(xc_value_and_vxc, inp, rdm1)
+    old_e, norm_gorb, cycle = np.inf, np.inf, 0
+    diis = Diis(S, np.eye(n), 10, diis_method)
+    data = ([], [], [], [])
+    D, occ, w, C = rdm1, mo_occ, None, None
+
+    def solve(F, occ):
+        w, C = safe_fock_solver(F, S)
+        occ = get_occ(w, np.array([int(v) for v in occ.sum(axis=1)]))
+        if smearing:
+            occ = smearing_occ(w, nelectron, smearing, smearing_sigma)
+        return w, C, occ, make_rdm1(C, occ)
+
+    def gnorm(C, occ, F):
+        return np.linalg.norm(orbital_grad(C, occ, carried_fock if carried_fock is not None else F))
+
+    while (abs(e - old_e) * HARTREE2KCALMOL > e_conv or norm_gorb > g_conv) and cycle < cycles:
+        old_e = e
+        if 0 <= cycle < diis_start_cycle - 1 and abs(damp_factor[0]) + abs(damp_factor[1]) > 1e-4:
+            fock = np.stack([damping(S, D[s], fock[s], damp_factor[s]) for s in range(2)])
+        if cycle >= diis_start_cycle:
+            fock, data = diis.run(D, fock, e, data, cycle)
+        if abs(level_shift_factor[0]) + abs(level_shift_factor[1]) > 1e-4:
+            fock = np.stack([level_shift(S, D[s], fock[s], level_shift_factor[s]) for s in range(2)])
+        w, C, occ, D = solve(fock, occ)
+        e, fock = predict(xc_value_and_vxc, inp, D)
+        norm_gorb = gnorm(C, occ, fock)
+        cycle += 1
+    if abs(e - old_e) * HARTREE2KCALMOL < e_conv and norm_gorb < g_conv:
+        w, C, occ, D = solve(fock, occ)
+        e, fock = predict(xc_value_and_vxc, inp, D)
+    return SCFState(D, fock, e, w, C, occ), cycle
+
+
+def simple_scf_loop(xc_value_and_vxc: Callable, inp: SCFInputs, rdm1, mo_coeff, mo_energy, mo_occ,
+                    mixing_factor=0.4, cycles=50) -> SCFState:
+    """evaluate.py:176-252: cycle 0 keeps the given orbitals (the density used is the given
+    rdm1), later cycles diagonalise the last Fock matrix and mix the densities."""
+    S = inp.s1e
+    D, C, w, occ = rdm1, mo_coeff, mo_energy, mo_occ
+    energies = []
+    fock = None
+    for cycle in range(cycles):
+        if cycle > 0:
+            w, C = safe_fock_solver(fock, S)
+        occ = get_occ(w, np.array([int(v) for v in occ.sum(axis=1)]))
+        if cycle == 0:
+            old = make_rdm1(C, occ)
+        else:
+            D = (1 - mixing_factor) * old + mixing_factor * make_rdm1(C, occ)
+            old = D
+        e, fock = predict(xc_value_and_vxc, inp, D)
+        energies.append(e)
+    return SCFState(D, fock, e, w, C, occ, energies)

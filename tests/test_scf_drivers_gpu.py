@@ -1,0 +1,117 @@
+"""GPU parity of the non-differentiable drivers (scf_loop with CDIIS / EDIIS / ADIIS,
+damping, level shift and smearing; simple_scf_loop) and their pieces against the FP64
+oracle restatements (oracle/scf.py) on identical inputs."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _water(basis="def2-svp", level=1, seed=11):
+    from graddft_amd.interface import geometry
+    from graddft_amd.interface.builder import molecule_from_geometry
+    sym, xyz = geometry.water(np.random.default_rng(seed))
+    return molecule_from_geometry(sym, xyz, basis, level, with_eri=True)
+
+
+def _oracle_inputs(mol):
+    from graddft_amd.interface.integrals import unpack_eri
+    from oracle.ao import eval_ao
+    from oracle.scf import SCFInputs, exchange_potential
+    from oracle.xc import to_inputs, xc_value_and_grads
+    eri = unpack_eri(mol.rep_tensor.cpu().numpy(), mol.nao)
+    D0 = mol.cache["rdm1_init"].cpu().numpy()
+    K0 = np.stack([exchange_potential(D0[s], eri) for s in range(2)])
+    inp = SCFInputs(mol.h1e.cpu().numpy(), mol.s1e.cpu().numpy(), eri, mol.nuclear_repulsion, K0, 0.2)
+    xin = to_inputs(*eval_ao(mol.basis, mol.grid.coords.cpu().numpy()), mol.grid.weights.cpu().numpy())
+    return inp, (lambda d: xc_value_and_grads("b3lyp", xin, d)[:2])
+
+
+def _sym(rng, *shape):
+    a = rng.normal(size=shape)
+    return a + np.swapaxes(a, -1, -2)
+
+
+def test_pieces_match_oracle():
+    _say("case: damping / level_shift / orbital_grad / get_occ / CDIIS / EDIIS / ADIIS weights, n=9")
+    from graddft_amd.evaluate import Diis, damping, level_shift
+    from graddft_amd.molecule import get_occ, orbital_grad
+    from oracle import scf as O
+    rng = np.random.default_rng(5)
+    n = 9
+    A = rng.normal(size=(n, n))
+    S = A @ A.T / n + np.eye(n)
+    D, F = _sym(rng, n, n), _sym(rng, n, n)
+    t = lambda x: torch.as_tensor(x, device="cuda")  # noqa: E731
+    assert np.abs(damping(t(S), t(D), t(F), 0.3).cpu().numpy() - O.damping(S, D, F, 0.3)).max() <= 1e-12
+    assert np.abs(level_shift(t(S), t(D), t(F), 0.2).cpu().numpy() - O.level_shift(S, D, F, 0.2)).max() <= 1e-12
+    C, F2 = rng.normal(size=(2, n, n)), _sym(rng, 2, n, n)
+    e = rng.normal(size=(2, n))
+    occ = O.get_occ(e, np.array([4, 3]))
+    assert np.array_equal(get_occ(t(e), [4, 3]).cpu().numpy(), occ)
+    g = orbital_grad(t(C), t(occ), t(F2)).cpu().numpy()
+    assert np.abs(g - O.orbital_grad(C, occ, F2)).max() <= 1e-12
+    k = 5
+    E = rng.normal(size=(k, 2, n, n))
+    errs = E - np.swapaxes(E, -1, -2)
+    dev = Diis(t(S), None, 8, "CDIIS")
+    ora = O.Diis(S, np.eye(n), 8, "CDIIS")
+    x = dev.cdiis_minimize(t(errs)).cpu().numpy()
+    assert np.abs(x - ora.cdiis_minimize(errs)).max() <= 1e-10
+    Ds, Fs, Es = _sym(rng, k, 2, n, n), _sym(rng, k, 2, n, n), rng.normal(size=k)
+    c, f = dev.ediis_minimize(t(Ds), t(Fs), t(Es))
+    c0, f0 = ora.ediis_minimize(Ds, Fs, Es)
+    assert abs(f - f0) <= 1e-9 * max(1.0, abs(f0)) and np.abs(c.cpu().numpy() - c0).max() <= 1e-5
+    c, f = dev.adiis_minimize(t(Ds), t(Fs), 3)
+    c0, f0 = ora.adiis_minimize(Ds, Fs, 3)
+    assert abs(f - f0) <= 1e-9 * max(1.0, abs(f0)) and np.abs(c.cpu().numpy() - c0).max() <= 1e-5
+
+
+@pytest.mark.parametrize("opts,tol", [
+    (dict(), 1e-8),
+    (dict(diis_start_cycle=3, damp_factor=(0.3, 0.3), level_shift_factor=(0.2, 0.2)), 1e-8),
+    (dict(diis_method="EDIIS2"), 1e-8),
+    (dict(diis_method="ADIIS", cycles=25), 5e-7),
+    (dict(smearing="fermi-dirac", smearing_sigma=0.01), 1e-8),
+], ids=["cdiis", "damp_shift", "ediis2", "adiis", "fermi"])
+def test_scf_loop_matches_oracle(opts, tol):
+    _say(f"case: scf_loop {opts} B3LYP water/def2-svp level 1")
+    from graddft_amd import B3LYP
+    from graddft_amd.evaluate import scf_loop
+    from oracle import scf as O
+    mol = _water()
+    out = scf_loop(B3LYP, **opts)(None, mol)
+    inp, xc = _oracle_inputs(mol)
+    st, cycles = O.scf_loop(xc, inp, mol.rdm1.cpu().numpy(), mol.mo_occ.cpu().numpy(), **opts)
+    assert abs(float(out.energy) - st.energy) <= tol, (float(out.energy), st.energy)
+    assert abs(out.cache["scf_cycles"] - cycles) <= (0 if tol <= 1e-8 else 1)
+    assert np.abs(out.rdm1.cpu().numpy() - st.rdm1).max() <= 1e-5
+
+
+def test_simple_scf_loop_matches_oracle():
+    _say("case: simple_scf_loop B3LYP water/def2-svp level 1, 10 cycles, mixing 0.4")
+    from graddft_amd import B3LYP
+    from graddft_amd.evaluate import simple_scf_loop
+    from oracle import scf as O
+    mol = _water()
+    out = simple_scf_loop(B3LYP, cycles=10, mixing_factor=0.4)(None, mol)
+    inp, xc = _oracle_inputs(mol)
+    st = O.simple_scf_loop(xc, inp, mol.rdm1.cpu().numpy(), mol.mo_coeff.cpu().numpy(),
+                           mol.mo_energy.cpu().numpy(), mol.mo_occ.cpu().numpy(), mixing_factor=0.4, cycles=10)
+    assert abs(float(out.energy) - st.energy) <= 1e-9, (float(out.energy), st.energy)
+    assert np.abs(out.rdm1.cpu().numpy() - st.rdm1).max() <= 1e-7
+
+
+def test_charge_check_rejects_a_wrong_density():
+    _say("case: simple_scf_loop charge check, rdm1 scaled by 1.1")
+    from graddft_amd import B3LYP
+    from graddft_amd.evaluate import simple_scf_loop
+    mol = _water()
+    bad = mol.replace(rdm1=mol.rdm1 * 1.1)
+    with pytest.raises(AssertionError, match="Total charge is not conserved"):
+        simple_scf_loop(B3LYP, cycles=2)(None, bad)
