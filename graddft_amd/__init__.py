@@ -3,7 +3,8 @@
 Public API (names follow XanaduAI/GradDFT's ``grad_dft`` package):
 ``Grid``, ``Molecule``, ``Functional``, ``NeuralFunctional``, ``MLPSpec``, ``B3LYP``,
 ``energy_predictor``, ``non_scf_predictor``, ``diff_scf_loop``, ``diff_simple_scf_loop``,
-``scf_loop``, ``simple_scf_loop``, ``Diis``, ``damping``, ``level_shift``, ``simple_energy_loss``,
+``scf_loop``, ``simple_scf_loop``, ``Diis``, ``damping``, ``level_shift``, ``mol_orb_optimizer``,
+``jitted_mol_orb_optimizer``, ``simple_energy_loss``,
 ``mse_energy_loss``, ``train_kernel``, ``adam``, ``molecule_from_geometry``.
 """
 from .molecule import Grid, Molecule
@@ -14,13 +15,14 @@ from .interface.builder import molecule_from_geometry
 from .interface.store import load_molecules, save_molecules
 from .train import (energy_predictor, simple_energy_loss, mse_energy_loss, train_kernel, adam,
                     apply_updates)
-from .evaluate import (Diis, JittableDiis, damping, diff_scf_loop, diff_simple_scf_loop, level_shift,
-                       non_scf_predictor, scf_loop, simple_scf_loop)
+from .evaluate import (Diis, JittableDiis, damping, diff_scf_loop, diff_simple_scf_loop, jitted_mol_orb_optimizer,
+                       level_shift, mol_orb_optimizer, non_scf_predictor, scf_loop, simple_scf_loop)
 from .eigenproblem import safe_eigh_vjp, safe_fock_solver, safe_general_eigh
 
 __all__ = ["Grid", "Molecule", "Functional", "NeuralFunctional", "B3LYPNeuralCorrection", "MLPSpec", "XCPlan",
            "XCEnergyFn", "XCEnergyVxcFn", "B3LYP", "B3LYP_GRID", "molecule_from_geometry",
            "energy_predictor", "non_scf_predictor", "simple_energy_loss", "mse_energy_loss",
            "train_kernel", "adam", "apply_updates", "JittableDiis", "Diis", "diff_scf_loop", "diff_simple_scf_loop",
-           "scf_loop", "simple_scf_loop", "damping", "level_shift", "safe_eigh_vjp",
+           "scf_loop", "simple_scf_loop", "damping", "level_shift", "mol_orb_optimizer",
+           "jitted_mol_orb_optimizer", "safe_eigh_vjp",
            "safe_fock_solver", "safe_general_eigh", "save_molecules", "load_molecules"]

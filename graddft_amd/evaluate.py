@@ -22,9 +22,9 @@ from typing import Callable, List, Optional
 import torch
 
 from . import _lib
-from .eigenproblem import cholesky_inv, orthogonal_eigh, scf_step
+from .eigenproblem import cholesky_inv, eigh, orthogonal_eigh, scf_step
 from .functional import Functional
-from .molecule import Molecule, orbital_grad
+from .molecule import Molecule, make_rdm1, orbital_grad
 from .train import energy_predictor
 
 
@@ -593,3 +593,94 @@ def simple_scf_loop(functional: Functional, mixing_factor: float = 0.4, chunk_si
         return atoms.replace(fock=fock, energy=predicted_e)
 
     return simple_scf_iterator
+
+
+# ---------------------------------------------------------------------------------------
+# Orbital optimizers (evaluate.py:595-914): minimise the predicted energy over orthonormal
+# orbitals C = (qr(W) D)^T with an optax-style optimizer on W; dE/dW by reverse mode through
+# the device predictor (dE/dD = V_xc + J + h) and torch's QR.
+# ---------------------------------------------------------------------------------------
+
+def whitening_matrix(S: torch.Tensor, whitening: str = "PCA") -> torch.Tensor:
+    """D with D^T D = S^-1 (``evaluate.py:709-723``): "PCA" diag(w^-1/2) V^T, "Cholesky"
+    chol(S^-1)^T, "ZCA" V diag(w^-1/2) V^T.  Eigenvector signs are fixed so that each
+    eigenvector's largest-magnitude component is positive (LAPACK leaves them arbitrary; the
+    starting orbitals after the QR step depend on them)."""
+    if whitening == "Cholesky":
+        return torch.linalg.cholesky(torch.linalg.inv(S)).T.contiguous()
+    if whitening not in ("PCA", "ZCA"):
+        raise ValueError(f"unknown whitening {whitening!r}; 'PCA', 'Cholesky' or 'ZCA'")
+    w, V = eigh(S[None].contiguous())
+    w, V = w[0], V[0]
+    idx = V.abs().argmax(dim=0)
+    V = V * torch.sign(V[idx, torch.arange(V.shape[1], device=V.device)])[None, :]
+    if whitening == "PCA":
+        return (w.rsqrt()[:, None] * V.T).contiguous()
+    return (V * w.rsqrt()[None, :]) @ V.T
+
+
+def _orbital_energy(compute_energy, params, molecule: Molecule, W: torch.Tensor, D: torch.Tensor):
+    """molecule_orbitals_iterator (``evaluate.py:646-688``): Q_s = qr(W_s), C_s = (Q_s D)^T,
+    rdm1 = make_rdm1(C, mo_occ), predicted energy."""
+    Q = torch.linalg.qr(W)[0]
+    C = torch.einsum("sij,jk->ski", Q, D)
+    rdm1 = make_rdm1(C, molecule.mo_occ)
+    e, _ = compute_energy(params, molecule.replace(rdm1=rdm1, mo_coeff=C, restricted=False))
+    return e
+
+
+def _orbital_step(compute_energy, params, molecule, W, D, tx, opt_state):
+    Wg = W.detach().requires_grad_(True)
+    with torch.enable_grad():
+        E = _orbital_energy(compute_energy, params, molecule, Wg, D)
+        g, = torch.autograd.grad(E, [Wg])
+    updates, opt_state = tx.update(g, opt_state, W)
+    return E.detach(), W + updates, opt_state
+
+
+def mol_orb_optimizer(fxc: Functional, tx, chunk_size: int = 1024, cycles: int = 500, e_conv: float = 1e-7,
+                      whitening: str = "PCA", verbose: int = 0, precision: str = "fp64", **kwargs) -> Callable:
+    """``mol_orb_optimizer`` (``evaluate.py:595-777``, D4FT, ICLR 2023): W = C^T D^-1 from the
+    molecule's orbitals, then while |dE| in kcal/mol > e_conv and cycle < cycles: energy and
+    dE/dW at W, one optimizer step.  Returns the molecule with the last predicted energy (its
+    density and orbitals are left as they came in, as in the reference)."""
+    compute_energy = energy_predictor(fxc, precision=precision, **kwargs)
+
+    def neural_iterator(params: Optional[torch.Tensor], molecule: Molecule) -> Molecule:
+        with torch.no_grad():
+            predicted_e, _ = compute_energy(params, molecule)
+            D = whitening_matrix(molecule.s1e, whitening)
+            W = torch.einsum("sji,jk->sik", molecule.mo_coeff, torch.linalg.inv(D))
+        opt_state = tx.init(W)
+        e, old_e, cycle = float(predicted_e), float("inf"), 0
+        while abs(e - old_e) * HARTREE2KCALMOL > e_conv and cycle < cycles:
+            old_e = e
+            predicted_e, W, opt_state = _orbital_step(compute_energy, params, molecule, W, D, tx, opt_state)
+            e = float(predicted_e)
+            cycle += 1
+            if verbose > 1:
+                print(f"cycle: {cycle}, predicted energy: {e:.7e}, energy difference: {abs(e - old_e):.4e}")
+            molecule = molecule.replace(energy=predicted_e)
+        molecule.cache["orb_cycles"] = cycle
+        return molecule
+
+    return neural_iterator
+
+
+def jitted_mol_orb_optimizer(functional: Functional, tx, cycles: int = 500, precision: str = "fp64",
+                             **kwargs) -> Callable:
+    """``jitted_mol_orb_optimizer`` (``evaluate.py:783-914``): PCA whitening and exactly
+    ``cycles`` optimizer steps; returns the molecule with the energy of the last step."""
+    compute_energy = energy_predictor(functional, precision=precision, **kwargs)
+
+    def neural_iterator(params: Optional[torch.Tensor], molecule: Molecule) -> Molecule:
+        with torch.no_grad():
+            predicted_e, _ = compute_energy(params, molecule)
+            D = whitening_matrix(molecule.s1e, "PCA")
+            W = torch.einsum("sji,jk->sik", molecule.mo_coeff, torch.linalg.inv(D))
+        opt_state = tx.init(W)
+        for _ in range(cycles):
+            predicted_e, W, opt_state = _orbital_step(compute_energy, params, molecule, W, D, tx, opt_state)
+        return molecule.replace(energy=predicted_e)
+
+    return neural_iterator

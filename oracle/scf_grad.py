@@ -214,3 +214,72 @@ def diff_simple_scf_energy(xc_energy: Callable, inp: TorchSCFInputs, rdm1: torch
         D = (1.0 - mixing_factor) * D + mixing_factor * Dnew
         e, fock = predict(xc_energy, inp, D)
     return e
+
+
+# ---------------------------------------------------------------------------------------
+# Orbital optimizers (grad_dft/evaluate.py:595-914): the energy as a function of a matrix W
+# whose QR factor Q gives orthonormal orbitals C = (Q D)^T, minimised by Adam.
+# ---------------------------------------------------------------------------------------
+
+def whitening_matrix(S: np.ndarray, whitening: str = "PCA") -> np.ndarray:
+    """evaluate.py:709-723: D with D^T D = S^-1.  PCA diag(w^-1/2) V^T, Cholesky chol(S^-1)^T,
+    ZCA V diag(w^-1/2) V^T; eigenvector signs fixed so each eigenvector's largest-magnitude
+    component is positive."""
+    if whitening == "Cholesky":
+        return np.linalg.cholesky(np.linalg.inv(S)).T
+    w, V = np.linalg.eigh(S)
+    idx = np.argmax(np.abs(V), axis=0)
+    V = V * np.sign(V[idx, np.arange(V.shape[1])])[None, :]
+    if whitening == "PCA":
+        return np.diag(w ** -0.5) @ V.T
+    if whitening == "ZCA":
+        return V @ np.diag(w ** -0.5) @ V.T
+    raise ValueError(whitening)
+
+
+def energy_only(xc_energy: Callable, inp: TorchSCFInputs, rdm1: torch.Tensor) -> torch.Tensor:
+    """train.py:123-216's energy (no Fock matrix)."""
+    Dt = rdm1.sum(0)
+    J = torch.einsum("pqrt,rt->pq", inp.eri, Dt)
+    e = xc_energy(rdm1) + inp.e_nuc + (Dt * inp.h1e).sum() + (Dt * J).sum() / 2.0
+    if inp.K0 is not None:
+        e = e - 0.5 * inp.a0 * (rdm1 * inp.K0).sum()
+    return e
+
+
+def orbital_energy(xc_energy: Callable, inp: TorchSCFInputs, W: torch.Tensor, D: torch.Tensor,
+                   mo_occ: torch.Tensor) -> torch.Tensor:
+    """molecule_orbitals_iterator (evaluate.py:646-688): Q = qr(W_s), C_s = (Q_s D)^T,
+    rdm1 = C diag(occ) C^T, predicted energy."""
+    Q = torch.stack([torch.linalg.qr(W[s])[0] for s in range(2)])
+    C = torch.einsum("sij,jk->ski", Q, D)
+    rdm1 = torch.einsum("sij,sj,skj->sik", C, mo_occ, C)
+    return energy_only(xc_energy, inp, rdm1)
+
+
+def mol_orb_optimizer(xc_energy: Callable, inp: TorchSCFInputs, mo_coeff: torch.Tensor, mo_occ: torch.Tensor,
+                      lr: float, cycles: int, e_conv: Optional[float] = 1e-7, whitening: str = "PCA",
+                      b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
+    """neural_iterator of evaluate.py:690-777 (e_conv given) or of jitted_mol_orb_optimizer
+    (evaluate.py:866-912, e_conv None: a fixed number of cycles), with optax.adam(lr).
+    Returns the list of predicted energies, one per cycle (the last is the returned one)."""
+    D = torch.tensor(whitening_matrix(inp.s1e.numpy(), whitening), dtype=DT)
+    W = torch.einsum("sji,jk->sik", mo_coeff, torch.linalg.inv(D))
+    mu = torch.zeros_like(W)
+    nu = torch.zeros_like(W)
+    out = []
+    old_e = float("inf")
+    e = float(energy_only(xc_energy, inp, torch.einsum("sij,sj,skj->sik", mo_coeff, mo_occ, mo_coeff)))
+    for t in range(1, cycles + 1):
+        if e_conv is not None and not abs(e - old_e) * 627.50947 > e_conv:
+            break
+        old_e = e
+        Wg = W.detach().requires_grad_(True)
+        E = orbital_energy(xc_energy, inp, Wg, D, mo_occ)
+        g, = torch.autograd.grad(E, [Wg])
+        e = float(E.detach())
+        out.append(e)
+        mu = b1 * mu + (1 - b1) * g
+        nu = b2 * nu + (1 - b2) * g * g
+        W = W - lr * (mu / (1 - b1 ** t)) / (torch.sqrt(nu / (1 - b2 ** t)) + eps)
+    return out

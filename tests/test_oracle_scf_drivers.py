@@ -230,3 +230,65 @@ def test_simple_scf_loop_first_cycle_and_fixed_point(water):
     st = O.simple_scf_loop(water["xc"], water["inp"], water["rdm1"], water["C"], water["e"], water["occ"],
                            mixing_factor=0.5, cycles=40)
     assert abs(st.energy - water["E"]) < 1e-6, (st.energy, water["E"])
+
+
+@pytest.fixture(scope="module")
+def water_sto3g():
+    import torch
+    from graddft_amd.interface.basis import build_basis, nuclear_repulsion
+    from graddft_amd.interface.builder import core_guess
+    from graddft_amd.interface.grids import molecular_grid
+    from graddft_amd.interface.integrals import int1e, int2e_packed, unpack_eri
+    from oracle import scf_grad as G
+    from oracle import xc as X
+    from oracle.ao import eval_ao
+    xyz = np.array([[0, 0, 0], [1.81, 0, 0], [1.81 * math.cos(1.824), 1.81 * math.sin(1.824), 0]])
+    bs = build_basis(["O", "H", "H"], xyz, "sto-3g")
+    S, T, V = int1e(bs, [8, 1, 1], xyz)
+    eri = unpack_eri(int2e_packed(bs), bs.nao)
+    D, C, e, occ = core_guess(T + V, S, 5)
+    rdm1 = np.stack([D, D])
+    K0 = np.stack([O.exchange_potential(rdm1[s], eri) for s in range(2)])
+    inp = O.SCFInputs(T + V, S, eri, nuclear_repulsion([8, 1, 1], xyz), K0, 0.2)
+    c, w = molecular_grid([8, 1, 1], xyz, 1)
+    xin = X.to_inputs(*eval_ao(bs, c), w)
+    xcv = lambda d: X.xc_value_and_grads("b3lyp", xin, d)[:2]  # noqa: E731
+    E_scf = O.diff_scf_loop(xcv, inp, rdm1, np.stack([occ, occ]), 25).energy
+    return dict(inp=G.TorchSCFInputs.from_numpy(inp), S=S, E_scf=E_scf,
+                xce=lambda d: X.xc_energy_from_rdm1("b3lyp", xin, d),
+                C=torch.tensor(np.stack([C, C])), occ=torch.tensor(np.stack([occ, occ])))
+
+
+@pytest.mark.parametrize("kind", ["PCA", "Cholesky", "ZCA"])
+def test_whitening_matrices(water_sto3g, kind):
+    from oracle import scf_grad as G
+    S = water_sto3g["S"]
+    D = G.whitening_matrix(S, kind)
+    assert np.allclose(D.T @ D, np.linalg.inv(S), atol=1e-10)
+    assert np.allclose(D @ S @ D.T, np.eye(len(S)), atol=1e-10)       # orbitals (Q D)^T are S-orthonormal
+
+
+def test_orbital_energy_gradient_and_variational_bound(water_sto3g):
+    """dE/dW through the QR factor against central differences; every optimizer energy is the
+    energy of an idempotent density, so none lies below the self-consistent minimum, and the
+    optimizer descends."""
+    import torch
+    from oracle import scf_grad as G
+    w = water_sto3g
+    D = torch.tensor(G.whitening_matrix(w["S"], "PCA"))
+    W = torch.einsum("sji,jk->sik", w["C"], torch.linalg.inv(D))
+    # a generic point (no Householder pivot near zero, which flips a QR column sign), the same
+    # for both spins (zeta = 0: the spin-polarisation term's log2(1 -+ zeta) stays smooth)
+    P0 = torch.tensor(np.random.default_rng(1).normal(size=W.shape[1:]))
+    W = W + 0.01 * torch.stack([P0, P0])
+    Wg = W.clone().requires_grad_(True)
+    E = G.orbital_energy(w["xce"], w["inp"], Wg, D, w["occ"])
+    g, = torch.autograd.grad(E, [Wg])
+    P = torch.tensor(np.random.default_rng(1).normal(size=W.shape))
+    h = 1e-5
+    fd = (G.orbital_energy(w["xce"], w["inp"], W + h * P, D, w["occ"])
+          - G.orbital_energy(w["xce"], w["inp"], W - h * P, D, w["occ"])) / (2 * h)
+    assert float((g * P).sum()) == pytest.approx(float(fd), rel=1e-6)
+    es = G.mol_orb_optimizer(w["xce"], w["inp"], w["C"], w["occ"], 0.01, 40, e_conv=None)
+    assert min(es) >= w["E_scf"] - 1e-8
+    assert es[-1] < es[0] - 0.1

@@ -29,7 +29,7 @@ def _oracle_inputs(mol):
     K0 = np.stack([exchange_potential(D0[s], eri) for s in range(2)])
     inp = SCFInputs(mol.h1e.cpu().numpy(), mol.s1e.cpu().numpy(), eri, mol.nuclear_repulsion, K0, 0.2)
     xin = to_inputs(*eval_ao(mol.basis, mol.grid.coords.cpu().numpy()), mol.grid.weights.cpu().numpy())
-    return inp, (lambda d: xc_value_and_grads("b3lyp", xin, d)[:2])
+    return inp, (lambda d: xc_value_and_grads("b3lyp", xin, d)[:2]), xin
 
 
 def _sym(rng, *shape):
@@ -86,7 +86,7 @@ def test_scf_loop_matches_oracle(opts, tol):
     from oracle import scf as O
     mol = _water()
     out = scf_loop(B3LYP, **opts)(None, mol)
-    inp, xc = _oracle_inputs(mol)
+    inp, xc, _ = _oracle_inputs(mol)
     st, cycles = O.scf_loop(xc, inp, mol.rdm1.cpu().numpy(), mol.mo_occ.cpu().numpy(), **opts)
     assert abs(float(out.energy) - st.energy) <= tol, (float(out.energy), st.energy)
     assert abs(out.cache["scf_cycles"] - cycles) <= (0 if tol <= 1e-8 else 1)
@@ -100,7 +100,7 @@ def test_simple_scf_loop_matches_oracle():
     from oracle import scf as O
     mol = _water()
     out = simple_scf_loop(B3LYP, cycles=10, mixing_factor=0.4)(None, mol)
-    inp, xc = _oracle_inputs(mol)
+    inp, xc, _ = _oracle_inputs(mol)
     st = O.simple_scf_loop(xc, inp, mol.rdm1.cpu().numpy(), mol.mo_coeff.cpu().numpy(),
                            mol.mo_energy.cpu().numpy(), mol.mo_occ.cpu().numpy(), mixing_factor=0.4, cycles=10)
     assert abs(float(out.energy) - st.energy) <= 1e-9, (float(out.energy), st.energy)
@@ -115,3 +115,38 @@ def test_charge_check_rejects_a_wrong_density():
     bad = mol.replace(rdm1=mol.rdm1 * 1.1)
     with pytest.raises(AssertionError, match="Total charge is not conserved"):
         simple_scf_loop(B3LYP, cycles=2)(None, bad)
+
+
+def _orbital_oracle(mol, lr, cycles, e_conv, whitening="PCA"):
+    from oracle import scf_grad as G
+    from oracle import xc as X
+    inp, _, xin = _oracle_inputs(mol)
+    ti = G.TorchSCFInputs.from_numpy(inp)
+    C = torch.tensor(mol.mo_coeff.cpu().numpy())
+    occ = torch.tensor(mol.mo_occ.cpu().numpy())
+    return G.mol_orb_optimizer(lambda d: X.xc_energy_from_rdm1("b3lyp", xin, d), ti, C, occ, lr, cycles,
+                               e_conv=e_conv, whitening=whitening)
+
+
+@pytest.mark.parametrize("whitening", ["PCA", "ZCA", "Cholesky"])
+def test_mol_orb_optimizer_matches_oracle(whitening):
+    _say(f"case: mol_orb_optimizer {whitening}, adam 0.01, 12 cycles, B3LYP water/def2-svp level 1")
+    from graddft_amd import B3LYP, adam
+    from graddft_amd.evaluate import mol_orb_optimizer
+    mol = _water()
+    out = mol_orb_optimizer(B3LYP, adam(0.01), cycles=12, e_conv=1e-7, whitening=whitening)(None, mol)
+    es = _orbital_oracle(mol, 0.01, 12, 1e-7, whitening)
+    assert out.cache["orb_cycles"] == len(es)
+    assert abs(float(out.energy) - es[-1]) <= 1e-8, (float(out.energy), es[-1])
+    assert es[-1] < es[0]
+
+
+def test_jitted_mol_orb_optimizer_matches_oracle():
+    _say("case: jitted_mol_orb_optimizer, adam 0.01, 8 cycles, B3LYP water/def2-svp level 1")
+    from graddft_amd import B3LYP, adam
+    from graddft_amd.evaluate import jitted_mol_orb_optimizer
+    mol = _water()
+    out = jitted_mol_orb_optimizer(B3LYP, adam(0.01), cycles=8)(None, mol)
+    es = _orbital_oracle(mol, 0.01, 8, None)
+    assert len(es) == 8
+    assert abs(float(out.energy) - es[-1]) <= 1e-8, (float(out.energy), es[-1])
