@@ -28,12 +28,14 @@ namespace gdx {
 
 constexpr int DN_PT = 64, DN_K = 16;
 
-// Column chunk of the density kernel: 128 output columns per pass for FP32, 64 for FP64
-// (accumulator registers: 4 components x CB/16 blocks of 16 x 16 per wave).
-template <typename T> struct DensCB { static constexpr int value = sizeof(T) == 4 ? 128 : 64; };
+// Column chunk of the density kernel: 64 output columns per pass (accumulator registers:
+// 4 components x CB/16 blocks of 16 x 16 per wave).  With 64 columns the FP32 kernel fits
+// two blocks per CU (228 VGPRs, no spills; 15% faster than one block of 128-column chunks on
+// benzene 6-31G*); a 128-column chunk at two blocks per CU spills.
+template <typename T> struct DensCB { static constexpr int value = 64; };
 
 template <typename T>
-__global__ __launch_bounds__(256) void density_ao_kernel(const T* __restrict__ ao, int npts_pad, int nP,
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void density_ao_kernel(const T* __restrict__ ao, int npts_pad, int nP,
                                                          const T* __restrict__ Dp,
                                                          double* __restrict__ dens_part) {
   typedef typename MfmaT<T>::acc_t acc_t;

@@ -139,3 +139,47 @@ def safe_eigh_vjp(evals: torch.Tensor, evecs: torch.Tensor, g_evals: torch.Tenso
     # inv(V^T) of the reference is V: the eigenvectors are orthonormal to rounding
     grad = evecs @ (0.5 * torch.diag(g_evals) + F * (Vt @ g_evecs)) @ Vt
     return grad + grad.T
+
+
+def householder_q(W: torch.Tensor) -> torch.Tensor:
+    """Q of W = QR for a batch of square matrices, by Householder reflections with LAPACK's
+    conventions (dgeqrf / dlarfg / dorgqr), which the reference's ``jnp.linalg.qr`` follows:
+    for column j, beta = -sign(alpha) * ||x|| (sign(0) = +1) with alpha the leading entry of
+    the sub-column x, no reflection when the part of x below alpha is zero (in particular for
+    the last column), Q = H_1 H_2 ... H_n.  The column signs of Q fix which orbitals the
+    orbital optimizer starts from, so they follow LAPACK rather than whatever the device QR
+    library picks.  Plain differentiable tensor ops (autograd gives dQ/dW)."""
+    b, m, n = W.shape
+    if m != n:
+        raise ValueError("householder_q expects square matrices")
+    A = W
+    vs, taus = [], []
+    for j in range(n):
+        x = A[:, j:, j]
+        alpha = x[:, 0]
+        if j == n - 1:
+            vs.append(torch.ones_like(x))
+            taus.append(torch.zeros_like(alpha))
+            break
+        tail = x[:, 1:]
+        xnorm2 = (tail * tail).sum(dim=1)
+        nz = xnorm2 > 0
+        safe2 = torch.where(nz, xnorm2, torch.ones_like(xnorm2))
+        sgn = torch.where(alpha >= 0, torch.ones_like(alpha), -torch.ones_like(alpha))
+        beta = -sgn * torch.sqrt(alpha * alpha + safe2)
+        tau = torch.where(nz, (beta - alpha) / beta, torch.zeros_like(alpha))
+        scale = torch.where(nz, 1.0 / (alpha - beta), torch.zeros_like(alpha))
+        v = torch.cat([torch.ones_like(alpha)[:, None], tail * scale[:, None]], dim=1)
+        sub = A[:, j:, j:]
+        w = tau[:, None] * torch.einsum("bi,bik->bk", v, sub)
+        sub = sub - v[:, :, None] * w[:, None, :]
+        A = torch.cat([A[:, :j, :], torch.cat([A[:, j:, :j], sub], dim=2)], dim=1)
+        vs.append(v)
+        taus.append(tau)
+    Q = torch.eye(m, dtype=W.dtype, device=W.device).expand(b, m, m)
+    for j in reversed(range(n)):
+        v, tau = vs[j], taus[j]
+        sub = Q[:, j:, :]
+        w = tau[:, None] * torch.einsum("bi,bik->bk", v, sub)
+        Q = torch.cat([Q[:, :j, :], sub - v[:, :, None] * w[:, None, :]], dim=1)
+    return Q

@@ -22,7 +22,7 @@ from typing import Callable, List, Optional
 import torch
 
 from . import _lib
-from .eigenproblem import cholesky_inv, eigh, orthogonal_eigh, scf_step
+from .eigenproblem import cholesky_inv, eigh, householder_q, orthogonal_eigh, scf_step
 from .functional import Functional
 from .molecule import Molecule, make_rdm1, orbital_grad
 from .train import energy_predictor
@@ -620,9 +620,9 @@ def whitening_matrix(S: torch.Tensor, whitening: str = "PCA") -> torch.Tensor:
 
 
 def _orbital_energy(compute_energy, params, molecule: Molecule, W: torch.Tensor, D: torch.Tensor):
-    """molecule_orbitals_iterator (``evaluate.py:646-688``): Q_s = qr(W_s), C_s = (Q_s D)^T,
-    rdm1 = make_rdm1(C, mo_occ), predicted energy."""
-    Q = torch.linalg.qr(W)[0]
+    """molecule_orbitals_iterator (``evaluate.py:646-688``): Q_s = qr(W_s) (LAPACK's
+    Householder signs), C_s = (Q_s D)^T, rdm1 = make_rdm1(C, mo_occ), predicted energy."""
+    Q = householder_q(W)
     C = torch.einsum("sij,jk->ski", Q, D)
     rdm1 = make_rdm1(C, molecule.mo_occ)
     e, _ = compute_energy(params, molecule.replace(rdm1=rdm1, mo_coeff=C, restricted=False))

@@ -292,3 +292,23 @@ def test_orbital_energy_gradient_and_variational_bound(water_sto3g):
     es = G.mol_orb_optimizer(w["xce"], w["inp"], w["C"], w["occ"], 0.01, 40, e_conv=None)
     assert min(es) >= w["E_scf"] - 1e-8
     assert es[-1] < es[0] - 0.1
+
+
+def test_householder_q_follows_lapack_signs():
+    """The product's QR (used by the orbital optimizers) reproduces LAPACK's Q, column signs
+    included, and its gradient: random matrices, and orthogonal ones with exact zero pivots
+    (no reflection, the sign is kept)."""
+    import torch
+    from graddft_amd.eigenproblem import householder_q
+    rng = np.random.default_rng(0)
+    for n in (1, 2, 7, 30):
+        W = torch.tensor(rng.normal(size=(2, n, n)))
+        assert (householder_q(W) - torch.linalg.qr(W)[0]).abs().max().item() < 1e-13
+    P = torch.tensor([[0.0, 1, 0, 0], [1, 0, 0, 0], [0, 0, -1, 0], [0, 0, 0, 1]])
+    W = torch.stack([torch.eye(4, dtype=torch.float64), P.double()])
+    assert torch.equal(householder_q(W), torch.linalg.qr(W)[0])
+    W = torch.tensor(rng.normal(size=(2, 6, 6)), requires_grad=True)
+    Pm = torch.tensor(rng.normal(size=(2, 6, 6)))
+    g, = torch.autograd.grad((householder_q(W) * Pm).sum(), [W])
+    g0, = torch.autograd.grad((torch.linalg.qr(W)[0] * Pm).sum(), [W])
+    assert (g - g0).abs().max().item() < 1e-12

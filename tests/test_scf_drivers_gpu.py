@@ -63,7 +63,10 @@ def test_pieces_match_oracle():
     ora = O.Diis(S, np.eye(n), 8, "CDIIS")
     x = dev.cdiis_minimize(t(errs)).cpu().numpy()
     assert np.abs(x - ora.cdiis_minimize(errs)).max() <= 1e-10
-    Ds, Fs, Es = _sym(rng, k, 2, n, n), _sym(rng, k, 2, n, n), rng.normal(size=k)
+    # F_i = H + D_i (a linear Fock map): the EDIIS and ADIIS costs are then convex on the
+    # simplex, so the minimiser is unique and both BFGS runs must find it
+    Ds, Es = _sym(rng, k, 2, n, n), rng.normal(size=k)
+    Fs = _sym(rng, 2, n, n)[None] + Ds
     c, f = dev.ediis_minimize(t(Ds), t(Fs), t(Es))
     c0, f0 = ora.ediis_minimize(Ds, Fs, Es)
     assert abs(f - f0) <= 1e-9 * max(1.0, abs(f0)) and np.abs(c.cpu().numpy() - c0).max() <= 1e-5
@@ -117,6 +120,16 @@ def test_charge_check_rejects_a_wrong_density():
         simple_scf_loop(B3LYP, cycles=2)(None, bad)
 
 
+def _generic_orbitals(mol):
+    """The molecule's orbitals mixed by a fixed random orthogonal matrix: still orthonormal,
+    but with no symmetry zeros, so no Householder pivot of W = C^T D^-1 sits at rounding level
+    (where LAPACK's column sign, and with it the starting orbitals, is decided by rounding)."""
+    n = mol.nao
+    R, _ = np.linalg.qr(np.random.default_rng(3).normal(size=(n, n)))
+    C = mol.mo_coeff.cpu().numpy() @ R
+    return mol.replace(mo_coeff=torch.as_tensor(C, device="cuda"))
+
+
 def _orbital_oracle(mol, lr, cycles, e_conv, whitening="PCA"):
     from oracle import scf_grad as G
     from oracle import xc as X
@@ -133,7 +146,7 @@ def test_mol_orb_optimizer_matches_oracle(whitening):
     _say(f"case: mol_orb_optimizer {whitening}, adam 0.01, 12 cycles, B3LYP water/def2-svp level 1")
     from graddft_amd import B3LYP, adam
     from graddft_amd.evaluate import mol_orb_optimizer
-    mol = _water()
+    mol = _generic_orbitals(_water())
     out = mol_orb_optimizer(B3LYP, adam(0.01), cycles=12, e_conv=1e-7, whitening=whitening)(None, mol)
     es = _orbital_oracle(mol, 0.01, 12, 1e-7, whitening)
     assert out.cache["orb_cycles"] == len(es)
@@ -145,7 +158,7 @@ def test_jitted_mol_orb_optimizer_matches_oracle():
     _say("case: jitted_mol_orb_optimizer, adam 0.01, 8 cycles, B3LYP water/def2-svp level 1")
     from graddft_amd import B3LYP, adam
     from graddft_amd.evaluate import jitted_mol_orb_optimizer
-    mol = _water()
+    mol = _generic_orbitals(_water())
     out = jitted_mol_orb_optimizer(B3LYP, adam(0.01), cycles=8)(None, mol)
     es = _orbital_oracle(mol, 0.01, 8, None)
     assert len(es) == 8
