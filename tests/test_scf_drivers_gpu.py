@@ -163,3 +163,26 @@ def test_jitted_mol_orb_optimizer_matches_oracle():
     es = _orbital_oracle(mol, 0.01, 8, None)
     assert len(es) == 8
     assert abs(float(out.energy) - es[-1]) <= 1e-8, (float(out.energy), es[-1])
+
+
+def test_scf_loop_neural_functional_matches_oracle():
+    _say("case: scf_loop CDIIS + level shift 0.1, neural B3LYP MLP h=32, water/def2-svp level 1")
+    from graddft_amd import MLPSpec, NeuralFunctional
+    from graddft_amd.evaluate import scf_loop
+    from graddft_amd.interface.integrals import unpack_eri
+    from oracle import scf as O
+    from oracle.ao import eval_ao
+    from oracle.xc import MLPSpec as OSpec
+    from oracle.xc import to_inputs, xc_value_and_grads
+    mol = _water()
+    f = NeuralFunctional(MLPSpec(nc=6, h=32, n_hidden=2, nd=4))
+    theta = f.init(seed=2)
+    out = scf_loop(f, level_shift_factor=(0.1, 0.1))(theta, mol)
+    eri = unpack_eri(mol.rep_tensor.cpu().numpy(), mol.nao)
+    inp = O.SCFInputs(mol.h1e.cpu().numpy(), mol.s1e.cpu().numpy(), eri, mol.nuclear_repulsion, None, 0.0)
+    xin = to_inputs(*eval_ao(mol.basis, mol.grid.coords.cpu().numpy()), mol.grid.weights.cpu().numpy())
+    th, spec = theta.cpu().numpy(), OSpec(6, 32, 2, 4)
+    st, cycles = O.scf_loop(lambda d: xc_value_and_grads("neural_b3lyp", xin, d, th, spec)[:2], inp,
+                            mol.rdm1.cpu().numpy(), mol.mo_occ.cpu().numpy(), level_shift_factor=(0.1, 0.1))
+    assert abs(float(out.energy) - st.energy) <= 1e-8, (float(out.energy), st.energy)
+    assert out.cache["scf_cycles"] == cycles
