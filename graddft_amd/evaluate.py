@@ -431,7 +431,7 @@ class Diis:
 
 
 def smearing_occupations(mo_energy: torch.Tensor, nelectron: float, smearing: str, sigma: float) -> torch.Tensor:
-    """Fractional occupations of ``scf_loop``'s smearing (``evaluate.py:471-497``): one chemical
+    """Fractional occupations of ``scf_loop``'s smearing (``evaluate.py:467-497``): one chemical
     potential mu for all spin orbitals, found by bisection (xtol = rtol = 1e-10) between the
     lowest and highest orbital energy so that the occupations sum to nelectron; "gaussian":
     erfc((e - mu) / sigma) / 2, "fermi-dirac": 1 / (exp((e - mu) / sigma) + 1)."""
@@ -602,7 +602,7 @@ def simple_scf_loop(functional: Functional, mixing_factor: float = 0.4, chunk_si
 # ---------------------------------------------------------------------------------------
 
 def whitening_matrix(S: torch.Tensor, whitening: str = "PCA") -> torch.Tensor:
-    """D with D^T D = S^-1 (``evaluate.py:709-723``): "PCA" diag(w^-1/2) V^T, "Cholesky"
+    """D with D^T D = S^-1 (``evaluate.py:729-740``): "PCA" diag(w^-1/2) V^T, "Cholesky"
     chol(S^-1)^T, "ZCA" V diag(w^-1/2) V^T.  Eigenvector signs are fixed so that each
     eigenvector's largest-magnitude component is positive (LAPACK leaves them arbitrary; the
     starting orbitals after the QR step depend on them)."""
@@ -620,7 +620,7 @@ def whitening_matrix(S: torch.Tensor, whitening: str = "PCA") -> torch.Tensor:
 
 
 def _orbital_energy(compute_energy, params, molecule: Molecule, W: torch.Tensor, D: torch.Tensor):
-    """molecule_orbitals_iterator (``evaluate.py:646-688``): Q_s = qr(W_s) (LAPACK's
+    """molecule_orbitals_iterator (``evaluate.py:642-699``): Q_s = qr(W_s) (LAPACK's
     Householder signs), C_s = (Q_s D)^T, rdm1 = make_rdm1(C, mo_occ), predicted energy."""
     Q = householder_q(W)
     C = torch.einsum("sij,jk->ski", Q, D)

@@ -328,7 +328,7 @@ class Diis:
 
 
 def smearing_occ(mo_energy, nelectron, smearing, sigma):
-    """evaluate.py:471-497 with one chemical potential over both spins' orbitals, (2, n)."""
+    """evaluate.py:467-497 with one chemical potential over both spins' orbitals, (2, n)."""
     from scipy.optimize import bisect
     from scipy.special import erfc
     es = np.hstack(mo_energy)

@@ -222,7 +222,7 @@ def diff_simple_scf_energy(xc_energy: Callable, inp: TorchSCFInputs, rdm1: torch
 # ---------------------------------------------------------------------------------------
 
 def whitening_matrix(S: np.ndarray, whitening: str = "PCA") -> np.ndarray:
-    """evaluate.py:709-723: D with D^T D = S^-1.  PCA diag(w^-1/2) V^T, Cholesky chol(S^-1)^T,
+    """evaluate.py:729-740: D with D^T D = S^-1.  PCA diag(w^-1/2) V^T, Cholesky chol(S^-1)^T,
     ZCA V diag(w^-1/2) V^T; eigenvector signs fixed so each eigenvector's largest-magnitude
     component is positive."""
     if whitening == "Cholesky":
@@ -249,7 +249,7 @@ def energy_only(xc_energy: Callable, inp: TorchSCFInputs, rdm1: torch.Tensor) ->
 
 def orbital_energy(xc_energy: Callable, inp: TorchSCFInputs, W: torch.Tensor, D: torch.Tensor,
                    mo_occ: torch.Tensor) -> torch.Tensor:
-    """molecule_orbitals_iterator (evaluate.py:646-688): Q = qr(W_s), C_s = (Q_s D)^T,
+    """molecule_orbitals_iterator (evaluate.py:642-699): Q = qr(W_s), C_s = (Q_s D)^T,
     rdm1 = C diag(occ) C^T, predicted energy."""
     Q = torch.stack([torch.linalg.qr(W[s])[0] for s in range(2)])
     C = torch.einsum("sij,jk->ski", Q, D)
@@ -260,8 +260,8 @@ def orbital_energy(xc_energy: Callable, inp: TorchSCFInputs, W: torch.Tensor, D:
 def mol_orb_optimizer(xc_energy: Callable, inp: TorchSCFInputs, mo_coeff: torch.Tensor, mo_occ: torch.Tensor,
                       lr: float, cycles: int, e_conv: Optional[float] = 1e-7, whitening: str = "PCA",
                       b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
-    """neural_iterator of evaluate.py:690-777 (e_conv given) or of jitted_mol_orb_optimizer
-    (evaluate.py:866-912, e_conv None: a fixed number of cycles), with optax.adam(lr).
+    """neural_iterator of evaluate.py:701-775 (e_conv given) or of jitted_mol_orb_optimizer
+    (evaluate.py:871-912, e_conv None: a fixed number of cycles), with optax.adam(lr).
     Returns the list of predicted energies, one per cycle (the last is the returned one)."""
     D = torch.tensor(whitening_matrix(inp.s1e.numpy(), whitening), dtype=DT)
     W = torch.einsum("sji,jk->sik", mo_coeff, torch.linalg.inv(D))
