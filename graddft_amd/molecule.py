@@ -100,6 +100,43 @@ class Molecule:
     def nelec(self) -> torch.Tensor:
         return self.mo_occ.sum(dim=1)
 
+    # -- grid densities and energy terms (molecule.py:108-151, 298-300) ------------------
+    # Values from the device kernels (the density kernel of the XC path, the packed-ERI
+    # Coulomb kernel); not on the autograd tape -- differentiable energies go through
+    # Functional.energy / energy_predictor.
+    def _grid_densities(self) -> torch.Tensor:
+        from .popular_functionals import B3LYP_GRID
+        with torch.no_grad():
+            return B3LYP_GRID.plan(self).densities(self.rdm1.detach())
+
+    def density(self) -> torch.Tensor:
+        """rho per spin at each grid point, (N, 2)."""
+        return self._grid_densities()[:, :, 0]
+
+    def grad_density(self) -> torch.Tensor:
+        """grad rho per spin, (N, 2, 3)."""
+        return self._grid_densities()[:, :, 1:4]
+
+    def kinetic_density(self) -> torch.Tensor:
+        """tau = 1/2 sum_ab D_ab grad phi_a . grad phi_b per spin, (N, 2)."""
+        return self._grid_densities()[:, :, 4]
+
+    def lapl_density(self) -> torch.Tensor:
+        """Laplacian of rho per spin, (N, 2)."""
+        return self._grid_densities()[:, :, 5]
+
+    def get_coulomb_potential(self) -> torch.Tensor:
+        """J of the total density from the packed ERI, (n, n)."""
+        from .scf import coulomb
+        with torch.no_grad():
+            return coulomb(self, self.rdm1.detach())
+
+    def nonXC(self) -> torch.Tensor:  # noqa: N802  (reference name)
+        """E_nuc + tr(D h) + 1/2 tr(D J) of the total density."""
+        from .scf import non_xc_energy
+        with torch.no_grad():
+            return non_xc_energy(self, self.rdm1.detach())
+
     # -- orbital helpers (molecule.py:302-330) ------------------------------------------
     def make_rdm1(self) -> torch.Tensor:
         return make_rdm1(self.mo_coeff, self.mo_occ)
