@@ -1,0 +1,75 @@
+"""CPU tests of the host-side logic of the non-differentiable drivers in graddft_amd.evaluate
+and graddft_amd.molecule (pure tensor code, no device library): damping, level shift,
+orbital gradient, aufbau occupations, smearing, the Diis history and the EDIIS / ADIIS
+weights, each against the oracle restatement (oracle/scf.py) on the same inputs."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import scf as O
+
+
+def _sym(rng, *shape):
+    a = rng.normal(size=shape)
+    return a + np.swapaxes(a, -1, -2)
+
+
+def test_damping_level_shift_orbital_grad_get_occ():
+    from graddft_amd.evaluate import damping, level_shift
+    from graddft_amd.molecule import get_occ, orbital_grad
+    rng = np.random.default_rng(0)
+    n = 8
+    A = rng.normal(size=(n, n))
+    S = A @ A.T / n + np.eye(n)
+    D, F = _sym(rng, n, n), _sym(rng, n, n)
+    t = torch.as_tensor
+    assert np.abs(damping(t(S), t(D), t(F), 0.4).numpy() - O.damping(S, D, F, 0.4)).max() <= 1e-12
+    assert np.abs(level_shift(t(S), t(D), t(F), 0.3).numpy() - O.level_shift(S, D, F, 0.3)).max() <= 1e-12
+    e = rng.normal(size=(2, n))
+    e[0, 3] = e[0, 5]                                       # a tie: index order decides
+    occ = get_occ(t(e), [4, 2]).numpy()
+    assert np.array_equal(occ, O.get_occ(e, np.array([4, 2])))
+    C, F2 = rng.normal(size=(2, n, n)), _sym(rng, 2, n, n)
+    g = orbital_grad(t(C), t(occ), t(F2)).numpy()
+    assert np.abs(g - O.orbital_grad(C, occ, F2)).max() <= 1e-12
+
+
+@pytest.mark.parametrize("kind", ["gaussian", "fermi-dirac"])
+def test_smearing_occupations_match_oracle(kind):
+    from graddft_amd.evaluate import smearing_occupations
+    rng = np.random.default_rng(1)
+    e = np.sort(rng.normal(size=(2, 9)), axis=1)
+    occ = smearing_occupations(torch.as_tensor(e), 7.0, kind, 0.1).numpy()
+    assert np.abs(occ - O.smearing_occ(e, 7.0, kind, 0.1)).max() <= 1e-9
+    with pytest.raises(ValueError):
+        smearing_occupations(torch.as_tensor(e), 7.0, "cold", 0.1)
+
+
+def test_diis_history_and_ediis_adiis_weights():
+    from graddft_amd.evaluate import Diis
+    rng = np.random.default_rng(2)
+    n, k = 5, 4
+    S = np.eye(n) + 0.05 * _sym(rng, n, n)
+    dev = Diis(torch.as_tensor(S), None, 3, "EDIIS")
+    data = dev.empty()
+    focks = []
+    for c in range(k):
+        Dk, Fk = _sym(rng, 2, n, n), _sym(rng, 2, n, n)
+        focks.append(Fk)
+        data = dev.update((torch.as_tensor(Dk), torch.as_tensor(Fk), torch.tensor(-1.0 - c)), data)
+    assert data[1].shape[0] == 3 and np.array_equal(data[1][0].numpy(), focks[1])
+    assert data[2].tolist() == [-2.0, -3.0, -4.0]
+    # F_i = H + D_i makes both costs convex on the simplex: one minimiser
+    Ds = _sym(rng, k, 2, n, n)
+    Fs = _sym(rng, 2, n, n)[None] + Ds
+    Es = rng.normal(size=k)
+    ora = O.Diis(S, np.eye(n), 8, "EDIIS")
+    c, f = dev.ediis_minimize(torch.as_tensor(Ds), torch.as_tensor(Fs), torch.as_tensor(Es))
+    c0, f0 = ora.ediis_minimize(Ds, Fs, Es)
+    assert abs(f - f0) <= 1e-9 * max(1.0, abs(f0)) and np.abs(c.numpy() - c0).max() <= 1e-5
+    c, f = dev.adiis_minimize(torch.as_tensor(Ds), torch.as_tensor(Fs), 2)
+    c0, f0 = ora.adiis_minimize(Ds, Fs, 2)
+    assert abs(f - f0) <= 1e-9 * max(1.0, abs(f0)) and np.abs(c.numpy() - c0).max() <= 1e-5
+    with pytest.raises(ValueError):
+        Diis(torch.as_tensor(S), None, 3, "DIIS").run((torch.as_tensor(Ds[0]), torch.as_tensor(Fs[0]), -1.0),
+                                                      dev.empty(), 0)
