@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--no-scf", action="store_true", help="skip the ms/SCF-iteration measurement")
     ap.add_argument("--scf-cycles", type=int, default=15)
     ap.add_argument("--no-train", action="store_true", help="skip the training-step measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the concurrent small-molecule batch")
     ap.add_argument("--no-graph", action="store_true", help="time the eager step instead of graph replays")
     ap.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (calibrates HBM byte counters)")
@@ -144,6 +145,44 @@ def scf_bench(args):
     if args.precision != "fp64":
         out["abs_dE_vs_fp64_Ha"] = abs(out[args.precision]["energy"] - out["fp64"]["energy"])
     return out
+
+
+def batch_bench(args):
+    """A batch of small molecules evaluated concurrently: CH4, C2H6, C3H8, C4H10 (seeded
+    geometries) def2-SVP, level-1 grids, B3LYP with exact exchange, diff_scf_loop of
+    args.scf_cycles cycles; the molecules' captured loops replayed one after another on one
+    stream vs batched_scf_loop (replays round-robin on 4 streams).  Mean of 5 batches each."""
+    import torch
+    from graddft_amd import B3LYP, batched_scf_loop, diff_scf_loop
+    from graddft_amd.interface import geometry
+    from graddft_amd.interface.builder import molecule_from_geometry
+    rng = np.random.default_rng(args.seed)
+    mols = []
+    for nc in (1, 2, 3, 4):
+        sym, xyz = geometry.alkane(rng, nc)
+        mols.append(molecule_from_geometry(sym, xyz, "def2-svp", 1, with_eri=True, name=f"C{nc}"))
+    prec = args.precision
+    seq = diff_scf_loop(B3LYP, cycles=args.scf_cycles, precision=prec, graph=True)
+    bat = batched_scf_loop(B3LYP, cycles=args.scf_cycles, precision=prec, n_streams=4)
+    bat(None, mols)                       # eager runs + captures
+    bat(None, mols)
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        seq_out = [seq(None, m) for m in mols]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        bat_out = bat(None, mols)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    dE = max(abs(float(a.energy) - float(b.energy)) for a, b in zip(seq_out, bat_out))
+    ms_seq, ms_bat = (t1 - t0) / reps * 1e3, (t2 - t1) / reps * 1e3
+    return {"workload": "CH4, C2H6, C3H8, C4H10 def2-SVP (nao " + ", ".join(str(m.nao) for m in mols)
+                        + f"), level-1 grids, B3LYP, diff_scf_loop {args.scf_cycles} cycles, {prec}",
+            "ms_per_batch_sequential_graphs": round(ms_seq, 3), "ms_per_batch_4_streams": round(ms_bat, 3),
+            "speedup": round(ms_seq / ms_bat, 3), "max_abs_dE_Ha": dE}
 
 
 def train_bench(args):
@@ -353,6 +392,7 @@ def main():
 
     scf = None if args.no_scf else scf_bench(args)
     train = None if args.no_train else train_bench(args)
+    batch = None if (args.no_batch or args.no_scf) else batch_bench(args)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -392,6 +432,7 @@ def main():
         "gate_1e-6_Ha": None if vs64 is None else vs64["abs_dE_Ha"] <= 1e-6,
         "scf": scf,
         "train_step": train,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -15,14 +15,15 @@ from .interface.builder import molecule_from_geometry
 from .interface.store import load_molecules, save_molecules
 from .train import (energy_predictor, simple_energy_loss, mse_energy_loss, train_kernel, adam,
                     apply_updates)
-from .evaluate import (Diis, JittableDiis, damping, diff_scf_loop, diff_simple_scf_loop, jitted_mol_orb_optimizer,
-                       level_shift, mol_orb_optimizer, non_scf_predictor, scf_loop, simple_scf_loop)
+from .evaluate import (Diis, JittableDiis, batched_scf_loop, damping, diff_scf_loop, diff_simple_scf_loop,
+                       jitted_mol_orb_optimizer, level_shift, mol_orb_optimizer, non_scf_predictor, scf_loop,
+                       simple_scf_loop)
 from .eigenproblem import safe_eigh_vjp, safe_fock_solver, safe_general_eigh
 
 __all__ = ["Grid", "Molecule", "Functional", "NeuralFunctional", "B3LYPNeuralCorrection", "MLPSpec", "XCPlan",
            "XCEnergyFn", "XCEnergyVxcFn", "B3LYP", "B3LYP_GRID", "molecule_from_geometry",
            "energy_predictor", "non_scf_predictor", "simple_energy_loss", "mse_energy_loss",
-           "train_kernel", "adam", "apply_updates", "JittableDiis", "Diis", "diff_scf_loop", "diff_simple_scf_loop",
+           "train_kernel", "adam", "apply_updates", "JittableDiis", "Diis", "diff_scf_loop", "batched_scf_loop", "diff_simple_scf_loop",
            "scf_loop", "simple_scf_loop", "damping", "level_shift", "mol_orb_optimizer",
            "jitted_mol_orb_optimizer", "safe_eigh_vjp",
            "safe_fock_solver", "safe_general_eigh", "save_molecules", "load_molecules"]

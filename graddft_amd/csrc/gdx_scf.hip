@@ -287,6 +287,9 @@ struct JacobiShared {
 // is the same eigendecomposition (eigenvalues, and eigenvectors up to sign).  On entry T
 // holds the matrix; A, V, T are n x n buffers (LDS or workspace) and the two products read
 // no global memory in their inner loops.  On exit A holds the (rotated) matrix, V the start.
+// V is kept transposed (V[j * n + i] = V_ij, column j contiguous) from here to jacobi_sort, so
+// the rotation of a column pair touches two contiguous rows of the buffer (coalesced in the
+// workspace path; consecutive banks in LDS).
 template <int NTH>
 __device__ void jacobi_start(double* A, double* V, double* T, int n, const double* __restrict__ V0) {
   const int tid = threadIdx.x;
@@ -299,21 +302,21 @@ __device__ void jacobi_start(double* A, double* V, double* T, int n, const doubl
     __syncthreads();
     return;
   }
-  for (size_t e = tid; e < nn; e += NTH) V[e] = V0[e];
+  for (size_t e = tid; e < nn; e += NTH) V[e] = V0[(e % n) * n + e / n];   // V0 transposed
   __syncthreads();
-  // A <- T V0
+  // A <- T V0   (V0_kj = V[j * n + k])
   for (size_t e = tid; e < nn; e += NTH) {
     const int i = (int)(e / n), j = (int)(e % n);
     double t = 0.0;
-    for (int k = 0; k < n; ++k) t += T[(size_t)i * n + k] * V[(size_t)k * n + j];
+    for (int k = 0; k < n; ++k) t += T[(size_t)i * n + k] * V[(size_t)j * n + k];
     A[e] = t;
   }
   __syncthreads();
-  // T <- V0^T A
+  // T <- V0^T A   (V0_ki = V[i * n + k])
   for (size_t e = tid; e < nn; e += NTH) {
     const int i = (int)(e / n), j = (int)(e % n);
     double t = 0.0;
-    for (int k = 0; k < n; ++k) t += V[(size_t)k * n + i] * A[(size_t)k * n + j];
+    for (int k = 0; k < n; ++k) t += V[(size_t)i * n + k] * A[(size_t)k * n + j];
     T[e] = t;
   }
   __syncthreads();
@@ -331,7 +334,8 @@ __device__ void jacobi_start(double* A, double* V, double* T, int n, const doubl
   __syncthreads();
 }
 
-// Cyclic Jacobi sweeps on A (symmetric, n x n) accumulating V <- V J; returns the sweeps run.
+// Cyclic Jacobi sweeps on A (symmetric, n x n) accumulating V <- V J (V stored transposed);
+// returns the sweeps run.
 template <int NTH>
 __device__ int jacobi_sweeps(double* A, double* V, int n, int max_sweeps, double tol, JacobiShared& sh) {
   const int tid = threadIdx.x;
@@ -388,46 +392,75 @@ __device__ int jacobi_sweeps(double* A, double* V, int n, int max_sweeps, double
         sh.cs_c[k] = c; sh.cs_s[k] = s; sh.pp[k] = p; sh.qq[k] = q;
       }
       __syncthreads();
-      for (int e = tid; e < nA + nV; e += NTH) {
-        if (e < nA) {
-          // 2x2 block: rows of pair k (row rotation first), columns of pair l
-          const int k = e / npair, l = e % npair;
-          const int pk = sh.pp[k], qk = sh.qq[k], pl = sh.pp[l], ql = sh.qq[l];
-          const bool rk = qk < n, rl = ql < n;   // false: the dummy player, no rotation
-          const double ck = sh.cs_c[k], sk = sh.cs_s[k], cl = sh.cs_c[l], sl = sh.cs_s[l];
-          double r_pp = A[(size_t)pk * n + pl];
-          double r_pq = rl ? A[(size_t)pk * n + ql] : 0.0;
-          double r_qp = rk ? A[(size_t)qk * n + pl] : 0.0;
-          double r_qq = (rk && rl) ? A[(size_t)qk * n + ql] : 0.0;
-          if (rk) {
-            const double apj = r_pp, aqj = r_qp;
-            r_pp = ck * apj - sk * aqj;
-            r_qp = sk * apj + ck * aqj;
-            const double apj2 = r_pq, aqj2 = r_qq;
-            r_pq = ck * apj2 - sk * aqj2;
-            r_qq = sk * apj2 + ck * aqj2;
-          }
-          if (rl) {
-            A[(size_t)pk * n + pl] = cl * r_pp - sl * r_pq;
-            A[(size_t)pk * n + ql] = sl * r_pp + cl * r_pq;
-            if (rk) {
-              A[(size_t)qk * n + pl] = cl * r_qp - sl * r_qq;
-              A[(size_t)qk * n + ql] = sl * r_qp + cl * r_qq;
+      // The 2x2 blocks of one round (and the V column pairs) are disjoint, so each thread
+      // issues the loads of JB items before any of their stores: JB memory round trips in
+      // flight instead of one (the workspace path runs at L2 latency otherwise).
+      constexpr int JB = 4;
+      for (int e0 = tid; e0 < nA + nV; e0 += NTH * JB) {
+        double x[JB][4];
+#pragma unroll
+        for (int u = 0; u < JB; ++u) {
+          const int e = e0 + u * NTH;
+          x[u][0] = x[u][1] = x[u][2] = x[u][3] = 0.0;
+          if (e < nA) {
+            const int k = e / npair, l = e % npair;
+            const int pk = sh.pp[k], qk = sh.qq[k], pl = sh.pp[l], ql = sh.qq[l];
+            const bool rk = qk < n, rl = ql < n;   // false: the dummy player, no rotation
+            x[u][0] = A[(size_t)pk * n + pl];
+            if (rl) x[u][1] = A[(size_t)pk * n + ql];
+            if (rk) x[u][2] = A[(size_t)qk * n + pl];
+            if (rk && rl) x[u][3] = A[(size_t)qk * n + ql];
+          } else if (e < nA + nV) {
+            const int ev = e - nA;
+            const int l = ev / n, i = ev % n;
+            const int pl = sh.pp[l], ql = sh.qq[l];
+            if (ql < n) {
+              x[u][0] = V[(size_t)pl * n + i];
+              x[u][1] = V[(size_t)ql * n + i];
             }
-          } else {
-            A[(size_t)pk * n + pl] = r_pp;
-            if (rk) A[(size_t)qk * n + pl] = r_qp;
           }
-        } else {
-          // V <- V J: row i, pair l
-          const int ev = e - nA;
-          const int l = ev / n, i = ev % n;
-          const int pl = sh.pp[l], ql = sh.qq[l];
-          if (ql >= n) continue;
-          const double c = sh.cs_c[l], s = sh.cs_s[l];
-          const double vip = V[(size_t)i * n + pl], viq = V[(size_t)i * n + ql];
-          V[(size_t)i * n + pl] = c * vip - s * viq;
-          V[(size_t)i * n + ql] = s * vip + c * viq;
+        }
+#pragma unroll
+        for (int u = 0; u < JB; ++u) {
+          const int e = e0 + u * NTH;
+          if (e < nA) {
+            // 2x2 block: rows of pair k (row rotation first), columns of pair l
+            const int k = e / npair, l = e % npair;
+            const int pk = sh.pp[k], qk = sh.qq[k], pl = sh.pp[l], ql = sh.qq[l];
+            const bool rk = qk < n, rl = ql < n;
+            const double ck = sh.cs_c[k], sk = sh.cs_s[k], cl = sh.cs_c[l], sl = sh.cs_s[l];
+            double r_pp = x[u][0], r_pq = x[u][1], r_qp = x[u][2], r_qq = x[u][3];
+            if (rk) {
+              const double apj = r_pp, aqj = r_qp;
+              r_pp = ck * apj - sk * aqj;
+              r_qp = sk * apj + ck * aqj;
+              const double apj2 = r_pq, aqj2 = r_qq;
+              r_pq = ck * apj2 - sk * aqj2;
+              r_qq = sk * apj2 + ck * aqj2;
+            }
+            if (rl) {
+              A[(size_t)pk * n + pl] = cl * r_pp - sl * r_pq;
+              A[(size_t)pk * n + ql] = sl * r_pp + cl * r_pq;
+              if (rk) {
+                A[(size_t)qk * n + pl] = cl * r_qp - sl * r_qq;
+                A[(size_t)qk * n + ql] = sl * r_qp + cl * r_qq;
+              }
+            } else {
+              A[(size_t)pk * n + pl] = r_pp;
+              if (rk) A[(size_t)qk * n + pl] = r_qp;
+            }
+          } else if (e < nA + nV) {
+            // V <- V J: row i, pair l
+            const int ev = e - nA;
+            const int l = ev / n, i = ev % n;
+            const int pl = sh.pp[l], ql = sh.qq[l];
+            if (ql < n) {
+              const double c = sh.cs_c[l], sv = sh.cs_s[l];
+              const double vip = x[u][0], viq = x[u][1];
+              V[(size_t)pl * n + i] = c * vip - sv * viq;
+              V[(size_t)ql * n + i] = sv * vip + c * viq;
+            }
+          }
         }
       }
       __syncthreads();
@@ -437,7 +470,7 @@ __device__ int jacobi_sweeps(double* A, double* V, int n, int max_sweeps, double
 }
 
 // Ascending eigenvalues (rank of each diagonal element of A, ties by index) to w and the
-// matching columns of V to Vo (either may live anywhere).
+// matching columns of V (stored transposed) to Vo, row-major (either may live anywhere).
 template <int NTH>
 __device__ void jacobi_sort(const double* A, const double* V, int n, double* __restrict__ w,
                             double* __restrict__ Vo) {
@@ -449,7 +482,7 @@ __device__ void jacobi_sort(const double* A, const double* V, int n, double* __r
       rank += (dj < di) || (dj == di && j < i);
     }
     w[rank] = di;
-    for (int row = 0; row < n; ++row) Vo[(size_t)row * n + rank] = V[(size_t)row * n + i];
+    for (int row = 0; row < n; ++row) Vo[(size_t)row * n + rank] = V[(size_t)i * n + row];
   }
 }
 

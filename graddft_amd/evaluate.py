@@ -684,3 +684,43 @@ def jitted_mol_orb_optimizer(functional: Functional, tx, cycles: int = 500, prec
         return molecule.replace(energy=predicted_e)
 
     return neural_iterator
+
+
+def batched_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp64", n_streams: int = 4,
+                     **kwargs) -> Callable:
+    """Forward self-consistent loops of a batch of molecules run concurrently: each molecule's
+    ``diff_scf_loop`` is one captured HIP graph (captured on its first call, after one eager
+    run), and the replays of different molecules are issued round-robin on ``n_streams``
+    streams (default priority), so small molecules -- each filling a fraction of the GPU --
+    overlap.  Results equal the molecule-by-molecule loop (``train.py:519-527`` evaluates a
+    batch that way).  Forward only; for reverse mode call ``diff_scf_loop`` per molecule.
+
+    ``iterator(params, molecules) -> [molecule]`` with the loops' outputs."""
+    loop = diff_scf_loop(functional, cycles=cycles, precision=precision, graph=True, **kwargs)
+    streams: List[torch.cuda.Stream] = []
+    captured = set()   # (id of the molecule's cache, params given): the cache holds the graph
+
+    def iterator(params: Optional[torch.Tensor], molecules: List[Molecule]) -> List[Molecule]:
+        with torch.no_grad():
+            for m in molecules:           # first call per molecule: eager run + capture
+                tag = (id(m.cache), params is None)
+                if tag not in captured:
+                    loop(params, m)
+                    captured.add(tag)
+            if not streams:
+                streams.extend(torch.cuda.Stream() for _ in range(max(1, n_streams)))
+            main = torch.cuda.current_stream()
+            for s in streams:
+                s.wait_stream(main)
+            outs = []
+            for i, m in enumerate(molecules):
+                with torch.cuda.stream(streams[i % len(streams)]):
+                    outs.append(loop(params, m))
+            for s in streams:
+                main.wait_stream(s)
+            for o in outs:                # outputs allocated on the side streams, used on main
+                for t in (o.energy, o.rdm1, o.fock, o.mo_coeff, o.mo_energy, o.mo_occ):
+                    t.record_stream(main)
+            return outs
+
+    return iterator

@@ -186,3 +186,25 @@ def test_scf_loop_neural_functional_matches_oracle():
                             mol.rdm1.cpu().numpy(), mol.mo_occ.cpu().numpy(), level_shift_factor=(0.1, 0.1))
     assert abs(float(out.energy) - st.energy) <= 1e-8, (float(out.energy), st.energy)
     assert out.cache["scf_cycles"] == cycles
+
+
+def test_batched_scf_loop_matches_molecule_by_molecule():
+    """Concurrent graph replays on 3 streams equal the eager per-molecule loops, on two
+    batches in a row with eager work and synchronisations between them."""
+    _say("case: batched_scf_loop, H2O x2 + CH4 + C2H6 def2-svp level 1, B3LYP 8 cycles, 3 streams")
+    from graddft_amd import B3LYP, batched_scf_loop, diff_scf_loop
+    from graddft_amd.interface import geometry
+    from graddft_amd.interface.builder import molecule_from_geometry
+    rng = np.random.default_rng(4)
+    mols = [_water(seed=11), _water(seed=12)]
+    for nc in (1, 2):
+        sym, xyz = geometry.alkane(rng, nc)
+        mols.append(molecule_from_geometry(sym, xyz, "def2-svp", 1, with_eri=True))
+    ref = [float(diff_scf_loop(B3LYP, cycles=8)(None, m).energy) for m in mols]
+    bat = batched_scf_loop(B3LYP, cycles=8, n_streams=3)
+    for _ in range(2):
+        outs = bat(None, mols)
+        torch.cuda.synchronize()
+        for o, e in zip(outs, ref):
+            assert abs(float(o.energy) - e) <= 1e-10, (float(o.energy), e)
+        _ = sum(float(m.rdm1.sum()) for m in mols)   # eager work between the batches
