@@ -262,7 +262,7 @@ __global__ __launch_bounds__(1024) void cholesky_inv_kernel(const double* __rest
 // A, V and the warm-start product live in LDS when 3 n^2 doubles fit (SMEM),
 // otherwise in the workspace; 1024 threads either way.
 constexpr int EIG_THREADS_SMEM = 1024, EIG_THREADS_GLOBAL = 1024;
-constexpr size_t EIG_SMEM_MAX = 64 * 1024;
+constexpr size_t EIG_SMEM_MAX = 64 * 1024;   // floor of the LDS budget (any device)
 
 __device__ __forceinline__ int rr_partner(int r, int k, int m, int* p, int* q) {
   // round r of the round-robin tournament on m (even) players: pair k
@@ -281,6 +281,24 @@ struct JacobiShared {
   double red[16];
   int done;
 };
+
+// Dynamic LDS a Jacobi workgroup may take on the current device: the per-workgroup LDS limit
+// (160 KB on gfx950) less the static JacobiShared block and 1 KB of slack, never below
+// EIG_SMEM_MAX.  Cached per device.
+static size_t eig_smem_budget() {
+  static size_t cached[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return EIG_SMEM_MAX;
+  if (cached[dev] == 0) {
+    int v = 0;
+    size_t b = EIG_SMEM_MAX;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+        (size_t)v > sizeof(JacobiShared) + 1024 + EIG_SMEM_MAX)
+      b = (size_t)v - sizeof(JacobiShared) - 1024;
+    cached[dev] = b;
+  }
+  return cached[dev];
+}
 
 // Warm start (V0 != NULL, orthonormal columns): the sweeps run on V0^T A V0 starting from
 // V = V0, so a matrix close to one already diagonalised by V0 needs few sweeps; the result
@@ -825,11 +843,14 @@ int gdx_eigh_batched(void* stream, int32_t nbatch, int32_t n, const double* A, c
   GDX_REQUIRE(V0 == nullptr || (V0 != V && V0 != A), "gdx_eigh_batched: V0 must not alias A or V");
   hipStream_t st = (hipStream_t)stream;
   const size_t smem = (size_t)3 * n * n * sizeof(double);
-  if (smem <= EIG_SMEM_MAX) {
-    GDX_LAUNCH("jacobi_eigh", (jacobi_eigh_kernel<true, EIG_THREADS_SMEM>), dim3(nbatch), dim3(EIG_THREADS_SMEM),
-               smem, st, A, n, V0, w, V, (double*)workspace, max_sweeps, tol, sweeps);
+  if (smem <= eig_smem_budget()) {
+    auto kern = jacobi_eigh_kernel<true, EIG_THREADS_SMEM>;
+    if (smem > EIG_SMEM_MAX)
+      GDX_HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    GDX_LAUNCH("jacobi_eigh", kern, dim3(nbatch), dim3(EIG_THREADS_SMEM), smem, st, A, n, V0, w, V,
+               (double*)workspace, max_sweeps, tol, sweeps);
   } else {
-    GDX_REQUIRE(workspace, "gdx_eigh_batched: workspace required for n > 52");
+    GDX_REQUIRE(workspace, "gdx_eigh_batched: workspace required when 3 n^2 doubles exceed the LDS budget");
     GDX_LAUNCH("jacobi_eigh", (jacobi_eigh_kernel<false, EIG_THREADS_GLOBAL>), dim3(nbatch),
                dim3(EIG_THREADS_GLOBAL), 0, st, A, n, V0, w, V, (double*)workspace, max_sweeps, tol, sweeps);
   }
@@ -852,11 +873,14 @@ int gdx_scf_step(void* stream, int32_t nspin, int32_t n, int32_t m, const double
   GDX_REQUIRE(V0 == nullptr || (V0 != V && V0 != C && V0 != rdm1), "gdx_scf_step: V0 must not alias an output");
   hipStream_t st = (hipStream_t)stream;
   const size_t smem = (size_t)4 * n * n * sizeof(double);
-  if (smem <= EIG_SMEM_MAX) {
-    GDX_LAUNCH("scf_step", (scf_step_kernel<true, EIG_THREADS_SMEM>), dim3(nspin), dim3(EIG_THREADS_SMEM), smem, st,
-               n, m, nspin, focks, x, Linv, V0, occ, F, w, V, C, rdm1, (double*)workspace, max_sweeps, tol, sweeps);
+  if (smem <= eig_smem_budget()) {
+    auto kern = scf_step_kernel<true, EIG_THREADS_SMEM>;
+    if (smem > EIG_SMEM_MAX)
+      GDX_HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    GDX_LAUNCH("scf_step", kern, dim3(nspin), dim3(EIG_THREADS_SMEM), smem, st, n, m, nspin, focks, x, Linv, V0, occ,
+               F, w, V, C, rdm1, (double*)workspace, max_sweeps, tol, sweeps);
   } else {
-    GDX_REQUIRE(workspace, "gdx_scf_step: workspace required for n > 45");
+    GDX_REQUIRE(workspace, "gdx_scf_step: workspace required when 4 n^2 doubles exceed the LDS budget");
     GDX_LAUNCH("scf_step", (scf_step_kernel<false, EIG_THREADS_GLOBAL>), dim3(nspin), dim3(EIG_THREADS_GLOBAL), 0,
                st, n, m, nspin, focks, x, Linv, V0, occ, F, w, V, C, rdm1, (double*)workspace, max_sweeps, tol,
                sweeps);

@@ -155,7 +155,8 @@ int gdx_cholesky_inv(void* stream, int32_t n, const double* S, double* L, double
  * A (nbatch, n, n) -> w (nbatch, n) ascending, V (nbatch, n, n) eigenvectors as columns.
  * V0 (nbatch, n, n), optional (NULL = none): orthonormal starting eigenvector estimates (the
  * previous SCF cycle's), which cut the Jacobi sweeps for a nearby matrix; it must not alias
- * A or V.  workspace: gdx_eigh_workspace_size bytes (used when n > 52).  sweeps: optional
+ * A or V.  workspace: gdx_eigh_workspace_size bytes (used when 3 n^2 doubles exceed the
+ * device's LDS per workgroup less 13 KB: n > 79 on gfx950's 160 KB).  sweeps: optional
  * device int per matrix (Jacobi sweeps used). */
 int gdx_eigh_workspace_size(int32_t nbatch, int32_t n, size_t* bytes);
 int gdx_eigh_batched(void* stream, int32_t nbatch, int32_t n, const double* A, const double* V0,
@@ -169,7 +170,8 @@ int gdx_eigh_batched(void* stream, int32_t nbatch, int32_t n, const double* A, c
  * x (nspin, m+1) from gdx_diis_coefficients), C' = sym(Linv F Linv^T) = V diag(w) V^T
  * (Jacobi as gdx_eigh_batched, warm-started from V0 when given), C = Linv^T V and
  * rdm1 = C diag(occ) C^T with occ (nspin, n).  Outputs F, w, V (orthonormal eigenvectors, the
- * next call's V0), C, rdm1.  workspace: gdx_scf_step_workspace_size bytes (used when n > 45). */
+ * next call's V0), C, rdm1.  workspace: gdx_scf_step_workspace_size bytes (used when 4 n^2
+ * doubles exceed the device's LDS per workgroup less 13 KB: n > 68 on gfx950's 160 KB). */
 int gdx_scf_step_workspace_size(int32_t nspin, int32_t n, size_t* bytes);
 int gdx_scf_step(void* stream, int32_t nspin, int32_t n, int32_t m, const double* focks, const double* x,
                  const double* Linv, const double* V0, const double* occ, double* F, double* w, double* V,
