@@ -300,6 +300,46 @@ static size_t eig_smem_budget() {
   return cached[dev];
 }
 
+// C = op(A) op(B) for n x n row-major matrices, op(X) = X or X^T, by one 1024-thread
+// workgroup through 32 x 32 LDS tiles (ta/tb: take the transpose).  Used where the matrices
+// live in the workspace: each element is fetched once per tile with coalesced loads instead
+// of n dependent L2 reads per output.  C must not alias A or B.  Every thread must call it.
+constexpr int MT = 32;
+__device__ void mm_tiled(double* __restrict__ C, const double* __restrict__ A, bool ta,
+                         const double* __restrict__ B, bool tb, int n, double (*As)[MT + 1],
+                         double (*Bs)[MT + 1]) {
+  const int tid = threadIdx.x, ty = tid / MT, tx = tid % MT;
+  const int nb = (n + MT - 1) / MT;
+  for (int bi = 0; bi < nb; ++bi)
+    for (int bj = 0; bj < nb; ++bj) {
+      double acc = 0.0;
+      for (int k0 = 0; k0 < n; k0 += MT) {
+        // As[r][k] = op(A)[bi*MT + r][k0 + k], Bs[k][c] = op(B)[k0 + k][bj*MT + c]
+        if (!ta) {
+          const int r = bi * MT + ty, k = k0 + tx;
+          As[ty][tx] = (r < n && k < n) ? A[(size_t)r * n + k] : 0.0;
+        } else {   // op(A)[r][k] = A[k][r]: read row k0+ty of A, columns bi*MT+tx
+          const int k = k0 + ty, r = bi * MT + tx;
+          As[tx][ty] = (r < n && k < n) ? A[(size_t)k * n + r] : 0.0;
+        }
+        if (!tb) {
+          const int k = k0 + ty, c = bj * MT + tx;
+          Bs[ty][tx] = (k < n && c < n) ? B[(size_t)k * n + c] : 0.0;
+        } else {   // op(B)[k][c] = B[c][k]: read row bj*MT+ty of B, columns k0+tx
+          const int c = bj * MT + ty, k = k0 + tx;
+          Bs[tx][ty] = (k < n && c < n) ? B[(size_t)c * n + k] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int kk = 0; kk < MT; ++kk) acc += As[ty][kk] * Bs[kk][tx];
+        __syncthreads();
+      }
+      const int r = bi * MT + ty, c = bj * MT + tx;
+      if (r < n && c < n) C[(size_t)r * n + c] = acc;
+    }
+  __syncthreads();
+}
+
 // Warm start (V0 != NULL, orthonormal columns): the sweeps run on V0^T A V0 starting from
 // V = V0, so a matrix close to one already diagonalised by V0 needs few sweeps; the result
 // is the same eigendecomposition (eigenvalues, and eigenvectors up to sign).  On entry T
@@ -309,7 +349,8 @@ static size_t eig_smem_budget() {
 // the rotation of a column pair touches two contiguous rows of the buffer (coalesced in the
 // workspace path; consecutive banks in LDS).
 template <int NTH>
-__device__ void jacobi_start(double* A, double* V, double* T, int n, const double* __restrict__ V0) {
+__device__ void jacobi_start(double* A, double* V, double* T, int n, const double* __restrict__ V0,
+                             double (*As)[MT + 1] = nullptr, double (*Bs)[MT + 1] = nullptr) {
   const int tid = threadIdx.x;
   const size_t nn = (size_t)n * n;
   if (V0 == nullptr) {
@@ -322,6 +363,10 @@ __device__ void jacobi_start(double* A, double* V, double* T, int n, const doubl
   }
   for (size_t e = tid; e < nn; e += NTH) V[e] = V0[(e % n) * n + e / n];   // V0 transposed
   __syncthreads();
+  if (As != nullptr) {
+    mm_tiled(A, T, false, V, true, n, As, Bs);    // A <- T V0
+    mm_tiled(T, V, false, A, false, n, As, Bs);   // T <- V0^T A
+  } else {
   // A <- T V0   (V0_kj = V[j * n + k])
   for (size_t e = tid; e < nn; e += NTH) {
     const int i = (int)(e / n), j = (int)(e % n);
@@ -338,6 +383,7 @@ __device__ void jacobi_start(double* A, double* V, double* T, int n, const doubl
     T[e] = t;
   }
   __syncthreads();
+  }
   // A <- T symmetrised (rounding); V already holds V0
   for (size_t e = tid; e < nn; e += NTH) {
     const int i = (int)(e / n), j = (int)(e % n);
@@ -410,75 +456,46 @@ __device__ int jacobi_sweeps(double* A, double* V, int n, int max_sweeps, double
         sh.cs_c[k] = c; sh.cs_s[k] = s; sh.pp[k] = p; sh.qq[k] = q;
       }
       __syncthreads();
-      // The 2x2 blocks of one round (and the V column pairs) are disjoint, so each thread
-      // issues the loads of JB items before any of their stores: JB memory round trips in
-      // flight instead of one (the workspace path runs at L2 latency otherwise).
-      constexpr int JB = 4;
-      for (int e0 = tid; e0 < nA + nV; e0 += NTH * JB) {
-        double x[JB][4];
-#pragma unroll
-        for (int u = 0; u < JB; ++u) {
-          const int e = e0 + u * NTH;
-          x[u][0] = x[u][1] = x[u][2] = x[u][3] = 0.0;
-          if (e < nA) {
-            const int k = e / npair, l = e % npair;
-            const int pk = sh.pp[k], qk = sh.qq[k], pl = sh.pp[l], ql = sh.qq[l];
-            const bool rk = qk < n, rl = ql < n;   // false: the dummy player, no rotation
-            x[u][0] = A[(size_t)pk * n + pl];
-            if (rl) x[u][1] = A[(size_t)pk * n + ql];
-            if (rk) x[u][2] = A[(size_t)qk * n + pl];
-            if (rk && rl) x[u][3] = A[(size_t)qk * n + ql];
-          } else if (e < nA + nV) {
-            const int ev = e - nA;
-            const int l = ev / n, i = ev % n;
-            const int pl = sh.pp[l], ql = sh.qq[l];
-            if (ql < n) {
-              x[u][0] = V[(size_t)pl * n + i];
-              x[u][1] = V[(size_t)ql * n + i];
-            }
+      for (int e = tid; e < nA + nV; e += NTH) {
+        if (e < nA) {
+          // 2x2 block: rows of pair k (row rotation first), columns of pair l
+          const int k = e / npair, l = e % npair;
+          const int pk = sh.pp[k], qk = sh.qq[k], pl = sh.pp[l], ql = sh.qq[l];
+          const bool rk = qk < n, rl = ql < n;   // false: the dummy player, no rotation
+          const double ck = sh.cs_c[k], sk = sh.cs_s[k], cl = sh.cs_c[l], sl = sh.cs_s[l];
+          double r_pp = A[(size_t)pk * n + pl];
+          double r_pq = rl ? A[(size_t)pk * n + ql] : 0.0;
+          double r_qp = rk ? A[(size_t)qk * n + pl] : 0.0;
+          double r_qq = (rk && rl) ? A[(size_t)qk * n + ql] : 0.0;
+          if (rk) {
+            const double apj = r_pp, aqj = r_qp;
+            r_pp = ck * apj - sk * aqj;
+            r_qp = sk * apj + ck * aqj;
+            const double apj2 = r_pq, aqj2 = r_qq;
+            r_pq = ck * apj2 - sk * aqj2;
+            r_qq = sk * apj2 + ck * aqj2;
           }
-        }
-#pragma unroll
-        for (int u = 0; u < JB; ++u) {
-          const int e = e0 + u * NTH;
-          if (e < nA) {
-            // 2x2 block: rows of pair k (row rotation first), columns of pair l
-            const int k = e / npair, l = e % npair;
-            const int pk = sh.pp[k], qk = sh.qq[k], pl = sh.pp[l], ql = sh.qq[l];
-            const bool rk = qk < n, rl = ql < n;
-            const double ck = sh.cs_c[k], sk = sh.cs_s[k], cl = sh.cs_c[l], sl = sh.cs_s[l];
-            double r_pp = x[u][0], r_pq = x[u][1], r_qp = x[u][2], r_qq = x[u][3];
+          if (rl) {
+            A[(size_t)pk * n + pl] = cl * r_pp - sl * r_pq;
+            A[(size_t)pk * n + ql] = sl * r_pp + cl * r_pq;
             if (rk) {
-              const double apj = r_pp, aqj = r_qp;
-              r_pp = ck * apj - sk * aqj;
-              r_qp = sk * apj + ck * aqj;
-              const double apj2 = r_pq, aqj2 = r_qq;
-              r_pq = ck * apj2 - sk * aqj2;
-              r_qq = sk * apj2 + ck * aqj2;
+              A[(size_t)qk * n + pl] = cl * r_qp - sl * r_qq;
+              A[(size_t)qk * n + ql] = sl * r_qp + cl * r_qq;
             }
-            if (rl) {
-              A[(size_t)pk * n + pl] = cl * r_pp - sl * r_pq;
-              A[(size_t)pk * n + ql] = sl * r_pp + cl * r_pq;
-              if (rk) {
-                A[(size_t)qk * n + pl] = cl * r_qp - sl * r_qq;
-                A[(size_t)qk * n + ql] = sl * r_qp + cl * r_qq;
-              }
-            } else {
-              A[(size_t)pk * n + pl] = r_pp;
-              if (rk) A[(size_t)qk * n + pl] = r_qp;
-            }
-          } else if (e < nA + nV) {
-            // V <- V J: row i, pair l
-            const int ev = e - nA;
-            const int l = ev / n, i = ev % n;
-            const int pl = sh.pp[l], ql = sh.qq[l];
-            if (ql < n) {
-              const double c = sh.cs_c[l], sv = sh.cs_s[l];
-              const double vip = x[u][0], viq = x[u][1];
-              V[(size_t)pl * n + i] = c * vip - sv * viq;
-              V[(size_t)ql * n + i] = sv * vip + c * viq;
-            }
+          } else {
+            A[(size_t)pk * n + pl] = r_pp;
+            if (rk) A[(size_t)qk * n + pl] = r_qp;
           }
+        } else {
+          // V <- V J on the transposed storage: rows pl, ql of the buffer, element i
+          const int ev = e - nA;
+          const int l = ev / n, i = ev % n;
+          const int pl = sh.pp[l], ql = sh.qq[l];
+          if (ql >= n) continue;
+          const double c = sh.cs_c[l], sv = sh.cs_s[l];
+          const double vip = V[(size_t)pl * n + i], viq = V[(size_t)ql * n + i];
+          V[(size_t)pl * n + i] = c * vip - sv * viq;
+          V[(size_t)ql * n + i] = sv * vip + c * viq;
         }
       }
       __syncthreads();
@@ -521,7 +538,12 @@ __global__ __launch_bounds__(NTH) void jacobi_eigh_kernel(const double* __restri
   const double* Asrc = Ain + (size_t)b * nn;
   for (size_t e = threadIdx.x; e < nn; e += NTH) T[e] = Asrc[e];
   __syncthreads();
-  jacobi_start<NTH>(A, V, T, n, V0in == nullptr ? nullptr : V0in + (size_t)b * nn);
+  if constexpr (SMEM) {
+    jacobi_start<NTH>(A, V, T, n, V0in == nullptr ? nullptr : V0in + (size_t)b * nn);
+  } else {
+    __shared__ double tA[MT][MT + 1], tB[MT][MT + 1];
+    jacobi_start<NTH>(A, V, T, n, V0in == nullptr ? nullptr : V0in + (size_t)b * nn, tA, tB);
+  }
   const int sweep = jacobi_sweeps<NTH>(A, V, n, max_sweeps, tol, sh);
   jacobi_sort<NTH>(A, V, n, w_out + (size_t)b * n, V_out + (size_t)b * nn);
   if (threadIdx.x == 0 && sweeps_out) sweeps_out[b] = sweep;
@@ -571,6 +593,12 @@ __global__ __launch_bounds__(NTH) void scf_step_kernel(int n, int m, int nspin, 
     L[e] = Linv[e];
   }
   __syncthreads();
+  // workspace path: LDS tiles for the n^3 products (unused in the LDS variant)
+  __shared__ double tA[SMEM ? 1 : MT][MT + 1], tB[SMEM ? 1 : MT][MT + 1];
+  if constexpr (!SMEM) {
+    mm_tiled(A, L, false, T, false, n, tA, tB);   // A <- L F
+    mm_tiled(V, A, false, L, true, n, tA, tB);    // V <- A L^T
+  } else {
   // A <- L F
   for (size_t e = tid; e < nn; e += NTH) {
     const int i = (int)(e / n), j = (int)(e % n);
@@ -587,18 +615,32 @@ __global__ __launch_bounds__(NTH) void scf_step_kernel(int n, int m, int nspin, 
     V[e] = t;
   }
   __syncthreads();
+  }
   // T <- (V + V^T) / 2
   for (size_t e = tid; e < nn; e += NTH) {
     const int i = (int)(e / n), j = (int)(e % n);
     T[e] = 0.5 * (V[e] + V[(size_t)j * n + i]);
   }
   __syncthreads();
-  jacobi_start<NTH>(A, V, T, n, V0in == nullptr ? nullptr : V0in + (size_t)s * nn);
+  if constexpr (SMEM)
+    jacobi_start<NTH>(A, V, T, n, V0in == nullptr ? nullptr : V0in + (size_t)s * nn);
+  else
+    jacobi_start<NTH>(A, V, T, n, V0in == nullptr ? nullptr : V0in + (size_t)s * nn, tA, tB);
   const int sweep = jacobi_sweeps<NTH>(A, V, n, max_sweeps, tol, sh);
   // sorted eigenpairs: w, V to the outputs and the sorted V to T
   jacobi_sort<NTH>(A, V, n, w_out + (size_t)s * n, T);
   __syncthreads();
   for (size_t e = tid; e < nn; e += NTH) V_out[(size_t)s * nn + e] = T[e];
+  const double* os = occ + (size_t)s * n;
+  if constexpr (!SMEM) {
+    mm_tiled(A, L, true, T, false, n, tA, tB);    // A <- L^T T  (C = Linv^T V)
+    for (size_t e = tid; e < nn; e += NTH) {
+      C_out[(size_t)s * nn + e] = A[e];
+      T[e] = A[e] * os[e % n];                    // T <- C diag(occ)
+    }
+    __syncthreads();
+    mm_tiled(D_out + (size_t)s * nn, T, false, A, true, n, tA, tB);   // D = T C^T
+  } else {
   // A <- L^T T  (C = Linv^T V)
   for (size_t e = tid; e < nn; e += NTH) {
     const int i = (int)(e / n), j = (int)(e % n);
@@ -609,7 +651,6 @@ __global__ __launch_bounds__(NTH) void scf_step_kernel(int n, int m, int nspin, 
   }
   __syncthreads();
   // T <- C diag(occ), then D[i][k] = sum_j T[i][j] C[k][j]
-  const double* os = occ + (size_t)s * n;
   for (size_t e = tid; e < nn; e += NTH) T[e] = A[e] * os[e % n];
   __syncthreads();
   for (size_t e = tid; e < nn; e += NTH) {
@@ -617,6 +658,7 @@ __global__ __launch_bounds__(NTH) void scf_step_kernel(int n, int m, int nspin, 
     double t = 0.0;
     for (int j = 0; j < n; ++j) t += T[(size_t)i * n + j] * A[(size_t)k * n + j];
     D_out[(size_t)s * nn + e] = t;
+  }
   }
   if (tid == 0 && sweeps_out) sweeps_out[s] = sweep;
 }
