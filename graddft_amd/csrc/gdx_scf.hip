@@ -340,6 +340,13 @@ __device__ void mm_tiled(double* __restrict__ C, const double* __restrict__ A, b
   __syncthreads();
 }
 
+// Workspace path: dynamic LDS for the matrix being rotated (n^2 doubles) when it fits beside
+// the static JacobiShared block and the product tiles, else 0 (A stays in the workspace).
+static size_t a_lds_bytes(int n) {
+  const size_t a = (size_t)n * n * sizeof(double), tiles = 2 * (size_t)MT * (MT + 1) * sizeof(double);
+  return a + tiles <= eig_smem_budget() ? a : 0;
+}
+
 // Warm start (V0 != NULL, orthonormal columns): the sweeps run on V0^T A V0 starting from
 // V = V0, so a matrix close to one already diagonalised by V0 needs few sweeps; the result
 // is the same eigendecomposition (eigenvalues, and eigenvectors up to sign).  On entry T
@@ -527,7 +534,8 @@ __global__ __launch_bounds__(NTH) void jacobi_eigh_kernel(const double* __restri
                                                           double* __restrict__ w_out,
                                                           double* __restrict__ V_out,
                                                           double* __restrict__ gws, int max_sweeps,
-                                                          double tol, int* __restrict__ sweeps_out) {
+                                                          double tol, int* __restrict__ sweeps_out,
+                                                          int a_lds) {
   extern __shared__ double smem[];
   __shared__ JacobiShared sh;
   const int b = blockIdx.x;
@@ -544,8 +552,14 @@ __global__ __launch_bounds__(NTH) void jacobi_eigh_kernel(const double* __restri
     __shared__ double tA[MT][MT + 1], tB[MT][MT + 1];
     jacobi_start<NTH>(A, V, T, n, V0in == nullptr ? nullptr : V0in + (size_t)b * nn, tA, tB);
   }
-  const int sweep = jacobi_sweeps<NTH>(A, V, n, max_sweeps, tol, sh);
-  jacobi_sort<NTH>(A, V, n, w_out + (size_t)b * n, V_out + (size_t)b * nn);
+  double* Ar = A;
+  if (!SMEM && a_lds) {   // workspace path, A alone fits in LDS: rotate it there
+    for (size_t e = threadIdx.x; e < nn; e += NTH) smem[e] = A[e];
+    __syncthreads();
+    Ar = smem;
+  }
+  const int sweep = jacobi_sweeps<NTH>(Ar, V, n, max_sweeps, tol, sh);
+  jacobi_sort<NTH>(Ar, V, n, w_out + (size_t)b * n, V_out + (size_t)b * nn);
   if (threadIdx.x == 0 && sweeps_out) sweeps_out[b] = sweep;
 }
 
@@ -565,7 +579,7 @@ __global__ __launch_bounds__(NTH) void scf_step_kernel(int n, int m, int nspin, 
                                                        double* __restrict__ w_out, double* __restrict__ V_out,
                                                        double* __restrict__ C_out, double* __restrict__ D_out,
                                                        double* __restrict__ gws, int max_sweeps, double tol,
-                                                       int* __restrict__ sweeps_out) {
+                                                       int* __restrict__ sweeps_out, int a_lds) {
   extern __shared__ double smem[];
   __shared__ JacobiShared sh;
   const int s = blockIdx.x, tid = threadIdx.x;
@@ -626,9 +640,15 @@ __global__ __launch_bounds__(NTH) void scf_step_kernel(int n, int m, int nspin, 
     jacobi_start<NTH>(A, V, T, n, V0in == nullptr ? nullptr : V0in + (size_t)s * nn);
   else
     jacobi_start<NTH>(A, V, T, n, V0in == nullptr ? nullptr : V0in + (size_t)s * nn, tA, tB);
-  const int sweep = jacobi_sweeps<NTH>(A, V, n, max_sweeps, tol, sh);
+  double* Ar = A;
+  if (!SMEM && a_lds) {   // workspace path, A alone fits in LDS: rotate it there
+    for (size_t e = tid; e < nn; e += NTH) smem[e] = A[e];
+    __syncthreads();
+    Ar = smem;
+  }
+  const int sweep = jacobi_sweeps<NTH>(Ar, V, n, max_sweeps, tol, sh);
   // sorted eigenpairs: w, V to the outputs and the sorted V to T
-  jacobi_sort<NTH>(A, V, n, w_out + (size_t)s * n, T);
+  jacobi_sort<NTH>(Ar, V, n, w_out + (size_t)s * n, T);
   __syncthreads();
   for (size_t e = tid; e < nn; e += NTH) V_out[(size_t)s * nn + e] = T[e];
   const double* os = occ + (size_t)s * n;
@@ -890,11 +910,15 @@ int gdx_eigh_batched(void* stream, int32_t nbatch, int32_t n, const double* A, c
     if (smem > EIG_SMEM_MAX)
       GDX_HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     GDX_LAUNCH("jacobi_eigh", kern, dim3(nbatch), dim3(EIG_THREADS_SMEM), smem, st, A, n, V0, w, V,
-               (double*)workspace, max_sweeps, tol, sweeps);
+               (double*)workspace, max_sweeps, tol, sweeps, 0);
   } else {
     GDX_REQUIRE(workspace, "gdx_eigh_batched: workspace required when 3 n^2 doubles exceed the LDS budget");
-    GDX_LAUNCH("jacobi_eigh", (jacobi_eigh_kernel<false, EIG_THREADS_GLOBAL>), dim3(nbatch),
-               dim3(EIG_THREADS_GLOBAL), 0, st, A, n, V0, w, V, (double*)workspace, max_sweeps, tol, sweeps);
+    auto kern = jacobi_eigh_kernel<false, EIG_THREADS_GLOBAL>;
+    const size_t dyn = a_lds_bytes(n);
+    if (dyn > EIG_SMEM_MAX)
+      GDX_HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+    GDX_LAUNCH("jacobi_eigh", kern, dim3(nbatch), dim3(EIG_THREADS_GLOBAL), dyn, st, A, n, V0, w, V,
+               (double*)workspace, max_sweeps, tol, sweeps, dyn > 0 ? 1 : 0);
   }
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
@@ -920,12 +944,15 @@ int gdx_scf_step(void* stream, int32_t nspin, int32_t n, int32_t m, const double
     if (smem > EIG_SMEM_MAX)
       GDX_HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     GDX_LAUNCH("scf_step", kern, dim3(nspin), dim3(EIG_THREADS_SMEM), smem, st, n, m, nspin, focks, x, Linv, V0, occ,
-               F, w, V, C, rdm1, (double*)workspace, max_sweeps, tol, sweeps);
+               F, w, V, C, rdm1, (double*)workspace, max_sweeps, tol, sweeps, 0);
   } else {
     GDX_REQUIRE(workspace, "gdx_scf_step: workspace required when 4 n^2 doubles exceed the LDS budget");
-    GDX_LAUNCH("scf_step", (scf_step_kernel<false, EIG_THREADS_GLOBAL>), dim3(nspin), dim3(EIG_THREADS_GLOBAL), 0,
-               st, n, m, nspin, focks, x, Linv, V0, occ, F, w, V, C, rdm1, (double*)workspace, max_sweeps, tol,
-               sweeps);
+    auto kern = scf_step_kernel<false, EIG_THREADS_GLOBAL>;
+    const size_t dyn = a_lds_bytes(n);
+    if (dyn > EIG_SMEM_MAX)
+      GDX_HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+    GDX_LAUNCH("scf_step", kern, dim3(nspin), dim3(EIG_THREADS_GLOBAL), dyn, st, n, m, nspin, focks, x, Linv, V0,
+               occ, F, w, V, C, rdm1, (double*)workspace, max_sweeps, tol, sweeps, dyn > 0 ? 1 : 0);
   }
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
