@@ -463,10 +463,14 @@ __device__ int jacobi_sweeps(double* A, double* V, int n, int max_sweeps, double
         sh.cs_c[k] = c; sh.cs_s[k] = s; sh.pp[k] = p; sh.qq[k] = q;
       }
       __syncthreads();
-      for (int e = tid; e < nA + nV; e += NTH) {
-        if (e < nA) {
-          // 2x2 block: rows of pair k (row rotation first), columns of pair l
-          const int k = e / npair, l = e % npair;
+      // items e = tid, tid + NTH, ...: first the 2x2 blocks (k, l) = (e / npair, e % npair),
+      // then the V rows (l, i) = (ev / n, ev % n), ev = e - nA; the indices are stepped
+      // instead of divided per item (NTH > npair and NTH > n - 1: one carry at most)
+      int e = tid;
+      {
+        int k = e / npair, l = e % npair;
+        const int dk = NTH / npair, dl = NTH % npair;
+        for (; e < nA; e += NTH) {
           const int pk = sh.pp[k], qk = sh.qq[k], pl = sh.pp[l], ql = sh.qq[l];
           const bool rk = qk < n, rl = ql < n;   // false: the dummy player, no rotation
           const double ck = sh.cs_c[k], sk = sh.cs_s[k], cl = sh.cs_c[l], sl = sh.cs_s[l];
@@ -474,7 +478,7 @@ __device__ int jacobi_sweeps(double* A, double* V, int n, int max_sweeps, double
           double r_pq = rl ? A[(size_t)pk * n + ql] : 0.0;
           double r_qp = rk ? A[(size_t)qk * n + pl] : 0.0;
           double r_qq = (rk && rl) ? A[(size_t)qk * n + ql] : 0.0;
-          if (rk) {
+          if (rk) {   // rows of pair k first
             const double apj = r_pp, aqj = r_qp;
             r_pp = ck * apj - sk * aqj;
             r_qp = sk * apj + ck * aqj;
@@ -482,7 +486,7 @@ __device__ int jacobi_sweeps(double* A, double* V, int n, int max_sweeps, double
             r_pq = ck * apj2 - sk * aqj2;
             r_qq = sk * apj2 + ck * aqj2;
           }
-          if (rl) {
+          if (rl) {   // then columns of pair l
             A[(size_t)pk * n + pl] = cl * r_pp - sl * r_pq;
             A[(size_t)pk * n + ql] = sl * r_pp + cl * r_pq;
             if (rk) {
@@ -493,16 +497,27 @@ __device__ int jacobi_sweeps(double* A, double* V, int n, int max_sweeps, double
             A[(size_t)pk * n + pl] = r_pp;
             if (rk) A[(size_t)qk * n + pl] = r_qp;
           }
-        } else {
-          // V <- V J on the transposed storage: rows pl, ql of the buffer, element i
-          const int ev = e - nA;
-          const int l = ev / n, i = ev % n;
+          l += dl;
+          k += dk;
+          if (l >= npair) { l -= npair; ++k; }
+        }
+      }
+      {
+        // V <- V J on the transposed storage: rows pl, ql of the buffer, element i
+        int ev = e - nA;
+        int l = ev / n, i = ev % n;
+        const int dlv = NTH / n, di = NTH % n;
+        for (; ev < nV; ev += NTH) {
           const int pl = sh.pp[l], ql = sh.qq[l];
-          if (ql >= n) continue;
-          const double c = sh.cs_c[l], sv = sh.cs_s[l];
-          const double vip = V[(size_t)pl * n + i], viq = V[(size_t)ql * n + i];
-          V[(size_t)pl * n + i] = c * vip - sv * viq;
-          V[(size_t)ql * n + i] = sv * vip + c * viq;
+          if (ql < n) {
+            const double c = sh.cs_c[l], sv = sh.cs_s[l];
+            const double vip = V[(size_t)pl * n + i], viq = V[(size_t)ql * n + i];
+            V[(size_t)pl * n + i] = c * vip - sv * viq;
+            V[(size_t)ql * n + i] = sv * vip + c * viq;
+          }
+          i += di;
+          l += dlv;
+          if (i >= n) { i -= n; ++l; }
         }
       }
       __syncthreads();
