@@ -77,8 +77,11 @@ __device__ __forceinline__ void split_range(int y, int kchunk, int kseg, int sps
   if (kend < kbeg) kend = kbeg;
 }
 
+// Blocks per CU: FP32 3 (2 with the ELU-backward epilogues); FP64 2 -- its 128 x 128 FP64
+// accumulators then live in VGPRs (256, no spills; 75-82 KB LDS a block), 28% faster for the
+// forward GEMM than one block with AGPR accumulators.
 template <typename T, bool TA, bool TB, int EPI>
-__global__ __launch_bounds__(256, sizeof(T) == 4 ? (EPI >= EPI_ELU_BWD ? 2 : 3) : 1) void gemm_tile_kernel(
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? (EPI >= EPI_ELU_BWD ? 2 : 3) : 2) void gemm_tile_kernel(
     int M, int N, const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb,
     T* __restrict__ C, int ldc, const T* __restrict__ bias, const T* __restrict__ Y, int ldy,
     int kchunk, T* __restrict__ colsum, int kseg, int sps, const T* __restrict__ X16 = nullptr,
