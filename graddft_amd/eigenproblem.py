@@ -34,6 +34,10 @@ def cholesky_inv(S: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
     status = torch.zeros(1, dtype=torch.int32, device=S.device)
     _lib.check(_lib.load().gdx_cholesky_inv(_stream(S), n, S.data_ptr(), L.data_ptr(), Linv.data_ptr(),
                                             status.data_ptr()), "gdx_cholesky_inv")
+    # the reference's cholesky returns NaN for a matrix that is not positive definite; the
+    # kernel flags it instead (read here, outside graph capture: one sync per molecule)
+    if not torch.cuda.is_current_stream_capturing() and int(status.item()) != 0:
+        raise _lib.GdxError("gdx_cholesky_inv: the overlap matrix is not positive definite")
     return L, Linv
 
 
@@ -74,8 +78,7 @@ class EighFn(torch.autograd.Function):
         wb, Vb = w.reshape(-1, n), V.reshape(-1, n, n)
         gw = torch.zeros_like(wb) if gw is None else gw.reshape(-1, n)
         gV = torch.zeros_like(Vb) if gV is None else gV.reshape(-1, n, n)
-        out = torch.stack([safe_eigh_vjp(wb[i], Vb[i], gw[i], gV[i]) for i in range(wb.shape[0])])
-        return out.reshape(V.shape), None
+        return safe_eigh_vjp(wb, Vb, gw, gV).reshape(V.shape), None
 
 
 def scf_step(focks: torch.Tensor, x: torch.Tensor, Linv: torch.Tensor, V0: Optional[torch.Tensor],
@@ -128,17 +131,19 @@ def safe_fock_solver(fock: torch.Tensor, overlap: torch.Tensor, Linv: torch.Tens
 
 def safe_eigh_vjp(evals: torch.Tensor, evecs: torch.Tensor, g_evals: torch.Tensor, g_evecs: torch.Tensor):
     """Reverse mode of safe_eigh (eigenproblem.py:54-103): Lorentzian-broadened gaps for
-    near-degenerate pairs, symmetrised result."""
-    diff = evals[None, :] - evals[:, None]
+    near-degenerate pairs, symmetrised result.  Batched over leading dimensions: evals (..., n),
+    evecs (..., n, n) with eigenvectors as columns."""
+    diff = evals[..., None, :] - evals[..., :, None]
     degen = diff.abs() < DEGEN_TOL
     regular = torch.nan_to_num(1.0 / diff)
     broadened = diff / (diff * diff + BROADENING)
-    F = 0.5 * torch.where(degen, broadened, regular)
-    F.fill_diagonal_(0.0)
-    Vt = evecs.T
+    n = evals.shape[-1]
+    off = ~torch.eye(n, dtype=torch.bool, device=evals.device)
+    F = torch.where(off, 0.5 * torch.where(degen, broadened, regular), torch.zeros_like(diff))
+    Vt = evecs.transpose(-1, -2)
     # inv(V^T) of the reference is V: the eigenvectors are orthonormal to rounding
-    grad = evecs @ (0.5 * torch.diag(g_evals) + F * (Vt @ g_evecs)) @ Vt
-    return grad + grad.T
+    grad = evecs @ (0.5 * torch.diag_embed(g_evals) + F * (Vt @ g_evecs)) @ Vt
+    return grad + grad.transpose(-1, -2)
 
 
 def householder_q(W: torch.Tensor) -> torch.Tensor:

@@ -181,7 +181,8 @@ def _aufbau_occupations(molecule: Molecule) -> torch.Tensor:
     """get_occ for ascending eigenvalues: the first N_e,s orbitals of each spin (cached)."""
     occ = molecule.cache.get("aufbau_occ")
     if occ is None:
-        nelecs = [int(round(float(v))) for v in molecule.mo_occ.sum(dim=1).tolist()]
+        # get_occ (molecule.py:851-889) casts the occupation sums to integers (truncation)
+        nelecs = [int(float(v)) for v in molecule.mo_occ.sum(dim=1).tolist()]
         occ = torch.zeros((2, molecule.nao), dtype=torch.float64, device=molecule.s1e.device)
         for s in range(2):
             occ[s, : nelecs[s]] = 1.0
@@ -712,9 +713,14 @@ def batched_scf_loop(functional: Functional, cycles: int = 25, precision: str = 
             main = torch.cuda.current_stream()
             for s in streams:
                 s.wait_stream(main)
+            # molecules sharing a cache (Molecule.replace keeps it; a molecule listed twice)
+            # replay the same captured graph into the same buffers: they share one stream, so
+            # their replays run one after another
+            lane = {}
             outs = []
-            for i, m in enumerate(molecules):
-                with torch.cuda.stream(streams[i % len(streams)]):
+            for m in molecules:
+                k = lane.setdefault(id(m.cache), len(lane) % len(streams))
+                with torch.cuda.stream(streams[k]):
                     outs.append(loop(params, m))
             for s in streams:
                 main.wait_stream(s)

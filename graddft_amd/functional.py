@@ -120,6 +120,8 @@ class XCPlan:
                                           self.workspace.data_ptr())
         _lib.check(rc, "gdx_xc_densities")
         self.generation += 1
+        # the densities pass overwrites the forward intermediates a backward would read
+        self.fwd_generation = -1
         return out[: m.grid_size]
 
     def backward(self, rdm1, params, g_exc, d_rdm1, d_params, generation: int) -> None:

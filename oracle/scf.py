@@ -47,6 +47,47 @@ def exchange_potential(D, eri):
     return np.einsum("abcd,bd->ac", eri, D)
 
 
+def _pair_index(n):
+    """pair(i, j) = i (i + 1) / 2 + j for i >= j, symmetric (n, n)."""
+    i, j = np.tril_indices(n)
+    pair = np.zeros((n, n), dtype=np.int64)
+    pair[i, j] = np.arange(len(i))
+    pair[j, i] = pair[i, j]
+    return pair
+
+
+def eri_pair_matrix(eri_packed, n):
+    """The 8-fold packed ERI as the symmetric (P, P) matrix E[pair(p,q), pair(r,s)] = (pq|rs),
+    P = n (n + 1) / 2 (memory 8 P^2 bytes instead of the dense tensor's 8 n^4)."""
+    P = n * (n + 1) // 2
+    E = np.zeros((P, P))
+    for I in range(P):
+        E[I, : I + 1] = eri_packed[I * (I + 1) // 2: I * (I + 1) // 2 + I + 1]
+    E = E + E.T
+    E[np.diag_indices(P)] *= 0.5
+    return E
+
+
+def coulomb_potential_pairs(D, E):
+    """J_pq = sum_rt (pq|rt) D_rt from the pair matrix (same as coulomb_potential)."""
+    n = D.shape[-1]
+    pair = _pair_index(n)
+    i, j = np.tril_indices(n)
+    d = np.where(i == j, 1.0, 2.0) * 0.5 * (D[i, j] + D[j, i])
+    return (E @ d)[pair]
+
+
+def exchange_potential_pairs(D, E):
+    """K_ac = sum_bd (ab|cd) D_bd from the pair matrix (same as exchange_potential)."""
+    n = D.shape[-1]
+    pair = _pair_index(n)
+    K = np.empty((n, n))
+    for a in range(n):
+        Ma = E[pair[a]][:, pair]            # (b, c, d) = (ab|cd)
+        K[a] = np.einsum("bcd,bd->c", Ma, D)
+    return K
+
+
 def non_xc(D, h1e, eri, e_nuc):
     """molecule.py:694-733 on D = D_up + D_dn."""
     J = coulomb_potential(D, eri)
@@ -65,14 +106,20 @@ class SCFInputs:
     e_nuc: float
     K0: Optional[np.ndarray]   # (2,n,n) exchange of the initial density, or None
     a0: float = 0.2
+    eri_pairs: Optional[np.ndarray] = None   # (P, P) pair matrix in place of the dense eri
+
+    def coulomb(self, D):
+        if self.eri_pairs is not None:
+            return coulomb_potential_pairs(D, self.eri_pairs)
+        return coulomb_potential(D, self.eri)
 
 
 def predict(xc_value_and_vxc: Callable, inp: SCFInputs, rdm1: np.ndarray):
     """train.py:123-216 with the stale-exchange B3LYP term: returns (E, fock (2,n,n))."""
     exc, vxc = xc_value_and_vxc(rdm1)
     Dt = rdm1.sum(axis=0)
-    J = coulomb_potential(Dt, inp.eri)
-    energy = exc + non_xc(Dt, inp.h1e, inp.eri, inp.e_nuc)
+    J = inp.coulomb(Dt)
+    energy = exc + inp.e_nuc + np.einsum("ij,ij->", Dt, inp.h1e) + np.einsum("pq,pq->", Dt, J) / 2.0
     fock = inp.h1e[None] + J[None] + vxc
     fock = abs_clip(fock)
     fock = 0.5 * (fock + fock.transpose(0, 2, 1))

@@ -79,6 +79,14 @@ def exchange_polarization_correction(e_PF, rho):
     return e_PF[:, 0] + (e_PF[:, 1] - e_PF[:, 0]) * _fzeta_x(zeta)
 
 
+# PW92 fits G(rs) = -2A (1 + alpha1 rs) ln(1 + 1 / (2A (beta1 rs^1/2 + beta2 rs + beta3 rs^3/2
+# + beta4 rs^2))) as (A, alpha1, beta1, beta2, beta3, beta4): the spin stiffness
+# (functional.py:1019-1024, used as alpha_c = -G), the paramagnetic and ferromagnetic gas
+# (popular_functionals.py:120-125).  Module-level so that tests can plant a mistake in one.
+PW92_ALPHA = (0.016887, 0.11125, 10.357, 3.6231, 0.88026, 0.49671)
+PW92_PARA = (0.031091, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294)
+PW92_FERRO = (0.015545, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517)
+
 # f''(0) of the correlation zeta function: (4/3)(1/3)*2 / (2(2^(1/3)-1))
 FZETA_PP0 = 4.0 / (9.0 * (2 ** (1.0 / 3.0) - 1))
 
@@ -96,9 +104,7 @@ def correlation_polarization_correction(e_tilde_PF, rho, clip_cte=CLIP):
         zp = torch.exp2(4 * torch.log2(1 + z) / 3)
         return (zm + zp - 2) / (2 * (2 ** (1 / 3) - 1))
 
-    A_ = 0.016887
-    alpha1 = 0.11125
-    beta1, beta2, beta3, beta4 = 10.357, 3.6231, 0.88026, 0.49671
+    A_, alpha1, beta1, beta2, beta3, beta4 = PW92_ALPHA
     ars = torch.exp2(math.log2(alpha1) + log_rs)
     brs_1_2 = torch.exp2(math.log2(beta1) + log_rs / 2)
     brs = torch.exp2(math.log2(beta2) + log_rs)
@@ -135,12 +141,7 @@ def b88_x_e(rho, grad_rho, clip_cte=CLIP):
 def pw92_c_e(rho, clip_cte=CLIP):
     """popular_functionals.py:105-139."""
     t = lambda *v: torch.tensor([list(v)], dtype=rho.dtype)  # noqa: E731
-    A_ = t(0.031091, 0.015545)
-    alpha1 = t(0.21370, 0.20548)
-    beta1 = t(7.5957, 14.1189)
-    beta2 = t(3.5876, 6.1977)
-    beta3 = t(1.6382, 3.3662)
-    beta4 = t(0.49294, 0.62517)
+    A_, alpha1, beta1, beta2, beta3, beta4 = (t(p, f) for p, f in zip(PW92_PARA, PW92_FERRO))
     log_rho = torch.log2(torch.clamp(rho.sum(dim=1, keepdim=True), min=clip_cte))
     log_rs = math.log2((3 / (4 * math.pi)) ** (1 / 3)) - log_rho / 3.0
     brs_1_2 = torch.exp2(log_rs / 2 + torch.log2(beta1))
@@ -152,6 +153,8 @@ def pw92_c_e(rho, clip_cte=CLIP):
     return correlation_polarization_correction(e_PF, rho, clip_cte) * rho.sum(dim=1)
 
 
+# VWN5 (paramagnetic, ferromagnetic), popular_functionals.py:158-161; A in the reference's
+# convention (the formula uses A / 2).
 VWN_A = (0.0621814, 0.0621814 / 2)
 VWN_B = (3.72744, 7.06042)
 VWN_C = (12.9352, 18.0578)

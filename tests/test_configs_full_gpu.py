@@ -1,0 +1,156 @@
+"""Workloads at the sizes they are specified at (GPU).
+
+* water def2-TZVP (nao 43), grid level 3, B3LYP: Functional.energy at the initial density and
+  the 15-cycle diff_scf_loop, FP64 device path against the oracle; FP32 against FP64; the
+  graph-replayed loop against the eager one;
+* naphthalene def2-SVP (nao 180), grid level 3 (203k points), packed ERI, 20-cycle
+  diff_scf_loop: B3LYP in FP64 / FP32 / split-BF16 with full-size properties of the result
+  (idempotent density in the S metric, electron count per spin, energies within the gate);
+  B3LYP plus the width-256 neural correction, energy and dE/dtheta in the three precisions;
+* naphthalene at grid level 1, B3LYP, 6 cycles: the FP64 device loop against the oracle
+  (packed-ERI pair matrix for J and the initial exchange, so the dense 8.4 GB tensor is never
+  formed).
+
+Gates: |dE| <= 1e-6 Ha, relative gradient error <= 1e-5 between precisions; FP64 against the
+oracle 1e-8 Ha.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _oracle_b3lyp(mol, pairs=False):
+    from graddft_amd.interface.integrals import unpack_eri
+    from oracle import scf as O
+    from oracle.ao import eval_ao
+    from oracle.xc import to_inputs, xc_value_and_grads
+    D0 = mol.cache["rdm1_init"].cpu().numpy()
+    eri = mol.rep_tensor.cpu().numpy()
+    if pairs:
+        E = O.eri_pair_matrix(eri, mol.nao)
+        K0 = np.stack([O.exchange_potential_pairs(D0[s], E) for s in range(2)])
+        inp = O.SCFInputs(mol.h1e.cpu().numpy(), mol.s1e.cpu().numpy(), None, mol.nuclear_repulsion, K0, 0.2,
+                          eri_pairs=E)
+    else:
+        dense = unpack_eri(eri, mol.nao)
+        K0 = np.stack([O.exchange_potential(D0[s], dense) for s in range(2)])
+        inp = O.SCFInputs(mol.h1e.cpu().numpy(), mol.s1e.cpu().numpy(), dense, mol.nuclear_repulsion, K0, 0.2)
+    xin = to_inputs(*eval_ao(mol.basis, mol.grid.coords.cpu().numpy()), mol.grid.weights.cpu().numpy())
+
+    def xc(rdm1):
+        e, g, _ = xc_value_and_grads("b3lyp", xin, rdm1)
+        return e, g
+
+    return inp, xc
+
+
+def _check_density(mol, D, nocc):
+    S = mol.s1e.cpu().numpy()
+    for s in range(2):
+        assert np.abs(D[s] @ S @ D[s] - D[s]).max() <= 1e-9          # idempotent in the S metric
+        assert abs(np.trace(D[s] @ S) - nocc) <= 1e-9                # N_e / 2 per spin
+
+
+def test_water_def2tzvp_level3_energy_and_15_cycle_loop():
+    _say("case: water def2-tzvp level 3 (nao 43), B3LYP: energy + 15-cycle diff_scf_loop vs oracle")
+    from graddft_amd import B3LYP, diff_scf_loop
+    from graddft_amd.interface import geometry
+    from graddft_amd.interface.builder import molecule_from_geometry
+    from oracle import scf as O
+    sym, xyz = geometry.water(np.random.default_rng(1))
+    mol = molecule_from_geometry(sym, xyz, "def2-tzvp", 3, with_eri=True)
+    assert mol.nao == 43
+    inp, xc = _oracle_b3lyp(mol)
+    D0 = mol.rdm1.cpu().numpy()
+    e_energy = B3LYP.energy(None, mol).item()
+    e0, _ = O.predict(xc, inp, D0)
+    _say(f"  Functional.energy {e_energy:.12f} oracle {e0:.12f}")
+    assert abs(e_energy - e0) <= 1e-8
+    out = diff_scf_loop(B3LYP, cycles=15)(None, mol)
+    st = O.diff_scf_loop(xc, inp, D0, mol.mo_occ.cpu().numpy(), 15)
+    _say(f"  15 cycles: {out.energy.item():.12f} oracle {st.energy:.12f}")
+    assert abs(out.energy.item() - st.energy) <= 1e-8
+    assert np.abs(out.rdm1.cpu().numpy() - st.rdm1).max() <= 1e-6
+    _check_density(mol, out.rdm1.cpu().numpy(), 5)
+    e32 = diff_scf_loop(B3LYP, cycles=15, precision="fp32")(None, mol).energy.item()
+    assert abs(e32 - out.energy.item()) <= 1e-6, (e32, out.energy.item())
+    g = diff_scf_loop(B3LYP, cycles=15, graph=True)
+    g(None, mol)
+    rep = g(None, mol)
+    assert abs(rep.energy.item() - out.energy.item()) <= 1e-12
+
+
+def _naphthalene(level):
+    from graddft_amd.interface import geometry
+    from graddft_amd.interface.builder import molecule_from_geometry
+    sym, xyz = geometry.naphthalene(np.random.default_rng(4))
+    mol = molecule_from_geometry(sym, xyz, "def2-svp", level, with_eri=True)
+    assert mol.nao == 180
+    return mol
+
+
+def test_naphthalene_level3_b3lyp_full_size():
+    _say("case: naphthalene def2-svp level 3 (nao 180), B3LYP, 20-cycle diff_scf_loop, fp64/fp32/bf16x6")
+    from graddft_amd import B3LYP, diff_scf_loop
+    mol = _naphthalene(3)
+    _say(f"  grid points {mol.grid_size}")
+    assert mol.grid_size > 200_000
+    out = {}
+    for prec in ("fp64", "fp32", "bf16x6"):
+        res = diff_scf_loop(B3LYP, cycles=20, precision=prec)(None, mol)
+        out[prec] = res.energy.item()
+        _say(f"  {prec}: E = {out[prec]:.10f}")
+        if prec == "fp64":
+            _check_density(mol, res.rdm1.cpu().numpy(), 34)
+    assert -386.0 < out["fp64"] < -383.0, out["fp64"]
+    for prec in ("fp32", "bf16x6"):
+        assert abs(out[prec] - out["fp64"]) <= 1e-6, (prec, out[prec], out["fp64"])
+
+
+def test_naphthalene_level3_neural_correction_full_size():
+    _say("case: naphthalene def2-svp level 3 (nao 180), B3LYP + neural correction 3x256, 20-cycle "
+         "diff_scf_loop, E and dE/dtheta, fp64/fp32/bf16x6")
+    from graddft_amd import B3LYPNeuralCorrection, MLPSpec, diff_scf_loop
+    mol = _naphthalene(3)
+    f = B3LYPNeuralCorrection(MLPSpec(nc=6, h=256, n_hidden=3, nd=4))
+    theta = f.init(seed=5)
+    out = {}
+    for prec in ("fp64", "fp32", "bf16x6"):
+        th = theta.detach().clone().requires_grad_(True)
+        res = diff_scf_loop(f, cycles=20, precision=prec)(th, mol)
+        g, = torch.autograd.grad(res.energy, [th])
+        out[prec] = (res.energy.item(), g.cpu().numpy())
+        _say(f"  {prec}: E = {out[prec][0]:.10f}")
+        if prec == "fp64":
+            _check_density(mol, res.rdm1.detach().cpu().numpy(), 34)
+    E0, g0 = out["fp64"]
+    assert np.isfinite(g0).all() and np.linalg.norm(g0) > 0
+    for prec in ("fp32", "bf16x6"):
+        E1, g1 = out[prec]
+        _say(f"  {prec} vs fp64: dE = {E1 - E0:.3e}, grad rel = {_rel(g1, g0):.3e}")
+        assert abs(E1 - E0) <= 1e-6, (prec, E1, E0)
+        assert _rel(g1, g0) <= 1e-5, (prec, _rel(g1, g0))
+
+
+def test_naphthalene_fp64_loop_matches_oracle():
+    _say("case: naphthalene def2-svp level 1 (nao 180), B3LYP, 6-cycle diff_scf_loop vs oracle")
+    from graddft_amd import B3LYP, diff_scf_loop
+    from oracle import scf as O
+    mol = _naphthalene(1)
+    out = diff_scf_loop(B3LYP, cycles=6)(None, mol)
+    inp, xc = _oracle_b3lyp(mol, pairs=True)
+    st = O.diff_scf_loop(xc, inp, mol.rdm1.cpu().numpy(), mol.mo_occ.cpu().numpy(), 6)
+    _say(f"  6 cycles: {out.energy.item():.12f} oracle {st.energy:.12f}")
+    assert abs(out.energy.item() - st.energy) <= 1e-8
+    assert np.abs(out.rdm1.cpu().numpy() - st.rdm1).max() <= 1e-6

@@ -73,3 +73,38 @@ def test_diis_history_and_ediis_adiis_weights():
     with pytest.raises(ValueError):
         Diis(torch.as_tensor(S), None, 3, "DIIS").run((torch.as_tensor(Ds[0]), torch.as_tensor(Fs[0]), -1.0),
                                                       dev.empty(), 0)
+
+
+class _StubLib:
+    """Stands in for libgdx.so in host-logic tests: every entry point succeeds, computes nothing."""
+
+    def __getattr__(self, name):
+        if name == "gdx_xc_workspace_size":
+            def ws(desc, nbytes):
+                nbytes._obj.value = 64
+                return 0
+            return ws
+        return lambda *a: 0
+
+
+def test_densities_pass_invalidates_pending_backward(monkeypatch):
+    """XCPlan.densities reuses the forward workspace: a backward of an earlier forward must
+    then refuse to run instead of reading overwritten intermediates."""
+    from graddft_amd import _lib
+    from graddft_amd.functional import XCPlan
+    from graddft_amd.molecule import Grid, Molecule
+    monkeypatch.setattr(_lib, "load", lambda *a: _StubLib())
+    monkeypatch.setattr(torch.cuda, "current_stream", lambda *a: type("S", (), {"cuda_stream": 0})())
+    n, N = 4, 8
+    z = lambda *s: torch.zeros(s, dtype=torch.float64)  # noqa: E731
+    mol = Molecule(grid=Grid(z(N, 3), z(N), z(N)), atom_index=torch.ones(1), nuclear_pos=z(1, 3),
+                   ao_planes=z(5, N, 16), rdm1=z(2, n, n), nuclear_repulsion=0.0, h1e=z(n, n),
+                   s1e=z(n, n), mo_coeff=z(2, n, n), mo_occ=z(2, n), mo_energy=z(2, n))
+    plan = XCPlan(mol, 0, None, "fp64")
+    exc = z()
+    plan.forward(mol.rdm1, None, exc)
+    gen = plan.generation
+    plan.backward(mol.rdm1, None, z(), z(2, n, n), None, gen)      # fine right after forward
+    plan.densities(mol.rdm1)
+    with pytest.raises(RuntimeError, match="overwritten"):
+        plan.backward(mol.rdm1, None, z(), z(2, n, n), None, gen)

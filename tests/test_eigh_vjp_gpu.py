@@ -1,0 +1,97 @@
+"""The device eigensolver's reverse mode (EighFn: device eigh + safe_eigh_vjp) against the
+oracle's SafeEigh (``grad_dft/utils/eigenproblem.py:54-103``), in the way of the reference's
+``tests/unit/test_eigenproblem.py:84-122``: Jacobians of (eigenvalues, eigenvectors) for
+random symmetric matrices n = 2..9 and 8 seeds, NaN-free Jacobians with a duplicated
+eigenvalue, the Lorentzian branch on near-degenerate 2x2 matrices, and a batch of two matrices
+in one call (the per-spin batch of the SCF loop)."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+SEEDS = [1984, 1993, 1945, 2001, 10, 29, 101, 1992]
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def rand_sym(n, seed):
+    a = np.random.default_rng(seed).normal(size=(n, n))
+    return 0.5 * (a + a.T)
+
+
+def degenerate_sym(n, seed):
+    rng = np.random.default_rng(seed)
+    w, V = np.linalg.eigh(rand_sym(n, seed))
+    i = int(rng.integers(0, n))
+    w[i] = w[i - 1]
+    return V @ np.diag(w) @ V.T
+
+
+def _jacobians(A):
+    from graddft_amd.eigenproblem import EighFn
+    from oracle.scf_grad import SafeEigh
+    jd = torch.autograd.functional.jacobian(lambda X: EighFn.apply(X), torch.tensor(A, device="cuda"))
+    jo = torch.autograd.functional.jacobian(lambda X: SafeEigh.apply(X), torch.tensor(A))
+    wd, Vd = EighFn.apply(torch.tensor(A, device="cuda"))
+    Vo = np.linalg.eigh(A)[1]
+    # eigenvector signs are arbitrary: align the device columns (and their Jacobians) to LAPACK's
+    sgn = np.sign((Vd.cpu().numpy() * Vo).sum(axis=0))
+    jV = jd[1].cpu().numpy() * sgn[None, :, None, None]
+    return (jd[0].cpu().numpy(), jV), (jo[0].numpy(), jo[1].numpy())
+
+
+@pytest.mark.parametrize("n", list(range(2, 10)))
+def test_eigh_jacobians_match_oracle(n):
+    _say(f"case: eigh reverse-mode Jacobian n={n}, 8 seeds")
+    for seed in SEEDS:
+        (jw, jV), (ow, oV) = _jacobians(rand_sym(n, seed))
+        assert np.abs(jw - ow).max() <= 1e-10, (n, seed)
+        assert np.abs(jV - oV).max() <= 1e-9 * max(1.0, np.abs(oV).max()), (n, seed)
+
+
+@pytest.mark.parametrize("n", list(range(2, 10)))
+def test_eigh_jacobians_degenerate_nan_free(n):
+    _say(f"case: eigh reverse-mode Jacobian, duplicated eigenvalue, n={n}")
+    from graddft_amd.eigenproblem import EighFn
+    for seed in SEEDS:
+        A = torch.tensor(degenerate_sym(n, seed), device="cuda")
+        jd = torch.autograd.functional.jacobian(lambda X: EighFn.apply(X), A)
+        assert not torch.isnan(jd[0]).any() and not torch.isnan(jd[1]).any(), (n, seed)
+
+
+@pytest.mark.parametrize("delta", [1e-3, 9e-7, 3e-7, 1e-8])
+def test_lorentzian_branch_matches_oracle(delta):
+    _say(f"case: safe_eigh_vjp 2x2, gap {delta}")
+    from graddft_amd.eigenproblem import safe_eigh_vjp
+    from oracle.scf_grad import SafeEigh
+    c, s = np.cos(0.3), np.sin(0.3)
+    V = np.array([[c, -s], [s, c]])
+    A = V @ np.diag([1.0, 1.0 + delta]) @ V.T
+    At = torch.tensor(A, requires_grad=True)
+    w, Vo = SafeEigh.apply(At)
+    gV = torch.zeros(2, 2, dtype=torch.float64)
+    gV[0, 1] = 1.0
+    g0, = torch.autograd.grad((Vo * gV).sum(), [At])
+    g1 = safe_eigh_vjp(w.detach().cuda(), Vo.detach().cuda(), torch.zeros(2, dtype=torch.float64, device="cuda"),
+                       gV.cuda())
+    assert np.abs(g1.cpu().numpy() - g0.numpy()).max() <= 1e-12 * max(1.0, np.abs(g0.numpy()).max())
+
+
+def test_batched_vjp_equals_per_matrix():
+    _say("case: EighFn backward on a batch of 2 (n=12) equals per-matrix backward")
+    from graddft_amd.eigenproblem import EighFn
+    A = torch.tensor(np.stack([rand_sym(12, 3), rand_sym(12, 4)]), device="cuda", requires_grad=True)
+    w, V = EighFn.apply(A)
+    rng = np.random.default_rng(1)
+    Gw = torch.tensor(rng.normal(size=(2, 12)), device="cuda")
+    GV = torch.tensor(rng.normal(size=(2, 12, 12)), device="cuda")
+    g, = torch.autograd.grad((w * Gw).sum() + (V * GV).sum(), [A])
+    for b in range(2):
+        Ab = A[b].detach().clone().requires_grad_(True)
+        wb, Vb = EighFn.apply(Ab)
+        sgn = torch.sign((Vb * V[b].detach()).sum(0))
+        gb, = torch.autograd.grad((wb * Gw[b]).sum() + (Vb * sgn[None, :] * GV[b]).sum(), [Ab])
+        assert (gb - g[b]).abs().max().item() <= 1e-12 * max(1.0, gb.abs().max().item())

@@ -118,3 +118,22 @@ def test_oracle_scf_reaches_self_consistency_h2o():
         comm = st.fock[s] @ Ds @ S - S @ Ds @ st.fock[s]
         assert np.abs(comm).max() < 1e-3
     assert -77.0 < st.energy < -75.0
+
+
+def test_pair_matrix_coulomb_and_exchange_equal_dense():
+    """The oracle's packed-ERI pair-matrix J and K (used where the dense n^4 tensor is too
+    large) equal the dense einsums of molecule.py:788-811 and the exchange K."""
+    from graddft_amd.interface import geometry
+    from graddft_amd.interface.basis import atomic_numbers, build_basis
+    from graddft_amd.interface.integrals import int2e_packed, unpack_eri
+    from oracle import scf as O
+    sym, xyz = geometry.water(np.random.default_rng(2))
+    bs = build_basis(sym, xyz, "def2-svp")
+    eri = int2e_packed(bs)
+    dense = unpack_eri(eri, bs.nao)
+    E = O.eri_pair_matrix(eri, bs.nao)
+    rng = np.random.default_rng(3)
+    D = rng.normal(size=(bs.nao, bs.nao))
+    for Dm in (D + D.T, D):
+        assert np.abs(O.coulomb_potential_pairs(Dm, E) - O.coulomb_potential(0.5 * (Dm + Dm.T), dense)).max() <= 1e-12
+        assert np.abs(O.exchange_potential_pairs(Dm, E) - O.exchange_potential(Dm, dense)).max() <= 1e-12

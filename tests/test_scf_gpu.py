@@ -59,13 +59,14 @@ def test_eigh_matches_lapack(n):
         assert np.abs(V[s].T @ V[s] - np.eye(n)).max() <= 1e-12
 
 
-def test_eigh_warm_start():
+@pytest.mark.parametrize("n", [43, 102, 180])
+def test_eigh_warm_start(n):
     """A warm start from the eigenvectors of a nearby matrix gives the same decomposition
-    in fewer sweeps (the SCF loop passes the previous cycle's eigenvectors)."""
-    _say("case: eigh warm start n=43")
+    (the SCF loop passes the previous cycle's eigenvectors; n = 43 keeps every buffer in LDS,
+    102 only the rotated matrix, 180 none)."""
+    _say(f"case: eigh warm start n={n}")
     import ctypes
     from graddft_amd import _lib
-    n = 43
     rng = np.random.default_rng(7)
     A = rng.normal(size=(2, n, n))
     A = A + A.transpose(0, 2, 1)
@@ -170,11 +171,12 @@ def test_diis_run_matches_oracle_with_gradients():
         assert np.abs(a - b).max() <= 1e-8 * max(1.0, np.abs(b).max())
 
 
-@pytest.mark.parametrize("n", [43, 50])
+@pytest.mark.parametrize("n", [43, 50, 102, 180])
 def test_scf_step_matches_unfused_path(n):
     """gdx_scf_step (extrapolation, eigenproblem, back-transform, density in one call; n = 43
-    in LDS, n = 50 in the workspace) against the separate path: einsum extrapolation,
-    orthogonal_eigh, einsum density -- cold and warm-started."""
+    with every buffer in LDS, 50 and 102 with the rotated matrix in LDS, 180 in the workspace)
+    against the separate path: einsum extrapolation, orthogonal_eigh, einsum density -- cold and
+    warm-started."""
     _say(f"case: scf_step fused vs unfused, n={n}")
     from graddft_amd.eigenproblem import orthogonal_eigh, scf_step
     rng = np.random.default_rng(n)
@@ -281,3 +283,24 @@ def test_graph_captured_scf_loop_replays_match_eager(kind):
         torch.cuda.synchronize()
         assert abs(out.energy.item() - ref.energy.item()) <= 1e-12, (i, out.energy.item(), ref.energy.item())
         assert torch.allclose(out.rdm1, ref.rdm1, rtol=0, atol=1e-11), i
+
+
+def test_batched_loop_with_shared_cache_matches_eager():
+    """batched_scf_loop with a molecule listed twice and a replace()d copy (all three share one
+    cache, hence one captured graph and its buffers) equals the molecule-by-molecule loop."""
+    _say("case: batched_scf_loop, shared cache [mol, mol.replace(rdm1=D2), mol]")
+    from graddft_amd import B3LYP, diff_scf_loop
+    from graddft_amd.evaluate import batched_scf_loop
+    mol = _water()
+    eager = diff_scf_loop(B3LYP, cycles=5)
+    ref1 = eager(None, mol)
+    D2 = (0.9 * mol.rdm1 + 0.1 * ref1.rdm1).contiguous()
+    mol2 = mol.replace(rdm1=D2)
+    ref2 = eager(None, mol2)
+    batched = batched_scf_loop(B3LYP, cycles=5, n_streams=3)
+    for _ in range(2):
+        outs = batched(None, [mol, mol2, mol])
+        torch.cuda.synchronize()
+        for o, r in zip(outs, [ref1, ref2, ref1]):
+            assert abs(o.energy.item() - r.energy.item()) <= 1e-12, (o.energy.item(), r.energy.item())
+            assert torch.allclose(o.rdm1, r.rdm1, rtol=0, atol=1e-11)
