@@ -13,7 +13,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgdx.so")
 
 GDX_FP64, GDX_FP32, GDX_BF16X6 = 0, 1, 2
-GDX_FUNC_B3LYP, GDX_FUNC_NEURAL_B3LYP, GDX_FUNC_B3LYP_NEURAL_CORR = 0, 1, 2
+GDX_FUNC_B3LYP, GDX_FUNC_NEURAL_B3LYP, GDX_FUNC_B3LYP_NEURAL_CORR, GDX_FUNC_LINEAR = 0, 1, 2, 3
+GDX_CORR_VWN5, GDX_CORR_PW92 = 0, 1
 
 
 class XCDesc(ctypes.Structure):
@@ -29,7 +30,8 @@ class XCDesc(ctypes.Structure):
         ("mlp_h", ctypes.c_int32),
         ("mlp_nhidden", ctypes.c_int32),
         ("mlp_nd", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("correlation", ctypes.c_int32),
+        ("coef", ctypes.c_double * 4),
     ]
 
 
@@ -54,6 +56,14 @@ SIGNATURES = {
     "gdx_scf_step_workspace_size": (ctypes.c_int, [_I, _I, ctypes.POINTER(_Z)]),
     "gdx_scf_step": (ctypes.c_int, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, ctypes.c_double,
                                     _P]),
+    "gdx_hf_plan_create": (ctypes.c_int, [_I, _P, _P, _P, _P, _I, ctypes.c_double, ctypes.POINTER(_P)]),
+    "gdx_hf_plan_destroy": (ctypes.c_int, [_P]),
+    "gdx_hf_plan_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    "gdx_hf_chi": (ctypes.c_int, [_P, _P, _I, _I, _P, _I, _P, ctypes.c_double, _P]),
+    "gdx_ao_matmul": (ctypes.c_int, [_P, _I, _I, _I, _I, _P, _I, _P, _P]),
+    "gdx_hf_energy_density": (ctypes.c_int, [_P, _I, _I, _I, _I, _P, _P, _P]),
+    "gdx_hf_fock_workspace_size": (ctypes.c_int, [_I, _I, _I, ctypes.POINTER(_Z)]),
+    "gdx_hf_fock": (ctypes.c_int, [_P, _I, _I, _I, _I, _P, _I, _P, _P, ctypes.c_int64, ctypes.c_double, _P, _P]),
     "gdx_profile_begin": (ctypes.c_int, []),
     "gdx_profile_end": (ctypes.c_int, [_I, ctypes.c_char_p, _P, _P, _P]),
     "gdx_xc_densities": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P]),

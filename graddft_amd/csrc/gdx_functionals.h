@@ -5,6 +5,7 @@
 // where-selects, so the derivatives match what jax.grad of the reference produces:
 //   lsda_x_e        grad_dft/popular_functionals.py:29-50
 //   b88_x_e         grad_dft/popular_functionals.py:52-103
+//   pw92_c_e        grad_dft/popular_functionals.py:105-139
 //   vwn_c_e         grad_dft/popular_functionals.py:141-195
 //   lyp_c_e         grad_dft/popular_functionals.py:197-269
 //   exchange_polarization_correction     grad_dft/functional.py:950-979
@@ -236,6 +237,28 @@ GDX_HD T vwn_c(const T& ru0, const T& rd0) {
   return correlation_pol(eP, eF, ru, rd) * (ru + rd);
 }
 
+// popular_functionals.py:105-139: Perdew-Wang 1992 fit G(rs) for the paramagnetic and the
+// ferromagnetic gas, combined by the polarisation correction, times the total density
+template <typename T>
+GDX_HD T pw92_eps(const T& log_rs, double A, double alpha1, double b1, double b2, double b3, double b4) {
+  const T brs_1_2 = fexp2(log_rs / 2.0 + log2(b1));
+  const T ars = fexp2(log_rs + log2(alpha1));
+  const T brs = fexp2(log_rs + log2(b2));
+  const T brs_3_2 = fexp2(3.0 * log_rs / 2.0 + log2(b3));
+  const T brs2 = fexp2(2.0 * log_rs + log2(b4));
+  return -2.0 * A * (1.0 + ars) * flog(1.0 + (1.0 / (2.0 * A)) / (brs_1_2 + brs + brs_3_2 + brs2));
+}
+template <typename T>
+GDX_HD T pw92_c(const T& ru, const T& rd) {
+  const T s = ru + rd;
+  const T log_rho = flog2(clipmin(s, CLIP));
+  const double l0 = log2(cbrt(3.0 / (4.0 * PI)));
+  const T log_rs = l0 - log_rho / 3.0;
+  const T eP = pw92_eps(log_rs, 0.031091, 0.21370, 7.5957, 3.5876, 1.6382, 0.49294);
+  const T eF = pw92_eps(log_rs, 0.015545, 0.20548, 14.1189, 6.1977, 3.3662, 0.62517);
+  return correlation_pol(eP, eF, ru, rd) * s;
+}
+
 // popular_functionals.py:197-269 (original LYP)
 template <typename T>
 GDX_HD T lyp_c(const T& ru0, const T& rd0, const T& gu, const T& gd, const T& lu, const T& ld) {
@@ -265,13 +288,14 @@ template <typename T>
 GDX_HD T abs_clip(const T& x) { return fabs(val(x)) > CLIP ? x : cst<T>(0.0); }
 
 // b3lyp_exhf_densities (popular_functionals.py:306-326) followed by the clip of
-// compute_densities (functional.py:184): e[0..3] = lda, b88, vwn, lyp.
+// compute_densities (functional.py:184): e[0..3] = lda, b88, vwn, lyp.  corr = 1 puts PW92
+// correlation (pw92_densities, :297-301) in the correlation slot e[2] in place of VWN.
 template <typename T>
 GDX_HD void b3lyp_parts(const T& ru, const T& rd, const T& gu, const T& gd, const T& lu,
-                        const T& ld, T* e) {
+                        const T& ld, int corr, T* e) {
   e[0] = abs_clip(lsda_x(ru, rd));
   e[1] = abs_clip(b88_x(ru, rd, gu, gd));
-  e[2] = abs_clip(vwn_c(ru, rd));
+  e[2] = abs_clip(corr ? pw92_c(ru, rd) : vwn_c(ru, rd));
   e[3] = abs_clip(lyp_c(ru, rd, gu, gd, lu, ld));
 }
 

@@ -40,6 +40,29 @@ constexpr int NT = 256;  // threads per block in every kernel here
 constexpr double SQUASH = 1e-4;
 constexpr int MAXND = 8;
 
+// Constant coefficients of a non-neural functional over the four energy-density slots
+// (lda, b88, vwn or pw92, lyp) and which correlation fills the third slot (0: VWN5, 1: PW92).
+struct LinCoef {
+  double c[4];
+  int corr;
+};
+inline bool is_neural(const gdx_xc_desc& d) {
+  return d.kind == GDX_FUNC_NEURAL_B3LYP || d.kind == GDX_FUNC_B3LYP_NEURAL_CORR;
+}
+inline LinCoef lin_coef(const gdx_xc_desc& d) {
+  LinCoef lc;
+  if (d.kind == GDX_FUNC_LINEAR) {
+    for (int f = 0; f < 4; ++f) lc.c[f] = d.coef[f];
+  } else {  // b3lyp_coefficients, grad_dft/popular_functionals.py:340-347 (without exact exchange)
+    lc.c[0] = 1.0 - 0.2;
+    lc.c[1] = 0.72;
+    lc.c[2] = 1.0 - 0.81;
+    lc.c[3] = 0.81;
+  }
+  lc.corr = d.correlation;
+  return lc;
+}
+
 __device__ __forceinline__ double clip30(double x) { return fabs(x) > 1e-30 ? x : 0.0; }
 __device__ __forceinline__ double elu_grad_from_out(double y) { return y > 0.0 ? 1.0 : y + 1.0; }
 
@@ -160,7 +183,7 @@ __global__ __launch_bounds__(256) void reduce_stage1_2_kernel(const TP* __restri
 template <typename T>
 __global__ __launch_bounds__(NT) void point_fwd_kernel(
     const double* __restrict__ dens_part, int nsd, int npts, int npts_pad,
-    const double* __restrict__ weights, int neural, double* __restrict__ dens_fin,
+    const double* __restrict__ weights, int neural, LinCoef lc, double* __restrict__ dens_fin,
     double* __restrict__ E, T* __restrict__ X, double* __restrict__ eps_part) {
   __shared__ double red[NT / 64];
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -182,7 +205,7 @@ __global__ __launch_bounds__(NT) void point_fwd_kernel(
     const double gu = d[0][1] * d[0][1] + d[0][2] * d[0][2] + d[0][3] * d[0][3];
     const double gd = d[1][1] * d[1][1] + d[1][2] * d[1][2] + d[1][3] * d[1][3];
     double e[4];
-    b3lyp_parts<double>(d[0][0], d[1][0], gu, gd, d[0][5], d[1][5], e);
+    b3lyp_parts<double>(d[0][0], d[1][0], gu, gd, d[0][5], d[1][5], lc.corr, e);
 #pragma unroll
     for (int f = 0; f < 4; ++f) E[(size_t)r * 4 + f] = e[f];
     if (neural) {
@@ -193,7 +216,7 @@ __global__ __launch_bounds__(NT) void point_fwd_kernel(
 #pragma unroll
       for (int j = 7; j < 16; ++j) X[(size_t)r * 16 + j] = (T)0;
     } else if (weights != nullptr && r < npts) {
-      const double c[4] = {1.0 - 0.2, 0.72, 1.0 - 0.81, 0.81};
+      const double* c = lc.c;
       double eps = c[0] * e[0] + c[1] * e[1] + c[2] * e[2] + c[3] * e[3];
       eps = clip30(clip30(eps));
       contrib = clip30(weights[r]) * eps;
@@ -384,10 +407,10 @@ template <typename T>
 __global__ __launch_bounds__(NT) void head_seed_kernel(
     const T* __restrict__ Cc, const double* __restrict__ E, const double* __restrict__ weights,
     int nd, int npts, int npts_pad, const double* __restrict__ g_exc, int neural,
-    double* __restrict__ GE, T* __restrict__ DZ16, double* __restrict__ dbpart, int head_mode) {
+    double* __restrict__ GE, T* __restrict__ DZ16, double* __restrict__ dbpart, int head_mode, LinCoef lc) {
   __shared__ double red[NT / 64];
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const double bc[4] = {1.0 - 0.2, 0.72, 1.0 - 0.81, 0.81};
+  const double* bc = lc.c;
   double dz[MAXND];
   for (int o = 0; o < MAXND; ++o) dz[o] = 0.0;
   if (r < npts_pad) {
@@ -428,7 +451,8 @@ __global__ void unpack_last_kernel(const double* __restrict__ dWt16, int h, int 
 template <typename T>
 __global__ __launch_bounds__(NT) void point_bwd_kernel(const double* __restrict__ dens_fin,
                                                        const double* __restrict__ GE, const T* __restrict__ DX,
-                                                       int nc, int npts, int npts_pad, double* __restrict__ vpt) {
+                                                       int nc, int npts, int npts_pad, int corr,
+                                                       double* __restrict__ vpt) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= npts_pad) return;
   double out[2][VPT_STRIDE];
@@ -442,7 +466,7 @@ __global__ __launch_bounds__(NT) void point_bwd_kernel(const double* __restrict_
     typedef Dual<6> D6;
     D6 e[4];
     b3lyp_parts<D6>(dvar<6>(d0[0], 0), dvar<6>(d1[0], 1), dvar<6>(gu, 2), dvar<6>(gd, 3),
-                    dvar<6>(d0[5], 4), dvar<6>(d1[5], 5), e);
+                    dvar<6>(d0[5], 4), dvar<6>(d1[5], 5), corr, e);
     double v[6] = {0, 0, 0, 0, 0, 0};
     for (int f = 0; f < 4; ++f) {
       const double ge = GE[(size_t)r * 4 + f];
@@ -495,7 +519,7 @@ __device__ __forceinline__ Dual<1> tan1(double v, double t) {
 template <typename T>
 __global__ __launch_bounds__(NT) void point_tangent_kernel(const double* __restrict__ dpart_t,
                                                            const double* __restrict__ dens_fin, int npts_pad,
-                                                           int neural, double* __restrict__ dfin_t,
+                                                           int neural, int corr, double* __restrict__ dfin_t,
                                                            double* __restrict__ Edot, T* __restrict__ Xdot) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= npts_pad) return;
@@ -521,7 +545,7 @@ __global__ __launch_bounds__(NT) void point_tangent_kernel(const double* __restr
   }
   Dual<1> e[4];
   b3lyp_parts<Dual<1>>(tan1(u[0][0], t[0][0]), tan1(u[1][0], t[1][0]), tan1(gam[0], gamt[0]),
-                       tan1(gam[1], gamt[1]), tan1(u[0][5], t[0][5]), tan1(u[1][5], t[1][5]), e);
+                       tan1(gam[1], gamt[1]), tan1(u[0][5], t[0][5]), tan1(u[1][5], t[1][5]), corr, e);
 #pragma unroll
   for (int f = 0; f < 4; ++f) Edot[(size_t)r * 4 + f] = e[f].d[0];
   if (neural) {
@@ -549,10 +573,11 @@ __global__ __launch_bounds__(NT) void hvp_seed_kernel(
     const T* __restrict__ Cc, const T* __restrict__ ZT, const double* __restrict__ E,
     const double* __restrict__ Edot, const double* __restrict__ weights, int nd, int npts, int npts_pad,
     const double* __restrict__ g_exc, int neural, double* __restrict__ GE, double* __restrict__ GEd,
-    T* __restrict__ DZ2, double* __restrict__ dbpart, double* __restrict__ edot_part, int head_mode) {
+    T* __restrict__ DZ2, double* __restrict__ dbpart, double* __restrict__ edot_part, int head_mode,
+    LinCoef lc) {
   __shared__ double red[NT / 64];
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const double bc[4] = {1.0 - 0.2, 0.72, 1.0 - 0.81, 0.81};
+  const double* bc = lc.c;
   double zb[MAXND];
   for (int o = 0; o < MAXND; ++o) zb[o] = 0.0;
   double ed_contrib = 0.0;
@@ -620,7 +645,8 @@ __global__ __launch_bounds__(NT) void point_hvp_bwd_kernel(const double* __restr
                                                            const double* __restrict__ dfin_t,
                                                            const double* __restrict__ GE,
                                                            const double* __restrict__ GEd, const T* __restrict__ DX2,
-                                                           int nc, int npts, int npts_pad, double* __restrict__ vpt) {
+                                                           int nc, int npts, int npts_pad, int corr,
+                                                           double* __restrict__ vpt) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= npts_pad) return;
   double out[2][VPT_STRIDE];
@@ -642,7 +668,8 @@ __global__ __launch_bounds__(NT) void point_hvp_bwd_kernel(const double* __restr
     H6 e[4];
     b3lyp_parts<H6>(dvar<6, Dual<1>>(tan1(u[0][0], t[0][0]), 0), dvar<6, Dual<1>>(tan1(u[1][0], t[1][0]), 1),
                     dvar<6, Dual<1>>(tan1(gam[0], gamt[0]), 2), dvar<6, Dual<1>>(tan1(gam[1], gamt[1]), 3),
-                    dvar<6, Dual<1>>(tan1(u[0][5], t[0][5]), 4), dvar<6, Dual<1>>(tan1(u[1][5], t[1][5]), 5), e);
+                    dvar<6, Dual<1>>(tan1(u[0][5], t[0][5]), 4), dvar<6, Dual<1>>(tan1(u[1][5], t[1][5]), 5), corr,
+                    e);
     // P_i = dPhi/dq_i, Pd_i = dPhi/dqdot_i over q = (rho_u, rho_d, gam_u, gam_d, lapl_u, lapl_d)
     double P[6] = {0, 0, 0, 0, 0, 0}, Pd[6] = {0, 0, 0, 0, 0, 0};
     for (int f = 0; f < 4; ++f) {
@@ -756,7 +783,7 @@ Layout make_layout(const gdx_xc_desc& d, bool hvp = false) {
   const size_t Np = d.npts_pad, nP = d.nao_pad;
   const size_t R = hvp ? 2 * Np : Np;  // rows of the activation-side buffers
   const size_t es = elem_size(d.precision);
-  const bool neural = d.kind != GDX_FUNC_B3LYP;
+  const bool neural = is_neural(d);
   const int hmode = d.kind == GDX_FUNC_B3LYP_NEURAL_CORR ? 1 : 0;
   L.nsd = d.spin_restricted ? 1 : 2;
   L.rows = (int)R;
@@ -841,13 +868,16 @@ int validate(const gdx_xc_desc* d) {
   GDX_REQUIRE(d->npts > 0 && d->nao > 0, "empty grid or basis");
   GDX_REQUIRE(d->npts_pad >= d->npts && d->npts_pad % 64 == 0, "npts_pad must be a multiple of 64 >= npts");
   GDX_REQUIRE(d->nao_pad >= d->nao && d->nao_pad % 16 == 0, "nao_pad must be a multiple of 16 >= nao");
-  GDX_REQUIRE(d->kind == GDX_FUNC_B3LYP || d->kind == GDX_FUNC_NEURAL_B3LYP || d->kind == GDX_FUNC_B3LYP_NEURAL_CORR,
+  GDX_REQUIRE(d->kind == GDX_FUNC_B3LYP || d->kind == GDX_FUNC_NEURAL_B3LYP || d->kind == GDX_FUNC_B3LYP_NEURAL_CORR ||
+                  d->kind == GDX_FUNC_LINEAR,
               "unknown functional kind");
+  GDX_REQUIRE(d->correlation == GDX_CORR_VWN5 || d->correlation == GDX_CORR_PW92,
+              "correlation must be GDX_CORR_VWN5 or GDX_CORR_PW92");
   if (d->precision != GDX_FP64 && d->precision != GDX_FP32 && d->precision != GDX_BF16X6) {
     set_error("unknown precision (GDX_FP64, GDX_FP32, GDX_BF16X6)");
     return GDX_ERR_UNSUPPORTED;
   }
-  if (d->kind != GDX_FUNC_B3LYP) {
+  if (is_neural(*d)) {
     GDX_REQUIRE(d->mlp_nc == 6, "coefficient inputs must be 6 (rho, |grad rho|^2, tau per spin)");
     GDX_REQUIRE(d->mlp_nd == 4, "MLP outputs must equal the 4 B3LYP energy densities");
     GDX_REQUIRE(d->mlp_h >= 16 && d->mlp_h % 16 == 0 && d->mlp_h <= 256,
@@ -929,7 +959,7 @@ template <typename T>
 void run_densities(const Ctx& c, const T* ao, const double* rdm1, const double* weights, double* dens_fin) {
   const gdx_xc_desc& d = c.d;
   const int Np = d.npts_pad, nP = d.nao_pad;
-  const bool neural = d.kind != GDX_FUNC_B3LYP;
+  const bool neural = is_neural(d);
   const int hmode = d.kind == GDX_FUNC_B3LYP_NEURAL_CORR ? 1 : 0;
   hipStream_t st = c.st;
   GDX_LAUNCH("pad_dm", pad_dm_kernel<T>, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, rdm1, d.nao, nP,
@@ -937,7 +967,7 @@ void run_densities(const Ctx& c, const T* ao, const double* rdm1, const double* 
   GDX_LAUNCH(tname<T>("density_f64", "density_f32"), density_ao_kernel<T>, dim3(Np / DN_PT, c.L.nsd), dim3(NT), 0,
              st, ao, Np, nP, c.P<T>(c.L.Dp), c.P<double>(c.L.dens_part));
   GDX_LAUNCH("point_fwd", point_fwd_kernel<T>, dim3(c.L.npt_blocks), dim3(NT), 0, st, c.P<double>(c.L.dens_part),
-             c.L.nsd, d.npts, Np, weights, neural ? 1 : 0, dens_fin, c.P<double>(c.L.E), c.P<T>(c.L.X),
+             c.L.nsd, d.npts, Np, weights, neural ? 1 : 0, lin_coef(d), dens_fin, c.P<double>(c.L.E), c.P<T>(c.L.X),
              c.P<double>(c.L.eps_part));
 }
 
@@ -1058,7 +1088,7 @@ int forward_impl(const Ctx& c, const T* ao, const double* weights, const double*
   const gdx_xc_desc& d = c.d;
   const Layout& L = c.L;
   hipStream_t st = c.st;
-  const bool neural = d.kind != GDX_FUNC_B3LYP;
+  const bool neural = is_neural(d);
   const int hmode = d.kind == GDX_FUNC_B3LYP_NEURAL_CORR ? 1 : 0;
   const int Np = d.npts_pad;
   run_densities<T>(c, ao, rdm1, weights, c.P<double>(L.dens_fin));
@@ -1111,7 +1141,7 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
   const gdx_xc_desc& d = c.d;
   const Layout& L = c.L;
   hipStream_t st = c.st;
-  const bool neural = d.kind != GDX_FUNC_B3LYP;
+  const bool neural = is_neural(d);
   const int hmode = d.kind == GDX_FUNC_B3LYP_NEURAL_CORR ? 1 : 0;
   const int Np = d.npts_pad, nP = d.nao_pad;
   const int nd = 4;
@@ -1129,7 +1159,7 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
     const T* W016 = c.P<T>(L.W16);
     GDX_LAUNCH("head_seed", head_seed_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, c.P<T>(L.C),
                c.P<double>(L.E), weights, nd, d.npts, Np, g_exc, 1, c.P<double>(L.GE), c.P<T>(L.DZ),
-               c.P<double>(L.dbpart), hmode);
+               c.P<double>(L.dbpart), hmode, lin_coef(d));
     const int rwblk = (Np + RW_ROWS - 1) / RW_ROWS;
     if (rows_path(h)) {
       GDX_LAUNCH("head_delta", delta_wgrad_rows_kernel<T>, dim3(rwblk), dim3(NT), 0, st, Hlast, h, Np, c.P<T>(L.DZ),
@@ -1208,11 +1238,12 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
     }
   } else {
     GDX_LAUNCH("head_seed", head_seed_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, nullptr, c.P<double>(L.E),
-               weights, nd, d.npts, Np, g_exc, 0, c.P<double>(L.GE), nullptr, nullptr, 0);
+               weights, nd, d.npts, Np, g_exc, 0, c.P<double>(L.GE), nullptr, nullptr, 0, lin_coef(d));
   }
   if (d_rdm1) {
     GDX_LAUNCH("point_bwd", point_bwd_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, c.P<double>(L.dens_fin),
-               c.P<double>(L.GE), neural ? c.P<T>(L.DX) : nullptr, d.mlp_nc, d.npts, Np, c.P<double>(L.vpt));
+               c.P<double>(L.GE), neural ? c.P<T>(L.DX) : nullptr, d.mlp_nc, d.npts, Np, d.correlation,
+               c.P<double>(L.vpt));
     // B3LYP on equal spin densities has equal spin potentials: form V once
     launch_vxc<T>(c, ao, (!neural && d.spin_restricted) ? 1 : 2, d_rdm1);
   }
@@ -1227,7 +1258,7 @@ int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm
   const gdx_xc_desc& d = c.d;
   const Layout& L = c.L;
   hipStream_t st = c.st;
-  const bool neural = d.kind != GDX_FUNC_B3LYP;
+  const bool neural = is_neural(d);
   const int hmode = d.kind == GDX_FUNC_B3LYP_NEURAL_CORR ? 1 : 0;
   const int Np = d.npts_pad, nP = d.nao_pad;
   const int nd = 4;
@@ -1241,7 +1272,7 @@ int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm
              ao, Np, nP, c.P<T>(L.Dp2), c.P<double>(L.dpart_t));
   T* X2 = c.P<T>(L.X);
   GDX_LAUNCH("point_tangent", point_tangent_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st,
-             c.P<double>(L.dpart_t), c.P<double>(L.dens_fin), Np, neural ? 1 : 0, c.P<double>(L.dfin_t),
+             c.P<double>(L.dpart_t), c.P<double>(L.dens_fin), Np, neural ? 1 : 0, d.correlation, c.P<double>(L.dfin_t),
              c.P<double>(L.Edot), X2 + (size_t)Np * 16);
   const ParamOffsets po = neural ? param_offsets(d) : ParamOffsets{};
   const int h = neural ? d.mlp_h : 0;
@@ -1275,7 +1306,7 @@ int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm
   GDX_LAUNCH("hvp_seed", hvp_seed_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, neural ? c.P<T>(L.C) : nullptr,
              neural ? c.P<T>(L.ZT) : nullptr, c.P<double>(L.E), c.P<double>(L.Edot), weights, nd, d.npts, Np, g_exc,
              neural ? 1 : 0, c.P<double>(L.GE), c.P<double>(L.GEd), neural ? c.P<T>(L.DZ) : nullptr,
-             c.P<double>(L.dbpart), c.P<double>(L.edot_part), hmode);
+             c.P<double>(L.dbpart), c.P<double>(L.edot_part), hmode, lin_coef(d));
   if (edot)
     GDX_LAUNCH("reduce_scalar", reduce_scalar_kernel, dim3(1), dim3(NT), 0, st, c.P<double>(L.edot_part),
                L.npt_blocks, edot);
@@ -1337,7 +1368,7 @@ int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm
   if (d_rdm1) {
     GDX_LAUNCH("point_hvp_bwd", point_hvp_bwd_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st,
                c.P<double>(L.dens_fin), c.P<double>(L.dfin_t), c.P<double>(L.GE), c.P<double>(L.GEd),
-               neural ? c.P<T>(L.DX) : nullptr, d.mlp_nc, d.npts, Np, c.P<double>(L.vpt));
+               neural ? c.P<T>(L.DX) : nullptr, d.mlp_nc, d.npts, Np, d.correlation, c.P<double>(L.vpt));
     launch_vxc<T>(c, ao, 2, d_rdm1, true);
   }
   GDX_HIP_CHECK(hipGetLastError());
@@ -1363,7 +1394,7 @@ int gdx_xc_forward(void* stream, const gdx_xc_desc* desc, const void* ao, const 
                    const double* rdm1, const double* params, double* exc, void* workspace) {
   int rc = validate(desc);
   if (rc != GDX_OK) return rc;
-  const bool neural = desc->kind != GDX_FUNC_B3LYP;
+  const bool neural = is_neural(*desc);
   GDX_REQUIRE(ao && weights && rdm1 && exc && workspace, "gdx_xc_forward: null pointer");
   GDX_REQUIRE(!neural || params, "gdx_xc_forward: neural functional needs params");
   Ctx c{*desc, make_layout(*desc), (char*)workspace, (hipStream_t)stream};
@@ -1397,7 +1428,7 @@ int gdx_xc_hvp(void* stream, const gdx_xc_desc* desc, const void* ao, const doub
                double* d_rdm1, double* d_params, void* workspace) {
   int rc = validate(desc);
   if (rc != GDX_OK) return rc;
-  const bool neural = desc->kind != GDX_FUNC_B3LYP;
+  const bool neural = is_neural(*desc);
   GDX_REQUIRE(ao && weights && rdm1 && dir && g_exc && workspace, "gdx_xc_hvp: null pointer");
   GDX_REQUIRE(!neural || params, "gdx_xc_hvp: neural functional needs params");
   Ctx c{*desc, make_layout(*desc, true), (char*)workspace, (hipStream_t)stream};
@@ -1411,7 +1442,7 @@ int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const void* ao, const
                     double* d_params, void* workspace) {
   int rc = validate(desc);
   if (rc != GDX_OK) return rc;
-  const bool neural = desc->kind != GDX_FUNC_B3LYP;
+  const bool neural = is_neural(*desc);
   GDX_REQUIRE(ao && weights && rdm1 && g_exc && workspace, "gdx_xc_backward: null pointer");
   GDX_REQUIRE(!neural || params, "gdx_xc_backward: neural functional needs params");
   Ctx c{*desc, make_layout(*desc), (char*)workspace, (hipStream_t)stream};

@@ -486,8 +486,10 @@ def scf_loop(functional: Functional, level_shift_factor=(0.0, 0.0), damp_factor=
     carried in (``Molecule.get_mo_grads`` on the loop's molecule, whose ``fock`` field the loop
     does not update); a molecule without one uses the loop's current Fock matrix.  Smearing
     uses one chemical potential for the spin orbitals of both spins and keeps the (2, n)
-    occupation layout.  The range-separated ``chi`` update is not part of this path
-    (exact exchange is the packed-ERI term).  Not differentiable (runs without autograd)."""
+    occupation layout.  A molecule built with ``omegas`` has its ``chi`` tensor rebuilt on the
+    device from every new density (``evaluate.py:510-522, 565-576``), so its semi-numerical
+    exact exchange follows the current density; without chi the exchange is the packed-ERI
+    term of the initial density.  Not differentiable (runs without autograd)."""
     compute_energy = energy_predictor(functional, precision=precision, **kwargs)
     spin_equal = damp_factor[0] == damp_factor[1] and level_shift_factor[0] == level_shift_factor[1]
 
@@ -500,6 +502,9 @@ def scf_loop(functional: Functional, level_shift_factor=(0.0, 0.0), damp_factor=
         molecule = molecule.replace(mo_occ=mo_occ)
         molecule = molecule.replace(rdm1=molecule.make_rdm1(),
                                     restricted=molecule.restricted and functional.spin_symmetric and spin_equal)
+        if molecule.omegas is not None and molecule.chi_grid is not None:
+            from .hf import generate_chi_tensor
+            molecule = molecule.replace(chi_grid=generate_chi_tensor(molecule, molecule.rdm1, molecule.omegas))
         return molecule, V
 
     def scf_iterator(params: Optional[torch.Tensor], molecule: Molecule) -> Molecule:

@@ -62,7 +62,8 @@ class XCPlan:
     The workspace carries forward intermediates to the backward call, so a plan serves one
     forward/backward pair at a time; ``generation`` guards against interleaving."""
 
-    def __init__(self, molecule: Molecule, kind: int, spec: Optional[MLPSpec], precision: str):
+    def __init__(self, molecule: Molecule, kind: int, spec: Optional[MLPSpec], precision: str,
+                 coef=None, correlation: int = 0):
         lib = _lib.load()
         d = _lib.XCDesc()
         d.npts = molecule.grid_size
@@ -74,6 +75,10 @@ class XCPlan:
         d.precision = PRECISIONS[precision]
         if spec is not None:
             d.mlp_nc, d.mlp_h, d.mlp_nhidden, d.mlp_nd = spec.nc, spec.h, spec.n_hidden, spec.nd
+        d.correlation = int(correlation)
+        if coef is not None:
+            for f, c in enumerate(coef):
+                d.coef[f] = float(c)
         self.desc = d
         nbytes = ctypes.c_size_t(0)
         _lib.check(lib.gdx_xc_workspace_size(ctypes.byref(d), ctypes.byref(nbytes)), "gdx_xc_workspace_size")
@@ -217,6 +222,8 @@ class Functional:
     kind = _lib.GDX_FUNC_B3LYP
     spec: Optional[MLPSpec] = None
     spin_symmetric = True
+    coef = None                       # constant coefficients of GDX_FUNC_LINEAR
+    correlation = _lib.GDX_CORR_VWN5  # correlation in the third energy-density slot
 
     def __init__(self, name: str = "B3LYP", is_xc: bool = True, exact_exchange: float = 0.0):
         self.name = name
@@ -225,10 +232,10 @@ class Functional:
         self.exact_exchange = exact_exchange
 
     def plan(self, molecule: Molecule, precision: str = "fp64") -> XCPlan:
-        key = ("xcplan", self.kind, self.spec, precision, molecule.restricted)
+        key = ("xcplan", self.kind, self.spec, precision, molecule.restricted, self.coef, self.correlation)
         p = molecule.cache.get(key)
         if p is None or p.molecule.ao_planes is not molecule.ao_planes:
-            p = XCPlan(molecule, self.kind, self.spec, precision)
+            p = XCPlan(molecule, self.kind, self.spec, precision, self.coef, self.correlation)
             molecule.cache[key] = p
         p.molecule = molecule
         return p
@@ -281,3 +288,22 @@ class B3LYPNeuralCorrection(NeuralFunctional):
 
     def __init__(self, spec: MLPSpec = MLPSpec(), name: str = "b3lyp_neural_corr", is_xc: bool = True):
         super().__init__(spec=spec, name=name, is_xc=is_xc, exact_exchange=0.2)
+
+
+class LinearFunctional(Functional):
+    """A functional with constant coefficients over the grid energy densities (the reference's
+    ``Functional(coefficients=lambda self, *_: jnp.array([[1.0]]), energy_densities=...)``
+    built-ins, ``popular_functionals.py:375-393``): E = sum_r w clip(sum_f c_f e_f) with
+    c over the slots (lda, b88, correlation, lyp); ``correlation`` is "vwn" (VWN5) or "pw92"."""
+
+    kind = _lib.GDX_FUNC_LINEAR
+
+    def __init__(self, coef, correlation: str = "vwn", name: str = "linear", is_xc: bool = True,
+                 exact_exchange: float = 0.0):
+        super().__init__(name=name, is_xc=is_xc, exact_exchange=exact_exchange)
+        if len(coef) != 4:
+            raise ValueError("coef: four coefficients (lda, b88, correlation, lyp)")
+        self.coef = tuple(float(c) for c in coef)
+        if correlation not in ("vwn", "pw92"):
+            raise ValueError("correlation must be 'vwn' or 'pw92'")
+        self.correlation = _lib.GDX_CORR_PW92 if correlation == "pw92" else _lib.GDX_CORR_VWN5

@@ -56,8 +56,11 @@ def core_guess(h1e: np.ndarray, s1e: np.ndarray, nocc: int):
 def molecule_from_geometry(symbols: Sequence[str], coords_bohr: np.ndarray, basis: str,
                            grid_level: int = 2, charge: int = 0, with_eri: bool = False,
                            device: str = "cuda", rdm1: Optional[np.ndarray] = None,
-                           name: str = "", npts_align: int = 128) -> Molecule:
-    """Closed-shell molecule with a core-Hamiltonian (or given) density, AOs on device."""
+                           name: str = "", npts_align: int = 128,
+                           omegas: Optional[Sequence[float]] = None) -> Molecule:
+    """Closed-shell molecule with a core-Hamiltonian (or given) density, AOs on device.  With
+    `omegas` the chi tensor of that density is built on the device, as ``molecule_from_pyscf``
+    does when given omegas (pyscf.py:152-160); B3LYP's exact exchange then uses it."""
     coords_bohr = np.asarray(coords_bohr, dtype=np.float64)
     Z = atomic_numbers(symbols)
     bs = build_basis(symbols, coords_bohr, basis)
@@ -102,4 +105,8 @@ def molecule_from_geometry(symbols: Sequence[str], coords_bohr: np.ndarray, basi
         name=name,
     )
     mol.cache["rdm1_init"] = mol.rdm1
+    if omegas is not None:
+        from ..hf import generate_chi_tensor
+        om = tuple(float(o) for o in omegas)
+        mol = mol.replace(omegas=om, chi_grid=generate_chi_tensor(mol, mol.rdm1, om))
     return mol

@@ -58,6 +58,8 @@ class Molecule:
     restricted: bool = True
     grid_level: int = 2
     name: str = ""
+    omegas: Optional[tuple] = None          # range-separation parameters chi was built for
+    chi_grid: Optional[torch.Tensor] = None  # (nomega, 2, n, npts_pad): chi, grid points contiguous
     cache: dict = field(default_factory=dict, repr=False, compare=False)
 
     # -- shapes ---------------------------------------------------------------------
@@ -89,6 +91,13 @@ class Molecule:
     @property
     def lapl_ao(self) -> torch.Tensor:
         return self.ao_planes[4, : self.grid_size, : self.nao]
+
+    @property
+    def chi(self) -> Optional[torch.Tensor]:
+        """chi in the reference's layout (N, nomega, 2, n) (molecule.py:82-102), a view."""
+        if self.chi_grid is None:
+            return None
+        return self.chi_grid[..., : self.grid_size].permute(3, 0, 1, 2)
 
     def replace(self, **kw) -> "Molecule":
         """Functional update, as flax ``struct.dataclass.replace``.  The per-molecule
@@ -136,6 +145,34 @@ class Molecule:
         from .scf import non_xc_energy
         with torch.no_grad():
             return non_xc_energy(self, self.rdm1.detach())
+
+    # -- semi-numerical exact exchange (molecule.py:153-250) --------------------------------
+    def select_HF_omegas(self, omegas) -> torch.Tensor:  # noqa: N802  (reference name)
+        """chi rows for `omegas`, device layout (nomega, 2, n, npts_pad); ValueError when chi is
+        missing or lacks an omega, as the reference."""
+        from .hf import select_HF_omegas
+        return select_HF_omegas(self, omegas)
+
+    def HF_energy_density(self, omegas) -> torch.Tensor:  # noqa: N802
+        """e_HF (nomega, 2, N) at the current rdm1 with the stored chi."""
+        from .hf import hf_energy_density
+        with torch.no_grad():
+            return hf_energy_density(self, omegas)
+
+    def HF_density_grad_2_Fock(self, functional=None, params=None, omegas=(0.0,), ehf=None,  # noqa: N802
+                               coefficient_inputs=None, densities_wout_hf=None, gr=None) -> torch.Tensor:
+        """Fock contribution of the e_HF dependence, (nomega, 2, n, n):
+        V = -1/2 sum_r gr phi_a chi_b with gr = dE_xc / d e_HF.  The reference obtains gr by
+        differentiating xc_energy; here gr is passed, or, for a functional whose e_HF
+        coefficient is the constant a0 = functional.exact_exchange (B3LYP), it is a0 w_r at
+        omega = 0 (and 0 for other omegas)."""
+        from .hf import hf_density_grad_2_fock
+        if gr is None:
+            a0 = float(getattr(functional, "exact_exchange", 0.0))
+            w = self.grid.weights.to(torch.float64)
+            gr = torch.stack([torch.stack([a0 * w if o == 0.0 else 0.0 * w] * 2) for o in omegas])
+        with torch.no_grad():
+            return hf_density_grad_2_fock(self, gr, omegas)
 
     # -- orbital helpers (molecule.py:302-330) ------------------------------------------
     def make_rdm1(self) -> torch.Tensor:

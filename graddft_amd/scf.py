@@ -65,8 +65,22 @@ def exchange(molecule: Molecule, D: torch.Tensor) -> torch.Tensor:
 
 
 def initial_exchange(molecule: Molecule) -> torch.Tensor:
-    """K[D0_s] of the density the molecule was built with (cached): the reference freezes
-    its exact-exchange tensor chi at that density (interface/pyscf.py:152-160)."""
+    """Exchange matrix of the density the molecule was built with (cached), K0 (2, n, n).
+
+    With a chi tensor at omega = 0 on the molecule this is the reference's semi-numerical term:
+    K~_s[a][b] = sum_r w_r phi_a(r) chi[r, 0, s, b] (``HF_energy_density`` /
+    ``HF_density_grad_2_Fock`` with B3LYP's constant coefficient, molecule.py:507-613); otherwise
+    the exchange of the initial density from the packed ERI, K[D0_s], which the grid form
+    approximates to the quadrature error.  Both are frozen at the initial density, as the
+    reference's chi (interface/pyscf.py:152-160)."""
+    if molecule.chi_grid is not None and molecule.omegas is not None and 0.0 in molecule.omegas:
+        hit = molecule.cache.get("K0_chi")
+        if hit is not None and hit[0] is molecule.chi_grid:
+            return hit[1]
+        from .hf import exchange_matrix
+        K0 = exchange_matrix(molecule)
+        molecule.cache["K0_chi"] = (molecule.chi_grid, K0)
+        return K0
     K0 = molecule.cache.get("K0")
     if K0 is None:
         D0 = molecule.cache.get("rdm1_init", molecule.rdm1)

@@ -44,7 +44,16 @@ enum { GDX_FP64 = 0, GDX_FP32 = 1, GDX_BF16X6 = 2 };
  *                        c = 2 sigmoid(z / 2) (NeuralFunctional, grad_dft/functional.py:345-419).
  *  GDX_FUNC_B3LYP_NEURAL_CORR  as GDX_FUNC_NEURAL_B3LYP with a linear head around the B3LYP
  *                        coefficients, c_f = c_B3LYP,f + z_f: B3LYP plus a neural correction. */
-enum { GDX_FUNC_B3LYP = 0, GDX_FUNC_NEURAL_B3LYP = 1, GDX_FUNC_B3LYP_NEURAL_CORR = 2 };
+enum { GDX_FUNC_B3LYP = 0, GDX_FUNC_NEURAL_B3LYP = 1, GDX_FUNC_B3LYP_NEURAL_CORR = 2, GDX_FUNC_LINEAR = 3 };
+
+/* GDX_FUNC_LINEAR: E = sum_r w * clip(sum_f coef[f] e_f(r)) over the four energy-density slots
+ * (lda, b88, correlation, lyp) with constant coefficients desc->coef -- the reference's
+ * built-in single-part functionals (grad_dft/popular_functionals.py:375-393): LSDA = (1,0,0,0),
+ * B88 = (1,1,0,0) (b88_density is [lda, b88] under a unit coefficient), VWN = (0,0,1,0) with
+ * GDX_CORR_VWN5, PW92 = (0,0,1,0) with GDX_CORR_PW92, LYP = (0,0,0,1).
+ * desc->correlation selects the correlation slot for every kind: VWN5 (vwn_c_e, :141-195) or
+ * PW92 (pw92_c_e, :105-139). */
+enum { GDX_CORR_VWN5 = 0, GDX_CORR_PW92 = 1 };
 
 /* Shape and functional description of one XC evaluation. */
 typedef struct gdx_xc_desc {
@@ -59,7 +68,8 @@ typedef struct gdx_xc_desc {
   int32_t mlp_h;           /* hidden width */
   int32_t mlp_nhidden;     /* hidden layers (dense + ELU each) */
   int32_t mlp_nd;          /* outputs = energy densities (4) */
-  int32_t reserved;
+  int32_t correlation;     /* GDX_CORR_VWN5 | GDX_CORR_PW92 */
+  double coef[4];          /* GDX_FUNC_LINEAR coefficients (ignored by the other kinds) */
 } gdx_xc_desc;
 
 const char* gdx_last_error(void);
@@ -188,6 +198,52 @@ int gdx_diis_solve(void* stream, int32_t nspin, int32_t m1, const double* B, dou
  * (nspin, m+1) are written.  m + 1 <= 16. */
 int gdx_diis_coefficients(void* stream, int32_t nspin, int32_t m, int64_t n2, const double* errs, int32_t cycle,
                           double* B, double* x);
+
+/* ---- Semi-numerical exact exchange ----------------------------------------------------
+ * nu_ab(r) = int dr' phi_a(r') f(|r - r'|) phi_b(r') at grid points r (f = 1/|r-r'|, or
+ * erf(omega |r-r'|)/|r-r'| for omega > 0): PySCF's int1e_grids_sph as called by
+ * grad_dft/external/_hf_density.py:18-67.  Device layouts: u, chi, v are (nspin, nao, npts_pad)
+ * (orbital-major, grid points contiguous); the reference's chi (N, omega, 2, n) is a permuted
+ * view of one (2, n, npts_pad) array per omega. */
+
+/* Plan for a basis (HOST arrays in the gdx_ao_eval packing, shells up to f): the Hermite
+ * expansion of every primitive pair of every ordered shell pair, folded with the contraction
+ * coefficients and component matrices, uploaded once.  Primitive pairs whose prefactor
+ * (2 pi / p) |c_a c_b| exp(-ab/p |A-B|^2) is below `screen` are dropped (0 keeps all).  The
+ * plan allocates device memory here, never in gdx_hf_chi; free it with gdx_hf_plan_destroy. */
+int gdx_hf_plan_create(int32_t nshell, const int32_t* shell_i32, const double* shell_f64, const double* prim_f64,
+                       const double* tmat_f64, int32_t nao, double screen, void** plan);
+int gdx_hf_plan_destroy(void* plan);
+int gdx_hf_plan_info(const void* plan, int64_t* nentries, int64_t* mdoubles);
+
+/* chi[s][b][r] = sum_a u[s][a][r] nu_ab(r) for r < npts (zero beyond): with u = D0_s^T phi(r)
+ * (gdx_ao_matmul of the initial density) this is generate_chi_tensor's
+ * chi[r, omega, s, b] = sum_ad D0_s[a][d] phi_a(r) nu_db(r), grad_dft/interface/pyscf.py:1067-1126.
+ * coords (npts, 3) bohr.  Bitwise reproducible (no atomics). */
+int gdx_hf_chi(void* stream, const void* plan, int32_t npts, int32_t npts_pad, const double* coords, int32_t nspin,
+               const double* u, double omega, double* chi);
+
+/* out[s][c][r] = sum_a phi[r][a] M_s[a][c]: phi is the AO value plane (npts_pad, nao_pad) of
+ * gdx_ao_eval, M (nspin, nao, nao).  With M = D_s: the orbital-side factor of chi (D0) and of
+ * HF_energy_density (current D), grad_dft/molecule.py:507-541. */
+int gdx_ao_matmul(void* stream, int32_t npts, int32_t npts_pad, int32_t nao, int32_t nao_pad, const double* phi,
+                  int32_t nspin, const double* M, double* out);
+
+/* e[s][r] = -1/2 sum_c chi[s][c][r] v[s][c][r] with v = gdx_ao_matmul(D): HF_energy_density,
+ * grad_dft/molecule.py:507-541 (one omega).  e (nspin, npts_pad). */
+int gdx_hf_energy_density(void* stream, int32_t npts, int32_t npts_pad, int32_t nao, int32_t nspin, const double* chi,
+                          const double* v, double* e);
+
+/* out[s][a][b] = scale * sum_r g_s(r) phi_a(r) chi[s][b][r], g_s = g + s * g_spin_stride: with
+ * g = gr (dE_xc / d e_HF per point) and scale = -1/2 this is HF_density_grad_2_Fock,
+ * grad_dft/molecule.py:545-613 (one omega); with g = the grid weights and scale = 1 it is the
+ * semi-numerical exchange matrix K~ of B3LYP's constant coefficient.  Partial sums over point
+ * chunks are reduced in a fixed order (bitwise reproducible).  workspace:
+ * gdx_hf_fock_workspace_size bytes. */
+int gdx_hf_fock_workspace_size(int32_t npts, int32_t nao, int32_t nspin, size_t* bytes);
+int gdx_hf_fock(void* stream, int32_t npts, int32_t npts_pad, int32_t nao, int32_t nao_pad, const double* phi,
+                int32_t nspin, const double* chi, const double* g, int64_t g_spin_stride, double scale, double* out,
+                void* workspace);
 
 /* Kernel timing (no counterpart in the reference): while on, every kernel this library
  * enqueues is bracketed by HIP events on its own stream.  gdx_profile_end waits for them and

@@ -25,6 +25,7 @@ DIIS quirks checked against hand-built buffers in ``tests/test_oracle_scf.py``.
 """
 from __future__ import annotations
 
+import dataclasses
 from dataclasses import dataclass, field
 from typing import Callable, List, Optional, Tuple
 
@@ -392,10 +393,11 @@ def smearing_occ(mo_energy, nelectron, smearing, sigma):
 def scf_loop(xc_value_and_vxc: Callable, inp: SCFInputs, rdm1: np.ndarray, mo_occ: np.ndarray, *,
              level_shift_factor=(0.0, 0.0), damp_factor=(0.0, 0.0), cycles=50, diis_start_cycle=0,
              e_conv=1e-5, g_conv=1e-5, diis_method="CDIIS", smearing=None, smearing_sigma=0.0,
-             carried_fock=None, nelectron=None):
-    """evaluate.py:387-589 (no chi update).  Returns (SCFState, cycles run).  The orbital
-    gradient uses the Fock matrix the molecule carried in (carried_fock), or the current one
-    when there is none."""
+             carried_fock=None, nelectron=None, exchange_update=None):
+    """evaluate.py:387-589.  Returns (SCFState, cycles run).  The orbital gradient uses the Fock
+    matrix the molecule carried in (carried_fock), or the current one when there is none.
+    exchange_update(D) -> K0, when given, is the chi update of evaluate.py:510-522 and 565-576:
+    the exact-exchange matrix is rebuilt from every new density before it is predicted at."""
     S = inp.s1e
     n = S.shape[0]
     if nelectron is None:
@@ -425,11 +427,15 @@ def scf_loop(xc_value_and_vxc: Callable, inp: SCFInputs, rdm1: np.ndarray, mo_oc
         if abs(level_shift_factor[0]) + abs(level_shift_factor[1]) > 1e-4:
             fock = np.stack([level_shift(S, D[s], fock[s], level_shift_factor[s]) for s in range(2)])
         w, C, occ, D = solve(fock, occ)
+        if exchange_update is not None:
+            inp = dataclasses.replace(inp, K0=exchange_update(D))
         e, fock = predict(xc_value_and_vxc, inp, D)
         norm_gorb = gnorm(C, occ, fock)
         cycle += 1
     if abs(e - old_e) * HARTREE2KCALMOL < e_conv and norm_gorb < g_conv:
         w, C, occ, D = solve(fock, occ)
+        if exchange_update is not None:
+            inp = dataclasses.replace(inp, K0=exchange_update(D))
         e, fock = predict(xc_value_and_vxc, inp, D)
     return SCFState(D, fock, e, w, C, occ), cycle
 

@@ -294,6 +294,9 @@ class XCInputs:
     chi: Optional[torch.Tensor] = None  # (N, 1, 2, n) for exact exchange
 
 
+LINEAR_KINDS = {"lsda": ("lda",), "b88": ("lda", "b88"), "vwn": ("vwn",), "pw92": ("pw92",), "lyp": ("lyp",)}
+
+
 def xc_energy_from_rdm1(kind: str, inp: XCInputs, rdm1, params=None,
                         spec: Optional[MLPSpec] = None):
     """E_xc(D) for ``kind`` in {'b3lyp', 'neural_b3lyp', 'b3lyp_neural_corr'} following compute_densities ->
@@ -311,6 +314,13 @@ def xc_energy_from_rdm1(kind: str, inp: XCInputs, rdm1, params=None,
             coefs = torch.tensor([B3LYP_COEFFS[:4]], dtype=DT)
         dens = abs_clip(dens)
         return xc_energy(inp.weights, coefs, dens)
+    if kind in LINEAR_KINDS:
+        # LSDA / B88 / VWN / PW92 / LYP (popular_functionals.py:375-393): a unit coefficient
+        # broadcast over the functional's energy densities (lsda_density ... lyp_density, :271-304)
+        parts = {"lda": lambda: lsda_x_e(rho), "b88": lambda: b88_x_e(rho, grho), "vwn": lambda: vwn_c_e(rho),
+                 "pw92": lambda: pw92_c_e(rho), "lyp": lambda: lyp_c_e(rho, grho, lap)}
+        dens = abs_clip(torch.stack([parts[k]() for k in LINEAR_KINDS[kind]], dim=1))
+        return xc_energy(inp.weights, torch.ones((1, dens.shape[1]), dtype=DT), dens)
     if kind in ("neural_b3lyp", "b3lyp_neural_corr"):
         tau = kinetic_density(rdm1, inp.grad_ao)
         x = neural_features(rho, grho, tau)

@@ -36,8 +36,9 @@ def _jacobians(A):
     jd = torch.autograd.functional.jacobian(lambda X: EighFn.apply(X), torch.tensor(A, device="cuda"))
     jo = torch.autograd.functional.jacobian(lambda X: SafeEigh.apply(X), torch.tensor(A))
     wd, Vd = EighFn.apply(torch.tensor(A, device="cuda"))
-    Vo = np.linalg.eigh(A)[1]
-    # eigenvector signs are arbitrary: align the device columns (and their Jacobians) to LAPACK's
+    Vo = torch.linalg.eigh(torch.tensor(A))[1].numpy()
+    # eigenvector signs are arbitrary: align the device columns (and their Jacobians) to those of
+    # the eigh SafeEigh differentiates (torch's LAPACK call; numpy's may pick other signs)
     sgn = np.sign((Vd.cpu().numpy() * Vo).sum(axis=0))
     jV = jd[1].cpu().numpy() * sgn[None, :, None, None]
     return (jd[0].cpu().numpy(), jV), (jo[0].numpy(), jo[1].numpy())

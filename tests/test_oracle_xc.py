@@ -211,3 +211,24 @@ def test_b3lyp_correction_head_reduces_to_b3lyp():
     lap = X.lapl_density(torch.tensor(rdm1), inp.ao, inp.grad_ao, inp.d2_ao)
     e = X.abs_clip(X.b3lyp_exhf_densities(rho, grho, lap))
     assert E1 - E_nc == pytest.approx(0.01 * float((inp.weights * e[:, 0]).sum()), rel=1e-9)
+
+
+def test_single_part_functionals_compose_b3lyp_and_have_consistent_gradients():
+    """LSDA, B88 = LSDA + gradient correction, VWN, LYP (popular_functionals.py:375-393) add up to
+    the B3LYP grid energy with its coefficients (no per-point clip is active on this grid); PW92
+    and its density gradient match central differences."""
+    inp, D, C, _ = _water_inputs(level=1)
+    D1 = C[:, :4] @ C[:, :4].T
+    rdm1 = np.stack([D, 0.5 * (D + D1)])
+    E = {k: X.xc_value_and_grads(k, inp, rdm1)[0] for k in ("lsda", "b88", "vwn", "lyp", "b3lyp")}
+    comb = 0.8 * E["lsda"] + 0.72 * (E["b88"] - E["lsda"]) + 0.19 * E["vwn"] + 0.81 * E["lyp"]
+    assert comb == pytest.approx(E["b3lyp"], rel=1e-12)
+    E0, gD, _ = X.xc_value_and_grads("pw92", inp, rdm1)
+    P = np.random.default_rng(2).normal(size=rdm1.shape)
+    P = 0.5 * (P + P.transpose(0, 2, 1))
+    h = 1e-6
+    Ep = X.xc_value_and_grads("pw92", inp, rdm1 + h * P)[0]
+    Em = X.xc_value_and_grads("pw92", inp, rdm1 - h * P)[0]
+    assert (Ep - Em) / (2 * h) == pytest.approx(np.sum(gD * P), rel=1e-6)
+    # PW92 and VWN5 are fits of the same correlation energy: the molecule's energies agree closely
+    assert abs(E0 - E["vwn"]) <= 1e-2 * abs(E["vwn"])
