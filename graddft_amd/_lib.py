@@ -45,6 +45,8 @@ SIGNATURES = {
     "gdx_xc_forward": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P, _P, _P]),
     "gdx_coulomb_workspace_size": (ctypes.c_int, [_I, ctypes.POINTER(_Z)]),
     "gdx_coulomb_packed": (ctypes.c_int, [_P, _I, _P, _P, _P, _P]),
+    "gdx_coulomb_noalloc_max_n": (ctypes.c_int, [_P]),
+    "gdx_coulomb_packed_noalloc": (ctypes.c_int, [_P, _I, _P, _P, _P]),
     "gdx_nonxc_energy": (ctypes.c_int, [_P, _I, _P, _P, _P, _P, ctypes.c_double, ctypes.c_double, _P]),
     "gdx_fock_assemble": (ctypes.c_int, [_P, _I, _P, _P, _P, _P, ctypes.c_double, _P]),
     "gdx_exchange_packed": (ctypes.c_int, [_P, _I, _I, _P, _P, _P]),

@@ -10,6 +10,9 @@
 //
 // Packed ERI order (shared with the host integral library): pair index I = i(i+1)/2 + j
 // (i >= j), element index I(I+1)/2 + K (I >= K).
+#include <mutex>
+#include <string>
+
 #include "gdx_common.h"
 #include "gdx_profile.h"
 
@@ -851,13 +854,22 @@ int gdx_coulomb_workspace_size(int32_t n, size_t* bytes) {
   return GDX_OK;
 }
 
-int gdx_coulomb_packed(void* stream, int32_t n, const double* eri, const double* rdm1, double* J, void* workspace) {
-  GDX_REQUIRE(n > 0 && eri && rdm1 && J && workspace, "gdx_coulomb_packed: bad arguments");
-  hipStream_t st = (hipStream_t)stream;
+}  // extern "C"
+
+namespace gdx {
+namespace {
+
+long coulomb_ws_doubles(long n) {
+  const long P = n * (n + 1) / 2;
+  const long nb = (P + coulomb_tile(P) - 1) / coulomb_tile(P);
+  return P + 2 * nb * P;
+}
+
+int coulomb_launch(hipStream_t st, int32_t n, const double* eri, const double* rdm1, double* J, double* ws) {
   const long P = (long)n * (n + 1) / 2;
   const int tt = coulomb_tile(P);
   const long nb = (P + tt - 1) / tt;
-  double* Dt = (double*)workspace;
+  double* Dt = ws;
   double* prow = Dt + P;
   double* pcol = prow + nb * P;
   GDX_LAUNCH("pack_density", pack_density_kernel, dim3(nblk(P)), dim3(NT), 0, st, rdm1, n, Dt);
@@ -870,6 +882,67 @@ int gdx_coulomb_packed(void* stream, int32_t n, const double* eri, const double*
              J);
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
+}
+
+// Scratch of gdx_coulomb_packed_noalloc: a fixed device array, one copy per GPU, held until
+// the process ends.  Host code never passes `coulomb_scratch` itself to a kernel (on the host
+// the name is only a stand-in); it uses the per-device address hipGetSymbolAddress returns.
+constexpr long COULOMB_SCRATCH_DOUBLES = 4L << 20;  // 32 MiB
+static __device__ __align__(256) double coulomb_scratch[COULOMB_SCRATCH_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Address of this device's scratch array (looked up once per device, with it current).
+int scratch_for_current_device(double** out) {
+  int dev = -1;
+  GDX_HIP_CHECK(hipGetDevice(&dev));
+  GDX_REQUIRE(dev >= 0 && dev < MAX_DEVICES, "gdx_coulomb_packed_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    GDX_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(coulomb_scratch)));
+    GDX_REQUIRE(p != nullptr, "gdx_coulomb_packed_noalloc: scratch symbol has no device address");
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return GDX_OK;
+}
+
+}  // namespace
+}  // namespace gdx
+
+extern "C" {
+
+int gdx_coulomb_packed(void* stream, int32_t n, const double* eri, const double* rdm1, double* J, void* workspace) {
+  GDX_REQUIRE(n > 0 && eri && rdm1 && J && workspace, "gdx_coulomb_packed: bad arguments");
+  return coulomb_launch((hipStream_t)stream, n, eri, rdm1, J, (double*)workspace);
+}
+
+int gdx_coulomb_noalloc_max_n(int32_t* n_max) {
+  GDX_REQUIRE(n_max, "gdx_coulomb_noalloc_max_n: bad arguments");
+  int32_t n = 1;
+  while (coulomb_ws_doubles(n + 1) <= COULOMB_SCRATCH_DOUBLES) ++n;
+  *n_max = n;
+  return GDX_OK;
+}
+
+int gdx_coulomb_packed_noalloc(void* stream, int32_t n, const double* eri, const double* rdm1, double* J) {
+  GDX_REQUIRE(n > 0 && eri && rdm1 && J, "gdx_coulomb_packed_noalloc: bad arguments");
+  if (coulomb_ws_doubles(n) > COULOMB_SCRATCH_DOUBLES) {
+    int32_t n_max = 0;
+    gdx_coulomb_noalloc_max_n(&n_max);
+    set_error("gdx_coulomb_packed_noalloc: n = " + std::to_string(n) + " needs " +
+              std::to_string(coulomb_ws_doubles(n) * 8) + " bytes of scratch; the fixed scratch holds " +
+              std::to_string(COULOMB_SCRATCH_DOUBLES * 8) + " (n <= " + std::to_string(n_max) +
+              "); use gdx_coulomb_packed with a workspace");
+    return GDX_ERR_INVALID;
+  }
+  double* ws = nullptr;
+  const int rc = scratch_for_current_device(&ws);
+  if (rc != GDX_OK) return rc;
+  return coulomb_launch((hipStream_t)stream, n, eri, rdm1, J, ws);
 }
 
 int gdx_nonxc_energy(void* stream, int32_t n, const double* rdm1, const double* h1e, const double* J,

@@ -139,6 +139,16 @@ int gdx_coulomb_workspace_size(int32_t n, size_t* bytes);
 int gdx_coulomb_packed(void* stream, int32_t n, const double* eri, const double* rdm1, double* J,
                        void* workspace);
 
+/* gdx_coulomb_packed without a workspace argument, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The kernels use a fixed 32 MiB device array
+ * of this library (one per GPU, held until the process ends) in place of the workspace, so
+ * the call allocates nothing; n above gdx_coulomb_noalloc_max_n (214) is rejected with
+ * GDX_ERR_INVALID.  Every call on one GPU shares that array: calls that overlap on one GPU
+ * (different streams, or host threads without ordering) race and must not be issued. */
+int gdx_coulomb_noalloc_max_n(int32_t* n_max);
+int gdx_coulomb_packed_noalloc(void* stream, int32_t n, const double* eri, const double* rdm1, double* J);
+
 /* Non-XC energy e_nuc + sum D h1e + 1/2 sum D J (D = D_up + D_dn) into one device double:
  * nonXC of grad_dft/molecule.py:694-783.  With K0 (2, n, n) non-NULL the exact-exchange energy
  * of B3LYP from the initial density, -(a0/2) sum_s sum D_s K0_s, is added (its grid form is
