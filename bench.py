@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--scf-cycles", type=int, default=15)
     ap.add_argument("--no-train", action="store_true", help="skip the training-step measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the concurrent small-molecule batch")
+    ap.add_argument("--no-train-batch", action="store_true", help="skip the batched training step")
+    ap.add_argument("--batch-size", type=int, default=8, help="alkanes in the batched training step")
+    ap.add_argument("--batch-max-nc", type=int, default=4, help="largest n_C of the batched training step")
     ap.add_argument("--no-graph", action="store_true", help="time the eager step instead of graph replays")
     ap.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (calibrates HBM byte counters)")
@@ -221,6 +224,63 @@ def train_bench(args):
     return out
 
 
+def train_batch_bench(args):
+    """A training step over a batch of alkanes (the ragged training-batch workload at reduced
+    size): n_C cycling over 1..args.batch_max_nc (seeded all-trans geometries, N(0, 0.02 A)
+    jitter), def2-SVP, level-2 grids, neural B3LYP with an MLP of 3 x 128, mse_energy_loss over
+    args.train_cycles-cycle differentiable SCF loops, forward + reverse to d loss / d theta.
+    The XC evaluation of every cycle batched over the molecules (batched_diff_scf_loop: one
+    MLP pass over all grid points, ragged AO-side launches) against the reference's
+    molecule-by-molecule loop (diff_scf_loop per molecule), both on the device."""
+    import torch
+    from graddft_amd import MLPSpec, NeuralFunctional, batched_diff_scf_loop, diff_scf_loop, mse_energy_loss
+    from graddft_amd.interface import geometry
+    from graddft_amd.interface.builder import molecule_from_geometry
+    rng = np.random.default_rng(args.seed + 7)
+    ncs = [1 + i % args.batch_max_nc for i in range(args.batch_size)]
+    mols = []
+    for nc in ncs:
+        sym, xyz = geometry.alkane(rng, nc)
+        mols.append(molecule_from_geometry(sym, xyz, "def2-svp", 2, with_eri=True, name=f"C{nc}", npts_align=256))
+    f = NeuralFunctional(MLPSpec(nc=6, h=128, n_hidden=3, nd=4))
+    theta = f.init(seed=args.seed)
+    truth = [-40.0 * nc for nc in ncs]
+    out = {"workload": f"{len(mols)} alkanes C_nH_2n+2, n_C in {sorted(set(ncs))}, def2-SVP (nao "
+                       f"{min(m.nao for m in mols)}-{max(m.nao for m in mols)}), level-2 grids "
+                       f"({sum(m.grid_size for m in mols)} pts in total), neural B3LYP MLP 3x128, mse_energy_loss "
+                       f"over {args.train_cycles}-cycle diff_scf_loops, forward + reverse"}
+    grads = {}
+
+    def step(loop, prec):
+        th = theta.detach().clone().requires_grad_(True)
+        loss = mse_energy_loss(th, loop, mols, truth)
+        g, = torch.autograd.grad(loss, [th])
+        return float(loss), g
+
+    for prec in (["fp64", args.precision] if args.precision != "fp64" else ["fp64"]):
+        res = {}
+        for name, loop in (("per_molecule", diff_scf_loop(f, cycles=args.train_cycles, precision=prec)),
+                           ("batched", batched_diff_scf_loop(f, cycles=args.train_cycles, precision=prec))):
+            step(loop, prec)                                   # warm-up: plans, workspaces
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            loss, g = step(loop, prec)
+            torch.cuda.synchronize()
+            res[name] = {"ms_per_step": round((time.perf_counter() - t0) * 1e3, 2), "loss": loss}
+            grads[(prec, name)] = g.cpu().numpy()
+        res["speedup"] = round(res["per_molecule"]["ms_per_step"] / res["batched"]["ms_per_step"], 3)
+        res["batched_vs_per_molecule_grad_rel"] = float(
+            np.linalg.norm(grads[(prec, "batched")] - grads[(prec, "per_molecule")])
+            / np.linalg.norm(grads[(prec, "per_molecule")]))
+        out[prec] = res
+    if args.precision != "fp64":
+        g0, g1 = grads[("fp64", "batched")], grads[(args.precision, "batched")]
+        out["grad_rel_err_vs_fp64"] = float(np.linalg.norm(g1 - g0) / np.linalg.norm(g0))
+        out["fp32_over_fp64_step_time"] = round(out[args.precision]["batched"]["ms_per_step"]
+                                                / out["fp64"]["batched"]["ms_per_step"], 3)
+    return out
+
+
 def measured_traffic():
     """{timer name: HBM bytes per launch} from the committed PMC summaries (profiles/traffic_*.txt,
     lines 'traffic <name> <bytes>'), measured on this workload with rocprofv3."""
@@ -393,6 +453,7 @@ def main():
     scf = None if args.no_scf else scf_bench(args)
     train = None if args.no_train else train_bench(args)
     batch = None if (args.no_batch or args.no_scf) else batch_bench(args)
+    train_batch = None if (args.no_train_batch or args.no_train) else train_batch_bench(args)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -433,6 +494,7 @@ def main():
         "scf": scf,
         "train_step": train,
         "batch": batch,
+        "train_batch": train_batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

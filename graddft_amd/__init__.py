@@ -21,6 +21,7 @@ from .evaluate import (Diis, JittableDiis, batched_scf_loop, damping, diff_scf_l
                        jitted_mol_orb_optimizer, level_shift, mol_orb_optimizer, non_scf_predictor, scf_loop,
                        simple_scf_loop)
 from .eigenproblem import safe_eigh_vjp, safe_fock_solver, safe_general_eigh
+from .batch import XCBatchPlan, batch_energy_predictor, batched_diff_scf_loop
 
 __all__ = ["Grid", "Molecule", "Functional", "LinearFunctional", "NeuralFunctional", "B3LYPNeuralCorrection",
            "MLPSpec", "XCPlan", "XCEnergyFn", "XCEnergyVxcFn", "B3LYP", "B3LYP_GRID", "LSDA", "B88", "VWN", "PW92",
@@ -29,4 +30,5 @@ __all__ = ["Grid", "Molecule", "Functional", "LinearFunctional", "NeuralFunction
            "train_kernel", "adam", "apply_updates", "JittableDiis", "Diis", "diff_scf_loop", "batched_scf_loop", "diff_simple_scf_loop",
            "scf_loop", "simple_scf_loop", "damping", "level_shift", "mol_orb_optimizer",
            "jitted_mol_orb_optimizer", "safe_eigh_vjp",
-           "safe_fock_solver", "safe_general_eigh", "save_molecules", "load_molecules"]
+           "safe_fock_solver", "safe_general_eigh", "save_molecules", "load_molecules", "XCBatchPlan",
+           "batch_energy_predictor", "batched_diff_scf_loop"]

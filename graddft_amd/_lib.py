@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libgdx.so")
 GDX_FP64, GDX_FP32, GDX_BF16X6 = 0, 1, 2
 GDX_FUNC_B3LYP, GDX_FUNC_NEURAL_B3LYP, GDX_FUNC_B3LYP_NEURAL_CORR, GDX_FUNC_LINEAR = 0, 1, 2, 3
 GDX_CORR_VWN5, GDX_CORR_PW92 = 0, 1
+GDX_STATUS_EIGH_NOT_CONVERGED, GDX_STATUS_NONFINITE_ENERGY, GDX_STATUS_CHARGE = 1, 2, 4
 
 
 class XCDesc(ctypes.Structure):
@@ -54,6 +55,7 @@ SIGNATURES = {
     "gdx_eigh_workspace_size": (ctypes.c_int, [_I, _I, ctypes.POINTER(_Z)]),
     "gdx_eigh_batched": (ctypes.c_int, [_P, _I, _I, _P, _P, _P, _P, _P, _I, ctypes.c_double, _P]),
     "gdx_diis_solve": (ctypes.c_int, [_P, _I, _I, _P, _P]),
+    "gdx_scf_status": (ctypes.c_int, [_P, _I, _I, _P, _P, _P, _P, _I, _P, ctypes.c_double, _P]),
     "gdx_diis_coefficients": (ctypes.c_int, [_P, _I, _I, ctypes.c_int64, _P, _I, _P, _P]),
     "gdx_scf_step_workspace_size": (ctypes.c_int, [_I, _I, ctypes.POINTER(_Z)]),
     "gdx_scf_step": (ctypes.c_int, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, ctypes.c_double,
@@ -73,6 +75,27 @@ SIGNATURES = {
     "gdx_xc_hvp_workspace_size": (ctypes.c_int, [ctypes.POINTER(XCDesc), ctypes.POINTER(_Z)]),
     "gdx_xc_hvp": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
 }
+
+class XCBatchMol(ctypes.Structure):
+    _fields_ = [
+        ("ao", ctypes.c_void_p),
+        ("weights", ctypes.c_void_p),
+        ("npts", ctypes.c_int32),
+        ("npts_pad", ctypes.c_int32),
+        ("nao", ctypes.c_int32),
+        ("nao_pad", ctypes.c_int32),
+    ]
+
+
+SIGNATURES.update({
+    "gdx_xc_batch_create": (ctypes.c_int, [ctypes.POINTER(XCDesc), _I, ctypes.POINTER(XCBatchMol), _I,
+                                           ctypes.POINTER(_P)]),
+    "gdx_xc_batch_destroy": (ctypes.c_int, [_P]),
+    "gdx_xc_batch_offsets": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
+    "gdx_xc_batch_forward": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "gdx_xc_batch_backward": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "gdx_xc_batch_hvp": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+})
 
 _lib = None
 

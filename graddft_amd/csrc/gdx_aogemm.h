@@ -34,10 +34,12 @@ constexpr int DN_PT = 64, DN_K = 16;
 // benzene 6-31G*); a 128-column chunk at two blocks per CU spills.
 template <typename T> struct DensCB { static constexpr int value = 64; };
 
+// One 64-point tile (bx) of one spin (s): ao (5, npts_pad, nP) planes of one molecule, Dp its
+// padded density (nsd, nP, nP); raw sums to dens_part[(s * out_rows + p) * 6 + v].
 template <typename T>
-__global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void density_ao_kernel(const T* __restrict__ ao, int npts_pad, int nP,
-                                                         const T* __restrict__ Dp,
-                                                         double* __restrict__ dens_part) {
+__device__ __forceinline__ void density_ao_body(const T* __restrict__ ao, int npts_pad, int nP,
+                                                const T* __restrict__ Dp, double* __restrict__ dens_part,
+                                                size_t out_rows, int bx, int s) {
   typedef typename MfmaT<T>::acc_t acc_t;
   constexpr int CB = DensCB<T>::value;
   constexpr int NJ = CB / 16;
@@ -60,8 +62,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void density_ao_kernel
   typedef T v4_t __attribute__((ext_vector_type(4)));
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int p0 = blockIdx.x * DN_PT;
-  const int s = blockIdx.y;
+  const int p0 = bx * DN_PT;
   const T* D = Dp + (size_t)s * nP * nP;
   const size_t plane = (size_t)npts_pad * nP;
   const int nk = nP / DN_K;
@@ -193,9 +194,45 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void density_ao_kernel
       const double t = sum16(part[q][v]);
       if ((lane & 15) == 0) {
         const int p = p0 + wave * 16 + MfmaT<T>::row(lane, q);
-        dens_part[((size_t)s * npts_pad + p) * DENS_STRIDE + v] = t;
+        dens_part[((size_t)s * out_rows + p) * DENS_STRIDE + v] = t;
       }
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void density_ao_kernel(const T* __restrict__ ao, int npts_pad,
+                                                                                 int nP, const T* __restrict__ Dp,
+                                                                                 double* __restrict__ dens_part) {
+  density_ao_body<T>(ao, npts_pad, nP, Dp, dens_part, (size_t)npts_pad, blockIdx.x, blockIdx.y);
+}
+
+// A batch of molecules with different grids and bases, one launch: molecule m owns tiles
+// [tile0_m, tile0_m + npts_pad_m / 64) of blockIdx.x and writes rows [row0_m, row0_m + npts_pad_m)
+// of every spin plane of the batch's dens_part (out_rows rows a plane).
+struct AoBatchMol {
+  const void* ao;      // (5, npts_pad, nP) planes in the element type
+  const void* Dp;      // padded density (or direction) of this molecule, (2, nP, nP)
+  long row0;           // first row in the batch's per-point arrays
+  int npts_pad, nP;
+  int tile0, ntile;    // first 64-point tile in the launch, tiles
+};
+
+__device__ __forceinline__ int batch_find(const AoBatchMol* __restrict__ mols, int nmol, int tile) {
+  int lo = 0, hi = nmol - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (mols[mid].tile0 <= tile) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void density_ao_batch_kernel(
+    const AoBatchMol* __restrict__ mols, int nmol, size_t out_rows, double* __restrict__ dens_part) {
+  const int m = batch_find(mols, nmol, blockIdx.x);
+  const AoBatchMol& M = mols[m];
+  density_ao_body<T>((const T*)M.ao, M.npts_pad, M.nP, (const T*)M.Dp, dens_part + (size_t)M.row0 * DENS_STRIDE,
+                     out_rows, blockIdx.x - M.tile0, blockIdx.y);
 }
 
 constexpr int VX_P = 4, VX_K = 4 * VX_P;
@@ -206,10 +243,12 @@ constexpr int VX_FLUSH = 8;   // FP32 path: stages (of VX_P points) per FP32 par
 // stages, so no FP32 sum runs over more than VX_FLUSH * VX_K terms -- needed when a block's
 // point chunk is long, because the V_xc integrand cancels strongly near the nuclei (large
 // Laplacian terms); short chunks go without it (fewer registers, more blocks per CU).
-template <typename T, bool SAME, int VT, bool FLUSH = false>
-__global__ __launch_bounds__(256, (FLUSH && SAME) ? 2 : 1) void vxc_ao_kernel(const T* __restrict__ ao, int npts_pad, int nP,
-                                                     const double* __restrict__ vpt, int chunk,
-                                                     double* __restrict__ vpart) {
+// Body of one block: output tile bx, point chunk by, of one spin whose per-point potentials
+// start at vp (rows of this molecule); the tile's partial sums go to out (nP x nP).
+template <typename T, bool SAME, int VT, bool FLUSH>
+__device__ __forceinline__ void vxc_ao_body(const T* __restrict__ ao, int npts_pad, int nP,
+                                            const double* __restrict__ vp, int chunk, double* __restrict__ out,
+                                            int bx, int by) {
   typedef typename MfmaT<T>::acc_t acc_t;
   constexpr int WB = VT / 32;          // 16-wide MFMA blocks per wave edge
   constexpr int CPT = VT / 64;         // columns staged per thread
@@ -217,12 +256,10 @@ __global__ __launch_bounds__(256, (FLUSH && SAME) ? 2 : 1) void vxc_ao_kernel(co
   __shared__ T Bs[2][VX_K][VT + 4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nt = (nP + VT - 1) / VT;
-  const int a0 = (blockIdx.x / nt) * VT, b0 = (blockIdx.x % nt) * VT;
-  const int s = blockIdx.z;
-  const int rbeg = blockIdx.y * chunk;
+  const int a0 = (bx / nt) * VT, b0 = (bx % nt) * VT;
+  const int rbeg = by * chunk;
   const int rend = min(npts_pad, rbeg + chunk);
   const size_t plane = (size_t)npts_pad * nP;
-  const double* vp = vpt + (size_t)s * npts_pad * VPT_STRIDE;
   const int wm = (wave >> 1) * (VT / 2), wn = (wave & 1) * (VT / 2);
   // staging role of this thread: point p of the stage, CPT adjacent columns
   const int sp = tid >> 6, sc = (tid & 63) * CPT;
@@ -308,7 +345,6 @@ __global__ __launch_bounds__(256, (FLUSH && SAME) ? 2 : 1) void vxc_ao_kernel(co
     if (st + 1 < nst) lstore(cur ^ 1);
     __syncthreads();
   }
-  double* out = vpart + ((size_t)blockIdx.y * gridDim.z + s) * nP * nP;
 #pragma unroll
   for (int i = 0; i < WB; ++i)
 #pragma unroll
@@ -322,6 +358,49 @@ __global__ __launch_bounds__(256, (FLUSH && SAME) ? 2 : 1) void vxc_ao_kernel(co
         if (row < nP && col < nP) out[(size_t)row * nP + col] = v;
       }
     }
+}
+
+template <typename T, bool SAME, int VT, bool FLUSH = false>
+__global__ __launch_bounds__(256, (FLUSH && SAME) ? 2 : 1) void vxc_ao_kernel(const T* __restrict__ ao, int npts_pad,
+                                                                             int nP, const double* __restrict__ vpt,
+                                                                             int chunk, double* __restrict__ vpart) {
+  const int s = blockIdx.z;
+  vxc_ao_body<T, SAME, VT, FLUSH>(ao, npts_pad, nP, vpt + (size_t)s * npts_pad * VPT_STRIDE, chunk,
+                                  vpart + ((size_t)blockIdx.y * gridDim.z + s) * nP * nP, blockIdx.x, blockIdx.y);
+}
+
+// V_xc of a batch of molecules in one launch per kernel variant: block (x, s) of entry e runs
+// tile (x - blk0_e) % ntile_e, chunk (x - blk0_e) / ntile_e of molecule mol_e, with the batch's
+// per-point potentials (rows row0 of every spin plane of vpt_rows rows) and partials to
+// vpart_e[(chunk * nsv + s) * nP^2].
+struct VxcBatchMol {
+  const void* ao;
+  double* vpart;
+  long row0;
+  int npts_pad, nP;
+  int blk0, ntile, nchunk, chunk;
+};
+
+__device__ __forceinline__ int vxc_batch_find(const VxcBatchMol* __restrict__ mols, int nmol, int blk) {
+  int lo = 0, hi = nmol - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (mols[mid].blk0 <= blk) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename T, bool SAME, int VT, bool FLUSH = false>
+__global__ __launch_bounds__(256, (FLUSH && SAME) ? 2 : 1) void vxc_ao_batch_kernel(
+    const VxcBatchMol* __restrict__ mols, int nmol, const double* __restrict__ vpt, size_t vpt_rows, int nsv) {
+  const int m = vxc_batch_find(mols, nmol, blockIdx.x);
+  const VxcBatchMol& M = mols[m];
+  const int local = blockIdx.x - M.blk0;
+  const int tile = local % M.ntile, chunk = local / M.ntile;
+  const int s = blockIdx.y;
+  vxc_ao_body<T, SAME, VT, FLUSH>((const T*)M.ao, M.npts_pad, M.nP,
+                                  vpt + ((size_t)s * vpt_rows + M.row0) * VPT_STRIDE, M.chunk,
+                                  M.vpart + ((size_t)chunk * nsv + s) * M.nP * M.nP, tile, chunk);
 }
 
 // vxc_bf16x6_kernel: vxc_ao_kernel's product on bf16 MFMA (v_mfma_f32_16x16x32_bf16) with

@@ -211,6 +211,52 @@ __global__ void exchange_scatter_kernel(const double* __restrict__ eri, long P, 
   }
 }
 
+// Status word of an SCF cycle, OR-ed into *status by one block (no atomics): bit 0 the
+// eigensolver ran max_sweeps without converging (sweeps[s] >= max_sweeps for a spin), bit 1
+// the energy is not finite, bit 2 |tr(D_s S) - sum_i occ_s,i| > tol for a spin (the orbitals
+// the density was built from are not S-orthonormal).  NaN inputs set bits 1 / 2.
+__global__ __launch_bounds__(NT) void scf_status_kernel(int nspin, int n, const double* __restrict__ rdm1,
+                                                       const double* __restrict__ S, const double* __restrict__ occ,
+                                                       const int* __restrict__ sweeps, int max_sweeps,
+                                                       const double* __restrict__ energy, double tol,
+                                                       int* __restrict__ status) {
+  __shared__ double red[2][NT / 64];
+  const int tid = threadIdx.x;
+  int bits = 0;
+  for (int s = 0; s < nspin; ++s) {
+    double tr = 0.0, nocc = 0.0;
+    const double* D = rdm1 + (size_t)s * n * n;
+    for (long e = tid; e < (long)n * n; e += NT) {
+      const int i = (int)(e / n), j = (int)(e % n);
+      tr += D[e] * S[(size_t)j * n + i];
+    }
+    for (int i = tid; i < n; i += NT) nocc += occ[(size_t)s * n + i];
+    tr = warp_sum(tr);
+    nocc = warp_sum(nocc);
+    if ((tid & 63) == 0) {
+      red[0][tid >> 6] = tr;
+      red[1][tid >> 6] = nocc;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double a = 0.0, b = 0.0;
+      for (int w = 0; w < NT / 64; ++w) {
+        a += red[0][w];
+        b += red[1][w];
+      }
+      if (!(fabs(a - b) <= tol)) bits |= 4;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (sweeps)
+      for (int s = 0; s < nspin; ++s)
+        if (sweeps[s] >= max_sweeps) bits |= 1;
+    if (energy && !isfinite(*energy)) bits |= 2;
+    *status |= bits;
+  }
+}
+
 __global__ void zero_kernel(double* __restrict__ p, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
@@ -1054,6 +1100,15 @@ int gdx_diis_coefficients(void* stream, int32_t nspin, int32_t m, int64_t n2, co
   GDX_LAUNCH("diis_gram", diis_gram_kernel, dim3(m * (m + 1) / 2, nspin), dim3(DIIS_THREADS), 0, st, errs, m, nspin,
              (long)n2, B);
   GDX_LAUNCH("diis_solve", diis_border_solve_kernel, dim3(nspin), dim3(64), 0, st, m, cycle, B, x);
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+int gdx_scf_status(void* stream, int32_t nspin, int32_t n, const double* rdm1, const double* S, const double* occ,
+                   const int32_t* sweeps, int32_t max_sweeps, const double* energy, double tol, int32_t* status) {
+  GDX_REQUIRE(nspin > 0 && n > 0 && rdm1 && S && occ && status, "gdx_scf_status: bad arguments");
+  GDX_LAUNCH("scf_status", scf_status_kernel, dim3(1), dim3(NT), 0, (hipStream_t)stream, nspin, n, rdm1, S, occ,
+             sweeps, max_sweeps, energy, tol, status);
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
 }

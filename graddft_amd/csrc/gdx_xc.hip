@@ -574,7 +574,7 @@ __global__ __launch_bounds__(NT) void hvp_seed_kernel(
     const double* __restrict__ Edot, const double* __restrict__ weights, int nd, int npts, int npts_pad,
     const double* __restrict__ g_exc, int neural, double* __restrict__ GE, double* __restrict__ GEd,
     T* __restrict__ DZ2, double* __restrict__ dbpart, double* __restrict__ edot_part, int head_mode,
-    LinCoef lc) {
+    LinCoef lc, const double* __restrict__ g_pt) {
   __shared__ double red[NT / 64];
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const double* bc = lc.c;
@@ -593,7 +593,7 @@ __global__ __launch_bounds__(NT) void hvp_seed_kernel(
       eps += c[o] * e[o];
     }
     const double m = (r < npts && fabs(eps) > 1e-30) ? clip30(weights[r]) : 0.0;
-    const double g = *g_exc;
+    const double g = g_pt ? g_pt[r] : *g_exc;
     double zdb[MAXND];
     for (int o = 0; o < nd; ++o) {
       const double cd = cp[o] * zt[o];
@@ -778,7 +778,7 @@ struct Layout {
 size_t elem_size(int precision) { return precision == GDX_FP64 ? 8 : 4; }
 bool mlp_bf16(const gdx_xc_desc& d) { return d.precision == GDX_BF16X6; }
 
-Layout make_layout(const gdx_xc_desc& d, bool hvp = false) {
+Layout make_layout(const gdx_xc_desc& d, bool hvp = false, bool batch = false) {
   Layout L{};
   const size_t Np = d.npts_pad, nP = d.nao_pad;
   const size_t R = hvp ? 2 * Np : Np;  // rows of the activation-side buffers
@@ -820,7 +820,7 @@ Layout make_layout(const gdx_xc_desc& d, bool hvp = false) {
   L.nparam = np;
   size_t off = 0;
   auto take = [&](size_t bytes) { size_t o = off; off += round_up(bytes, 256); return o; };
-  L.Dp = take(2 * nP * nP * es);
+  L.Dp = take(batch ? 0 : 2 * nP * nP * es);
   L.dens_part = take((size_t)L.nsd * Np * DENS_STRIDE * 8);
   L.dens_fin = take(Np * 2 * DENS_STRIDE * 8);
   L.E = take(Np * 4 * 8);
@@ -837,7 +837,7 @@ Layout make_layout(const gdx_xc_desc& d, bool hvp = false) {
   L.G = take(neural ? 2 * R * h * es : 0);
   L.DX = take(neural ? R * d.mlp_nc * es : 0);
   L.vpt = take(2 * Np * VPT_STRIDE * 8);
-  L.vpart = take((size_t)L.nchunk * nP * nP * 8);
+  L.vpart = take(batch ? 0 : (size_t)L.nchunk * nP * nP * 8);
   size_t wbytes = 0;
   if (neural) {
     wbytes = (size_t)L.nsplit * h * h * es;
@@ -850,7 +850,7 @@ Layout make_layout(const gdx_xc_desc& d, bool hvp = false) {
   // two-stage reductions: up to two at once (reduce_slices2: the largest one plus the last layer's bias)
   L.rtmp = take((size_t)RED_GROUPS * (std::max(std::max(h * h, 16 * h), 2 * nP * nP) + 64) * 8);
   if (hvp) {
-    L.Dp2 = take(2 * nP * nP * es);
+    L.Dp2 = take(batch ? 0 : 2 * nP * nP * es);
     L.dpart_t = take(2 * Np * DENS_STRIDE * 8);
     L.dfin_t = take(Np * 2 * DENS_STRIDE * 8);
     L.Edot = take(Np * 4 * 8);
@@ -946,13 +946,174 @@ ParamOffsets param_offsets(const gdx_xc_desc& d) {
   return o;
 }
 
+// ---------------------------------------------------------------- molecule batch ----
+//
+// A batch of molecules (one functional, one precision, different grids and bases) runs as one
+// evaluation: every per-point buffer is the concatenation of the molecules' rows (each
+// molecule's npts_pad a multiple of 256, so every 256-, 128- and 64-row block belongs to one
+// molecule), the MLP GEMMs run once over all rows, and the AO-side kernels (padding of D, the
+// density kernel, V_xc and its reduction) are single launches over a ragged queue of
+// (molecule, tile) work items.  Per-molecule scalars (E_xc, edot) are segment sums of the
+// block partials; per-molecule cotangents enter as per-point scales.
+
+struct PadBatchMol {   // rdm1_cat[dm_off ...] (2, n, n) -> Dp region [dp_off ...] (2, nP, nP)
+  long dm_off, dp_off;
+  int n, nP, blk0, nblk;
+};
+struct RedBatchMol {   // V partials of one molecule -> d_rdm1_cat[out_off ...] (2, n, n)
+  const double* vpart;
+  long out_off;
+  int nchunk, nP, n, blk0;
+};
+
+template <typename T>
+__global__ void pad_dm_batch_kernel(const PadBatchMol* __restrict__ mols, int nmol, const double* __restrict__ rdm1,
+                                    T* __restrict__ Dp) {
+  int lo = 0, hi = nmol - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (mols[mid].blk0 <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
+  }
+  const PadBatchMol& M = mols[lo];
+  const long i = (long)(blockIdx.x - M.blk0) * blockDim.x + threadIdx.x;
+  const long tot = 2L * M.nP * M.nP;
+  if (i >= tot) return;
+  const int sp = (int)(i / ((long)M.nP * M.nP));
+  const int r = (int)((i / M.nP) % M.nP);
+  const int c = (int)(i % M.nP);
+  Dp[M.dp_off + i] = (T)((r < M.n && c < M.n) ? rdm1[M.dm_off + ((long)sp * M.n + r) * M.n + c] : 0.0);
+}
+
+// out[m] = sum of part[seg[m] .. seg[m+1]) in the order of reduce_scalar_kernel
+__global__ void reduce_segments_kernel(const double* __restrict__ part, const int* __restrict__ seg,
+                                       double* __restrict__ out) {
+  __shared__ double red[NT / 64];
+  const int m = blockIdx.x;
+  const int b = seg[m], n = seg[m + 1] - b;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += part[b + i];
+  acc = warp_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < NT / 64; ++w) t += red[w];
+    out[m] = t;
+  }
+}
+
+// per-point g (g_pt[r] = g[m(r)]) and weights scaled by it (wg[r] = g[m(r)] w[r]); rows[m]
+// is molecule m's first row, rows[nmol] the total
+__global__ void segment_scale_kernel(const int* __restrict__ rows, int nmol, const double* __restrict__ g,
+                                     const double* __restrict__ w, double* __restrict__ g_pt, double* __restrict__ wg) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows[nmol]) return;
+  int lo = 0, hi = nmol - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (rows[mid] <= r) lo = mid; else hi = mid - 1;
+  }
+  const double gm = g[lo];
+  if (g_pt) g_pt[r] = gm;
+  if (wg) wg[r] = gm * w[r];
+}
+
+// V partials (nchunk, nsv, nP, nP) of each molecule -> d_rdm1_cat (2, n, n) blocks, as
+// reduce_vxc_kernel (nsv = 1: spin 0 copied to spin 1)
+__global__ void reduce_vxc_batch_kernel(const RedBatchMol* __restrict__ mols, int nmol, int nsv,
+                                        double* __restrict__ out) {
+  int lo = 0, hi = nmol - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (mols[mid].blk0 <= (int)blockIdx.x) lo = mid; else hi = mid - 1;
+  }
+  const RedBatchMol& M = mols[lo];
+  const long i = (long)(blockIdx.x - M.blk0) * blockDim.x + threadIdx.x;
+  const long n = M.n, nP = M.nP;
+  if (i >= 2L * n * n) return;
+  const int s = nsv == 1 ? 0 : (int)(i / (n * n));
+  const long a = (i / n) % n, b = i % n;
+  const long off = ((long)s * nP + a) * nP + b;
+  const long stride = (long)nsv * nP * nP;
+  double acc = 0.0;
+  for (int c = 0; c < M.nchunk; ++c) acc += M.vpart[c * stride + off];
+  out[M.out_off + i] = acc;
+}
+
+constexpr int VXB_CHUNK = 2048;   // points per V_xc block of a batch (multiple of VX_P)
+
+struct BatchCtx {
+  int nmol = 0;
+  int ntile_total = 0;                       // 64-point tiles of the density launch
+  const AoBatchMol* d_ao = nullptr;          // Dp -> the densities' padded matrices
+  const AoBatchMol* d_ao_t = nullptr;        // Dp -> the HVP direction's padded matrices
+  const PadBatchMol* d_pad = nullptr;
+  int pad_blocks = 0;
+  void* Dp = nullptr;                        // padded matrices (sum of 2 nP^2 elements)
+  void* Dp_t = nullptr;
+  const VxcBatchMol* d_vx[2] = {nullptr, nullptr};   // single-tile (SAME) / multi-tile groups
+  int nvx[2] = {0, 0}, vx_blocks[2] = {0, 0};
+  const RedBatchMol* d_red = nullptr;
+  int red_blocks = 0;
+  const int* seg256 = nullptr;               // (nmol + 1) segment starts in 256-row blocks
+  const int* seg64 = nullptr;                // in 64-row blocks
+  const int* rows = nullptr;                 // (nmol + 1) first rows
+  const double* weights = nullptr;           // concatenated grid weights
+  double* wg = nullptr;                      // weights scaled by the molecules' cotangents
+  double* g_pt = nullptr;                    // per-point cotangent
+  double* one = nullptr;                     // device 1.0
+  double* scal = nullptr;                    // nmol doubles of scratch
+  long dm_total = 0;                         // elements of a concatenated (2, n, n) array
+};
+
 struct Ctx {
   const gdx_xc_desc& d;
   Layout L;
   char* ws;
   hipStream_t st;
+  const BatchCtx* b = nullptr;
   template <typename U> U* P(size_t off) const { return (U*)(ws + off); }
 };
+
+template <typename T>
+void launch_density_batch(const Ctx& c, const double* mats, bool tangent, int nspin, double* out) {
+  const BatchCtx& B = *c.b;
+  hipStream_t st = c.st;
+  GDX_LAUNCH("pad_dm", pad_dm_batch_kernel<T>, dim3(B.pad_blocks), dim3(NT), 0, st, B.d_pad, B.nmol, mats,
+             (T*)(tangent ? B.Dp_t : B.Dp));
+  GDX_LAUNCH(tname<T>("density_f64", "density_f32"), density_ao_batch_kernel<T>, dim3(B.ntile_total, nspin),
+             dim3(NT), 0, st, tangent ? B.d_ao_t : B.d_ao, B.nmol, (size_t)c.d.npts_pad, out);
+}
+
+template <typename T>
+void launch_vxc_batch(const Ctx& c, int nsv, double* d_rdm1) {
+  const BatchCtx& B = *c.b;
+  hipStream_t st = c.st;
+  const char* nm = tname<T>("vxc_f64", "vxc_f32");
+  const double* vpt = c.P<double>(c.L.vpt);
+  const size_t rows = (size_t)c.d.npts_pad;
+  for (int g = 0; g < 2; ++g) {
+    if (!B.nvx[g]) continue;
+    const dim3 grid(B.vx_blocks[g], nsv);
+    if constexpr (sizeof(T) == 8) {
+      if (g == 0)
+        GDX_LAUNCH(nm, (vxc_ao_batch_kernel<double, true, 128>), grid, dim3(NT), 0, st, B.d_vx[g], B.nvx[g], vpt,
+                   rows, nsv);
+      else
+        GDX_LAUNCH(nm, (vxc_ao_batch_kernel<double, false, 128>), grid, dim3(NT), 0, st, B.d_vx[g], B.nvx[g], vpt,
+                   rows, nsv);
+    } else {
+      if (g == 0)
+        GDX_LAUNCH(nm, (vxc_ao_batch_kernel<float, true, 128, true>), grid, dim3(NT), 0, st, B.d_vx[g], B.nvx[g],
+                   vpt, rows, nsv);
+      else
+        GDX_LAUNCH(nm, (vxc_ao_batch_kernel<float, false, 128, true>), grid, dim3(NT), 0, st, B.d_vx[g], B.nvx[g],
+                   vpt, rows, nsv);
+    }
+  }
+  GDX_LAUNCH("reduce_vxc", reduce_vxc_batch_kernel, dim3(B.red_blocks), dim3(NT), 0, st, B.d_red, B.nmol, nsv,
+             d_rdm1);
+}
 
 // densities: pad D -> density_ao -> point_fwd
 template <typename T>
@@ -962,10 +1123,14 @@ void run_densities(const Ctx& c, const T* ao, const double* rdm1, const double* 
   const bool neural = is_neural(d);
   const int hmode = d.kind == GDX_FUNC_B3LYP_NEURAL_CORR ? 1 : 0;
   hipStream_t st = c.st;
-  GDX_LAUNCH("pad_dm", pad_dm_kernel<T>, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, rdm1, d.nao, nP,
-             c.P<T>(c.L.Dp));
-  GDX_LAUNCH(tname<T>("density_f64", "density_f32"), density_ao_kernel<T>, dim3(Np / DN_PT, c.L.nsd), dim3(NT), 0,
-             st, ao, Np, nP, c.P<T>(c.L.Dp), c.P<double>(c.L.dens_part));
+  if (c.b) {
+    launch_density_batch<T>(c, rdm1, false, c.L.nsd, c.P<double>(c.L.dens_part));
+  } else {
+    GDX_LAUNCH("pad_dm", pad_dm_kernel<T>, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, rdm1, d.nao, nP,
+               c.P<T>(c.L.Dp));
+    GDX_LAUNCH(tname<T>("density_f64", "density_f32"), density_ao_kernel<T>, dim3(Np / DN_PT, c.L.nsd), dim3(NT), 0,
+               st, ao, Np, nP, c.P<T>(c.L.Dp), c.P<double>(c.L.dens_part));
+  }
   GDX_LAUNCH("point_fwd", point_fwd_kernel<T>, dim3(c.L.npt_blocks), dim3(NT), 0, st, c.P<double>(c.L.dens_part),
              c.L.nsd, d.npts, Np, weights, neural ? 1 : 0, lin_coef(d), dens_fin, c.P<double>(c.L.E), c.P<T>(c.L.X),
              c.P<double>(c.L.eps_part));
@@ -1130,7 +1295,14 @@ int forward_impl(const Ctx& c, const T* ao, const double* weights, const double*
       nred = Np / 64;
     }
   }
-  GDX_LAUNCH("reduce_scalar", reduce_scalar_kernel, dim3(1), dim3(NT), 0, st, c.P<double>(L.eps_part), nred, exc);
+  if (c.b) {
+    // block partials of 256 rows (point_fwd) or 64 rows (head kernels)
+    const int* seg = nred == L.npt_blocks ? c.b->seg256 : c.b->seg64;
+    GDX_LAUNCH("reduce_scalar", reduce_segments_kernel, dim3(c.b->nmol), dim3(NT), 0, st, c.P<double>(L.eps_part), seg,
+               exc);
+  } else {
+    GDX_LAUNCH("reduce_scalar", reduce_scalar_kernel, dim3(1), dim3(NT), 0, st, c.P<double>(L.eps_part), nred, exc);
+  }
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
 }
@@ -1245,7 +1417,10 @@ int backward_impl(const Ctx& c, const T* ao, const double* weights, const double
                c.P<double>(L.GE), neural ? c.P<T>(L.DX) : nullptr, d.mlp_nc, d.npts, Np, d.correlation,
                c.P<double>(L.vpt));
     // B3LYP on equal spin densities has equal spin potentials: form V once
-    launch_vxc<T>(c, ao, (!neural && d.spin_restricted) ? 1 : 2, d_rdm1);
+    if (c.b)
+      launch_vxc_batch<T>(c, (!neural && d.spin_restricted) ? 1 : 2, d_rdm1);
+    else
+      launch_vxc<T>(c, ao, (!neural && d.spin_restricted) ? 1 : 2, d_rdm1);
   }
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
@@ -1262,14 +1437,18 @@ int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm
   const int hmode = d.kind == GDX_FUNC_B3LYP_NEURAL_CORR ? 1 : 0;
   const int Np = d.npts_pad, nP = d.nao_pad;
   const int nd = 4;
-  double* scal = c.P<double>(L.scal);
+  double* scal = c.b ? c.b->scal : c.P<double>(L.scal);
   int rc = forward_impl<T>(c, ao, weights, rdm1, params, exc ? exc : scal);
   if (rc != GDX_OK) return rc;
   // tangent densities along dir (both spins, general direction)
-  GDX_LAUNCH("pad_dm", pad_dm_kernel<T>, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, dir, d.nao, nP,
-             c.P<T>(L.Dp2));
-  GDX_LAUNCH(tname<T>("density_f64", "density_f32"), density_ao_kernel<T>, dim3(Np / DN_PT, 2), dim3(NT), 0, st,
-             ao, Np, nP, c.P<T>(L.Dp2), c.P<double>(L.dpart_t));
+  if (c.b) {
+    launch_density_batch<T>(c, dir, true, 2, c.P<double>(L.dpart_t));
+  } else {
+    GDX_LAUNCH("pad_dm", pad_dm_kernel<T>, dim3(nblocks(2L * nP * nP)), dim3(NT), 0, st, dir, d.nao, nP,
+               c.P<T>(L.Dp2));
+    GDX_LAUNCH(tname<T>("density_f64", "density_f32"), density_ao_kernel<T>, dim3(Np / DN_PT, 2), dim3(NT), 0, st,
+               ao, Np, nP, c.P<T>(L.Dp2), c.P<double>(L.dpart_t));
+  }
   T* X2 = c.P<T>(L.X);
   GDX_LAUNCH("point_tangent", point_tangent_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st,
              c.P<double>(L.dpart_t), c.P<double>(L.dens_fin), Np, neural ? 1 : 0, d.correlation, c.P<double>(L.dfin_t),
@@ -1306,8 +1485,11 @@ int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm
   GDX_LAUNCH("hvp_seed", hvp_seed_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st, neural ? c.P<T>(L.C) : nullptr,
              neural ? c.P<T>(L.ZT) : nullptr, c.P<double>(L.E), c.P<double>(L.Edot), weights, nd, d.npts, Np, g_exc,
              neural ? 1 : 0, c.P<double>(L.GE), c.P<double>(L.GEd), neural ? c.P<T>(L.DZ) : nullptr,
-             c.P<double>(L.dbpart), c.P<double>(L.edot_part), hmode, lin_coef(d));
-  if (edot)
+             c.P<double>(L.dbpart), c.P<double>(L.edot_part), hmode, lin_coef(d), c.b ? c.b->g_pt : nullptr);
+  if (edot && c.b)
+    GDX_LAUNCH("reduce_scalar", reduce_segments_kernel, dim3(c.b->nmol), dim3(NT), 0, st, c.P<double>(L.edot_part),
+               c.b->seg256, edot);
+  else if (edot)
     GDX_LAUNCH("reduce_scalar", reduce_scalar_kernel, dim3(1), dim3(NT), 0, st, c.P<double>(L.edot_part),
                L.npt_blocks, edot);
   if (neural) {
@@ -1369,7 +1551,10 @@ int hvp_impl(const Ctx& c, const T* ao, const double* weights, const double* rdm
     GDX_LAUNCH("point_hvp_bwd", point_hvp_bwd_kernel<T>, dim3(L.npt_blocks), dim3(NT), 0, st,
                c.P<double>(L.dens_fin), c.P<double>(L.dfin_t), c.P<double>(L.GE), c.P<double>(L.GEd),
                neural ? c.P<T>(L.DX) : nullptr, d.mlp_nc, d.npts, Np, d.correlation, c.P<double>(L.vpt));
-    launch_vxc<T>(c, ao, 2, d_rdm1, true);
+    if (c.b)
+      launch_vxc_batch<T>(c, 2, d_rdm1);
+    else
+      launch_vxc<T>(c, ao, 2, d_rdm1, true);
   }
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
@@ -1435,6 +1620,269 @@ int gdx_xc_hvp(void* stream, const gdx_xc_desc* desc, const void* ao, const doub
   if (desc->precision == GDX_FP64)
     return hvp_impl<double>(c, (const double*)ao, weights, rdm1, params, dir, g_exc, exc, edot, d_rdm1, d_params);
   return hvp_impl<float>(c, (const float*)ao, weights, rdm1, params, dir, g_exc, exc, edot, d_rdm1, d_params);
+}
+
+}  // extern "C"
+
+namespace gdx {
+namespace {
+
+struct XcBatchPlan {
+  gdx_xc_desc d{};
+  Layout L{}, Lh{};
+  char* ws = nullptr;    // forward / backward workspace
+  char* wsh = nullptr;   // second-order workspace
+  char* tab = nullptr;   // tables and batch buffers
+  BatchCtx B;
+  std::vector<long> dm_off;
+  bool hvp = false;
+};
+
+void batch_free(XcBatchPlan* P) {
+  if (!P) return;
+  if (P->ws) (void)hipFree(P->ws);
+  if (P->wsh) (void)hipFree(P->wsh);
+  if (P->tab) (void)hipFree(P->tab);
+  delete P;
+}
+
+}  // namespace
+}  // namespace gdx
+
+extern "C" {
+
+int gdx_xc_batch_create(const gdx_xc_desc* desc, int32_t nmol, const gdx_xc_batch_mol* mols, int32_t with_hvp,
+                        void** plan) {
+  GDX_REQUIRE(desc && mols && plan && nmol > 0, "gdx_xc_batch_create: bad arguments");
+  *plan = nullptr;
+  long rows = 0;
+  int maxP = 16;
+  for (int m = 0; m < nmol; ++m) {
+    const gdx_xc_batch_mol& M = mols[m];
+    GDX_REQUIRE(M.ao && M.weights, "gdx_xc_batch_create: null AO planes or weights");
+    GDX_REQUIRE(M.npts > 0 && M.npts_pad >= M.npts && M.npts_pad % 256 == 0,
+                "gdx_xc_batch_create: every molecule's npts_pad must be a multiple of 256 >= npts");
+    GDX_REQUIRE(M.nao > 0 && M.nao_pad >= M.nao && M.nao_pad % 16 == 0,
+                "gdx_xc_batch_create: nao_pad must be a multiple of 16 >= nao");
+    rows += M.npts_pad;
+    maxP = std::max(maxP, (int)M.nao_pad);
+  }
+  GDX_REQUIRE(rows < (1L << 31), "gdx_xc_batch_create: more than 2^31 grid rows in the batch");
+  XcBatchPlan* P = new XcBatchPlan();
+  P->d = *desc;
+  P->d.npts = (int)rows;
+  P->d.npts_pad = (int)rows;
+  P->d.nao = maxP;
+  P->d.nao_pad = maxP;
+  int rc = validate(&P->d);
+  if (rc != GDX_OK) {
+    delete P;
+    return rc;
+  }
+  const gdx_xc_desc& d = P->d;
+  const size_t es = elem_size(d.precision);
+  const int nsv_max = 2;
+  P->hvp = with_hvp != 0;
+  P->L = make_layout(d, false, true);
+  if (P->hvp) P->Lh = make_layout(d, true, true);
+  // host tables
+  std::vector<AoBatchMol> ao(nmol), ao_t(nmol);
+  std::vector<PadBatchMol> pad(nmol);
+  std::vector<VxcBatchMol> vx[2];
+  std::vector<RedBatchMol> red(nmol);
+  std::vector<int> seg256(nmol + 1), seg64(nmol + 1), rowv(nmol + 1);
+  long row = 0, dm = 0, dp = 0, vp = 0;
+  int tile = 0, pblk = 0, rblk = 0, vblk[2] = {0, 0};
+  P->dm_off.resize(nmol + 1);
+  std::vector<long> vpart_off(nmol);
+  for (int m = 0; m < nmol; ++m) {
+    const gdx_xc_batch_mol& M = mols[m];
+    const int nP = M.nao_pad, n = M.nao;
+    P->dm_off[m] = dm;
+    seg256[m] = (int)(row / 256);
+    seg64[m] = (int)(row / 64);
+    rowv[m] = (int)row;
+    ao[m].ao = M.ao;
+    ao[m].row0 = row;
+    ao[m].npts_pad = M.npts_pad;
+    ao[m].nP = nP;
+    ao[m].tile0 = tile;
+    ao[m].ntile = M.npts_pad / DN_PT;
+    pad[m].dm_off = dm;
+    pad[m].dp_off = dp;
+    pad[m].n = n;
+    pad[m].nP = nP;
+    pad[m].blk0 = pblk;
+    pad[m].nblk = (int)((2L * nP * nP + NT - 1) / NT);
+    const int nt = (nP + 127) / 128;
+    VxcBatchMol v{};
+    v.ao = M.ao;
+    v.row0 = row;
+    v.npts_pad = M.npts_pad;
+    v.nP = nP;
+    v.ntile = nt * nt;
+    v.chunk = VXB_CHUNK;
+    v.nchunk = (M.npts_pad + VXB_CHUNK - 1) / VXB_CHUNK;
+    const int g = nt == 1 ? 0 : 1;
+    v.blk0 = vblk[g];
+    vblk[g] += v.ntile * v.nchunk;
+    vpart_off[m] = vp;
+    vx[g].push_back(v);
+    red[m].out_off = dm;
+    red[m].nchunk = v.nchunk;
+    red[m].nP = nP;
+    red[m].n = n;
+    red[m].blk0 = rblk;
+    rblk += (int)((2L * n * n + NT - 1) / NT);
+    tile += ao[m].ntile;
+    pblk += pad[m].nblk;
+    row += M.npts_pad;
+    dm += 2L * n * n;
+    dp += 2L * nP * nP;
+    vp += (long)v.nchunk * nsv_max * nP * nP;
+  }
+  seg256[nmol] = (int)(row / 256);
+  seg64[nmol] = (int)(row / 64);
+  rowv[nmol] = (int)row;
+  P->dm_off[nmol] = dm;
+  // device memory: tables, per-point buffers, padded matrices, V partials
+  size_t off = 0;
+  auto take = [&](size_t bytes) { size_t o = off; off += round_up(std::max<size_t>(bytes, 8), 256); return o; };
+  const size_t o_ao = take(nmol * sizeof(AoBatchMol)), o_aot = take(nmol * sizeof(AoBatchMol));
+  const size_t o_pad = take(nmol * sizeof(PadBatchMol));
+  const size_t o_vx0 = take(vx[0].size() * sizeof(VxcBatchMol)), o_vx1 = take(vx[1].size() * sizeof(VxcBatchMol));
+  const size_t o_red = take(nmol * sizeof(RedBatchMol));
+  const size_t o_s256 = take((nmol + 1) * sizeof(int)), o_s64 = take((nmol + 1) * sizeof(int));
+  const size_t o_rows = take((nmol + 1) * sizeof(int));
+  const size_t o_w = take(rows * 8), o_wg = take(rows * 8), o_gpt = take(rows * 8);
+  const size_t o_one = take(8), o_scal = take(nmol * 8);
+  const size_t o_dp = take(dp * es), o_dpt = take(P->hvp ? dp * es : 0);
+  const size_t o_vp = take(vp * 8);
+  auto fail = [&](hipError_t e, const char* what) {
+    set_error(std::string("gdx_xc_batch_create: ") + what + ": " + hipGetErrorString(e));
+    batch_free(P);
+    return GDX_ERR_HIP;
+  };
+  hipError_t e = hipMalloc((void**)&P->tab, off);
+  if (e != hipSuccess) return fail(e, "hipMalloc");
+  e = hipMalloc((void**)&P->ws, P->L.total);
+  if (e != hipSuccess) return fail(e, "hipMalloc");
+  if (P->hvp) {
+    e = hipMalloc((void**)&P->wsh, P->Lh.total);
+    if (e != hipSuccess) return fail(e, "hipMalloc");
+  }
+  char* T0 = P->tab;
+  for (int m = 0; m < nmol; ++m) {
+    ao[m].Dp = T0 + o_dp + pad[m].dp_off * es;
+    ao_t[m] = ao[m];
+    ao_t[m].Dp = T0 + o_dpt + pad[m].dp_off * es;
+    red[m].vpart = (const double*)(T0 + o_vp) + vpart_off[m];
+  }
+  {
+    int k[2] = {0, 0};
+    for (int m = 0; m < nmol; ++m) {
+      const int g = ((mols[m].nao_pad + 127) / 128) == 1 ? 0 : 1;
+      vx[g][k[g]++].vpart = (double*)(T0 + o_vp) + vpart_off[m];
+    }
+  }
+  const double one = 1.0;
+  struct Up { size_t off; const void* src; size_t bytes; };
+  const Up ups[] = {{o_ao, ao.data(), nmol * sizeof(AoBatchMol)}, {o_aot, ao_t.data(), nmol * sizeof(AoBatchMol)},
+                    {o_pad, pad.data(), nmol * sizeof(PadBatchMol)},
+                    {o_vx0, vx[0].data(), vx[0].size() * sizeof(VxcBatchMol)},
+                    {o_vx1, vx[1].data(), vx[1].size() * sizeof(VxcBatchMol)},
+                    {o_red, red.data(), nmol * sizeof(RedBatchMol)},
+                    {o_s256, seg256.data(), (nmol + 1) * sizeof(int)}, {o_s64, seg64.data(), (nmol + 1) * sizeof(int)},
+                    {o_rows, rowv.data(), (nmol + 1) * sizeof(int)}, {o_one, &one, 8}};
+  for (const Up& u : ups) {
+    if (!u.bytes) continue;
+    e = hipMemcpy(T0 + u.off, u.src, u.bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(e, "hipMemcpy");
+  }
+  for (int m = 0; m < nmol; ++m) {   // concatenated weights (zero past each molecule's own rows)
+    e = hipMemcpy(T0 + o_w + (size_t)rowv[m] * 8, mols[m].weights, (size_t)mols[m].npts_pad * 8,
+                  hipMemcpyDeviceToDevice);
+    if (e != hipSuccess) return fail(e, "hipMemcpy");
+  }
+  BatchCtx& B = P->B;
+  B.nmol = nmol;
+  B.ntile_total = tile;
+  B.d_ao = (const AoBatchMol*)(T0 + o_ao);
+  B.d_ao_t = (const AoBatchMol*)(T0 + o_aot);
+  B.d_pad = (const PadBatchMol*)(T0 + o_pad);
+  B.pad_blocks = pblk;
+  B.Dp = T0 + o_dp;
+  B.Dp_t = T0 + o_dpt;
+  B.d_vx[0] = (const VxcBatchMol*)(T0 + o_vx0);
+  B.d_vx[1] = (const VxcBatchMol*)(T0 + o_vx1);
+  B.nvx[0] = (int)vx[0].size();
+  B.nvx[1] = (int)vx[1].size();
+  B.vx_blocks[0] = vblk[0];
+  B.vx_blocks[1] = vblk[1];
+  B.d_red = (const RedBatchMol*)(T0 + o_red);
+  B.red_blocks = rblk;
+  B.seg256 = (const int*)(T0 + o_s256);
+  B.seg64 = (const int*)(T0 + o_s64);
+  B.rows = (const int*)(T0 + o_rows);
+  B.weights = (const double*)(T0 + o_w);
+  B.wg = (double*)(T0 + o_wg);
+  B.g_pt = (double*)(T0 + o_gpt);
+  B.one = (double*)(T0 + o_one);
+  B.scal = (double*)(T0 + o_scal);
+  B.dm_total = dm;
+  *plan = P;
+  return GDX_OK;
+}
+
+int gdx_xc_batch_destroy(void* plan) {
+  batch_free((XcBatchPlan*)plan);
+  return GDX_OK;
+}
+
+int gdx_xc_batch_offsets(const void* plan, int64_t* dm_off) {
+  GDX_REQUIRE(plan && dm_off, "gdx_xc_batch_offsets: null pointer");
+  const XcBatchPlan* P = (const XcBatchPlan*)plan;
+  for (size_t m = 0; m < P->dm_off.size(); ++m) dm_off[m] = P->dm_off[m];
+  return GDX_OK;
+}
+
+int gdx_xc_batch_forward(void* stream, void* plan, const double* rdm1, const double* params, double* exc) {
+  GDX_REQUIRE(plan && rdm1 && exc, "gdx_xc_batch_forward: null pointer");
+  XcBatchPlan* P = (XcBatchPlan*)plan;
+  GDX_REQUIRE(!is_neural(P->d) || params, "gdx_xc_batch_forward: neural functional needs params");
+  Ctx c{P->d, P->L, P->ws, (hipStream_t)stream, &P->B};
+  if (P->d.precision == GDX_FP64) return forward_impl<double>(c, nullptr, P->B.weights, rdm1, params, exc);
+  return forward_impl<float>(c, nullptr, P->B.weights, rdm1, params, exc);
+}
+
+int gdx_xc_batch_backward(void* stream, void* plan, const double* params, const double* g_exc, double* d_rdm1,
+                          double* d_params) {
+  GDX_REQUIRE(plan && g_exc, "gdx_xc_batch_backward: null pointer");
+  XcBatchPlan* P = (XcBatchPlan*)plan;
+  GDX_REQUIRE(!is_neural(P->d) || params, "gdx_xc_batch_backward: neural functional needs params");
+  hipStream_t st = (hipStream_t)stream;
+  const BatchCtx& B = P->B;
+  GDX_LAUNCH("segment_scale", segment_scale_kernel, dim3(nblocks(P->d.npts_pad)), dim3(NT), 0, st, B.rows, B.nmol,
+             g_exc, B.weights, (double*)nullptr, B.wg);
+  Ctx c{P->d, P->L, P->ws, st, &P->B};
+  if (P->d.precision == GDX_FP64) return backward_impl<double>(c, nullptr, B.wg, params, B.one, d_rdm1, d_params);
+  return backward_impl<float>(c, nullptr, B.wg, params, B.one, d_rdm1, d_params);
+}
+
+int gdx_xc_batch_hvp(void* stream, void* plan, const double* rdm1, const double* params, const double* dir,
+                     const double* g_exc, double* exc, double* edot, double* d_rdm1, double* d_params) {
+  GDX_REQUIRE(plan && rdm1 && dir && g_exc, "gdx_xc_batch_hvp: null pointer");
+  XcBatchPlan* P = (XcBatchPlan*)plan;
+  GDX_REQUIRE(P->hvp, "gdx_xc_batch_hvp: plan created without the second-order workspace");
+  GDX_REQUIRE(!is_neural(P->d) || params, "gdx_xc_batch_hvp: neural functional needs params");
+  hipStream_t st = (hipStream_t)stream;
+  const BatchCtx& B = P->B;
+  GDX_LAUNCH("segment_scale", segment_scale_kernel, dim3(nblocks(P->d.npts_pad)), dim3(NT), 0, st, B.rows, B.nmol,
+             g_exc, B.weights, B.g_pt, (double*)nullptr);
+  Ctx c{P->d, P->Lh, P->wsh, st, &P->B};
+  if (P->d.precision == GDX_FP64)
+    return hvp_impl<double>(c, nullptr, B.weights, rdm1, params, dir, B.one, exc, edot, d_rdm1, d_params);
+  return hvp_impl<float>(c, nullptr, B.weights, rdm1, params, dir, B.one, exc, edot, d_rdm1, d_params);
 }
 
 int gdx_xc_backward(void* stream, const gdx_xc_desc* desc, const void* ao, const double* weights,

@@ -41,9 +41,12 @@ def cholesky_inv(S: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
     return L, Linv
 
 
-def eigh(A: torch.Tensor, V0: Optional[torch.Tensor] = None) -> Tuple[torch.Tensor, torch.Tensor]:
+def eigh(A: torch.Tensor, V0: Optional[torch.Tensor] = None,
+         sweeps: Optional[torch.Tensor] = None) -> Tuple[torch.Tensor, torch.Tensor]:
     """Batched eigh of symmetric A (..., n, n): ascending eigenvalues, eigenvectors as columns.
-    V0 (same shape, orthonormal columns): starting eigenvector estimates (warm start)."""
+    V0 (same shape, orthonormal columns): starting eigenvector estimates (warm start).
+    sweeps (int32 device tensor, one per matrix): Jacobi sweeps used (JACOBI_MAX_SWEEPS = not
+    converged)."""
     shape = A.shape
     n = shape[-1]
     Ab = A.reshape(-1, n, n).contiguous()
@@ -56,7 +59,8 @@ def eigh(A: torch.Tensor, V0: Optional[torch.Tensor] = None) -> Tuple[torch.Tens
     _lib.check(lib.gdx_eigh_workspace_size(nb, n, ctypes.byref(nbytes)), "gdx_eigh_workspace_size")
     ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=A.device)
     _lib.check(lib.gdx_eigh_batched(_stream(A), nb, n, Ab.data_ptr(), 0 if V0b is None else V0b.data_ptr(),
-                                    w.data_ptr(), V.data_ptr(), ws.data_ptr(), JACOBI_MAX_SWEEPS, JACOBI_TOL, 0),
+                                    w.data_ptr(), V.data_ptr(), ws.data_ptr(), JACOBI_MAX_SWEEPS, JACOBI_TOL,
+                                    0 if sweeps is None else sweeps.data_ptr()),
                "gdx_eigh_batched")
     return w.reshape(shape[:-1]), V.reshape(shape)
 
@@ -65,8 +69,8 @@ class EighFn(torch.autograd.Function):
     """Batched eigh on the device with the reverse mode of safe_eigh (safe_eigh_vjp)."""
 
     @staticmethod
-    def forward(ctx, A, V0=None):
-        w, V = eigh(A, V0)
+    def forward(ctx, A, V0=None, sweeps=None):
+        w, V = eigh(A, V0, sweeps)
         ctx.save_for_backward(w, V)
         return w, V
 
@@ -78,11 +82,11 @@ class EighFn(torch.autograd.Function):
         wb, Vb = w.reshape(-1, n), V.reshape(-1, n, n)
         gw = torch.zeros_like(wb) if gw is None else gw.reshape(-1, n)
         gV = torch.zeros_like(Vb) if gV is None else gV.reshape(-1, n, n)
-        return safe_eigh_vjp(wb, Vb, gw, gV).reshape(V.shape), None
+        return safe_eigh_vjp(wb, Vb, gw, gV).reshape(V.shape), None, None
 
 
 def scf_step(focks: torch.Tensor, x: torch.Tensor, Linv: torch.Tensor, V0: Optional[torch.Tensor],
-             occ: torch.Tensor):
+             occ: torch.Tensor, sweeps: Optional[torch.Tensor] = None):
     """Forward-only diagonalisation step of the SCF loop in one device call (gdx_scf_step):
     F = sum_i x[:, 1+i] focks[i], the eigenproblem of sym(Linv F Linv^T) warm-started from V0,
     C = Linv^T V and the density C diag(occ) C^T.  Returns (F, w, C, V, rdm1)."""
@@ -102,18 +106,19 @@ def scf_step(focks: torch.Tensor, x: torch.Tensor, Linv: torch.Tensor, V0: Optio
     _lib.check(lib.gdx_scf_step(_stream(focks), ns, n, m, focks.data_ptr(), xc.data_ptr(), Lc.data_ptr(),
                                 0 if V0c is None else V0c.data_ptr(), oc.data_ptr(), F.data_ptr(), w.data_ptr(),
                                 V.data_ptr(), C.data_ptr(), D.data_ptr(), ws.data_ptr(), JACOBI_MAX_SWEEPS,
-                                JACOBI_TOL, 0), "gdx_scf_step")
+                                JACOBI_TOL, 0 if sweeps is None else sweeps.data_ptr()), "gdx_scf_step")
     return F, w, C, V, D
 
 
-def orthogonal_eigh(A: torch.Tensor, Linv: torch.Tensor, V0: Optional[torch.Tensor] = None):
+def orthogonal_eigh(A: torch.Tensor, Linv: torch.Tensor, V0: Optional[torch.Tensor] = None,
+                    sweeps: Optional[torch.Tensor] = None):
     """(w, C, V): eigh of Linv A Linv^T = V diag(w) V^T and C = Linv^T V; V0 warm-starts."""
     C = Linv @ A @ Linv.transpose(-1, -2)
     C = 0.5 * (C + C.transpose(-1, -2))
     if torch.is_grad_enabled() and C.requires_grad:
-        w, V = EighFn.apply(C, V0)
+        w, V = EighFn.apply(C, V0, sweeps)
     else:
-        w, V = eigh(C, V0)
+        w, V = eigh(C, V0, sweeps)
     return w, Linv.transpose(-1, -2) @ V, V
 
 
@@ -188,3 +193,29 @@ def householder_q(W: torch.Tensor) -> torch.Tensor:
         w = tau[:, None] * torch.einsum("bi,bik->bk", v, sub)
         Q = torch.cat([Q[:, :j, :], sub - v[:, :, None] * w[:, None, :]], dim=1)
     return Q
+
+
+STATUS_FLAGS = {_lib.GDX_STATUS_EIGH_NOT_CONVERGED: f"the Jacobi eigensolver did not converge in {JACOBI_MAX_SWEEPS} sweeps",
+                _lib.GDX_STATUS_NONFINITE_ENERGY: "the energy is not finite",
+                _lib.GDX_STATUS_CHARGE: "tr(D S) differs from the occupation count (orbitals not S-orthonormal)"}
+
+
+def scf_status(rdm1: torch.Tensor, S: torch.Tensor, occ: torch.Tensor, sweeps: Optional[torch.Tensor],
+               energy: Optional[torch.Tensor], status: torch.Tensor, tol: float = 1e-6) -> None:
+    """OR this cycle's checks into the device status word (gdx_scf_status); no host sync."""
+    ns, n = int(rdm1.shape[0]), int(rdm1.shape[-1])
+    D = rdm1.detach().contiguous()
+    oc = occ.detach().contiguous()
+    e = None if energy is None else energy.detach().reshape(()).contiguous()
+    _lib.check(_lib.load().gdx_scf_status(_stream(D), ns, n, D.data_ptr(), S.contiguous().data_ptr(), oc.data_ptr(),
+                                          0 if sweeps is None else sweeps.data_ptr(), JACOBI_MAX_SWEEPS,
+                                          0 if e is None else e.data_ptr(), float(tol), status.data_ptr()),
+               "gdx_scf_status")
+
+
+def raise_on_status(status: torch.Tensor, what: str) -> None:
+    """Read the status word (one host sync) and raise GdxError naming every flag set."""
+    bits = int(status.item())
+    if bits:
+        msgs = [m for b, m in STATUS_FLAGS.items() if bits & b]
+        raise _lib.GdxError(f"{what}: " + "; ".join(msgs))

@@ -22,7 +22,7 @@ from typing import Callable, List, Optional
 import torch
 
 from . import _lib
-from .eigenproblem import cholesky_inv, eigh, householder_q, orthogonal_eigh, scf_step
+from .eigenproblem import cholesky_inv, eigh, householder_q, orthogonal_eigh, raise_on_status, scf_status, scf_step
 from .functional import Functional
 from .molecule import Molecule, make_rdm1, orbital_grad
 from .train import energy_predictor
@@ -190,8 +190,19 @@ def _aufbau_occupations(molecule: Molecule) -> torch.Tensor:
     return occ
 
 
+def _status_buffers(molecule: Molecule):
+    """(status word, Jacobi sweeps per spin) device int32 buffers of a molecule's loop; kept in
+    the molecule cache so a captured loop writes the same buffers on every replay."""
+    buf = molecule.cache.get("scf_status")
+    if buf is None:
+        dev = molecule.s1e.device
+        buf = (torch.zeros(1, dtype=torch.int32, device=dev), torch.zeros(2, dtype=torch.int32, device=dev))
+        molecule.cache["scf_status"] = buf
+    return buf
+
+
 def diff_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp64", graph: bool = False,
-                  refine_cycles: Optional[int] = None, **kwargs) -> Callable:
+                  refine_cycles: Optional[int] = None, check_status: bool = True, **kwargs) -> Callable:
     """graph=True (forward only, autograd not recording): the whole loop is captured once per
     molecule as one HIP graph (after an eager run that sets up plans and workspaces) and
     replayed on later calls -- no host work between the kernels, as the reference's jitted
@@ -201,7 +212,11 @@ def diff_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp
     and the energy they return, use the FP64 grid path.  The returned energy is then an FP64
     evaluation, and a loop stopped before self-consistency -- first-order sensitive to the
     Fock matrices of its last cycles -- has the FP32 perturbations of the earlier cycles damped
-    by the iteration (naphthalene def2-SVP, 20 cycles: |dE| vs fp64 1e-6 Ha -> 2e-8 Ha)."""
+    by the iteration (naphthalene def2-SVP, 20 cycles: |dE| vs fp64 1e-6 Ha -> 2e-8 Ha).
+
+    Every cycle ORs its checks into a device status word (gdx_scf_status: eigensolver
+    convergence, a finite energy, tr(D S) equal to the occupation count); the word is read once
+    after the loop (or graph replay) and, with check_status, a set flag raises GdxError."""
     if refine_cycles is None:
         refine_cycles = 0 if precision == "fp64" else 1
     compute_energy = energy_predictor(functional, precision=precision, **kwargs)
@@ -213,6 +228,8 @@ def diff_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp
         if Linv is None:
             _, Linv = cholesky_inv(S)
             molecule.cache["Linv"] = Linv
+        status, sweeps = _status_buffers(molecule)
+        status.zero_()
         predicted_e, fock = compute_energy(params, molecule)
         molecule = molecule.replace(fock=fock)
         diis = JittableDiis(S, max_diis=10)
@@ -227,24 +244,30 @@ def diff_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp
             if fused:
                 data = diis.update(molecule.rdm1, molecule.fock, predicted_e, data, cycle)
                 _, x = _diis_coefficients(data[3], cycle)
-                F, mo_energy, mo_coeff, V, rdm1 = scf_step(data[1], x, Linv, V, occ)
+                F, mo_energy, mo_coeff, V, rdm1 = scf_step(data[1], x, Linv, V, occ, sweeps)
             else:
                 F, data = diis.run(molecule.rdm1, molecule.fock, predicted_e, data, cycle)
                 # safe_fock_solver, warm-started from the previous cycle's orthonormal eigenvectors
-                mo_energy, mo_coeff, V = orthogonal_eigh(F, Linv, V)
+                mo_energy, mo_coeff, V = orthogonal_eigh(F, Linv, V, sweeps)
                 # eigenvalues ascend, so the aufbau occupation of get_occ is the first nelec_s
                 rdm1 = torch.einsum("sij,sj,skj->sik", mo_coeff, occ, mo_coeff)
             molecule = molecule.replace(fock=F, mo_coeff=mo_coeff, mo_energy=mo_energy, mo_occ=occ,
                                         rdm1=rdm1, restricted=molecule.restricted and functional.spin_symmetric)
             predicted_e, fock = predictor(params, molecule)
             molecule = molecule.replace(fock=fock)
+            scf_status(molecule.rdm1, S, occ, sweeps, predicted_e, status)
         return molecule.replace(energy=predicted_e)
+
+    def checked(out: Molecule) -> Molecule:
+        if check_status:
+            raise_on_status(_status_buffers(out)[0], "diff_scf_loop")
+        return out
 
     def scf_iterator(params: Optional[torch.Tensor], molecule: Molecule) -> Molecule:
         recording = torch.is_grad_enabled() and (
             (params is not None and params.requires_grad) or molecule.rdm1.requires_grad)
         if not graph or recording:
-            return run_loop(params, molecule)
+            return checked(run_loop(params, molecule))
         key = ("scf_graph", id(functional), cycles, precision, refine_cycles, params is None)
         entry = molecule.cache.get(key)
         if entry is None:
@@ -261,15 +284,15 @@ def diff_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp
             with torch.cuda.graph(g):
                 m_out = run_loop(static_p, m_in)
             molecule.cache[key] = (g, static_p, static_D, m_out)
-            return out
+            return checked(out)
         g, static_p, static_D, m_out = entry
         if static_p is not None:
             static_p.copy_(params.detach())
         static_D.copy_(molecule.rdm1.detach())
         g.replay()
-        return molecule.replace(energy=m_out.energy.clone(), rdm1=m_out.rdm1.clone(), fock=m_out.fock.clone(),
-                                mo_coeff=m_out.mo_coeff.clone(), mo_energy=m_out.mo_energy.clone(),
-                                mo_occ=m_out.mo_occ.clone(), restricted=m_out.restricted)
+        return checked(molecule.replace(energy=m_out.energy.clone(), rdm1=m_out.rdm1.clone(), fock=m_out.fock.clone(),
+                                        mo_coeff=m_out.mo_coeff.clone(), mo_energy=m_out.mo_energy.clone(),
+                                        mo_occ=m_out.mo_occ.clone(), restricted=m_out.restricted))
 
     return scf_iterator
 
@@ -701,8 +724,10 @@ def batched_scf_loop(functional: Functional, cycles: int = 25, precision: str = 
     overlap.  Results equal the molecule-by-molecule loop (``train.py:519-527`` evaluates a
     batch that way).  Forward only; for reverse mode call ``diff_scf_loop`` per molecule.
 
-    ``iterator(params, molecules) -> [molecule]`` with the loops' outputs."""
-    loop = diff_scf_loop(functional, cycles=cycles, precision=precision, graph=True, **kwargs)
+    ``iterator(params, molecules) -> [molecule]`` with the loops' outputs.  The molecules'
+    status words are read once, after every replay has finished (check_status)."""
+    check = kwargs.pop("check_status", True)
+    loop = diff_scf_loop(functional, cycles=cycles, precision=precision, graph=True, check_status=False, **kwargs)
     streams: List[torch.cuda.Stream] = []
     captured = set()   # (id of the molecule's cache, params given): the cache holds the graph
 
@@ -732,6 +757,9 @@ def batched_scf_loop(functional: Functional, cycles: int = 25, precision: str = 
             for o in outs:                # outputs allocated on the side streams, used on main
                 for t in (o.energy, o.rdm1, o.fock, o.mo_coeff, o.mo_energy, o.mo_occ):
                     t.record_stream(main)
+            if check:
+                for i, o in enumerate(outs):
+                    raise_on_status(_status_buffers(o)[0], f"batched_scf_loop (molecule {i})")
             return outs
 
     return iterator

@@ -56,6 +56,19 @@ class MLPSpec:
         return 2 * sum(i * o for i, o in self.shapes())
 
 
+def ao_planes_for(m: Molecule, precision: str) -> torch.Tensor:
+    """AO planes in the element type of a precision: the FP64 planes, or their FP32 copy
+    (made once and cached on the molecule)."""
+    if precision == "fp64":
+        return m.ao_planes
+    key = ("ao_planes_f32", m.ao_planes.data_ptr())
+    a = m.cache.get(key)
+    if a is None:
+        a = m.ao_planes.to(torch.float32)
+        m.cache[key] = a
+    return a
+
+
 class XCPlan:
     """Descriptor + device workspace for one (molecule, functional, precision).
 
@@ -95,15 +108,7 @@ class XCPlan:
 
     def _ao(self) -> torch.Tensor:
         """AO planes in the element type of the path (FP32 copy cached on the molecule)."""
-        m = self.molecule
-        if self.precision == "fp64":
-            return m.ao_planes
-        key = ("ao_planes_f32", m.ao_planes.data_ptr())
-        a = m.cache.get(key)
-        if a is None:
-            a = m.ao_planes.to(torch.float32)
-            m.cache[key] = a
-        return a
+        return ao_planes_for(self.molecule, self.precision)
 
     def forward(self, rdm1: torch.Tensor, params: Optional[torch.Tensor], exc: torch.Tensor) -> None:
         m = self.molecule

@@ -78,12 +78,15 @@ def mse_energy_loss(params: torch.Tensor, compute_energy: Callable,
                     atoms_list: Union[Molecule, Sequence[Molecule]], truth_energies: Sequence[float],
                     elec_num_norm: bool = True) -> torch.Tensor:
     """``mse_energy_loss`` (train.py:478-530): sum_m ((E_m - E_m^ref) / N_e,m)^2 / M, with the
-    molecules evaluated one after another as in the reference.  Differentiable w.r.t. params."""
+    molecules evaluated one after another as in the reference, or all at once when compute_energy
+    is batched (``batched_diff_scf_loop``).  Differentiable w.r.t. params."""
     if isinstance(atoms_list, Molecule):
         atoms_list = [atoms_list]
     total = 0.0
+    # a batched compute_energy (batched_diff_scf_loop) evaluates the whole list at once
+    outs = compute_energy(params, list(atoms_list)) if getattr(compute_energy, "batched", False) else None
     for i, atoms in enumerate(atoms_list):
-        out = compute_energy(params, atoms)
+        out = outs[i] if outs is not None else compute_energy(params, atoms)
         diff = out.energy - truth_energies[i]
         if elec_num_norm:
             diff = diff / num_electrons(atoms)

@@ -128,6 +128,38 @@ int gdx_xc_hvp(void* stream, const gdx_xc_desc* desc, const void* ao, const doub
                const double* rdm1, const double* params, const double* dir, const double* g_exc,
                double* exc, double* edot, double* d_rdm1, double* d_params, void* workspace);
 
+/* ---- A batch of molecules ----------------------------------------------------------
+ * The XC evaluation of several molecules (one functional and precision, different grids and
+ * bases) as one: per-point buffers concatenate the molecules' rows, the MLP GEMMs run once
+ * over all rows, and the AO-side kernels (density, V_xc, their padding and reduction) are
+ * single launches over a ragged queue of (molecule, tile) work items.  This replaces the
+ * molecule-by-molecule loop of the reference's losses (grad_dft/train.py:519-527) for the XC
+ * part of every SCF cycle.  Densities, their cotangents and outputs are concatenated: molecule
+ * m's (2, n_m, n_m) block starts at dm_off[m] (gdx_xc_batch_offsets, nmol + 1 entries). */
+typedef struct gdx_xc_batch_mol {
+  const void* ao;          /* gdx_ao_eval planes (element type of the precision), npts_pad x nao_pad */
+  const double* weights;   /* (npts_pad) grid weights, zero past npts */
+  int32_t npts, npts_pad;  /* npts_pad a multiple of 256 */
+  int32_t nao, nao_pad;
+} gdx_xc_batch_mol;
+/* desc: kind, precision, spin_restricted (all molecules), MLP and correlation fields; its size
+ * fields are ignored.  The plan allocates its workspaces here (nothing is allocated by the
+ * calls below) and copies the weights; the AO planes are referenced and must outlive it. */
+int gdx_xc_batch_create(const gdx_xc_desc* desc, int32_t nmol, const gdx_xc_batch_mol* mols, int32_t with_hvp,
+                        void** plan);
+int gdx_xc_batch_destroy(void* plan);
+int gdx_xc_batch_offsets(const void* plan, int64_t* dm_off);
+/* exc (nmol) = E_xc of every molecule (gdx_xc_forward per molecule). */
+int gdx_xc_batch_forward(void* stream, void* plan, const double* rdm1, const double* params, double* exc);
+/* With cotangents g (nmol): d_rdm1 = g_m dE_m/dD_m (concatenated), d_params = sum_m g_m dE_m/dtheta.
+ * Uses the state of the last gdx_xc_batch_forward on the plan. */
+int gdx_xc_batch_backward(void* stream, void* plan, const double* params, const double* g_exc, double* d_rdm1,
+                          double* d_params);
+/* Second-order pass of every molecule (gdx_xc_hvp): Phi = sum_m g_m E_m + <dir_m, dE_m/dD_m>;
+ * exc, edot (nmol each), d_rdm1 (concatenated), d_params; plan created with with_hvp. */
+int gdx_xc_batch_hvp(void* stream, void* plan, const double* rdm1, const double* params, const double* dir,
+                     const double* g_exc, double* exc, double* edot, double* d_rdm1, double* d_params);
+
 /* ---- SCF side ---------------------------------------------------------------------
  * eri: 8-fold packed (ij|kl), pair index I = i(i+1)/2 + j (i >= j), element I(I+1)/2 + K
  * (I >= K); every matrix row-major, rdm1 (2, n, n). */
@@ -196,6 +228,18 @@ int gdx_scf_step_workspace_size(int32_t nspin, int32_t n, size_t* bytes);
 int gdx_scf_step(void* stream, int32_t nspin, int32_t n, int32_t m, const double* focks, const double* x,
                  const double* Linv, const double* V0, const double* occ, double* F, double* w, double* V,
                  double* C, double* rdm1, void* workspace, int32_t max_sweeps, double tol, int32_t* sweeps);
+
+/* Status word of an SCF cycle (no counterpart in the reference, whose checks are host
+ * asserts: charge conservation and NaN-free DIIS, grad_dft/evaluate.py:212-217, 506-508, 1384).
+ * OR-ed into the device int *status, so a whole captured loop reports once:
+ *  GDX_STATUS_EIGH_NOT_CONVERGED  a Jacobi eigensolve reached max_sweeps (sweeps: device int per
+ *                                 spin from gdx_scf_step / gdx_eigh_batched, may be NULL)
+ *  GDX_STATUS_NONFINITE_ENERGY    *energy (device double, may be NULL) is NaN or infinite
+ *  GDX_STATUS_CHARGE              |tr(D_s S) - sum occ_s| > tol for a spin (rdm1 (nspin, n, n),
+ *                                 S (n, n), occ (nspin, n)) */
+enum { GDX_STATUS_EIGH_NOT_CONVERGED = 1, GDX_STATUS_NONFINITE_ENERGY = 2, GDX_STATUS_CHARGE = 4 };
+int gdx_scf_status(void* stream, int32_t nspin, int32_t n, const double* rdm1, const double* S, const double* occ,
+                   const int32_t* sweeps, int32_t max_sweeps, const double* energy, double tol, int32_t* status);
 
 /* x_s = B_s^-1 e_0 for nspin bordered DIIS matrices B (nspin, m1, m1), m1 <= 16:
  * JittableDiis.cdiis_minimize, grad_dft/evaluate.py:1198-1205. */

@@ -108,3 +108,14 @@ def test_densities_pass_invalidates_pending_backward(monkeypatch):
     plan.densities(mol.rdm1)
     with pytest.raises(RuntimeError, match="overwritten"):
         plan.backward(mol.rdm1, None, z(), z(2, n, n), None, gen)
+
+
+def test_status_word_decoding():
+    """raise_on_status names every flag of the device status word and is silent for 0."""
+    import torch
+    from graddft_amd import _lib
+    from graddft_amd.eigenproblem import raise_on_status
+    raise_on_status(torch.zeros(1, dtype=torch.int32), "loop")
+    with pytest.raises(_lib.GdxError) as e:
+        raise_on_status(torch.tensor([5], dtype=torch.int32), "loop")
+    assert "did not converge" in str(e.value) and "tr(D S)" in str(e.value) and "finite" not in str(e.value)

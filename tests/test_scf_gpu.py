@@ -304,3 +304,34 @@ def test_batched_loop_with_shared_cache_matches_eager():
         for o, r in zip(outs, [ref1, ref2, ref1]):
             assert abs(o.energy.item() - r.energy.item()) <= 1e-12, (o.energy.item(), r.energy.item())
             assert torch.allclose(o.rdm1, r.rdm1, rtol=0, atol=1e-11)
+
+
+def test_scf_status_word_flags_and_loop_raises(monkeypatch):
+    """gdx_scf_status sets one bit per failed check (eigensolver at max_sweeps, non-finite energy,
+    tr(D S) != occupation count); diff_scf_loop raises when an eigensolve is cut short by a tiny
+    sweep limit set from the host (the kernels still finish normally)."""
+    _say("case: scf status word (direct calls) and a loop with max_sweeps = 1")
+    from graddft_amd import B3LYP, _lib, diff_scf_loop, eigenproblem
+    from graddft_amd.eigenproblem import scf_status
+    dev = "cuda"
+    n = 5
+    rng = np.random.default_rng(3)
+    C = np.linalg.qr(rng.normal(size=(n, n)))[0]
+    occ = np.zeros((2, n))
+    occ[:, :2] = 1.0
+    D = np.einsum("ij,sj,kj->sik", C, occ, C)
+    t = lambda a, dt=torch.float64: torch.as_tensor(a, dtype=dt, device=dev)  # noqa: E731
+    S = t(np.eye(n))
+    cases = [(D, occ, [1, 2], 0.5, 0),
+             (D, occ, [1, eigenproblem.JACOBI_MAX_SWEEPS], 0.5, _lib.GDX_STATUS_EIGH_NOT_CONVERGED),
+             (D, occ, [1, 2], float("nan"), _lib.GDX_STATUS_NONFINITE_ENERGY),
+             (1.1 * D, occ, [1, 2], 0.5, _lib.GDX_STATUS_CHARGE)]
+    for Dm, oc, sw, e, want in cases:
+        status = torch.zeros(1, dtype=torch.int32, device=dev)
+        scf_status(t(Dm), S, t(oc), t(sw, torch.int32), t(e), status)
+        assert int(status.item()) == want, (sw, e, int(status.item()), want)
+    mol = _water()
+    diff_scf_loop(B3LYP, cycles=3)(None, mol)                      # converged: no flag
+    monkeypatch.setattr(eigenproblem, "JACOBI_MAX_SWEEPS", 1)
+    with pytest.raises(_lib.GdxError, match="did not converge"):
+        diff_scf_loop(B3LYP, cycles=3)(None, mol)
