@@ -16,6 +16,7 @@ grid.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -40,6 +41,7 @@ def parse():
     ap.add_argument("--no-train", action="store_true", help="skip the training-step measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the concurrent small-molecule batch")
     ap.add_argument("--no-train-batch", action="store_true", help="skip the batched training step")
+    ap.add_argument("--no-eri", action="store_true", help="skip the device ERI generation measurement")
     ap.add_argument("--batch-size", type=int, default=8, help="alkanes in the batched training step")
     ap.add_argument("--batch-max-nc", type=int, default=4, help="largest n_C of the batched training step")
     ap.add_argument("--no-graph", action="store_true", help="time the eager step instead of graph replays")
@@ -148,6 +150,60 @@ def scf_bench(args):
     if args.precision != "fp64":
         out["abs_dE_vs_fp64_Ha"] = abs(out[args.precision]["energy"] - out["fp64"]["energy"])
     return out
+
+
+def eri_bench(args):
+    """Packed ERI generation for naphthalene (seeded geometry) def2-SVP, nao 180, ~1.3e8
+    elements: the device kernels (gdx_eri_packed, HIP events, mean of 3 calls after a warm-up)
+    and the plan build on the host, beside the host integral library (OpenMP, one call) on the
+    same basis; max |device - host| over all elements."""
+    import torch
+    from graddft_amd import _lib
+    from graddft_amd.interface import geometry
+    from graddft_amd.interface.basis import build_basis, pack_shells
+    from graddft_amd.interface.integrals import int2e_packed, npair
+    sym, xyz = geometry.naphthalene(np.random.default_rng(args.seed))
+    bs = build_basis(sym, xyz, "def2-svp")
+    lib = _lib.load()
+    pk = pack_shells(bs)
+    ptr = lambda a: a.ctypes.data  # noqa: E731
+    handle = ctypes.c_void_p(0)
+    t0 = time.perf_counter()
+    _lib.check(lib.gdx_eri_plan_create(len(bs.shells), ptr(pk["shell_i32"]), ptr(pk["shell_f64"]),
+                                       ptr(pk["prim_f64"]), ptr(pk["tmat_f64"]), bs.nao, 0.0, ctypes.byref(handle)),
+               "gdx_eri_plan_create")
+    t_plan = time.perf_counter() - t0
+    P = npair(bs.nao)
+    nel = P * (P + 1) // 2
+    eri = torch.empty(nel, dtype=torch.float64, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    try:
+        _lib.check(lib.gdx_eri_packed(st, handle, eri.data_ptr()), "gdx_eri_packed")
+        torch.cuda.synchronize()
+        reps = 3
+        with _lib.KernelTimer() as kt:
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                _lib.check(lib.gdx_eri_packed(st, handle, eri.data_ptr()), "gdx_eri_packed")
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+        nq = ctypes.c_int64(0)
+        ne = ctypes.c_int64(0)
+        lib.gdx_eri_plan_info(handle, ctypes.byref(nq), ctypes.byref(ne))
+    finally:
+        lib.gdx_eri_plan_destroy(handle)
+    k_ms = kt.result["eri"][0] / reps
+    h0 = time.perf_counter()
+    host = int2e_packed(bs)
+    t_host = time.perf_counter() - h0
+    err = float(np.abs(eri.cpu().numpy() - host).max())
+    return {"workload": f"naphthalene def2-SVP nao={bs.nao}, {len(bs.shells)} shells, packed ERI {nel} elements",
+            "elements": nel, "shell_quartets": int(nq.value), "primitive_pairs": int(ne.value),
+            "device_ms": round((t1 - t0) / reps * 1e3, 2), "device_kernel_ms": round(k_ms, 2),
+            "device_elements_per_s": round(nel / (k_ms / 1e3), 1), "plan_ms": round(t_plan * 1e3, 1),
+            "host_library_ms": round(t_host * 1e3, 1), "host_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
+            "speedup_vs_host": round(t_host * 1e3 / ((t1 - t0) / reps * 1e3 + t_plan * 1e3), 2),
+            "max_abs_diff_vs_host": err}
 
 
 def batch_bench(args):
@@ -454,6 +510,7 @@ def main():
     train = None if args.no_train else train_bench(args)
     batch = None if (args.no_batch or args.no_scf) else batch_bench(args)
     train_batch = None if (args.no_train_batch or args.no_train) else train_batch_bench(args)
+    eri = None if args.no_eri else eri_bench(args)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -495,6 +552,7 @@ def main():
         "train_step": train,
         "batch": batch,
         "train_batch": train_batch,
+        "eri": eri,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "gdx_common.h"
+#include "gdx_md.h"
 #include "gdx_profile.h"
 
 namespace gdx {
@@ -46,107 +47,9 @@ constexpr int HF_LMAX = 3;                 // f shells
 constexpr int HF_LTOT = 2 * HF_LMAX;       // la + lb
 constexpr int HF_MAXF = 10;                // AO components of a shell (Cartesian f)
 constexpr int HF_NSPIN_MAX = 2;
-// Boys function table: F_n(T_k) for T_k = k h, k < BOYS_ROWS, n <= BOYS_NCOL - 1; Taylor
-// expansion about the nearest T_k with BOYS_TAYLOR terms (|T - T_k| <= h / 2: remainder below
-// (h/2)^7 / 7! ~ 1.5e-13 F_{n+7} <= 1e-14); T >= BOYS_TMAX: the asymptotic F_0 = sqrt(pi/T)/2
-// (erfc(sqrt(40)) < 1e-18) and upward recursion.
-constexpr int BOYS_TAYLOR = 7;
-constexpr int BOYS_NCOL = HF_LTOT + BOYS_TAYLOR;  // orders 0..12
-constexpr double BOYS_H = 0.1;
-constexpr int BOYS_ROWS = 400;
-constexpr double BOYS_TMAX = 39.9;
-constexpr double HF_PI = 3.14159265358979323846;
-
-__host__ __device__ constexpr int n_tuv(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
-// flat index of (t, u, v): shells of k = t + u + v in increasing k; inside a shell t descending,
-// then u descending
-__host__ __device__ constexpr int tuv_index(int t, int u, int v) {
-  return (t + u + v) * (t + u + v + 1) * (t + u + v + 2) / 6 + (u + v) * (u + v + 1) / 2 + v;
-}
-
-// F_n(T_k) for every table row: series for the highest order, then downward recursion
-// F_{n-1} = (2 T F_n + e^-T) / (2n - 1) (stable).  One thread per row.
-__global__ void boys_table_kernel(double* __restrict__ tab) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= BOYS_ROWS) return;
-  const double T = k * BOYS_H;
-  const int nmax = BOYS_NCOL - 1;
-  // F_n(T) = e^-T sum_i (2T)^i / ((2n+1)(2n+3)...(2n+2i+1))
-  double term = 1.0 / (2 * nmax + 1), sum = term;
-  for (int i = 1; i < 2000; ++i) {
-    term *= 2.0 * T / (2 * nmax + 2 * i + 1);
-    sum += term;
-    if (term < 1e-18 * sum) break;
-  }
-  const double e = exp(-T);
-  double F = e * sum;
-  tab[k * BOYS_NCOL + nmax] = F;
-  for (int n = nmax; n > 0; --n) {
-    F = (2.0 * T * F + e) / (2 * n - 1);
-    tab[k * BOYS_NCOL + n - 1] = F;
-  }
-}
-
-template <int L>
-__device__ __forceinline__ void boys_eval(double T, const double* __restrict__ tab, double (&F)[L + 1]) {
-  if (T < BOYS_TMAX) {
-    const int k = (int)(T * (1.0 / BOYS_H) + 0.5);
-    const double d = k * BOYS_H - T;  // dF_n/dT = -F_{n+1}: F_n(T) = sum_j F_{n+j}(T_k) d^j / j!
-    const double* row = tab + k * BOYS_NCOL + L;
-    double f = row[BOYS_TAYLOR - 1];
-#pragma unroll
-    for (int j = BOYS_TAYLOR - 2; j >= 0; --j) f = row[j] + f * d / (j + 1);
-    F[L] = f;
-    const double e = exp(-T);
-#pragma unroll
-    for (int n = L; n > 0; --n) F[n - 1] = (2.0 * T * F[n] + e) / (2 * n - 1);
-  } else {
-    F[0] = 0.5 * sqrt(HF_PI / T);
-    const double e = exp(-T), i2t = 0.5 / T;
-#pragma unroll
-    for (int n = 0; n < L; ++n) F[n + 1] = ((2 * n + 1) * F[n] - e) * i2t;
-  }
-}
-
-// Hermite Coulomb integrals R_tuv = R^{(0)}_tuv for t + u + v <= L, in place over decreasing
-// order n:  R^{(n)}_000 = (-2p)^n F_n,  R^{(n)}_{t+1,u,v} = t R^{(n+1)}_{t-1,u,v} + X R^{(n+1)}_{t,u,v}
-// (likewise u, v).  Entries of sum k at level n read level n+1 entries of sum k-1 and k-2, which
-// are still in place when a level is filled in decreasing k.
-template <int L>
-__device__ __forceinline__ void hermite_coulomb(double p, double X, double Y, double Z, const double (&F)[L + 1],
-                                                double (&R)[n_tuv(L)]) {
-  double m2p[L + 1];
-  m2p[0] = 1.0;
-#pragma unroll
-  for (int n = 1; n <= L; ++n) m2p[n] = m2p[n - 1] * (-2.0 * p);
-  R[0] = m2p[L] * F[L];
-#pragma unroll
-  for (int n = L - 1; n >= 0; --n) {
-#pragma unroll
-    for (int k = L - n; k >= 1; --k) {
-#pragma unroll
-      for (int t = k; t >= 0; --t) {
-#pragma unroll
-        for (int u = k - t; u >= 0; --u) {
-          const int v = k - t - u;
-          double val;
-          if (t > 0) {
-            val = X * R[tuv_index(t - 1, u, v)];
-            if (t > 1) val += (t - 1) * R[tuv_index(t - 2, u, v)];
-          } else if (u > 0) {
-            val = Y * R[tuv_index(t, u - 1, v)];
-            if (u > 1) val += (u - 1) * R[tuv_index(t, u - 2, v)];
-          } else {
-            val = Z * R[tuv_index(t, u, v - 1)];
-            if (v > 1) val += (v - 1) * R[tuv_index(t, u, v - 2)];
-          }
-          R[tuv_index(t, u, v)] = val;
-        }
-      }
-    }
-    R[0] = m2p[n] * F[n];
-  }
-}
+using md::n_tuv;
+using md::tuv_index;
+constexpr int BOYS_NCOL = HF_LTOT + md::BOYS_TAYLOR;  // orders 0..12
 
 // One launch per L.  grid.x: 256-point tiles; grid.y: output shells B that pair with some shell
 // A at la + lb = L.  bl[y] = {B ao offset, B components, first entry, end entry}.  Entry e:
@@ -181,9 +84,9 @@ __global__ __launch_bounds__(256) void hf_chi_kernel(int npts, int npts_pad, con
       scale = sqrt(o2 / (p + o2));
     }
     double F[L + 1];
-    boys_eval<L>(pe * (X * X + Y * Y + Z * Z), boys, F);
+    md::boys_eval<L, BOYS_NCOL>(pe * (X * X + Y * Y + Z * Z), boys, F);
     double R[NT];
-    hermite_coulomb<L>(pe, X, Y, Z, F, R);
+    md::hermite_coulomb<L>(pe, X, Y, Z, F, R);
 #pragma unroll
     for (int q = 0; q < NT; ++q) R[q] *= scale;
     const int aoff = ei[2 * e], nfA = ei[2 * e + 1];
@@ -344,53 +247,6 @@ __global__ void hf_kmat_reduce_kernel(const double* __restrict__ part, int nchun
 inline unsigned nblk256(long n) { return (unsigned)((n + 255) / 256); }
 
 // ---------------------------------------------------------------- host: plan ----------
-inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
-
-struct Shell {
-  int l, nprim, poff, aoff, nfunc, toff;
-  double c[3];
-};
-
-// Hermite expansion coefficients of one Cartesian direction: E[i][j][t], i <= la, j <= lb.
-struct Herm {
-  double E[HF_LMAX + 1][HF_LMAX + 1][HF_LTOT + 1];
-};
-
-void hermite_1d(int la, int lb, double a, double b, double XAB, Herm& H) {
-  std::memset(&H, 0, sizeof(H));
-  const double p = a + b, mu = a * b / p;
-  const double XPA = -b * XAB / p, XPB = a * XAB / p, o2p = 0.5 / p;
-  H.E[0][0][0] = std::exp(-mu * XAB * XAB);
-  for (int i = 0; i <= la; ++i)
-    for (int j = 0; j <= lb; ++j) {
-      if (i == 0 && j == 0) continue;
-      for (int t = 0; t <= i + j; ++t) {
-        double v;
-        if (i > 0) {  // E^{i,j}_t = E^{i-1,j}_{t-1}/(2p) + X_PA E^{i-1,j}_t + (t+1) E^{i-1,j}_{t+1}
-          v = XPA * H.E[i - 1][j][t];
-          if (t > 0) v += o2p * H.E[i - 1][j][t - 1];
-          if (t + 1 <= i - 1 + j) v += (t + 1) * H.E[i - 1][j][t + 1];
-        } else {
-          v = XPB * H.E[i][j - 1][t];
-          if (t > 0) v += o2p * H.E[i][j - 1][t - 1];
-          if (t + 1 <= i + j - 1) v += (t + 1) * H.E[i][j - 1][t + 1];
-        }
-        H.E[i][j][t] = v;
-      }
-    }
-}
-
-void cart_exponents(int l, int (*e)[3]) {
-  int k = 0;
-  for (int x = l; x >= 0; --x)
-    for (int y = l - x; y >= 0; --y) {
-      e[k][0] = x;
-      e[k][1] = y;
-      e[k][2] = l - x - y;
-      ++k;
-    }
-}
-
 struct HFPlan {
   int nshell = 0, nao = 0;
   int nb[HF_LTOT + 1] = {};   // output shells per L
@@ -463,16 +319,10 @@ int gdx_hf_plan_create(int32_t nshell, const int32_t* shell_i32, const double* s
               "gdx_hf_plan_create: bad arguments");
   GDX_REQUIRE(screen >= 0.0, "gdx_hf_plan_create: negative screening threshold");
   *plan_out = nullptr;
-  std::vector<Shell> sh(nshell);
+  std::vector<md::Shell> sh(nshell);
   for (int s = 0; s < nshell; ++s) {
-    Shell& x = sh[s];
-    x.l = shell_i32[6 * s];
-    x.nprim = shell_i32[6 * s + 1];
-    x.poff = shell_i32[6 * s + 2];
-    x.aoff = shell_i32[6 * s + 3];
-    x.nfunc = shell_i32[6 * s + 4];
-    x.toff = shell_i32[6 * s + 5];
-    for (int k = 0; k < 3; ++k) x.c[k] = shell_f64[3 * s + k];
+    sh[s] = md::shell_view(shell_i32, shell_f64, s);
+    const md::Shell& x = sh[s];
     GDX_REQUIRE(x.l >= 0 && x.l <= HF_LMAX && x.nfunc > 0 && x.nfunc <= HF_MAXF && x.aoff + x.nfunc <= nao,
                 "gdx_hf_plan_create: shell with l > 3 or components outside the basis");
   }
@@ -487,15 +337,11 @@ int gdx_hf_plan_create(int32_t nshell, const int32_t* shell_i32, const double* s
   std::vector<int4> bl[HF_LTOT + 1];
   for (int L = 0; L <= HF_LTOT; ++L) {
     for (int sb = 0; sb < nshell; ++sb) {
-      const Shell& B = sh[sb];
+      const md::Shell& B = sh[sb];
       const int first = (int)ents.size();
       for (int sa = 0; sa < nshell; ++sa) {
-        const Shell& A = sh[sa];
+        const md::Shell& A = sh[sa];
         if (A.l + B.l != L) continue;
-        const int nA = ncart(A.l), nB = ncart(B.l), NT = n_tuv(L);
-        int ea[HF_MAXF][3], eb[HF_MAXF][3];
-        cart_exponents(A.l, ea);
-        cart_exponents(B.l, eb);
         const double AB[3] = {A.c[0] - B.c[0], A.c[1] - B.c[1], A.c[2] - B.c[2]};
         for (int i = 0; i < A.nprim; ++i) {
           const double a = prim_f64[2 * (A.poff + i)], ca = prim_f64[2 * (A.poff + i) + 1];
@@ -503,46 +349,12 @@ int gdx_hf_plan_create(int32_t nshell, const int32_t* shell_i32, const double* s
             const double b = prim_f64[2 * (B.poff + j)], cb = prim_f64[2 * (B.poff + j) + 1];
             const double p = a + b;
             const double KAB = std::exp(-a * b / p * (AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2]));
-            const double pref = 2.0 * HF_PI / p * ca * cb;
-            if (std::fabs(pref) * KAB < screen) continue;
-            Herm H[3];
-            for (int d = 0; d < 3; ++d) hermite_1d(A.l, B.l, a, b, AB[d], H[d]);
-            // Cartesian block mc[ia][jb][q], then M[fa][fb][q] = pref sum TA[fa][ia] TB[fb][jb] mc
-            std::vector<double> mc((size_t)nA * nB * NT, 0.0);
-            for (int ia = 0; ia < nA; ++ia)
-              for (int jb = 0; jb < nB; ++jb) {
-                const int tx = ea[ia][0] + eb[jb][0], ty = ea[ia][1] + eb[jb][1], tz = ea[ia][2] + eb[jb][2];
-                for (int t = 0; t <= tx; ++t)
-                  for (int uu = 0; uu <= ty; ++uu)
-                    for (int v = 0; v <= tz; ++v)
-                      mc[((size_t)ia * nB + jb) * NT + tuv_index(t, uu, v)] =
-                          H[0].E[ea[ia][0]][eb[jb][0]][t] * H[1].E[ea[ia][1]][eb[jb][1]][uu] *
-                          H[2].E[ea[ia][2]][eb[jb][2]][v];
-              }
-            const double* TA = tmat_f64 + A.toff;
-            const double* TB = tmat_f64 + B.toff;
+            if (std::fabs(2.0 * md::PI / p * ca * cb) * KAB < screen) continue;
             Ent en;
-            en.p = p;
-            for (int d = 0; d < 3; ++d) en.P[d] = (a * A.c[d] + b * B.c[d]) / p;
             en.aoff = A.aoff;
             en.nfA = A.nfunc;
             en.moff = (long)Mall.size();
-            Mall.resize(Mall.size() + (size_t)A.nfunc * B.nfunc * NT, 0.0);
-            double* Mo = Mall.data() + en.moff;
-            for (int fa = 0; fa < A.nfunc; ++fa)
-              for (int fb = 0; fb < B.nfunc; ++fb)
-                for (int ia = 0; ia < nA; ++ia) {
-                  const double ta = TA[fa * nA + ia];
-                  if (ta == 0.0) continue;
-                  for (int jb = 0; jb < nB; ++jb) {
-                    const double tb = TB[fb * nB + jb];
-                    if (tb == 0.0) continue;
-                    const double w = pref * ta * tb;
-                    const double* src = mc.data() + ((size_t)ia * nB + jb) * NT;
-                    double* dst = Mo + ((size_t)fa * B.nfunc + fb) * NT;
-                    for (int q = 0; q < NT; ++q) dst[q] += w * src[q];
-                  }
-                }
+            md::pair_matrix(A, B, a, ca, b, cb, tmat_f64, Mall, en.p, en.P);
             ents.push_back(en);
           }
         }
@@ -590,9 +402,10 @@ int gdx_hf_plan_create(int32_t nshell, const int32_t* shell_i32, const double* s
     if (P->nb[L]) HF_UPLOAD(P->d_bl[L], bl[L]);
   }
 #undef HF_UPLOAD
-  e = hipMalloc((void**)&P->d_boys, sizeof(double) * BOYS_ROWS * BOYS_NCOL);
+  e = hipMalloc((void**)&P->d_boys, sizeof(double) * md::BOYS_ROWS * BOYS_NCOL);
   if (e != hipSuccess) return fail(e, "hipMalloc");
-  hipLaunchKernelGGL(boys_table_kernel, dim3((BOYS_ROWS + 255) / 256), dim3(256), 0, (hipStream_t)0, P->d_boys);
+  hipLaunchKernelGGL(md::boys_table_kernel<BOYS_NCOL>, dim3((md::BOYS_ROWS + 255) / 256), dim3(256), 0,
+                     (hipStream_t)0, P->d_boys);
   e = hipGetLastError();
   if (e != hipSuccess) return fail(e, "boys_table_kernel");
   e = hipStreamSynchronize((hipStream_t)0);

@@ -22,7 +22,7 @@ from .. import _lib
 from ..molecule import Grid, Molecule
 from .basis import atomic_numbers, build_basis, nuclear_repulsion, pack_shells
 from .grids import molecular_grid
-from .integrals import int1e, int2e_packed
+from .integrals import int1e, int2e_packed, int2e_packed_device
 
 
 def pad_to(x: int, m: int) -> int:
@@ -57,10 +57,12 @@ def molecule_from_geometry(symbols: Sequence[str], coords_bohr: np.ndarray, basi
                            grid_level: int = 2, charge: int = 0, with_eri: bool = False,
                            device: str = "cuda", rdm1: Optional[np.ndarray] = None,
                            name: str = "", npts_align: int = 128,
-                           omegas: Optional[Sequence[float]] = None) -> Molecule:
+                           omegas: Optional[Sequence[float]] = None, eri_on: str = "device") -> Molecule:
     """Closed-shell molecule with a core-Hamiltonian (or given) density, AOs on device.  With
     `omegas` the chi tensor of that density is built on the device, as ``molecule_from_pyscf``
-    does when given omegas (pyscf.py:152-160); B3LYP's exact exchange then uses it."""
+    does when given omegas (pyscf.py:152-160); B3LYP's exact exchange then uses it.
+    with_eri: the packed ERI, computed on the device (eri_on="device", shells up to d) or by the
+    host integral library (eri_on="host", and for bases with f shells)."""
     coords_bohr = np.asarray(coords_bohr, dtype=np.float64)
     Z = atomic_numbers(symbols)
     bs = build_basis(symbols, coords_bohr, basis)
@@ -83,7 +85,12 @@ def molecule_from_geometry(symbols: Sequence[str], coords_bohr: np.ndarray, basi
     wpad[:N] = gw
     grid = Grid(coords=dev(gc), weights=dev(gw), weights_pad=dev(wpad))
     ao = eval_ao_device(bs, gc, npts_pad, nao_pad, device)
-    eri = dev(int2e_packed(bs)) if with_eri else None
+    eri = None
+    if with_eri:
+        if eri_on == "device":
+            eri = int2e_packed_device(bs, device)
+        if eri is None:
+            eri = dev(int2e_packed(bs))
     mol = Molecule(
         grid=grid,
         atom_index=torch.as_tensor(Z, device=device),

@@ -2,7 +2,8 @@
 
 Stands in for PySCF's ``mf.get_ovlp`` / ``mf.get_hcore`` / ``mol.intor('int2e')``
 (``grad_dft/interface/pyscf.py:797, 828-829``).  The library is built from
-``graddft_amd/csrc/gdx_ints.cpp`` by :mod:`graddft_amd.build_ext`.
+``graddft_amd/csrc/gdx_ints.cpp`` by :mod:`graddft_amd.build_ext`.  ``int2e_packed_device``
+computes the packed ERI on the GPU instead (bases with shells up to d).
 """
 from __future__ import annotations
 
@@ -65,6 +66,30 @@ def int2e_packed(bs: BasisSet) -> np.ndarray:
                                  _ptr(pk["prim_f64"]), _ptr(pk["tmat_f64"]), bs.nao, _ptr(eri))
     if rc != 0:
         raise RuntimeError(f"gdx_int2e_packed failed ({rc})")
+    return eri
+
+
+def int2e_packed_device(bs: BasisSet, device: str = "cuda", screen: float = 0.0):
+    """The packed ERI computed on the device (gdx_eri_plan_create + gdx_eri_packed) as a torch
+    tensor, or None when the basis has f shells (the device kernels take shells up to d)."""
+    import torch
+    from .. import _lib
+    lib = _lib.load()
+    pk = pack_shells(bs)
+    handle = ctypes.c_void_p(0)
+    rc = lib.gdx_eri_plan_create(len(bs.shells), _ptr(pk["shell_i32"]), _ptr(pk["shell_f64"]), _ptr(pk["prim_f64"]),
+                                 _ptr(pk["tmat_f64"]), bs.nao, float(screen), ctypes.byref(handle))
+    if rc == -3:   # GDX_ERR_UNSUPPORTED: shells beyond d
+        return None
+    _lib.check(rc, "gdx_eri_plan_create")
+    try:
+        P = npair(bs.nao)
+        eri = torch.empty(P * (P + 1) // 2, dtype=torch.float64, device=device)
+        st = torch.cuda.current_stream(eri.device).cuda_stream
+        _lib.check(lib.gdx_eri_packed(st, handle, eri.data_ptr()), "gdx_eri_packed")
+        torch.cuda.current_stream(eri.device).synchronize()
+    finally:
+        lib.gdx_eri_plan_destroy(handle)
     return eri
 
 

@@ -253,6 +253,20 @@ int gdx_diis_solve(void* stream, int32_t nspin, int32_t m1, const double* B, dou
 int gdx_diis_coefficients(void* stream, int32_t nspin, int32_t m, int64_t n2, const double* errs, int32_t cycle,
                           double* B, double* x);
 
+/* ---- Electron-repulsion integrals on the device ---------------------------------------
+ * The packed ERI of a basis (HOST shell arrays in the gdx_ao_eval packing, shells up to d):
+ * mol.intor("int2e") as called by calc_eri_with_pyscf, grad_dft/interface/pyscf.py:746-803,
+ * in the 8-fold packed layout of the SCF entry points (P(P+1)/2 doubles, P = n(n+1)/2).
+ * The plan (primitive-pair expansions and the shell-quartet lists, grouped by angular momenta)
+ * is built and uploaded once; gdx_eri_packed writes every element (no clearing needed) and is
+ * reproducible.  Primitive pairs with (2 pi / p)|c_a c_b| exp(-ab/p |A-B|^2) < screen are
+ * dropped (0 keeps all).  GDX_ERR_UNSUPPORTED for a basis with f shells. */
+int gdx_eri_plan_create(int32_t nshell, const int32_t* shell_i32, const double* shell_f64, const double* prim_f64,
+                        const double* tmat_f64, int32_t nao, double screen, void** plan);
+int gdx_eri_plan_destroy(void* plan);
+int gdx_eri_plan_info(const void* plan, int64_t* nquartets, int64_t* nentries);
+int gdx_eri_packed(void* stream, const void* plan, double* eri);
+
 /* ---- Semi-numerical exact exchange ----------------------------------------------------
  * nu_ab(r) = int dr' phi_a(r') f(|r - r'|) phi_b(r') at grid points r (f = 1/|r-r'|, or
  * erf(omega |r-r'|)/|r-r'| for omega > 0): PySCF's int1e_grids_sph as called by
