@@ -10,11 +10,13 @@
 //
 // Packed ERI order (shared with the host integral library): pair index I = i(i+1)/2 + j
 // (i >= j), element index I(I+1)/2 + K (I >= K).
+#include <algorithm>
 #include <mutex>
 #include <string>
 
 #include "gdx_common.h"
 #include "gdx_profile.h"
+#include "gdx_tridiag.h"
 
 namespace gdx {
 namespace {
@@ -747,6 +749,138 @@ __global__ __launch_bounds__(NTH) void scf_step_kernel(int n, int m, int nspin, 
   if (tid == 0 && sweeps_out) sweeps_out[s] = sweep;
 }
 
+// ------------------------------------------------------------- tridiagonal path ------
+// Where the Jacobi buffers do not fit LDS (n > ~70) and n <= tri::NMAX the eigensolver is the
+// Householder / bisection / inverse-iteration sequence of gdx_tridiag.h, and the SCF step's
+// n^3 products run as many-workgroup tiled GEMMs instead of inside one workgroup.
+
+// C[b] = alpha op(A[b]) diag(S[b]) op(B[b]) + beta C0[b] for n x n row-major matrices (op =
+// transpose when TA / TB; S and C0 may be NULL; C may alias C0 but not A or B); batch strides
+// sA, sB, sS, sC (0: shared by the batch; C0 uses sC).  32 x 32 output tile per 256-thread
+// workgroup, 2 x 2 outputs per thread, K staged through LDS 32 at a time.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void small_gemm_kernel(int n, const double* __restrict__ A, long sA,
+                                                         const double* __restrict__ B, long sB,
+                                                         const double* __restrict__ S, long sS, double alpha,
+                                                         const double* C0, double beta, double* C, long sC) {
+  __shared__ double As[32][33], Bs[32][33];
+  const int b = blockIdx.z, tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+  A += (size_t)b * sA;
+  B += (size_t)b * sB;
+  C += (size_t)b * sC;
+  if (C0) C0 += (size_t)b * sC;
+  if (S) S += (size_t)b * sS;
+  double acc00 = 0.0, acc01 = 0.0, acc10 = 0.0, acc11 = 0.0;
+  for (int k0 = 0; k0 < n; k0 += 32) {
+    for (int q = tid; q < 1024; q += 256) {
+      const int u = q >> 5, v = q & 31;   // v runs along contiguous memory
+      if (!TA) {
+        const int r = r0 + u, kk = k0 + v;
+        As[u][v] = (r < n && kk < n) ? A[(size_t)r * n + kk] : 0.0;
+      } else {
+        const int kk = k0 + u, r = r0 + v;
+        As[v][u] = (r < n && kk < n) ? A[(size_t)kk * n + r] : 0.0;
+      }
+      if (!TB) {
+        const int kk = k0 + u, c = c0 + v;
+        Bs[u][v] = (kk < n && c < n) ? B[(size_t)kk * n + c] * (S ? S[kk] : 1.0) : 0.0;
+      } else {
+        const int c = c0 + u, kk = k0 + v;
+        Bs[v][u] = (kk < n && c < n) ? B[(size_t)c * n + kk] * (S ? S[kk] : 1.0) : 0.0;
+      }
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int kk = 0; kk < 32; ++kk) {
+      const double a0 = As[ty][kk], a1 = As[ty + 16][kk], b0 = Bs[kk][tx], b1 = Bs[kk][tx + 16];
+      acc00 += a0 * b0;
+      acc01 += a0 * b1;
+      acc10 += a1 * b0;
+      acc11 += a1 * b1;
+    }
+    __syncthreads();
+  }
+  const int ra = r0 + ty, rb = r0 + ty + 16, ca = c0 + tx, cb = c0 + tx + 16;
+  auto out = [&](int r, int c, double acc) {
+    if (r >= n || c >= n) return;
+    const size_t o = (size_t)r * n + c;
+    C[o] = C0 ? alpha * acc + beta * C0[o] : alpha * acc;
+  };
+  out(ra, ca, acc00);
+  out(ra, cb, acc01);
+  out(rb, ca, acc10);
+  out(rb, cb, acc11);
+}
+
+template <bool TA, bool TB>
+static void small_gemm(hipStream_t st, int nb, int n, const double* A, long sA, const double* B, long sB,
+                       const double* S, long sS, double* C, long sC, double alpha = 1.0,
+                       const double* C0 = nullptr, double beta = 0.0) {
+  auto kern = small_gemm_kernel<TA, TB>;
+  const unsigned t = (unsigned)((n + 31) / 32);
+  GDX_LAUNCH("scf_gemm", kern, dim3(t, t, nb), dim3(256), 0, st, n, A, sA, B, sB, S, sS, alpha, C0, beta, C, sC);
+}
+
+// F = sum_i x_{1+i} F_i per spin, summed in history order (as scf_step_kernel does).
+__global__ void fock_extrap_kernel(int n, int m, int nspin, const double* __restrict__ focks,
+                                   const double* __restrict__ x, double* __restrict__ F) {
+  const size_t nn = (size_t)n * n;
+  const size_t e = (size_t)blockIdx.x * NT + threadIdx.x;
+  const int s = blockIdx.y;
+  if (e >= nn) return;
+  const double* xs = x + (size_t)s * (m + 1) + 1;
+  double f = 0.0;
+  for (int i = 0; i < m; ++i) f += xs[i] * focks[((size_t)i * nspin + s) * nn + e];
+  F[(size_t)s * nn + e] = f;
+}
+
+static bool use_tridiag(int n, size_t lds_need) { return lds_need > eig_smem_budget() && n <= tri::NMAX; }
+
+template <int RA, int RR>
+static void tri_reduce_launch(hipStream_t st, int nb, int n, const double* A, long sA, double* d, double* e,
+                              double* tau, double* Vr) {
+  auto kern = tri::tri_reduce_kernel<RA, RR>;
+  GDX_LAUNCH("tri_reduce", kern, dim3(nb), dim3(tri::RED_THREADS), 0, st, n, A, sA, d, e, tau, Vr);
+}
+
+// Eigen-decomposition of nb symmetric matrices A[b] (stride sA; symmetrised on load) into
+// ascending w (nb x n) and eigenvector columns Z (nb x n x n); ws: nb * tri::eig_ws_doubles(n).
+static int tri_eigh(hipStream_t st, int nb, int n, const double* A, long sA, double* w, double* Z, double* ws,
+                    int* sweeps) {
+  const size_t nn = (size_t)n * n;
+  double* Vr = ws;
+  double* X = Vr + nb * nn;
+  double* scr = X + nb * nn;
+  double* d = scr + 2 * nb * nn;
+  double* e = d + (size_t)nb * n;
+  double* tau = e + (size_t)nb * n;
+  double* tn = tau + (size_t)nb * n;
+  switch ((n + 31) / 32) {
+    case 1: tri_reduce_launch<1, 1>(st, nb, n, A, sA, d, e, tau, Vr); break;
+    case 2: tri_reduce_launch<2, 2>(st, nb, n, A, sA, d, e, tau, Vr); break;
+    case 3: tri_reduce_launch<3, 3>(st, nb, n, A, sA, d, e, tau, Vr); break;
+    case 4: tri_reduce_launch<4, 4>(st, nb, n, A, sA, d, e, tau, Vr); break;
+    case 5: tri_reduce_launch<5, 5>(st, nb, n, A, sA, d, e, tau, Vr); break;
+    case 6: tri_reduce_launch<6, 5>(st, nb, n, A, sA, d, e, tau, Vr); break;
+    case 7: tri_reduce_launch<7, 5>(st, nb, n, A, sA, d, e, tau, Vr); break;
+    default: return GDX_ERR_INVALID;
+  }
+  GDX_LAUNCH("tri_bisect", tri::tri_bisect_kernel, dim3((n + 3) / 4, nb), dim3(256), 0, st, n, d, e, w, tn, sweeps);
+  GDX_LAUNCH("tri_invit", tri::tri_invit_kernel, dim3((n + tri::INV_TPB - 1) / tri::INV_TPB, nb),
+             dim3(tri::INV_TPB), 0, st, n, d, e, w, tn, X);
+  GDX_LAUNCH("tri_orth", tri::tri_orth_kernel, dim3((n + 3) / 4, nb), dim3(256), 0, st, n, d, e, w, tn, X);
+  // first-order Loewdin polish of the vectors computed independently (orthogonality
+  // ~ eps ||T|| / gap between clusters): X <- X (I - (X^T X - I) / 2) = 1.5 X - 0.5 X (X^T X)
+  double* G = scr;
+  double* Xp = scr + nb * nn;
+  small_gemm<true, false>(st, nb, n, X, (long)nn, X, (long)nn, nullptr, 0, G, (long)nn);
+  small_gemm<false, false>(st, nb, n, X, (long)nn, G, (long)nn, nullptr, 0, Xp, (long)nn, -0.5, X, 1.5);
+  GDX_LAUNCH("tri_back", tri::tri_back_kernel, dim3((n + tri::BT_COLS - 1) / tri::BT_COLS, nb), dim3(256), 0, st, n,
+             Vr, tau, Xp, Z, (long)nn);
+  return GDX_OK;
+}
+
 // ------------------------------------------------------------- DIIS solve ----------
 // x = B^-1 e_0 for one bordered DIIS matrix B (m1 x m1, m1 <= DIIS_MAXM1) by Gaussian
 // elimination with partial pivoting (the first row of largest |pivot| wins), in one wave:
@@ -1029,7 +1163,9 @@ int gdx_cholesky_inv(void* stream, int32_t n, const double* S, double* L, double
 
 int gdx_eigh_workspace_size(int32_t nbatch, int32_t n, size_t* bytes) {
   GDX_REQUIRE(nbatch > 0 && n > 0 && bytes, "gdx_eigh_workspace_size: bad arguments");
-  *bytes = (size_t)nbatch * 3 * n * n * sizeof(double);
+  size_t per = (size_t)3 * n * n;
+  if (n <= tri::NMAX) per = std::max(per, tri::eig_ws_doubles(n));
+  *bytes = (size_t)nbatch * per * sizeof(double);
   return GDX_OK;
 }
 
@@ -1039,7 +1175,11 @@ int gdx_eigh_batched(void* stream, int32_t nbatch, int32_t n, const double* A, c
   GDX_REQUIRE(V0 == nullptr || (V0 != V && V0 != A), "gdx_eigh_batched: V0 must not alias A or V");
   hipStream_t st = (hipStream_t)stream;
   const size_t smem = (size_t)3 * n * n * sizeof(double);
-  if (smem <= eig_smem_budget()) {
+  if (use_tridiag(n, smem)) {   // V0 is not used: the result does not depend on a start
+    GDX_REQUIRE(workspace, "gdx_eigh_batched: workspace required when 3 n^2 doubles exceed the LDS budget");
+    const int rc = tri_eigh(st, nbatch, n, A, (long)n * n, w, V, (double*)workspace, sweeps);
+    if (rc != GDX_OK) return rc;
+  } else if (smem <= eig_smem_budget()) {
     auto kern = jacobi_eigh_kernel<true, EIG_THREADS_SMEM>;
     if (smem > EIG_SMEM_MAX)
       GDX_HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
@@ -1060,7 +1200,9 @@ int gdx_eigh_batched(void* stream, int32_t nbatch, int32_t n, const double* A, c
 
 int gdx_scf_step_workspace_size(int32_t nspin, int32_t n, size_t* bytes) {
   GDX_REQUIRE(nspin > 0 && n > 0 && bytes, "gdx_scf_step_workspace_size: bad arguments");
-  *bytes = (size_t)nspin * 4 * n * n * sizeof(double);
+  size_t per = (size_t)4 * n * n;
+  if (n <= tri::NMAX) per = std::max(per, (size_t)2 * n * n + tri::eig_ws_doubles(n));
+  *bytes = (size_t)nspin * per * sizeof(double);
   return GDX_OK;
 }
 
@@ -1073,7 +1215,22 @@ int gdx_scf_step(void* stream, int32_t nspin, int32_t n, int32_t m, const double
   GDX_REQUIRE(V0 == nullptr || (V0 != V && V0 != C && V0 != rdm1), "gdx_scf_step: V0 must not alias an output");
   hipStream_t st = (hipStream_t)stream;
   const size_t smem = (size_t)4 * n * n * sizeof(double);
-  if (smem <= eig_smem_budget()) {
+  if (use_tridiag(n, smem)) {
+    // F = sum x_i F_i; B = Linv F Linv^T; B = V diag(w) V^T; C = Linv^T V; D = C diag(occ) C^T
+    GDX_REQUIRE(workspace, "gdx_scf_step: workspace required when 4 n^2 doubles exceed the LDS budget");
+    const long nn = (long)n * n;
+    double* T1 = (double*)workspace;
+    double* B = T1 + nspin * nn;
+    double* ews = B + nspin * nn;
+    GDX_LAUNCH("fock_extrap", fock_extrap_kernel, dim3((unsigned)((nn + NT - 1) / NT), nspin), dim3(NT), 0, st, n, m,
+               nspin, focks, x, F);
+    small_gemm<false, false>(st, nspin, n, Linv, 0, F, nn, nullptr, 0, T1, nn);
+    small_gemm<false, true>(st, nspin, n, T1, nn, Linv, 0, nullptr, 0, B, nn);
+    const int rc = tri_eigh(st, nspin, n, B, nn, w, V, ews, sweeps);
+    if (rc != GDX_OK) return rc;
+    small_gemm<true, false>(st, nspin, n, Linv, 0, V, nn, nullptr, 0, C, nn);
+    small_gemm<false, true>(st, nspin, n, C, nn, C, nn, occ, n, rdm1, nn);
+  } else if (smem <= eig_smem_budget()) {
     auto kern = scf_step_kernel<true, EIG_THREADS_SMEM>;
     if (smem > EIG_SMEM_MAX)
       GDX_HIP_CHECK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
