@@ -30,6 +30,22 @@ PEAK = {"fp64": ("TFLOP/s", 78.6), "fp32": ("TFLOP/s", 157.3), "bf16x6": ("TFLOP
 DTYPE = {"fp64": "f64", "fp32": "f32", "bf16x6": "bf16x6"}
 
 
+def scf_roofline(npts, n, nsd, f_mlp, ms_per_iter, precision):
+    """ms per SCF iteration against the roofline: algorithmic work per iteration W = N (n_s 16 n^2
+    + 2 F_mlp) flops (density phi D and grad-phi D, V_xc, the MLP forward and its adjoint) at
+    the precision's peak (FP64 78.6 TF/s; FP32 MFMA 157.3 TF/s for fp32 and bf16x6, whose grid
+    GEMMs run in FP32), plus the packed ERI read once (8 n_pack bytes) at 8 TB/s."""
+    P = n * (n + 1) // 2
+    eri_bytes = 8 * (P * (P + 1) // 2)
+    flops = npts * (nsd * 16 * n * n + 2 * f_mlp)
+    peak = 78.6e12 if precision == "fp64" else 157.3e12
+    t_c, t_m = flops / peak, eri_bytes / 8e12
+    return {"flops_per_iter": int(flops), "eri_bytes_per_iter": int(eri_bytes), "peak_tflops": peak / 1e12,
+            "roofline_us": round((t_c + t_m) * 1e6, 2), "compute_us": round(t_c * 1e6, 2),
+            "hbm_us": round(t_m * 1e6, 2), "measured_us": round(ms_per_iter * 1e3, 2),
+            "frac": round((t_c + t_m) / (ms_per_iter / 1e3), 4)}
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -42,6 +58,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the concurrent small-molecule batch")
     ap.add_argument("--no-train-batch", action="store_true", help="skip the batched training step")
     ap.add_argument("--no-eri", action="store_true", help="skip the device ERI generation measurement")
+    ap.add_argument("--no-config4", action="store_true", help="skip the naphthalene dense-ERI SCF workload")
     ap.add_argument("--batch-size", type=int, default=8, help="alkanes in the batched training step")
     ap.add_argument("--batch-max-nc", type=int, default=4, help="largest n_C of the batched training step")
     ap.add_argument("--no-graph", action="store_true", help="time the eager step instead of graph replays")
@@ -140,7 +157,9 @@ def scf_bench(args):
             gbs = eri_bytes * ct[1] / (ct[0] / 1e3) / 1e9
             coul = {"eri_bytes": eri_bytes, "avg_launch_us": round(ct[0] / ct[1] * 1e3, 2), "GB/s": round(gbs, 1),
                     "frac_of_8TB/s": round(gbs / 8000.0, 4)}
-        out[prec] = {"ms_per_iter": round((t3 - t2) / reps / args.scf_cycles * 1e3, 4),
+        ms_it = (t3 - t2) / reps / args.scf_cycles * 1e3
+        out[prec] = {"ms_per_iter": round(ms_it, 4),
+                     "scf_roofline": scf_roofline(mol.grid_size, mol.nao, 1 if mol.restricted else 2, 0, ms_it, prec),
                      "coulomb_packed_eri": coul,
                      "ms_per_iter_eager": round((t1 - t0) / args.scf_cycles * 1e3, 4),
                      "energy": float(gres.energy),
@@ -204,6 +223,89 @@ def eri_bench(args):
             "host_library_ms": round(t_host * 1e3, 1), "host_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
             "speedup_vs_host": round(t_host * 1e3 / ((t1 - t0) / reps * 1e3 + t_plan * 1e3), 2),
             "max_abs_diff_vs_host": err}
+
+
+def config4_bench(args):
+    """Naphthalene (seeded geometry) def2-SVP, nao 180, level-3 grid, packed ERI generated on
+    the device (1.06 GB), B3LYP plus the width-256 neural correction (MLP 3x256): the 20-cycle
+    diff_scf_loop forward replayed as one HIP graph (ms per iteration, mean of 3 replays) in
+    FP64 and args.precision, with the eager loop's kernel breakdown, J's bandwidth on the packed
+    ERI and the iteration's roofline; then forward + reverse (energy and dE/dtheta, eager) in
+    both precisions with the energy and gradient differences."""
+    import torch
+    from graddft_amd import B3LYPNeuralCorrection, MLPSpec, _lib, diff_scf_loop
+    from graddft_amd.interface import geometry
+    from graddft_amd.interface.builder import molecule_from_geometry
+    cycles = 20
+    sym, xyz = geometry.naphthalene(np.random.default_rng(args.seed))
+    t0 = time.perf_counter()
+    mol = molecule_from_geometry(sym, xyz, "def2-svp", 3, with_eri=True, name="naphthalene")
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    spec = MLPSpec(nc=6, h=256, n_hidden=3, nd=4)
+    f = B3LYPNeuralCorrection(spec)
+    theta = f.init(seed=args.seed)
+    nsd = 1 if mol.restricted else 2
+    out = {"workload": f"naphthalene def2-SVP nao={mol.nao}, grid level 3 ({mol.grid_size} pts), packed ERI "
+                       f"{mol.rep_tensor.numel()} elements (device-generated), B3LYP + neural correction MLP 3x256, "
+                       f"diff_scf_loop {cycles} cycles",
+           "npts": mol.grid_size, "nao": mol.nao, "molecule_build_s": round(t_build, 2)}
+    P = mol.nao * (mol.nao + 1) // 2
+    eri_bytes = 8 * (P * (P + 1) // 2)
+    res = {}
+    for prec in (["fp64", args.precision] if args.precision != "fp64" else ["fp64"]):
+        loop = diff_scf_loop(f, cycles=cycles, precision=prec)
+        with torch.no_grad():
+            loop(theta, mol)
+            torch.cuda.synchronize()
+            with _lib.KernelTimer() as kt:
+                t0 = time.perf_counter()
+                r_e = loop(theta, mol)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+            gloop = diff_scf_loop(f, cycles=cycles, precision=prec, graph=True)
+            gloop(theta, mol)
+            gloop(theta, mol)
+            torch.cuda.synchronize()
+            reps = 3
+            t2 = time.perf_counter()
+            for _ in range(reps):
+                r_g = gloop(theta, mol)
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+        ms_it = (t3 - t2) / reps / cycles * 1e3
+        ct = kt.result.get("coulomb_tile")
+        coul = None
+        if ct is not None:
+            gbs = eri_bytes * ct[1] / (ct[0] / 1e3) / 1e9
+            coul = {"eri_bytes": eri_bytes, "avg_launch_us": round(ct[0] / ct[1] * 1e3, 2), "GB/s": round(gbs, 1),
+                    "frac_of_8TB/s": round(gbs / 8000.0, 4)}
+        # forward + reverse through the loop
+        th = theta.detach().clone().requires_grad_(True)
+        r = loop(th, mol)
+        torch.autograd.grad(r.energy, [th])
+        torch.cuda.synchronize()
+        th = theta.detach().clone().requires_grad_(True)
+        t4 = time.perf_counter()
+        r = loop(th, mol)
+        g, = torch.autograd.grad(r.energy, [th])
+        torch.cuda.synchronize()
+        t5 = time.perf_counter()
+        res[prec] = (float(r_g.energy), g.detach().cpu().numpy())
+        out[prec] = {"ms_per_iter": round(ms_it, 4), "ms_per_iter_eager": round((t1 - t0) / cycles * 1e3, 4),
+                     "scf_roofline": scf_roofline(mol.grid_size, mol.nao, nsd, spec.flops_per_point(), ms_it, prec),
+                     "coulomb_packed_eri": coul, "energy": float(r_g.energy),
+                     "graph_vs_eager_abs_dE": abs(float(r_g.energy) - float(r_e.energy)),
+                     "fwd_rev_ms_per_step": round((t5 - t4) * 1e3, 2),
+                     "kernels_ms_per_iter": {k: round(v[0] / cycles, 4) for k, v in
+                                             sorted(kt.result.items(), key=lambda kv: -kv[1][0])[:10]}}
+    if args.precision != "fp64":
+        e64, g64 = res["fp64"]
+        e1, g1 = res[args.precision]
+        out["abs_dE_vs_fp64_Ha"] = abs(e1 - e64)
+        out["grad_rel_err_vs_fp64"] = float(np.linalg.norm(g1 - g64) / np.linalg.norm(g64))
+        out["speedup_ms_per_iter"] = round(out["fp64"]["ms_per_iter"] / out[args.precision]["ms_per_iter"], 3)
+    return out
 
 
 def batch_bench(args):
@@ -511,6 +613,7 @@ def main():
     batch = None if (args.no_batch or args.no_scf) else batch_bench(args)
     train_batch = None if (args.no_train_batch or args.no_train) else train_batch_bench(args)
     eri = None if args.no_eri else eri_bench(args)
+    config4 = None if (args.no_config4 or args.no_scf) else config4_bench(args)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -553,6 +656,7 @@ def main():
         "batch": batch,
         "train_batch": train_batch,
         "eri": eri,
+        "config4": config4,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -57,6 +57,8 @@ SIGNATURES = {
     "gdx_diis_solve": (ctypes.c_int, [_P, _I, _I, _P, _P]),
     "gdx_scf_status": (ctypes.c_int, [_P, _I, _I, _P, _P, _P, _P, _I, _P, ctypes.c_double, _P]),
     "gdx_diis_coefficients": (ctypes.c_int, [_P, _I, _I, ctypes.c_int64, _P, _I, _P, _P]),
+    "gdx_diis_error_workspace_size": (ctypes.c_int, [_I, _I, ctypes.POINTER(_Z)]),
+    "gdx_diis_error": (ctypes.c_int, [_P, _I, _I, _P, _P, _P, _P, _P]),
     "gdx_scf_step_workspace_size": (ctypes.c_int, [_I, _I, ctypes.POINTER(_Z)]),
     "gdx_scf_step": (ctypes.c_int, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, ctypes.c_double,
                                     _P]),

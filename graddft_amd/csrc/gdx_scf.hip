@@ -835,6 +835,16 @@ __global__ void fock_extrap_kernel(int n, int m, int nspin, const double* __rest
   F[(size_t)s * nn + e] = f;
 }
 
+// err = A - A^T per spin (A n x n, row-major).
+__global__ void antisym_kernel(int n, const double* __restrict__ A, double* __restrict__ err) {
+  const size_t nn = (size_t)n * n;
+  const size_t e = (size_t)blockIdx.x * NT + threadIdx.x;
+  const int s = blockIdx.y;
+  if (e >= nn) return;
+  const int i = (int)(e / n), j = (int)(e % n);
+  err[s * nn + e] = A[s * nn + e] - A[s * nn + (size_t)j * n + i];
+}
+
 static bool use_tridiag(int n, size_t lds_need) { return lds_need > eig_smem_budget() && n <= tri::NMAX; }
 
 template <int RA, int RR>
@@ -1245,6 +1255,26 @@ int gdx_scf_step(void* stream, int32_t nspin, int32_t n, int32_t m, const double
     GDX_LAUNCH("scf_step", kern, dim3(nspin), dim3(EIG_THREADS_GLOBAL), dyn, st, n, m, nspin, focks, x, Linv, V0,
                occ, F, w, V, C, rdm1, (double*)workspace, max_sweeps, tol, sweeps, dyn > 0 ? 1 : 0);
   }
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+int gdx_diis_error_workspace_size(int32_t nspin, int32_t n, size_t* bytes) {
+  GDX_REQUIRE(nspin > 0 && n > 0 && bytes, "gdx_diis_error_workspace_size: bad arguments");
+  *bytes = (size_t)2 * nspin * n * n * sizeof(double);
+  return GDX_OK;
+}
+
+int gdx_diis_error(void* stream, int32_t nspin, int32_t n, const double* F, const double* D, const double* S,
+                   double* err, void* workspace) {
+  GDX_REQUIRE(nspin > 0 && nspin <= 64 && n > 0 && F && D && S && err && workspace, "gdx_diis_error: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const long nn = (long)n * n;
+  double* T = (double*)workspace;
+  double* A = T + nspin * nn;
+  small_gemm<false, false>(st, nspin, n, F, nn, D, nn, nullptr, 0, T, nn);   // T = F D
+  small_gemm<false, false>(st, nspin, n, T, nn, S, 0, nullptr, 0, A, nn);    // A = T S
+  GDX_LAUNCH("diis_error", antisym_kernel, dim3((unsigned)((nn + NT - 1) / NT), nspin), dim3(NT), 0, st, n, A, err);
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
 }

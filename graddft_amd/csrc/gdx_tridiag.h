@@ -453,10 +453,12 @@ __global__ __launch_bounds__(INV_TPB) void tri_invit_kernel(int n, const double*
   for (int i = 0; i < n; ++i) X[(size_t)i * n + j] = y[i * INV_TPB] * scale;
 }
 
-// Clusters: runs of eigenvalues with consecutive gaps <= 1e-3 ||T||.  Independent inverse
-// iteration can return nearly parallel vectors for members of a cluster (for a double
-// eigenvalue both iterates may settle on the same direction), and orthogonalising them
-// afterwards would leave mostly rounding noise.  So, one wave per cluster (its first index):
+// Clusters: runs of eigenvalues with consecutive gaps <= 1e-9 ||T|| (numerically double
+// eigenvalues).  Independent inverse iteration can return nearly parallel vectors for members
+// of such a cluster (both iterates may settle on the same direction), and orthogonalising them
+// afterwards would leave mostly rounding noise.  Wider gaps need nothing here: the vectors are
+// then orthogonal to ~eps ||T|| / gap <= 1e-7, which the first-order Loewdin polish that
+// follows reduces to its square, without raising the residuals (eps ||T||).  So, one wave per cluster (its first index):
 // each later member m is orthogonalised against the members before it (modified Gram-Schmidt,
 // twice) and solved again with its own factorisation of T - w_m I, twice, then orthogonalised
 // and normalised once more.  The solve amplifies whatever survives the projection, so the
@@ -484,7 +486,7 @@ __global__ __launch_bounds__(256) void tri_orth_kernel(int n, const double* __re
   const int j = blockIdx.x * 4 + wv;
   if (j >= n) return;
   const double* w = w_all + (size_t)b * n;
-  const double ctol = 1e-3 * tn_all[b];
+  const double ctol = 1e-9 * tn_all[b];
   if (j > 0 && !(w[j] - w[j - 1] > ctol)) return;   // not the first of its cluster
   int end = j + 1;
   while (end < n && !(w[end] - w[end - 1] > ctol)) ++end;

@@ -16,6 +16,7 @@ autograd Functions over the device kernels when autograd is recording.
 """
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass
 from typing import Callable, List, Optional
 
@@ -94,6 +95,68 @@ class DiisSolveFn(torch.autograd.Function):
         return -y.unsqueeze(-1) * x.unsqueeze(-2)
 
 
+def _diis_error_device(F, D, S):
+    ns, n = F.shape[0], F.shape[-1]
+    nbytes = ctypes.c_size_t(0)
+    _lib.check(_lib.load().gdx_diis_error_workspace_size(ns, n, ctypes.byref(nbytes)),
+               "gdx_diis_error_workspace_size")
+    # per call (the caching allocator; a captured graph gets its own): loops of different
+    # molecules may run concurrently on several streams
+    ws = torch.empty(int(nbytes.value) // 8, dtype=torch.float64, device=F.device)
+    err = torch.empty_like(F)
+    _lib.check(_lib.load().gdx_diis_error(torch.cuda.current_stream(F.device).cuda_stream, ns, n, F.data_ptr(),
+                                          D.data_ptr(), S.data_ptr(), err.data_ptr(), ws.data_ptr()),
+               "gdx_diis_error")
+    return err
+
+
+class DiisErrorFn(torch.autograd.Function):
+    """err = F D S - (F D S)^T per spin (JittableDiis.update, evaluate.py:1111-1123) in one device
+    call (gdx_diis_error).  Reverse, with H = errbar - errbar^T: Fbar = H (D S)^T,
+    Dbar = F^T H S^T."""
+
+    @staticmethod
+    def forward(ctx, F, D, S):
+        F, D, S = F.contiguous(), D.contiguous(), S.contiguous()
+        ctx.save_for_backward(F, D, S)
+        return _diis_error_device(F, D, S)
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, gE):
+        F, D, S = ctx.saved_tensors
+        H = gE - gE.transpose(-1, -2)
+        gF = gD = None
+        if ctx.needs_input_grad[0]:
+            gF = H @ (D @ S).transpose(-1, -2)
+        if ctx.needs_input_grad[1]:
+            gD = F.transpose(-1, -2) @ H @ S.transpose(-1, -2)
+        return gF, gD, None
+
+
+class DiisExtrapolateFn(torch.autograd.Function):
+    """F_s = sum_i x_si F_is (JittableDiis.run's extrapolation, evaluate.py:1198-1205).  Reverse:
+    Fbar_is = x_si Fbar_s, and xbar_si = <Fbar_s, F_is> as an elementwise product reduced over
+    the n^2 entries (written as a product with an n^2-long inner dimension and m x nspin
+    outputs, the library GEMM runs it in a handful of workgroups)."""
+
+    @staticmethod
+    def forward(ctx, x, focks):
+        ctx.save_for_backward(x, focks)
+        return torch.einsum("si,isjk->sjk", x, focks)
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, gF):
+        x, focks = ctx.saved_tensors
+        gx = gfocks = None
+        if ctx.needs_input_grad[0]:
+            gx = (focks * gF.unsqueeze(0)).sum(dim=(2, 3)).transpose(0, 1)
+        if ctx.needs_input_grad[1]:
+            gfocks = x.transpose(0, 1)[:, :, None, None] * gF.unsqueeze(0)
+        return gx, gfocks
+
+
 @dataclass
 class JittableDiis:
     overlap_matrix: torch.Tensor
@@ -106,8 +169,7 @@ class JittableDiis:
 
     def update(self, rdm1, fock, energy, data, cycle: int):
         dens, focks, energies, errs = data
-        fds = fock @ rdm1 @ self.overlap_matrix
-        err = fds - fds.transpose(1, 2)
+        err = DiisErrorFn.apply(fock, rdm1, self.overlap_matrix)
         if cycle > self.max_diis:
             errs = torch.cat([errs[1:], err[None]])
             dens = torch.cat([dens[1:], rdm1[None]])
@@ -131,7 +193,7 @@ class JittableDiis:
     def run(self, rdm1, fock, energy, data, cycle: int):
         data = self.update(rdm1, fock, energy, data, cycle)
         x = self.coefficients(data[3], cycle)
-        F = torch.einsum("si,isjk->sjk", x, data[1])
+        F = DiisExtrapolateFn.apply(x, data[1])
         return F, data
 
 

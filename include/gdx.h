@@ -245,6 +245,13 @@ int gdx_scf_status(void* stream, int32_t nspin, int32_t n, const double* rdm1, c
  * JittableDiis.cdiis_minimize, grad_dft/evaluate.py:1198-1205. */
 int gdx_diis_solve(void* stream, int32_t nspin, int32_t m1, const double* B, double* x);
 
+/* DIIS error of a cycle, JittableDiis.update (grad_dft/evaluate.py:1111-1123):
+ * err_s = F_s D_s S - (F_s D_s S)^T for nspin n x n matrices (S shared).  workspace:
+ * gdx_diis_error_workspace_size bytes. */
+int gdx_diis_error_workspace_size(int32_t nspin, int32_t n, size_t* bytes);
+int gdx_diis_error(void* stream, int32_t nspin, int32_t n, const double* F, const double* D, const double* S,
+                   double* err, void* workspace);
+
 /* JittableDiis coefficients (grad_dft/evaluate.py:1185-1205) in one call: errs (m, nspin,
  * n2) holds the m stored error vectors FDS - SDF (n2 = n^2 elements each); the Gram matrix per
  * spin, the bordered matrix B (nspin, m+1, m+1) -- +1 borders for the slots i <= cycle, the

@@ -351,3 +351,25 @@ def test_scf_status_word_flags_and_loop_raises(monkeypatch):
     monkeypatch.setattr(eigenproblem, "JACOBI_MAX_SWEEPS", 1)
     with pytest.raises(_lib.GdxError, match="did not converge"):
         diff_scf_loop(B3LYP, cycles=3)(None, mol)
+
+
+@pytest.mark.parametrize("n", [7, 43, 180])
+def test_diis_error_matches_torch(n):
+    """gdx_diis_error (F D S - (F D S)^T through the tiled GEMMs) against torch, and the
+    DiisErrorFn reverse mode against finite differences (gradcheck, n = 7)."""
+    _say(f"case: diis error n={n}")
+    from graddft_amd.evaluate import DiisErrorFn
+    rng = np.random.default_rng(n)
+    t = lambda a: torch.as_tensor(a, device="cuda")  # noqa: E731
+    F = rng.normal(size=(2, n, n))
+    D = rng.normal(size=(2, n, n))
+    X = rng.normal(size=(n, n))
+    S = X @ X.T / n + np.eye(n)
+    err = DiisErrorFn.apply(t(F), t(D), t(S)).cpu().numpy()
+    A = F @ D @ S
+    ref = A - A.transpose(0, 2, 1)
+    assert np.abs(err - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    if n == 7:
+        Ft = t(F).requires_grad_(True)
+        Dt = t(D).requires_grad_(True)
+        assert torch.autograd.gradcheck(lambda a, b: DiisErrorFn.apply(a, b, t(S)), (Ft, Dt))
