@@ -478,6 +478,7 @@ def main():
         return
     import torch
     from graddft_amd import MLPSpec, NeuralFunctional, _lib
+    from graddft_amd.evaluate import capture_graph
     from graddft_amd.interface import geometry
     from graddft_amd.interface.builder import molecule_from_geometry
 
@@ -520,8 +521,7 @@ def main():
         with torch.cuda.stream(side):
             step()
         torch.cuda.current_stream().wait_stream(side)
-        with torch.cuda.graph(graph):
-            step()
+        capture_graph(graph, step)
         for _ in range(args.warmup):
             graph.replay()
         torch.cuda.synchronize()
@@ -574,8 +574,7 @@ def main():
             with torch.cuda.stream(side):
                 step64()
             torch.cuda.current_stream().wait_stream(side)
-            with torch.cuda.graph(g64):
-                step64()
+            capture_graph(g64, step64)
             g64.replay()
             torch.cuda.synchronize()
             run64 = g64.replay

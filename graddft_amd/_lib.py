@@ -42,6 +42,7 @@ SIGNATURES = {
     "gdx_last_error": (ctypes.c_char_p, []),
     "gdx_version": (ctypes.c_int, []),
     "gdx_ao_eval": (ctypes.c_int, [_P, _I, _P, _P, _P, _P, _I, _I, _P, _I, _I, _P]),
+    "gdx_xc_autotune": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P]),
     "gdx_xc_workspace_size": (ctypes.c_int, [ctypes.POINTER(XCDesc), ctypes.POINTER(_Z)]),
     "gdx_xc_forward": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P, _P, _P]),
     "gdx_coulomb_workspace_size": (ctypes.c_int, [_I, ctypes.POINTER(_Z)]),

@@ -237,6 +237,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void density_ao_batch_
 
 constexpr int VX_P = 4, VX_K = 4 * VX_P;
 constexpr int VX_FLUSH = 8;   // FP32 path: stages (of VX_P points) per FP32 partial sum
+constexpr int VB_FLUSH = 4;   // split-BF16 path: stages (of VB_P points) per FP32 partial sum
 
 // VT: output tile edge (128, or 64 for nao_pad <= 64); each wave owns a (VT/2)^2 quarter.
 // FLUSH (FP32 only): the MFMA accumulators are added into FP64 registers every VX_FLUSH
@@ -361,7 +362,7 @@ __device__ __forceinline__ void vxc_ao_body(const T* __restrict__ ao, int npts_p
 }
 
 template <typename T, bool SAME, int VT, bool FLUSH = false>
-__global__ __launch_bounds__(256, (FLUSH && SAME) ? 2 : 1) void vxc_ao_kernel(const T* __restrict__ ao, int npts_pad,
+__global__ __launch_bounds__(256, VT == 64 ? 4 : ((FLUSH && SAME) ? 2 : 1)) void vxc_ao_kernel(const T* __restrict__ ao, int npts_pad,
                                                                              int nP, const double* __restrict__ vpt,
                                                                              int chunk, double* __restrict__ vpart) {
   const int s = blockIdx.z;
@@ -504,7 +505,7 @@ __global__ __launch_bounds__(256) void vxc_bf16x6_kernel(const float* __restrict
   const int g = lane >> 4, r = lane & 15;
   for (int st = 0; st < nst; ++st) {
     const int cur = st & 1;
-    if (st > 0 && st % (VX_FLUSH / 2) == 0) {   // same point count per FP32 sum as vxc_ao_kernel
+    if (st > 0 && st % VB_FLUSH == 0) {   // 32 points (128 k) per FP32 sum
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll

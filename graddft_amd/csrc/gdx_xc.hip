@@ -31,6 +31,9 @@
 #include "gdx_rows.h"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 namespace gdx {
@@ -722,13 +725,27 @@ struct VxcSplit {
 // SCF adjoint through many cycles is sensitive to the forward Fock matrices, and 500-point
 // FP32 sums measurably degrade it (relative gradient error 7e-5 against 5e-6 flushed)
 constexpr int VX_FLUSH_MIN = 0;
+// Output tile edge of the V_xc build chosen by gdx_xc_autotune (measured on the device for a
+// shape), keyed by (N_pad, n_pad, nsv, precision, force_flush); absent: the default rule.
+std::mutex vxc_tune_mu;
+std::map<std::tuple<int, int, int, int, int>, int> vxc_tune;
+
+int tuned_vt(int Np, int nP, int nsv, int prec, bool force_flush) {
+  std::lock_guard<std::mutex> g(vxc_tune_mu);
+  const auto it = vxc_tune.find(std::make_tuple(Np, nP, nsv, prec, (int)force_flush));
+  return it == vxc_tune.end() ? 0 : it->second;
+}
+
 // prec: GDX_FP64 / GDX_FP32 / GDX_BF16X6 (bf16 MFMA for tiles of 128).  force_flush: the
 // FP32 build always flushes its accumulators into FP64 (used by the second-order pass, whose
-// potentials cancel more strongly than the first-order ones).
-VxcSplit vxc_split(int Np, int nP, int nsv, int prec, bool force_flush = false) {
+// potentials cancel more strongly than the first-order ones).  vt: output tile edge (64 or
+// 128); 0 takes the tuned value for this shape, else 64 for n_pad <= 64 and 128 above.
+VxcSplit vxc_split(int Np, int nP, int nsv, int prec, bool force_flush = false, int vt = 0) {
   VxcSplit v{};
   const bool fp64 = prec == GDX_FP64;
-  v.vt = nP <= 64 ? 64 : 128;
+  if (vt == 0) vt = tuned_vt(Np, nP, nsv, prec, force_flush);
+  if (vt == 0) vt = nP <= 64 ? 64 : 128;
+  v.vt = vt;
   const int nt = (nP + v.vt - 1) / v.vt;
   v.ntile = nt * nt;
   int resident = v.vt == 64 ? (fp64 ? 4 : 7) : (fp64 ? 2 : 3);
@@ -753,11 +770,11 @@ VxcSplit vxc_split(int Np, int nP, int nsv, int prec, bool force_flush = false) 
   };
   split(resident);
   v.flush = false;
-  if (!fp64 && prec != GDX_BF16X6 && v.vt == 128 && (force_flush || v.chunk > VX_FLUSH_MIN)) {
-    // the flushing kernel holds 64 FP64 accumulators a lane: two blocks per CU for the
-    // single-tile case (its launch bounds), one otherwise
+  if (!fp64 && !(prec == GDX_BF16X6 && v.vt == 128) && (force_flush || v.chunk > VX_FLUSH_MIN)) {
+    // the flushing kernel holds 4 WB^2 FP64 accumulators a lane: for tiles of 128, two
+    // blocks per CU for the single-tile case (its launch bounds), one otherwise
     v.flush = true;
-    split(v.ntile == 1 ? 2 : 1);
+    split(v.vt == 64 ? 4 : (v.ntile == 1 ? 2 : 1));
   }
   return v;
 }
@@ -788,9 +805,14 @@ Layout make_layout(const gdx_xc_desc& d, bool hvp = false, bool batch = false) {
   L.nsd = d.spin_restricted ? 1 : 2;
   L.rows = (int)R;
   {
-    const VxcSplit v1 = vxc_split((int)Np, (int)nP, 1, d.precision, hvp),
-                   v2 = vxc_split((int)Np, (int)nP, 2, d.precision, hvp);
-    L.nchunk = std::max(v1.nchunk, 2 * v2.nchunk);   // partial slots: nchunk x nsv
+    // partial slots nchunk x nsv, for every tile edge the tuner may pick
+    L.nchunk = 0;
+    for (int vt : {64, 128})
+      for (bool ff : {false, true}) {
+        const VxcSplit v1 = vxc_split((int)Np, (int)nP, 1, d.precision, ff, vt),
+                       v2 = vxc_split((int)Np, (int)nP, 2, d.precision, ff, vt);
+        L.nchunk = std::max(L.nchunk, std::max(v1.nchunk, 2 * v2.nchunk));
+      }
   }
   // split-K for the hidden weight-gradient GEMMs: 128 splits of whole 32-row stages, the
   // last one of a segment shorter; for hvp the value and tangent rows are two segments of
@@ -1194,6 +1216,48 @@ void reduce_slices2(const Ctx& c, const TP* p0, int ns0, long st0, long len0, do
              len0, o0, nb0, t1, G1, len1, len1, o1);
 }
 
+// The V_xc kernel of a split: partial sums of the nsv spins into L.vpart.
+template <typename T>
+void launch_vxc_kernel(const Ctx& c, const T* ao, int nsv, const VxcSplit& v) {
+  const gdx_xc_desc& d = c.d;
+  const Layout& L = c.L;
+  hipStream_t st = c.st;
+  const int Np = d.npts_pad, nP = d.nao_pad;
+  const char* nm = tname<T>("vxc_f64", "vxc_f32");
+  const dim3 grid(v.ntile, v.nchunk, nsv);
+  const double* vpt = c.P<double>(L.vpt);
+  double* vpart = c.P<double>(L.vpart);
+  const bool same = v.ntile == 1;
+  if constexpr (sizeof(T) == 4) {
+    if (d.precision == GDX_BF16X6 && v.vt == 128) {
+      if (same)
+        GDX_LAUNCH("vxc_bf16x6", vxc_bf16x6_kernel<true>, grid, dim3(NT), 0, st, ao, Np, nP, vpt, v.chunk, vpart);
+      else
+        GDX_LAUNCH("vxc_bf16x6", vxc_bf16x6_kernel<false>, grid, dim3(NT), 0, st, ao, Np, nP, vpt, v.chunk, vpart);
+      return;
+    }
+    if (v.flush) {
+      if (v.vt == 64 && same)
+        GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 64, true>), grid, dim3(NT), 0, st, ao, Np, nP, vpt, v.chunk, vpart);
+      else if (v.vt == 64)
+        GDX_LAUNCH(nm, (vxc_ao_kernel<T, false, 64, true>), grid, dim3(NT), 0, st, ao, Np, nP, vpt, v.chunk, vpart);
+      else if (same)
+        GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 128, true>), grid, dim3(NT), 0, st, ao, Np, nP, vpt, v.chunk, vpart);
+      else
+        GDX_LAUNCH(nm, (vxc_ao_kernel<T, false, 128, true>), grid, dim3(NT), 0, st, ao, Np, nP, vpt, v.chunk, vpart);
+      return;
+    }
+  }
+  if (v.vt == 64 && same)
+    GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 64>), grid, dim3(NT), 0, st, ao, Np, nP, vpt, v.chunk, vpart);
+  else if (v.vt == 64)
+    GDX_LAUNCH(nm, (vxc_ao_kernel<T, false, 64>), grid, dim3(NT), 0, st, ao, Np, nP, vpt, v.chunk, vpart);
+  else if (same)
+    GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 128>), grid, dim3(NT), 0, st, ao, Np, nP, vpt, v.chunk, vpart);
+  else
+    GDX_LAUNCH(nm, (vxc_ao_kernel<T, false, 128>), grid, dim3(NT), 0, st, ao, Np, nP, vpt, v.chunk, vpart);
+}
+
 // V_s (nsv spins) from the per-point potentials in L.vpt, reduced into d_rdm1 (2, n, n)
 template <typename T>
 void launch_vxc(const Ctx& c, const T* ao, int nsv, double* d_rdm1, bool force_flush = false) {
@@ -1202,41 +1266,7 @@ void launch_vxc(const Ctx& c, const T* ao, int nsv, double* d_rdm1, bool force_f
   hipStream_t st = c.st;
   const int Np = d.npts_pad, nP = d.nao_pad;
   const VxcSplit v = vxc_split(Np, nP, nsv, d.precision, force_flush);
-  const char* nm = tname<T>("vxc_f64", "vxc_f32");
-  const dim3 grid(v.ntile, v.nchunk, nsv);
-  bool done = false;
-  if constexpr (sizeof(T) == 4) {
-    if (d.precision == GDX_BF16X6 && v.vt == 128) {
-      if (v.ntile == 1)
-        GDX_LAUNCH("vxc_bf16x6", vxc_bf16x6_kernel<true>, grid, dim3(NT), 0, st, ao, Np, nP, c.P<double>(L.vpt),
-                   v.chunk, c.P<double>(L.vpart));
-      else
-        GDX_LAUNCH("vxc_bf16x6", vxc_bf16x6_kernel<false>, grid, dim3(NT), 0, st, ao, Np, nP, c.P<double>(L.vpt),
-                   v.chunk, c.P<double>(L.vpart));
-      done = true;
-    }
-  }
-  if constexpr (sizeof(T) == 4) {
-    if (!done && v.flush) {
-      if (v.ntile == 1)
-        GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 128, true>), grid, dim3(NT), 0, st, ao, Np, nP, c.P<double>(L.vpt),
-                   v.chunk, c.P<double>(L.vpart));
-      else
-        GDX_LAUNCH(nm, (vxc_ao_kernel<T, false, 128, true>), grid, dim3(NT), 0, st, ao, Np, nP, c.P<double>(L.vpt),
-                   v.chunk, c.P<double>(L.vpart));
-      done = true;
-    }
-  }
-  if (done) {
-  } else if (v.vt == 64)
-    GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 64>), grid, dim3(NT), 0, st, ao, Np, nP, c.P<double>(L.vpt), v.chunk,
-               c.P<double>(L.vpart));
-  else if (v.ntile == 1)
-    GDX_LAUNCH(nm, (vxc_ao_kernel<T, true, 128>), grid, dim3(NT), 0, st, ao, Np, nP, c.P<double>(L.vpt), v.chunk,
-               c.P<double>(L.vpart));
-  else
-    GDX_LAUNCH(nm, (vxc_ao_kernel<T, false, 128>), grid, dim3(NT), 0, st, ao, Np, nP, c.P<double>(L.vpt), v.chunk,
-               c.P<double>(L.vpart));
+  launch_vxc_kernel<T>(c, ao, nsv, v);
   // chunk partials -> RED_GROUPS partials (first stage), then the (2, n, n) output
   const long len = (long)nsv * nP * nP;
   const int per = (v.nchunk + RED_GROUPS - 1) / RED_GROUPS;
@@ -1585,6 +1615,86 @@ int gdx_xc_forward(void* stream, const gdx_xc_desc* desc, const void* ao, const 
   Ctx c{*desc, make_layout(*desc), (char*)workspace, (hipStream_t)stream};
   if (desc->precision == GDX_FP64) return forward_impl<double>(c, (const double*)ao, weights, rdm1, params, exc);
   return forward_impl<float>(c, (const float*)ao, weights, rdm1, params, exc);
+}
+
+int gdx_xc_autotune(void* stream, const gdx_xc_desc* desc, const void* ao, void* workspace, int32_t* vt_inout) {
+  int rc = validate(desc);
+  if (rc != GDX_OK) return rc;
+  GDX_REQUIRE(ao && workspace, "gdx_xc_autotune: null pointer");
+  const int pin = vt_inout ? *vt_inout : 0;
+  GDX_REQUIRE(pin == 0 || pin == 64 || pin == 128, "gdx_xc_autotune: the tile may be pinned to 64 or 128 only");
+  if (pin != 0) {   // pinned by the caller: record without measuring
+    const bool nrl = is_neural(*desc);
+    const int nsv_fwd = (!nrl && desc->spin_restricted) ? 1 : 2;
+    std::lock_guard<std::mutex> g(vxc_tune_mu);
+    vxc_tune[std::make_tuple(desc->npts_pad, desc->nao_pad, nsv_fwd, desc->precision, 0)] = pin;
+    vxc_tune[std::make_tuple(desc->npts_pad, desc->nao_pad, 2, desc->precision, 1)] = pin;
+    return GDX_OK;
+  }
+  Ctx c{*desc, make_layout(*desc), (char*)workspace, (hipStream_t)stream};
+  const int Np = desc->npts_pad, nP = desc->nao_pad;
+  const bool neural = is_neural(*desc);
+  // the builds a plan runs: the first-order V_xc (nsv spins) and the second-order one (two
+  // spins, FP32 accumulators always flushed)
+  const int nsv_fwd = (!neural && desc->spin_restricted) ? 1 : 2;
+  const std::pair<int, bool> builds[2] = {{nsv_fwd, false}, {2, true}};
+  hipEvent_t e0, e1;
+  GDX_HIP_CHECK(hipEventCreate(&e0));
+  GDX_HIP_CHECK(hipEventCreate(&e1));
+  // the timing launches read the potentials region: clear it first (contents are overwritten
+  // by every real build)
+  GDX_HIP_CHECK(hipMemsetAsync(c.P<double>(c.L.vpt), 0, (size_t)2 * Np * VPT_STRIDE * sizeof(double), c.st));
+  int first = 0;
+  for (int b = 0; b < 2; ++b) {
+    const int nsv = builds[b].first;
+    const bool ff = builds[b].second;
+    const int known = tuned_vt(Np, nP, nsv, desc->precision, ff);
+    if (known != 0) {   // measured before for this shape: keep it (plans of one shape agree)
+      if (b == 0) first = known;
+      continue;
+    }
+    if (desc->precision == GDX_BF16X6) {
+      // split-BF16 keeps its own kernel (tiles of 128): the FP32 kernel of the 64 tiles would
+      // change its accuracy, not only its speed
+      const int vt = nP <= 64 ? 64 : 128;
+      {
+        std::lock_guard<std::mutex> g(vxc_tune_mu);
+        vxc_tune[std::make_tuple(Np, nP, nsv, desc->precision, (int)ff)] = vt;
+      }
+      if (b == 0) first = vt;
+      continue;
+    }
+    int best_vt = 0;
+    float best_ms = 0.0f;
+    for (int vt : {64, 128}) {
+      const VxcSplit v = vxc_split(Np, nP, nsv, desc->precision, ff, vt);
+      auto run = [&]() {
+        if (desc->precision == GDX_FP64) launch_vxc_kernel<double>(c, (const double*)ao, nsv, v);
+        else launch_vxc_kernel<float>(c, (const float*)ao, nsv, v);
+      };
+      run();   // warm-up
+      GDX_HIP_CHECK(hipEventRecord(e0, c.st));
+      for (int r = 0; r < 3; ++r) run();
+      GDX_HIP_CHECK(hipEventRecord(e1, c.st));
+      GDX_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      GDX_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (best_vt == 0 || ms < best_ms) {
+        best_vt = vt;
+        best_ms = ms;
+      }
+    }
+    {
+      std::lock_guard<std::mutex> g(vxc_tune_mu);
+      vxc_tune[std::make_tuple(Np, nP, nsv, desc->precision, (int)ff)] = best_vt;
+    }
+    if (b == 0) first = best_vt;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  GDX_HIP_CHECK(hipGetLastError());
+  if (vt_inout) *vt_inout = first;
+  return GDX_OK;
 }
 
 int gdx_xc_densities(void* stream, const gdx_xc_desc* desc, const void* ao, const double* rdm1, double* dens,

@@ -17,6 +17,7 @@ autograd Functions over the device kernels when autograd is recording.
 from __future__ import annotations
 
 import ctypes
+import gc
 from dataclasses import dataclass
 from typing import Callable, List, Optional
 
@@ -93,6 +94,23 @@ class DiisSolveFn(torch.autograd.Function):
         B, x = ctx.saved_tensors
         y = torch.linalg.solve(B.transpose(1, 2), gx.unsqueeze(-1)).squeeze(-1)
         return -y.unsqueeze(-1) * x.unsqueeze(-2)
+
+
+def capture_graph(g: "torch.cuda.CUDAGraph", fn: Callable):
+    """Capture fn() into g with Python's cyclic garbage collector run first and paused during
+    the capture.  Molecules and their XC plans reference each other, so a dead molecule --
+    with the graphs of its captured loops in its cache -- is freed only by the cyclic
+    collector; a CUDAGraph destroyed while another capture is in progress makes calls a
+    capturing stream does not permit, and its destructor then aborts the process."""
+    gc.collect()
+    enabled = gc.isenabled()
+    gc.disable()
+    try:
+        with torch.cuda.graph(g):
+            return fn()
+    finally:
+        if enabled:
+            gc.enable()
 
 
 def _diis_error_device(F, D, S):
@@ -343,8 +361,7 @@ def diff_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp
                 run_loop(static_p, m_in)
             torch.cuda.current_stream().wait_stream(side)
             g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
-                m_out = run_loop(static_p, m_in)
+            m_out = capture_graph(g, lambda: run_loop(static_p, m_in))
             molecule.cache[key] = (g, static_p, static_D, m_out)
             return checked(out)
         g, static_p, static_D, m_out = entry

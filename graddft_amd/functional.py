@@ -56,6 +56,11 @@ class MLPSpec:
         return 2 * sum(i * o for i, o in self.shapes())
 
 
+# Tile shapes of the V_xc build measured on the device per shape (gdx_xc_autotune) when a plan
+# is created; False keeps the fixed rule (64 for n_pad <= 64, else 128).
+AUTOTUNE = True
+
+
 def ao_planes_for(m: Molecule, precision: str) -> torch.Tensor:
     """AO planes in the element type of a precision: the FP64 planes, or their FP32 copy
     (made once and cached on the molecule)."""
@@ -101,6 +106,15 @@ class XCPlan:
         self.generation = 0
         self.fwd_generation = -1
         self.hvp_workspace = None
+        # measured choice of the V_xc output tile for this shape (first plan of a shape in the
+        # process measures; later ones reuse it)
+        self.vxc_tile = None
+        if AUTOTUNE and not torch.cuda.is_current_stream_capturing():
+            vt = ctypes.c_int32(0)
+            st = torch.cuda.current_stream(molecule.ao_planes.device).cuda_stream
+            _lib.check(lib.gdx_xc_autotune(st, ctypes.byref(d), self._ao().data_ptr(), self.workspace.data_ptr(),
+                                           ctypes.byref(vt)), "gdx_xc_autotune")
+            self.vxc_tile = int(vt.value)
 
     @staticmethod
     def _ptr(t: Optional[torch.Tensor]) -> int:

@@ -101,6 +101,15 @@ int gdx_xc_workspace_size(const gdx_xc_desc* desc, size_t* bytes);
 int gdx_xc_forward(void* stream, const gdx_xc_desc* desc, const void* ao, const double* weights,
                    const double* rdm1, const double* params, double* exc, void* workspace);
 
+/* Autotuner (no counterpart in the reference): times the candidate output tiles (64, 128) of
+ * the V_xc build for this descriptor's shape on the device, with the given AO planes and
+ * workspace (whose contents it overwrites), and records the faster one for every later call
+ * with the same shape (N_pad, n_pad, spins, precision) in this process; a shape measured
+ * before keeps its choice.  vt_inout (may be NULL): 0 on entry measures, 64 or 128 pins that
+ * tile without measuring; on exit the first-order choice.  Synchronises the stream; not for
+ * use under graph capture. */
+int gdx_xc_autotune(void* stream, const gdx_xc_desc* desc, const void* ao, void* workspace, int32_t* vt_inout);
+
 /* Grid densities per spin, (npts_pad, 2, 6) doubles: rho, d rho/dx, d rho/dy, d rho/dz,
  * tau, laplacian -- Molecule.density / grad_density / kinetic_density / lapl_density of
  * grad_dft/molecule.py:388-502 in one pass.  Uses (and overwrites) the workspace. */

@@ -119,3 +119,36 @@ def test_status_word_decoding():
     with pytest.raises(_lib.GdxError) as e:
         raise_on_status(torch.tensor([5], dtype=torch.int32), "loop")
     assert "did not converge" in str(e.value) and "tr(D S)" in str(e.value) and "finite" not in str(e.value)
+
+
+def test_capture_graph_pauses_the_cyclic_collector(monkeypatch):
+    """capture_graph collects garbage first and keeps the cyclic collector paused while the
+    capture runs (a CUDAGraph freed by it mid-capture would abort the process), restoring the
+    collector's state afterwards, also when the captured function raises."""
+    import contextlib
+    import gc
+
+    import torch
+
+    from graddft_amd import evaluate
+
+    calls = []
+    monkeypatch.setattr(torch.cuda, "graph", lambda g: contextlib.nullcontext())
+    real_collect = gc.collect
+    monkeypatch.setattr(gc, "collect", lambda *a: calls.append("collect") or real_collect(*a))
+    seen = {}
+
+    def body():
+        seen["enabled"] = gc.isenabled()
+        return 7
+
+    assert gc.isenabled()
+    assert evaluate.capture_graph(object(), body) == 7
+    assert calls == ["collect"] and seen["enabled"] is False and gc.isenabled()
+
+    def boom():
+        raise RuntimeError("x")
+
+    with pytest.raises(RuntimeError):
+        evaluate.capture_graph(object(), boom)
+    assert gc.isenabled()
