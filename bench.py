@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import argparse
 import ctypes
+import gc
 import json
 import math
 import os
@@ -59,8 +60,8 @@ def parse():
     ap.add_argument("--no-train-batch", action="store_true", help="skip the batched training step")
     ap.add_argument("--no-eri", action="store_true", help="skip the device ERI generation measurement")
     ap.add_argument("--no-config4", action="store_true", help="skip the naphthalene dense-ERI SCF workload")
-    ap.add_argument("--batch-size", type=int, default=8, help="alkanes in the batched training step")
-    ap.add_argument("--batch-max-nc", type=int, default=4, help="largest n_C of the batched training step")
+    ap.add_argument("--batch-size", type=int, default=64, help="alkanes in the batched training step")
+    ap.add_argument("--batch-max-nc", type=int, default=8, help="largest n_C of the batched training step")
     ap.add_argument("--no-graph", action="store_true", help="time the eager step instead of graph replays")
     ap.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (calibrates HBM byte counters)")
@@ -382,60 +383,98 @@ def train_bench(args):
     return out
 
 
-def train_batch_bench(args):
-    """A training step over a batch of alkanes (the ragged training-batch workload at reduced
-    size): n_C cycling over 1..args.batch_max_nc (seeded all-trans geometries, N(0, 0.02 A)
-    jitter), def2-SVP, level-2 grids, neural B3LYP with an MLP of 3 x 128, mse_energy_loss over
-    args.train_cycles-cycle differentiable SCF loops, forward + reverse to d loss / d theta.
-    The XC evaluation of every cycle batched over the molecules (batched_diff_scf_loop: one
-    MLP pass over all grid points, ragged AO-side launches) against the reference's
-    molecule-by-molecule loop (diff_scf_loop per molecule), both on the device."""
+def _release_xc_plans(mols):
+    """Drop the XC plans (per molecule and batched) cached on the molecules and return their
+    device memory: the batched plan's workspaces are allocated by the library, outside torch's
+    caching allocator, and molecules referenced by their plans are freed only by the cyclic
+    garbage collector."""
+    import gc
     import torch
-    from graddft_amd import MLPSpec, NeuralFunctional, batched_diff_scf_loop, diff_scf_loop, mse_energy_loss
+    for m in mols:
+        for k in [k for k in m.cache if isinstance(k, tuple) and k and k[0] in ("xcplan", "xcbatch")]:
+            plan = m.cache.pop(k)
+            if hasattr(plan, "close"):
+                plan.close()
+    gc.collect()
+    torch.cuda.empty_cache()
+
+
+def train_batch_bench(args):
+    """The ragged training batch: args.batch_size linear alkanes C_nH_2n+2 with n_C ~
+    U{1..args.batch_max_nc} from the seed (all-trans geometries, N(0, 0.02 A) jitter),
+    def2-SVP, level-2 grids, device-generated packed ERIs, neural B3LYP with an MLP of 3 x 128;
+    one training step = mse_energy_loss over args.train_cycles-cycle differentiable SCF loops,
+    forward + reverse to d loss / d theta, with the XC evaluation of every cycle batched over
+    the molecules (batched_diff_scf_loop: one MLP pass over all grid points, ragged AO-side
+    launches).  One warm-up and one timed step per precision.  Roofline: W_pt = n_s 32 n^2 +
+    6 F_mlp flops per grid point and cycle (forward, SCF adjoint with the second-order pass;
+    n_s = 2) at the precision's peak.  Each precision runs every cycle in itself
+    (refine_cycles=0).  Per molecule, |E_fp32 - E_fp64| is reported: loops that
+    are still far from self-consistency after the cycles amplify any perturbation."""
+    import torch
+    from graddft_amd import MLPSpec, NeuralFunctional, batched_diff_scf_loop
     from graddft_amd.interface import geometry
     from graddft_amd.interface.builder import molecule_from_geometry
-    rng = np.random.default_rng(args.seed + 7)
-    ncs = [1 + i % args.batch_max_nc for i in range(args.batch_size)]
+    rng = np.random.default_rng(args.seed)
+    ncs = [int(x) for x in rng.integers(1, args.batch_max_nc + 1, args.batch_size)]
+    t0 = time.perf_counter()
     mols = []
     for nc in ncs:
         sym, xyz = geometry.alkane(rng, nc)
         mols.append(molecule_from_geometry(sym, xyz, "def2-svp", 2, with_eri=True, name=f"C{nc}", npts_align=256))
-    f = NeuralFunctional(MLPSpec(nc=6, h=128, n_hidden=3, nd=4))
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    spec = MLPSpec(nc=6, h=128, n_hidden=3, nd=4)
+    f = NeuralFunctional(spec)
     theta = f.init(seed=args.seed)
-    truth = [-40.0 * nc for nc in ncs]
-    out = {"workload": f"{len(mols)} alkanes C_nH_2n+2, n_C in {sorted(set(ncs))}, def2-SVP (nao "
-                       f"{min(m.nao for m in mols)}-{max(m.nao for m in mols)}), level-2 grids "
-                       f"({sum(m.grid_size for m in mols)} pts in total), neural B3LYP MLP 3x128, mse_energy_loss "
-                       f"over {args.train_cycles}-cycle diff_scf_loops, forward + reverse"}
-    grads = {}
-
-    def step(loop, prec):
-        th = theta.detach().clone().requires_grad_(True)
-        loss = mse_energy_loss(th, loop, mols, truth)
-        g, = torch.autograd.grad(loss, [th])
-        return float(loss), g
-
+    truth = torch.tensor([-40.0 * nc for nc in ncs], dtype=torch.float64, device="cuda")
+    cyc = args.train_cycles
+    npts = [m.grid_size for m in mols]
+    ns = [m.nao for m in mols]
+    flops = sum(N * cyc * (2 * 32 * n * n + 6 * spec.flops_per_point()) for N, n in zip(npts, ns))
+    out = {"workload": f"{len(mols)} alkanes C_nH_2n+2, n_C ~ U{{1..{args.batch_max_nc}}} (seeded), def2-SVP (nao "
+                       f"{min(ns)}-{max(ns)}), level-2 grids ({sum(npts)} pts in total), neural B3LYP MLP 3x128, "
+                       f"mse_energy_loss over {cyc}-cycle diff_scf_loops, forward + reverse, XC batched per cycle",
+           "n_molecules": len(mols), "npts_total": sum(npts), "molecule_build_s": round(t_build, 1),
+           "flops_per_step": int(flops)}
+    grads, energies = {}, {}
     for prec in (["fp64", args.precision] if args.precision != "fp64" else ["fp64"]):
-        res = {}
-        for name, loop in (("per_molecule", diff_scf_loop(f, cycles=args.train_cycles, precision=prec)),
-                           ("batched", batched_diff_scf_loop(f, cycles=args.train_cycles, precision=prec))):
-            step(loop, prec)                                   # warm-up: plans, workspaces
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            loss, g = step(loop, prec)
-            torch.cuda.synchronize()
-            res[name] = {"ms_per_step": round((time.perf_counter() - t0) * 1e3, 2), "loss": loss}
-            grads[(prec, name)] = g.cpu().numpy()
-        res["speedup"] = round(res["per_molecule"]["ms_per_step"] / res["batched"]["ms_per_step"], 3)
-        res["batched_vs_per_molecule_grad_rel"] = float(
-            np.linalg.norm(grads[(prec, "batched")] - grads[(prec, "per_molecule")])
-            / np.linalg.norm(grads[(prec, "per_molecule")]))
-        out[prec] = res
+        # every cycle in the one precision (refine_cycles=0): the mixed-precision default would
+        # also hold the FP64 batch plan, whose second-order workspace is 81 GB at this size
+        loop = batched_diff_scf_loop(f, cycles=cyc, precision=prec, refine_cycles=0)
+
+        def step():
+            th = theta.detach().clone().requires_grad_(True)
+            E = torch.stack([o.energy for o in loop(th, mols)])
+            loss = ((E - truth) ** 2).mean()
+            g, = torch.autograd.grad(loss, [th])
+            return float(loss.detach()), g, E.detach()
+
+        _release_xc_plans(mols)                  # the previous precision's plans and workspaces
+        free, _ = torch.cuda.mem_get_info()
+        print(f"[bench] train_batch {prec}: device memory free {free / 2**30:.1f} GiB", file=sys.stderr, flush=True)
+        step()                                   # warm-up: plans, workspaces
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss, g, E = step()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        peak = 78.6e12 if prec == "fp64" else 157.3e12
+        grads[prec] = g.cpu().numpy()
+        energies[prec] = E.cpu().numpy()
+        out[prec] = {"s_per_step": round(dt, 3), "loss": loss, "peak_tflops": peak / 1e12,
+                     "achieved_tflops": round(flops / dt / 1e12, 2), "roofline_frac": round(flops / peak / dt, 4),
+                     "grid_pts_cycles_per_s": round(sum(npts) * cyc / dt, 1)}
     if args.precision != "fp64":
-        g0, g1 = grads[("fp64", "batched")], grads[(args.precision, "batched")]
+        g0, g1 = grads["fp64"], grads[args.precision]
+        dE = np.abs(energies[args.precision] - energies["fp64"])
         out["grad_rel_err_vs_fp64"] = float(np.linalg.norm(g1 - g0) / np.linalg.norm(g0))
-        out["fp32_over_fp64_step_time"] = round(out[args.precision]["batched"]["ms_per_step"]
-                                                / out["fp64"]["batched"]["ms_per_step"], 3)
+        out["per_molecule_abs_dE_Ha"] = {"max": float(dE.max()), "median": float(np.median(dE)),
+                                         "within_1e-6": int((dE <= 1e-6).sum()), "of": len(mols)}
+        out["fp32_over_fp64_step_time"] = round(out[args.precision]["s_per_step"] / out["fp64"]["s_per_step"], 3)
+    _release_xc_plans(mols)
+    for m in mols:
+        m.cache.clear()
     return out
 
 
@@ -607,12 +646,24 @@ def main():
     w_pt = 8 * n * n * nsd + 8 * n * n * nsv + 3 * spec.flops_per_point()
     step_tflops = w_pt * N / (ms / 1e3) / 1e12
 
-    scf = None if args.no_scf else scf_bench(args)
-    train = None if args.no_train else train_bench(args)
-    batch = None if (args.no_batch or args.no_scf) else batch_bench(args)
-    train_batch = None if (args.no_train_batch or args.no_train) else train_batch_bench(args)
-    eri = None if args.no_eri else eri_bench(args)
-    config4 = None if (args.no_config4 or args.no_scf) else config4_bench(args)
+    def section(run, skip):
+        # each section's molecules hold device memory until the cyclic collector frees them
+        # (molecules and their XC plans reference each other): collect between sections
+        if skip:
+            return None
+        gc.collect()
+        torch.cuda.empty_cache()
+        free, total = torch.cuda.mem_get_info()
+        print(f"[bench] {run.__name__}: device memory free {free / 2**30:.1f} of {total / 2**30:.1f} GiB, "
+              f"torch allocated {torch.cuda.memory_allocated() / 2**30:.1f} GiB", file=sys.stderr, flush=True)
+        return run(args)
+
+    scf = section(scf_bench, args.no_scf)
+    train = section(train_bench, args.no_train)
+    batch = section(batch_bench, args.no_batch or args.no_scf)
+    train_batch = section(train_batch_bench, args.no_train_batch or args.no_train)
+    eri = section(eri_bench, args.no_eri)
+    config4 = section(config4_bench, args.no_config4 or args.no_scf)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -64,11 +64,16 @@ class XCBatchPlan:
         self.device = molecules[0].ao_planes.device
         self.restricted = restricted
 
-    def __del__(self):
+    def close(self) -> None:
+        """Free the plan's device workspaces now (they are allocated by the library, outside
+        torch's caching allocator); the plan is unusable afterwards."""
         h = getattr(self, "handle", None)
         if h is not None and h.value and _lib._lib is not None:
             _lib._lib.gdx_xc_batch_destroy(h)
-            self.handle = None
+        self.handle = None
+
+    def __del__(self):
+        self.close()
 
     @property
     def nmol(self) -> int:
