@@ -186,11 +186,12 @@ def batch_plan(functional: Functional, molecules: Sequence[Molecule], precision:
 def batch_energy_predictor(functional: Functional, precision: str = "fp64") -> Callable:
     """``energy_predictor`` (train.py:34-218) of a list of molecules with one batched XC
     evaluation: returns (energies, focks), one per molecule."""
-    from .scf import CoulombFn, FockAssembleFn, NonXCFn, coulomb, fock_assemble, non_xc_energy
+    from .scf import CoulombFn, ExchangeFn, FockAssembleFn, NonXCFn, coulomb, exchange, fock_assemble, non_xc_energy
 
     def predict(params: Optional[torch.Tensor], molecules: Sequence[Molecule]):
         plan = batch_plan(functional, molecules, precision)
         a0 = functional.exact_exchange
+        current = bool(a0) and functional.exchange == "current"
         recording = torch.is_grad_enabled() and (
             (params is not None and params.requires_grad) or any(m.rdm1.requires_grad for m in molecules))
         D = plan.concat([m.rdm1 for m in molecules])
@@ -199,8 +200,9 @@ def batch_energy_predictor(functional: Functional, precision: str = "fp64") -> C
             exc, vxc = XCBatchEnergyVxcFn.apply(D, params, plan)
             for i, (m, v) in enumerate(zip(molecules, plan.split(vxc))):
                 J = CoulombFn.apply(m.rdm1, m)
-                energies.append(exc[i] + NonXCFn.apply(m.rdm1, J, m, a0))
-                focks.append(FockAssembleFn.apply(J, v, m, a0))
+                K = ExchangeFn.apply(m.rdm1, m) if current else None
+                energies.append(exc[i] + NonXCFn.apply(m.rdm1, J, K, m, a0))
+                focks.append(FockAssembleFn.apply(J, v, K, m, a0))
             return energies, focks
         p = None if params is None else params.detach().contiguous()
         D = D.detach()
@@ -211,8 +213,9 @@ def batch_energy_predictor(functional: Functional, precision: str = "fp64") -> C
         for i, (m, v) in enumerate(zip(molecules, plan.split(vxc))):
             Dm = m.rdm1.detach().contiguous()
             J = coulomb(m, Dm)
-            energies.append(exc[i] + non_xc_energy(m, Dm, J, exact_exchange=a0))
-            focks.append(fock_assemble(m, J, v, exact_exchange=a0))
+            K = exchange(m, Dm) if current else None
+            energies.append(exc[i] + non_xc_energy(m, Dm, J, exact_exchange=a0, K=K))
+            focks.append(fock_assemble(m, J, v, exact_exchange=a0, K=K))
         return energies, focks
 
     return predict

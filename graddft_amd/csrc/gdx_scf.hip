@@ -171,46 +171,36 @@ __global__ void fock_assemble_kernel(const double* __restrict__ h1e, const doubl
 }
 
 // ------------------------------------------------------------- exchange (K) ---------
-// K_s[a][c] += (ab|cd) D_s[b][d] for every distinct index permutation of each unique
-// packed integral (atomic scatter; used once per molecule for the initial density).
-__device__ __forceinline__ void unpair(long I, int& i, int& j) {
-  i = (int)((sqrt(8.0 * (double)I + 1.0) - 1.0) / 2.0);
-  while ((long)i * (i + 1) / 2 > I) --i;
-  while ((long)(i + 1) * (i + 2) / 2 <= I) ++i;
-  j = (int)(I - (long)i * (i + 1) / 2);
+// K_s[a][c] = sum_bd (ab|cd) D_s[b][d] for general D (the reverse of K is K itself: the
+// operator is self-adjoint under <A, B> = sum A_ac B_ac).  One wave per (s, a, c); its lanes
+// run over d (for d <= c and pair(a,b) >= pair(c,d) the packed elements are contiguous), b
+// sequentially; each lane sums its own d in a fixed order and the lanes are combined in a
+// fixed pattern, so K is reproducible bit for bit.
+__device__ __forceinline__ long pair_index(int i, int j) {
+  return i >= j ? (long)i * (i + 1) / 2 + j : (long)j * (j + 1) / 2 + i;
 }
 
-__global__ void exchange_scatter_kernel(const double* __restrict__ eri, long P, int n, int ns,
-                                        const double* __restrict__ D, double* __restrict__ K) {
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long total = P * (P + 1) / 2;
-  if (idx >= total) return;
-  long I;
-  {
-    long t = (long)((sqrt(8.0 * (double)idx + 1.0) - 1.0) / 2.0);
-    while (t * (t + 1) / 2 > idx) --t;
-    while ((t + 1) * (t + 2) / 2 <= idx) ++t;
-    I = t;
+__global__ __launch_bounds__(256) void exchange_gather_kernel(const double* __restrict__ eri, int n, int ns,
+                                                              const double* __restrict__ D, double* __restrict__ K) {
+  const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const long nn = (long)n * n;
+  if (w >= ns * nn) return;
+  const int s = (int)(w / nn);
+  const int a = (int)((w % nn) / n), c = (int)(w % n);
+  const double* Ds = D + (size_t)s * nn;
+  double acc = 0.0;
+  for (int b = 0; b < n; ++b) {
+    const long P1 = pair_index(a, b);
+    const long t1 = P1 * (P1 + 1) / 2;
+    for (int d = lane; d < n; d += 64) {
+      const long P2 = pair_index(c, d);
+      const long idx = P1 >= P2 ? t1 + P2 : P2 * (P2 + 1) / 2 + P1;
+      acc += eri[idx] * Ds[(size_t)b * n + d];
+    }
   }
-  const long Kp = idx - I * (I + 1) / 2;
-  const double v = eri[idx];
-  if (v == 0.0) return;
-  int i, j, k, l;
-  unpair(I, i, j);
-  unpair(Kp, k, l);
-  int perm[8][4] = {{i, j, k, l}, {j, i, k, l}, {i, j, l, k}, {j, i, l, k},
-                    {k, l, i, j}, {l, k, i, j}, {k, l, j, i}, {l, k, j, i}};
-  for (int p = 0; p < 8; ++p) {
-    bool dup = false;
-    for (int q = 0; q < p; ++q)
-      if (perm[q][0] == perm[p][0] && perm[q][1] == perm[p][1] && perm[q][2] == perm[p][2] &&
-          perm[q][3] == perm[p][3])
-        dup = true;
-    if (dup) continue;
-    const int a = perm[p][0], b = perm[p][1], c = perm[p][2], d = perm[p][3];
-    for (int s = 0; s < ns; ++s)
-      atomicAdd(&K[((size_t)s * n + a) * n + c], v * D[((size_t)s * n + b) * n + d]);
-  }
+  acc = warp_sum(acc);
+  if (lane == 0) K[w] = acc;
 }
 
 // Status word of an SCF cycle, OR-ed into *status by one block (no atomics): bit 0 the
@@ -1155,11 +1145,11 @@ int gdx_fock_assemble(void* stream, int32_t n, const double* h1e, const double* 
 
 int gdx_exchange_packed(void* stream, int32_t n, int32_t nspin, const double* eri, const double* D, double* K) {
   GDX_REQUIRE(n > 0 && nspin > 0 && eri && D && K, "gdx_exchange_packed: bad arguments");
+  GDX_REQUIRE(D != K, "gdx_exchange_packed: K must not alias D");
   hipStream_t st = (hipStream_t)stream;
-  const long P = (long)n * (n + 1) / 2;
-  const long total = P * (P + 1) / 2;
-  GDX_LAUNCH("zero", zero_kernel, dim3(nblk((long)nspin * n * n)), dim3(NT), 0, st, K, (long)nspin * n * n);
-  GDX_LAUNCH("exchange_scatter", exchange_scatter_kernel, dim3(nblk(total)), dim3(NT), 0, st, eri, P, n, nspin, D, K);
+  const long waves = (long)nspin * n * n;
+  GDX_LAUNCH("exchange", exchange_gather_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, eri, n, nspin,
+             D, K);
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
 }

@@ -22,6 +22,7 @@ second-order pass ``gdx_xc_hvp``, which is what reverse mode through an SCF loop
 """
 from __future__ import annotations
 
+import copy
 import ctypes
 import math
 from dataclasses import dataclass
@@ -109,7 +110,7 @@ class XCPlan:
         # measured choice of the V_xc output tile for this shape (first plan of a shape in the
         # process measures; later ones reuse it)
         self.vxc_tile = None
-        if AUTOTUNE and not torch.cuda.is_current_stream_capturing():
+        if AUTOTUNE and molecule.ao_planes.is_cuda and not torch.cuda.is_current_stream_capturing():
             vt = ctypes.c_int32(0)
             st = torch.cuda.current_stream(molecule.ao_planes.device).cuda_stream
             _lib.check(lib.gdx_xc_autotune(st, ctypes.byref(d), self._ao().data_ptr(), self.workspace.data_ptr(),
@@ -244,11 +245,26 @@ class Functional:
     coef = None                       # constant coefficients of GDX_FUNC_LINEAR
     correlation = _lib.GDX_CORR_VWN5  # correlation in the third energy-density slot
 
+    # exact exchange from the initial density ("initial": the reference's frozen chi) or from
+    # the density being evaluated ("current", K[D] from the packed ERI every evaluation)
+    exchange = "initial"
+
     def __init__(self, name: str = "B3LYP", is_xc: bool = True, exact_exchange: float = 0.0):
         self.name = name
         self.is_xc = is_xc
         # fraction a0 of exact exchange added outside the grid (B3LYP: 0.2)
         self.exact_exchange = exact_exchange
+
+    def with_exchange(self, mode: str) -> "Functional":
+        """A copy of this functional whose exact-exchange term uses K0 of the initial density
+        (mode "initial", the reference's semantics: chi is built once from the initial density,
+        interface/pyscf.py:152-160) or K[D] of the density being evaluated (mode "current":
+        E_x = -(a0/2) sum_s tr(D_s K[D_s]), Fock term -a0 K[D], both differentiated through K)."""
+        if mode not in ("initial", "current"):
+            raise ValueError(f"exchange mode must be 'initial' or 'current', not {mode!r}")
+        f = copy.copy(self)
+        f.exchange = mode
+        return f
 
     def plan(self, molecule: Molecule, precision: str = "fp64") -> XCPlan:
         key = ("xcplan", self.kind, self.spec, precision, molecule.restricted, self.coef, self.correlation)
@@ -268,10 +284,11 @@ class Functional:
     def energy(self, params: Optional[torch.Tensor], molecule: Molecule, precision: str = "fp64"):
         """Total energy (functional.py:255-288): E_xc (+ nonXC when is_xc).  The exact-exchange
         part of B3LYP (from the molecule's initial density) is added to E_xc."""
-        from .scf import exact_exchange_energy, non_xc_energy
+        from .scf import ExchangeFn, exact_exchange_energy, non_xc_energy
         e = self.xc_energy(params, molecule, precision)
         if self.exact_exchange:
-            e = e + exact_exchange_energy(molecule, molecule.rdm1, self.exact_exchange)
+            K = ExchangeFn.apply(molecule.rdm1, molecule) if self.exchange == "current" else None
+            e = e + exact_exchange_energy(molecule, molecule.rdm1, self.exact_exchange, K)
         if self.is_xc:
             e = e + non_xc_energy(molecule, molecule.rdm1)
         return e

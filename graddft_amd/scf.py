@@ -90,12 +90,13 @@ def initial_exchange(molecule: Molecule) -> torch.Tensor:
 
 
 def non_xc_energy(molecule: Molecule, rdm1: Optional[torch.Tensor] = None, J: Optional[torch.Tensor] = None,
-                  exact_exchange: float = 0.0) -> torch.Tensor:
-    """e_nuc + sum D h1e + 1/2 sum D J (+ stale exact exchange when exact_exchange != 0)."""
+                  exact_exchange: float = 0.0, K: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """e_nuc + sum D h1e + 1/2 sum D J - (a0/2) sum_s tr(D_s K_s) when exact_exchange = a0 != 0,
+    K the exchange matrices given, else those of the initial density (initial_exchange)."""
     D = (molecule.rdm1 if rdm1 is None else rdm1).contiguous()
     if J is None:
         J = coulomb(molecule, D)
-    K0 = initial_exchange(molecule) if exact_exchange else None
+    K0 = (K.contiguous() if K is not None else initial_exchange(molecule)) if exact_exchange else None
     out = torch.empty((), dtype=torch.float64, device=D.device)
     _lib.check(_lib.load().gdx_nonxc_energy(_stream(D), molecule.nao, D.data_ptr(), molecule.h1e.data_ptr(),
                                             J.data_ptr(), 0 if K0 is None else K0.data_ptr(), float(exact_exchange),
@@ -103,14 +104,17 @@ def non_xc_energy(molecule: Molecule, rdm1: Optional[torch.Tensor] = None, J: Op
     return out
 
 
-def exact_exchange_energy(molecule: Molecule, rdm1: torch.Tensor, a0: float) -> torch.Tensor:
-    """-(a0/2) sum_s tr(D_s K0_s)."""
-    K0 = initial_exchange(molecule)
+def exact_exchange_energy(molecule: Molecule, rdm1: torch.Tensor, a0: float,
+                          K: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """-(a0/2) sum_s tr(D_s K_s), K the given exchange matrices or K0 of the initial density."""
+    K0 = initial_exchange(molecule) if K is None else K
     return -0.5 * a0 * (rdm1 * K0).sum()
 
 
-def fock_assemble(molecule: Molecule, J: torch.Tensor, vxc: torch.Tensor, exact_exchange: float = 0.0) -> torch.Tensor:
-    K0 = initial_exchange(molecule) if exact_exchange else None
+def fock_assemble(molecule: Molecule, J: torch.Tensor, vxc: torch.Tensor, exact_exchange: float = 0.0,
+                  K: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """h + J + vxc - a0 K (symmetrised, clipped), K the given exchange matrices or K0."""
+    K0 = (K.contiguous() if K is not None else initial_exchange(molecule)) if exact_exchange else None
     n = molecule.nao
     F = torch.empty((2, n, n), dtype=torch.float64, device=J.device)
     _lib.check(_lib.load().gdx_fock_assemble(_stream(J), n, molecule.h1e.data_ptr(), J.data_ptr(),
@@ -139,43 +143,65 @@ class CoulombFn(torch.autograd.Function):
         return torch.stack([dD, dD]), None
 
 
-class NonXCFn(torch.autograd.Function):
-    """non_xc_energy(D, J) with J an input: dE/dD_s = h + J/2 (- a0/2 K0_s), dE/dJ = D_t/2."""
+class ExchangeFn(torch.autograd.Function):
+    """K[D] from the packed ERI; reverse Dbar = K[Kbar] (the exchange operator is self-adjoint
+    under the Frobenius product, by the (ab|cd) = (ba|dc) symmetry)."""
 
     @staticmethod
-    def forward(ctx, rdm1, J, molecule: Molecule, exact_exchange: float):
+    def forward(ctx, rdm1, molecule: Molecule):
         ctx.molecule = molecule
-        ctx.a0 = exact_exchange
-        ctx.save_for_backward(rdm1, J)
-        return non_xc_energy(molecule, rdm1, J, exact_exchange=exact_exchange)
+        return exchange(molecule, rdm1.detach())
 
     @staticmethod
     @torch.autograd.function.once_differentiable
     def backward(ctx, g):
-        rdm1, J = ctx.saved_tensors
+        return exchange(ctx.molecule, g.contiguous()), None
+
+
+class NonXCFn(torch.autograd.Function):
+    """non_xc_energy(D, J, K) with J and K inputs: dE/dD_s = h + J/2 - (a0/2) K_s,
+    dE/dJ = D_t/2, dE/dK_s = -(a0/2) D_s.  K = None: the initial density's K0, a constant."""
+
+    @staticmethod
+    def forward(ctx, rdm1, J, K, molecule: Molecule, exact_exchange: float):
+        ctx.molecule = molecule
+        ctx.a0 = exact_exchange
+        ctx.has_k = K is not None
+        ctx.save_for_backward(rdm1, J, K)
+        return non_xc_energy(molecule, rdm1, J, exact_exchange=exact_exchange, K=K)
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, g):
+        rdm1, J, K = ctx.saved_tensors
         m = ctx.molecule
         dD = (m.h1e + 0.5 * J).expand(2, -1, -1).clone()
+        dK = None
         if ctx.a0:
-            dD = dD - 0.5 * ctx.a0 * initial_exchange(m)
+            dD = dD - 0.5 * ctx.a0 * (K if ctx.has_k else initial_exchange(m))
+            if ctx.has_k:
+                dK = g * (-0.5 * ctx.a0) * rdm1
         dJ = 0.5 * rdm1.sum(0)
-        return g * dD, g * dJ, None, None
+        return g * dD, g * dJ, dK, None, None
 
 
 class FockAssembleFn(torch.autograd.Function):
     """fock_assemble(J, vxc) with the reverse of its clips and symmetrisation."""
 
     @staticmethod
-    def forward(ctx, J, vxc, molecule: Molecule, exact_exchange: float):
+    def forward(ctx, J, vxc, K, molecule: Molecule, exact_exchange: float):
         ctx.molecule = molecule
         ctx.a0 = exact_exchange
-        ctx.save_for_backward(J, vxc)
-        return fock_assemble(molecule, J, vxc, exact_exchange=exact_exchange)
+        ctx.has_k = K is not None
+        ctx.save_for_backward(J, vxc, K)
+        return fock_assemble(molecule, J, vxc, exact_exchange=exact_exchange, K=K)
 
     @staticmethod
     @torch.autograd.function.once_differentiable
     def backward(ctx, g):
-        J, vxc = ctx.saved_tensors
+        J, vxc, K = ctx.saved_tensors
         m = ctx.molecule
+        gK = None
         x1 = m.h1e + J + vxc
         m1 = x1.abs() > CLIP
         y1 = torch.where(m1, x1, torch.zeros_like(x1))
@@ -183,13 +209,15 @@ class FockAssembleFn(torch.autograd.Function):
         m2 = x2.abs() > CLIP
         y = torch.where(m2, x2, torch.zeros_like(x2))
         if ctx.a0:
-            X = -0.5 * ctx.a0 * initial_exchange(m)
+            X = -0.5 * ctx.a0 * (K if ctx.has_k else initial_exchange(m))
             x3 = y + X + X.transpose(1, 2)
             m3 = x3.abs() > CLIP
             g = torch.where(m3, g, torch.zeros_like(g))
             y = torch.where(m3, x3, torch.zeros_like(x3))
         g = torch.where(y.abs() > CLIP, g, torch.zeros_like(g))
+        if ctx.a0 and ctx.has_k:   # x3 holds X + X^T, X = -(a0/2) K
+            gK = -0.5 * ctx.a0 * (g + g.transpose(1, 2))
         g = torch.where(m2, g, torch.zeros_like(g))
         g = 0.5 * (g + g.transpose(1, 2))
         g = torch.where(m1, g, torch.zeros_like(g))
-        return g.sum(0), g, None, None
+        return g.sum(0), g, gK, None, None

@@ -24,7 +24,7 @@ import torch
 
 from .functional import Functional, XCEnergyVxcFn
 from .molecule import Molecule
-from .scf import CoulombFn, FockAssembleFn, NonXCFn, coulomb, fock_assemble, non_xc_energy
+from .scf import CoulombFn, ExchangeFn, FockAssembleFn, NonXCFn, coulomb, exchange, fock_assemble, non_xc_energy
 
 
 def _recording(*ts) -> bool:
@@ -36,11 +36,13 @@ def energy_predictor(functional: Functional, precision: str = "fp64", **kwargs) 
         plan = functional.plan(molecule, precision)
         D = molecule.rdm1
         a0 = functional.exact_exchange
+        current = bool(a0) and functional.exchange == "current"
         if _recording(params, D):
             exc, vxc = XCEnergyVxcFn.apply(D, params, plan)
             J = CoulombFn.apply(D, molecule)
-            energy = exc + NonXCFn.apply(D, J, molecule, a0)
-            fock = FockAssembleFn.apply(J, vxc, molecule, a0)
+            K = ExchangeFn.apply(D, molecule) if current else None
+            energy = exc + NonXCFn.apply(D, J, K, molecule, a0)
+            fock = FockAssembleFn.apply(J, vxc, K, molecule, a0)
             return energy, fock
         D = D.detach().contiguous()
         exc = torch.empty((), dtype=torch.float64, device=D.device)
@@ -50,8 +52,9 @@ def energy_predictor(functional: Functional, precision: str = "fp64", **kwargs) 
         plan.forward(D, p, exc)
         plan.backward(D, p, one, vxc, None, plan.generation)
         J = coulomb(molecule, D)
-        energy = exc + non_xc_energy(molecule, D, J, exact_exchange=a0)
-        fock = fock_assemble(molecule, J, vxc, exact_exchange=a0)
+        K = exchange(molecule, D) if current else None
+        energy = exc + non_xc_energy(molecule, D, J, exact_exchange=a0, K=K)
+        fock = fock_assemble(molecule, J, vxc, exact_exchange=a0, K=K)
         return energy, fock
 
     return predict

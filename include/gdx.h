@@ -203,9 +203,11 @@ int gdx_nonxc_energy(void* stream, int32_t n, const double* rdm1, const double* 
 int gdx_fock_assemble(void* stream, int32_t n, const double* h1e, const double* J, const double* vxc,
                       const double* K0, double a0, double* fock);
 
-/* Exchange K_s[a][c] = sum_bd (ab|cd) D_s[b][d] for nspin densities (nspin, n, n): the
- * exact-exchange term of B3LYP from the initial density (stands in for the semi-numerical
- * chi term, grad_dft/molecule.py:507-613, interface/pyscf.py:1067-1126). */
+/* Exchange K_s[a][c] = sum_bd (ab|cd) D_s[b][d] for nspin matrices (nspin, n, n), D general
+ * (not necessarily symmetric): the exact-exchange term of B3LYP from the initial density
+ * (stands in for the semi-numerical chi term, grad_dft/molecule.py:507-613,
+ * interface/pyscf.py:1067-1126), or from the current one.  Each element is summed in a fixed
+ * order (reproducible bit for bit); K must not alias D. */
 int gdx_exchange_packed(void* stream, int32_t n, int32_t nspin, const double* eri, const double* D, double* K);
 
 /* S = L L^T and Linv = L^-1 (the cholesky / inv of safe_general_eigh,

@@ -227,8 +227,12 @@ class SCFState:
 
 
 def diff_scf_loop(xc_value_and_vxc: Callable, inp: SCFInputs, rdm1: np.ndarray, mo_occ: np.ndarray,
-                  cycles: int) -> SCFState:
-    """evaluate.py:938-1038."""
+                  cycles: int, exchange_update: Optional[Callable] = None) -> SCFState:
+    """evaluate.py:938-1038.  exchange_update(D) -> K, when given, rebuilds the exact-exchange
+    matrices from every density before it is predicted at (the "current" exchange of the
+    device loop; the reference's chi stays at the initial density)."""
+    if exchange_update is not None:
+        inp = dataclasses.replace(inp, K0=exchange_update(rdm1))
     e, fock = predict(xc_value_and_vxc, inp, rdm1)
     diis = JittableDiis(inp.s1e, 10)
     data = diis.zeros(rdm1.shape[-1])
@@ -239,6 +243,8 @@ def diff_scf_loop(xc_value_and_vxc: Callable, inp: SCFInputs, rdm1: np.ndarray, 
         w, C = safe_fock_solver(F, inp.s1e)
         occ = get_occ(w, nelecs)
         D = make_rdm1(C, occ)
+        if exchange_update is not None:
+            inp = dataclasses.replace(inp, K0=exchange_update(D))
         e, fock = predict(xc_value_and_vxc, inp, D)
         st = SCFState(D, fock, e, w, C, occ, st.energies + [e])
     return st

@@ -102,11 +102,15 @@ class TorchSCFInputs:
     e_nuc: float
     K0: Optional[torch.Tensor]
     a0: float = 0.2
+    # exact exchange rebuilt from the density being predicted at: K_s = sum_bd (ab|cd) D_s[b][d]
+    # on the tape (the device's "current" exchange); False keeps K0
+    current_exchange: bool = False
 
     @staticmethod
-    def from_numpy(inp) -> "TorchSCFInputs":
+    def from_numpy(inp, current_exchange: bool = False) -> "TorchSCFInputs":
         t = lambda a: None if a is None else torch.tensor(a, dtype=DT)  # noqa: E731
-        return TorchSCFInputs(t(inp.h1e), t(inp.s1e), t(inp.eri), float(inp.e_nuc), t(inp.K0), inp.a0)
+        return TorchSCFInputs(t(inp.h1e), t(inp.s1e), t(inp.eri), float(inp.e_nuc), t(inp.K0), inp.a0,
+                              current_exchange)
 
 
 def predict(xc_energy: Callable, inp: TorchSCFInputs, rdm1: torch.Tensor):
@@ -126,8 +130,9 @@ def predict(xc_energy: Callable, inp: TorchSCFInputs, rdm1: torch.Tensor):
     fock = 0.5 * (fock + fock.transpose(1, 2))
     fock = abs_clip(fock)
     if inp.K0 is not None:
-        energy = energy - 0.5 * inp.a0 * (rdm1 * inp.K0).sum()
-        v = -0.5 * inp.a0 * inp.K0
+        K = torch.einsum("abcd,sbd->sac", inp.eri, rdm1) if inp.current_exchange else inp.K0
+        energy = energy - 0.5 * inp.a0 * (rdm1 * K).sum()
+        v = -0.5 * inp.a0 * K
         fock = abs_clip(fock + v + v.transpose(1, 2))
     fock = abs_clip(fock)
     return energy, fock
