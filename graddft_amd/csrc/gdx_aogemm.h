@@ -392,7 +392,7 @@ __device__ __forceinline__ int vxc_batch_find(const VxcBatchMol* __restrict__ mo
 }
 
 template <typename T, bool SAME, int VT, bool FLUSH = false>
-__global__ __launch_bounds__(256, (FLUSH && SAME) ? 2 : 1) void vxc_ao_batch_kernel(
+__global__ __launch_bounds__(256, VT == 64 ? 4 : ((FLUSH && SAME) ? 2 : 1)) void vxc_ao_batch_kernel(
     const VxcBatchMol* __restrict__ mols, int nmol, const double* __restrict__ vpt, size_t vpt_rows, int nsv) {
   const int m = vxc_batch_find(mols, nmol, blockIdx.x);
   const VxcBatchMol& M = mols[m];

@@ -1063,6 +1063,8 @@ __global__ void reduce_vxc_batch_kernel(const RedBatchMol* __restrict__ mols, in
 }
 
 constexpr int VXB_CHUNK = 2048;   // points per V_xc block of a batch (multiple of VX_P)
+// V_xc group of a molecule in a batch by its padded basis size (see BatchCtx::d_vx)
+inline int vxb_group(int nP) { return nP <= 64 ? 0 : (nP <= 128 ? 1 : 2); }
 
 struct BatchCtx {
   int nmol = 0;
@@ -1073,8 +1075,10 @@ struct BatchCtx {
   int pad_blocks = 0;
   void* Dp = nullptr;                        // padded matrices (sum of 2 nP^2 elements)
   void* Dp_t = nullptr;
-  const VxcBatchMol* d_vx[2] = {nullptr, nullptr};   // single-tile (SAME) / multi-tile groups
-  int nvx[2] = {0, 0}, vx_blocks[2] = {0, 0};
+  // V_xc groups by padded basis size: <= 64 (one tile of 64), <= 128 (one tile of 128),
+  // larger (tiles of 64: no more padding than 128 tiles up to 256, four blocks per CU)
+  const VxcBatchMol* d_vx[3] = {nullptr, nullptr, nullptr};
+  int nvx[3] = {0, 0, 0}, vx_blocks[3] = {0, 0, 0};
   const RedBatchMol* d_red = nullptr;
   int red_blocks = 0;
   const int* seg256 = nullptr;               // (nmol + 1) segment starts in 256-row blocks
@@ -1114,23 +1118,25 @@ void launch_vxc_batch(const Ctx& c, int nsv, double* d_rdm1) {
   const char* nm = tname<T>("vxc_f64", "vxc_f32");
   const double* vpt = c.P<double>(c.L.vpt);
   const size_t rows = (size_t)c.d.npts_pad;
-  for (int g = 0; g < 2; ++g) {
+  for (int g = 0; g < 3; ++g) {
     if (!B.nvx[g]) continue;
     const dim3 grid(B.vx_blocks[g], nsv);
+    const VxcBatchMol* t = B.d_vx[g];
+    const int nv = B.nvx[g];
     if constexpr (sizeof(T) == 8) {
       if (g == 0)
-        GDX_LAUNCH(nm, (vxc_ao_batch_kernel<double, true, 128>), grid, dim3(NT), 0, st, B.d_vx[g], B.nvx[g], vpt,
-                   rows, nsv);
+        GDX_LAUNCH(nm, (vxc_ao_batch_kernel<double, true, 64>), grid, dim3(NT), 0, st, t, nv, vpt, rows, nsv);
+      else if (g == 1)
+        GDX_LAUNCH(nm, (vxc_ao_batch_kernel<double, true, 128>), grid, dim3(NT), 0, st, t, nv, vpt, rows, nsv);
       else
-        GDX_LAUNCH(nm, (vxc_ao_batch_kernel<double, false, 128>), grid, dim3(NT), 0, st, B.d_vx[g], B.nvx[g], vpt,
-                   rows, nsv);
+        GDX_LAUNCH(nm, (vxc_ao_batch_kernel<double, false, 64>), grid, dim3(NT), 0, st, t, nv, vpt, rows, nsv);
     } else {
       if (g == 0)
-        GDX_LAUNCH(nm, (vxc_ao_batch_kernel<float, true, 128, true>), grid, dim3(NT), 0, st, B.d_vx[g], B.nvx[g],
-                   vpt, rows, nsv);
+        GDX_LAUNCH(nm, (vxc_ao_batch_kernel<float, true, 64, true>), grid, dim3(NT), 0, st, t, nv, vpt, rows, nsv);
+      else if (g == 1)
+        GDX_LAUNCH(nm, (vxc_ao_batch_kernel<float, true, 128, true>), grid, dim3(NT), 0, st, t, nv, vpt, rows, nsv);
       else
-        GDX_LAUNCH(nm, (vxc_ao_batch_kernel<float, false, 128, true>), grid, dim3(NT), 0, st, B.d_vx[g], B.nvx[g],
-                   vpt, rows, nsv);
+        GDX_LAUNCH(nm, (vxc_ao_batch_kernel<float, false, 64, true>), grid, dim3(NT), 0, st, t, nv, vpt, rows, nsv);
     }
   }
   GDX_LAUNCH("reduce_vxc", reduce_vxc_batch_kernel, dim3(B.red_blocks), dim3(NT), 0, st, B.d_red, B.nmol, nsv,
@@ -1798,11 +1804,11 @@ int gdx_xc_batch_create(const gdx_xc_desc* desc, int32_t nmol, const gdx_xc_batc
   // host tables
   std::vector<AoBatchMol> ao(nmol), ao_t(nmol);
   std::vector<PadBatchMol> pad(nmol);
-  std::vector<VxcBatchMol> vx[2];
+  std::vector<VxcBatchMol> vx[3];
   std::vector<RedBatchMol> red(nmol);
   std::vector<int> seg256(nmol + 1), seg64(nmol + 1), rowv(nmol + 1);
   long row = 0, dm = 0, dp = 0, vp = 0;
-  int tile = 0, pblk = 0, rblk = 0, vblk[2] = {0, 0};
+  int tile = 0, pblk = 0, rblk = 0, vblk[3] = {0, 0, 0};
   P->dm_off.resize(nmol + 1);
   std::vector<long> vpart_off(nmol);
   for (int m = 0; m < nmol; ++m) {
@@ -1824,7 +1830,9 @@ int gdx_xc_batch_create(const gdx_xc_desc* desc, int32_t nmol, const gdx_xc_batc
     pad[m].nP = nP;
     pad[m].blk0 = pblk;
     pad[m].nblk = (int)((2L * nP * nP + NT - 1) / NT);
-    const int nt = (nP + 127) / 128;
+    const int g = vxb_group(nP);
+    const int vt = g == 1 ? 128 : 64;
+    const int nt = (nP + vt - 1) / vt;
     VxcBatchMol v{};
     v.ao = M.ao;
     v.row0 = row;
@@ -1833,7 +1841,6 @@ int gdx_xc_batch_create(const gdx_xc_desc* desc, int32_t nmol, const gdx_xc_batc
     v.ntile = nt * nt;
     v.chunk = VXB_CHUNK;
     v.nchunk = (M.npts_pad + VXB_CHUNK - 1) / VXB_CHUNK;
-    const int g = nt == 1 ? 0 : 1;
     v.blk0 = vblk[g];
     vblk[g] += v.ntile * v.nchunk;
     vpart_off[m] = vp;
@@ -1860,7 +1867,8 @@ int gdx_xc_batch_create(const gdx_xc_desc* desc, int32_t nmol, const gdx_xc_batc
   auto take = [&](size_t bytes) { size_t o = off; off += round_up(std::max<size_t>(bytes, 8), 256); return o; };
   const size_t o_ao = take(nmol * sizeof(AoBatchMol)), o_aot = take(nmol * sizeof(AoBatchMol));
   const size_t o_pad = take(nmol * sizeof(PadBatchMol));
-  const size_t o_vx0 = take(vx[0].size() * sizeof(VxcBatchMol)), o_vx1 = take(vx[1].size() * sizeof(VxcBatchMol));
+  const size_t o_vx0 = take(vx[0].size() * sizeof(VxcBatchMol)), o_vx1 = take(vx[1].size() * sizeof(VxcBatchMol)),
+               o_vx2 = take(vx[2].size() * sizeof(VxcBatchMol));
   const size_t o_red = take(nmol * sizeof(RedBatchMol));
   const size_t o_s256 = take((nmol + 1) * sizeof(int)), o_s64 = take((nmol + 1) * sizeof(int));
   const size_t o_rows = take((nmol + 1) * sizeof(int));
@@ -1889,9 +1897,9 @@ int gdx_xc_batch_create(const gdx_xc_desc* desc, int32_t nmol, const gdx_xc_batc
     red[m].vpart = (const double*)(T0 + o_vp) + vpart_off[m];
   }
   {
-    int k[2] = {0, 0};
+    int k[3] = {0, 0, 0};
     for (int m = 0; m < nmol; ++m) {
-      const int g = ((mols[m].nao_pad + 127) / 128) == 1 ? 0 : 1;
+      const int g = vxb_group(mols[m].nao_pad);
       vx[g][k[g]++].vpart = (double*)(T0 + o_vp) + vpart_off[m];
     }
   }
@@ -1901,6 +1909,7 @@ int gdx_xc_batch_create(const gdx_xc_desc* desc, int32_t nmol, const gdx_xc_batc
                     {o_pad, pad.data(), nmol * sizeof(PadBatchMol)},
                     {o_vx0, vx[0].data(), vx[0].size() * sizeof(VxcBatchMol)},
                     {o_vx1, vx[1].data(), vx[1].size() * sizeof(VxcBatchMol)},
+                    {o_vx2, vx[2].data(), vx[2].size() * sizeof(VxcBatchMol)},
                     {o_red, red.data(), nmol * sizeof(RedBatchMol)},
                     {o_s256, seg256.data(), (nmol + 1) * sizeof(int)}, {o_s64, seg64.data(), (nmol + 1) * sizeof(int)},
                     {o_rows, rowv.data(), (nmol + 1) * sizeof(int)}, {o_one, &one, 8}};
@@ -1923,12 +1932,12 @@ int gdx_xc_batch_create(const gdx_xc_desc* desc, int32_t nmol, const gdx_xc_batc
   B.pad_blocks = pblk;
   B.Dp = T0 + o_dp;
   B.Dp_t = T0 + o_dpt;
-  B.d_vx[0] = (const VxcBatchMol*)(T0 + o_vx0);
-  B.d_vx[1] = (const VxcBatchMol*)(T0 + o_vx1);
-  B.nvx[0] = (int)vx[0].size();
-  B.nvx[1] = (int)vx[1].size();
-  B.vx_blocks[0] = vblk[0];
-  B.vx_blocks[1] = vblk[1];
+  const size_t o_vx[3] = {o_vx0, o_vx1, o_vx2};
+  for (int g = 0; g < 3; ++g) {
+    B.d_vx[g] = (const VxcBatchMol*)(T0 + o_vx[g]);
+    B.nvx[g] = (int)vx[g].size();
+    B.vx_blocks[g] = vblk[g];
+  }
   B.d_red = (const RedBatchMol*)(T0 + o_red);
   B.red_blocks = rblk;
   B.seg256 = (const int*)(T0 + o_s256);

@@ -1,7 +1,8 @@
 """The molecule batch (gdx_xc_batch_*: concatenated grid rows, one MLP pass, ragged AO-side
 launches) against the same molecules evaluated one at a time, on the GPU.
 
-Molecules of different bases and grids (CH4 / C2H6 def2-SVP, H2O def2-TZVP, level-1 grids):
+Molecules of different bases and grids (CH4 / C2H6 / C6H14 def2-SVP, H2O def2-TZVP, level-1
+grids; padded bases of 48, 48, 64 and 160, which take the three V_xc tile groups):
 E_xc per molecule, dE/dD per molecule and sum_m g_m dE_m/dtheta for per-molecule cotangents;
 the second-order pass with per-molecule directions; the differentiable SCF loop with the
 batched XC (energies and the parameter gradient of their sum); the batched mse_energy_loss.
@@ -30,7 +31,7 @@ def _mols(with_eri=False):
     rng = np.random.default_rng(9)
     out = []
     for sym_xyz, basis in ((geometry.alkane(rng, 1), "def2-svp"), (geometry.water(rng), "def2-tzvp"),
-                           (geometry.alkane(rng, 2), "def2-svp")):
+                           (geometry.alkane(rng, 2), "def2-svp"), (geometry.alkane(rng, 6), "def2-svp")):
         sym, xyz = sym_xyz
         out.append(molecule_from_geometry(sym, xyz, basis, 1, with_eri=with_eri, npts_align=256))
     return out
@@ -47,7 +48,7 @@ def _functional(kind):
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("kind", ["b3lyp", "neural"])
 def test_batch_xc_matches_per_molecule(kind, precision):
-    _say(f"case: batch XC vs per molecule, {kind}, {precision}, CH4/H2O/C2H6")
+    _say(f"case: batch XC vs per molecule, {kind}, {precision}, CH4/H2O/C2H6/C6H14")
     from graddft_amd.batch import XCBatchPlan
     f, theta = _functional(kind)
     mols = _mols()
@@ -57,7 +58,7 @@ def test_batch_xc_matches_per_molecule(kind, precision):
     D = plan.concat([m.rdm1 for m in mols])
     exc = torch.empty(len(mols), dtype=torch.float64, device="cuda")
     vxc = torch.empty_like(D)
-    g = torch.tensor([0.3, -1.2, 0.7], dtype=torch.float64, device="cuda")
+    g = torch.tensor([0.3, -1.2, 0.7, 0.45], dtype=torch.float64, device="cuda")
     dth = None if theta is None else torch.empty_like(theta)
     plan.forward(D, theta, exc)
     plan.backward(theta, g, vxc, dth)
@@ -94,7 +95,7 @@ def test_batch_hvp_matches_per_molecule(precision):
     plan = XCBatchPlan(f, mols, precision)
     rng = np.random.default_rng(5)
     dirs = [torch.as_tensor(rng.normal(size=(2, m.nao, m.nao)) * 1e-2, device="cuda") for m in mols]
-    g = torch.tensor([0.5, 1.5, -0.25], dtype=torch.float64, device="cuda")
+    g = torch.tensor([0.5, 1.5, -0.25, 0.8], dtype=torch.float64, device="cuda")
     D = plan.concat([m.rdm1 for m in mols])
     exc = torch.empty(len(mols), dtype=torch.float64, device="cuda")
     edot = torch.empty_like(exc)
