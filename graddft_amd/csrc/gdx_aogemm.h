@@ -36,12 +36,11 @@ template <typename T> struct DensCB { static constexpr int value = 64; };
 
 // One 64-point tile (bx) of one spin (s): ao (5, npts_pad, nP) planes of one molecule, Dp its
 // padded density (nsd, nP, nP); raw sums to dens_part[(s * out_rows + p) * 6 + v].
-template <typename T>
+template <typename T, int CB = DensCB<T>::value>
 __device__ __forceinline__ void density_ao_body(const T* __restrict__ ao, int npts_pad, int nP,
                                                 const T* __restrict__ Dp, double* __restrict__ dens_part,
                                                 size_t out_rows, int bx, int s) {
   typedef typename MfmaT<T>::acc_t acc_t;
-  constexpr int CB = DensCB<T>::value;
   constexpr int NJ = CB / 16;
   constexpr int VEC = 16 / sizeof(T);
   constexpr int KV = DN_K / VEC;                           // vectors per 16-k row segment
