@@ -60,6 +60,7 @@ def parse():
     ap.add_argument("--no-train-batch", action="store_true", help="skip the batched training step")
     ap.add_argument("--no-eri", action="store_true", help="skip the device ERI generation measurement")
     ap.add_argument("--no-config4", action="store_true", help="skip the naphthalene dense-ERI SCF workload")
+    ap.add_argument("--no-config5", action="store_true", help="skip the 40-carbon grid-XC workload")
     ap.add_argument("--batch-size", type=int, default=64, help="alkanes in the batched training step")
     ap.add_argument("--batch-max-nc", type=int, default=8, help="largest n_C of the batched training step")
     ap.add_argument("--no-graph", action="store_true", help="time the eager step instead of graph replays")
@@ -306,6 +307,91 @@ def config4_bench(args):
         out["abs_dE_vs_fp64_Ha"] = abs(e1 - e64)
         out["grad_rel_err_vs_fp64"] = float(np.linalg.norm(g1 - g64) / np.linalg.norm(g64))
         out["speedup_ms_per_iter"] = round(out["fp64"]["ms_per_iter"] / out[args.precision]["ms_per_iter"], 3)
+    return out
+
+
+def config5_bench(args):
+    """40 carbon atoms at seeded positions (12 A cube), def2-SVP (nao 560), level-2 grid, D from
+    120 seeded S-orthonormal orbitals, the 3x256 neural functional: one E_xc + dE/dD + dE/dtheta
+    evaluation (forward + adjoint, eager, mean of 3 after one warm-up) in FP64, FP32 and split-BF16,
+    with the differences against FP64 and the step's MFMA work.  Also the input build of this
+    molecule: Becke weights and S, T, V on the device against the host paths."""
+    import torch
+    from graddft_amd import MLPSpec, NeuralFunctional
+    from graddft_amd.interface import geometry
+    from graddft_amd.interface.basis import atomic_numbers, build_basis
+    from graddft_amd.interface.builder import molecule_from_geometry
+    from graddft_amd.interface.grids import molecular_grid
+    from graddft_amd.interface.integrals import int1e, int1e_device
+    rng = np.random.default_rng(5)
+    sym, xyz = geometry.carbon_cloud(rng)
+    Z = atomic_numbers(sym)
+    bs = build_basis(sym, xyz, "def2-svp")
+    level = 2
+    inputs = {}
+    t0 = time.perf_counter()
+    molecular_grid(Z, xyz, level, weights_on="device")
+    inputs["grid_device_s"] = round(time.perf_counter() - t0, 3)
+    t0 = time.perf_counter()
+    molecular_grid(Z, xyz, level)
+    inputs["grid_host_s"] = round(time.perf_counter() - t0, 3)
+    int1e_device(bs, Z, xyz)
+    t0 = time.perf_counter()
+    S_d, _, _ = int1e_device(bs, Z, xyz)
+    inputs["int1e_device_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    t0 = time.perf_counter()
+    S, _, _ = int1e(bs, Z, xyz)
+    inputs["int1e_host_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    inputs["host_threads"] = torch.get_num_threads()
+    C = rng.normal(size=(bs.nao, 120))
+    w, V = np.linalg.eigh(C.T @ S @ C)
+    C = C @ V @ np.diag(w ** -0.5) @ V.T
+    t0 = time.perf_counter()
+    mol = molecule_from_geometry(sym, xyz, "def2-svp", level, rdm1=C @ C.T, name="carbon40")
+    torch.cuda.synchronize()
+    inputs["molecule_build_s"] = round(time.perf_counter() - t0, 2)
+    spec = MLPSpec(nc=6, h=256, n_hidden=3, nd=4)
+    f = NeuralFunctional(spec)
+    theta = f.init(seed=6)
+    N, n = mol.grid_size, mol.nao
+    nsd = 1 if mol.restricted else 2
+    w_pt = 8 * n * n * nsd + 8 * n * n * 2 + 3 * spec.flops_per_point()
+    out = {"workload": f"40 C (seeded, 12 A cube) def2-SVP nao={n}, grid level {level} ({N} pts), D from 120 "
+                       "orbitals, MLP 3x256: E_xc + dE/dD + dE/dtheta", "npts": N, "nao": n, "inputs": inputs,
+           "flop_per_point": w_pt}
+    D = mol.rdm1.contiguous()
+    one = torch.ones((), dtype=torch.float64, device="cuda")
+    res = {}
+    reps = 3
+    for prec in ("fp64", "fp32", "bf16x6"):
+        plan = f.plan(mol, prec)
+        exc = torch.empty((), dtype=torch.float64, device="cuda")
+        dD = torch.empty_like(D)
+        dth = torch.empty_like(theta)
+        plan.forward(D, theta, exc)
+        plan.backward(D, theta, one, dD, dth, plan.generation)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            plan.forward(D, theta, exc)
+            plan.backward(D, theta, one, dD, dth, plan.generation)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        # split-BF16 mixes bf16 products (hidden layers) with FP32 ones: no single peak
+        peak = PEAK[prec][1] if prec != "bf16x6" else None
+        tf = w_pt * N / (ms / 1e3) / 1e12
+        res[prec] = (float(exc), dD.clone(), dth.clone())
+        out[prec] = {"ms_per_step": round(ms, 3), "grid_pts_per_s": round(N / (ms / 1e3), 1),
+                     "algorithmic_tflops": round(tf, 2), "peak": peak,
+                     "frac": round(tf / peak, 4) if peak else None}
+        del plan
+    e64, d64, t64 = res["fp64"]
+    rel = lambda a, b: float(torch.linalg.norm(a - b) / torch.linalg.norm(b))  # noqa: E731
+    for prec in ("fp32", "bf16x6"):
+        e1, d1, t1 = res[prec]
+        out[prec].update({"abs_dE_vs_fp64_Ha": abs(e1 - e64), "rel_err_dE_dD": rel(d1, d64),
+                          "rel_err_dE_dtheta": rel(t1, t64),
+                          "speedup_vs_fp64": round(out["fp64"]["ms_per_step"] / out[prec]["ms_per_step"], 3)})
     return out
 
 
@@ -664,6 +750,7 @@ def main():
     train_batch = section(train_batch_bench, args.no_train_batch or args.no_train)
     eri = section(eri_bench, args.no_eri)
     config4 = section(config4_bench, args.no_config4 or args.no_scf)
+    config5 = section(config5_bench, args.no_config5)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -707,6 +794,7 @@ def main():
         "train_batch": train_batch,
         "eri": eri,
         "config4": config4,
+        "config5": config5,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

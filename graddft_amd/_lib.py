@@ -67,6 +67,10 @@ SIGNATURES = {
     "gdx_eri_plan_destroy": (ctypes.c_int, [_P]),
     "gdx_eri_plan_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     "gdx_eri_packed": (ctypes.c_int, [_P, _P, _P]),
+    "gdx_int1e_plan_create": (ctypes.c_int, [_I, _P, _P, _P, _P, _I, _I, _I, _P, _P, ctypes.POINTER(_P)]),
+    "gdx_int1e_plan_destroy": (ctypes.c_int, [_P]),
+    "gdx_int1e_eval": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "gdx_becke_weights": (ctypes.c_int, [_P, _I, _P, _P, _P, _I, _P, _P, _P, _P]),
     "gdx_hf_plan_create": (ctypes.c_int, [_I, _P, _P, _P, _P, _I, ctypes.c_double, ctypes.POINTER(_P)]),
     "gdx_hf_plan_destroy": (ctypes.c_int, [_P]),
     "gdx_hf_plan_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),

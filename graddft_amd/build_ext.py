@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 
-HIP_SOURCES = ["gdx_ao.hip", "gdx_xc.hip", "gdx_scf.hip", "gdx_hf.hip", "gdx_eri.hip"]
+HIP_SOURCES = ["gdx_ao.hip", "gdx_xc.hip", "gdx_scf.hip", "gdx_hf.hip", "gdx_eri.hip", "gdx_int1e.hip",
+               "gdx_becke.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

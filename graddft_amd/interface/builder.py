@@ -3,8 +3,10 @@
 Plays the role of ``molecule_from_pyscf`` (``grad_dft/interface/pyscf.py:105-201``) and
 its packaging step ``_package_outputs`` (``:808-864``) without PySCF:
 
-* basis shells from :mod:`.basis`, integration grid from :mod:`.grids`;
-* S, T, V (and optionally the packed ERI) from the host integral library;
+* basis shells from :mod:`.basis`, integration grid from :mod:`.grids` (atomic rules on the
+  host, Becke partition weights on the device by default);
+* S, T, V and, optionally, the packed ERI on the device by default (``gdx_int1e_eval``,
+  ``gdx_eri_packed``), or from the host integral library;
 * the initial density from the core-Hamiltonian guess (lowest ``nelec/2`` orbitals of
   ``h1e C = S C e``), split into two equal spin halves exactly as the reference splits an
   RKS density (``pyscf.py:830-838``: ``rdm1 / 2`` per spin, ``mo_occ / 2``);
@@ -22,7 +24,7 @@ from .. import _lib
 from ..molecule import Grid, Molecule
 from .basis import atomic_numbers, build_basis, nuclear_repulsion, pack_shells
 from .grids import molecular_grid
-from .integrals import int1e, int2e_packed, int2e_packed_device
+from .integrals import int1e, int1e_device, int2e_packed, int2e_packed_device
 
 
 def pad_to(x: int, m: int) -> int:
@@ -57,18 +59,27 @@ def molecule_from_geometry(symbols: Sequence[str], coords_bohr: np.ndarray, basi
                            grid_level: int = 2, charge: int = 0, with_eri: bool = False,
                            device: str = "cuda", rdm1: Optional[np.ndarray] = None,
                            name: str = "", npts_align: int = 128,
-                           omegas: Optional[Sequence[float]] = None, eri_on: str = "device") -> Molecule:
+                           omegas: Optional[Sequence[float]] = None, eri_on: str = "device",
+                           int1e_on: str = "device", grid_on: str = "device") -> Molecule:
     """Closed-shell molecule with a core-Hamiltonian (or given) density, AOs on device.  With
     `omegas` the chi tensor of that density is built on the device, as ``molecule_from_pyscf``
     does when given omegas (pyscf.py:152-160); B3LYP's exact exchange then uses it.
     with_eri: the packed ERI, computed on the device (eri_on="device", shells up to d) or by the
-    host integral library (eri_on="host", and for bases with f shells)."""
+    host integral library (eri_on="host", and for bases with f shells).  int1e_on / grid_on:
+    "device" (S, T, V and the Becke weights by the HIP kernels) or "host"."""
+    for option, v in (("eri_on", eri_on), ("int1e_on", int1e_on), ("grid_on", grid_on)):
+        if v not in ("host", "device"):
+            raise ValueError(f"{option} must be 'host' or 'device', not {v!r}")
     coords_bohr = np.asarray(coords_bohr, dtype=np.float64)
     Z = atomic_numbers(symbols)
     bs = build_basis(symbols, coords_bohr, basis)
-    gc, gw = molecular_grid(Z, coords_bohr, grid_level)
-    S, T, V = int1e(bs, Z, coords_bohr)
-    h1e = T + V
+    gc, gw = molecular_grid(Z, coords_bohr, grid_level, weights_on=grid_on)
+    if int1e_on == "device":
+        S_d, T_d, V_d = int1e_device(bs, Z, coords_bohr, device)
+        S, h1e = S_d.cpu().numpy(), (T_d + V_d).cpu().numpy()
+    else:
+        S, T, V = int1e(bs, Z, coords_bohr)
+        h1e = T + V
     nelec = sum(Z) - charge
     if nelec % 2:
         raise ValueError("only closed-shell molecules are supported")

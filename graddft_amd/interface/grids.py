@@ -14,7 +14,9 @@ module builds grids with the same published ingredients PySCF's defaults use:
   points whose weight is below 1e-15.
 
 Point sets are *not* bit-identical to PySCF's; the workloads only need grids of the
-stated level and size class.
+stated level and size class.  ``molecular_grid(..., weights_on="device")`` evaluates the Becke
+partition (the O(points x atoms^2) part) with the ``gdx_becke_weights`` kernel; the atomic rules
+stay on the host.
 """
 from __future__ import annotations
 
@@ -110,12 +112,8 @@ def becke_weights(coords: np.ndarray, owner: int, atom_xyz: np.ndarray,
     natm = len(atom_xyz)
     if natm == 1:
         return np.ones(len(coords))
-    rad = np.sqrt(BRAGG_BOHR[np.asarray(charges)]) + 1e-200
-    rr = rad[:, None] / rad[None, :]
-    a = 0.25 * (rr.T - rr)
-    a = np.clip(a, -0.5, 0.5)
+    adist, a = becke_adjustment(charges, atom_xyz)
     dist = np.linalg.norm(coords[None, :, :] - atom_xyz[:, None, :], axis=2)
-    adist = np.linalg.norm(atom_xyz[:, None, :] - atom_xyz[None, :, :], axis=2)
     p = np.ones((natm, len(coords)))
     for i in range(natm):
         for j in range(i):
@@ -128,18 +126,52 @@ def becke_weights(coords: np.ndarray, owner: int, atom_xyz: np.ndarray,
     return p[owner] / p.sum(axis=0)
 
 
+def becke_adjustment(charges: Sequence[int], atom_xyz: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    """Inter-atomic distances |R_i - R_j| and the size adjustment a_ij of the Becke partition."""
+    rad = np.sqrt(BRAGG_BOHR[np.asarray(charges)]) + 1e-200
+    rr = rad[:, None] / rad[None, :]
+    a = np.clip(0.25 * (rr.T - rr), -0.5, 0.5)
+    adist = np.linalg.norm(atom_xyz[:, None, :] - atom_xyz[None, :, :], axis=2)
+    return adist, a
+
+
+def becke_weights_device(coords: np.ndarray, owner: np.ndarray, vol: np.ndarray, atom_xyz: np.ndarray,
+                         charges: Sequence[int], device: str = "cuda") -> np.ndarray:
+    """vol * Becke cell function of each point's owner atom, evaluated by ``gdx_becke_weights``."""
+    import torch
+    from .. import _lib
+    lib = _lib.load()
+    natm = len(atom_xyz)
+    adist, a = becke_adjustment(charges, atom_xyz)
+    dev = lambda x, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(x), dtype=dt, device=device)  # noqa
+    c, o, v = dev(coords), dev(owner, torch.int32), dev(vol)
+    at, ad, aa = dev(atom_xyz), dev(adist), dev(a)
+    w = torch.empty(len(coords), dtype=torch.float64, device=device)
+    st = torch.cuda.current_stream(w.device).cuda_stream
+    _lib.check(lib.gdx_becke_weights(st, len(coords), c.data_ptr(), o.data_ptr(), v.data_ptr(), natm, at.data_ptr(),
+                                     ad.data_ptr(), aa.data_ptr(), w.data_ptr()), "gdx_becke_weights")
+    return w.cpu().numpy()
+
+
 def molecular_grid(charges: Sequence[int], atom_xyz_bohr: np.ndarray, level: int,
-                   cutoff: float = 1e-15) -> Tuple[np.ndarray, np.ndarray]:
-    """All atoms' points with Becke-partitioned weights; tiny weights removed."""
+                   cutoff: float = 1e-15, weights_on: str = "host") -> Tuple[np.ndarray, np.ndarray]:
+    """All atoms' points with Becke-partitioned weights; tiny weights removed.  weights_on:
+    "host" (numpy) or "device" (the gdx_becke_weights kernel; same formulas, same order)."""
+    if weights_on not in ("host", "device"):
+        raise ValueError(f"weights_on must be 'host' or 'device', not {weights_on!r}")
     atom_xyz = np.asarray(atom_xyz_bohr, dtype=np.float64)
-    cs, ws = [], []
+    cs, vs, owners = [], [], []
     for ia, z in enumerate(charges):
         c, v = atomic_grid(int(z), level)
-        c = c + atom_xyz[ia]
-        w = v * becke_weights(c, ia, atom_xyz, charges)
-        cs.append(c)
-        ws.append(w)
-    coords = np.concatenate(cs)
-    weights = np.concatenate(ws)
+        cs.append(c + atom_xyz[ia])
+        vs.append(v)
+        owners.append(np.full(len(v), ia, dtype=np.int32))
+    if weights_on == "device":
+        coords = np.concatenate(cs)
+        weights = becke_weights_device(coords, np.concatenate(owners), np.concatenate(vs), atom_xyz, charges)
+    else:
+        ws = [v * becke_weights(c, ia, atom_xyz, charges) for ia, (c, v) in enumerate(zip(cs, vs))]
+        coords = np.concatenate(cs)
+        weights = np.concatenate(ws)
     keep = np.abs(weights) > cutoff
     return np.ascontiguousarray(coords[keep]), np.ascontiguousarray(weights[keep])

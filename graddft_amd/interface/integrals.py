@@ -2,8 +2,9 @@
 
 Stands in for PySCF's ``mf.get_ovlp`` / ``mf.get_hcore`` / ``mol.intor('int2e')``
 (``grad_dft/interface/pyscf.py:797, 828-829``).  The library is built from
-``graddft_amd/csrc/gdx_ints.cpp`` by :mod:`graddft_amd.build_ext`.  ``int2e_packed_device``
-computes the packed ERI on the GPU instead (bases with shells up to d).
+``graddft_amd/csrc/gdx_ints.cpp`` by :mod:`graddft_amd.build_ext`.  ``int1e_device`` and
+``int2e_packed_device`` compute S, T, V and the packed ERI on the GPU instead (through
+``libgdx.so``: shells up to f for S, T, V, up to d for the ERI).
 """
 from __future__ import annotations
 
@@ -91,6 +92,33 @@ def int2e_packed_device(bs: BasisSet, device: str = "cuda", screen: float = 0.0)
     finally:
         lib.gdx_eri_plan_destroy(handle)
     return eri
+
+
+def int1e_device(bs: BasisSet, charges: Sequence[float], atom_xyz: np.ndarray, device: str = "cuda"):
+    """Overlap S, kinetic T and nuclear attraction V on the device (gdx_int1e_plan_create +
+    gdx_int1e_eval), each a (nao, nao) float64 torch tensor."""
+    import torch
+    from .. import _lib
+    lib = _lib.load()
+    pk = pack_shells(bs)
+    ch = np.ascontiguousarray(np.asarray(charges, dtype=np.float64))
+    xyz = np.ascontiguousarray(np.asarray(atom_xyz, dtype=np.float64).reshape(-1, 3))
+    if len(ch) != len(xyz):
+        raise ValueError("int1e_device: one charge per atom position")
+    handle = ctypes.c_void_p(0)
+    rc = lib.gdx_int1e_plan_create(len(bs.shells), _ptr(pk["shell_i32"]), _ptr(pk["shell_f64"]),
+                                   _ptr(pk["prim_f64"]), _ptr(pk["tmat_f64"]), int(pk["tmat_f64"].size), bs.nao,
+                                   len(ch), _ptr(ch), _ptr(xyz), ctypes.byref(handle))
+    _lib.check(rc, "gdx_int1e_plan_create")
+    try:
+        n = bs.nao
+        S, T, V = (torch.empty((n, n), dtype=torch.float64, device=device) for _ in range(3))
+        st = torch.cuda.current_stream(S.device).cuda_stream
+        _lib.check(lib.gdx_int1e_eval(st, handle, S.data_ptr(), T.data_ptr(), V.data_ptr()), "gdx_int1e_eval")
+        torch.cuda.current_stream(S.device).synchronize()
+    finally:
+        lib.gdx_int1e_plan_destroy(handle)
+    return S, T, V
 
 
 def unpack_eri(eri: np.ndarray, n: int) -> np.ndarray:

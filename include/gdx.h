@@ -285,6 +285,31 @@ int gdx_eri_plan_destroy(void* plan);
 int gdx_eri_plan_info(const void* plan, int64_t* nquartets, int64_t* nentries);
 int gdx_eri_packed(void* stream, const void* plan, double* eri);
 
+/* ---- One-electron integrals and grid weights on the device -----------------------------
+ * S (overlap), T (kinetic) and V (nuclear attraction, -sum_C Z_C <a|1/|r-C||b>): mf.get_ovlp()
+ * and mf.get_hcore() = T + V as read by _package_outputs, grad_dft/interface/pyscf.py:828-829.
+ * The plan takes HOST arrays in the gdx_ao_eval packing (shells up to f; ntmat = doubles in
+ * tmat_f64), the nuclear charges (natm) and positions (natm, 3; bohr), and uploads the
+ * primitive-pair data once.  gdx_int1e_eval writes every element of the device (nao, nao)
+ * outputs S, T, V (no clearing needed); results are reproducible. */
+int gdx_int1e_plan_create(int32_t nshell, const int32_t* shell_i32, const double* shell_f64, const double* prim_f64,
+                          const double* tmat_f64, int32_t ntmat, int32_t nao, int32_t natm, const double* charges,
+                          const double* atom_xyz, void** plan);
+int gdx_int1e_plan_destroy(void* plan);
+int gdx_int1e_eval(void* stream, const void* plan, double* S, double* T, double* V);
+
+/* Becke fuzzy-cell weights of an atom-centred grid: the partition behind the weights of PySCF's
+ * dft.Grids that grid_from_pyscf takes (grad_dft/interface/pyscf.py:54-60).  Device inputs:
+ * coords (npts, 3), owner (npts; the atom whose radial x angular rule made the point), vol
+ * (npts; that rule's quadrature volume), atoms (natm, 3), adist (natm, natm; |R_i - R_j|),
+ * adjust (natm, natm; the antisymmetric size adjustment a_ij).  Output (device): weights[r] =
+ * vol[r] P_owner(r) / sum_i P_i(r) with P_i = prod_{j != i} s(nu_ij), nu_ij = mu_ij +
+ * a_ij (1 - mu_ij^2), mu_ij = (|r - R_i| - |r - R_j|) / |R_i - R_j|, s = (1 - f(f(f(nu)))) / 2,
+ * f(x) = (3 - x^2) x / 2, evaluated in a fixed order without contraction (equal to the same
+ * formulas evaluated sequentially in double precision).  adist/adjust may be NULL when natm = 1. */
+int gdx_becke_weights(void* stream, int32_t npts, const double* coords, const int32_t* owner, const double* vol,
+                      int32_t natm, const double* atoms, const double* adist, const double* adjust, double* weights);
+
 /* ---- Semi-numerical exact exchange ----------------------------------------------------
  * nu_ab(r) = int dr' phi_a(r') f(|r - r'|) phi_b(r') at grid points r (f = 1/|r-r'|, or
  * erf(omega |r-r'|)/|r-r'| for omega > 0): PySCF's int1e_grids_sph as called by
