@@ -28,9 +28,12 @@ constexpr int NT = 256;
 // M the symmetric P x P matrix whose lower triangle is the packed ERI.  Tiles of
 // JT x JT elements of the lower triangle: each block forms row sums (M Dt over its
 // columns) and column sums (M^T Dt over its strictly-lower rows); partial vectors are
-// added by coulomb_finish.  Every packed element is read exactly once.
+// added by coulomb_finish.  Every packed element is read exactly once.  Tiles strictly below
+// the diagonal and inside the matrix (all but O(P / JT) of them) take a path without
+// per-element bounds checks, so a thread issues a row's loads back to back.
 constexpr int JT = 256;   // large tile (P >= 4096)
 constexpr int JTS = 64;   // small tile
+constexpr long J_ROWS_MIN = 12000;   // P from which coulomb_rows_kernel reads the tiles
 
 __global__ void pack_density_kernel(const double* __restrict__ rdm1, int n, double* __restrict__ Dt) {
   const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -48,6 +51,14 @@ __global__ void pack_density_kernel(const double* __restrict__ rdm1, int n, doub
   Dt[I] = ii == j ? dij : dij + dji;
 }
 
+__device__ __forceinline__ void coulomb_tile_of(long x, long& bi, long& bk) {
+  // triangular tile index -> (bi, bk), bk <= bi
+  bi = (long)((sqrt(8.0 * (double)x + 1.0) - 1.0) / 2.0);
+  while (bi * (bi + 1) / 2 > x) --bi;
+  while ((bi + 1) * (bi + 2) / 2 <= x) ++bi;
+  bk = x - bi * (bi + 1) / 2;
+}
+
 template <int TT>
 __global__ __launch_bounds__(NT) void coulomb_tile_kernel(const double* __restrict__ eri, long P,
                                                           const double* __restrict__ Dt,
@@ -56,12 +67,8 @@ __global__ __launch_bounds__(NT) void coulomb_tile_kernel(const double* __restri
   // columns K = K0 + 16 j + tx
   constexpr int NA = TT / 16;
   __shared__ double cs[16][TT + 1];
-  // triangular tile index -> (bi, bk), bk <= bi
-  const long x = blockIdx.x;
-  long bi = (long)((sqrt(8.0 * (double)x + 1.0) - 1.0) / 2.0);
-  while (bi * (bi + 1) / 2 > x) --bi;
-  while ((bi + 1) * (bi + 2) / 2 <= x) ++bi;
-  const long bk = x - bi * (bi + 1) / 2;
+  long bi, bk;
+  coulomb_tile_of(blockIdx.x, bi, bk);
   const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
   const long I0 = bi * TT, K0 = bk * TT;
   double dk[NA], csum[NA];
@@ -72,24 +79,44 @@ __global__ __launch_bounds__(NT) void coulomb_tile_kernel(const double* __restri
     csum[j] = 0.0;
   }
   double rsum[NA];
+  if (bk < bi && I0 + TT <= P) {
+    // every K < every I of the tile, every I < P
 #pragma unroll
-  for (int a = 0; a < NA; ++a) {
-    const long I = I0 + ty + 16 * a;
-    double r = 0.0;
-    if (I < P) {
+    for (int a = 0; a < NA; ++a) {
+      const long I = I0 + ty + 16 * a;
       const double di = Dt[I];
-      const double* row = eri + I * (I + 1) / 2;
+      const double* row = eri + I * (I + 1) / 2 + K0 + tx;
+      double m[NA];
+#pragma unroll
+      for (int j = 0; j < NA; ++j) m[j] = __builtin_nontemporal_load(row + 16 * j);
+      double r = 0.0;
 #pragma unroll
       for (int j = 0; j < NA; ++j) {
-        const long K = K0 + j * 16 + tx;
-        if (K <= I) {
-          const double m = row[K];
-          r += m * dk[j];
-          if (K < I) csum[j] += m * di;
+        r += m[j] * dk[j];
+        csum[j] += m[j] * di;
+      }
+      rsum[a] = r;
+    }
+  } else {
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      const long I = I0 + ty + 16 * a;
+      double r = 0.0;
+      if (I < P) {
+        const double di = Dt[I];
+        const double* row = eri + I * (I + 1) / 2;
+#pragma unroll
+        for (int j = 0; j < NA; ++j) {
+          const long K = K0 + j * 16 + tx;
+          if (K <= I) {
+            const double m = row[K];
+            r += m * dk[j];
+            if (K < I) csum[j] += m * di;
+          }
         }
       }
+      rsum[a] = r;
     }
-    rsum[a] = r;
   }
   // row sums over the 16 threads of this ty (consecutive lanes)
 #pragma unroll
@@ -108,6 +135,88 @@ __global__ __launch_bounds__(NT) void coulomb_tile_kernel(const double* __restri
     for (int t = 0; t < 16; ++t) c += cs[t][k];
     if (K < P) pcol[bi * P + K] = c;
   }
+}
+
+// Large P: the same JT x JT tiles and partial vectors, read with a wave per row: lane l takes
+// columns K0 + l + 64 c (c < 4), so one load instruction covers 64 consecutive elements of one
+// row.  Each wave takes groups of J_RG consecutive rows; the J_RG row partials of a group are
+// summed across the wave by a halving butterfly (lane l ends with the row l >> 4), the column
+// partials stay in registers until the four waves add them through LDS.
+constexpr int J_RG = 4;
+
+template <bool CHECK>
+__device__ __forceinline__ void coulomb_row_group(const double* __restrict__ eri, long P, const double* __restrict__ Dt,
+                                                  long Ib, long K0, int lane, const double (&dk)[4],
+                                                  double (&csum)[4], double* __restrict__ prow_b) {
+  double v[J_RG];
+#pragma unroll
+  for (int a = 0; a < J_RG; ++a) {
+    const long I = Ib + a;
+    double r = 0.0;
+    if (!CHECK || I < P) {
+      const double di = Dt[I];
+      const double* row = eri + I * (I + 1) / 2 + K0 + lane;
+      double m[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) m[c] = (!CHECK || K0 + lane + 64 * c <= I) ? __builtin_nontemporal_load(row + 64 * c) : 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        r += m[c] * dk[c];
+        if (!CHECK || K0 + lane + 64 * c < I) csum[c] += m[c] * di;
+      }
+    }
+    v[a] = r;
+  }
+  // halving butterfly over lane bits 5 and 4: lane l keeps row (l >> 4), summed over 4 lanes
+  {
+    const bool up = (lane & 32) != 0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const double send = up ? v[i] : v[i + 2], keep = up ? v[i + 2] : v[i];
+      v[i] = keep + __shfl_xor(send, 32, 64);
+    }
+    const bool up2 = (lane & 16) != 0;
+    const double send = up2 ? v[0] : v[1], keep = up2 ? v[1] : v[0];
+    v[0] = keep + __shfl_xor(send, 16, 64);
+  }
+  double r = v[0];
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
+  const long I = Ib + (lane >> 4);
+  if ((lane & 15) == 0 && (!CHECK || I < P)) prow_b[I] = r;
+}
+
+__global__ __launch_bounds__(NT) void coulomb_rows_kernel(const double* __restrict__ eri, long P,
+                                                          const double* __restrict__ Dt,
+                                                          double* __restrict__ prow, double* __restrict__ pcol) {
+  static_assert(NT == 256 && JT == 256, "four waves, one column per thread in the epilogue");
+  __shared__ double cs[4][JT];
+  long bi, bk;
+  coulomb_tile_of(blockIdx.x, bi, bk);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long I0 = bi * JT, K0 = bk * JT;
+  double dk[4], csum[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const long K = K0 + lane + 64 * c;
+    dk[c] = K < P ? Dt[K] : 0.0;
+    csum[c] = 0.0;
+  }
+  const bool check = !(bk < bi && I0 + JT <= P);
+  for (int g = 0; g < JT / (4 * J_RG); ++g) {
+    const long Ib = I0 + (long)(g * 4 + wave) * J_RG;
+    if (check)
+      coulomb_row_group<true>(eri, P, Dt, Ib, K0, lane, dk, csum, prow + bk * P);
+    else
+      coulomb_row_group<false>(eri, P, Dt, Ib, K0, lane, dk, csum, prow + bk * P);
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) cs[wave][lane + 64 * c] = csum[c];
+  __syncthreads();
+  const int k = threadIdx.x;
+  const long K = K0 + k;
+  const double c = (cs[0][k] + cs[1][k]) + (cs[2][k] + cs[3][k]);
+  if (K < P) pcol[bi * P + K] = c;
 }
 
 // J~_I = sum_{bk <= I/JT} prow[bk][I] + sum_{bi >= I/JT} pcol[bi][I], unpacked to J (n x n)
@@ -1054,7 +1163,9 @@ int coulomb_launch(hipStream_t st, int32_t n, const double* eri, const double* r
   double* pcol = prow + nb * P;
   GDX_LAUNCH("pack_density", pack_density_kernel, dim3(nblk(P)), dim3(NT), 0, st, rdm1, n, Dt);
   const dim3 grid((unsigned)(nb * (nb + 1) / 2));
-  if (tt == JT)
+  if (tt == JT && P >= J_ROWS_MIN)
+    GDX_LAUNCH("coulomb_tile", coulomb_rows_kernel, grid, dim3(NT), 0, st, eri, P, Dt, prow, pcol);
+  else if (tt == JT)
     GDX_LAUNCH("coulomb_tile", coulomb_tile_kernel<JT>, grid, dim3(NT), 0, st, eri, P, Dt, prow, pcol);
   else
     GDX_LAUNCH("coulomb_tile", coulomb_tile_kernel<JTS>, grid, dim3(NT), 0, st, eri, P, Dt, prow, pcol);
