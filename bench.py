@@ -436,7 +436,10 @@ def batch_bench(args):
 def train_bench(args):
     """One training step through the differentiable SCF loop: H2O (seeded geometry)
     def2-TZVP, level-3 grid, neural B3LYP with an MLP of 3 x 128, args.train_cycles-cycle
-    diff_scf_loop forward + reverse mode to d loss / d theta (simple_energy_loss)."""
+    diff_scf_loop forward + reverse mode to d loss / d theta (simple_energy_loss).  ms_per_step:
+    the Python-driven step (mean of 5); ms_per_step_graph: the loop and its reverse pass replayed
+    as captured graphs (mean of 5); kernel breakdown and total from one more eager step with
+    per-launch events (which slow that step down; it is not timed)."""
     import torch
     from graddft_amd import MLPSpec, NeuralFunctional, _lib, diff_scf_loop, simple_energy_loss
     from graddft_amd.interface import geometry
@@ -449,17 +452,35 @@ def train_bench(args):
                        f"MLP 3x128, diff_scf_loop {args.train_cycles} cycles forward + reverse "
                        "(simple_energy_loss)"}
     grads = {}
+    reps = 5
     for prec in (["fp64", args.precision] if args.precision != "fp64" else ["fp64"]):
         loop = diff_scf_loop(f, cycles=args.train_cycles, precision=prec)
         simple_energy_loss(theta, loop, mol, -76.4)   # warm-up
         torch.cuda.synchronize()
-        with _lib.KernelTimer() as kt:
-            t0 = time.perf_counter()
+        t0 = time.perf_counter()
+        for _ in range(reps):
             (cost, e), g = simple_energy_loss(theta, loop, mol, -76.4)
+        torch.cuda.synchronize()
+        ms_eager = (time.perf_counter() - t0) / reps * 1e3
+        with _lib.KernelTimer() as kt:
+            simple_energy_loss(theta, loop, mol, -76.4)
             torch.cuda.synchronize()
-            t1 = time.perf_counter()
-        grads[prec] = g.cpu().numpy()
-        out[prec] = {"ms_per_step": round((t1 - t0) * 1e3, 3), "energy": float(e),
+        # the loop and its reverse pass as two captured graphs (graph=True under autograd)
+        gloop = diff_scf_loop(f, cycles=args.train_cycles, precision=prec, graph=True)
+        simple_energy_loss(theta, gloop, mol, -76.4)  # eager run + captures
+        simple_energy_loss(theta, gloop, mol, -76.4)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            (cost_g, e_g), g_g = simple_energy_loss(theta, gloop, mol, -76.4)
+        torch.cuda.synchronize()
+        ms_graph = (time.perf_counter() - t0) / reps * 1e3
+        grads[prec] = g_g.cpu().numpy()
+        out[prec] = {"ms_per_step": round(ms_eager, 3), "ms_per_step_graph": round(ms_graph, 3),
+                     "kernels_total_ms_per_step": round(sum(v[0] for v in kt.result.values()), 3),
+                     "kernel_launches_per_step": sum(v[1] for v in kt.result.values()),
+                     "energy": float(e_g), "graph_vs_eager_abs_dE": abs(float(e_g) - float(e)),
+                     "graph_vs_eager_grad_rel": float(torch.linalg.norm(g_g - g) / torch.linalg.norm(g)),
                      "kernels_ms_per_step": {k: round(v[0], 4) for k, v in
                                              sorted(kt.result.items(), key=lambda kv: -kv[1][0])[:8]}}
     if args.precision != "fp64":

@@ -69,11 +69,11 @@ class DiisCoefficientsFn(torch.autograd.Function):
         m, ns = errs.shape[0], errs.shape[1]
         gx = torch.zeros_like(x)
         gx[:, 1:] = gc
-        y = torch.linalg.solve(B.transpose(1, 2), gx.unsqueeze(-1)).squeeze(-1)
+        # solve_ex: no error check, so no host synchronisation (the reverse pass is capturable)
+        y = torch.linalg.solve_ex(B.transpose(1, 2), gx.unsqueeze(-1))[0].squeeze(-1)
         G = -(y.unsqueeze(-1) * x.unsqueeze(-2))[:, 1:, 1:]
-        inactive = torch.arange(m, device=errs.device) > ctx.cycle
         G = G.clone()
-        G[:, inactive, inactive] = 0.0
+        G.diagonal(dim1=1, dim2=2)[:, ctx.cycle + 1:] = 0.0     # inactive slots: constant 1
         E = errs.reshape(m, ns, -1).transpose(0, 1)                 # (ns, m, n2)
         Ebar = torch.matmul(G + G.transpose(1, 2), E)               # (ns, m, n2)
         return Ebar.transpose(0, 1).reshape(errs.shape), None
@@ -92,13 +92,32 @@ class DiisSolveFn(torch.autograd.Function):
     @torch.autograd.function.once_differentiable
     def backward(ctx, gx):
         B, x = ctx.saved_tensors
-        y = torch.linalg.solve(B.transpose(1, 2), gx.unsqueeze(-1)).squeeze(-1)
+        y = torch.linalg.solve_ex(B.transpose(1, 2), gx.unsqueeze(-1))[0].squeeze(-1)
         return -y.unsqueeze(-1) * x.unsqueeze(-2)
 
 
-def capture_graph(g: "torch.cuda.CUDAGraph", fn: Callable):
-    """Capture fn() into g with Python's cyclic garbage collector run first and paused during
-    the capture.  Molecules and their XC plans reference each other, so a dead molecule --
+class _GraphedLoopFn(torch.autograd.Function):
+    """A captured diff_scf_loop: forward replays the loop's graph (its inputs already copied in)
+    and returns its energy; backward writes the energy's cotangent into the reverse graph's
+    input and replays it."""
+
+    @staticmethod
+    def forward(ctx, params, g_fwd, g_bwd, static_g, energy_out, d_params_out):
+        g_fwd.replay()
+        ctx.g_bwd, ctx.static_g, ctx.d_params_out = g_bwd, static_g, d_params_out
+        return energy_out.detach().clone()
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, g):
+        ctx.static_g.copy_(g.detach().reshape(()))
+        ctx.g_bwd.replay()
+        return ctx.d_params_out.clone(), None, None, None, None, None
+
+
+def capture_graph(g: "torch.cuda.CUDAGraph", fn: Callable, pool=None):
+    """Capture fn() into g (in the memory pool of another graph when pool is given) with
+    Python's cyclic garbage collector run first and paused during the capture.  Molecules and their XC plans reference each other, so a dead molecule --
     with the graphs of its captured loops in its cache -- is freed only by the cyclic
     collector; a CUDAGraph destroyed while another capture is in progress makes calls a
     capturing stream does not permit, and its destructor then aborts the process."""
@@ -106,7 +125,7 @@ def capture_graph(g: "torch.cuda.CUDAGraph", fn: Callable):
     enabled = gc.isenabled()
     gc.disable()
     try:
-        with torch.cuda.graph(g):
+        with torch.cuda.graph(g, pool=pool):
             return fn()
     finally:
         if enabled:
@@ -283,10 +302,14 @@ def _status_buffers(molecule: Molecule):
 
 def diff_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp64", graph: bool = False,
                   refine_cycles: Optional[int] = None, check_status: bool = True, **kwargs) -> Callable:
-    """graph=True (forward only, autograd not recording): the whole loop is captured once per
-    molecule as one HIP graph (after an eager run that sets up plans and workspaces) and
-    replayed on later calls -- no host work between the kernels, as the reference's jitted
-    fori_loop.  The rdm1 and params of each call are copied into the captured inputs.
+    """graph=True: the whole loop is captured once per molecule as one HIP graph (after an eager
+    run that sets up plans and workspaces) and replayed on later calls -- no host work between
+    the kernels, as the reference's jitted fori_loop.  The rdm1 and params of each call are
+    copied into the captured inputs.  When autograd records (params requiring grad), the reverse
+    pass through the loop is captured as a second graph in the same memory pool: the call
+    returns an energy whose backward replays it (d energy / d params; one backward per call,
+    the forward replay overwrites the saved intermediates).  rdm1 requiring grad takes the eager
+    path.
 
     refine_cycles (mixed precisions; default 1, 0 for fp64): the last refine_cycles cycles,
     and the energy they return, use the FP64 grid path.  The returned energy is then an FP64
@@ -343,9 +366,42 @@ def diff_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp
             raise_on_status(_status_buffers(out)[0], "diff_scf_loop")
         return out
 
+    def graphed_reverse(params: torch.Tensor, molecule: Molecule) -> Molecule:
+        key = ("scf_graph_ad", id(functional), cycles, precision, refine_cycles)
+        entry = molecule.cache.get(key)
+        if entry is None:
+            out = run_loop(params, molecule)          # plans, workspaces, Cholesky, K0
+            static_p = params.detach().clone().requires_grad_(True)
+            static_D = molecule.rdm1.detach().clone()
+            static_g = torch.ones((), dtype=torch.float64, device=static_D.device)
+            m_in = molecule.replace(rdm1=static_D)
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):            # warm-up of both passes off the capture
+                w = run_loop(static_p, m_in)
+                torch.autograd.grad(w.energy, [static_p], grad_outputs=static_g)
+            torch.cuda.current_stream().wait_stream(side)
+            g_fwd, g_bwd = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+            m_out = capture_graph(g_fwd, lambda: run_loop(static_p, m_in))
+            d_p, = capture_graph(g_bwd, lambda: torch.autograd.grad(m_out.energy, [static_p], grad_outputs=static_g,
+                                                                    retain_graph=True), pool=g_fwd.pool())
+            molecule.cache[key] = (g_fwd, g_bwd, static_p, static_D, static_g, m_out, d_p)
+            return checked(out)
+        g_fwd, g_bwd, static_p, static_D, static_g, m_out, d_p = entry
+        with torch.no_grad():
+            static_p.copy_(params.detach())
+            static_D.copy_(molecule.rdm1.detach())
+        energy = _GraphedLoopFn.apply(params, g_fwd, g_bwd, static_g, m_out.energy, d_p)
+        return checked(molecule.replace(energy=energy, rdm1=m_out.rdm1.detach().clone(),
+                                        fock=m_out.fock.detach().clone(), mo_coeff=m_out.mo_coeff.detach().clone(),
+                                        mo_energy=m_out.mo_energy.detach().clone(), mo_occ=m_out.mo_occ.clone(),
+                                        restricted=m_out.restricted))
+
     def scf_iterator(params: Optional[torch.Tensor], molecule: Molecule) -> Molecule:
         recording = torch.is_grad_enabled() and (
             (params is not None and params.requires_grad) or molecule.rdm1.requires_grad)
+        if graph and recording and params is not None and params.requires_grad and not molecule.rdm1.requires_grad:
+            return graphed_reverse(params, molecule)
         if not graph or recording:
             return checked(run_loop(params, molecule))
         key = ("scf_graph", id(functional), cycles, precision, refine_cycles, params is None)

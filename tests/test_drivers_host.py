@@ -133,7 +133,7 @@ def test_capture_graph_pauses_the_cyclic_collector(monkeypatch):
     from graddft_amd import evaluate
 
     calls = []
-    monkeypatch.setattr(torch.cuda, "graph", lambda g: contextlib.nullcontext())
+    monkeypatch.setattr(torch.cuda, "graph", lambda g, pool=None: contextlib.nullcontext())
     real_collect = gc.collect
     monkeypatch.setattr(gc, "collect", lambda *a: calls.append("collect") or real_collect(*a))
     seen = {}

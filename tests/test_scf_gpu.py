@@ -340,6 +340,38 @@ def test_graph_captured_scf_loop_replays_match_eager(kind):
         assert torch.allclose(out.rdm1, ref.rdm1, rtol=0, atol=1e-11), i
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_graph_captured_reverse_pass_matches_eager(precision):
+    """diff_scf_loop(graph=True) under autograd: the loop and its reverse pass replayed as two
+    captured graphs give the eager loop's energy and d loss / d theta for several parameter
+    vectors in turn (each replay with new parameters, eager device work in between), and a
+    loss whose cotangent is not 1 (the reverse graph's input is the energy's cotangent)."""
+    _say(f"case: graph-captured forward + reverse diff_scf_loop, neural, water/def2-svp level 1, 8 cycles, "
+         f"{precision}")
+    from graddft_amd import MLPSpec, NeuralFunctional, diff_scf_loop, energy_predictor, simple_energy_loss
+    mol = _water()
+    f = NeuralFunctional(MLPSpec(nc=6, h=32, n_hidden=2, nd=4))
+    thetas = [f.init(seed=1), f.init(seed=2), f.init(seed=3), f.init(seed=1)]
+    eager = diff_scf_loop(f, cycles=8, precision=precision)
+    graphed = diff_scf_loop(f, cycles=8, precision=precision, graph=True)
+    simple_energy_loss(thetas[0], graphed, mol, -76.0)       # eager run + both captures
+    for i, th in enumerate(thetas):
+        (c0, e0), g0 = simple_energy_loss(th, eager, mol, -76.0)
+        energy_predictor(f)(th, mol)                           # unrelated eager work between replays
+        (c1, e1), g1 = simple_energy_loss(th, graphed, mol, -76.0)
+        torch.cuda.synchronize()
+        rel = float(torch.linalg.norm(g1 - g0) / torch.linalg.norm(g0))
+        _say(f"  theta {i}: E {e1.item():.12f} (eager {e0.item():.12f}), grad rel diff {rel:.2e}")
+        assert abs(e1.item() - e0.item()) <= 1e-12, i
+        assert rel <= 1e-12, i
+    # a cotangent other than the loss's: d (3 E) / d theta, replayed and eager
+    th = thetas[1].detach().clone().requires_grad_(True)
+    gg, = torch.autograd.grad(3.0 * graphed(th, mol).energy, [th])
+    th2 = thetas[1].detach().clone().requires_grad_(True)
+    ge, = torch.autograd.grad(3.0 * eager(th2, mol).energy, [th2])
+    assert float(torch.linalg.norm(gg - ge) / torch.linalg.norm(ge)) <= 1e-12
+
+
 def test_batched_loop_with_shared_cache_matches_eager():
     """batched_scf_loop with a molecule listed twice and a replace()d copy (all three share one
     cache, hence one captured graph and its buffers) equals the molecule-by-molecule loop."""
