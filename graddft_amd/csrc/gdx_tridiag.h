@@ -176,7 +176,9 @@ __global__ __launch_bounds__(RED_THREADS) void tri_reduce_kernel(int n, const do
     auto vat = [&](int base, int blk, int x) -> double {
       return x == k + 1 ? 1.0 : ((x > k + 1 && x < n) ? sm[xsb + base + 32 * blk] * scal : 0.0);
     };
-    // p = A v: row sums and strictly-lower column sums
+    // p = A v: row sums and strictly-lower column sums.  Blocks of column block c < kc (every
+    // row and column index <= k) no longer take part: v and p vanish there, the update leaves
+    // them unchanged, and p is read only at indices > k -- the step skips them (uniform branches).
     double vcol[RA], wcol[RA];
 #pragma unroll
     for (int c = 0; c < RA; ++c) {
@@ -185,11 +187,13 @@ __global__ __launch_bounds__(RED_THREADS) void tri_reduce_kernel(int n, const do
     }
 #pragma unroll
     for (int a = 0; a < RA; ++a) {
+      if (a < kc) continue;
       const int i = rg + 32 * a;
       const double vr = vat(orow, a, i);
       double s = 0.0;
 #pragma unroll
       for (int c = 0; c <= a; ++c) {
+        if (c < kc) continue;
         const double l = get(a, c);
         s += l * vcol[c];
         if (c < a || !dg) wcol[c] += l * vr;   // unowned (up) entries hold 0
@@ -199,6 +203,7 @@ __global__ __launch_bounds__(RED_THREADS) void tri_reduce_kernel(int n, const do
     }
 #pragma unroll
     for (int c = 0; c < RA; ++c) {
+      if (c < kc) continue;
       const double t = wcol[c] + swap_halves(wcol[c]);
       const int j = cg + 32 * c;
       if (lane < 32 && j < n) sm[COLP + wv * NMAX + ocol + 32 * c] = t;
@@ -221,17 +226,19 @@ __global__ __launch_bounds__(RED_THREADS) void tri_reduce_kernel(int n, const do
     const double K = 0.5 * tau * wave_sum(dot);
 #pragma unroll
     for (int c = 0; c < RA; ++c) {
+      if (c < kc) continue;
       const int j = cg + 32 * c;
       wcol[c] = (j < n ? sm[PS + ocol + 32 * c] : 0.0) - K * vcol[c];
     }
 #pragma unroll
     for (int a = 0; a < RA; ++a) {
+      if (a < kc) continue;
       const int i = rg + 32 * a;
       const double vr = vat(orow, a, i);
       const double wr = (i < n ? sm[PS + orow + 32 * a] : 0.0) - K * vr;
 #pragma unroll
       for (int c = 0; c <= a; ++c)
-        if (c < a || !up) put(a, c, get(a, c) - (vr * wcol[c] + wr * vcol[c]));
+        if (c >= kc && (c < a || !up)) put(a, c, get(a, c) - (vr * wcol[c] + wr * vcol[c]));
     }
   }
   // the last 2 x 2 block (or the single element)
