@@ -61,6 +61,7 @@ def parse():
     ap.add_argument("--no-eri", action="store_true", help="skip the device ERI generation measurement")
     ap.add_argument("--no-config4", action="store_true", help="skip the naphthalene dense-ERI SCF workload")
     ap.add_argument("--no-config5", action="store_true", help="skip the 40-carbon grid-XC workload")
+    ap.add_argument("--no-chi", action="store_true", help="skip the semi-numerical exchange (chi) measurement")
     ap.add_argument("--batch-size", type=int, default=64, help="alkanes in the batched training step")
     ap.add_argument("--batch-max-nc", type=int, default=8, help="largest n_C of the batched training step")
     ap.add_argument("--no-graph", action="store_true", help="time the eager step instead of graph replays")
@@ -392,6 +393,59 @@ def config5_bench(args):
         out[prec].update({"abs_dE_vs_fp64_Ha": abs(e1 - e64), "rel_err_dE_dD": rel(d1, d64),
                           "rel_err_dE_dtheta": rel(t1, t64),
                           "speedup_vs_fp64": round(out["fp64"]["ms_per_step"] / out[prec]["ms_per_step"], 3)})
+    return out
+
+
+def chi_bench(args):
+    """Semi-numerical exchange: chi (generate_chi_tensor at omega = 0) on the device for water
+    def2-TZVP (f shells) and benzene 6-31G* on level-3 grids, after one warm-up build (mean of 3;
+    kernel split from per-launch events).  Work unit: one (grid point, primitive-pair entry)
+    evaluation of the nu integrals (Boys function, Hermite R, contraction).  CPU: the FP64 numpy
+    oracle (nu on the grid, then the chi contraction) on the first 128 points, one process."""
+    import torch
+    from graddft_amd import _lib, hf
+    from graddft_amd.interface import geometry
+    from graddft_amd.interface.builder import molecule_from_geometry
+    from oracle.hf import chi_tensor, nu_on_grid
+    out = {}
+    for name, sym_xyz, basis in (("water", geometry.water(np.random.default_rng(args.seed)), "def2-tzvp"),
+                                 ("benzene", geometry.benzene(np.random.default_rng(args.seed)), "6-31g*")):
+        sym, xyz = sym_xyz
+        mol = molecule_from_geometry(sym, xyz, basis, 3, name=name)
+        N, n = mol.grid_size, mol.nao
+        t0 = time.perf_counter()
+        plan = hf.hf_plan(mol)
+        t_plan = time.perf_counter() - t0
+        entries, _ = plan.info()
+        hf.generate_chi_tensor(mol, mol.rdm1, [0.0])
+        torch.cuda.synchronize()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            chi = hf.generate_chi_tensor(mol, mol.rdm1, [0.0])
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        with _lib.KernelTimer() as kt:
+            hf.generate_chi_tensor(mol, mol.rdm1, [0.0])
+            torch.cuda.synchronize()
+        S = 128
+        c = mol.grid.coords[:S].cpu().numpy()
+        ao = mol.ao_planes[0][:S, :n].cpu().numpy()
+        D = mol.rdm1.cpu().numpy()
+        t0 = time.perf_counter()
+        chi_o = chi_tensor(D[:1], ao, nu_on_grid(mol.basis, c, 0.0))
+        t_cpu = time.perf_counter() - t0
+        got = chi[0, :1, :, :S].permute(2, 0, 1).cpu().numpy()
+        out[name] = {"basis": basis, "nao": n, "npts": N, "primitive_pair_entries": entries,
+                     "plan_ms": round(t_plan * 1e3, 1), "ms_per_chi": round(ms, 3),
+                     "grid_pts_per_s": round(N / (ms / 1e3), 1),
+                     "entry_point_evals_per_s": round(entries * N / (ms / 1e3), 1),
+                     "kernels_ms": {k: round(v[0], 3) for k, v in sorted(kt.result.items(), key=lambda kv: -kv[1][0])},
+                     "cpu_baseline": {"value": round(S / t_cpu, 1), "unit": "grid_pts/s", "kind": "port",
+                                      "cores": 1,
+                                      "sample": f"FP64 numpy oracle, chi at the first {S} grid points"},
+                     "max_rel_diff_vs_oracle_sample": float(np.abs(got - chi_o).max() / np.abs(chi_o).max())}
+        del mol, plan, chi
     return out
 
 
@@ -772,6 +826,7 @@ def main():
     eri = section(eri_bench, args.no_eri)
     config4 = section(config4_bench, args.no_config4 or args.no_scf)
     config5 = section(config5_bench, args.no_config5)
+    chi = section(chi_bench, args.no_chi)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -816,6 +871,7 @@ def main():
         "eri": eri,
         "config4": config4,
         "config5": config5,
+        "hf_chi": chi,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
