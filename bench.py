@@ -351,6 +351,19 @@ def config5_bench(args):
     mol = molecule_from_geometry(sym, xyz, "def2-svp", level, rdm1=C @ C.T, name="carbon40")
     torch.cuda.synchronize()
     inputs["molecule_build_s"] = round(time.perf_counter() - t0, 2)
+    # AO planes of this grid (5 x npts_pad x nao_pad doubles, written once)
+    from graddft_amd.interface.builder import eval_ao_device
+    gc_np = mol.grid.coords.cpu().numpy()
+    planes = eval_ao_device(bs, gc_np, mol.npts_pad, mol.nao_pad)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    planes = eval_ao_device(bs, gc_np, mol.npts_pad, mol.nao_pad)
+    torch.cuda.synchronize()
+    ao_s = time.perf_counter() - t0
+    inputs["ao_eval_ms"] = round(ao_s * 1e3, 2)
+    inputs["ao_planes_bytes"] = planes.numel() * 8
+    inputs["ao_eval_GB_per_s"] = round(planes.numel() * 8 / ao_s / 1e9, 1)
+    del planes
     spec = MLPSpec(nc=6, h=256, n_hidden=3, nd=4)
     f = NeuralFunctional(spec)
     theta = f.init(seed=6)

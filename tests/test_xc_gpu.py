@@ -47,10 +47,11 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def test_ao_planes_match_oracle():
-    _say("case: AO planes water/def2-tzvp level 1")
+@pytest.mark.parametrize("basis", ["def2-tzvp", "6-31g*"])
+def test_ao_planes_match_oracle(basis):
+    _say(f"case: AO planes water/{basis} level 1")
     from oracle.ao import eval_ao
-    mol = _build("water", "def2-tzvp", 1, seed=1)
+    mol = _build("water", basis, 1, seed=1)
     c = mol.grid.coords.cpu().numpy()
     ao, g, h = eval_ao(mol.basis, c)
     N, n = ao.shape
