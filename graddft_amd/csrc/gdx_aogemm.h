@@ -116,6 +116,9 @@ __device__ __forceinline__ void density_ao_body(const T* __restrict__ ao, int np
     for (int c = 0; c < 4; ++c)
 #pragma unroll
       for (int j = 0; j < NJ; ++j) acc[c][j] = (acc_t){0, 0, 0, 0};
+    // 16-column blocks of this chunk inside the matrix (the last chunk of an nP that is not a
+    // multiple of CB has fewer): the MFMAs of the others would only multiply zero columns
+    const int njv = min(NJ, (nP - c0) / 16);
     gload(0);
     lstore(0);
     __syncthreads();
@@ -137,7 +140,8 @@ __device__ __forceinline__ void density_ao_body(const T* __restrict__ ao, int np
 #pragma unroll
           for (int c = 0; c < 4; ++c)
 #pragma unroll
-            for (int j = 0; j < NJ; ++j) acc[c][j] = MfmaT<T>::op(af[c][kk], bf[j], acc[c][j]);
+            for (int j = 0; j < NJ; ++j)
+              if (j < njv) acc[c][j] = MfmaT<T>::op(af[c][kk], bf[j], acc[c][j]);
         }
       }
       if (kt + 1 < nk) lstore(cur ^ 1);
@@ -145,7 +149,7 @@ __device__ __forceinline__ void density_ao_body(const T* __restrict__ ao, int np
     }
     // ---- phase 2: contraction with the AO values at the chunk's columns, 16 columns per
     // stage staged through LDS (coalesced loads of the next stage overlap this one) ----
-    const int nst = min(NJ, (nP - c0) / 16);
+    const int nst = njv;
     auto cload = [&](int st) {
 #pragma unroll
       for (int i = 0; i < NVC; ++i) {
@@ -261,6 +265,9 @@ __device__ __forceinline__ void vxc_ao_body(const T* __restrict__ ao, int npts_p
   const int rend = min(npts_pad, rbeg + chunk);
   const size_t plane = (size_t)npts_pad * nP;
   const int wm = (wave >> 1) * (VT / 2), wn = (wave & 1) * (VT / 2);
+  // 16 x 16 blocks of this wave's quarter inside the nP x nP matrix (edge tiles of an nP that
+  // is not a multiple of VT have fewer; the others would only accumulate zeros)
+  const int niv = max(0, min(WB, (nP - a0 - wm) / 16)), njv = max(0, min(WB, (nP - b0 - wn) / 16));
   // staging role of this thread: point p of the stage, CPT adjacent columns
   const int sp = tid >> 6, sc = (tid & 63) * CPT;
   typedef T v2 __attribute__((ext_vector_type(CPT)));
@@ -340,7 +347,8 @@ __device__ __forceinline__ void vxc_ao_body(const T* __restrict__ ao, int npts_p
 #pragma unroll
       for (int i = 0; i < WB; ++i)
 #pragma unroll
-        for (int j = 0; j < WB; ++j) acc[i][j] = MfmaT<T>::op(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < WB; ++j)
+          if (i < niv && j < njv) acc[i][j] = MfmaT<T>::op(af[i], bf[j], acc[i][j]);
     }
     if (st + 1 < nst) lstore(cur ^ 1);
     __syncthreads();
