@@ -437,6 +437,8 @@ __global__ __launch_bounds__(256) void vxc_bf16x6_kernel(const float* __restrict
   const size_t plane = (size_t)npts_pad * nP;
   const double* vp = vpt + (size_t)s * npts_pad * VPT_STRIDE;
   const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  // 16 x 16 blocks of this wave's quarter inside the matrix (the rest multiply zero padding)
+  const int niv = max(0, min(4, (nP - a0 - wm) / 16)), njv = max(0, min(4, (nP - b0 - wn) / 16));
   // staging role: points 2 pp, 2 pp + 1 of the stage; columns 2 cc, 2 cc + 1 of the tile
   const int pp = tid >> 6, cc = tid & 63;
   v2f ra[2][4], rb[2][5];
@@ -535,6 +537,7 @@ __global__ __launch_bounds__(256) void vxc_bf16x6_kernel(const float* __restrict
       for (int p = 0; p < 3; ++p) bq[p] = *(const bf8*)&Bs[cur][p][wn + j * 16 + r][8 * g];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
+        if (i >= niv || j >= njv) continue;
         f4 cacc = acc[i][j];
         cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][2], bq[0], cacc, 0, 0, 0);
         cacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][1], bq[1], cacc, 0, 0, 0);

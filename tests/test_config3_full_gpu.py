@@ -73,4 +73,11 @@ def test_config3_full_size_training_step():
     dE = np.abs(E["fp32"] - E["fp64"])
     _say(f"  |dE| fp32 vs fp64: median {np.median(dE):.2e}, max {dE.max():.2e}, "
          f"within 1e-6: {(dE <= 1e-6).sum()}/64, within 1e-5: {(dE <= 1e-5).sum()}/64")
+    # the batched plans' workspaces (tens of GB, allocated by the library) go now, not when the
+    # cyclic collector next runs
+    for m in mols + umols:
+        for k in [k for k in m.cache if isinstance(k, tuple) and k and k[0] in ("xcplan", "xcbatch")]:
+            plan = m.cache.pop(k)
+            if hasattr(plan, "close"):
+                plan.close()
     assert np.median(dE) <= 1e-6
