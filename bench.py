@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--no-chi", action="store_true", help="skip the semi-numerical exchange (chi) measurement")
     ap.add_argument("--batch-size", type=int, default=64, help="alkanes in the batched training step")
     ap.add_argument("--batch-max-nc", type=int, default=8, help="largest n_C of the batched training step")
+    ap.add_argument("--batch-init", default="b3lyp", choices=["b3lyp", "core"],
+                    help="initial densities of the training batch: self-consistent B3LYP or the core guess")
     ap.add_argument("--no-graph", action="store_true", help="time the eager step instead of graph replays")
     ap.add_argument("--copy-only", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (calibrates HBM byte counters)")
@@ -582,15 +584,18 @@ def train_batch_bench(args):
     the molecules (batched_diff_scf_loop: one MLP pass over all grid points, ragged AO-side
     launches).  One warm-up and one timed step per precision.  Roofline: W_pt = n_s 32 n^2 +
     6 F_mlp flops per grid point and cycle (forward, SCF adjoint with the second-order pass;
-    n_s = 2) at the precision's peak.  Each precision runs every cycle in itself
-    (refine_cycles=0).  Per molecule, |E_fp32 - E_fp64| is reported: loops that
-    are still far from self-consistency after the cycles amplify any perturbation."""
+    n_s = 2) at the precision's peak.  Runs: FP64; the mixed precision in every cycle
+    (refine_cycles=0); and the mixed precision with the loops' default last FP64 cycle.  The
+    molecules start from self-consistent B3LYP densities (--batch-init b3lyp, as the
+    reference's molecules carry a converged PySCF density) or the core guess.  Per molecule,
+    |E - E_fp64| is reported: loops far from self-consistency amplify any perturbation."""
     import torch
     from graddft_amd import MLPSpec, NeuralFunctional, batched_diff_scf_loop
     from graddft_amd.interface import geometry
     from graddft_amd.interface.builder import molecule_from_geometry
     rng = np.random.default_rng(args.seed)
     ncs = [int(x) for x in rng.integers(1, args.batch_max_nc + 1, args.batch_size)]
+    from graddft_amd.interface.builder import converged_molecule
     t0 = time.perf_counter()
     mols = []
     for nc in ncs:
@@ -598,6 +603,13 @@ def train_batch_bench(args):
         mols.append(molecule_from_geometry(sym, xyz, "def2-svp", 2, with_eri=True, name=f"C{nc}", npts_align=256))
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
+    # the reference's molecules carry a converged mean-field density (molecule_from_pyscf after
+    # mf.kernel()): self-consistent B3LYP here, unless the core-Hamiltonian guess is asked for
+    t0 = time.perf_counter()
+    if args.batch_init == "b3lyp":
+        mols = [converged_molecule(m) for m in mols]
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t0
     spec = MLPSpec(nc=6, h=128, n_hidden=3, nd=4)
     f = NeuralFunctional(spec)
     theta = f.init(seed=args.seed)
@@ -610,12 +622,18 @@ def train_batch_bench(args):
                        f"{min(ns)}-{max(ns)}), level-2 grids ({sum(npts)} pts in total), neural B3LYP MLP 3x128, "
                        f"mse_energy_loss over {cyc}-cycle diff_scf_loops, forward + reverse, XC batched per cycle",
            "n_molecules": len(mols), "npts_total": sum(npts), "molecule_build_s": round(t_build, 1),
+           "initial_density": "self-consistent B3LYP (scf_loop)" if args.batch_init == "b3lyp" else "core Hamiltonian",
+           "initial_density_s": round(t_init, 1),
            "flops_per_step": int(flops)}
     grads, energies = {}, {}
-    for prec in (["fp64", args.precision] if args.precision != "fp64" else ["fp64"]):
-        # every cycle in the one precision (refine_cycles=0): the mixed-precision default would
-        # also hold the FP64 batch plan, whose second-order workspace is 81 GB at this size
-        loop = batched_diff_scf_loop(f, cycles=cyc, precision=prec, refine_cycles=0)
+    runs = [("fp64", "fp64", 0)]
+    if args.precision != "fp64":
+        # every cycle in the mixed precision, and the loops' default for mixed precisions: the
+        # last cycle (and the returned energy) on the FP64 path, which also holds the FP64 batch
+        # plan (its second-order workspace is 81 GB at this size)
+        runs += [(args.precision, args.precision, 0), (args.precision + "_fp64_last_cycle", args.precision, 1)]
+    for key, prec, refine in runs:
+        loop = batched_diff_scf_loop(f, cycles=cyc, precision=prec, refine_cycles=refine)
 
         def step():
             th = theta.detach().clone().requires_grad_(True)
@@ -624,9 +642,9 @@ def train_batch_bench(args):
             g, = torch.autograd.grad(loss, [th])
             return float(loss.detach()), g, E.detach()
 
-        _release_xc_plans(mols)                  # the previous precision's plans and workspaces
+        _release_xc_plans(mols)                  # the previous run's plans and workspaces
         free, _ = torch.cuda.mem_get_info()
-        print(f"[bench] train_batch {prec}: device memory free {free / 2**30:.1f} GiB", file=sys.stderr, flush=True)
+        print(f"[bench] train_batch {key}: device memory free {free / 2**30:.1f} GiB", file=sys.stderr, flush=True)
         step()                                   # warm-up: plans, workspaces
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -634,18 +652,18 @@ def train_batch_bench(args):
         torch.cuda.synchronize()
         dt = time.perf_counter() - t1
         peak = 78.6e12 if prec == "fp64" else 157.3e12
-        grads[prec] = g.cpu().numpy()
-        energies[prec] = E.cpu().numpy()
-        out[prec] = {"s_per_step": round(dt, 3), "loss": loss, "peak_tflops": peak / 1e12,
-                     "achieved_tflops": round(flops / dt / 1e12, 2), "roofline_frac": round(flops / peak / dt, 4),
-                     "grid_pts_cycles_per_s": round(sum(npts) * cyc / dt, 1)}
-    if args.precision != "fp64":
-        g0, g1 = grads["fp64"], grads[args.precision]
-        dE = np.abs(energies[args.precision] - energies["fp64"])
-        out["grad_rel_err_vs_fp64"] = float(np.linalg.norm(g1 - g0) / np.linalg.norm(g0))
-        out["per_molecule_abs_dE_Ha"] = {"max": float(dE.max()), "median": float(np.median(dE)),
-                                         "within_1e-6": int((dE <= 1e-6).sum()), "of": len(mols)}
-        out["fp32_over_fp64_step_time"] = round(out[args.precision]["s_per_step"] / out["fp64"]["s_per_step"], 3)
+        grads[key] = g.cpu().numpy()
+        energies[key] = E.cpu().numpy()
+        out[key] = {"s_per_step": round(dt, 3), "loss": loss, "peak_tflops": peak / 1e12,
+                    "achieved_tflops": round(flops / dt / 1e12, 2), "roofline_frac": round(flops / peak / dt, 4),
+                    "grid_pts_cycles_per_s": round(sum(npts) * cyc / dt, 1), "refine_cycles": refine}
+        if key != "fp64":
+            g0 = grads["fp64"]
+            dE = np.abs(energies[key] - energies["fp64"])
+            out[key].update({"grad_rel_err_vs_fp64": float(np.linalg.norm(grads[key] - g0) / np.linalg.norm(g0)),
+                             "per_molecule_abs_dE_Ha": {"max": float(dE.max()), "median": float(np.median(dE)),
+                                                        "within_1e-6": int((dE <= 1e-6).sum()), "of": len(mols)},
+                             "step_time_over_fp64": round(dt / out["fp64"]["s_per_step"], 3)})
     _release_xc_plans(mols)
     for m in mols:
         m.cache.clear()

@@ -14,6 +14,7 @@ its packaging step ``_package_outputs`` (``:808-864``) without PySCF:
 """
 from __future__ import annotations
 
+import math
 from typing import Optional, Sequence
 
 import numpy as np
@@ -128,3 +129,42 @@ def molecule_from_geometry(symbols: Sequence[str], coords_bohr: np.ndarray, basi
         om = tuple(float(o) for o in omegas)
         mol = mol.replace(omegas=om, chi_grid=generate_chi_tensor(mol, mol.rdm1, om))
     return mol
+
+
+def converged_molecule(mol: Molecule, functional=None, round_cycles: int = 10, max_rounds: int = 6,
+                       e_tol: float = 1e-10) -> Molecule:
+    """The molecule carrying the density, orbitals and orbital energies of a self-consistent
+    calculation with `functional` (B3LYP by default, its exact exchange rebuilt from every
+    density as a Kohn-Sham code does): the state ``molecule_from_pyscf(mf)`` packages from a
+    mean-field object after ``mf.kernel()`` (interface/pyscf.py:105-201, 823-845), where this
+    builder's default is the core-Hamiltonian guess.
+
+    Rounds of `round_cycles` CDIIS cycles of the device loop (``diff_scf_loop``, no reverse
+    pass), each restarted from the previous round's density with a fresh DIIS history, until
+    two rounds' energies agree to `e_tol` Ha.  A fixed-length DIIS run kept going past
+    convergence meets an exactly singular error matrix (the reference's ``inv(B)`` then
+    returns non-finite coefficients too); restarting keeps the history well conditioned, and a
+    round whose result is not finite is discarded.  The result has a fresh cache whose initial
+    density (for K0 and chi) is the converged one."""
+    import torch
+    from ..evaluate import diff_scf_loop
+    from ..popular_functionals import B3LYP
+    f = (functional or B3LYP)
+    if getattr(f, "exact_exchange", 0.0):
+        f = f.with_exchange("current")
+    loop = diff_scf_loop(f, cycles=round_cycles, check_status=False)
+    cur, e_prev = mol, None
+    with torch.no_grad():
+        for _ in range(max_rounds):
+            out = loop(None, cur)
+            e = float(out.energy)
+            if not (math.isfinite(e) and bool(torch.isfinite(out.rdm1).all())):
+                break
+            cur = cur.replace(rdm1=out.rdm1.detach().contiguous(), mo_coeff=out.mo_coeff.detach(),
+                              mo_energy=out.mo_energy.detach(), mo_occ=out.mo_occ.detach())
+            if e_prev is not None and abs(e - e_prev) <= e_tol:
+                break
+            e_prev = e
+    new = mol.replace(rdm1=cur.rdm1, mo_coeff=cur.mo_coeff, mo_energy=cur.mo_energy, mo_occ=cur.mo_occ)
+    new.cache = {"rdm1_init": new.rdm1}
+    return new

@@ -9,10 +9,12 @@ Full-size properties:
   the FP32 densities' limit; mixed-precision loops end with an FP64 cycle for that reason);
 * every molecule's FP64 density after the loop holds its electron count per spin and is
   idempotent in the S metric (D S D = D) to 1e-8; loss and gradient finite in both precisions;
-* the loop end-points, FP32 against FP64: median |dE| within 1e-6 Ha.  The spread above it is
-  reported, not gated: several loops are far from self-consistency after 10 cycles from the
-  core guess (energies still moving by hartrees a cycle, HOMO-LUMO gaps ~0.01 Ha), and there
-  any perturbation grows ~400x a cycle; their FP64 trajectory is the reference.
+* the molecules start, as the reference's do (molecule_from_pyscf after a converged PySCF run),
+  from self-consistent B3LYP densities (interface.builder.converged_molecule); the loop
+  end-points of the mixed-precision default (FP32 cycles, last cycle FP64) against FP64: every
+  molecule within 1e-6 Ha and the loss gradient within 1e-5 relative -- the gates.  (From the
+  core-Hamiltonian guess several loops are far from self-consistency after 10 cycles, and there
+  any perturbation grows ~400x a cycle.)
 """
 import numpy as np
 import pytest
@@ -28,14 +30,16 @@ def _say(msg):
 def test_config3_full_size_training_step():
     from graddft_amd import MLPSpec, NeuralFunctional, batched_diff_scf_loop
     from graddft_amd.interface import geometry
-    from graddft_amd.interface.builder import molecule_from_geometry
-    _say("case: 64 alkanes def2-svp level 2, neural 3x128, 10-cycle batched loops, mse loss, fp64/fp32")
+    from graddft_amd.interface.builder import converged_molecule, molecule_from_geometry
+    _say("case: 64 alkanes def2-svp level 2 from self-consistent B3LYP densities, neural 3x128, 10-cycle "
+         "batched loops, mse loss, fp64/fp32")
     rng = np.random.default_rng(2)
     ncs = [int(x) for x in rng.integers(1, 9, 64)]
     mols = []
     for nc in ncs:
         sym, xyz = geometry.alkane(rng, nc)
-        mols.append(molecule_from_geometry(sym, xyz, "def2-svp", 2, with_eri=True, npts_align=256))
+        mols.append(converged_molecule(molecule_from_geometry(sym, xyz, "def2-svp", 2, with_eri=True,
+                                                              npts_align=256)))
     _say(f"  {sum(m.grid_size for m in mols)} grid points, nao {min(m.nao for m in mols)}-{max(m.nao for m in mols)}")
     f = NeuralFunctional(MLPSpec(nc=6, h=128, n_hidden=3, nd=4))
     theta = f.init(seed=2)
@@ -52,7 +56,7 @@ def test_config3_full_size_training_step():
     dx = np.abs(exc["fp32"] - exc["fp64"])
     _say(f"  E_xc at the initial densities, fp32 vs fp64: max |dE| {dx.max():.2e} Ha")
     assert (dx <= 1e-7 * np.abs(exc["fp64"])).all(), (dx / np.abs(exc["fp64"])).max()
-    E = {}
+    E, G = {}, {}
     for prec in ("fp64", "fp32"):
         loop = batched_diff_scf_loop(f, cycles=10, precision=prec)
         th = theta.detach().clone().requires_grad_(True)
@@ -62,6 +66,7 @@ def test_config3_full_size_training_step():
         g, = torch.autograd.grad(loss, [th])
         assert torch.isfinite(loss) and torch.isfinite(g).all() and g.norm() > 0
         E[prec] = Es.detach().cpu().numpy()
+        G[prec] = g.detach().cpu().numpy()
         if prec == "fp64":
             for m, o, nc in zip(mols, outs, ncs):
                 D = o.rdm1.detach()
@@ -71,8 +76,9 @@ def test_config3_full_size_training_step():
                     assert abs(float(torch.trace(D[s] @ S)) - ne) <= 1e-8
                     assert float((D[s] @ S @ D[s] - D[s]).abs().max()) <= 1e-8
     dE = np.abs(E["fp32"] - E["fp64"])
+    g_rel = float(np.linalg.norm(G["fp32"] - G["fp64"]) / np.linalg.norm(G["fp64"]))
     _say(f"  |dE| fp32 vs fp64: median {np.median(dE):.2e}, max {dE.max():.2e}, "
-         f"within 1e-6: {(dE <= 1e-6).sum()}/64, within 1e-5: {(dE <= 1e-5).sum()}/64")
+         f"within 1e-6: {(dE <= 1e-6).sum()}/64; loss gradient rel {g_rel:.2e}")
     # the batched plans' workspaces (tens of GB, allocated by the library) go now, not when the
     # cyclic collector next runs
     for m in mols + umols:
@@ -80,4 +86,5 @@ def test_config3_full_size_training_step():
             plan = m.cache.pop(k)
             if hasattr(plan, "close"):
                 plan.close()
-    assert np.median(dE) <= 1e-6
+    assert dE.max() <= 1e-6
+    assert g_rel <= 1e-5
