@@ -622,7 +622,7 @@ def train_batch_bench(args):
                        f"{min(ns)}-{max(ns)}), level-2 grids ({sum(npts)} pts in total), neural B3LYP MLP 3x128, "
                        f"mse_energy_loss over {cyc}-cycle diff_scf_loops, forward + reverse, XC batched per cycle",
            "n_molecules": len(mols), "npts_total": sum(npts), "molecule_build_s": round(t_build, 1),
-           "initial_density": "self-consistent B3LYP (scf_loop)" if args.batch_init == "b3lyp" else "core Hamiltonian",
+           "initial_density": "self-consistent B3LYP (device CDIIS, converged_molecule)" if args.batch_init == "b3lyp" else "core Hamiltonian",
            "initial_density_s": round(t_init, 1),
            "flops_per_step": int(flops)}
     grads, energies = {}, {}
