@@ -995,7 +995,10 @@ static int tri_eigh(hipStream_t st, int nb, int n, const double* A, long sA, dou
 // elimination with partial pivoting (the first row of largest |pivot| wins), in one wave:
 // lane t < m1 holds row t and the right-hand side in registers; the pivot search, the row
 // swap and the pivot row are read across lanes with v_readlane (uniform lane indices), so
-// the solve needs no LDS round trips.
+// the solve needs no LDS round trips.  An exactly singular B -- a loop at a bitwise fixed
+// point stores identical error vectors, whose Gram rows coincide -- gets 0 for the unknowns
+// of its zero pivots (those slots drop out of the extrapolation) where an inverse would give
+// non-finite coefficients.
 constexpr int DIIS_MAXM1 = 16;
 __device__ __forceinline__ double lane_bcast(double v, int lane) {
   const long long b = __double_as_longlong(v);
@@ -1035,9 +1038,10 @@ __device__ void diis_solve_wave(const double* __restrict__ Bs, int m1, double* _
           else if (t == p) r[j] = vc;
         }
       }
-      // eliminate below the pivot with the pivot row (columns j >= c and the right-hand side)
+      // eliminate below the pivot with the pivot row (columns j >= c and the right-hand side);
+      // an exactly zero pivot (the column is already zero below it) eliminates nothing
       const double pc = lane_bcast(r[c], c);
-      const double f = (t > c && t < m1) ? r[c] / pc : 0.0;
+      const double f = (t > c && t < m1 && pc != 0.0) ? r[c] / pc : 0.0;
 #pragma unroll
       for (int j = c; j <= DIIS_MAXM1; ++j) {
         if (j < m1 || j == DIIS_MAXM1) {
@@ -1058,7 +1062,7 @@ __device__ void diis_solve_wave(const double* __restrict__ Bs, int m1, double* _
 #pragma unroll
       for (int j = i + 1; j < DIIS_MAXM1; ++j)
         if (j < m1) v -= r[j] * x[j];
-      x[i] = lane_bcast(v / r[i], i);
+      x[i] = lane_bcast(r[i] != 0.0 ? v / r[i] : 0.0, i);
     }
   }
   if (t < m1) {

@@ -141,11 +141,9 @@ def converged_molecule(mol: Molecule, functional=None, round_cycles: int = 10, m
 
     Rounds of `round_cycles` CDIIS cycles of the device loop (``diff_scf_loop``, no reverse
     pass), each restarted from the previous round's density with a fresh DIIS history, until
-    two rounds' energies agree to `e_tol` Ha.  A fixed-length DIIS run kept going past
-    convergence meets an exactly singular error matrix (the reference's ``inv(B)`` then
-    returns non-finite coefficients too); restarting keeps the history well conditioned, and a
-    round whose result is not finite is discarded.  The result has a fresh cache whose initial
-    density (for K0 and chi) is the converged one."""
+    two rounds' energies agree to `e_tol` Ha (a history of nearly identical error vectors is a
+    badly conditioned DIIS system); a round whose result is not finite is discarded.  The
+    result has a fresh cache whose initial density (for K0 and chi) is the converged one."""
     import torch
     from ..evaluate import diff_scf_loop
     from ..popular_functionals import B3LYP
