@@ -9,8 +9,8 @@ Public API (names follow XanaduAI/GradDFT's ``grad_dft`` package):
 ``mse_energy_loss``, ``train_kernel``, ``adam``, ``molecule_from_geometry``.
 """
 from .molecule import Grid, Molecule
-from .functional import (Functional, LinearFunctional, NeuralFunctional, B3LYPNeuralCorrection, MLPSpec, XCPlan,
-                         XCEnergyFn, XCEnergyVxcFn)
+from .functional import (Functional, LinearFunctional, NeuralFunctional, B3LYPNeuralCorrection, CustomFunctional,
+                         MLPSpec, XCPlan, XCEnergyFn, XCEnergyVxcFn)
 from .popular_functionals import B3LYP, B3LYP_GRID, B88, LSDA, LYP, PW92, VWN
 from .hf import generate_chi_tensor
 from .interface.builder import molecule_from_geometry
@@ -24,6 +24,7 @@ from .eigenproblem import safe_eigh_vjp, safe_fock_solver, safe_general_eigh
 from .batch import XCBatchPlan, batch_energy_predictor, batched_diff_scf_loop
 
 __all__ = ["Grid", "Molecule", "Functional", "LinearFunctional", "NeuralFunctional", "B3LYPNeuralCorrection",
+           "CustomFunctional",
            "MLPSpec", "XCPlan", "XCEnergyFn", "XCEnergyVxcFn", "B3LYP", "B3LYP_GRID", "LSDA", "B88", "VWN", "PW92",
            "LYP", "generate_chi_tensor", "molecule_from_geometry",
            "energy_predictor", "non_scf_predictor", "simple_energy_loss", "mse_energy_loss",

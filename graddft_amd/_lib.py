@@ -81,6 +81,8 @@ SIGNATURES = {
     "gdx_hf_fock": (ctypes.c_int, [_P, _I, _I, _I, _I, _P, _I, _P, _P, ctypes.c_int64, ctypes.c_double, _P, _P]),
     "gdx_profile_begin": (ctypes.c_int, []),
     "gdx_profile_end": (ctypes.c_int, [_I, ctypes.c_char_p, _P, _P, _P]),
+    "gdx_xc_custom_inputs": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P, _P, _P]),
+    "gdx_xc_custom_vxc": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P, _P]),
     "gdx_xc_densities": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P]),
     "gdx_xc_backward": (ctypes.c_int, [_P, ctypes.POINTER(XCDesc), _P, _P, _P, _P, _P, _P, _P, _P]),
     "gdx_xc_hvp_workspace_size": (ctypes.c_int, [ctypes.POINTER(XCDesc), ctypes.POINTER(_Z)]),

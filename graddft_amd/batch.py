@@ -30,6 +30,8 @@ class XCBatchPlan:
 
     def __init__(self, functional: Functional, molecules: Sequence[Molecule], precision: str = "fp64",
                  restricted: Optional[bool] = None, hvp: bool = True):
+        if getattr(functional, "custom", False):
+            raise NotImplementedError("custom coefficient functions are evaluated per molecule (energy_predictor)")
         lib = _lib.load()
         if restricted is None:
             restricted = functional.spin_symmetric and all(m.restricted for m in molecules)

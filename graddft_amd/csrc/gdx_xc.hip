@@ -1164,6 +1164,32 @@ void run_densities(const Ctx& c, const T* ao, const double* rdm1, const double* 
              c.P<double>(c.L.eps_part));
 }
 
+// Custom coefficient functions: the energy densities E (npts_pad, 4) and the coefficient
+// inputs X (npts_pad, 16, first six columns) of the first npts points out to the caller, and the
+// caller's cotangents of both (dE/de_f, dE/dx_j) into the buffers point_bwd reads (zero rows
+// past npts).
+template <typename T>
+__global__ void custom_out_kernel(int npts, const double* __restrict__ E, const T* __restrict__ X,
+                                  double* __restrict__ e_out, double* __restrict__ x_out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= npts) return;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) e_out[(size_t)r * 4 + f] = E[(size_t)r * 4 + f];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) x_out[(size_t)r * 6 + j] = (double)X[(size_t)r * 16 + j];
+}
+
+template <typename T>
+__global__ void custom_in_kernel(int npts, int npts_pad, int nc, const double* __restrict__ ge,
+                                 const double* __restrict__ dx, double* __restrict__ GE, T* __restrict__ DX) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= npts_pad) return;
+  const bool in = r < npts;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) GE[(size_t)r * 4 + f] = in ? ge[(size_t)r * 4 + f] : 0.0;
+  for (int j = 0; j < nc; ++j) DX[(size_t)r * nc + j] = (T)(in && j < 6 ? dx[(size_t)r * 6 + j] : 0.0);
+}
+
 // hidden-layer weights in the path's element type (params are FP64 masters)
 template <typename T>
 const T* lowp_params(const Ctx& c, const double* params) {
@@ -1712,6 +1738,40 @@ int gdx_xc_densities(void* stream, const gdx_xc_desc* desc, const void* ao, cons
   // weights == nullptr: the per-point kernel forms densities and energy densities only
   if (desc->precision == GDX_FP64) run_densities<double>(c, (const double*)ao, rdm1, nullptr, dens);
   else run_densities<float>(c, (const float*)ao, rdm1, nullptr, dens);
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+int gdx_xc_custom_inputs(void* stream, const gdx_xc_desc* desc, const void* ao, const double* weights,
+                         const double* rdm1, void* workspace, double* e_out, double* x_out) {
+  int rc = validate(desc);
+  if (rc != GDX_OK) return rc;
+  GDX_REQUIRE(desc->kind == GDX_FUNC_NEURAL_B3LYP && desc->mlp_nc == 6 && desc->precision == GDX_FP64,
+              "gdx_xc_custom_inputs: needs kind GDX_FUNC_NEURAL_B3LYP (the six coefficient inputs), mlp_nc 6, FP64");
+  GDX_REQUIRE(ao && weights && rdm1 && workspace && e_out && x_out, "gdx_xc_custom_inputs: null pointer");
+  Ctx c{*desc, make_layout(*desc), (char*)workspace, (hipStream_t)stream};
+  run_densities<double>(c, (const double*)ao, rdm1, weights, c.P<double>(c.L.dens_fin));
+  GDX_LAUNCH("custom_out", custom_out_kernel<double>, dim3(nblocks(desc->npts)), dim3(NT), 0, c.st, desc->npts,
+             c.P<double>(c.L.E), c.P<double>(c.L.X), e_out, x_out);
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+int gdx_xc_custom_vxc(void* stream, const gdx_xc_desc* desc, const void* ao, const double* g_e, const double* g_x,
+                      void* workspace, double* d_rdm1) {
+  int rc = validate(desc);
+  if (rc != GDX_OK) return rc;
+  GDX_REQUIRE(desc->kind == GDX_FUNC_NEURAL_B3LYP && desc->mlp_nc == 6 && desc->precision == GDX_FP64,
+              "gdx_xc_custom_vxc: needs kind GDX_FUNC_NEURAL_B3LYP (the six coefficient inputs), mlp_nc 6, FP64");
+  GDX_REQUIRE(ao && g_e && g_x && workspace && d_rdm1, "gdx_xc_custom_vxc: null pointer");
+  Ctx c{*desc, make_layout(*desc), (char*)workspace, (hipStream_t)stream};
+  const gdx_xc_desc& d = *desc;
+  GDX_LAUNCH("custom_in", custom_in_kernel<double>, dim3(nblocks(d.npts_pad)), dim3(NT), 0, c.st, d.npts,
+             d.npts_pad, d.mlp_nc, g_e, g_x, c.P<double>(c.L.GE), c.P<double>(c.L.DX));
+  GDX_LAUNCH("point_bwd", point_bwd_kernel<double>, dim3(c.L.npt_blocks), dim3(NT), 0, c.st,
+             c.P<double>(c.L.dens_fin), c.P<double>(c.L.GE), c.P<double>(c.L.DX), d.mlp_nc, d.npts, d.npts_pad,
+             d.correlation, c.P<double>(c.L.vpt));
+  launch_vxc<double>(c, (const double*)ao, 2, d_rdm1);
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
 }

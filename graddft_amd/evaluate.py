@@ -320,6 +320,8 @@ def diff_scf_loop(functional: Functional, cycles: int = 25, precision: str = "fp
     Every cycle ORs its checks into a device status word (gdx_scf_status: eigensolver
     convergence, a finite energy, tr(D S) equal to the occupation count); the word is read once
     after the loop (or graph replay) and, with check_status, a set flag raises GdxError."""
+    if graph and getattr(functional, "custom", False):
+        raise NotImplementedError("graph=True: a custom coefficient function runs the eager loop")
     if refine_cycles is None:
         refine_cycles = 0 if precision == "fp64" else 1
     compute_energy = energy_predictor(functional, precision=precision, **kwargs)

@@ -26,7 +26,7 @@ import copy
 import ctypes
 import math
 from dataclasses import dataclass
-from typing import Optional
+from typing import Callable, Optional
 
 import numpy as np
 import torch
@@ -81,6 +81,12 @@ class XCPlan:
     The workspace carries forward intermediates to the backward call, so a plan serves one
     forward/backward pair at a time; ``generation`` guards against interleaving."""
 
+    custom = False   # plans of custom coefficient functions serve custom_inputs/custom_vxc only
+
+    def _no_custom(self):
+        if self.custom:
+            raise RuntimeError("this plan belongs to a custom coefficient function: use custom_inputs/custom_vxc")
+
     def __init__(self, molecule: Molecule, kind: int, spec: Optional[MLPSpec], precision: str,
                  coef=None, correlation: int = 0):
         lib = _lib.load()
@@ -126,6 +132,7 @@ class XCPlan:
         return ao_planes_for(self.molecule, self.precision)
 
     def forward(self, rdm1: torch.Tensor, params: Optional[torch.Tensor], exc: torch.Tensor) -> None:
+        self._no_custom()
         m = self.molecule
         st = torch.cuda.current_stream(m.ao_planes.device).cuda_stream
         rc = _lib.load().gdx_xc_forward(st, ctypes.byref(self.desc), self._ao().data_ptr(),
@@ -150,6 +157,7 @@ class XCPlan:
         return out[: m.grid_size]
 
     def backward(self, rdm1, params, g_exc, d_rdm1, d_params, generation: int) -> None:
+        self._no_custom()
         if generation != self.fwd_generation:
             raise RuntimeError("XC workspace was overwritten by another forward before this backward")
         m = self.molecule
@@ -161,7 +169,36 @@ class XCPlan:
         _lib.check(rc, "gdx_xc_backward")
 
 
+    def custom_inputs(self, rdm1: torch.Tensor):
+        """(e (N, 4), x (N, 6)): the B3LYP energy densities and the six coefficient inputs at
+        rdm1 (gdx_xc_custom_inputs); the plan's workspace keeps the densities for custom_vxc."""
+        m = self.molecule
+        N = m.grid_size
+        e = torch.empty((N, 4), dtype=torch.float64, device=rdm1.device)
+        x = torch.empty((N, 6), dtype=torch.float64, device=rdm1.device)
+        st = torch.cuda.current_stream(m.ao_planes.device).cuda_stream
+        _lib.check(_lib.load().gdx_xc_custom_inputs(st, ctypes.byref(self.desc), self._ao().data_ptr(),
+                                                    m.grid.weights_pad.data_ptr(), rdm1.contiguous().data_ptr(),
+                                                    self.workspace.data_ptr(), e.data_ptr(), x.data_ptr()),
+                   "gdx_xc_custom_inputs")
+        self.generation += 1
+        self.fwd_generation = self.generation
+        return e, x
+
+    def custom_vxc(self, g_e: torch.Tensor, g_x: torch.Tensor, generation: int) -> torch.Tensor:
+        """dE/dD (2, n, n) from the cotangents of the custom inputs (gdx_xc_custom_vxc)."""
+        if generation != self.fwd_generation:
+            raise RuntimeError("XC workspace was overwritten by another forward before this backward")
+        m = self.molecule
+        out = torch.empty((2, m.nao, m.nao), dtype=torch.float64, device=g_e.device)
+        st = torch.cuda.current_stream(m.ao_planes.device).cuda_stream
+        _lib.check(_lib.load().gdx_xc_custom_vxc(st, ctypes.byref(self.desc), self._ao().data_ptr(),
+                                                 g_e.contiguous().data_ptr(), g_x.contiguous().data_ptr(),
+                                                 self.workspace.data_ptr(), out.data_ptr()), "gdx_xc_custom_vxc")
+        return out
+
     def hvp(self, rdm1, params, direction, g_exc, exc=None, edot=None, d_rdm1=None, d_params=None) -> None:
+        self._no_custom()
         """gdx_xc_hvp: gradients of g E + <direction, dE/dD> w.r.t. rdm1 and params."""
         m = self.molecule
         lib = _lib.load()
@@ -234,6 +271,28 @@ class XCEnergyVxcFn(torch.autograd.Function):
         d_params = torch.empty_like(p) if (p is not None and ctx.needs_input_grad[1]) else None
         ctx.plan.hvp(rdm1, p, direction, g, d_rdm1=d_rdm1, d_params=d_params)
         return d_rdm1, d_params, None
+
+
+class CustomInputsFn(torch.autograd.Function):
+    """(e, x) = per-point energy densities and coefficient inputs of rdm1 (device kernels);
+    backward: dE/dD from the cotangents (g_e, g_x) (point adjoint + V_xc build on the device)."""
+
+    @staticmethod
+    def forward(ctx, rdm1, plan: XCPlan):
+        e, x = plan.custom_inputs(rdm1.detach())
+        ctx.plan = plan
+        ctx.gen = plan.generation
+        return e, x
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, g_e, g_x):
+        plan = ctx.plan
+        N = plan.molecule.grid_size
+        dev = plan.molecule.ao_planes.device
+        g_e = torch.zeros((N, 4), dtype=torch.float64, device=dev) if g_e is None else g_e
+        g_x = torch.zeros((N, 6), dtype=torch.float64, device=dev) if g_x is None else g_x
+        return plan.custom_vxc(g_e.to(torch.float64), g_x.to(torch.float64), ctx.gen), None
 
 
 class Functional:
@@ -324,6 +383,69 @@ class B3LYPNeuralCorrection(NeuralFunctional):
 
     def __init__(self, spec: MLPSpec = MLPSpec(), name: str = "b3lyp_neural_corr", is_xc: bool = True):
         super().__init__(spec=spec, name=name, is_xc=is_xc, exact_exchange=0.2)
+
+
+class CustomFunctional(Functional):
+    """``Functional(coefficients=..., energy_densities=..., coefficient_inputs=...)`` with a user
+    coefficient function (``functional.py:40-253``): E_xc = sum_r clip(w_r) clip(sum_f c_f(x_r)
+    clip(e_f(r))) with the four B3LYP energy densities e_f (lda, b88, correlation, lyp) and the
+    six coefficient inputs x = log(|.| + 1e-4) of (rho, |grad rho|^2, tau) per spin, both from
+    the device kernels.  ``coefficients(params, x)`` is any torch function returning (N, 4)
+    (an ``nn.Module`` call, a closure over its own parameters, ...); autograd runs through it,
+    and dE/dD comes back through the device's point adjoint and V_xc build.  FP64.  Energies,
+    V_xc (``energy_predictor``, the SCF loops' forward, ``scf_loop``) and dE/dtheta at a fixed
+    density are supported; reverse mode through an SCF loop needs the second-order pass, which
+    a custom function does not have, and raises NotImplementedError."""
+
+    kind = _lib.GDX_FUNC_NEURAL_B3LYP
+    spin_symmetric = False
+    custom = True
+
+    def __init__(self, coefficients: Callable, name: str = "custom", is_xc: bool = True,
+                 exact_exchange: float = 0.0, correlation: str = "vwn"):
+        super().__init__(name=name, is_xc=is_xc, exact_exchange=exact_exchange)
+        if not callable(coefficients):
+            raise TypeError("coefficients must be callable as coefficients(params, x)")
+        if correlation not in ("vwn", "pw92"):
+            raise ValueError("correlation must be 'vwn' or 'pw92'")
+        self.coefficients = coefficients
+        self.correlation = _lib.GDX_CORR_PW92 if correlation == "pw92" else _lib.GDX_CORR_VWN5
+        # the kind's MLP fields only size the workspace: the smallest valid network
+        self.spec = MLPSpec(nc=6, h=16, n_hidden=1, nd=4)
+
+    def plan(self, molecule: Molecule, precision: str = "fp64") -> XCPlan:
+        if precision != "fp64":
+            raise NotImplementedError("custom coefficient functions run in FP64")
+        key = ("xcplan-custom", molecule.restricted, self.correlation)
+        p = molecule.cache.get(key)
+        if p is None or p.molecule.ao_planes is not molecule.ao_planes:
+            p = XCPlan(molecule, self.kind, self.spec, "fp64", None, self.correlation)
+            p.custom = True
+            molecule.cache[key] = p
+        p.molecule = molecule
+        return p
+
+    def xc_energy(self, params, molecule: Molecule, precision: str = "fp64",
+                  rdm1: Optional[torch.Tensor] = None) -> torch.Tensor:
+        if precision != "fp64":
+            raise NotImplementedError("custom coefficient functions run in FP64")
+        D = molecule.rdm1 if rdm1 is None else rdm1
+        e, x = CustomInputsFn.apply(D, self.plan(molecule, "fp64"))
+        c = self.coefficients(params, x)
+        if tuple(c.shape) != (molecule.grid_size, 4):
+            raise ValueError(f"coefficients must return ({molecule.grid_size}, 4), got {tuple(c.shape)}")
+        clip = lambda t: torch.where(t.abs() > 1e-30, t, torch.zeros_like(t))  # noqa: E731
+        ed = clip((c.to(torch.float64) * clip(e)).sum(dim=1))
+        return (clip(molecule.grid.weights) * ed).sum()
+
+    def exc_vxc(self, params, molecule: Molecule, rdm1: torch.Tensor):
+        """(E_xc, dE_xc/dD) at rdm1, not recorded for reverse mode through the caller."""
+        with torch.enable_grad():
+            D = rdm1.detach().requires_grad_(True)
+            p = None if params is None else params.detach()
+            E = self.xc_energy(p, molecule, rdm1=D)
+            vxc, = torch.autograd.grad(E, [D])
+        return E.detach(), vxc
 
 
 class LinearFunctional(Functional):

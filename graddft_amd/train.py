@@ -32,6 +32,9 @@ def _recording(*ts) -> bool:
 
 
 def energy_predictor(functional: Functional, precision: str = "fp64", **kwargs) -> Callable:
+    if getattr(functional, "custom", False):
+        return _custom_predictor(functional, precision)
+
     def predict(params: Optional[torch.Tensor], molecule: Molecule) -> Tuple[torch.Tensor, torch.Tensor]:
         plan = functional.plan(molecule, precision)
         D = molecule.rdm1
@@ -51,6 +54,54 @@ def energy_predictor(functional: Functional, precision: str = "fp64", **kwargs) 
         p = None if params is None else params.detach().contiguous()
         plan.forward(D, p, exc)
         plan.backward(D, p, one, vxc, None, plan.generation)
+        J = coulomb(molecule, D)
+        K = exchange(molecule, D) if current else None
+        energy = exc + non_xc_energy(molecule, D, J, exact_exchange=a0, K=K)
+        fock = fock_assemble(molecule, J, vxc, exact_exchange=a0, K=K)
+        return energy, fock
+
+    return predict
+
+
+class _NoSecondOrderFn(torch.autograd.Function):
+    """Passes V_xc of a custom coefficient function through; reverse mode through it would need
+    the second derivatives of the user's function, which the device path does not form."""
+
+    @staticmethod
+    def forward(ctx, vxc, *inputs):
+        return vxc.clone()
+
+    @staticmethod
+    def backward(ctx, g):
+        raise NotImplementedError("reverse mode through V_xc of a custom coefficient function (an SCF loop "
+                                  "or a loss of the Fock matrix) needs its second derivatives; differentiate "
+                                  "the energy at a fixed density instead")
+
+
+def _custom_predictor(functional, precision: str) -> Callable:
+    """energy_predictor of a CustomFunctional: E_xc through autograd over the device-formed
+    inputs and the user's coefficient function, V_xc = dE_xc/dD by one reverse pass."""
+    if precision != "fp64":
+        raise NotImplementedError("custom coefficient functions run in FP64")
+
+    def predict(params: Optional[torch.Tensor], molecule: Molecule) -> Tuple[torch.Tensor, torch.Tensor]:
+        D = molecule.rdm1
+        a0 = functional.exact_exchange
+        current = bool(a0) and functional.exchange == "current"
+        if _recording(params, D):
+            with torch.enable_grad():
+                Dg = D if D.requires_grad else D.detach().requires_grad_(True)
+                exc = functional.xc_energy(params, molecule, rdm1=Dg)
+                vxc, = torch.autograd.grad(exc, [Dg], retain_graph=True)
+            ins = [t for t in (params, D) if t is not None and t.requires_grad]
+            vxc = _NoSecondOrderFn.apply(vxc, *ins)
+            J = CoulombFn.apply(D, molecule)
+            K = ExchangeFn.apply(D, molecule) if current else None
+            energy = exc + NonXCFn.apply(D, J, K, molecule, a0)
+            fock = FockAssembleFn.apply(J, vxc, K, molecule, a0)
+            return energy, fock
+        D = D.detach().contiguous()
+        exc, vxc = functional.exc_vxc(params, molecule, D)
         J = coulomb(molecule, D)
         K = exchange(molecule, D) if current else None
         energy = exc + non_xc_energy(molecule, D, J, exact_exchange=a0, K=K)

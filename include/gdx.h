@@ -110,6 +110,22 @@ int gdx_xc_forward(void* stream, const gdx_xc_desc* desc, const void* ao, const 
  * use under graph capture. */
 int gdx_xc_autotune(void* stream, const gdx_xc_desc* desc, const void* ao, void* workspace, int32_t* vt_inout);
 
+/* Custom coefficient functions (the reference's Functional(coefficients=..., ...) with a user
+ * module, grad_dft/functional.py:160-253): the library forms the four B3LYP energy densities
+ * e_f (lda, b88, correlation, lyp; unclipped) and the six coefficient inputs x_j =
+ * log(|feature| + 1e-4) of (rho_up, rho_dn, |grad rho|^2 up/dn, tau up/dn) per point; the caller
+ * evaluates its coefficients and E = sum_r w clip(sum_f c_f clip(e_f)) itself, and hands the
+ * cotangents back for dE/dD.  desc: kind GDX_FUNC_NEURAL_B3LYP (its MLP fields only size the
+ * workspace), mlp_nc 6, GDX_FP64.
+ * gdx_xc_custom_inputs: e_out (npts, 4), x_out (npts, 6) (device), from rdm1 (2, nao, nao).
+ * gdx_xc_custom_vxc: d_rdm1 (2, nao, nao) = sum_r (g_e . de/dD + g_x . dx/dD) for cotangents
+ *   g_e (npts, 4), g_x (npts, 6); reads the densities the preceding gdx_xc_custom_inputs call
+ *   left in the same workspace. */
+int gdx_xc_custom_inputs(void* stream, const gdx_xc_desc* desc, const void* ao, const double* weights,
+                         const double* rdm1, void* workspace, double* e_out, double* x_out);
+int gdx_xc_custom_vxc(void* stream, const gdx_xc_desc* desc, const void* ao, const double* g_e, const double* g_x,
+                      void* workspace, double* d_rdm1);
+
 /* Grid densities per spin, (npts_pad, 2, 6) doubles: rho, d rho/dx, d rho/dy, d rho/dz,
  * tau, laplacian -- Molecule.density / grad_density / kinetic_density / lapl_density of
  * grad_dft/molecule.py:388-502 in one pass.  Uses (and overwrites) the workspace. */

@@ -15,11 +15,14 @@ Restates, expression by expression, the reference:
   the log(|x| + 1e-4) input squash of the reference's neural examples
   (``examples/intermediate_notebooks/neural_metaGGA_01.ipynb``, nn_coefficients).
 
+* custom coefficient functions -- ``grad_dft/functional.py:160-253`` with a user
+  ``coefficients`` callable over the squashed inputs (custom_energy_from_rdm1).
+
 Pinning: brute-force loops on tiny inputs and the identities
 ``sum_r w rho_s = tr(D_s S)``, ``lapl = 4 tau + 2 sum D phi lapl(phi)``;
 closed forms for LDA exchange and the zeta = 0 / +-1 limits of the polarisation
-corrections; central finite differences for every derivative
-(``tests/test_oracle_xc.py``).  flax defaults beyond ELU/Glorot are not used.
+corrections; central finite differences for every derivative; the custom path against
+the neural, B3LYP and PW92 paths it reduces to (``tests/test_oracle_xc.py``).  flax defaults beyond ELU/Glorot are not used.
 """
 from __future__ import annotations
 
@@ -345,6 +348,29 @@ def xc_value_and_grads(kind: str, inp: XCInputs, rdm1: np.ndarray, params=None,
     gD = grads[0].numpy()
     gth = grads[1].numpy() if th is not None else None
     return float(E.detach()), gD, gth
+
+
+def custom_energy_from_rdm1(inp: XCInputs, rdm1, coefficients, params=None, correlation: str = "vwn"):
+    """E_xc(D) of Functional(coefficients=user function) (functional.py:160-253): energy
+    densities (lda, b88, correlation, lyp) clipped, coefficient inputs x = log(|f| + 1e-4) of
+    the six features, c = coefficients(params, x) (N, 4), then xc_energy."""
+    rho = density(rdm1, inp.ao)
+    grho = grad_density(rdm1, inp.ao, inp.grad_ao)
+    lap = lapl_density(rdm1, inp.ao, inp.grad_ao, inp.d2_ao)
+    tau = kinetic_density(rdm1, inp.grad_ao)
+    corr = vwn_c_e(rho) if correlation == "vwn" else pw92_c_e(rho)
+    dens = abs_clip(torch.stack((lsda_x_e(rho), b88_x_e(rho, grho), corr, lyp_c_e(rho, grho, lap)), dim=1))
+    x = torch.log(torch.abs(neural_features(rho, grho, tau)) + SQUASH_OFFSET)
+    return xc_energy(inp.weights, coefficients(params, x), dens)
+
+
+def custom_value_and_grads(inp: XCInputs, rdm1: np.ndarray, coefficients, params=None, correlation: str = "vwn"):
+    """(E_xc, dE/dD, dE/dtheta) of custom_energy_from_rdm1."""
+    D = torch.tensor(rdm1, dtype=DT, requires_grad=True)
+    th = None if params is None else torch.tensor(params, dtype=DT, requires_grad=True)
+    E = custom_energy_from_rdm1(inp, D, coefficients, th, correlation)
+    grads = torch.autograd.grad(E, [D] + ([th] if th is not None else []))
+    return float(E.detach()), grads[0].numpy(), (grads[1].numpy() if th is not None else None)
 
 
 def to_inputs(ao, grad_ao, d2_ao, weights, chi=None) -> XCInputs:

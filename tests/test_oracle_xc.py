@@ -232,3 +232,35 @@ def test_single_part_functionals_compose_b3lyp_and_have_consistent_gradients():
     assert (Ep - Em) / (2 * h) == pytest.approx(np.sum(gD * P), rel=1e-6)
     # PW92 and VWN5 are fits of the same correlation energy: the molecule's energies agree closely
     assert abs(E0 - E["vwn"]) <= 1e-2 * abs(E["vwn"])
+
+
+def test_custom_coefficients_reduce_to_neural_and_b3lyp():
+    """A user coefficient function of the squashed inputs x = log(|f| + 1e-4): the MLP head
+    applied to x gives the neural functional's energy and gradients (the squash moves from the
+    head to the inputs); constant B3LYP coefficients give the B3LYP grid energy; with the PW92
+    slot and coefficients (0, 0, 1, 0) the energy is the single-part PW92 functional's."""
+    inp, D, C, _ = _water_inputs(level=1)
+    D1 = C[:, :4] @ C[:, :4].T
+    rdm1 = np.stack([D, 0.5 * (D + D1)])
+    spec = X.MLPSpec(6, 16, 2, 4)
+    th = X.init_mlp_params(spec, 5)
+
+    def mlp_of_x(params, x):
+        y = x
+        layers = X.unflatten(spec, params)
+        for W, b in layers[:-1]:
+            y = torch.nn.functional.elu(y @ W + b)
+        W, b = layers[-1]
+        return 2.0 * torch.sigmoid((y @ W + b) / 2.0)
+
+    E0, gD0, gth0 = X.xc_value_and_grads("neural_b3lyp", inp, rdm1, th, spec)
+    E1, gD1, gth1 = X.custom_value_and_grads(inp, rdm1, mlp_of_x, th)
+    assert E1 == pytest.approx(E0, rel=1e-13)
+    assert np.abs(gD1 - gD0).max() <= 1e-12 * np.abs(gD0).max()
+    assert np.abs(gth1 - gth0).max() <= 1e-12 * np.abs(gth0).max()
+    b3 = torch.tensor([X.B3LYP_COEFFS[:4]], dtype=DT)
+    Eb = X.custom_value_and_grads(inp, rdm1, lambda p, x: b3.expand(x.shape[0], 4))[0]
+    assert Eb == pytest.approx(X.xc_value_and_grads("b3lyp", inp, rdm1)[0], rel=1e-13)
+    one = torch.tensor([[0.0, 0.0, 1.0, 0.0]], dtype=DT)
+    Ep = X.custom_value_and_grads(inp, rdm1, lambda p, x: one.expand(x.shape[0], 4), correlation="pw92")[0]
+    assert Ep == pytest.approx(X.xc_value_and_grads("pw92", inp, rdm1)[0], rel=1e-13)
