@@ -99,25 +99,37 @@ __global__ __launch_bounds__(256) void delta_wgrad_rows_kernel(const T* __restri
   }
 }
 
-// out (rows x nout) = A (rows x K) Bt^T, Bt (16 x K) in LDS; 16-row groups per wave.  Lane l
-// loads VEC = 16 / sizeof(T) consecutive elements of row l % 16 at column
+// out (rows x nout) = A (rows x K) Bt^T, Bt (16 x K) in LDS; one 16-row group per wave.  Lane
+// l loads VEC = 16 / sizeof(T) consecutive elements of row l % 16 at column
 // t (4 VEC) + (l / 16) VEC for every t; element q of chunk t is the A operand of MFMA
 // (t, q), whose k-slice l / 16 then stands for that column -- B is read with the same
-// column map, so the products sum over every k exactly once.
+// column map, so the products sum over every k exactly once.  The row loads are issued
+// before Bt is staged (the two latencies overlap), and Bt's LDS rows are padded by 16 bytes
+// so the 16 rows' B vectors fall in different banks.
 template <typename T, bool HEAD>
 __global__ __launch_bounds__(256) void rowdot_mfma_kernel(int rows, int K, const T* __restrict__ A,
                                                           const T* __restrict__ Bt, T* __restrict__ out, int nout,
                                                           const T* __restrict__ bias, const double* __restrict__ E,
                                                           const double* __restrict__ weights, int npts,
                                                           double* __restrict__ eps_part, int head_mode) {
+  static_assert(RD_ROWS == 64, "one 16-row group per wave of a 256-thread block");
   typedef typename MfmaT<T>::acc_t acc_t;
   constexpr int VEC = 16 / sizeof(T);
   typedef T vec_t __attribute__((ext_vector_type(VEC)));
   constexpr int MAXC = 256 / (4 * VEC);  // chunks for K <= 256
-  __shared__ __align__(16) T Bs[16 * 256];
+  constexpr int LDB = 256 + VEC;
+  __shared__ __align__(16) T Bs[16 * LDB];
   __shared__ double zs[4][16][17];
   __shared__ double red[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nch = K / (4 * VEC);
+  const int rl = lane & 15, ks = lane >> 4;
+  const int r0 = blockIdx.x * RD_ROWS + wave * 16;
+  const int r = r0 + rl;
+  vec_t x[MAXC];
+#pragma unroll
+  for (int t = 0; t < MAXC; ++t)
+    if (t < nch) x[t] = r < rows ? *(const vec_t*)(A + (size_t)r * K + t * 4 * VEC + ks * VEC) : (vec_t)(T)0;
   {
     // stage Bt (16 x K, K <= 256) with 16-byte loads, all issued before the LDS stores
     constexpr int NBV = 16 * 256 / (256 * VEC);
@@ -130,31 +142,19 @@ __global__ __launch_bounds__(256) void rowdot_mfma_kernel(int rows, int K, const
 #pragma unroll
     for (int i = 0; i < NBV; ++i) {
       const int e = (tid + 256 * i) * VEC;
-      if (e < 16 * K) *(vec_t*)&Bs[e] = bv[i];
+      if (e < 16 * K) *(vec_t*)&Bs[(e / K) * LDB + e % K] = bv[i];
     }
   }
   __syncthreads();
-  const int nch = K / (4 * VEC);
-  const int rl = lane & 15, ks = lane >> 4;
   double contrib = 0.0;
-  const int ngroups = RD_ROWS / 16;
-  for (int gi = wave; gi < ngroups; gi += 4) {
-    const int r0 = blockIdx.x * RD_ROWS + gi * 16;
-    if (r0 >= rows) break;
-    const int r = r0 + rl;
-    vec_t x[MAXC];
-#pragma unroll
-    for (int t = 0; t < MAXC; ++t)
-      if (t < nch) x[t] = r < rows ? *(const vec_t*)(A + (size_t)r * K + t * 4 * VEC + ks * VEC) : (vec_t)(T)0;
+  if (r0 < rows) {
     acc_t acc = (acc_t){0, 0, 0, 0};
 #pragma unroll
     for (int t = 0; t < MAXC; ++t) {
       if (t < nch) {
+        const vec_t b = *(const vec_t*)&Bs[rl * LDB + t * 4 * VEC + ks * VEC];
 #pragma unroll
-        for (int q = 0; q < VEC; ++q) {
-          const T b = Bs[rl * K + t * 4 * VEC + ks * VEC + q];
-          acc = MfmaT<T>::op(x[t][q], b, acc);
-        }
+        for (int q = 0; q < VEC; ++q) acc = MfmaT<T>::op(x[t][q], b[q], acc);
       }
     }
     if (!HEAD) {
@@ -163,31 +163,29 @@ __global__ __launch_bounds__(256) void rowdot_mfma_kernel(int rows, int K, const
         const int rr = r0 + MfmaT<T>::row(lane, q);
         if (rr < rows && rl < nout) out[(size_t)rr * nout + rl] = acc[q];
       }
-      continue;
-    }
+    } else {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) zs[wave][MfmaT<T>::row(lane, q)][rl] = (double)acc[q];
-    // zs is private to this wave: lanes 0..15 finish one row each (wave-scope ordering of the
-    // LDS writes above before the reads below)
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    if (lane < 16) {
-      const int rr = r0 + lane;
-      if (rr < npts) {
-        double eps = 0.0;
-        for (int o = 0; o < nout; ++o) {
-          const double zz = zs[wave][lane][o] + (double)bias[o];
-          const double c = head_coef(zz, head_mode, o);
-          out[(size_t)rr * nout + o] = (T)c;
-          eps += c * E[(size_t)rr * nout + o];
+      for (int q = 0; q < 4; ++q) zs[wave][MfmaT<T>::row(lane, q)][rl] = (double)acc[q];
+      // zs is private to this wave: lanes 0..15 finish one row each (wave-scope ordering of
+      // the LDS writes above before the reads below)
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (lane < 16) {
+        const int rr = r0 + lane;
+        if (rr < npts) {
+          double eps = 0.0;
+          for (int o = 0; o < nout; ++o) {
+            const double zz = zs[wave][lane][o] + (double)bias[o];
+            const double c = head_coef(zz, head_mode, o);
+            out[(size_t)rr * nout + o] = (T)c;
+            eps += c * E[(size_t)rr * nout + o];
+          }
+          eps = fabs(eps) > 1e-30 ? eps : 0.0;
+          const double wt = weights[rr];
+          contrib += (fabs(wt) > 1e-30 ? wt : 0.0) * eps;
         }
-        eps = fabs(eps) > 1e-30 ? eps : 0.0;
-        const double wt = weights[rr];
-        contrib += (fabs(wt) > 1e-30 ? wt : 0.0) * eps;
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
   }
   if (HEAD) {
     contrib = warp_sum(contrib);
