@@ -152,3 +152,65 @@ def test_capture_graph_pauses_the_cyclic_collector(monkeypatch):
     with pytest.raises(RuntimeError):
         evaluate.capture_graph(object(), boom)
     assert gc.isenabled()
+
+
+def test_custom_functional_host_logic(monkeypatch):
+    """CustomFunctional: argument checks, a plan of its own that refuses the MLP entry points,
+    E = sum_r clip(w) clip(sum_f c_f clip(e_f)) over the device inputs, and the cotangents it
+    hands back (dE/de_f = clip(w) c_f where the clipped sum is kept, dE/dx through the user's
+    function) -- with the device calls replaced by fixed inputs on the CPU."""
+    from graddft_amd import _lib
+    from graddft_amd.functional import CustomFunctional, XCPlan
+    from graddft_amd.molecule import Grid, Molecule
+    monkeypatch.setattr(_lib, "load", lambda *a: _StubLib())
+    monkeypatch.setattr(torch.cuda, "current_stream", lambda *a: type("S", (), {"cuda_stream": 0})())
+    with pytest.raises(TypeError):
+        CustomFunctional(3.0)
+    with pytest.raises(ValueError):
+        CustomFunctional(lambda p, x: x, correlation="lyp")
+    n, N = 4, 6
+    z = lambda *s: torch.zeros(s, dtype=torch.float64)  # noqa: E731
+    rng = np.random.default_rng(3)
+    w = torch.tensor(rng.uniform(0.1, 1.0, N))
+    w[2] = 1e-31                                           # clipped weight
+    mol = Molecule(grid=Grid(z(N, 3), w, z(N)), atom_index=torch.ones(1), nuclear_pos=z(1, 3),
+                   ao_planes=z(5, N, 16), rdm1=z(2, n, n), nuclear_repulsion=0.0, h1e=z(n, n),
+                   s1e=z(n, n), mo_coeff=z(2, n, n), mo_occ=z(2, n), mo_energy=z(2, n))
+    e = torch.tensor(rng.normal(size=(N, 4)))
+    e[4, :] = 0.0                                          # clipped energy density sum
+    x = torch.tensor(rng.normal(size=(N, 6)))
+    seen = {}
+
+    def fake_inputs(self, rdm1):
+        self.generation += 1
+        self.fwd_generation = self.generation
+        return e.clone(), x.clone()
+
+    def fake_vxc(self, g_e, g_x, gen):
+        seen["g_e"], seen["g_x"] = g_e.clone(), g_x.clone()
+        return torch.full((2, n, n), float(g_e.sum() + g_x.sum()), dtype=torch.float64)
+
+    monkeypatch.setattr(XCPlan, "custom_inputs", fake_inputs)
+    monkeypatch.setattr(XCPlan, "custom_vxc", fake_vxc)
+    A = torch.tensor(rng.normal(scale=0.3, size=(6, 4)), requires_grad=True)
+    f = CustomFunctional(lambda p, xx: 1.0 + torch.tanh(xx @ p))
+    with pytest.raises(NotImplementedError):
+        f.plan(mol, "fp32")
+    plan = f.plan(mol)
+    assert plan.custom and f.plan(mol) is plan
+    with pytest.raises(RuntimeError, match="custom"):
+        plan.forward(mol.rdm1, None, z())
+    D = mol.rdm1.clone().requires_grad_(True)
+    E = f.xc_energy(A, mol, rdm1=D)
+    c = 1.0 + np.tanh(x.numpy() @ A.detach().numpy())
+    eps = (c * e.numpy()).sum(axis=1)
+    keep = (np.abs(eps) > 1e-30) & (np.abs(w.numpy()) > 1e-30)
+    gD, gA = torch.autograd.grad(E, [D, A])
+    assert float(E.detach()) == pytest.approx(float((w.numpy() * eps)[keep].sum()), rel=1e-14)
+    ge_ref = np.where(keep[:, None], w.numpy()[:, None] * c, 0.0)
+    assert np.allclose(seen["g_e"].numpy(), ge_ref, rtol=1e-14, atol=0)
+    # dE/dx_j = sum_f clip(w) e_f dc_f/dx_j, dc/dz = 1 - tanh^2
+    dz = np.where(keep[:, None], w.numpy()[:, None] * e.numpy() * (1 - np.tanh(x.numpy() @ A.detach().numpy()) ** 2), 0)
+    assert np.allclose(seen["g_x"].numpy(), dz @ A.detach().numpy().T, rtol=1e-12, atol=1e-15)
+    assert np.allclose(gA.numpy(), x.numpy().T @ dz, rtol=1e-12, atol=1e-15)
+    assert float(gD[0, 0, 0]) == pytest.approx(float(seen["g_e"].sum() + seen["g_x"].sum()), rel=1e-14)
