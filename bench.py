@@ -230,6 +230,52 @@ def eri_bench(args):
             "max_abs_diff_vs_host": err}
 
 
+def exchange_bench(mol):
+    """K[D] (both spins) of the molecule's density from the exchange-ordered ERI copy (its
+    one-time build timed separately) and from the gather over the packed ERI, per call, with
+    the copy's streaming rate."""
+    import torch
+    from graddft_amd import scf
+    n = mol.nao
+    D = mol.rdm1.detach().contiguous()
+    scf.release_exchange_xo(mol)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            r = fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps, r
+
+    t0 = time.perf_counter()
+    built = scf._exchange_xo(mol, n, D.device)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    res = {"n": n, "xo_bytes": 8 * (n * n * (n * n + 1) // 2)}
+    if built is None:
+        res["xo"] = "over budget"
+    else:
+        ms_xo, K_xo = timed(lambda: scf.exchange(mol, D), 5)
+        res.update({"xo_build_ms": round(1e3 * t_build, 1), "xo_ms_per_call": round(ms_xo, 3),
+                    "xo_GB_per_s": round(D.shape[0] * res["xo_bytes"] / (ms_xo * 1e6), 1)})
+    scf.release_exchange_xo(mol)
+    keep = scf.XO_MAX_BYTES
+    scf.XO_MAX_BYTES = 0
+    try:
+        ms_g, K_g = timed(lambda: scf.exchange(mol, D), 2)
+    finally:
+        scf.XO_MAX_BYTES = keep
+    res["gather_ms_per_call"] = round(ms_g, 3)
+    if built is not None:
+        res["speedup"] = round(ms_g / ms_xo, 2)
+        res["max_rel_diff"] = float((K_xo - K_g).abs().max() / K_g.abs().max())
+    return res
+
+
 def config4_bench(args):
     """Naphthalene (seeded geometry) def2-SVP, nao 180, level-3 grid, packed ERI generated on
     the device (1.06 GB), starting from its self-consistent B3LYP density, B3LYP plus the
@@ -265,6 +311,7 @@ def config4_bench(args):
            "npts": mol.grid_size, "nao": mol.nao, "molecule_build_s": round(t_build, 2),
            "initial_density": "self-consistent B3LYP (device CDIIS, converged_molecule)",
            "initial_density_s": round(t_init, 2)}
+    out["exchange"] = exchange_bench(mol)
     P = mol.nao * (mol.nao + 1) // 2
     eri_bytes = 8 * (P * (P + 1) // 2)
     res = {}

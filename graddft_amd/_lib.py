@@ -52,6 +52,10 @@ SIGNATURES = {
     "gdx_nonxc_energy": (ctypes.c_int, [_P, _I, _P, _P, _P, _P, ctypes.c_double, ctypes.c_double, _P]),
     "gdx_fock_assemble": (ctypes.c_int, [_P, _I, _P, _P, _P, _P, ctypes.c_double, _P]),
     "gdx_exchange_packed": (ctypes.c_int, [_P, _I, _I, _P, _P, _P]),
+    "gdx_exchange_xo_size": (ctypes.c_int, [_I, ctypes.POINTER(ctypes.c_size_t)]),
+    "gdx_exchange_xo_build": (ctypes.c_int, [_P, _I, _P, _P]),
+    "gdx_exchange_xo_workspace_size": (ctypes.c_int, [_I, ctypes.POINTER(ctypes.c_size_t)]),
+    "gdx_exchange_xo": (ctypes.c_int, [_P, _I, _I, _P, _P, _P, _P]),
     "gdx_cholesky_inv": (ctypes.c_int, [_P, _I, _P, _P, _P, _P]),
     "gdx_eigh_workspace_size": (ctypes.c_int, [_I, _I, ctypes.POINTER(_Z)]),
     "gdx_eigh_batched": (ctypes.c_int, [_P, _I, _I, _P, _P, _P, _P, _P, _I, ctypes.c_double, _P]),

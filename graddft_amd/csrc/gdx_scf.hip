@@ -312,6 +312,38 @@ __global__ __launch_bounds__(256) void exchange_gather_kernel(const double* __re
   if (lane == 0) K[w] = acc;
 }
 
+// Exchange-ordered packed ERI ("XO"): K as a symmetric matrix-vector product.  Over ordered
+// pairs I = a n + c, M[I][J] = (ab|cd) for J = b n + d is symmetric (M[J][I] = (ba|dc)), and
+// K_s = M vec(D_s) with vec(D)[b n + d] = D[b][d] -- so the Coulomb kernels' packed
+// lower-triangle GEMV serves K with P' = n^2 "pairs" and the density matrix itself as the
+// vector.  xo[I (I + 1) / 2 + J] = M[I][J] for J <= I (n^2 (n^2 + 1) / 2 doubles, 4x the
+// 8-fold packed ERI).  One block per row I, its threads striding over J (coalesced stores,
+// gathered loads: built once per molecule).
+__global__ __launch_bounds__(NT) void xo_build_kernel(const double* __restrict__ eri, int n,
+                                                      double* __restrict__ xo) {
+  const long I = blockIdx.x;
+  const int a = (int)(I / n), c = (int)(I % n);
+  double* row = xo + I * (I + 1) / 2;
+  for (long J = threadIdx.x; J <= I; J += NT) {
+    const int b = (int)(J / n), d = (int)(J % n);
+    const long P1 = pair_index(a, b), P2 = pair_index(c, d);
+    row[J] = P1 >= P2 ? eri[P1 * (P1 + 1) / 2 + P2] : eri[P2 * (P2 + 1) / 2 + P1];
+  }
+}
+
+// K[I] = sum of the row and column partials of the XO product (coulomb_finish_kernel without
+// the pair unpacking: I = a n + c is K's own row-major index).
+__global__ void xo_finish_kernel(const double* __restrict__ prow, const double* __restrict__ pcol, long P, int nb,
+                                 int tt, double* __restrict__ K) {
+  const long I = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= P) return;
+  const int t = (int)(I / tt);
+  double acc = 0.0;
+  for (int b = 0; b <= t; ++b) acc += prow[b * P + I];
+  for (int b = t; b < nb; ++b) acc += pcol[b * P + I];
+  K[I] = acc;
+}
+
 // Status word of an SCF cycle, OR-ed into *status by one block (no atomics): bit 0 the
 // eigensolver ran max_sweeps without converging (sweeps[s] >= max_sweeps for a spin), bit 1
 // the energy is not finite, bit 2 |tr(D_s S) - sum_i occ_s,i| > tol for a spin (the orbitals
@@ -1265,6 +1297,55 @@ int gdx_exchange_packed(void* stream, int32_t n, int32_t nspin, const double* er
   const long waves = (long)nspin * n * n;
   GDX_LAUNCH("exchange", exchange_gather_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, eri, n, nspin,
              D, K);
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+int gdx_exchange_xo_size(int32_t n, size_t* bytes) {
+  GDX_REQUIRE(n > 0 && n <= 2048 && bytes, "gdx_exchange_xo_size: bad arguments");
+  const long P = (long)n * n;
+  *bytes = (size_t)(P * (P + 1) / 2) * sizeof(double);
+  return GDX_OK;
+}
+
+int gdx_exchange_xo_build(void* stream, int32_t n, const double* eri, double* xo) {
+  GDX_REQUIRE(n > 0 && n <= 2048 && eri && xo, "gdx_exchange_xo_build: bad arguments");
+  GDX_LAUNCH("exchange_xo_build", xo_build_kernel, dim3((unsigned)((long)n * n)), dim3(NT), 0, (hipStream_t)stream,
+             eri, n, xo);
+  GDX_HIP_CHECK(hipGetLastError());
+  return GDX_OK;
+}
+
+int gdx_exchange_xo_workspace_size(int32_t n, size_t* bytes) {
+  GDX_REQUIRE(n > 0 && n <= 2048 && bytes, "gdx_exchange_xo_workspace_size: bad arguments");
+  const long P = (long)n * n;
+  const long nb = (P + coulomb_tile(P) - 1) / coulomb_tile(P);
+  *bytes = (size_t)(2 * nb * P) * sizeof(double);
+  return GDX_OK;
+}
+
+int gdx_exchange_xo(void* stream, int32_t n, int32_t nspin, const double* xo, const double* D, double* K,
+                    void* workspace) {
+  GDX_REQUIRE(n > 0 && n <= 2048 && nspin > 0 && xo && D && K && workspace, "gdx_exchange_xo: bad arguments");
+  GDX_REQUIRE(D != K, "gdx_exchange_xo: K must not alias D");
+  hipStream_t st = (hipStream_t)stream;
+  const long P = (long)n * n;
+  const int tt = coulomb_tile(P);
+  const long nb = (P + tt - 1) / tt;
+  double* prow = (double*)workspace;
+  double* pcol = prow + nb * P;
+  const dim3 grid((unsigned)(nb * (nb + 1) / 2));
+  for (int s = 0; s < nspin; ++s) {
+    const double* Ds = D + (size_t)s * P;
+    if (tt == JT && P >= J_ROWS_MIN)
+      GDX_LAUNCH("exchange_xo", coulomb_rows_kernel, grid, dim3(NT), 0, st, xo, P, Ds, prow, pcol);
+    else if (tt == JT)
+      GDX_LAUNCH("exchange_xo", coulomb_tile_kernel<JT>, grid, dim3(NT), 0, st, xo, P, Ds, prow, pcol);
+    else
+      GDX_LAUNCH("exchange_xo", coulomb_tile_kernel<JTS>, grid, dim3(NT), 0, st, xo, P, Ds, prow, pcol);
+    GDX_LAUNCH("exchange_xo_finish", xo_finish_kernel, dim3(nblk(P)), dim3(NT), 0, st, prow, pcol, P, (int)nb, tt,
+               K + (size_t)s * P);
+  }
   GDX_HIP_CHECK(hipGetLastError());
   return GDX_OK;
 }

@@ -165,4 +165,7 @@ def converged_molecule(mol: Molecule, functional=None, round_cycles: int = 10, m
             e_prev = e
     new = mol.replace(rdm1=cur.rdm1, mo_coeff=cur.mo_coeff, mo_energy=cur.mo_energy, mo_occ=cur.mo_occ)
     new.cache = {"rdm1_init": new.rdm1}
+    # the rounds' exchange-ordered ERI copy (4x the packed ERI) lives in the input's cache
+    from ..scf import release_exchange_xo
+    release_exchange_xo(mol)
     return new

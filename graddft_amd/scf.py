@@ -53,12 +53,53 @@ def coulomb(molecule: Molecule, rdm1: Optional[torch.Tensor] = None) -> torch.Te
     return J
 
 
+# Largest exchange-ordered ERI copy built per molecule (bytes); it is also kept under a
+# quarter of the device memory free when it is built.  Larger molecules use the gather kernel.
+XO_MAX_BYTES = 16 << 30
+
+
+def _exchange_xo(molecule: Molecule, n: int, device):
+    """(xo, workspace) of the molecule's exchange-ordered ERI copy, built on first use and
+    kept in its cache; None when it would exceed the budget or cannot be built now (under
+    graph capture, which must not allocate)."""
+    hit = molecule.cache.get("eri_xo")
+    if hit is not None and hit[0] is molecule.rep_tensor:
+        return hit[1], hit[2]
+    if torch.cuda.is_current_stream_capturing():
+        return None
+    lib = _lib.load()
+    nbytes, wbytes = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    _lib.check(lib.gdx_exchange_xo_size(n, ctypes.byref(nbytes)), "gdx_exchange_xo_size")
+    _lib.check(lib.gdx_exchange_xo_workspace_size(n, ctypes.byref(wbytes)), "gdx_exchange_xo_workspace_size")
+    free, _ = torch.cuda.mem_get_info(device)
+    if nbytes.value > XO_MAX_BYTES or nbytes.value + wbytes.value > free // 4:
+        return None
+    xo = torch.empty(nbytes.value // 8, dtype=torch.float64, device=device)
+    ws = torch.empty(wbytes.value // 8, dtype=torch.float64, device=device)
+    _lib.check(lib.gdx_exchange_xo_build(_stream(xo), n, molecule.rep_tensor.data_ptr(), xo.data_ptr()),
+               "gdx_exchange_xo_build")
+    molecule.cache["eri_xo"] = (molecule.rep_tensor, xo, ws)
+    return xo, ws
+
+
+def release_exchange_xo(molecule: Molecule) -> None:
+    """Drop the molecule's exchange-ordered ERI copy (4x the packed ERI)."""
+    molecule.cache.pop("eri_xo", None)
+
+
 def exchange(molecule: Molecule, D: torch.Tensor) -> torch.Tensor:
-    """K_s[a][c] = sum_bd (ab|cd) D_s[b][d] for D (ns, n, n)."""
+    """K_s[a][c] = sum_bd (ab|cd) D_s[b][d] for D (ns, n, n): from the exchange-ordered ERI
+    copy (a symmetric product read at streaming rate; built once per molecule) when it fits the
+    budget, else gathered from the packed ERI."""
     _require_eri(molecule)
     D = D.contiguous()
     ns, n = D.shape[0], D.shape[-1]
     K = torch.empty_like(D)
+    xo = _exchange_xo(molecule, n, D.device)
+    if xo is not None:
+        _lib.check(_lib.load().gdx_exchange_xo(_stream(D), n, ns, xo[0].data_ptr(), D.data_ptr(), K.data_ptr(),
+                                               xo[1].data_ptr()), "gdx_exchange_xo")
+        return K
     _lib.check(_lib.load().gdx_exchange_packed(_stream(D), n, ns, molecule.rep_tensor.data_ptr(), D.data_ptr(),
                                                K.data_ptr()), "gdx_exchange_packed")
     return K

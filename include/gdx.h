@@ -226,6 +226,21 @@ int gdx_fock_assemble(void* stream, int32_t n, const double* h1e, const double* 
  * order (reproducible bit for bit); K must not alias D. */
 int gdx_exchange_packed(void* stream, int32_t n, int32_t nspin, const double* eri, const double* D, double* K);
 
+/* The same K_s from an exchange-ordered copy of the ERI, read once per spin at streaming rate:
+ * over ordered pairs I = a n + c the matrix M[I][b n + d] = (ab|cd) is symmetric, and
+ * K_s = M vec(D_s) is the packed symmetric product of gdx_coulomb_packed with P' = n^2.
+ * gdx_exchange_xo_size: bytes of the copy, n^2 (n^2 + 1) / 2 doubles (4x the packed ERI;
+ * water def2-TZVP 14 MB, naphthalene def2-SVP 4.2 GB).  gdx_exchange_xo_build: the copy from the
+ * 8-fold packed ERI (once per molecule).  gdx_exchange_xo: K (nspin, n, n) from it, D general;
+ * workspace gdx_exchange_xo_workspace_size bytes.  Summation order fixed (reproducible bit for
+ * bit); agrees with gdx_exchange_packed to rounding.  Replaces the same reference term as
+ * gdx_exchange_packed (grad_dft/molecule.py:507-613). */
+int gdx_exchange_xo_size(int32_t n, size_t* bytes);
+int gdx_exchange_xo_build(void* stream, int32_t n, const double* eri, double* xo);
+int gdx_exchange_xo_workspace_size(int32_t n, size_t* bytes);
+int gdx_exchange_xo(void* stream, int32_t n, int32_t nspin, const double* xo, const double* D, double* K,
+                    void* workspace);
+
 /* S = L L^T and Linv = L^-1 (the cholesky / inv of safe_general_eigh,
  * grad_dft/utils/eigenproblem.py:125-126).  status: device int, 1 if S is not positive definite. */
 int gdx_cholesky_inv(void* stream, int32_t n, const double* S, double* L, double* Linv, int32_t* status);

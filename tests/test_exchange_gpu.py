@@ -1,9 +1,11 @@
-"""Exact exchange from the packed ERI on the device (gdx_exchange_packed, a gather with a fixed
-summation order) and the "current" exchange mode (Functional.with_exchange("current"):
+"""Exact exchange from the packed ERI on the device (gdx_exchange_xo, a symmetric product over an
+exchange-ordered copy, and gdx_exchange_packed, a gather; both with a fixed summation order)
+and the "current" exchange mode (Functional.with_exchange("current"):
 E_x = -(a0/2) sum_s tr(D_s K[D_s]), Fock -a0 K[D], rebuilt from every density).
 
 * K against the dense-ERI contraction for a non-symmetric D (the reverse pass applies K to
-  cotangents), 1e-12 relative; two calls equal bit for bit;
+  cotangents), 1e-12 relative, on both paths; two calls equal bit for bit; the two paths agree
+  to 1e-13 on butane and pentane def2-SVP (n 106, 130);
 * diff_scf_loop with current exchange against the oracle loop with exchange_update, water
   def2-SVP level 1, 12 cycles: energy 1e-8 Ha, density 1e-6; the graph-captured loop equals the
   eager one; the current and frozen loops differ;
@@ -30,10 +32,14 @@ def _water(basis="def2-svp", seed=11):
     return molecule_from_geometry(sym, xyz, basis, 1, with_eri=True)
 
 
-def test_exchange_gather_general_density_and_reproducible():
-    _say("case: K[D] for non-symmetric D, water def2-svp")
+@pytest.mark.parametrize("path", ["xo", "gather"])
+def test_exchange_gather_general_density_and_reproducible(path, monkeypatch):
+    _say(f"case: K[D] for non-symmetric D, water def2-svp, {path} path")
+    from graddft_amd import scf
     from graddft_amd.interface.integrals import unpack_eri
     from graddft_amd.scf import exchange
+    if path == "gather":
+        monkeypatch.setattr(scf, "XO_MAX_BYTES", 0)
     mol = _water()
     n = mol.nao
     rng = np.random.default_rng(1)
@@ -46,6 +52,31 @@ def test_exchange_gather_general_density_and_reproducible():
     assert torch.equal(K1, K2)
     K1 = K1.cpu().numpy()
     assert np.abs(K1 - K0).max() <= 1e-12 * np.abs(K0).max()
+    assert ("eri_xo" in mol.cache) == (path == "xo")
+
+
+@pytest.mark.parametrize("carbons", [4, 5])
+def test_exchange_xo_matches_gather(carbons, monkeypatch):
+    """The exchange-ordered product against the gather from the packed ERI (independent index
+    paths over the same integrals): butane def2-SVP (n 106, P' = n^2 = 11236: 256-tiles) and
+    pentane (n 130, P' = 16900: the wave-per-row reader), ragged edge tiles; 1e-13 relative."""
+    from graddft_amd import scf
+    from graddft_amd.interface import geometry
+    from graddft_amd.interface.builder import molecule_from_geometry
+    sym, xyz = geometry.alkane(np.random.default_rng(3), carbons)
+    mol = molecule_from_geometry(sym, xyz, "def2-svp", 1, with_eri=True)
+    n = mol.nao
+    _say(f"case: XO vs gather exchange, C{carbons} def2-svp n={n}")
+    D = torch.as_tensor(np.random.default_rng(2).normal(size=(2, n, n)), device="cuda")
+    K_xo = scf.exchange(mol, D)
+    assert "eri_xo" in mol.cache
+    assert torch.equal(scf.exchange(mol, D), K_xo)
+    scf.release_exchange_xo(mol)
+    monkeypatch.setattr(scf, "XO_MAX_BYTES", 0)
+    K_g = scf.exchange(mol, D)
+    err = float((K_xo - K_g).abs().max() / K_g.abs().max())
+    _say(f"  max rel diff {err:.2e}")
+    assert err <= 1e-13
 
 
 def _oracle_inputs(mol):
