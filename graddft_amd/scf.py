@@ -125,8 +125,12 @@ def initial_exchange(molecule: Molecule) -> torch.Tensor:
     K0 = molecule.cache.get("K0")
     if K0 is None:
         D0 = molecule.cache.get("rdm1_init", molecule.rdm1)
+        had_xo = "eri_xo" in molecule.cache
         K0 = exchange(molecule, D0)
         molecule.cache["K0"] = K0
+        if not had_xo:
+            # one K: do not keep an exchange-ordered copy (4x the packed ERI) built for it
+            release_exchange_xo(molecule)
     return K0
 
 
