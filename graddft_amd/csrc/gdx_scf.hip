@@ -331,6 +331,99 @@ __global__ __launch_bounds__(NT) void xo_build_kernel(const double* __restrict__
   }
 }
 
+// Both spins in one pass over the XO copy: coulomb_rows_kernel with two right-hand sides
+// (row partials of spin s to prow_s, column partials to pcol_s).  Per spin the products, the
+// butterfly and the LDS sums run in the single-spin kernel's order, so each spin's result is
+// bit for bit the one-spin product's.
+template <bool CHECK>
+__device__ __forceinline__ void xo_row_group2(const double* __restrict__ xo, long P, const double* __restrict__ D0,
+                                              const double* __restrict__ D1, long Ib, long K0, int lane,
+                                              const double (&dk)[2][4], double (&csum)[2][4],
+                                              double* __restrict__ prow0, double* __restrict__ prow1) {
+  double v[2][J_RG];
+#pragma unroll
+  for (int a = 0; a < J_RG; ++a) {
+    const long I = Ib + a;
+    double r0 = 0.0, r1 = 0.0;
+    if (!CHECK || I < P) {
+      const double di0 = D0[I], di1 = D1[I];
+      const double* row = xo + I * (I + 1) / 2 + K0 + lane;
+      double m[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) m[c] = (!CHECK || K0 + lane + 64 * c <= I) ? __builtin_nontemporal_load(row + 64 * c) : 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        r0 += m[c] * dk[0][c];
+        r1 += m[c] * dk[1][c];
+        if (!CHECK || K0 + lane + 64 * c < I) {
+          csum[0][c] += m[c] * di0;
+          csum[1][c] += m[c] * di1;
+        }
+      }
+    }
+    v[0][a] = r0;
+    v[1][a] = r1;
+  }
+#pragma unroll
+  for (int sp = 0; sp < 2; ++sp) {
+    double* w = v[sp];
+    const bool up = (lane & 32) != 0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const double send = up ? w[i] : w[i + 2], keep = up ? w[i + 2] : w[i];
+      w[i] = keep + __shfl_xor(send, 32, 64);
+    }
+    const bool up2 = (lane & 16) != 0;
+    const double send = up2 ? w[0] : w[1], keep = up2 ? w[1] : w[0];
+    double r = keep + __shfl_xor(send, 16, 64);
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
+    const long I = Ib + (lane >> 4);
+    if ((lane & 15) == 0 && (!CHECK || I < P)) (sp == 0 ? prow0 : prow1)[I] = r;
+  }
+}
+
+__global__ __launch_bounds__(NT) void xo_rows2_kernel(const double* __restrict__ xo, long P,
+                                                      const double* __restrict__ D0, const double* __restrict__ D1,
+                                                      double* __restrict__ prow, double* __restrict__ pcol,
+                                                      long nbP) {
+  static_assert(NT == 256 && JT == 256, "four waves, one column per thread in the epilogue");
+  __shared__ double cs[2][4][JT];
+  long bi, bk;
+  coulomb_tile_of(blockIdx.x, bi, bk);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long I0 = bi * JT, K0 = bk * JT;
+  double dk[2][4], csum[2][4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const long K = K0 + lane + 64 * c;
+    dk[0][c] = K < P ? D0[K] : 0.0;
+    dk[1][c] = K < P ? D1[K] : 0.0;
+    csum[0][c] = 0.0;
+    csum[1][c] = 0.0;
+  }
+  const bool check = !(bk < bi && I0 + JT <= P);
+  for (int g = 0; g < JT / (4 * J_RG); ++g) {
+    const long Ib = I0 + (long)(g * 4 + wave) * J_RG;
+    if (check)
+      xo_row_group2<true>(xo, P, D0, D1, Ib, K0, lane, dk, csum, prow + bk * P, prow + nbP + bk * P);
+    else
+      xo_row_group2<false>(xo, P, D0, D1, Ib, K0, lane, dk, csum, prow + bk * P, prow + nbP + bk * P);
+  }
+#pragma unroll
+  for (int sp = 0; sp < 2; ++sp)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) cs[sp][wave][lane + 64 * c] = csum[sp][c];
+  __syncthreads();
+  const int k = threadIdx.x;
+  const long K = K0 + k;
+#pragma unroll
+  for (int sp = 0; sp < 2; ++sp) {
+    const double c = (cs[sp][0][k] + cs[sp][1][k]) + (cs[sp][2][k] + cs[sp][3][k]);
+    if (K < P) pcol[sp * nbP + bi * P + K] = c;
+  }
+}
+
 // K[I] = sum of the row and column partials of the XO product (coulomb_finish_kernel without
 // the pair unpacking: I = a n + c is K's own row-major index).
 __global__ void xo_finish_kernel(const double* __restrict__ prow, const double* __restrict__ pcol, long P, int nb,
@@ -1320,7 +1413,7 @@ int gdx_exchange_xo_workspace_size(int32_t n, size_t* bytes) {
   GDX_REQUIRE(n > 0 && n <= 2048 && bytes, "gdx_exchange_xo_workspace_size: bad arguments");
   const long P = (long)n * n;
   const long nb = (P + coulomb_tile(P) - 1) / coulomb_tile(P);
-  *bytes = (size_t)(2 * nb * P) * sizeof(double);
+  *bytes = (size_t)(4 * nb * P) * sizeof(double);   // row and column partials of two spins
   return GDX_OK;
 }
 
@@ -1333,9 +1426,17 @@ int gdx_exchange_xo(void* stream, int32_t n, int32_t nspin, const double* xo, co
   const int tt = coulomb_tile(P);
   const long nb = (P + tt - 1) / tt;
   double* prow = (double*)workspace;
-  double* pcol = prow + nb * P;
+  double* pcol = prow + 2 * nb * P;
   const dim3 grid((unsigned)(nb * (nb + 1) / 2));
-  for (int s = 0; s < nspin; ++s) {
+  int s0 = 0;
+  if (nspin >= 2 && tt == JT && P >= J_ROWS_MIN) {   // the large case: both spins in one pass
+    GDX_LAUNCH("exchange_xo", xo_rows2_kernel, grid, dim3(NT), 0, st, xo, P, D, D + P, prow, pcol, nb * P);
+    for (int s = 0; s < 2; ++s)
+      GDX_LAUNCH("exchange_xo_finish", xo_finish_kernel, dim3(nblk(P)), dim3(NT), 0, st, prow + s * nb * P,
+                 pcol + s * nb * P, P, (int)nb, tt, K + (size_t)s * P);
+    s0 = 2;
+  }
+  for (int s = s0; s < nspin; ++s) {
     const double* Ds = D + (size_t)s * P;
     if (tt == JT && P >= J_ROWS_MIN)
       GDX_LAUNCH("exchange_xo", coulomb_rows_kernel, grid, dim3(NT), 0, st, xo, P, Ds, prow, pcol);

@@ -59,7 +59,8 @@ def test_exchange_gather_general_density_and_reproducible(path, monkeypatch):
 def test_exchange_xo_matches_gather(carbons, monkeypatch):
     """The exchange-ordered product against the gather from the packed ERI (independent index
     paths over the same integrals): butane def2-SVP (n 106, P' = n^2 = 11236: 256-tiles) and
-    pentane (n 130, P' = 16900: the wave-per-row reader), ragged edge tiles; 1e-13 relative."""
+    pentane (n 130, P' = 16900: the wave-per-row reader, both spins in one pass), ragged edge
+    tiles; 1e-13 relative; the spins one at a time give the same bits."""
     from graddft_amd import scf
     from graddft_amd.interface import geometry
     from graddft_amd.interface.builder import molecule_from_geometry
@@ -71,6 +72,9 @@ def test_exchange_xo_matches_gather(carbons, monkeypatch):
     K_xo = scf.exchange(mol, D)
     assert "eri_xo" in mol.cache
     assert torch.equal(scf.exchange(mol, D), K_xo)
+    # one spin at a time (pentane: the two-spin pass of the wave-per-row reader) -- same bits
+    K_sep = torch.cat([scf.exchange(mol, D[0:1]), scf.exchange(mol, D[1:2])])
+    assert torch.equal(K_sep, K_xo)
     scf.release_exchange_xo(mol)
     monkeypatch.setattr(scf, "XO_MAX_BYTES", 0)
     K_g = scf.exchange(mol, D)
