@@ -89,6 +89,22 @@ def exchange_potential_pairs(D, E):
     return K
 
 
+def exchange_ordered_matrix(eri_packed, n):
+    """M[a n + c, b n + d] = (ab|cd) over ordered pairs from the 8-fold packed ERI: the
+    exchange operator as an (n^2, n^2) matrix, K = M vec(D) (the layout of gdx_exchange_xo)."""
+    pair = _pair_index(n)
+    P1 = pair[:, None, :, None]                  # pair(a, b) at [a, c, b, d]
+    P2 = pair[None, :, None, :]                  # pair(c, d)
+    hi, lo = np.maximum(P1, P2), np.minimum(P1, P2)
+    return eri_packed[hi * (hi + 1) // 2 + lo].reshape(n * n, n * n)
+
+
+def exchange_potential_ordered(D, M):
+    """K_ac = sum_bd (ab|cd) D_bd as the product M vec(D) (same as exchange_potential)."""
+    n = D.shape[-1]
+    return (M @ D.reshape(n * n)).reshape(n, n)
+
+
 def non_xc(D, h1e, eri, e_nuc):
     """molecule.py:694-733 on D = D_up + D_dn."""
     J = coulomb_potential(D, eri)

@@ -137,3 +137,10 @@ def test_pair_matrix_coulomb_and_exchange_equal_dense():
     for Dm in (D + D.T, D):
         assert np.abs(O.coulomb_potential_pairs(Dm, E) - O.coulomb_potential(0.5 * (Dm + Dm.T), dense)).max() <= 1e-12
         assert np.abs(O.exchange_potential_pairs(Dm, E) - O.exchange_potential(Dm, dense)).max() <= 1e-12
+    # exchange over ordered pairs: a symmetric (n^2, n^2) matrix times vec(D); a layout that
+    # swapped (b, c) (the Coulomb pairing) would fail the comparison
+    M = O.exchange_ordered_matrix(eri, bs.nao)
+    assert np.array_equal(M, M.T)
+    for Dm in (D + D.T, D):
+        assert np.abs(O.exchange_potential_ordered(Dm, M) - O.exchange_potential(Dm, dense)).max() <= 1e-12
+    assert np.abs(O.exchange_potential_ordered(D, M) - O.coulomb_potential(D, dense)).max() > 1e-3
