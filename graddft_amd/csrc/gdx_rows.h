@@ -170,7 +170,26 @@ __global__ __launch_bounds__(256) void rowdot_mfma_kernel(int rows, int K, const
       // the LDS writes above before the reads below)
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
-      if (lane < 16) {
+      if (nout <= 4) {
+        // all 64 lanes: row lane / 4, output lane % 4; eps of a row summed over its four lanes
+        // as (c0 e0 + c1 e1) + (c2 e2 + c3 e3)
+        const int hr = lane >> 2, ho = lane & 3;
+        const int rr = r0 + hr;
+        double prod = 0.0;
+        if (rr < npts && ho < nout) {
+          const double zz = zs[wave][hr][ho] + (double)bias[ho];
+          const double c = head_coef(zz, head_mode, ho);
+          out[(size_t)rr * nout + ho] = (T)c;
+          prod = c * E[(size_t)rr * nout + ho];
+        }
+        const double pr = prod + __shfl_xor(prod, 1, 64);
+        double eps = pr + __shfl_xor(pr, 2, 64);
+        if (ho == 0 && rr < npts) {
+          eps = fabs(eps) > 1e-30 ? eps : 0.0;
+          const double wt = weights[rr];
+          contrib += (fabs(wt) > 1e-30 ? wt : 0.0) * eps;
+        }
+      } else if (lane < 16) {
         const int rr = r0 + lane;
         if (rr < npts) {
           double eps = 0.0;
